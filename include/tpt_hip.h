@@ -1,0 +1,185 @@
+/*
+ * tpt_hip.h -- C ABI of the MI355X (gfx950) implementation of tpt-sim's
+ * per-tick simulation step.
+ *
+ * Every entry point replaces one Rust function of the reference's tick path;
+ * the citation on each declaration names the function (crate-relative path,
+ * file:line) whose behaviour it reproduces.  Buffers keep the reference's
+ * structure-of-arrays layout and word formats (SimBuffers,
+ * crates/tpt-sim/src/buffers/mod.rs:81-393): slot arrays are indexed by slot
+ * id, pixel maps by y*xres+x, cell grids by cy*xcells+cx (cells are 4x4 px),
+ * floats are IEEE-754 binary32, pmap words are ((slot+1)<<9)|type.
+ *
+ * Conventions
+ *  - All functions return 0 on success or a negative TPT_E* code.  Invalid
+ *    dimensions, lengths or ids are rejected with TPT_EINVAL and leave the
+ *    state untouched (the reference's step functions silently no-op on the
+ *    same inputs, e.g. crates/tpt-sim/src/air_kernel/mod.rs:124-126).
+ *  - One context owns one HIP stream; calls on one context are not
+ *    thread-safe.  Device work is asynchronous: tpt_download and
+ *    tpt_synchronize wait for it.
+ *  - No PyTorch, CUDA or other framework types cross this boundary.
+ */
+#ifndef TPT_HIP_H
+#define TPT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__) || defined(__clang__)
+#define TPT_API __attribute__((visibility("default")))
+#else
+#define TPT_API
+#endif
+
+#define TPT_OK 0
+#define TPT_EINVAL (-1)   /* bad argument, dimension, length or buffer id   */
+#define TPT_ENOMEM (-2)   /* device allocation failed                        */
+#define TPT_EDEVICE (-3)  /* HIP runtime error (launch, copy, sync)          */
+#define TPT_ESTATE (-4)   /* call not valid in the context's current state   */
+
+#define TPT_PT_NUM 256            /* element slots (constants.rs:60)           */
+#define TPT_ELEM_STRIDE 32        /* u32 words per GpuElementProps row         */
+#define TPT_MOVE_META_WORDS 12    /* u32 words per element in move_lut         */
+#define TPT_MOVE_LUT_WORDS (TPT_PT_NUM * TPT_MOVE_META_WORDS + TPT_PT_NUM * TPT_PT_NUM)
+#define TPT_MAX_CAPACITY 0x7FFFFFu /* 23-bit pmap slot field (layout.rs:21-31) */
+#define TPT_MAX_DIM 2896u          /* largest canvas side (layout.rs:33-40)    */
+
+/* Mirror of the simulation-relevant fields of SimulationConfig
+ * (crates/tpt-core/src/particle.rs:166-346).  Enums use the reference's
+ * discriminants: edge_mode Void=0 Solid=1 Loop=2; gravity_mode Vertical=0
+ * Off=1 Radial=2 Custom=3; air_mode On=0 PressureOff=1 VelocityOff=2 Off=3
+ * NoUpdate=4; convection_mode 0=none 1=legacy 2=Boussinesq. */
+typedef struct tpt_config {
+    uint32_t xres;              /* canvas width in px, multiple of 4        */
+    uint32_t yres;              /* canvas height in px, multiple of 4       */
+    uint32_t capacity;          /* particle pool slots (<= TPT_MAX_CAPACITY) */
+    uint32_t aheat_enable;
+    uint32_t heat_enable;
+    uint32_t newton_grav;
+    uint32_t edge_mode;
+    uint32_t gravity_mode;
+    uint32_t air_mode;
+    uint32_t convection_mode;
+    float ambient_air_temp;
+    float edge_pressure;
+    float edge_velocity_x;
+    float edge_velocity_y;
+    float custom_gravity_x;
+    float custom_gravity_y;
+    float vorticity_coeff;
+    /* 0: reference gravity-mass table (only GRVT/NBHL/NWHL/GBMB/GPMP deposit,
+     *    gravity_kernel.rs:275-362);
+     * 1: extension -- every live particle deposits its element weight
+     *    (GpuElementProps.weight) as mass.  No reference counterpart. */
+    uint32_t grav_mass_mode;
+} tpt_config;
+
+/* Device buffers, named after the SimBuffers fields they mirror. */
+typedef enum tpt_buf_id {
+    /* slot arrays: capacity words each (buffers/mod.rs:81-140) */
+    TPT_BUF_PART_TY = 0, TPT_BUF_PART_LIFE, TPT_BUF_PART_CTYPE, TPT_BUF_PART_X,
+    TPT_BUF_PART_Y, TPT_BUF_PART_VX, TPT_BUF_PART_VY, TPT_BUF_PART_TEMP,
+    TPT_BUF_PART_EQUIL, TPT_BUF_PART_FLAGS, TPT_BUF_PART_TMP, TPT_BUF_PART_TMP2,
+    TPT_BUF_PART_TMP3, TPT_BUF_PART_TMP4, TPT_BUF_PART_DCOLOUR,
+    TPT_BUF_INTENT_TARGET,
+    /* pixel maps: xres*yres words each */
+    TPT_BUF_PMAP, TPT_BUF_PHOTONS, TPT_BUF_PMAP_COUNT, TPT_BUF_CELL_WINNER,
+    /* cell grids: xcells*ycells words each */
+    TPT_BUF_AIR_VX, TPT_BUF_AIR_VY, TPT_BUF_AIR_PV, TPT_BUF_AIR_HV,
+    TPT_BUF_AIR_OVX, TPT_BUF_AIR_OVY, TPT_BUF_AIR_OPV, TPT_BUF_AIR_OHV,
+    TPT_BUF_FAN_VX, TPT_BUF_FAN_VY, TPT_BUF_WALL_BMAP, TPT_BUF_WALL_EMAP,
+    TPT_BUF_BLOCK_AIR, TPT_BUF_BLOCK_AIRH, TPT_BUF_ELEM_BLOCK_AIR,
+    TPT_BUF_ELEM_BLOCK_AIRH, TPT_BUF_GRAV_MASK, TPT_BUF_GRAV_MASS,
+    TPT_BUF_GRAV_MASS_PREV, TPT_BUF_GRAV_FX, TPT_BUF_GRAV_FY,
+    /* tables */
+    TPT_BUF_ELEM_PROPS,  /* TPT_PT_NUM*TPT_ELEM_STRIDE words   */
+    TPT_BUF_MOVE_LUT,    /* TPT_MOVE_LUT_WORDS words            */
+    TPT_BUF_COUNT
+} tpt_buf_id;
+
+typedef struct tpt_ctx tpt_ctx;
+
+/* Error text of the last failing call on this thread ("" if none). */
+TPT_API const char *tpt_last_error(void);
+
+/* SimulationEngine::new (crates/tpt-sim/src/lib.rs:140) together with
+ * SimBuffers::new (crates/tpt-sim/src/buffers/layout.rs:42-257): allocates
+ * every buffer for the canvas, zero-filled, and creates the context's
+ * stream.  The reference's host shadows stay with the caller. */
+TPT_API int tpt_create(const tpt_config *cfg, tpt_ctx **out);
+TPT_API int tpt_destroy(tpt_ctx *ctx);
+
+/* Replaces SimBuffers::upload_element_tables + move_kernel::refresh_move_tables
+ * (crates/tpt-sim/src/move_kernel/mod.rs:1578-1590): the caller passes the
+ * GpuElementProps rows (elements/mod.rs:482-556) and the packed movement LUT
+ * (build_move_lut, move_kernel/mod.rs:1534-1576). */
+TPT_API int tpt_set_element_table(tpt_ctx *ctx, const uint32_t *props, size_t props_words,
+                          const uint32_t *move_lut, size_t lut_words);
+
+/* SimBuffers::sync_host_to_gpu / sync_gpu_to_host for one buffer
+ * (crates/tpt-sim/src/buffers/gpu_sync.rs:31, 244).  Byte offset and length
+ * must lie inside the buffer and be multiples of 4. */
+TPT_API int tpt_upload(tpt_ctx *ctx, int buf, const void *src, size_t offset_bytes, size_t bytes);
+TPT_API int tpt_download(tpt_ctx *ctx, int buf, void *dst, size_t offset_bytes, size_t bytes);
+TPT_API int tpt_buffer_bytes(tpt_ctx *ctx, int buf, size_t *bytes);
+
+/* SimBuffers::used_slot_bound (crates/tpt-sim/src/buffers/mod.rs:552-559):
+ * slot passes scan [0, bound).  The host owns slot allocation, so it sets
+ * the bound after loading particles; clamped to [1, capacity]. */
+TPT_API int tpt_set_slot_bound(tpt_ctx *ctx, uint32_t bound);
+
+/* Per-pass entry points, in tick order. */
+
+/* air_kernel::step_air_gpu (crates/tpt-sim/src/air_kernel/mod.rs:120-299). */
+TPT_API int tpt_step_air(tpt_ctx *ctx);
+
+/* gravity_kernel::step_gravity_gpu (crates/tpt-sim/src/gravity_kernel.rs:826-1016)
+ * including GravityFft::solve (crates/tpt-sim/src/gravity_gpu/mod.rs:114).
+ * When newton_grav is 0 the force grids are zeroed
+ * (SimBuffers::zero_gravity_forces, lib.rs:344-352). */
+TPT_API int tpt_step_gravity(tpt_ctx *ctx, uint32_t seed);
+
+/* GravityFft::solve alone (gravity_gpu/mod.rs:114-135): solves the force
+ * grids from TPT_BUF_GRAV_MASS_PREV (fixed point, 1/65536 units). */
+TPT_API int tpt_gravity_solve(tpt_ctx *ctx);
+
+/* SimBuffers::recalc_free_particles_device
+ * (crates/tpt-sim/src/buffers/spatial.rs:371-472). */
+TPT_API int tpt_recalc_free_particles(tpt_ctx *ctx, uint32_t seed, int stacking);
+
+/* thermal_kernel::step_thermal_and_transitions_gpu
+ * (crates/tpt-sim/src/thermal_kernel.rs:1574-1717). */
+TPT_API int tpt_step_thermal(tpt_ctx *ctx, uint32_t seed, int legacy);
+
+/* move_kernel::step_movement_gpu (crates/tpt-sim/src/move_kernel/mod.rs:1176-1227). */
+TPT_API int tpt_step_movement(tpt_ctx *ctx, uint32_t seed);
+
+/* SimulationEngine::step (crates/tpt-sim/src/lib.rs:303-459) for the passes
+ * above, with seed = rotl(frame_count*0x9E3779B9, 13) ^ 0x85EBCA6B
+ * (lib.rs:315-318) and CheckStacking on every 10th frame (lib.rs:359). */
+TPT_API int tpt_step(tpt_ctx *ctx, uint64_t frame_count);
+
+/* The tick seed of lib.rs:315-318. */
+TPT_API uint32_t tpt_frame_seed(uint64_t frame_count);
+
+/* Waits for all work queued on the context's stream. */
+TPT_API int tpt_synchronize(tpt_ctx *ctx);
+
+/* Pass timing with HIP events on the context's stream (no reference
+ * counterpart; the reference only times whole frames, tpt-app main.rs:637-726).
+ * When enabled every kernel launch is bracketed by events; totals accumulate
+ * until tpt_reset_pass_times.  Names are static strings. */
+TPT_API int tpt_set_profiling(tpt_ctx *ctx, int enable);
+TPT_API int tpt_reset_pass_times(tpt_ctx *ctx);
+TPT_API int tpt_get_pass_times(tpt_ctx *ctx, const char **names, double *total_ms,
+                       uint64_t *launches, int max_entries, int *n_entries);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TPT_HIP_H */

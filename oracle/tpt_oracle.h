@@ -1,0 +1,74 @@
+/*
+ * CPU oracle for the tpt-sim tick -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A single-threaded C restatement of the reference kernels, written from the
+ * Rust/CubeCL source text of pppopipupu/The-Powder-Toy-Rust-wgpu.  Each
+ * function cites the reference kernel it follows.  Only the tests, the smoke
+ * check and the CPU-baseline leg of the benchmark may load it; the product
+ * library (libtpt_hip) never does.
+ *
+ * Numerics: built with -ffp-contract=off and no fast-math; expressions keep
+ * the reference's order; float->int conversions saturate toward zero with
+ * NaN -> 0 (the same convention as the HIP kernels).
+ *
+ * Races in the reference (non-atomic read-modify-writes) are replaced by one
+ * canonical serialisation shared with the HIP kernels; each affected function
+ * names the rule it implements.
+ */
+#ifndef TPT_ORACLE_H
+#define TPT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Simulation state: every pointer is a host array in the reference layout
+ * (slot arrays of `capacity` words, pixel maps of xres*yres words, cell grids
+ * of xcells*ycells words).  Field names follow SimBuffers. */
+typedef struct oracle_state {
+    /* config (SimulationConfig, crates/tpt-core/src/particle.rs:166-346) */
+    uint32_t xres, yres, capacity, slot_bound;
+    uint32_t aheat_enable, heat_enable, newton_grav;
+    uint32_t edge_mode, gravity_mode, air_mode, convection_mode, grav_mass_mode;
+    float ambient_air_temp, edge_pressure, edge_velocity_x, edge_velocity_y;
+    float custom_gravity_x, custom_gravity_y, vorticity_coeff;
+    /* slot arrays */
+    uint32_t *part_ty; int32_t *part_life; uint32_t *part_ctype;
+    float *part_x, *part_y, *part_vx, *part_vy, *part_temp, *part_equil;
+    uint32_t *part_flags; int32_t *part_tmp, *part_tmp2, *part_tmp3, *part_tmp4;
+    uint32_t *part_dcolour, *intent_target;
+    /* pixel maps */
+    uint32_t *pmap, *photons, *pmap_count, *cell_winner;
+    /* cell grids */
+    float *air_vx, *air_vy, *air_pv, *air_hv, *air_ovx, *air_ovy, *air_opv, *air_ohv;
+    float *fan_vx, *fan_vy;
+    uint32_t *wall_bmap, *wall_emap, *block_air, *block_airh, *elem_block_air, *elem_block_airh;
+    uint32_t *grav_mask; int32_t *grav_mass, *grav_mass_prev;
+    float *grav_fx, *grav_fy;
+    /* tables */
+    const uint32_t *elem_props;   /* 256 x 32 */
+    const uint32_t *move_lut;     /* 256 x 12 + 256 x 256 */
+} oracle_state;
+
+/* air_kernel::step_air_gpu (air_kernel/mod.rs:120-299). */
+void oracle_step_air(oracle_state *s);
+
+/* The individual air kernels, in launch order. */
+void oracle_air_pressure(oracle_state *s);      /* air_kernel/pressure.rs:174-238 */
+void oracle_air_velocity(oracle_state *s);      /* air_kernel/pressure.rs:248-299 */
+void oracle_air_smooth_advect(oracle_state *s); /* air_kernel/advect.rs:176-360   */
+void oracle_air_heat_convect(oracle_state *s);  /* air_kernel/heat.rs:31-253      */
+void oracle_air_copy_back(oracle_state *s);     /* air_kernel/settle.rs:22-28     */
+void oracle_air_block_masks(oracle_state *s);   /* air_kernel/settle.rs:51-93     */
+void oracle_hot_air_pressure(oracle_state *s);  /* air_kernel/settle.rs:119-160   */
+
+/* far_advect_target (air_kernel/advect.rs:66-152); exposed for unit tests. */
+void oracle_far_advect_target(uint32_t x, uint32_t y, float dx, float dy, const uint32_t *block,
+                              uint32_t xcells, uint32_t ycells, uint32_t heat, float *tx, float *ty);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,275 @@
+// Context, buffers, tables, pass timing and the whole-tick driver of
+// libtpt_hip (C ABI in include/tpt_hip.h).
+//
+// The buffer set mirrors SimBuffers (crates/tpt-sim/src/buffers/mod.rs:81-393,
+// allocated by layout.rs:42-257): one device allocation per logical array so
+// a caller can upload/download exactly the reference's arrays.
+
+#include "tpt_internal.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+const char *const tpt_pass_names[PASS_COUNT] = {
+    "air_pressure_velocity", "air_smooth_advect", "air_heat_convect", "air_settle",
+    "grav_mass_seed", "grav_fft_rows", "grav_fft_cols", "grav_fft_inverse_extract",
+    "grav_apply", "grav_build_mass", "recalc_clear", "recalc_scatter", "check_stacking",
+    "thermal_conduction", "thermal_conduction_cells", "thermal_apply", "phase_transition",
+    "phase_transition_cells", "refresh_pmap_types", "air_deposit", "air_deposit_cells",
+    "move_clear_intents", "move_velocity_intent", "move_commit", "move_rebuild_maps",
+};
+
+static thread_local char g_err[512];
+
+extern "C" const char *tpt_last_error(void) { return g_err; }
+
+int tpt_set_error(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int tpt_check_launch(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return tpt_set_error(TPT_EDEVICE, "launch of %s failed: %s", what, hipGetErrorString(e));
+    return TPT_OK;
+}
+
+// Folds the oldest pending measurement of `pass` into its total.
+static void fold_oldest(tpt_ctx *ctx, int pass) {
+    const int R = tpt_ctx::RING;
+    int slot = (ctx->ev_head[pass] - ctx->ev_pending[pass] + R) % R;
+    float ms = 0.0f;
+    hipEventSynchronize(ctx->ev_stop[pass][slot]);
+    if (hipEventElapsedTime(&ms, ctx->ev_start[pass][slot], ctx->ev_stop[pass][slot]) == hipSuccess)
+        ctx->pass_ms[pass] += ms;
+    ctx->ev_pending[pass] -= 1;
+}
+
+void tpt_pass_begin(tpt_ctx *ctx, int pass) {
+    if (!ctx->profiling) return;
+    if (ctx->ev_pending[pass] == tpt_ctx::RING) fold_oldest(ctx, pass);
+    hipEventRecord(ctx->ev_start[pass][ctx->ev_head[pass]], ctx->stream);
+}
+
+void tpt_pass_end(tpt_ctx *ctx, int pass) {
+    if (!ctx->profiling) return;
+    int slot = ctx->ev_head[pass];
+    hipEventRecord(ctx->ev_stop[pass][slot], ctx->stream);
+    ctx->ev_head[pass] = (slot + 1) % tpt_ctx::RING;
+    ctx->ev_pending[pass] += 1;
+    ctx->pass_launches[pass] += 1;
+}
+
+static size_t slot_words(const tpt_ctx *c) { return c->cfg.capacity; }
+
+static int alloc_buffers(tpt_ctx *c) {
+    const size_t cap = slot_words(c);
+    for (int id = 0; id < TPT_BUF_COUNT; ++id) {
+        size_t words = 0;
+        if (id <= TPT_BUF_INTENT_TARGET) words = cap;
+        else if (id <= TPT_BUF_CELL_WINNER) words = c->pixels;
+        else if (id <= TPT_BUF_GRAV_FY) words = c->cells;
+        else if (id == TPT_BUF_ELEM_PROPS) words = (size_t)TPT_PT_NUM * TPT_ELEM_STRIDE;
+        else if (id == TPT_BUF_MOVE_LUT) words = TPT_MOVE_LUT_WORDS;
+        c->buf_bytes[id] = words * 4;
+        if (hipMalloc(&c->buf[id], words * 4) != hipSuccess)
+            return tpt_set_error(TPT_ENOMEM, "hipMalloc of buffer %d (%zu bytes) failed", id, words * 4);
+        if (hipMemsetAsync(c->buf[id], 0, words * 4, c->stream) != hipSuccess)
+            return tpt_set_error(TPT_EDEVICE, "hipMemsetAsync of buffer %d failed", id);
+    }
+    // Every slot starts as "no intent" (move_kernel/mod.rs:124).
+    if (hipMemsetAsync(c->buf[TPT_BUF_INTENT_TARGET], 0xFF, cap * 4, c->stream) != hipSuccess)
+        return tpt_set_error(TPT_EDEVICE, "intent clear failed");
+    size_t cb = (size_t)c->cells * 4;
+    if (hipMalloc(&c->cell_scratch_u, cb) != hipSuccess || hipMalloc(&c->cell_scratch_f, cb) != hipSuccess ||
+        hipMalloc(&c->cell_scratch_u2, cb) != hipSuccess || hipMalloc(&c->cell_scratch_f2, cb) != hipSuccess ||
+        hipMalloc(&c->slot_scratch_u, cap * 4) != hipSuccess || hipMalloc(&c->counters, 256) != hipSuccess)
+        return tpt_set_error(TPT_ENOMEM, "scratch allocation failed");
+    hipMemsetAsync(c->cell_scratch_u, 0, cb, c->stream);
+    hipMemsetAsync(c->cell_scratch_u2, 0, cb, c->stream);
+    hipMemsetAsync(c->counters, 0, 256, c->stream);
+    return TPT_OK;
+}
+
+static void free_all(tpt_ctx *c) {
+    for (int id = 0; id < TPT_BUF_COUNT; ++id)
+        if (c->buf[id]) hipFree(c->buf[id]);
+    void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
+                     c->slot_scratch_u, c->counters};
+    for (void *p : extra)
+        if (p) hipFree(p);
+    tpt_gravity_free(c);
+    for (int p = 0; p < PASS_COUNT; ++p)
+        for (int r = 0; r < tpt_ctx::RING; ++r) {
+            if (c->ev_start[p][r]) hipEventDestroy(c->ev_start[p][r]);
+            if (c->ev_stop[p][r]) hipEventDestroy(c->ev_stop[p][r]);
+        }
+    if (c->stream) hipStreamDestroy(c->stream);
+}
+
+extern "C" int tpt_create(const tpt_config *cfg, tpt_ctx **out) {
+    g_err[0] = 0;
+    if (!cfg || !out) return tpt_set_error(TPT_EINVAL, "null argument");
+    *out = nullptr;
+    if (cfg->xres < TPT_CELL || cfg->yres < TPT_CELL || cfg->xres % TPT_CELL || cfg->yres % TPT_CELL ||
+        cfg->xres > TPT_MAX_DIM || cfg->yres > TPT_MAX_DIM)
+        return tpt_set_error(TPT_EINVAL, "canvas %ux%u must be a multiple of 4 in [4, %u]", cfg->xres,
+                             cfg->yres, TPT_MAX_DIM);
+    if (cfg->capacity == 0 || cfg->capacity > TPT_MAX_CAPACITY)
+        return tpt_set_error(TPT_EINVAL, "capacity %u outside [1, 0x7FFFFF]", cfg->capacity);
+    if (cfg->edge_mode > 2 || cfg->gravity_mode > 3 || cfg->air_mode > 4 || cfg->convection_mode > 2 ||
+        cfg->grav_mass_mode > 1)
+        return tpt_set_error(TPT_EINVAL, "enum field out of range");
+    tpt_ctx *c = new (std::nothrow) tpt_ctx();
+    if (!c) return tpt_set_error(TPT_ENOMEM, "host allocation failed");
+    c->cfg = *cfg;
+    c->xcells = cfg->xres / TPT_CELL;
+    c->ycells = cfg->yres / TPT_CELL;
+    c->cells = c->xcells * c->ycells;
+    c->pixels = cfg->xres * cfg->yres;
+    if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return tpt_set_error(TPT_EDEVICE, "no HIP device or stream creation failed");
+    }
+    int rc = alloc_buffers(c);
+    if (rc == TPT_OK) {
+        for (int p = 0; p < PASS_COUNT && rc == TPT_OK; ++p)
+            for (int r = 0; r < tpt_ctx::RING && rc == TPT_OK; ++r)
+                if (hipEventCreate(&c->ev_start[p][r]) != hipSuccess || hipEventCreate(&c->ev_stop[p][r]) != hipSuccess)
+                    rc = tpt_set_error(TPT_EDEVICE, "event creation failed");
+    }
+    if (rc == TPT_OK && hipStreamSynchronize(c->stream) != hipSuccess)
+        rc = tpt_set_error(TPT_EDEVICE, "initial clear failed");
+    if (rc != TPT_OK) {
+        free_all(c);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return TPT_OK;
+}
+
+extern "C" int tpt_destroy(tpt_ctx *ctx) {
+    if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
+    hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    delete ctx;
+    return TPT_OK;
+}
+
+extern "C" int tpt_set_element_table(tpt_ctx *ctx, const uint32_t *props, size_t props_words,
+                                     const uint32_t *move_lut, size_t lut_words) {
+    if (!ctx || !props || !move_lut) return tpt_set_error(TPT_EINVAL, "null argument");
+    if (props_words != (size_t)TPT_PT_NUM * TPT_ELEM_STRIDE || lut_words != (size_t)TPT_MOVE_LUT_WORDS)
+        return tpt_set_error(TPT_EINVAL, "table sizes %zu/%zu, expected %u/%u", props_words, lut_words,
+                             TPT_PT_NUM * TPT_ELEM_STRIDE, TPT_MOVE_LUT_WORDS);
+    TPT_HIP_CHECK(hipMemcpyAsync(ctx->buf[TPT_BUF_ELEM_PROPS], props, props_words * 4, hipMemcpyHostToDevice, ctx->stream));
+    TPT_HIP_CHECK(hipMemcpyAsync(ctx->buf[TPT_BUF_MOVE_LUT], move_lut, lut_words * 4, hipMemcpyHostToDevice, ctx->stream));
+    TPT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->tables_set = true;
+    return TPT_OK;
+}
+
+static int check_range(tpt_ctx *ctx, int buf, size_t off, size_t bytes) {
+    if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
+    if (buf < 0 || buf >= TPT_BUF_COUNT) return tpt_set_error(TPT_EINVAL, "buffer id %d out of range", buf);
+    if (off % 4 || bytes % 4) return tpt_set_error(TPT_EINVAL, "offset/length must be multiples of 4");
+    if (off > ctx->buf_bytes[buf] || bytes > ctx->buf_bytes[buf] - off)
+        return tpt_set_error(TPT_EINVAL, "range [%zu, %zu) outside buffer %d of %zu bytes", off, off + bytes, buf,
+                             ctx->buf_bytes[buf]);
+    return TPT_OK;
+}
+
+extern "C" int tpt_upload(tpt_ctx *ctx, int buf, const void *src, size_t off, size_t bytes) {
+    int rc = check_range(ctx, buf, off, bytes);
+    if (rc) return rc;
+    if (!src && bytes) return tpt_set_error(TPT_EINVAL, "null source");
+    if (!bytes) return TPT_OK;
+    TPT_HIP_CHECK(hipMemcpyAsync((char *)ctx->buf[buf] + off, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    TPT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (buf == TPT_BUF_ELEM_PROPS || buf == TPT_BUF_MOVE_LUT) ctx->tables_set = true;
+    return TPT_OK;
+}
+
+extern "C" int tpt_download(tpt_ctx *ctx, int buf, void *dst, size_t off, size_t bytes) {
+    int rc = check_range(ctx, buf, off, bytes);
+    if (rc) return rc;
+    if (!dst && bytes) return tpt_set_error(TPT_EINVAL, "null destination");
+    if (!bytes) return TPT_OK;
+    TPT_HIP_CHECK(hipMemcpyAsync(dst, (char *)ctx->buf[buf] + off, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    TPT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return TPT_OK;
+}
+
+extern "C" int tpt_buffer_bytes(tpt_ctx *ctx, int buf, size_t *bytes) {
+    if (!ctx || !bytes) return tpt_set_error(TPT_EINVAL, "null argument");
+    if (buf < 0 || buf >= TPT_BUF_COUNT) return tpt_set_error(TPT_EINVAL, "buffer id %d out of range", buf);
+    *bytes = ctx->buf_bytes[buf];
+    return TPT_OK;
+}
+
+extern "C" int tpt_set_slot_bound(tpt_ctx *ctx, uint32_t bound) {
+    if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
+    if (bound < 1) bound = 1;
+    if (bound > ctx->cfg.capacity) bound = ctx->cfg.capacity;
+    ctx->slot_bound = bound;
+    return TPT_OK;
+}
+
+extern "C" int tpt_synchronize(tpt_ctx *ctx) {
+    if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
+    TPT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return TPT_OK;
+}
+
+extern "C" uint32_t tpt_frame_seed(uint64_t frame_count) {
+    uint32_t v = (uint32_t)frame_count * 0x9E3779B9u;
+    v = (v << 13) | (v >> 19);
+    return v ^ 0x85EBCA6Bu;
+}
+
+extern "C" int tpt_step_air(tpt_ctx *ctx) {
+    if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
+    if (!ctx->tables_set) return tpt_set_error(TPT_ESTATE, "element table not set");
+    return tpt_air_step(ctx);
+}
+
+extern "C" int tpt_set_profiling(tpt_ctx *ctx, int enable) {
+    if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
+    ctx->profiling = enable != 0;
+    return TPT_OK;
+}
+
+extern "C" int tpt_reset_pass_times(tpt_ctx *ctx) {
+    if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
+    TPT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (int p = 0; p < PASS_COUNT; ++p) {
+        ctx->pass_ms[p] = 0.0;
+        ctx->pass_launches[p] = 0;
+        ctx->ev_pending[p] = 0;
+    }
+    return TPT_OK;
+}
+
+extern "C" int tpt_get_pass_times(tpt_ctx *ctx, const char **names, double *total_ms, uint64_t *launches,
+                                  int max_entries, int *n_entries) {
+    if (!ctx || !n_entries) return tpt_set_error(TPT_EINVAL, "null argument");
+    TPT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (int p = 0; p < PASS_COUNT; ++p)
+        while (ctx->ev_pending[p] > 0) fold_oldest(ctx, p);
+    int n = PASS_COUNT < max_entries ? PASS_COUNT : max_entries;
+    for (int p = 0; p < n; ++p) {
+        if (names) names[p] = tpt_pass_names[p];
+        if (total_ms) total_ms[p] = ctx->pass_ms[p];
+        if (launches) launches[p] = ctx->pass_launches[p];
+    }
+    *n_entries = n;
+    return TPT_OK;
+}

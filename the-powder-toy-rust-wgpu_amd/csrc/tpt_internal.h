@@ -1,0 +1,141 @@
+// Internal declarations shared by the HIP translation units of libtpt_hip.
+//
+// Numerics contract for every kernel in this library: no FMA contraction
+// (built with -ffp-contract=off), IEEE binary32 division and square root
+// (hipcc's correctly rounded defaults), expressions evaluated in the order
+// the reference writes them, and float->int conversions through tpt_f2i
+// (truncation toward zero, saturating, NaN -> 0).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/tpt_hip.h"
+
+#define TPT_CELL 4
+
+// Passes that carry their own timing slot (see tpt_get_pass_times).
+enum tpt_pass {
+    PASS_AIR_PV = 0,      // pressure + velocity, fused
+    PASS_AIR_ADVECT,      // smoothing, far advection, vorticity, fans, caps
+    PASS_AIR_HEAT,        // ambient heat diffusion/advection + convection
+    PASS_AIR_SETTLE,      // copy-back, wall masks, emap decay, HotAir
+    PASS_GRAV_MASS,       // tool seed / mass snapshot bookkeeping
+    PASS_GRAV_FFT_ROWS,   // padded forward row FFT
+    PASS_GRAV_FFT_COLS,   // forward column FFT * kernel spectra, inverse column FFT
+    PASS_GRAV_FFT_INV,    // inverse row FFT + force extraction
+    PASS_GRAV_APPLY,      // Loss + Newtonian force on particles
+    PASS_GRAV_BUILD,      // fixed-point mass deposit
+    PASS_RECALC_CLEAR,    // pmap/photons/pmap_count clear
+    PASS_RECALC_SCATTER,  // life decay + map scatter
+    PASS_STACKING,        // CheckStacking
+    PASS_CONDUCT,         // conduction gate, ambient exchange, equilibrium
+    PASS_CONDUCT_CELL,    // per-cell resolution of air_hv / block_airh writes
+    PASS_CONDUCT_APPLY,   // last-writer temperature apply
+    PASS_TRANSITION,      // phase transitions
+    PASS_TRANSITION_CELL, // per-cell resolution of air_pv writes
+    PASS_REFRESH_PMAP,    // pmap type refresh
+    PASS_AIR_DEPOSIT,     // particle -> air drag
+    PASS_AIR_DEPOSIT_CELL,
+    PASS_MOVE_CLEAR,      // cell_winner clear
+    PASS_MOVE_INTENT,     // velocity integration + destination + claim
+    PASS_MOVE_COMMIT,     // winner check + move/swap
+    PASS_MOVE_REBUILD,    // pmap/photons rebuild
+    PASS_COUNT
+};
+
+extern const char *const tpt_pass_names[PASS_COUNT];
+
+struct tpt_fft_state {
+    uint32_t width = 0, height = 0;   // padded transform size (powers of two)
+    uint32_t xcells = 0, ycells = 0;
+    float2 *work = nullptr;           // padded mass spectrum, width*height
+    float2 *spec_x = nullptr;         // column-transformed products
+    float2 *spec_y = nullptr;
+    float2 *kernel_x = nullptr;       // cached kernel spectra
+    float2 *kernel_y = nullptr;
+    float2 *twiddle = nullptr;        // exp(-2 pi i k / N), N = max(width,height)
+    bool kernels_ready = false;
+};
+
+struct tpt_ctx {
+    tpt_config cfg;
+    uint32_t xcells = 0, ycells = 0, cells = 0, pixels = 0;
+    uint32_t slot_bound = 1;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    void *buf[TPT_BUF_COUNT] = {};
+    size_t buf_bytes[TPT_BUF_COUNT] = {};
+    bool tables_set = false;
+
+    // Library-internal scratch (not part of the reference layout).
+    uint32_t *cell_scratch_u = nullptr;   // cells words
+    float *cell_scratch_f = nullptr;      // cells words
+    uint32_t *cell_scratch_u2 = nullptr;  // cells words
+    float *cell_scratch_f2 = nullptr;     // cells words
+    uint32_t *slot_scratch_u = nullptr;   // capacity words
+    uint32_t *counters = nullptr;         // small device counters
+    tpt_fft_state fft;
+
+    // Profiling: a ring of event pairs per pass, folded lazily so that
+    // timing never stalls the launch stream.
+    static constexpr int RING = 32;
+    bool profiling = false;
+    hipEvent_t ev_start[PASS_COUNT][RING] = {};
+    hipEvent_t ev_stop[PASS_COUNT][RING] = {};
+    int ev_head[PASS_COUNT] = {};     // next ring slot to record
+    int ev_pending[PASS_COUNT] = {};  // recorded, not yet folded
+    double pass_ms[PASS_COUNT] = {};
+    uint64_t pass_launches[PASS_COUNT] = {};
+};
+
+// Error plumbing ----------------------------------------------------------
+int tpt_set_error(int code, const char *fmt, ...);
+#define TPT_HIP_CHECK(expr)                                                       \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess)                                                     \
+            return tpt_set_error(TPT_EDEVICE, "%s failed: %s (%s:%d)", #expr,     \
+                                 hipGetErrorString(_e), __FILE__, __LINE__);      \
+    } while (0)
+
+// Pass timing: bracket launches with events when profiling is on.
+void tpt_pass_begin(tpt_ctx *ctx, int pass);
+void tpt_pass_end(tpt_ctx *ctx, int pass);
+int tpt_check_launch(const char *what);
+
+template <typename T> static inline T *tpt_buf(tpt_ctx *ctx, int id) {
+    return reinterpret_cast<T *>(ctx->buf[id]);
+}
+
+// Pass drivers implemented per translation unit.
+int tpt_air_step(tpt_ctx *ctx);
+int tpt_gravity_step(tpt_ctx *ctx, uint32_t seed);
+int tpt_gravity_solve_only(tpt_ctx *ctx);
+int tpt_gravity_free(tpt_ctx *ctx);
+int tpt_recalc_step(tpt_ctx *ctx, uint32_t seed, int stacking);
+int tpt_thermal_step(tpt_ctx *ctx, uint32_t seed, int legacy);
+int tpt_movement_step(tpt_ctx *ctx, uint32_t seed);
+
+// Device helpers ------------------------------------------------------------
+__device__ __forceinline__ float tpt_clampf(float v, float lo, float hi) {
+    float out = v;
+    if (out < lo) out = lo;
+    if (out > hi) out = hi;
+    return out;
+}
+
+// Truncation toward zero with saturation; NaN -> 0.
+__host__ __device__ __forceinline__ int tpt_f2i(float f) {
+    if (!(f == f)) return 0;
+    if (f >= 2147483648.0f) return 2147483647;
+    if (f <= -2147483648.0f) return (int)0x80000000u;
+    return (int)f;
+}
+
+__host__ __device__ __forceinline__ uint32_t tpt_f2u(float f) {
+    if (!(f == f) || f <= 0.0f) return 0u;
+    if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)f;
+}
