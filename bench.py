@@ -1,0 +1,182 @@
+"""Benchmark: plain HIP tick throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2048x2048]
+
+Prints ONE JSON line.  A step is one tick of the built passes over the
+synthetic 2048x2048 workload (512x512 air cells, ~2.1M particles, 64 wall
+rectangles), with every buffer resident in HBM before timing starts.
+`roofline` reports the dominant kernel's achieved bandwidth from HIP events
+recorded on the library's own stream; `cpu_baseline` times the CPU oracle
+(single thread) on a bounded number of ticks of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "the-powder-toy-rust-wgpu_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+
+# Passes the tick runs, in SimulationEngine::step order (lib.rs:334-409).
+TICK_PASSES = ["air"]
+
+# Algorithmic bytes per launch of each kernel: every array the kernel must
+# read or write, once (stencil re-reads and LDS reuse are free).
+#   C = air cells, P = pixels.
+def kernel_bytes(name, C, P, N):
+    table = {
+        # pv, vx, vy, block_air in; opv, ovx, ovy out
+        "air_pressure_velocity": 28 * C,
+        # ovx, ovy, opv, block_air, wall_bmap, fan_vx, fan_vy in; vx, vy, pv out
+        "air_smooth_advect": 40 * C,
+        # hv, vx, vy, block_airh in; ohv, vx', vy' out
+        "air_heat_convect": 28 * C,
+        # ohv, vx', vy', wall_bmap, wall_emap, elem masks x2, pv, 16 pmap words in;
+        # hv, vx, vy, wall_emap, block_air, block_airh, pv out
+        "air_settle": 124 * C,
+    }
+    return table.get(name)
+
+
+def run_pass(ctx, name, frame):
+    from tpt_hip import lib as L
+    seed = L.frame_seed(frame)
+    if name == "air":
+        ctx.step_air()
+    elif name == "gravity":
+        ctx.step_gravity(seed)
+    elif name == "recalc":
+        ctx.recalc_free_particles(seed, frame % 10 == 0)
+    elif name == "thermal":
+        ctx.step_thermal(seed)
+    elif name == "movement":
+        ctx.step_movement(seed)
+    elif name == "tick":
+        ctx.step(frame)
+
+
+def oracle_pass(o, name, frame):
+    if name == "air":
+        o.call("oracle_step_air")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="2048x2048")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(args.calibrate_copy << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": a.numel() * 4, "launches": 10}))
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    from tpt_hip import lib as L
+    from tpt_hip import scene as S
+
+    st = S.make_scene(args.workload)
+    C, P = st.xcells * st.ycells, st.cfg["xres"] * st.cfg["yres"]
+    N = int((st.arrays["part_ty"] != 0).sum())
+    passes = TICK_PASSES
+    ctx = L.Context(st.cfg)
+    ctx.load_state(st)
+    frame = 0
+    for _ in range(args.warmup):
+        for p in passes:
+            run_pass(ctx, p, frame)
+        frame += 1
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for p in passes:
+            run_pass(ctx, p, frame)
+        frame += 1
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    ticks_per_s = args.steps / dt
+
+    # Per-kernel durations from HIP events on the library's stream, over a
+    # second run of the same number of ticks.
+    ctx.set_profiling(True)
+    ctx.reset_pass_times()
+    for _ in range(args.steps):
+        for p in passes:
+            run_pass(ctx, p, frame)
+        frame += 1
+    times = ctx.pass_times()
+    ctx.set_profiling(False)
+    per_kernel = {}
+    for name, (ms, n) in times.items():
+        if n:
+            b = kernel_bytes(name, C, P, N)
+            avg_us = ms / n * 1e3
+            per_kernel[name] = {"avg_us": round(avg_us, 3), "launches": n,
+                                "alg_bytes": b,
+                                "gbs": round(b / (avg_us * 1e-6) / 1e9, 1) if b else None}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches"])
+    d = per_kernel[dom]
+    achieved = d["gbs"]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                "traffic": None, "alg_bytes_per_launch": d["alg_bytes"], "avg_us": d["avg_us"]}
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ref = S.make_scene(args.workload)
+        o = O.OracleState(ref)
+        n_cpu, t0 = 0, time.perf_counter()
+        while True:
+            for p in passes:
+                oracle_pass(o, p, n_cpu)
+            n_cpu += 1
+            el = time.perf_counter() - t0
+            if el >= args.cpu_seconds or n_cpu >= args.steps:
+                break
+        cpu = {"value": round(n_cpu / el, 3), "unit": "ticks/s", "cores": 1, "kind": "port",
+               "sample": f"{n_cpu} ticks of the same {args.workload} workload and passes, single-threaded C oracle"}
+
+    out = {
+        "metric": "ticks/s @2048x2048 plain vs batched vs air-surrogate (relL2<=1e-2); HBM GB/s",
+        "value": round(ticks_per_s, 2),
+        "unit": "ticks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload} plain tick, passes: {'+'.join(passes)}",
+                   "air_cells": C, "pixels": P, "particles": N, "parallelism": "single-gpu"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "per_kernel": per_kernel,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -219,12 +219,11 @@ def place_particles(st, rng, n, mix=MIX_BASE, temp_range=(273.0, 373.0), patches
     a["part_y"][slots] = (px // xres).astype(np.float32)
     a["part_vx"][slots] = 0.0
     a["part_vy"][slots] = 0.0
-    a["part_life"][slots] = np.array([props[t]["default_life"] for t in ty], dtype=np.int32)
-    a["part_ctype"][slots] = np.array([props[t]["default_ctype"] for t in ty], dtype=np.uint32)
-    a["part_tmp"][slots] = np.array([props[t]["default_tmp"] for t in ty], dtype=np.int32)
-    a["part_tmp2"][slots] = np.array([props[t]["default_tmp2"] for t in ty], dtype=np.int32)
-    a["part_tmp3"][slots] = np.array([props[t]["default_tmp3"] for t in ty], dtype=np.int32)
-    a["part_tmp4"][slots] = np.array([props[t]["default_tmp4"] for t in ty], dtype=np.int32)
+    for field, src, dt in (("part_life", "default_life", np.int32), ("part_ctype", "default_ctype", np.uint32),
+                           ("part_tmp", "default_tmp", np.int32), ("part_tmp2", "default_tmp2", np.int32),
+                           ("part_tmp3", "default_tmp3", np.int32), ("part_tmp4", "default_tmp4", np.int32)):
+        table = np.array([e[src] for e in props], dtype=np.int64).astype(dt)
+        a[field][slots] = table[ty]
     a["part_temp"][slots] = rng.uniform(temp_range[0], temp_range[1], size=total).astype(np.float32)
     fire = ty == E.pt("FIRE")
     if fire.any():
