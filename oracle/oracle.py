@@ -71,9 +71,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run `make -C oracle`")
         _lib = ctypes.CDLL(LIB_PATH)
-        for name in ("oracle_step_air", "oracle_air_pressure", "oracle_air_velocity",
-                     "oracle_air_smooth_advect", "oracle_air_heat_convect",
-                     "oracle_air_copy_back", "oracle_air_block_masks", "oracle_hot_air_pressure"):
+        for name in STATE_FUNCS:
             f = getattr(_lib, name)
             f.argtypes = [ctypes.POINTER(OracleStateC)]
             f.restype = None
@@ -81,6 +79,17 @@ def lib():
             ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float, _u32p,
             ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p]
         _lib.oracle_far_advect_target.restype = None
+        _lib.oracle_gravity_solve_f64.argtypes = [ctypes.POINTER(OracleStateC), _f64p, _f64p]
+        _lib.oracle_gravity_solve_f64.restype = None
+        _lib.oracle_fft_twiddle_f32.argtypes = [ctypes.c_uint32, _f32p, _f32p]
+        _lib.oracle_fft_transform_f32.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                  ctypes.c_uint32, _f32p, _f32p, ctypes.c_uint32,
+                                                  ctypes.c_float]
+        _lib.oracle_gravity_mask.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
+        _lib.oracle_lowbias_hash.argtypes = [ctypes.c_uint32]
+        _lib.oracle_lowbias_hash.restype = ctypes.c_uint32
+        _lib.oracle_recalc_free_particles.argtypes = [ctypes.POINTER(OracleStateC), ctypes.c_uint32, ctypes.c_int]
+        _lib.oracle_recalc_free_particles.restype = None
         for name, argt in _OPTIONAL.items():
             f = getattr(_lib, name, None)
             if f is not None:
@@ -90,6 +99,15 @@ def lib():
 
 
 _OPTIONAL = {}
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+# Oracle entry points that take only the state.
+STATE_FUNCS = [
+    "oracle_step_air", "oracle_air_pressure", "oracle_air_velocity", "oracle_air_smooth_advect",
+    "oracle_air_heat_convect", "oracle_air_copy_back", "oracle_air_block_masks",
+    "oracle_hot_air_pressure", "oracle_gravity_solve", "oracle_build_gravity_mass",
+    "oracle_apply_gravity", "oracle_step_gravity",
+]
 
 
 def register(name, argtypes, restype=None):
@@ -128,3 +146,29 @@ class OracleState:
     def call(self, fname, *args):
         self.c.slot_bound = self.state.slot_bound
         return getattr(lib(), fname)(ctypes.byref(self.c), *args)
+
+
+def gravity_solve_f64(state):
+    """FP64 gravity field of state.grav_mass_prev -> (fx, fy) float64 arrays."""
+    o = OracleState(state)
+    n = state.xcells * state.ycells
+    fx, fy = np.zeros(n), np.zeros(n)
+    lib().oracle_gravity_solve_f64(ctypes.byref(o.c), fx.ctypes.data_as(_f64p), fy.ctypes.data_as(_f64p))
+    return fx, fy
+
+
+def fft_f32(data, rows, length, inner, stride, sign, tw_bits):
+    """The reference's radix-2 transform (fft.rs) on an interleaved float32 array, in place."""
+    n = 1 << tw_bits
+    tre, tim = np.zeros(max(n // 2, 1), np.float32), np.zeros(max(n // 2, 1), np.float32)
+    lib().oracle_fft_twiddle_f32(n, tre.ctypes.data_as(_f32p), tim.ctypes.data_as(_f32p))
+    lib().oracle_fft_transform_f32(data.ctypes.data_as(_f32p), rows, length, inner, stride,
+                                   tre.ctypes.data_as(_f32p), tim.ctypes.data_as(_f32p), tw_bits, sign)
+    return data
+
+
+def gravity_mask(wall, xcells, ycells):
+    wall = np.ascontiguousarray(wall, dtype=np.uint32)
+    mask = np.zeros(xcells * ycells, dtype=np.uint32)
+    lib().oracle_gravity_mask(wall.ctypes.data_as(_u32p), xcells, ycells, mask.ctypes.data_as(_u32p))
+    return mask

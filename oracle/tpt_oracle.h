@@ -68,6 +68,21 @@ void oracle_hot_air_pressure(oracle_state *s);  /* air_kernel/settle.rs:119-160 
 void oracle_far_advect_target(uint32_t x, uint32_t y, float dx, float dy, const uint32_t *block,
                               uint32_t xcells, uint32_t ycells, uint32_t heat, float *tx, float *ty);
 
+/* Gravity (gravity_gpu/{mod,fft}.rs, gravity_kernel.rs). */
+void oracle_fft_twiddle_f32(uint32_t max_len, float *re, float *im);
+void oracle_fft_transform_f32(float *data, uint32_t rows, uint32_t len, uint32_t inner, uint32_t stride,
+                              const float *tw_re, const float *tw_im, uint32_t tw_bits, float sign);
+void oracle_gravity_solve(oracle_state *s);
+void oracle_gravity_solve_f64(const oracle_state *s, double *fx, double *fy);
+void oracle_build_gravity_mass(oracle_state *s);
+void oracle_apply_gravity(oracle_state *s);
+void oracle_step_gravity(oracle_state *s);
+void oracle_gravity_mask(const uint32_t *wall, uint32_t xc, uint32_t yc, uint32_t *mask);
+
+/* RecalcFreeParticles + CheckStacking (buffers/spatial.rs:81-472). */
+uint32_t oracle_lowbias_hash(uint32_t seed);
+void oracle_recalc_free_particles(oracle_state *s, uint32_t seed, int stacking);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,83 @@
+"""HIP gravity vs the oracle.
+
+The field is held to the FP64 solve within 1e-4 (max |error| / max |field|,
+the tolerance of the gravity gate); the fixed-point mass grids are bit-exact;
+velocities after Loss + force are bit-exact where the field is exactly zero
+(reference mass table and the config elements: no mass, no force) and within
+the field tolerance otherwise."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tpt_hip import lib as L
+from tpt_hip import scene as S
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+
+
+def rel_err(got, want):
+    scale = np.abs(want).max()
+    return float(np.abs(got.astype(np.float64) - want).max() / scale) if scale else float(np.abs(got).max())
+
+
+@pytest.mark.parametrize("workload,scale", [("612x384", 1.0), ("1600x900", 0.1), ("2048x2048", 0.02)])
+def test_solve_within_1e4_of_fp64(gpu, workload, scale):
+    st = S.make_scene(workload, scale_particles=scale, grav_mass_mode=1)
+    rng = np.random.default_rng(7)
+    n = st.xcells * st.ycells
+    st.arrays["grav_mass_prev"][:] = (rng.uniform(0, 100, n) * 65536).astype(np.int32) * (rng.random(n) < 0.3)
+    st.arrays["grav_mask"][:] = O.gravity_mask(st.arrays["wall_bmap"], st.xcells, st.ycells)
+    print(f"[gravity solve] {workload}", flush=True)
+    fx64, fy64 = O.gravity_solve_f64(st)
+    with L.Context(st.cfg) as ctx:
+        ctx.load_state(st)
+        ctx.gravity_solve()
+        fx = ctx.download("grav_fx", np.float32, n)
+        fy = ctx.download("grav_fy", np.float32, n)
+    ex, ey = rel_err(fx, fx64), rel_err(fy, fy64)
+    print(f"  rel err fx {ex:.2e} fy {ey:.2e}")
+    assert ex < TOL and ey < TOL
+
+
+def test_point_mass_field(gpu):
+    st = S.SimState(S.default_config(256, 256, capacity=1024))
+    st.arrays["grav_mask"][:] = 0xFFFFFFFF
+    st.arrays["grav_mass_prev"].reshape(64, 64)[20, 30] = 5 * 65536
+    with L.Context(st.cfg) as ctx:
+        ctx.load_state(st)
+        ctx.gravity_solve()
+        fx = ctx.download("grav_fx", np.float32).reshape(64, 64)
+    G = float(np.float32(6.673e-1))
+    assert fx[20, 34] == pytest.approx(-G * 5 / 16.0, rel=1e-4)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_step_gravity_against_oracle(gpu, mode):
+    st = S.make_scene("612x384", grav_mass_mode=mode)
+    st.arrays["part_vx"][: st.slot_bound] = np.random.default_rng(1).uniform(-1, 1, st.slot_bound).astype(np.float32)
+    ref = st.copy()
+    o = O.OracleState(ref)
+    with L.Context(st.cfg) as ctx:
+        ctx.load_state(st)
+        for t in range(3):
+            ctx.step_gravity(L.frame_seed(t))
+            o.call("oracle_step_gravity")
+        ctx.fetch(st, ["grav_mass", "grav_mass_prev", "grav_fx", "grav_fy", "part_vx", "part_vy"])
+    assert np.array_equal(st.arrays["grav_mass"], ref.arrays["grav_mass"])
+    assert np.array_equal(st.arrays["grav_mass_prev"], ref.arrays["grav_mass_prev"])
+    if mode == 0:
+        for n in ("grav_fx", "grav_fy", "part_vx", "part_vy"):
+            assert np.array_equal(st.arrays[n].view(np.uint32), ref.arrays[n].view(np.uint32)), n
+    else:
+        assert rel_err(st.arrays["grav_fx"], ref.arrays["grav_fx"].astype(np.float64)) < TOL
+        assert rel_err(st.arrays["part_vx"], ref.arrays["part_vx"].astype(np.float64)) < TOL
+
+
+def test_newton_grav_off_clears_forces(gpu):
+    st = S.make_scene("612x384", scale_particles=0.1, newton_grav=False)
+    st.arrays["grav_fx"][:] = 3.0
+    with L.Context(st.cfg) as ctx:
+        ctx.load_state(st)
+        ctx.step_gravity(0)
+        assert not ctx.download("grav_fx", np.float32).any()

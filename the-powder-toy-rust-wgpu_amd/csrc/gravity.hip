@@ -1,0 +1,450 @@
+// Newtonian gravity for gfx950: gravity_kernel::step_gravity_gpu
+// (crates/tpt-sim/src/gravity_kernel.rs:826-1016) with GravityFft::solve
+// (crates/tpt-sim/src/gravity_gpu/mod.rs:114-278).
+//
+// The reference transforms the whole zero-padded W x H grid with one launch
+// per radix-2 stage (22 full-grid passes per 2-D transform at 1024^2, three
+// transforms per tick).  Here every 1-D transform runs inside LDS and the
+// solve is three launches:
+//
+//   grav_rows_fwd   rows 0..ycells-1 only (the other rows of the padded mass
+//                   grid are zero, so is their transform): mass -> float,
+//                   mask, zero-pad and forward row FFT, fused.
+//   grav_cols       a tile of adjacent columns per workgroup: forward column
+//                   FFT over the ycells non-zero rows, product with both
+//                   cached kernel spectra, both inverse column FFTs; only the
+//                   ycells rows the extraction reads are stored.
+//   grav_rows_inv   both inverse row FFTs of one row + real-part extraction
+//                   into grav_fx / grav_fy under the gravity mask.
+//
+// The kernel spectra (build_kernel_field_kernel, fft.rs:209-236, forward
+// transformed) depend on the canvas only and are built once per context.
+// Twiddles are exp(-2 pi i k / N) evaluated in double and rounded once.
+
+#include "tpt_internal.h"
+
+#include <math.h>
+#include <stdlib.h>
+
+namespace {
+
+constexpr int FFT_THREADS = 256;
+constexpr int COL_TILE_ELEMS = 8192;  // complex values per column tile: 64 KiB of LDS
+constexpr int MAX_EPT = COL_TILE_ELEMS / FFT_THREADS;
+constexpr float M_GRAV = 6.673e-1f;
+constexpr int GRAV_BLOCK = 256;
+
+__device__ __forceinline__ uint32_t bitrev(uint32_t i, int bits) {
+    return bits == 0 ? 0u : (__brev(i) >> (32 - bits));
+}
+
+// In-place radix-2 DIT transform of `cols` interleaved sequences of length n
+// (element i of sequence c at buf[i * cols + c]); input in bit-reversed order.
+__device__ void lds_fft(float2 *buf, int n, int logn, int cols, const float2 *__restrict__ tw, int tw_bits,
+                        float sign) {
+    const int butterflies = (n >> 1) * cols;
+    for (int s = 1; s <= logn; ++s) {
+        const int half = 1 << (s - 1);
+        const int shift = tw_bits - s;
+        for (int b = threadIdx.x; b < butterflies; b += blockDim.x) {
+            const int c = b % cols, k = b / cols;
+            const int pos = k & (half - 1);
+            const int i0 = ((k >> (s - 1)) << s) + pos, i1 = i0 + half;
+            float2 w = tw[pos << shift];
+            w.y *= sign;
+            const float2 x = buf[i0 * cols + c], y = buf[i1 * cols + c];
+            const float tr = w.x * y.x - w.y * y.y;
+            const float ti = w.x * y.y + w.y * y.x;
+            buf[i0 * cols + c] = make_float2(x.x + tr, x.y + ti);
+            buf[i1 * cols + c] = make_float2(x.x - tr, x.y - ti);
+        }
+        __syncthreads();
+    }
+}
+
+struct FftDims {
+    int xc, yc, W, H, logW, logH, tw_bits, cols;
+};
+
+// build_padded_mass_kernel (fft.rs:153-178) + forward row transform.
+__global__ __launch_bounds__(FFT_THREADS) void grav_rows_fwd(const int32_t *__restrict__ mass,
+                                                             const uint32_t *__restrict__ mask,
+                                                             float2 *__restrict__ work,
+                                                             const float2 *__restrict__ tw, FftDims d) {
+    extern __shared__ float2 sm[];
+    const int row = blockIdx.x;
+    for (int i = threadIdx.x; i < d.W; i += blockDim.x) {
+        float v = 0.0f;
+        if (i < d.xc) {
+            const int c = row * d.xc + i;
+            if (mask[c] != 0u) v = (float)mass[c] * (1.0f / 65536.0f);
+        }
+        sm[bitrev(i, d.logW)] = make_float2(v, 0.0f);
+    }
+    __syncthreads();
+    lds_fft(sm, d.W, d.logW, 1, tw, d.tw_bits, 1.0f);
+    for (int i = threadIdx.x; i < d.W; i += blockDim.x) work[(size_t)row * d.W + i] = sm[i];
+}
+
+// Forward column transform, products with both kernel spectra
+// (complex_mul_kernel, fft.rs:129-140) and both inverse column transforms.
+__global__ __launch_bounds__(FFT_THREADS) void grav_cols(const float2 *__restrict__ work,
+                                                         const float2 *__restrict__ kx,
+                                                         const float2 *__restrict__ ky, float2 *__restrict__ sx,
+                                                         float2 *__restrict__ sy, const float2 *__restrict__ tw,
+                                                         FftDims d, int rows_in, int rows_out) {
+    extern __shared__ float2 sm[];
+    const int cols = d.cols, c0 = blockIdx.x * cols, total = d.H * cols;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int r = e / cols, c = e % cols;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (r < rows_in) v = work[(size_t)r * d.W + c0 + c];
+        sm[bitrev(r, d.logH) * cols + c] = v;
+    }
+    __syncthreads();
+    lds_fft(sm, d.H, d.logH, cols, tw, d.tw_bits, 1.0f);
+    float2 reg[MAX_EPT];
+#pragma unroll
+    for (int j = 0; j < MAX_EPT; ++j) {
+        const int e = threadIdx.x + j * FFT_THREADS;
+        if (e < total) reg[j] = sm[e];
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        const float2 *__restrict__ K = pass ? ky : kx;
+        float2 *__restrict__ out = pass ? sy : sx;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < MAX_EPT; ++j) {
+            const int e = threadIdx.x + j * FFT_THREADS;
+            if (e < total) {
+                const int r = e / cols, c = e % cols;
+                const float2 k = K[(size_t)r * d.W + c0 + c];
+                const float2 a = reg[j];
+                sm[bitrev(r, d.logH) * cols + c] = make_float2(a.x * k.x - a.y * k.y, a.x * k.y + a.y * k.x);
+            }
+        }
+        __syncthreads();
+        lds_fft(sm, d.H, d.logH, cols, tw, d.tw_bits, -1.0f);
+        for (int e = threadIdx.x; e < rows_out * cols; e += blockDim.x) {
+            const int r = e / cols, c = e % cols;
+            out[(size_t)r * d.W + c0 + c] = sm[e];
+        }
+    }
+}
+
+// Inverse row transforms + extract_force_kernel (fft.rs:182-200).
+__global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__restrict__ sx,
+                                                             const float2 *__restrict__ sy,
+                                                             const uint32_t *__restrict__ mask,
+                                                             float *__restrict__ fx, float *__restrict__ fy,
+                                                             const float2 *__restrict__ tw, FftDims d) {
+    extern __shared__ float2 sm[];
+    const int row = blockIdx.x;
+    for (int i = threadIdx.x; i < d.W; i += blockDim.x) {
+        const uint32_t r = bitrev(i, d.logW);
+        sm[r * 2 + 0] = sx[(size_t)row * d.W + i];
+        sm[r * 2 + 1] = sy[(size_t)row * d.W + i];
+    }
+    __syncthreads();
+    lds_fft(sm, d.W, d.logW, 2, tw, d.tw_bits, -1.0f);
+    for (int i = threadIdx.x; i < d.xc; i += blockDim.x) {
+        const int c = row * d.xc + i;
+        const bool on = mask[c] != 0u;
+        fx[c] = on ? sm[i * 2 + 0].x : 0.0f;
+        fy[c] = on ? sm[i * 2 + 1].x : 0.0f;
+    }
+}
+
+// Kernel-spectrum construction (once per context).
+__global__ void grav_kernel_field(float2 *__restrict__ kx, float2 *__restrict__ ky, FftDims d, float scale) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= d.W * d.H) return;
+    const int px = id % d.W, py = id / d.W;
+    float dx = (float)px, dy = (float)py;
+    if (px * 2 >= d.W) dx = (float)px - (float)d.W;
+    if (py * 2 >= d.H) dy = (float)py - (float)d.H;
+    float vx = 0.0f, vy = 0.0f;
+    if (dx != 0.0f || dy != 0.0f) {
+        const float dist = sqrtf(dx * dx + dy * dy);
+        const float inv = scale / (dist * dist * dist);
+        vx = dx * inv;
+        vy = dy * inv;
+    }
+    kx[id] = make_float2(vx, 0.0f);
+    ky[id] = make_float2(vy, 0.0f);
+}
+
+__global__ __launch_bounds__(FFT_THREADS) void grav_rows_inplace(float2 *__restrict__ data,
+                                                                 const float2 *__restrict__ tw, FftDims d) {
+    extern __shared__ float2 sm[];
+    const size_t base = (size_t)blockIdx.x * d.W;
+    for (int i = threadIdx.x; i < d.W; i += blockDim.x) sm[bitrev(i, d.logW)] = data[base + i];
+    __syncthreads();
+    lds_fft(sm, d.W, d.logW, 1, tw, d.tw_bits, 1.0f);
+    for (int i = threadIdx.x; i < d.W; i += blockDim.x) data[base + i] = sm[i];
+}
+
+__global__ __launch_bounds__(FFT_THREADS) void grav_cols_inplace(float2 *__restrict__ data,
+                                                                 const float2 *__restrict__ tw, FftDims d) {
+    extern __shared__ float2 sm[];
+    const int cols = d.cols, c0 = blockIdx.x * cols, total = d.H * cols;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int r = e / cols, c = e % cols;
+        sm[bitrev(r, d.logH) * cols + c] = data[(size_t)r * d.W + c0 + c];
+    }
+    __syncthreads();
+    lds_fft(sm, d.H, d.logH, cols, tw, d.tw_bits, 1.0f);
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int r = e / cols, c = e % cols;
+        data[(size_t)r * d.W + c0 + c] = sm[e];
+    }
+}
+
+// apply_gravity_kernel (gravity_kernel.rs:434-507), NewtonianGravity = 1.
+__global__ __launch_bounds__(GRAV_BLOCK) void grav_apply(const uint32_t *__restrict__ ty_a,
+                                                         const float *__restrict__ px_a,
+                                                         const float *__restrict__ py_a, float *__restrict__ vx_a,
+                                                         float *__restrict__ vy_a, const uint32_t *__restrict__ flags_a,
+                                                         const float *__restrict__ fx, const float *__restrict__ fy,
+                                                         const uint32_t *__restrict__ wall_bmap,
+                                                         const uint32_t *__restrict__ wall_emap,
+                                                         const uint32_t *__restrict__ lut, uint32_t slots, int xres,
+                                                         int yres, int xc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    const uint32_t ty = ty_a[i];
+    if (ty == 0u || ty >= (uint32_t)TPT_PT_NUM) return;
+    const uint32_t flags = flags_a[i];
+    if (flags & 0x10u) return;
+    const int xi = tpt_f2i(px_a[i] + 0.5f), yi = tpt_f2i(py_a[i] + 0.5f);
+    if (xi < 4 || yi < 4 || xi >= xres - 4 || yi >= yres - 4) return;
+    const int cell = (yi / TPT_CELL) * xc + xi / TPT_CELL;
+    if (wall_bmap[cell] == 18u && (wall_emap[cell] & 0xFFu) < 8u) return;
+    const uint32_t props = lut[ty * 12u];
+    const float loss = __uint_as_float(lut[ty * 12u + 7u]);
+    float vx = vx_a[i], vy = vy_a[i];
+    if (!(ty == 78u && (flags & 0x08u))) {  // PT_SPNG with FLAG_MOVABLE keeps its velocity
+        vx *= loss;
+        vy *= loss;
+    }
+    if ((props & 0x04u) == 0u) {  // TYPE_SOLID receives no force
+        vx += 1.0f * fx[cell];
+        vy += 1.0f * fy[cell];
+    }
+    vx_a[i] = vx;
+    vy_a[i] = vy;
+}
+
+// build_gravity_mass_kernel (gravity_kernel.rs:275-362).  mode 1 deposits
+// the element weight of every live particle.
+__global__ __launch_bounds__(GRAV_BLOCK) void grav_build_mass(
+    const uint32_t *__restrict__ ty_a, const float *__restrict__ px_a, const float *__restrict__ py_a,
+    const int32_t *__restrict__ life_a, const int32_t *__restrict__ tmp_a, const float *__restrict__ temp_a,
+    const uint32_t *__restrict__ mask, const uint32_t *__restrict__ props, int32_t *__restrict__ mass,
+    uint32_t slots, int xres, int yres, int xc, int yc, uint32_t mode) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    const uint32_t ty = ty_a[i];
+    if (ty == 0u) return;
+    float m = 0.0f;
+    bool contributes = false;
+    if (mode == 1u) {
+        if (ty < (uint32_t)TPT_PT_NUM) {
+            m = (float)(int32_t)props[ty * 32u + 2u];
+            contributes = true;
+        }
+    } else if (ty == 39u) {  // GRVT
+        m = 0.2f * (float)min(max(tmp_a[i], -100), 100);
+        contributes = true;
+    } else if (ty == 40u || ty == 41u) {  // NBHL / NWHL
+        float hole = 0.1f;
+        const int32_t tmp = tmp_a[i];
+        if (tmp != 0) hole = tpt_clampf(0.001f * (float)tmp, 0.1f, 51.2f);
+        if (ty == 41u) hole = -hole;
+        m = hole;
+        contributes = true;
+    } else if (ty == 42u) {  // GBMB
+        const int32_t life = life_a[i];
+        if (life > 20) {
+            m = 20.0f;
+            contributes = true;
+        } else if (life >= 1) {
+            m = -80.0f;
+            contributes = true;
+        }
+    } else if (ty == 43u) {  // GPMP
+        if (life_a[i] == 10) {
+            m = 0.2f * (temp_a[i] - 273.15f);
+            contributes = true;
+        }
+    }
+    if (!contributes) return;
+    const int xi = tpt_f2i(px_a[i] + 0.5f), yi = tpt_f2i(py_a[i] + 0.5f);
+    if (xi < 0 || yi < 0 || xi >= xres || yi >= yres) return;
+    const int cx = xi / TPT_CELL, cy = yi / TPT_CELL;
+    if (cx >= xc || cy >= yc) return;
+    const int cell = cy * xc + cx;
+    if (mask[cell] == 0u) return;
+    float scaled = m * 65536.0f;
+    if (scaled >= 0.0f) scaled += 0.5f;
+    else scaled -= 0.5f;
+    const int32_t q = tpt_f2i(scaled);
+    if (q != 0) atomicAdd(&mass[cell], q);
+}
+
+__global__ void fill_u32(uint32_t *__restrict__ p, uint32_t v, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void copy_u32(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+uint32_t npow2(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+int ilog2(uint32_t v) {
+    int b = 0;
+    while ((1u << b) < v) ++b;
+    return b;
+}
+
+FftDims dims_of(const tpt_ctx *ctx) {
+    FftDims d;
+    d.xc = (int)ctx->xcells;
+    d.yc = (int)ctx->ycells;
+    d.W = (int)ctx->fft.width;
+    d.H = (int)ctx->fft.height;
+    d.logW = ilog2(ctx->fft.width);
+    d.logH = ilog2(ctx->fft.height);
+    d.tw_bits = ilog2(ctx->fft.width > ctx->fft.height ? ctx->fft.width : ctx->fft.height);
+    int cols = COL_TILE_ELEMS / d.H;
+    if (cols < 1) cols = 1;
+    if (cols > d.W) cols = d.W;
+    d.cols = cols;
+    return d;
+}
+
+int ensure_fft(tpt_ctx *ctx) {
+    tpt_fft_state &f = ctx->fft;
+    if (f.kernels_ready) return TPT_OK;
+    f.xcells = ctx->xcells;
+    f.ycells = ctx->ycells;
+    f.width = npow2(ctx->xcells * 2);
+    f.height = npow2(ctx->ycells * 2);
+    const size_t grid = (size_t)f.width * f.height;
+    const size_t rows_grid = (size_t)f.width * ctx->ycells;
+    const uint32_t ntw = f.width > f.height ? f.width : f.height;
+    const uint32_t tw_len = ntw / 2 ? ntw / 2 : 1;
+    if (hipMalloc(&f.work, rows_grid * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&f.spec_x, rows_grid * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&f.spec_y, rows_grid * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&f.kernel_x, grid * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&f.kernel_y, grid * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&f.twiddle, tw_len * sizeof(float2)) != hipSuccess)
+        return tpt_set_error(TPT_ENOMEM, "gravity FFT buffers (%ux%u) could not be allocated", f.width, f.height);
+    float2 *tw = (float2 *)malloc(tw_len * sizeof(float2));
+    if (!tw) return tpt_set_error(TPT_ENOMEM, "host twiddle table");
+    for (uint32_t k = 0; k < tw_len; ++k) {
+        const double a = -2.0 * 3.14159265358979323846 * (double)k / (double)ntw;
+        tw[k] = make_float2((float)cos(a), (float)sin(a));
+    }
+    hipError_t e = hipMemcpyAsync(f.twiddle, tw, tw_len * sizeof(float2), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    free(tw);
+    if (e != hipSuccess) return tpt_set_error(TPT_EDEVICE, "twiddle upload: %s", hipGetErrorString(e));
+
+    const FftDims d = dims_of(ctx);
+    // The inverse transform is unnormalised; the kernel carries 1/(W*H)
+    // (gravity_gpu/mod.rs:176-180).
+    const float scale = -M_GRAV / (float)(f.width * f.height);
+    hipLaunchKernelGGL(grav_kernel_field, dim3((grid + 255) / 256), dim3(256), 0, ctx->stream, f.kernel_x,
+                       f.kernel_y, d, scale);
+    for (float2 *k : {f.kernel_x, f.kernel_y}) {
+        hipLaunchKernelGGL(grav_rows_inplace, dim3(f.height), dim3(FFT_THREADS), f.width * sizeof(float2),
+                           ctx->stream, k, f.twiddle, d);
+        hipLaunchKernelGGL(grav_cols_inplace, dim3(f.width / d.cols), dim3(FFT_THREADS),
+                           (size_t)d.H * d.cols * sizeof(float2), ctx->stream, k, f.twiddle, d);
+    }
+    int rc = tpt_check_launch("gravity kernel spectra");
+    if (rc) return rc;
+    f.kernels_ready = true;
+    return TPT_OK;
+}
+
+int solve(tpt_ctx *ctx) {
+    int rc = ensure_fft(ctx);
+    if (rc) return rc;
+    const tpt_fft_state &f = ctx->fft;
+    const FftDims d = dims_of(ctx);
+    tpt_pass_begin(ctx, PASS_GRAV_FFT_ROWS);
+    hipLaunchKernelGGL(grav_rows_fwd, dim3(d.yc), dim3(FFT_THREADS), d.W * sizeof(float2), ctx->stream,
+                       tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_PREV), tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK),
+                       f.work, f.twiddle, d);
+    tpt_pass_end(ctx, PASS_GRAV_FFT_ROWS);
+    tpt_pass_begin(ctx, PASS_GRAV_FFT_COLS);
+    hipLaunchKernelGGL(grav_cols, dim3(d.W / d.cols), dim3(FFT_THREADS), (size_t)d.H * d.cols * sizeof(float2),
+                       ctx->stream, f.work, f.kernel_x, f.kernel_y, f.spec_x, f.spec_y, f.twiddle, d, d.yc, d.yc);
+    tpt_pass_end(ctx, PASS_GRAV_FFT_COLS);
+    tpt_pass_begin(ctx, PASS_GRAV_FFT_INV);
+    hipLaunchKernelGGL(grav_rows_inv, dim3(d.yc), dim3(FFT_THREADS), 2 * d.W * sizeof(float2), ctx->stream,
+                       f.spec_x, f.spec_y, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK),
+                       tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), tpt_buf<float>(ctx, TPT_BUF_GRAV_FY), f.twiddle, d);
+    tpt_pass_end(ctx, PASS_GRAV_FFT_INV);
+    return tpt_check_launch("gravity solve");
+}
+
+}  // namespace
+
+int tpt_gravity_solve_only(tpt_ctx *ctx) { return solve(ctx); }
+
+int tpt_gravity_step(tpt_ctx *ctx, uint32_t) {
+    const uint32_t cells = ctx->cells;
+    const dim3 cgrid((cells + 255) / 256), blk(256);
+    if (!ctx->cfg.newton_grav) {
+        // SimBuffers::zero_gravity_forces (gpu_sync.rs:358-372).
+        hipLaunchKernelGGL(fill_u32, cgrid, blk, 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_FX), 0u, cells);
+        hipLaunchKernelGGL(fill_u32, cgrid, blk, 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_FY), 0u, cells);
+        return tpt_check_launch("gravity force clear");
+    }
+    int rc = solve(ctx);
+    if (rc) return rc;
+    const uint32_t slots = ctx->slot_bound;
+    const dim3 sgrid((slots + GRAV_BLOCK - 1) / GRAV_BLOCK);
+    tpt_pass_begin(ctx, PASS_GRAV_APPLY);
+    hipLaunchKernelGGL(grav_apply, sgrid, dim3(GRAV_BLOCK), 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY),
+                       tpt_buf<float>(ctx, TPT_BUF_PART_X), tpt_buf<float>(ctx, TPT_BUF_PART_Y),
+                       tpt_buf<float>(ctx, TPT_BUF_PART_VX), tpt_buf<float>(ctx, TPT_BUF_PART_VY),
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_PART_FLAGS), tpt_buf<float>(ctx, TPT_BUF_GRAV_FX),
+                       tpt_buf<float>(ctx, TPT_BUF_GRAV_FY), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP),
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_MOVE_LUT), slots,
+                       (int)ctx->cfg.xres, (int)ctx->cfg.yres, (int)ctx->xcells);
+    tpt_pass_end(ctx, PASS_GRAV_APPLY);
+    tpt_pass_begin(ctx, PASS_GRAV_BUILD);
+    hipLaunchKernelGGL(fill_u32, cgrid, blk, 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASS), 0u, cells);
+    hipLaunchKernelGGL(grav_build_mass, sgrid, dim3(GRAV_BLOCK), 0, ctx->stream,
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<float>(ctx, TPT_BUF_PART_X),
+                       tpt_buf<float>(ctx, TPT_BUF_PART_Y), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
+                       tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP), tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK), tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS),
+                       tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS), slots, (int)ctx->cfg.xres, (int)ctx->cfg.yres,
+                       (int)ctx->xcells, (int)ctx->ycells, ctx->cfg.grav_mass_mode);
+    // prev = current: the snapshot the next tick's solve reads.
+    hipLaunchKernelGGL(copy_u32, cgrid, blk, 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASS),
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASS_PREV), cells);
+    tpt_pass_end(ctx, PASS_GRAV_BUILD);
+    return tpt_check_launch("gravity apply/build");
+}
+
+int tpt_gravity_free(tpt_ctx *ctx) {
+    tpt_fft_state &f = ctx->fft;
+    for (float2 *p : {f.work, f.spec_x, f.spec_y, f.kernel_x, f.kernel_y, f.twiddle})
+        if (p) (void)hipFree(p);
+    f = tpt_fft_state();
+    return TPT_OK;
+}
