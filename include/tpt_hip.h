@@ -99,6 +99,8 @@ typedef enum tpt_buf_id {
     /* tables */
     TPT_BUF_ELEM_PROPS,  /* TPT_PT_NUM*TPT_ELEM_STRIDE words   */
     TPT_BUF_MOVE_LUT,    /* TPT_MOVE_LUT_WORDS words            */
+    TPT_BUF_PHOTON_MASK, /* TPT_PT_NUM words: PhotonReflectWavelengths
+                            (build_photon_mask, move_kernel/mod.rs:1592-1606) */
     TPT_BUF_COUNT
 } tpt_buf_id;
 

@@ -1,0 +1,963 @@
+/*
+ * Oracle: lock-free movement pass -- TEST INFRASTRUCTURE ONLY (see tpt_oracle.h).
+ *
+ * Restates crates/tpt-sim/src/move_kernel/: the driver (mod.rs:1176-1227),
+ * air_deposit_kernel (mod.rs:297-346), the eval/do_move helpers
+ * (mod.rs:838-1151), velocity_and_intent_kernel and its stages
+ * (velocity.rs:98-262, 649-1157), the Falldown >= 1 chain (liquid.rs:137-963),
+ * commit_move_kernel (commit.rs:76-535) and the map rebuild
+ * (commit.rs:541-603).
+ *
+ * Canonical rules (shared with the HIP kernels):
+ *  R1  air deposit: all eligible particles read the pre-pass air velocity;
+ *      the cell keeps the write of its highest eligible slot.
+ *  R7  map rebuild: the highest matter slot owns a pixel, INVIS/FILT only
+ *      take an empty one (the lowest such slot), energy: highest slot.
+ *  SAW the WOOD -> SAWD conversion of a rejected probe (mod.rs:213-232) is
+ *      applied after the intent pass (the reference writes it mid-pass).
+ * Not restated (no counterpart in this library): photon refraction at glass
+ * (velocity.rs:262-647), the e == 2 pair pass (reaction_kernel/energy_entry.rs),
+ * WL_DETECT charging and its emap closure (mod.rs:473-828).
+ */
+#include "tpt_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle_util.h"
+
+#define NO_INTENT 0xFFFFFFFFu
+#define DIE_INTENT 0xFFFFFFFEu
+#define INTENT_HALF_MIRROR 0x40000000u
+#define INTENT_PIXEL_MASK 0x3FFFFFFFu
+#define DO_BLOCKED 0u
+#define DO_OK 1u
+#define DO_DIE 2u
+#define ISTP 2.0f
+#define MAX_VELOCITY 1.0e4f
+#define MAX_PLAN_STEPS 5001u
+#define EDGE_VOID 0u
+#define EDGE_LOOP 2u
+
+typedef struct {
+    const oracle_state *s;
+    int xr, yr;
+    uint32_t xc, edge;
+    uint8_t *saw; /* deferred WOOD -> SAWD (per slot) */
+} mctx;
+
+/* mod.rs:1147-1151 */
+uint32_t oracle_pcg_hash(uint32_t seed) {
+    uint32_t state = seed * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+static uint32_t can_move_at(const oracle_state *s, uint32_t mty, uint32_t dty) {
+    return s->move_lut[256 * 12 + mty * 256 + dty];
+}
+
+/* mod.rs:1079-1104 */
+static int wall_blocks_particle(uint32_t wall, uint32_t emap, uint32_t props) {
+    if (wall == 8 || wall == 1 || wall == 9) return 1;
+    if (wall == 6) return (props & 0x2u) == 0;
+    if (wall == 10) return (props & 0x1u) == 0;
+    if (wall == 13) return (props & 0x8u) == 0;
+    if (wall == 15) return (props & 0x10u) == 0;
+    if (wall == 2) return (emap & 0xFFu) == 0;
+    if (wall == 3) return (props & 0x4u) != 0;
+    return 0;
+}
+
+/* mod.rs:1114-1131 */
+static int wall_kills_particle(uint32_t wall, uint32_t emap, uint32_t props) {
+    if (wall == 8 || wall == 1 || wall == 9 || wall == 7) return 1;
+    if (wall == 6) return (props & 0x2u) == 0;
+    if (wall == 10) return (props & 0x1u) == 0;
+    if (wall == 13) return (props & 0x8u) == 0;
+    if (wall == 15) return (props & 0x10u) == 0;
+    if (wall == 2) return (emap & 0xFFu) == 0;
+    return 0;
+}
+
+/* mod.rs:838-891 */
+static uint32_t eval_move_at(const mctx *m, uint32_t props, uint32_t ty, int nx, int ny) {
+    const oracle_state *s = m->s;
+    if (!(nx >= 0 && nx < m->xr && ny >= 0 && ny < m->yr)) return 0;
+    uint32_t d_ty = s->pmap[(uint32_t)(ny * m->xr + nx)] & 0x1FFu;
+    uint32_t wcell = ((uint32_t)ny / 4) * m->xc + (uint32_t)nx / 4;
+    uint32_t wall = s->wall_bmap[wcell], emap = s->wall_emap[wcell];
+    uint32_t can = can_move_at(s, ty, d_ty);
+    if (can == 3) {
+        if (d_ty == 23 || d_ty == 75) can = 1;                                     /* VOID, PVOD */
+        else if (d_ty == 58 || d_ty == 43 || d_ty == 124 || d_ty == 60) can = 2;    /* LCRY GPMP INVIS SWCH */
+        else can = 1;
+    }
+    uint32_t result = can;
+    if (wall != 0) {
+        if (wall_blocks_particle(wall, emap, props)) {
+            result = 0;
+        } else if (wall == 12 && (emap & 0xFFu) == 0) {
+            uint32_t dest_props = s->move_lut[d_ty * 12];
+            if ((props & 0x4u) == 0 && (dest_props & 0x4u) == 0) result = 2;
+        }
+    }
+    return result;
+}
+
+/* mod.rs:959-967 */
+static float remainder_p(float x, float y) {
+    float out = x - floorf(x / y) * y;
+    if (out < 0.0f) out += y;
+    else if (out >= y) out -= y;
+    return out;
+}
+
+/* mod.rs:976-984 */
+static int wrap_loop_pixel(int n, uint32_t res) {
+    int lo = 4, span = (int)res - 8;
+    if (span > 0 && (n < lo || n >= lo + span)) return lo + ora_f2i(remainder_p((float)(n - lo), (float)span));
+    return n;
+}
+
+/* mod.rs:1009-1056 */
+static uint32_t do_move_at(const mctx *m, uint32_t props, uint32_t ty, int nx, int ny, int *ox, int *oy) {
+    int mx = nx, my = ny;
+    if (m->edge == EDGE_LOOP) {
+        mx = wrap_loop_pixel(nx, (uint32_t)m->xr);
+        my = wrap_loop_pixel(ny, (uint32_t)m->yr);
+    }
+    uint32_t status = DO_BLOCKED;
+    if (mx < 0 || my < 0 || mx >= m->xr || my >= m->yr) {
+        if (m->edge != EDGE_LOOP) status = DO_DIE;
+    } else if (eval_move_at(m, props, ty, mx, my) != 0) {
+        int in_band = mx >= 4 && my >= 4 && mx < m->xr - 4 && my < m->yr - 4;
+        status = DO_OK;
+        if (!in_band && m->edge != EDGE_LOOP) {
+            status = DO_BLOCKED;
+            if (m->edge == EDGE_VOID) status = DO_DIE;
+        }
+    }
+    *ox = mx;
+    *oy = my;
+    return status;
+}
+
+/* mod.rs:213-232, deferred (rule SAW) */
+static void saw_wood_at(mctx *m, int nx, int ny, float vx, float vy) {
+    const oracle_state *s = m->s;
+    if (nx >= 0 && ny >= 0 && nx < m->xr && ny < m->yr) {
+        uint32_t v = s->pmap[(uint32_t)(ny * m->xr + nx)];
+        if (v >= 512u && (v & 0x1FFu) == 15u && vx * vx + vy * vy > 25.0f) m->saw[(v >> 9) - 1] = 1;
+    }
+}
+
+static int round_i32(float v) { return ora_f2i(v + 0.5f); }
+
+/* liquid.rs:80-118 */
+static int plan_sample_blocked(const mctx *m, uint32_t props, uint32_t ty, int nx, int ny, int closed_start) {
+    const oracle_state *s = m->s;
+    if (nx < 0 || ny < 0 || nx >= m->xr || ny >= m->yr) return 1;
+    uint32_t packed = s->pmap[(uint32_t)(ny * m->xr + nx)];
+    uint32_t d_ty = packed & 0x1FFu;
+    uint32_t raw_can = can_move_at(s, ty, d_ty);
+    uint32_t eval = eval_move_at(m, props, ty, nx, ny);
+    uint32_t wcell = ((uint32_t)ny / 4) * m->xc + (uint32_t)nx / 4;
+    uint32_t wall = s->wall_bmap[wcell];
+    int closed_now = wall == 12 && (s->wall_emap[wcell] & 0xFFu) == 0;
+    return eval == 0 || (raw_can == 3 && eval == 1) || (ty == 29 && packed != 0) || wall == 7 || closed_now != closed_start;
+}
+
+typedef struct {
+    int fin_x, fin_y, clear_x, clear_y;
+    float clear_xf, clear_yf;
+} plan_t;
+
+static float wrap_band(float f, int res) {
+    int ok = f >= 4.0f - 0.5f && f < (float)res - 4.0f - 0.5f;
+    if (!ok) f = remainder_p(f - 4.0f + 0.5f, (float)res - 8.0f) + 4.0f - 0.5f;
+    return f;
+}
+
+/* liquid.rs:137-261 */
+static plan_t plan_move(const mctx *m, float vx, float vy, float px, float py, int xi, int yi, uint32_t props, uint32_t ty) {
+    const oracle_state *s = m->s;
+    float mv = fabsf(vx);
+    if (fabsf(vy) > mv) mv = fabsf(vy);
+    plan_t p;
+    p.clear_xf = px;
+    p.clear_yf = py;
+    float fin_xf = px + vx, fin_yf = py + vy;
+    p.fin_x = round_i32(fin_xf);
+    p.fin_y = round_i32(fin_yf);
+    p.clear_x = xi;
+    p.clear_y = yi;
+    if (mv >= ISTP) {
+        uint32_t sc = ((uint32_t)yi / 4) * m->xc + (uint32_t)xi / 4;
+        int start_closed = s->wall_bmap[sc] == 12 && (s->wall_emap[sc] & 0xFFu) == 0;
+        float step_xf = vx * ISTP / mv, step_yf = vy * ISTP / mv;
+        float walk_xf = px, walk_yf = py, remaining = mv;
+        int planned = 0;
+        uint32_t guard = 0;
+        while (!planned && guard < MAX_PLAN_STEPS) {
+            guard++;
+            remaining -= ISTP;
+            walk_xf += step_xf;
+            walk_yf += step_yf;
+            if (m->edge == EDGE_LOOP) {
+                walk_xf = wrap_band(walk_xf, m->xr);
+                walk_yf = wrap_band(walk_yf, m->yr);
+            }
+            int nfx = round_i32(walk_xf), nfy = round_i32(walk_yf);
+            if (remaining <= 0.0f) {
+                fin_xf = px + vx;
+                fin_yf = py + vy;
+                if (m->edge == EDGE_LOOP) {
+                    fin_xf = wrap_band(fin_xf, m->xr);
+                    fin_yf = wrap_band(fin_yf, m->yr);
+                }
+                p.fin_x = round_i32(fin_xf);
+                p.fin_y = round_i32(fin_yf);
+                p.clear_xf = fin_xf - step_xf;
+                p.clear_yf = fin_yf - step_yf;
+                p.clear_x = round_i32(p.clear_xf);
+                p.clear_y = round_i32(p.clear_yf);
+                planned = 1;
+            } else if (plan_sample_blocked(m, props, ty, nfx, nfy, start_closed)) {
+                p.fin_x = nfx;
+                p.fin_y = nfy;
+                p.clear_xf = walk_xf - step_xf;
+                p.clear_yf = walk_yf - step_yf;
+                p.clear_x = round_i32(p.clear_xf);
+                p.clear_y = round_i32(p.clear_yf);
+                planned = 1;
+            }
+        }
+    }
+    return p;
+}
+
+/* velocity.rs:133-170 */
+static void base_gravity(const oracle_state *s, float gravity, float px, float py, float *gx, float *gy) {
+    *gx = 0.0f;
+    *gy = 0.0f;
+    if (s->gravity_mode == 0) {
+        *gy = gravity;
+    } else if (s->gravity_mode == 2) {
+        float dx = px - (float)(s->xres / 2), dy = py - (float)(s->yres / 2);
+        if (dx != 0.0f || dy != 0.0f) {
+            float d = 0.01f - sqrtf(dx * dx + dy * dy);
+            *gx = gravity * (dx / d);
+            *gy = gravity * (dy / d);
+        }
+    } else if (s->gravity_mode == 3) {
+        *gx = gravity * s->custom_gravity_x;
+        *gy = gravity * s->custom_gravity_y;
+    }
+}
+
+/* velocity.rs:98-111 */
+static int step_axis(float v, uint32_t h) {
+    float a = v < 0.0f ? -v : v;
+    float r = (float)(h & 0xFFFFu) * (1.0f / 65535.0f);
+    if (a >= 1.0f || r < a) {
+        if (v > 0.0f) return 1;
+        if (v < 0.0f) return -1;
+    }
+    return 0;
+}
+
+/* liquid.rs:264-286 */
+static void liquid_neighbourhood(const mctx *m, uint32_t ty, int xi, int yi, uint32_t *nt, uint32_t *space) {
+    *nt = 0;
+    *space = 0;
+    for (int oy = 0; oy < 3; ++oy)
+        for (int ox = 0; ox < 3; ++ox) {
+            if (ox == 1 && oy == 1) continue;
+            int nx = xi + ox - 1, ny = yi + oy - 1;
+            uint32_t nty = 0;
+            if (nx >= 0 && nx < m->xr && ny >= 0 && ny < m->yr) nty = m->s->pmap[(uint32_t)(ny * m->xr + nx)] & 0x1FFu;
+            if (nty == 0) *space += 1;
+            if (nty != ty) *nt += 1;
+        }
+}
+
+static uint32_t pix(const mctx *m, int x, int y) { return (uint32_t)(y * m->xr + x); }
+
+/* liquid.rs:298-350 */
+static uint32_t diagonal_liquid_move(const mctx *m, float vx, float vy, float cxf, float cyf, int rdir, uint32_t props, uint32_t ty) {
+    float ddx = vx - vy * (float)rdir;
+    float ddy = vy + vx * (float)rdir;
+    float dmv = fabsf(ddx);
+    if (fabsf(ddy) > dmv) dmv = fabsf(ddy);
+    if (!(dmv > 0.0001f)) return NO_INTENT;
+    ddx /= dmv;
+    ddy /= dmv;
+    int ox, oy;
+    uint32_t d1 = do_move_at(m, props, ty, round_i32(cxf + ddx), round_i32(cyf + ddy), &ox, &oy);
+    if (d1 == DO_DIE) return DIE_INTENT;
+    if (d1 == DO_OK) return pix(m, ox, oy);
+    uint32_t d2 = do_move_at(m, props, ty, round_i32(cxf + ddy * (float)rdir), round_i32(cyf - ddx * (float)rdir), &ox, &oy);
+    if (d2 == DO_DIE) return DIE_INTENT;
+    if (d2 == DO_OK) return pix(m, ox, oy);
+    return NO_INTENT;
+}
+
+/* liquid.rs:630-733 */
+static uint32_t scan_liquid_vertical(const mctx *m, int xi, int yi, int fin_y, int clear_x, int clear_y, int rdir, int rt,
+                                     uint32_t ty, uint32_t props, int *fall) {
+    const oracle_state *s = m->s;
+    int scan_x = xi, scan_y = yi, hit = 0, die = 0, stop = 0, steps = 0;
+    int j = clear_x + rdir;
+    *fall = 0;
+    while (!stop && steps < rt) {
+        if (j < 0 || j >= m->xr || j < clear_x - rt || j >= clear_x + rt) {
+            stop = 1;
+        } else {
+            if (fin_y >= 0 && fin_y < m->yr) {
+                uint32_t d_ty = s->pmap[pix(m, j, fin_y)] & 0x1FFu;
+                uint32_t wcell = ((uint32_t)fin_y / 4) * m->xc + (uint32_t)j / 4;
+                if (d_ty != ty || s->wall_bmap[wcell] != 0) {
+                    int ox, oy;
+                    uint32_t d = do_move_at(m, props, ty, j, fin_y, &ox, &oy);
+                    if (d == DO_DIE) {
+                        die = 1;
+                        stop = 1;
+                    } else if (d == DO_OK) {
+                        hit = 1;
+                        *fall = 1;
+                        scan_x = ox;
+                        scan_y = oy;
+                        stop = 1;
+                    }
+                }
+            }
+            if (!stop && fin_y != clear_y && clear_y >= 0 && clear_y < m->yr) {
+                uint32_t d_ty = s->pmap[pix(m, j, clear_y)] & 0x1FFu;
+                uint32_t wcell = ((uint32_t)clear_y / 4) * m->xc + (uint32_t)j / 4;
+                if (d_ty != ty || s->wall_bmap[wcell] != 0) {
+                    int ox, oy;
+                    uint32_t d = do_move_at(m, props, ty, j, clear_y, &ox, &oy);
+                    if (d == DO_DIE) {
+                        die = 1;
+                        stop = 1;
+                    } else if (d == DO_OK) {
+                        hit = 1;
+                        *fall = 1;
+                        scan_x = ox;
+                        scan_y = oy;
+                        stop = 1;
+                    }
+                }
+            }
+            if (!stop && clear_y >= 0 && clear_y < m->yr) {
+                uint32_t d_ty = s->pmap[pix(m, j, clear_y)] & 0x1FFu;
+                uint32_t wall = s->wall_bmap[((uint32_t)clear_y / 4) * m->xc + (uint32_t)j / 4];
+                if (d_ty != ty || (wall != 0 && wall != 4)) stop = 1;
+            }
+            if (!stop) {
+                j += rdir;
+                steps++;
+            }
+        }
+    }
+    if (die) return DIE_INTENT;
+    if (hit) return pix(m, scan_x, scan_y);
+    return NO_INTENT;
+}
+
+/* liquid.rs:735-787 */
+static uint32_t scroll_liquid_down(const mctx *m, int scan_x, int scan_y, float vy, int rt, uint32_t ty, uint32_t props) {
+    const oracle_state *s = m->s;
+    uint32_t target = pix(m, scan_x, scan_y);
+    int vr = vy > 0.0f ? 1 : -1;
+    int jj = scan_y + vr, stop = 0, steps = 0;
+    while (!stop && steps < rt) {
+        if (jj < 0 || jj >= m->yr || jj < scan_y - rt || jj >= scan_y + rt) {
+            stop = 1;
+        } else {
+            uint32_t d_ty = s->pmap[pix(m, scan_x, jj)] & 0x1FFu;
+            uint32_t wall = s->wall_bmap[((uint32_t)jj / 4) * m->xc + (uint32_t)scan_x / 4];
+            if (d_ty != ty || wall != 0) {
+                int ox, oy;
+                uint32_t d = do_move_at(m, props, ty, scan_x, jj, &ox, &oy);
+                if (d == DO_DIE) {
+                    target = DIE_INTENT;
+                    stop = 1;
+                } else if (d == DO_OK) {
+                    target = pix(m, ox, oy);
+                    stop = 1;
+                } else if (d_ty != ty || wall != 4) {
+                    stop = 1;
+                }
+            }
+            if (!stop) {
+                jj += vr;
+                steps++;
+            }
+        }
+    }
+    return target;
+}
+
+/* liquid.rs:801-963 */
+static uint32_t scan_liquid_perpendicular(const mctx *m, float cxf, float cyf, int cx, int cy, int rdir, int rt, float gravity,
+                                          uint32_t props, uint32_t ty) {
+    const oracle_state *s = m->s;
+    uint32_t target = NO_INTENT;
+    float nxf = cxf, nyf = cyf, pgx = 0.0f, pgy = 0.0f;
+    int nx = cx, ny = cy, moved = 0, j = 0, stop = 0;
+    while (!stop && j < rt) {
+        float gx, gy;
+        base_gravity(s, gravity, (float)nx, (float)ny, &gx, &gy);
+        float gmv = fabsf(gx);
+        if (fabsf(gy) > gmv) gmv = fabsf(gy);
+        if (gmv < 0.0001f) {
+            stop = 1;
+        } else {
+            gx /= gmv;
+            gy /= gmv;
+            if (j > 0) {
+                nxf += (float)rdir * (gy * 2.0f - pgy);
+                nyf += -(float)rdir * (gx * 2.0f - pgx);
+            } else {
+                nxf += (float)rdir * gy;
+                nyf += -(float)rdir * gx;
+            }
+            pgx = gx;
+            pgy = gy;
+            nx = round_i32(nxf);
+            ny = round_i32(nyf);
+            if (nx < 0 || ny < 0 || nx >= m->xr || ny >= m->yr) {
+                stop = 1;
+            } else {
+                uint32_t d_ty = s->pmap[pix(m, nx, ny)] & 0x1FFu;
+                uint32_t wall = s->wall_bmap[((uint32_t)ny / 4) * m->xc + (uint32_t)nx / 4];
+                if (d_ty != ty || wall != 0) {
+                    int ox, oy;
+                    uint32_t d = do_move_at(m, props, ty, nx, ny, &ox, &oy);
+                    if (d == DO_DIE) {
+                        target = DIE_INTENT;
+                        stop = 1;
+                    } else if (d == DO_OK) {
+                        target = pix(m, ox, oy);
+                        moved = 1;
+                        stop = 1;
+                    } else if (d_ty != ty || wall != 4) {
+                        stop = 1;
+                    }
+                }
+            }
+        }
+        if (!stop) j++;
+    }
+    if (moved) {
+        int j2 = 0, stop2 = 0;
+        while (!stop2 && j2 < rt) {
+            float gx, gy;
+            base_gravity(s, gravity, (float)nx, (float)ny, &gx, &gy);
+            float gmv = fabsf(gx);
+            if (fabsf(gy) > gmv) gmv = fabsf(gy);
+            if (gmv < 0.0001f) {
+                stop2 = 1;
+            } else {
+                gx /= gmv;
+                gy /= gmv;
+                nxf += gx;
+                nyf += gy;
+                int nnx = round_i32(nxf), nny = round_i32(nyf);
+                if (nnx < 0 || nny < 0 || nnx >= m->xr || nny >= m->yr) {
+                    stop2 = 1;
+                } else {
+                    uint32_t d_ty = s->pmap[pix(m, nnx, nny)] & 0x1FFu;
+                    uint32_t wall = s->wall_bmap[((uint32_t)nny / 4) * m->xc + (uint32_t)nnx / 4];
+                    if (d_ty != ty || wall != 0) {
+                        int ox, oy;
+                        uint32_t d = do_move_at(m, props, ty, nnx, nny, &ox, &oy);
+                        if (d == DO_DIE) {
+                            target = DIE_INTENT;
+                            stop2 = 1;
+                        } else if (d == DO_OK) {
+                            target = pix(m, ox, oy);
+                            stop2 = 1;
+                        } else if (d_ty != ty || wall != 4) {
+                            stop2 = 1;
+                        }
+                    }
+                }
+            }
+            if (!stop2) j2++;
+        }
+    }
+    return target;
+}
+
+/* liquid.rs:388-617 */
+static uint32_t fall_down_destination(mctx *m, float *pvx, float *pvy, float px, float py, int xi, int yi, int sx, int sy,
+                                      uint32_t props, uint32_t ty, uint32_t falldown, uint32_t flags, float collision,
+                                      uint32_t h2, float gravity, int32_t gel_tmp) {
+    const oracle_state *s = m->s;
+    float vx = *pvx, vy = *pvy;
+    plan_t p = plan_move(m, vx, vy, px, py, xi, yi, props, ty);
+    if (p.fin_x == xi && p.fin_y == yi && (sx != 0 || sy != 0)) {
+        p.fin_x = xi + sx;
+        p.fin_y = yi + sy;
+        p.clear_x = xi;
+        p.clear_y = yi;
+        p.clear_xf = px;
+        p.clear_yf = py;
+    }
+    uint32_t target = NO_INTENT;
+    if (p.fin_x != xi || p.fin_y != yi) {
+        int ox, oy;
+        uint32_t d0 = do_move_at(m, props, ty, p.fin_x, p.fin_y, &ox, &oy);
+        if (d0 == DO_DIE) target = DIE_INTENT;
+        else if (d0 == DO_OK) target = pix(m, ox, oy);
+        else saw_wood_at(m, ox, oy, vx, vy);
+        if (target == NO_INTENT && p.fin_x != xi) {
+            uint32_t d1 = do_move_at(m, props, ty, p.fin_x, p.clear_y, &ox, &oy);
+            if (d1 == DO_DIE) target = DIE_INTENT;
+            else if (d1 == DO_OK) target = pix(m, ox, oy);
+            else saw_wood_at(m, ox, oy, vx, vy);
+        }
+        if (target == NO_INTENT && p.fin_y != yi) {
+            uint32_t d2 = do_move_at(m, props, ty, p.clear_x, p.fin_y, &ox, &oy);
+            if (d2 == DO_DIE) target = DIE_INTENT;
+            else if (d2 == DO_OK) target = pix(m, ox, oy);
+            else saw_wood_at(m, ox, oy, vx, vy);
+        }
+        if (target == NO_INTENT) {
+            uint32_t nt, space;
+            liquid_neighbourhood(m, ty, xi, yi, &nt, &space);
+            int rdir = (h2 & 2u) == 0 ? 1 : -1;
+            int clear_moved = p.clear_x != xi || p.clear_y != yi;
+            int moving = fabsf(vx) > 0.01f || fabsf(vy) > 0.01f;
+            if ((clear_moved || nt > 0 || space > 0) && moving)
+                target = diagonal_liquid_move(m, vx, vy, p.clear_xf, p.clear_yf, rdir, props, ty);
+            if (target == NO_INTENT) {
+                if (falldown > 1) {
+                    if (s->newton_grav == 0 && s->gravity_mode == 0 && vy > fabsf(vx)) {
+                        int rt = ((flags & 0x01u) && nt == 0) ? 10 : 30;
+                        if (ty == 150) rt = ora_f2i((float)gel_tmp * 0.20f + 5.0f); /* GEL */
+                        int fall;
+                        uint32_t st = scan_liquid_vertical(m, xi, yi, p.fin_y, p.clear_x, p.clear_y, rdir, rt, ty, props, &fall);
+                        if (st != NO_INTENT) {
+                            if (fall) target = scroll_liquid_down(m, (int)(st % (uint32_t)m->xr), (int)(st / (uint32_t)m->xr), vy, rt, ty, props);
+                            else target = st;
+                        }
+                    } else {
+                        float gx, gy;
+                        base_gravity(s, gravity, (float)xi, (float)yi, &gx, &gy);
+                        float gnorm = fabsf(gx);
+                        if (fabsf(gy) > gnorm) gnorm = fabsf(gy);
+                        int aligned = 0;
+                        if (gnorm > 0.0001f) {
+                            float ngx = gx / gnorm, ngy = gy / gnorm;
+                            if (fabsf(ngx * vx + ngy * vy) > fabsf(ngy * vx - ngx * vy)) aligned = 1;
+                        }
+                        if (aligned) {
+                            int rt = ((flags & 0x01u) && nt == 0) ? 10 : 30;
+                            target = scan_liquid_perpendicular(m, p.clear_xf, p.clear_yf, p.clear_x, p.clear_y, rdir, rt, gravity, props, ty);
+                        }
+                    }
+                }
+                if (target == NO_INTENT && clear_moved) {
+                    uint32_t d = do_move_at(m, props, ty, p.clear_x, p.clear_y, &ox, &oy);
+                    if (d == DO_DIE) target = DIE_INTENT;
+                    else if (d == DO_OK) target = pix(m, ox, oy);
+                }
+            }
+            vx *= collision;
+            vy *= collision;
+        }
+    }
+    *pvx = vx;
+    *pvy = vy;
+    return target;
+}
+
+/* velocity.rs:755-928 without the glass refraction */
+static uint32_t gas_destination(mctx *m, float *pvx, float *pvy, float px, float py, int xi, int yi, int sx, int sy,
+                                uint32_t props, uint32_t ty, uint32_t slot, float collision, uint32_t seed) {
+    const oracle_state *s = m->s;
+    float vx = *pvx, vy = *pvy;
+    plan_t p = plan_move(m, vx, vy, px, py, xi, yi, props, ty);
+    int fin_x = p.fin_x, fin_y = p.fin_y;
+    if (fin_x == xi && fin_y == yi && (sx != 0 || sy != 0)) {
+        fin_x = xi + sx;
+        fin_y = yi + sy;
+    }
+    uint32_t target = NO_INTENT;
+    int blocked = 0, died = 0;
+    if (fin_x != xi || fin_y != yi) {
+        int ox, oy;
+        uint32_t d = do_move_at(m, props, ty, fin_x, fin_y, &ox, &oy);
+        if (d == DO_OK) {
+            target = pix(m, ox, oy);
+        } else if (d == DO_DIE) {
+            died = 1;
+        } else {
+            blocked = 1;
+            if (ty == 29) {
+                uint32_t dest_px = pix(m, ox, oy);
+                uint32_t dest_ty = s->pmap[dest_px] & 0x1FFu;
+                uint32_t src_ty = s->pmap[pix(m, xi, yi)] & 0x1FFu;
+                uint32_t roll = oracle_pcg_hash(seed ^ (slot * 0x9E3779B9u) ^ (dest_px * 0x85EBCA6Bu));
+                if ((dest_ty == 46 && (roll & 1u) == 0) || src_ty == 46) {
+                    blocked = 0;
+                    target = INTENT_HALF_MIRROR | dest_px;
+                } else {
+                    target = 0x80000000u | dest_px;
+                    saw_wood_at(m, ox, oy, vx, vy);
+                }
+            } else {
+                saw_wood_at(m, ox, oy, vx, vy);
+            }
+        }
+    }
+    if (died) {
+        target = DIE_INTENT;
+    } else if (blocked) {
+        if (ty == 29) {
+            if (fin_x != xi) vx *= collision;
+            if (fin_y != yi) vy *= collision;
+        } else {
+            int bx = fin_x, by = fin_y;
+            if (bx > xi + 2) bx = xi + 2;
+            if (bx < xi - 2) bx = xi - 2;
+            if (by > yi + 2) by = yi + 2;
+            if (by < yi - 2) by = yi - 2;
+            int rx = 2 * xi - bx, ry = 2 * yi - by, ox, oy;
+            uint32_t d1 = do_move_at(m, props, ty, rx, by, &ox, &oy);
+            if (d1 != DO_BLOCKED) {
+                if (d1 == DO_DIE) target = DIE_INTENT;
+                else if (ox != xi || oy != yi) target = pix(m, ox, oy);
+                vx *= collision;
+            } else {
+                saw_wood_at(m, ox, oy, vx, vy);
+                uint32_t d2 = do_move_at(m, props, ty, bx, ry, &ox, &oy);
+                if (d2 != DO_BLOCKED) {
+                    if (d2 == DO_DIE) target = DIE_INTENT;
+                    else if (ox != xi || oy != yi) target = pix(m, ox, oy);
+                    vy *= collision;
+                } else {
+                    saw_wood_at(m, ox, oy, vx, vy);
+                    vx *= collision;
+                    vy *= collision;
+                }
+            }
+        }
+    }
+    *pvx = vx;
+    *pvy = vy;
+    return target;
+}
+
+/* air_deposit_kernel (mod.rs:297-346), rule R1 */
+void oracle_air_deposit(oracle_state *s) {
+    const uint32_t xres = s->xres, yres = s->yres, xc = xres / 4, n = xc * (yres / 4);
+    uint32_t *key = calloc(n, 4);
+    for (uint32_t i = 0; i < s->slot_bound; ++i) {
+        uint32_t ty = s->part_ty[i];
+        if (ty == 0 || ty >= 256) continue;
+        uint32_t flags = s->part_flags[i];
+        if ((flags & 0x10u) || (flags & 0x02u)) continue;
+        int xi = ora_f2i(s->part_x[i] + 0.5f), yi = ora_f2i(s->part_y[i] + 0.5f);
+        if (xi < 4 || yi < 4 || xi >= (int)xres - 4 || yi >= (int)yres - 4) continue;
+        uint32_t cell = ((uint32_t)yi / 4) * xc + (uint32_t)xi / 4;
+        if (s->wall_bmap[cell] == 18u && (s->wall_emap[cell] & 0xFFu) < 8u) continue;
+        key[cell] = i + 1;
+    }
+    for (uint32_t c = 0; c < n; ++c) {
+        if (!key[c]) continue;
+        uint32_t i = key[c] - 1, base = s->part_ty[i] * 12 + 4;
+        float drag = ora_bits_f(s->move_lut[base + 1]), loss = ora_bits_f(s->move_lut[base + 2]);
+        s->air_vx[c] = s->air_vx[c] * loss + drag * s->part_vx[i];
+        s->air_vy[c] = s->air_vy[c] * loss + drag * s->part_vy[i];
+    }
+    free(key);
+}
+
+/* velocity_and_intent_kernel (velocity.rs:951-1157) for all slots. */
+void oracle_velocity_intent(oracle_state *s, uint32_t seed) {
+    mctx m = {s, (int)s->xres, (int)s->yres, s->xres / 4, s->edge_mode, calloc(s->capacity, 1)};
+    const uint32_t n_pix = s->xres * s->yres;
+    memset(s->cell_winner, 0, sizeof(uint32_t) * n_pix);
+    for (uint32_t i = 0; i < s->slot_bound; ++i) {
+        s->intent_target[i] = NO_INTENT;
+        uint32_t ty = s->part_ty[i];
+        if (ty == 0 || ty >= 256) continue;
+        uint32_t flags = s->part_flags[i];
+        if ((flags & 0x10u) || (flags & 0x02u)) continue;
+        float px = s->part_x[i], py = s->part_y[i];
+        int xi = ora_f2i(px + 0.5f), yi = ora_f2i(py + 0.5f);
+        if (xi < 0 || xi >= m.xr || yi < 0 || yi >= m.yr) {
+            s->part_ty[i] = 0;
+            continue;
+        }
+        uint32_t cell = ((uint32_t)yi / 4) * m.xc + (uint32_t)xi / 4;
+        uint32_t meta = ty * 12, props = s->move_lut[meta], falldown = s->move_lut[meta + 1], rank = s->move_lut[meta + 2];
+        float advection = ora_bits_f(s->move_lut[meta + 4]), loss = ora_bits_f(s->move_lut[meta + 7]);
+        float collision = ora_bits_f(s->move_lut[meta + 8]), gravity = ora_bits_f(s->move_lut[meta + 9]);
+        float diffusion = ora_bits_f(s->move_lut[meta + 10]);
+        /* pre_move_checks (velocity.rs:235-260) */
+        uint32_t wall = s->wall_bmap[cell], emap = s->wall_emap[cell];
+        int in_margin = !(xi < 4 || yi < 4 || xi >= m.xr - 4 || yi >= m.yr - 4);
+        int stickman = ty == 59 || ty == 137 || ty == 159;
+        if (!in_margin || (!stickman && wall_kills_particle(wall, emap, props))) {
+            s->part_ty[i] = 0;
+            continue;
+        }
+        if (wall == 18u && (emap & 0xFFu) < 8u) continue;
+        float gx, gy;
+        base_gravity(s, gravity, (float)xi, (float)yi, &gx, &gy);
+        float ax = s->air_vx[cell], ay = s->air_vy[cell];
+        uint32_t h1 = oracle_pcg_hash(seed ^ (i * 2654435769u)), h2 = oracle_pcg_hash(h1);
+        float loss_here = s->newton_grav ? 1.0f : loss;
+        float rdx = (float)(h1 & 0xFFFFu) * (2.0f / 65535.0f) - 1.0f;
+        float rdy = (float)(h2 & 0xFFFFu) * (2.0f / 65535.0f) - 1.0f;
+        float vx = s->part_vx[i] * loss_here + advection * ax + gx + diffusion * rdx;
+        float vy = s->part_vy[i] * loss_here + advection * ay + gy + diffusion * rdy;
+        /* clamp_plan_velocity (velocity.rs:205-219) */
+        float mv = fabsf(vx);
+        if (fabsf(vy) > mv) mv = fabsf(vy);
+        if (mv > MAX_VELOCITY) {
+            vx = vx * (MAX_VELOCITY / mv);
+            vy = vy * (MAX_VELOCITY / mv);
+        }
+        int sx = step_axis(vx, h1), sy = step_axis(vy, h2);
+        uint32_t target;
+        if (falldown == 0)
+            target = gas_destination(&m, &vx, &vy, px, py, xi, yi, sx, sy, props, ty, i, collision, seed);
+        else
+            target = fall_down_destination(&m, &vx, &vy, px, py, xi, yi, sx, sy, props, ty, falldown, flags, collision, h2,
+                                           gravity, s->part_tmp[i]);
+        if (target == DIE_INTENT) {
+            s->part_ty[i] = 0;
+            continue;
+        }
+        if (target != NO_INTENT) {
+            s->intent_target[i] = target;
+            if ((target & 0x80000000u) == 0) {
+                uint32_t p = target & INTENT_PIXEL_MASK;
+                uint32_t key = ((rank & 0x1FFu) << 23) | ((i + 1) & 0x7FFFFFu);
+                if (key > s->cell_winner[p]) s->cell_winner[p] = key;
+            }
+        }
+        s->part_vx[i] = vx;
+        s->part_vy[i] = vy;
+    }
+    for (uint32_t i = 0; i < s->capacity; ++i)
+        if (m.saw[i]) s->part_ty[i] = 180u; /* PT_SAWD */
+    free(m.saw);
+}
+
+/* commit.rs:76-146 */
+static int void_admits(uint32_t ty, uint32_t vctype, int32_t vtmp) {
+    if (vctype == 0) return 1;
+    int matches = vctype == ty, inv = (vtmp & 1) != 0;
+    return inv ? !matches : matches;
+}
+
+static uint32_t resolve_can(oracle_state *s, uint32_t can, uint32_t ty, uint32_t dty, uint32_t other, float pv) {
+    if (can != 3) return can;
+    if (dty == 58 && ty == 29) return s->part_life[other] > 5 ? 2 : 0;
+    if (dty == 43 && ty == 29) return s->part_life[other] < 10 ? 2 : 0;
+    if (dty == 124) {
+        float res = 4.0f;
+        if (s->part_tmp[other] > 0) res = (float)s->part_tmp[other];
+        return (pv < -res || pv > res) ? 2 : 0;
+    }
+    if (dty == 75) return (s->part_life[other] == 10 && void_admits(ty, s->part_ctype[other], s->part_tmp[other])) ? 1 : 0;
+    if (dty == 23) return void_admits(ty, s->part_ctype[other], s->part_tmp[other]) ? 1 : 0;
+    if (dty == 60) return ty == 151 ? (s->part_life[other] >= 10 ? 2 : 0) : 1;
+    return 1;
+}
+
+static int is_heat_insulator_c(uint32_t ty, int32_t life, int32_t tmp, uint32_t hc) {
+    return hc == 0 || (ty == 68 && life != 10) || ((ty == 82 || ty == 162) && (tmp & 1) == 0);
+}
+
+static uint32_t pmap_at(const oracle_state *s, int x, int y) {
+    if (x < 0 || y < 0 || x >= (int)s->xres || y >= (int)s->yres) return 0;
+    return s->pmap[(uint32_t)y * s->xres + (uint32_t)x];
+}
+
+/* commit_move_kernel (commit.rs:181-535) for all slots. */
+void oracle_commit(oracle_state *s) {
+    const uint32_t xres = s->xres, yres = s->yres, xc = xres / 4;
+    for (uint32_t i = 0; i < s->slot_bound; ++i) {
+        uint32_t ty = s->part_ty[i];
+        if (ty == 0 || ty >= 256) continue;
+        uint32_t raw = s->intent_target[i];
+        if (raw == NO_INTENT) {
+            if (s->move_lut[ty * 12 + 1] >= 2 && (s->part_flags[i] & 0x10u) == 0) s->part_flags[i] |= 0x01u;
+            continue;
+        }
+        int half_mirror = (raw & INTENT_HALF_MIRROR) != 0;
+        uint32_t target = raw & INTENT_PIXEL_MASK;
+        if (raw & 0x80000000u) {
+            if (ty == 29) {
+                uint32_t hit = target & 0x7FFFFFFFu, r = s->pmap[hit];
+                if (r >= 512u) {
+                    uint32_t rty = r & 0x1FFu, rslot = (r >> 9) - 1;
+                    uint32_t mask = s->photon_mask[rty];
+                    if (rty == 190) {
+                        int bin = (int)(s->part_ctype[rslot] / 4u);
+                        if (bin < 0) bin = 0;
+                        if (bin > 25) bin = 25;
+                        mask = 0x1Fu << (uint32_t)bin;
+                    } else if (rty == 194) {
+                        uint32_t colour = (s->part_ctype[rslot] >> 6) & 0x3Fu;
+                        if ((colour & 48u) == 0) mask |= 1u << 25;
+                        if ((colour & 12u) == 0) mask |= 1u << 15;
+                        if ((colour & 3u) == 0) mask |= 1u << 5;
+                    }
+                    s->part_ctype[i] &= mask;
+                    if (rty == 50 && s->part_life[rslot] == 0) {
+                        int hx = (int)(hit % xres), hy = (int)(hit / xres);
+                        if ((pmap_at(s, hx - 1, hy) & 0x1FFu) == 51 || (pmap_at(s, hx + 1, hy) & 0x1FFu) == 51 ||
+                            (pmap_at(s, hx, hy - 1) & 0x1FFu) == 51 || (pmap_at(s, hx, hy + 1) & 0x1FFu) == 51) {
+                            s->part_ctype[rslot] = 50;
+                            s->part_ty[rslot] = 12;
+                            s->part_life[rslot] = 4;
+                        }
+                    }
+                    if (s->heat_enable) {
+                        if (rty == 62 || rty == 27) s->part_temp[rslot] = s->part_temp[i];
+                        uint32_t hc = s->move_lut[rty * 12 + 3];
+                        if (!is_heat_insulator_c(rty, s->part_life[rslot], s->part_tmp[rslot], hc) && rty != 134) {
+                            float avg = (s->part_temp[rslot] + s->part_temp[i]) * 0.5f;
+                            if (avg > 9999.0f) avg = 9999.0f;
+                            if (avg < 0.0f) avg = 0.0f;
+                            s->part_temp[i] = avg;
+                            s->part_temp[rslot] = avg;
+                        }
+                    }
+                }
+            }
+            continue;
+        }
+        if (target >= xres * yres) continue;
+        s->part_flags[i] &= 0xFFFFFFFEu;
+        uint32_t key = s->cell_winner[target];
+        if ((key & 0x7FFFFFu) != i + 1) continue;
+        if (half_mirror) {
+            s->part_x[i] = (float)(target % xres);
+            s->part_y[i] = (float)(target / xres);
+            continue;
+        }
+        int nx = (int)(target % xres), ny = (int)(target / xres);
+        uint32_t wcell = ((uint32_t)ny / 4) * xc + (uint32_t)nx / 4, wall = s->wall_bmap[wcell];
+        uint32_t dest = s->pmap[target], dty = dest & 0x1FFu, docc = dest >> 9;
+        uint32_t mprops = s->move_lut[ty * 12], dprops = s->move_lut[dty * 12];
+        uint32_t can = 0, other = i;
+        if (docc != 0) {
+            other = docc - 1;
+            can = resolve_can(s, can_move_at(s, ty, dty), ty, dty, other, s->air_pv[wcell]);
+        }
+        int ehole_exit = 0;
+        if (wall == 12 && (s->wall_emap[wcell] & 0xFFu) == 0) {
+            int cx = ora_f2i(s->part_x[i] + 0.5f) / 4, cy = ora_f2i(s->part_y[i] + 0.5f) / 4;
+            uint32_t here = (uint32_t)cy * xc + (uint32_t)cx;
+            if (s->wall_bmap[here] == 12 && (s->wall_emap[here] & 0xFFu) == 0) ehole_exit = 1;
+        }
+        if (ehole_exit) {
+        } else if (docc != 0 && can == 0) {
+        } else if (docc != 0 && can == 1 && other != i) {
+            if (dty == 23 || dty == 75) {
+                s->part_ty[i] = 0;
+            } else if (dty == 91 || dty == 40) {
+                float t = s->part_temp[other] + s->part_temp[i] / 2.0f;
+                s->part_temp[other] = ora_clampf(t, 0.0f, 9999.0f);
+                s->part_ty[i] = 0;
+            } else if (dty == 92 || dty == 41) {
+                if (ty == 122) {
+                    float t = s->part_temp[other] - (9999.0f - s->part_temp[i]) / 2.0f;
+                    s->part_temp[other] = ora_clampf(t, 0.0f, 9999.0f);
+                    s->part_ty[i] = 0;
+                }
+            } else if (dty == 111) {
+                if (ty == 31) {
+                    if (s->part_life[other] < 6000) s->part_life[other] += 1;
+                    s->part_temp[other] = 0.0f;
+                    s->part_ty[i] = 0;
+                }
+            } else if (dty == 165 || dty == 166) {
+                if (mprops & 0x10u) {
+                    s->part_tmp[other] += 20;
+                    s->part_ty[i] = 0;
+                }
+            } else if (ty == 30 && (dprops & 0x100u)) {
+                s->part_ty[i] = 0;
+            } else if (ty == 30) {
+                int own_x = ora_f2i(s->part_x[i] + 0.5f), own_y = ora_f2i(s->part_y[i] + 0.5f);
+                uint32_t own_cell = ((uint32_t)own_y / 4) * xc + (uint32_t)own_x / 4;
+                uint32_t own = s->pmap[(uint32_t)own_y * xres + (uint32_t)own_x];
+                int drag = own != 0 && (s->move_lut[(own & 0x1FFu) * 12] & 0x80u) != 0;
+                if (drag && s->wall_bmap[own_cell] != 15u) {
+                    uint32_t under = (own >> 9) - 1;
+                    s->part_x[under] = (float)nx;
+                    s->part_y[under] = (float)ny;
+                    s->part_x[other] = (float)own_x;
+                    s->part_y[other] = (float)own_y;
+                }
+                s->part_x[i] = (float)nx;
+                s->part_y[i] = (float)ny;
+            } else if (ty == 42 && s->part_life[i] > 0) {
+            } else {
+                uint32_t oraw = s->intent_target[other], otgt = oraw & INTENT_PIXEL_MASK;
+                int other_moves = 0;
+                if (oraw != NO_INTENT && (oraw & 0x80000000u) == 0 && otgt < xres * yres &&
+                    (s->cell_winner[otgt] & 0x7FFFFFu) == other + 1)
+                    other_moves = 1;
+                if (!other_moves) {
+                    float ox = s->part_x[other], oy = s->part_y[other];
+                    s->part_x[other] = s->part_x[i];
+                    s->part_y[other] = s->part_y[i];
+                    s->part_x[i] = ox;
+                    s->part_y[i] = oy;
+                }
+            }
+        } else {
+            s->part_x[i] = (float)nx;
+            s->part_y[i] = (float)ny;
+        }
+    }
+}
+
+/* clear_spatial_maps + scatter_spatial_maps (commit.rs:541-603), rule R7 */
+void oracle_rebuild_maps(oracle_state *s) {
+    const uint32_t xres = s->xres, yres = s->yres;
+    memset(s->pmap, 0, sizeof(uint32_t) * xres * yres);
+    memset(s->photons, 0, sizeof(uint32_t) * xres * yres);
+    for (uint32_t i = 0; i < s->slot_bound; ++i) {
+        uint32_t ty = s->part_ty[i];
+        if (ty == 0 || ty >= 256) continue;
+        int xi = ora_f2i(s->part_x[i] + 0.5f), yi = ora_f2i(s->part_y[i] + 0.5f);
+        if (xi < 0 || xi >= (int)xres || yi < 0 || yi >= (int)yres) continue;
+        uint32_t idx = (uint32_t)yi * xres + (uint32_t)xi, packed = ((i + 1) << 9) | (ty & 0x1FFu);
+        if (s->elem_props[ty * 32 + 1] & 0x10u) s->photons[idx] = packed;
+        else if (s->pmap[idx] == 0 || (ty != 124 && ty != 134)) s->pmap[idx] = packed;
+    }
+}
+
+/* step_movement_gpu (mod.rs:1176-1227) without the pair pass and WL_DETECT. */
+void oracle_step_movement(oracle_state *s, uint32_t seed) {
+    oracle_air_deposit(s);
+    oracle_velocity_intent(s, seed);
+    oracle_commit(s);
+    oracle_rebuild_maps(s);
+}
+
+/* SimulationEngine::step (lib.rs:303-459) over the passes this build covers. */
+void oracle_step(oracle_state *s, uint64_t frame_count) {
+    uint32_t v = (uint32_t)frame_count * 0x9E3779B9u;
+    uint32_t seed = ((v << 13) | (v >> 19)) ^ 0x85EBCA6Bu;
+    oracle_step_air(s);
+    oracle_step_gravity(s);
+    oracle_recalc_free_particles(s, seed, (frame_count % 10u) == 0);
+    oracle_step_thermal(s, seed, s->heat_enable ? 0 : 1);
+    oracle_step_movement(s, seed);
+}

@@ -53,6 +53,7 @@ ARRAYS = [
     ("grav_mass_prev", np.int32, "cell"), ("grav_fx", np.float32, "cell"),
     ("grav_fy", np.float32, "cell"),
     ("elem_props", np.uint32, "props"), ("move_lut", np.uint32, "lut"),
+    ("photon_mask", np.uint32, "table"),
 ]
 _PTR = {np.uint32: _u32p, np.int32: _i32p, np.float32: _f32p}
 
@@ -90,6 +91,23 @@ def lib():
         _lib.oracle_lowbias_hash.restype = ctypes.c_uint32
         _lib.oracle_recalc_free_particles.argtypes = [ctypes.POINTER(OracleStateC), ctypes.c_uint32, ctypes.c_int]
         _lib.oracle_recalc_free_particles.restype = None
+        _lib.oracle_pcg_hash.argtypes = [ctypes.c_uint32]
+        _lib.oracle_pcg_hash.restype = ctypes.c_uint32
+        for name, extra in (("oracle_air_deposit", []), ("oracle_velocity_intent", [ctypes.c_uint32]),
+                            ("oracle_commit", []), ("oracle_rebuild_maps", []),
+                            ("oracle_step_movement", [ctypes.c_uint32]), ("oracle_step", [ctypes.c_uint64])):
+            f = getattr(_lib, name)
+            f.argtypes = [ctypes.POINTER(OracleStateC)] + extra
+            f.restype = None
+        _lib.oracle_gpu_hash.argtypes = [ctypes.c_uint32]
+        _lib.oracle_gpu_hash.restype = ctypes.c_uint32
+        for name, extra in (("oracle_thermal_conduction", [ctypes.c_uint32]), ("oracle_thermal_apply", []),
+                            ("oracle_phase_transitions", [ctypes.c_uint32, ctypes.c_int]),
+                            ("oracle_refresh_pmap_types", []),
+                            ("oracle_step_thermal", [ctypes.c_uint32, ctypes.c_int])):
+            f = getattr(_lib, name)
+            f.argtypes = [ctypes.POINTER(OracleStateC)] + extra
+            f.restype = None
         for name, argt in _OPTIONAL.items():
             f = getattr(_lib, name, None)
             if f is not None:

@@ -23,6 +23,7 @@
  */
 #include "tpt_oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -213,7 +214,7 @@ static void convection_gravity(const oracle_state *s, int x, int y, uint32_t ci,
     } else if (s->gravity_mode == 2) {
         float dx = (float)x - (float)s->xres * 0.5f, dy = (float)y - (float)s->yres * 0.5f;
         if (dx != 0.0f || dy != 0.0f) {
-            float d = 0.01f - sqrtf_(dx * dx + dy * dy);
+            float d = 0.01f - sqrtf(dx * dx + dy * dy);
             gx = pg * (dx / d);
             gy = pg * (dy / d);
         }
@@ -262,7 +263,7 @@ static void convection_swaps(oracle_state *s, uint32_t seed) {
     for (uint32_t i = 0; i < n; ++i) {
         uint32_t t = s->part_ty[i];
         uint32_t roll = oracle_gpu_hash(seed ^ (i * PARTICLE_HASH_SALT) ^ t ^ TRANSITION_HASH_SALT);
-        tgt[i] = (s->heat_enable) ? convect_target(s, i, roll) : 0xFFFFFFFFu;
+        tgt[i] = convect_target(s, i, roll);
     }
     /* min proposer slot touching each particle */
     uint32_t *touch = malloc(sizeof(uint32_t) * (n ? n : 1));
@@ -492,7 +493,8 @@ static void transition_one(oracle_state *s, uint32_t i, uint32_t seed, int legac
                     }
                 } else {
                     uint32_t wall = s->wall_bmap[wi] & 0xFFu;
-                    if (wall == 17u || wall == 2u || wall == 6u || wall == 1u || wall == 11u || wall == 12u) spark[wi] = 1;
+                    /* WL_DETECT, WL_EWALL, WL_ALLOWLIQUID, WL_WALLELEC, WL_ALLOWALLELEC, WL_EHOLE */
+                    if (wall == 3u || wall == 2u || wall == 6u || wall == 1u || wall == 11u || wall == 12u) spark[wi] = 1;
                 }
             }
         }

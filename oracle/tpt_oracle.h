@@ -50,6 +50,7 @@ typedef struct oracle_state {
     /* tables */
     const uint32_t *elem_props;   /* 256 x 32 */
     const uint32_t *move_lut;     /* 256 x 12 + 256 x 256 */
+    const uint32_t *photon_mask;  /* 256 */
 } oracle_state;
 
 /* air_kernel::step_air_gpu (air_kernel/mod.rs:120-299). */
@@ -82,6 +83,25 @@ void oracle_gravity_mask(const uint32_t *wall, uint32_t xc, uint32_t yc, uint32_
 /* RecalcFreeParticles + CheckStacking (buffers/spatial.rs:81-472). */
 uint32_t oracle_lowbias_hash(uint32_t seed);
 void oracle_recalc_free_particles(oracle_state *s, uint32_t seed, int stacking);
+
+/* Conduction, apply, phase transitions, pmap refresh (thermal_kernel.rs). */
+uint32_t oracle_gpu_hash(uint32_t seed);
+void oracle_thermal_conduction(oracle_state *s, uint32_t seed);
+void oracle_thermal_apply(oracle_state *s);
+void oracle_phase_transitions(oracle_state *s, uint32_t seed, int legacy);
+void oracle_refresh_pmap_types(oracle_state *s);
+void oracle_step_thermal(oracle_state *s, uint32_t seed, int legacy);
+
+/* Movement (move_kernel/{mod,velocity,liquid,commit}.rs). */
+uint32_t oracle_pcg_hash(uint32_t seed);
+void oracle_air_deposit(oracle_state *s);
+void oracle_velocity_intent(oracle_state *s, uint32_t seed);
+void oracle_commit(oracle_state *s);
+void oracle_rebuild_maps(oracle_state *s);
+void oracle_step_movement(oracle_state *s, uint32_t seed);
+
+/* SimulationEngine::step (lib.rs:303-459) over the passes above. */
+void oracle_step(oracle_state *s, uint64_t frame_count);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,88 @@
+"""HIP movement pass and whole tick vs the oracle (bit-exact integer state
+and fields), including the 2048x2048 workload at full size through
+size-independent invariants."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tpt_hip import elements as E
+from tpt_hip import lib as L
+from tpt_hip import scene as S
+
+pytestmark = pytest.mark.gpu
+INT_FIELDS = ["part_ty", "part_x", "part_y", "part_life", "part_ctype", "part_tmp", "part_flags", "pmap",
+              "photons", "intent_target", "cell_winner"]
+FLOAT_FIELDS = ["part_vx", "part_vy", "part_temp", "air_vx", "air_vy", "air_pv", "air_hv"]
+
+
+def compare(ref, got, names, label):
+    """Integer arrays bit for bit; float arrays by value (+0 == -0: the sign of
+    a zero produced by a zero-mass FFT carries no information)."""
+    for n in names:
+        a, b = ref.arrays[n], got.arrays[n]
+        if a.dtype != np.float32:
+            a, b = a.view(np.uint32), b.view(np.uint32)
+        if not np.array_equal(a, b):
+            bad = np.flatnonzero(a != b)
+            raise AssertionError(f"{label}: {n} differs at {len(bad)} entries, first {bad[0]}: "
+                                 f"oracle {ref.arrays[n][bad[0]]!r} hip {got.arrays[n][bad[0]]!r}")
+
+
+@pytest.mark.parametrize("workload,scale,ticks,newton", [("612x384", 1.0, 10, True), ("612x384", 1.0, 10, False),
+                                                          ("1600x900", 0.25, 4, True)])
+def test_movement_bit_exact(gpu, workload, scale, ticks, newton):
+    st = S.make_scene(workload, scale_particles=scale, newton_grav=newton)
+    ref = st.copy()
+    o = O.OracleState(ref)
+    print(f"[move parity] {workload} newton={newton}", flush=True)
+    with L.Context(st.cfg) as ctx:
+        ctx.load_state(st)
+        for frame in range(ticks):
+            seed = L.frame_seed(frame)
+            ctx.recalc_free_particles(seed, False)
+            o.call("oracle_recalc_free_particles", seed, 0)
+            ctx.step_movement(seed)
+            o.call("oracle_step_movement", seed)
+        ctx.fetch(st)
+    compare(ref, st, INT_FIELDS + FLOAT_FIELDS, f"{workload} after {ticks} ticks")
+
+
+@pytest.mark.parametrize("workload,scale,ticks", [("612x384", 1.0, 20), ("1600x900", 0.5, 6), ("2048x2048", 0.1, 3)])
+def test_whole_tick_bit_exact(gpu, workload, scale, ticks):
+    st = S.make_scene(workload, scale_particles=scale)
+    ref = st.copy()
+    o = O.OracleState(ref)
+    print(f"[tick parity] {workload} ticks={ticks}", flush=True)
+    with L.Context(st.cfg) as ctx:
+        ctx.load_state(st)
+        for frame in range(ticks):
+            ctx.step(frame)
+            o.call("oracle_step", frame)
+        ctx.fetch(st)
+    compare(ref, st, INT_FIELDS + FLOAT_FIELDS + ["block_air", "block_airh", "wall_emap", "grav_mass"],
+            f"{workload} tick {ticks}")
+
+
+def test_full_size_2048_invariants(gpu):
+    """configs[2] at full size (2.1M particles): the oracle is too slow for many
+    ticks, so check what any correct tick preserves."""
+    st = S.make_scene("2048x2048")
+    n0 = int(np.count_nonzero(st.arrays["part_ty"][: st.slot_bound]))
+    with L.Context(st.cfg) as ctx:
+        ctx.load_state(st)
+        for frame in range(20):
+            ctx.step(frame)
+        ctx.fetch(st, ["part_ty", "part_x", "part_y", "pmap", "air_pv", "air_hv", "part_temp"])
+    ty = st.arrays["part_ty"][: st.slot_bound]
+    live = np.flatnonzero(ty)
+    xs, ys = st.arrays["part_x"][live], st.arrays["part_y"][live]
+    flat = ys.astype(np.int64) * 2048 + xs.astype(np.int64)
+    gas = (E.element_props_table()[ty[live].astype(np.int64) * 32 + 1] & 0x10) != 0
+    assert len(np.unique(flat[~gas])) == int((~gas).sum())        # one matter particle per pixel
+    pm = st.arrays["pmap"]
+    assert np.count_nonzero(pm) == int((~gas).sum())               # the map holds every particle once
+    assert np.array_equal(np.sort((pm[pm != 0] >> 9) - 1), np.sort(live[~gas]).astype(np.uint32))
+    assert len(live) <= n0 and len(live) > 0.99 * n0               # walled box; transitions keep count
+    assert np.isfinite(st.arrays["air_pv"]).all() and np.abs(st.arrays["air_pv"]).max() <= 256
+    t = st.arrays["part_temp"][live]
+    assert t.min() >= 0 and t.max() <= 9999

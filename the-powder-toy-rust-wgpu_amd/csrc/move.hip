@@ -1,0 +1,1032 @@
+// Lock-free movement for gfx950: move_kernel::step_movement_gpu
+// (crates/tpt-sim/src/move_kernel/mod.rs:1176-1227).
+//
+//   mv_air_deposit(+_cells)  particle -> air drag (mod.rs:297-346); the racy
+//                            per-cell store becomes "highest eligible slot
+//                            writes" via atomicMax on slot+1 and a cell pass.
+//   mv_clear                 cell_winner = 0 (mod.rs:268-273), 16-byte stores.
+//   mv_intent                velocity integration, PlanMove and the per-
+//                            Falldown destination chain, then the claim
+//                            atomicMax(cell_winner[pixel],
+//                            (weight_rank << 23) | (slot + 1))
+//                            (velocity.rs:951-1157, liquid.rs:137-963).
+//   mv_saw                   WOOD -> SAWD of rejected probes, after the pass.
+//   mv_commit                winner check, consume/swap/move, FLAG_STAGNANT
+//                            (commit.rs:181-535).
+//   mv_map_clear / mv_map_scatter / mv_map_invis
+//                            pmap/photons rebuild (commit.rs:541-603): the
+//                            highest matter slot owns a pixel (atomicMax on the
+//                            packed word), INVIS/FILT only take an empty one.
+//
+// Not built: photon refraction at glass (velocity.rs:262-647), the e == 2 pair
+// pass (reaction_kernel/energy_entry.rs) and WL_DETECT charging
+// (mod.rs:473-828); a photon that enters a pixel is moved without refraction.
+
+#include "tpt_internal.h"
+
+namespace {
+
+constexpr int BLK = 256;
+constexpr uint32_t NO_INTENT = 0xFFFFFFFFu;
+constexpr uint32_t DIE_INTENT = 0xFFFFFFFEu;
+constexpr uint32_t HALF_MIRROR = 0x40000000u;
+constexpr uint32_t PIXEL_MASK = 0x3FFFFFFFu;
+constexpr uint32_t DO_BLOCKED = 0u, DO_OK = 1u, DO_DIE = 2u;
+constexpr float ISTP = 2.0f;
+constexpr float MAX_VELOCITY = 1.0e4f;
+constexpr uint32_t MAX_PLAN_STEPS = 5001u;
+constexpr uint32_t EDGE_VOID = 0u, EDGE_LOOP = 2u;
+constexpr uint32_t LUT_CAN = 256u * 12u;
+
+struct MoveEnv {
+    const uint32_t *__restrict__ pmap;
+    const uint32_t *__restrict__ wall;
+    const uint32_t *__restrict__ emap;
+    const uint32_t *__restrict__ lut;
+    int xr, yr, xc;
+    uint32_t edge;
+    uint32_t gravity_mode;
+    float cgx, cgy;
+    uint32_t newton_grav;
+    uint32_t *__restrict__ saw_list;
+    uint32_t *__restrict__ saw_count;
+};
+
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t seed) {
+    const uint32_t state = seed * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+__device__ __forceinline__ int round_i32(float v) { return tpt_f2i(v + 0.5f); }
+
+__device__ __forceinline__ bool wall_blocks_particle(uint32_t wall, uint32_t emap, uint32_t props) {
+    if (wall == 8u || wall == 1u || wall == 9u) return true;
+    if (wall == 6u) return (props & 0x2u) == 0u;
+    if (wall == 10u) return (props & 0x1u) == 0u;
+    if (wall == 13u) return (props & 0x8u) == 0u;
+    if (wall == 15u) return (props & 0x10u) == 0u;
+    if (wall == 2u) return (emap & 0xFFu) == 0u;
+    if (wall == 3u) return (props & 0x4u) != 0u;
+    return false;
+}
+
+__device__ __forceinline__ bool wall_kills_particle(uint32_t wall, uint32_t emap, uint32_t props) {
+    if (wall == 8u || wall == 1u || wall == 9u || wall == 7u) return true;
+    if (wall == 6u) return (props & 0x2u) == 0u;
+    if (wall == 10u) return (props & 0x1u) == 0u;
+    if (wall == 13u) return (props & 0x8u) == 0u;
+    if (wall == 15u) return (props & 0x10u) == 0u;
+    if (wall == 2u) return (emap & 0xFFu) == 0u;
+    return false;
+}
+
+__device__ __forceinline__ int wcell(const MoveEnv &e, int x, int y) { return (y / TPT_CELL) * e.xc + x / TPT_CELL; }
+
+// eval_move_at (mod.rs:838-891).
+__device__ uint32_t eval_move_at(const MoveEnv &e, uint32_t props, uint32_t ty, int nx, int ny) {
+    if (!(nx >= 0 && nx < e.xr && ny >= 0 && ny < e.yr)) return 0u;
+    const uint32_t d_ty = e.pmap[ny * e.xr + nx] & 0x1FFu;
+    const int wc = wcell(e, nx, ny);
+    const uint32_t wall = e.wall[wc], em = e.emap[wc];
+    uint32_t can = e.lut[LUT_CAN + ty * 256u + d_ty];
+    if (can == 3u) {
+        if (d_ty == 23u || d_ty == 75u) can = 1u;
+        else if (d_ty == 58u || d_ty == 43u || d_ty == 124u || d_ty == 60u) can = 2u;
+        else can = 1u;
+    }
+    uint32_t result = can;
+    if (wall != 0u) {
+        if (wall_blocks_particle(wall, em, props)) {
+            result = 0u;
+        } else if (wall == 12u && (em & 0xFFu) == 0u) {
+            const uint32_t dest_props = e.lut[d_ty * 12u];
+            if ((props & 0x4u) == 0u && (dest_props & 0x4u) == 0u) result = 2u;
+        }
+    }
+    return result;
+}
+
+__device__ __forceinline__ float remainder_p(float x, float y) {
+    float out = x - floorf(x / y) * y;
+    if (out < 0.0f) out += y;
+    else if (out >= y) out -= y;
+    return out;
+}
+
+__device__ __forceinline__ int wrap_loop_pixel(int n, int res) {
+    const int lo = 4, span = res - 8;
+    if (span > 0 && (n < lo || n >= lo + span)) return lo + tpt_f2i(remainder_p((float)(n - lo), (float)span));
+    return n;
+}
+
+// do_move_at (mod.rs:1009-1056).
+__device__ uint32_t do_move_at(const MoveEnv &e, uint32_t props, uint32_t ty, int nx, int ny, int &ox, int &oy) {
+    int mx = nx, my = ny;
+    if (e.edge == EDGE_LOOP) {
+        mx = wrap_loop_pixel(nx, e.xr);
+        my = wrap_loop_pixel(ny, e.yr);
+    }
+    uint32_t status = DO_BLOCKED;
+    if (mx < 0 || my < 0 || mx >= e.xr || my >= e.yr) {
+        if (e.edge != EDGE_LOOP) status = DO_DIE;
+    } else if (eval_move_at(e, props, ty, mx, my) != 0u) {
+        const bool in_band = mx >= 4 && my >= 4 && mx < e.xr - 4 && my < e.yr - 4;
+        status = DO_OK;
+        if (!in_band && e.edge != EDGE_LOOP) {
+            status = DO_BLOCKED;
+            if (e.edge == EDGE_VOID) status = DO_DIE;
+        }
+    }
+    ox = mx;
+    oy = my;
+    return status;
+}
+
+__device__ void saw_wood_at(const MoveEnv &e, int nx, int ny, float vx, float vy) {
+    if (nx >= 0 && ny >= 0 && nx < e.xr && ny < e.yr) {
+        const uint32_t v = e.pmap[ny * e.xr + nx];
+        if (v >= 512u && (v & 0x1FFu) == 15u && vx * vx + vy * vy > 25.0f)
+            e.saw_list[atomicAdd(e.saw_count, 1u)] = (v >> 9) - 1u;
+    }
+}
+
+__device__ __forceinline__ uint32_t pix(const MoveEnv &e, int x, int y) { return (uint32_t)(y * e.xr + x); }
+
+// plan_sample_blocked (liquid.rs:80-118).
+__device__ bool plan_sample_blocked(const MoveEnv &e, uint32_t props, uint32_t ty, int nx, int ny, bool closed_start) {
+    if (nx < 0 || ny < 0 || nx >= e.xr || ny >= e.yr) return true;
+    const uint32_t packed = e.pmap[ny * e.xr + nx];
+    const uint32_t d_ty = packed & 0x1FFu;
+    const uint32_t raw_can = e.lut[LUT_CAN + ty * 256u + d_ty];
+    const uint32_t ev = eval_move_at(e, props, ty, nx, ny);
+    const int wc = wcell(e, nx, ny);
+    const uint32_t wall = e.wall[wc];
+    const bool closed_now = wall == 12u && (e.emap[wc] & 0xFFu) == 0u;
+    return ev == 0u || (raw_can == 3u && ev == 1u) || (ty == 29u && packed != 0u) || wall == 7u ||
+           closed_now != closed_start;
+}
+
+struct Plan {
+    int fin_x, fin_y, clear_x, clear_y;
+    float clear_xf, clear_yf;
+};
+
+__device__ __forceinline__ float wrap_band(float f, int res) {
+    const bool ok = f >= 4.0f - 0.5f && f < (float)res - 4.0f - 0.5f;
+    if (!ok) f = remainder_p(f - 4.0f + 0.5f, (float)res - 8.0f) + 4.0f - 0.5f;
+    return f;
+}
+
+// plan_move (liquid.rs:137-261).
+__device__ Plan plan_move(const MoveEnv &e, float vx, float vy, float px, float py, int xi, int yi, uint32_t props,
+                          uint32_t ty) {
+    float mv = fabsf(vx);
+    if (fabsf(vy) > mv) mv = fabsf(vy);
+    Plan p;
+    p.clear_xf = px;
+    p.clear_yf = py;
+    float fin_xf = px + vx, fin_yf = py + vy;
+    p.fin_x = round_i32(fin_xf);
+    p.fin_y = round_i32(fin_yf);
+    p.clear_x = xi;
+    p.clear_y = yi;
+    if (mv >= ISTP) {
+        const int sc = wcell(e, xi, yi);
+        const bool start_closed = e.wall[sc] == 12u && (e.emap[sc] & 0xFFu) == 0u;
+        const float step_xf = vx * ISTP / mv, step_yf = vy * ISTP / mv;
+        float walk_xf = px, walk_yf = py, remaining = mv;
+        for (uint32_t guard = 0; guard < MAX_PLAN_STEPS; ++guard) {
+            remaining -= ISTP;
+            walk_xf += step_xf;
+            walk_yf += step_yf;
+            if (e.edge == EDGE_LOOP) {
+                walk_xf = wrap_band(walk_xf, e.xr);
+                walk_yf = wrap_band(walk_yf, e.yr);
+            }
+            const int nfx = round_i32(walk_xf), nfy = round_i32(walk_yf);
+            if (remaining <= 0.0f) {
+                fin_xf = px + vx;
+                fin_yf = py + vy;
+                if (e.edge == EDGE_LOOP) {
+                    fin_xf = wrap_band(fin_xf, e.xr);
+                    fin_yf = wrap_band(fin_yf, e.yr);
+                }
+                p.fin_x = round_i32(fin_xf);
+                p.fin_y = round_i32(fin_yf);
+                p.clear_xf = fin_xf - step_xf;
+                p.clear_yf = fin_yf - step_yf;
+                p.clear_x = round_i32(p.clear_xf);
+                p.clear_y = round_i32(p.clear_yf);
+                break;
+            }
+            if (plan_sample_blocked(e, props, ty, nfx, nfy, start_closed)) {
+                p.fin_x = nfx;
+                p.fin_y = nfy;
+                p.clear_xf = walk_xf - step_xf;
+                p.clear_yf = walk_yf - step_yf;
+                p.clear_x = round_i32(p.clear_xf);
+                p.clear_y = round_i32(p.clear_yf);
+                break;
+            }
+        }
+    }
+    return p;
+}
+
+// base_gravity (velocity.rs:133-170).
+__device__ __forceinline__ void base_gravity(const MoveEnv &e, float gravity, float px, float py, float &gx,
+                                             float &gy) {
+    gx = 0.0f;
+    gy = 0.0f;
+    if (e.gravity_mode == 0u) {
+        gy = gravity;
+    } else if (e.gravity_mode == 2u) {
+        const float dx = px - (float)(e.xr / 2), dy = py - (float)(e.yr / 2);
+        if (dx != 0.0f || dy != 0.0f) {
+            const float d = 0.01f - sqrtf(dx * dx + dy * dy);
+            gx = gravity * (dx / d);
+            gy = gravity * (dy / d);
+        }
+    } else if (e.gravity_mode == 3u) {
+        gx = gravity * e.cgx;
+        gy = gravity * e.cgy;
+    }
+}
+
+__device__ __forceinline__ int step_axis(float v, uint32_t h) {
+    const float a = v < 0.0f ? -v : v;
+    const float r = (float)(h & 0xFFFFu) * (1.0f / 65535.0f);
+    if (a >= 1.0f || r < a) {
+        if (v > 0.0f) return 1;
+        if (v < 0.0f) return -1;
+    }
+    return 0;
+}
+
+__device__ uint32_t diagonal_liquid_move(const MoveEnv &e, float vx, float vy, float cxf, float cyf, int rdir,
+                                         uint32_t props, uint32_t ty) {
+    float ddx = vx - vy * (float)rdir;
+    float ddy = vy + vx * (float)rdir;
+    float dmv = fabsf(ddx);
+    if (fabsf(ddy) > dmv) dmv = fabsf(ddy);
+    if (!(dmv > 0.0001f)) return NO_INTENT;
+    ddx /= dmv;
+    ddy /= dmv;
+    int ox, oy;
+    const uint32_t d1 = do_move_at(e, props, ty, round_i32(cxf + ddx), round_i32(cyf + ddy), ox, oy);
+    if (d1 == DO_DIE) return DIE_INTENT;
+    if (d1 == DO_OK) return pix(e, ox, oy);
+    const uint32_t d2 =
+        do_move_at(e, props, ty, round_i32(cxf + ddy * (float)rdir), round_i32(cyf - ddx * (float)rdir), ox, oy);
+    if (d2 == DO_DIE) return DIE_INTENT;
+    if (d2 == DO_OK) return pix(e, ox, oy);
+    return NO_INTENT;
+}
+
+// scan_liquid_vertical (liquid.rs:630-733).
+__device__ uint32_t scan_liquid_vertical(const MoveEnv &e, int xi, int yi, int fin_y, int clear_x, int clear_y,
+                                         int rdir, int rt, uint32_t ty, uint32_t props, bool &fall) {
+    int scan_x = xi, scan_y = yi, steps = 0;
+    bool hit = false, die = false, stop = false;
+    int j = clear_x + rdir;
+    fall = false;
+    while (!stop && steps < rt) {
+        if (j < 0 || j >= e.xr || j < clear_x - rt || j >= clear_x + rt) {
+            stop = true;
+        } else {
+            if (fin_y >= 0 && fin_y < e.yr) {
+                const uint32_t d_ty = e.pmap[pix(e, j, fin_y)] & 0x1FFu;
+                if (d_ty != ty || e.wall[wcell(e, j, fin_y)] != 0u) {
+                    int ox, oy;
+                    const uint32_t d = do_move_at(e, props, ty, j, fin_y, ox, oy);
+                    if (d == DO_DIE) {
+                        die = true;
+                        stop = true;
+                    } else if (d == DO_OK) {
+                        hit = true;
+                        fall = true;
+                        scan_x = ox;
+                        scan_y = oy;
+                        stop = true;
+                    }
+                }
+            }
+            if (!stop && fin_y != clear_y && clear_y >= 0 && clear_y < e.yr) {
+                const uint32_t d_ty = e.pmap[pix(e, j, clear_y)] & 0x1FFu;
+                if (d_ty != ty || e.wall[wcell(e, j, clear_y)] != 0u) {
+                    int ox, oy;
+                    const uint32_t d = do_move_at(e, props, ty, j, clear_y, ox, oy);
+                    if (d == DO_DIE) {
+                        die = true;
+                        stop = true;
+                    } else if (d == DO_OK) {
+                        hit = true;
+                        fall = true;
+                        scan_x = ox;
+                        scan_y = oy;
+                        stop = true;
+                    }
+                }
+            }
+            if (!stop && clear_y >= 0 && clear_y < e.yr) {
+                const uint32_t d_ty = e.pmap[pix(e, j, clear_y)] & 0x1FFu;
+                const uint32_t wall = e.wall[wcell(e, j, clear_y)];
+                if (d_ty != ty || (wall != 0u && wall != 4u)) stop = true;
+            }
+            if (!stop) {
+                j += rdir;
+                steps++;
+            }
+        }
+    }
+    if (die) return DIE_INTENT;
+    if (hit) return pix(e, scan_x, scan_y);
+    return NO_INTENT;
+}
+
+// scroll_liquid_down (liquid.rs:735-787).
+__device__ uint32_t scroll_liquid_down(const MoveEnv &e, int scan_x, int scan_y, float vy, int rt, uint32_t ty,
+                                       uint32_t props) {
+    uint32_t target = pix(e, scan_x, scan_y);
+    const int vr = vy > 0.0f ? 1 : -1;
+    int jj = scan_y + vr, steps = 0;
+    bool stop = false;
+    while (!stop && steps < rt) {
+        if (jj < 0 || jj >= e.yr || jj < scan_y - rt || jj >= scan_y + rt) {
+            stop = true;
+        } else {
+            const uint32_t d_ty = e.pmap[pix(e, scan_x, jj)] & 0x1FFu;
+            const uint32_t wall = e.wall[wcell(e, scan_x, jj)];
+            if (d_ty != ty || wall != 0u) {
+                int ox, oy;
+                const uint32_t d = do_move_at(e, props, ty, scan_x, jj, ox, oy);
+                if (d == DO_DIE) {
+                    target = DIE_INTENT;
+                    stop = true;
+                } else if (d == DO_OK) {
+                    target = pix(e, ox, oy);
+                    stop = true;
+                } else if (d_ty != ty || wall != 4u) {
+                    stop = true;
+                }
+            }
+            if (!stop) {
+                jj += vr;
+                steps++;
+            }
+        }
+    }
+    return target;
+}
+
+// scan_liquid_perpendicular (liquid.rs:801-963).
+__device__ uint32_t scan_liquid_perpendicular(const MoveEnv &e, float cxf, float cyf, int cx, int cy, int rdir,
+                                              int rt, float gravity, uint32_t props, uint32_t ty) {
+    uint32_t target = NO_INTENT;
+    float nxf = cxf, nyf = cyf, pgx = 0.0f, pgy = 0.0f;
+    int nx = cx, ny = cy, j = 0;
+    bool moved = false, stop = false;
+    while (!stop && j < rt) {
+        float gx, gy;
+        base_gravity(e, gravity, (float)nx, (float)ny, gx, gy);
+        float gmv = fabsf(gx);
+        if (fabsf(gy) > gmv) gmv = fabsf(gy);
+        if (gmv < 0.0001f) {
+            stop = true;
+        } else {
+            gx /= gmv;
+            gy /= gmv;
+            if (j > 0) {
+                nxf += (float)rdir * (gy * 2.0f - pgy);
+                nyf += -(float)rdir * (gx * 2.0f - pgx);
+            } else {
+                nxf += (float)rdir * gy;
+                nyf += -(float)rdir * gx;
+            }
+            pgx = gx;
+            pgy = gy;
+            nx = round_i32(nxf);
+            ny = round_i32(nyf);
+            if (nx < 0 || ny < 0 || nx >= e.xr || ny >= e.yr) {
+                stop = true;
+            } else {
+                const uint32_t d_ty = e.pmap[pix(e, nx, ny)] & 0x1FFu;
+                const uint32_t wall = e.wall[wcell(e, nx, ny)];
+                if (d_ty != ty || wall != 0u) {
+                    int ox, oy;
+                    const uint32_t d = do_move_at(e, props, ty, nx, ny, ox, oy);
+                    if (d == DO_DIE) {
+                        target = DIE_INTENT;
+                        stop = true;
+                    } else if (d == DO_OK) {
+                        target = pix(e, ox, oy);
+                        moved = true;
+                        stop = true;
+                    } else if (d_ty != ty || wall != 4u) {
+                        stop = true;
+                    }
+                }
+            }
+        }
+        if (!stop) j++;
+    }
+    if (moved) {
+        int j2 = 0;
+        bool stop2 = false;
+        while (!stop2 && j2 < rt) {
+            float gx, gy;
+            base_gravity(e, gravity, (float)nx, (float)ny, gx, gy);
+            float gmv = fabsf(gx);
+            if (fabsf(gy) > gmv) gmv = fabsf(gy);
+            if (gmv < 0.0001f) {
+                stop2 = true;
+            } else {
+                gx /= gmv;
+                gy /= gmv;
+                nxf += gx;
+                nyf += gy;
+                const int nnx = round_i32(nxf), nny = round_i32(nyf);
+                if (nnx < 0 || nny < 0 || nnx >= e.xr || nny >= e.yr) {
+                    stop2 = true;
+                } else {
+                    const uint32_t d_ty = e.pmap[pix(e, nnx, nny)] & 0x1FFu;
+                    const uint32_t wall = e.wall[wcell(e, nnx, nny)];
+                    if (d_ty != ty || wall != 0u) {
+                        int ox, oy;
+                        const uint32_t d = do_move_at(e, props, ty, nnx, nny, ox, oy);
+                        if (d == DO_DIE) {
+                            target = DIE_INTENT;
+                            stop2 = true;
+                        } else if (d == DO_OK) {
+                            target = pix(e, ox, oy);
+                            stop2 = true;
+                        } else if (d_ty != ty || wall != 4u) {
+                            stop2 = true;
+                        }
+                    }
+                }
+            }
+            if (!stop2) j2++;
+        }
+    }
+    return target;
+}
+
+// fall_down_destination (liquid.rs:388-617).
+__device__ uint32_t fall_down_destination(const MoveEnv &e, float &vx, float &vy, float px, float py, int xi, int yi,
+                                          int sx, int sy, uint32_t props, uint32_t ty, uint32_t falldown,
+                                          uint32_t flags, float collision, uint32_t h2, float gravity,
+                                          int32_t gel_tmp) {
+    Plan p = plan_move(e, vx, vy, px, py, xi, yi, props, ty);
+    if (p.fin_x == xi && p.fin_y == yi && (sx != 0 || sy != 0)) {
+        p.fin_x = xi + sx;
+        p.fin_y = yi + sy;
+        p.clear_x = xi;
+        p.clear_y = yi;
+        p.clear_xf = px;
+        p.clear_yf = py;
+    }
+    uint32_t target = NO_INTENT;
+    if (p.fin_x == xi && p.fin_y == yi) return target;
+    int ox, oy;
+    const uint32_t d0 = do_move_at(e, props, ty, p.fin_x, p.fin_y, ox, oy);
+    if (d0 == DO_DIE) target = DIE_INTENT;
+    else if (d0 == DO_OK) target = pix(e, ox, oy);
+    else saw_wood_at(e, ox, oy, vx, vy);
+    if (target == NO_INTENT && p.fin_x != xi) {
+        const uint32_t d1 = do_move_at(e, props, ty, p.fin_x, p.clear_y, ox, oy);
+        if (d1 == DO_DIE) target = DIE_INTENT;
+        else if (d1 == DO_OK) target = pix(e, ox, oy);
+        else saw_wood_at(e, ox, oy, vx, vy);
+    }
+    if (target == NO_INTENT && p.fin_y != yi) {
+        const uint32_t d2 = do_move_at(e, props, ty, p.clear_x, p.fin_y, ox, oy);
+        if (d2 == DO_DIE) target = DIE_INTENT;
+        else if (d2 == DO_OK) target = pix(e, ox, oy);
+        else saw_wood_at(e, ox, oy, vx, vy);
+    }
+    if (target != NO_INTENT) return target;
+    // liquid_neighbourhood (liquid.rs:264-286)
+    uint32_t nt = 0, space = 0;
+    for (int oy2 = 0; oy2 < 3; ++oy2)
+        for (int ox2 = 0; ox2 < 3; ++ox2) {
+            if (ox2 == 1 && oy2 == 1) continue;
+            const int nx = xi + ox2 - 1, ny = yi + oy2 - 1;
+            uint32_t nty = 0u;
+            if (nx >= 0 && nx < e.xr && ny >= 0 && ny < e.yr) nty = e.pmap[ny * e.xr + nx] & 0x1FFu;
+            if (nty == 0u) space++;
+            if (nty != ty) nt++;
+        }
+    const int rdir = (h2 & 2u) == 0u ? 1 : -1;
+    const bool clear_moved = p.clear_x != xi || p.clear_y != yi;
+    const bool moving = fabsf(vx) > 0.01f || fabsf(vy) > 0.01f;
+    if ((clear_moved || nt > 0u || space > 0u) && moving)
+        target = diagonal_liquid_move(e, vx, vy, p.clear_xf, p.clear_yf, rdir, props, ty);
+    if (target == NO_INTENT) {
+        if (falldown > 1u) {
+            if (e.newton_grav == 0u && e.gravity_mode == 0u && vy > fabsf(vx)) {
+                int rt = ((flags & 0x01u) && nt == 0u) ? 10 : 30;
+                if (ty == 150u) rt = tpt_f2i((float)gel_tmp * 0.20f + 5.0f);
+                bool fall;
+                const uint32_t st =
+                    scan_liquid_vertical(e, xi, yi, p.fin_y, p.clear_x, p.clear_y, rdir, rt, ty, props, fall);
+                if (st != NO_INTENT) {
+                    if (fall)
+                        target = scroll_liquid_down(e, (int)(st % (uint32_t)e.xr), (int)(st / (uint32_t)e.xr), vy, rt,
+                                                    ty, props);
+                    else
+                        target = st;
+                }
+            } else {
+                float gx, gy;
+                base_gravity(e, gravity, (float)xi, (float)yi, gx, gy);
+                float gnorm = fabsf(gx);
+                if (fabsf(gy) > gnorm) gnorm = fabsf(gy);
+                bool aligned = false;
+                if (gnorm > 0.0001f) {
+                    const float ngx = gx / gnorm, ngy = gy / gnorm;
+                    aligned = fabsf(ngx * vx + ngy * vy) > fabsf(ngy * vx - ngx * vy);
+                }
+                if (aligned) {
+                    const int rt = ((flags & 0x01u) && nt == 0u) ? 10 : 30;
+                    target = scan_liquid_perpendicular(e, p.clear_xf, p.clear_yf, p.clear_x, p.clear_y, rdir, rt,
+                                                       gravity, props, ty);
+                }
+            }
+        }
+        if (target == NO_INTENT && clear_moved) {
+            const uint32_t d = do_move_at(e, props, ty, p.clear_x, p.clear_y, ox, oy);
+            if (d == DO_DIE) target = DIE_INTENT;
+            else if (d == DO_OK) target = pix(e, ox, oy);
+        }
+    }
+    vx *= collision;
+    vy *= collision;
+    return target;
+}
+
+// gas_destination (velocity.rs:755-928) without the glass refraction.
+__device__ uint32_t gas_destination(const MoveEnv &e, float &vx, float &vy, float px, float py, int xi, int yi, int sx,
+                                    int sy, uint32_t props, uint32_t ty, uint32_t slot, float collision,
+                                    uint32_t seed) {
+    const Plan p = plan_move(e, vx, vy, px, py, xi, yi, props, ty);
+    int fin_x = p.fin_x, fin_y = p.fin_y;
+    if (fin_x == xi && fin_y == yi && (sx != 0 || sy != 0)) {
+        fin_x = xi + sx;
+        fin_y = yi + sy;
+    }
+    uint32_t target = NO_INTENT;
+    bool blocked = false, died = false;
+    if (fin_x != xi || fin_y != yi) {
+        int ox, oy;
+        const uint32_t d = do_move_at(e, props, ty, fin_x, fin_y, ox, oy);
+        if (d == DO_OK) {
+            target = pix(e, ox, oy);
+        } else if (d == DO_DIE) {
+            died = true;
+        } else {
+            blocked = true;
+            if (ty == 29u) {
+                const uint32_t dest_px = pix(e, ox, oy);
+                const uint32_t dest_ty = e.pmap[dest_px] & 0x1FFu;
+                const uint32_t src_ty = e.pmap[pix(e, xi, yi)] & 0x1FFu;
+                const uint32_t roll = pcg_hash(seed ^ (slot * 0x9E3779B9u) ^ (dest_px * 0x85EBCA6Bu));
+                if ((dest_ty == 46u && (roll & 1u) == 0u) || src_ty == 46u) {
+                    blocked = false;
+                    target = HALF_MIRROR | dest_px;
+                } else {
+                    target = 0x80000000u | dest_px;
+                    saw_wood_at(e, ox, oy, vx, vy);
+                }
+            } else {
+                saw_wood_at(e, ox, oy, vx, vy);
+            }
+        }
+    }
+    if (died) {
+        target = DIE_INTENT;
+    } else if (blocked) {
+        if (ty == 29u) {
+            if (fin_x != xi) vx *= collision;
+            if (fin_y != yi) vy *= collision;
+        } else {
+            const int bx = min(max(fin_x, xi - 2), xi + 2), by = min(max(fin_y, yi - 2), yi + 2);
+            const int rx = 2 * xi - bx, ry = 2 * yi - by;
+            int ox, oy;
+            const uint32_t d1 = do_move_at(e, props, ty, rx, by, ox, oy);
+            if (d1 != DO_BLOCKED) {
+                if (d1 == DO_DIE) target = DIE_INTENT;
+                else if (ox != xi || oy != yi) target = pix(e, ox, oy);
+                vx *= collision;
+            } else {
+                saw_wood_at(e, ox, oy, vx, vy);
+                const uint32_t d2 = do_move_at(e, props, ty, bx, ry, ox, oy);
+                if (d2 != DO_BLOCKED) {
+                    if (d2 == DO_DIE) target = DIE_INTENT;
+                    else if (ox != xi || oy != yi) target = pix(e, ox, oy);
+                    vy *= collision;
+                } else {
+                    saw_wood_at(e, ox, oy, vx, vy);
+                    vx *= collision;
+                    vy *= collision;
+                }
+            }
+        }
+    }
+    return target;
+}
+
+struct Slots {
+    uint32_t *ty;
+    float *x, *y, *vx, *vy;
+    uint32_t *flags;
+    int32_t *tmp;
+    uint32_t *intent;
+};
+
+__global__ __launch_bounds__(BLK) void mv_air_deposit(Slots s, const uint32_t *__restrict__ wall,
+                                                      const uint32_t *__restrict__ emap, uint32_t *__restrict__ key,
+                                                      uint32_t slots, int xres, int yres, int xc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    const uint32_t ty = s.ty[i];
+    if (ty == 0u || ty >= 256u) return;
+    const uint32_t flags = s.flags[i];
+    if ((flags & 0x10u) || (flags & 0x02u)) return;
+    const int xi = tpt_f2i(s.x[i] + 0.5f), yi = tpt_f2i(s.y[i] + 0.5f);
+    if (xi < 4 || yi < 4 || xi >= xres - 4 || yi >= yres - 4) return;
+    const int cell = (yi / TPT_CELL) * xc + xi / TPT_CELL;
+    if (wall[cell] == 18u && (emap[cell] & 0xFFu) < 8u) return;
+    atomicMax(&key[cell], i + 1u);
+}
+
+__global__ void mv_air_deposit_cells(Slots s, const uint32_t *__restrict__ lut, float *__restrict__ air_vx,
+                                     float *__restrict__ air_vy, uint32_t *__restrict__ key, uint32_t cells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const uint32_t k = key[c];
+    if (!k) return;
+    const uint32_t i = k - 1u, base = s.ty[i] * 12u + 4u;
+    const float drag = __uint_as_float(lut[base + 1u]), loss = __uint_as_float(lut[base + 2u]);
+    air_vx[c] = air_vx[c] * loss + drag * s.vx[i];
+    air_vy[c] = air_vy[c] * loss + drag * s.vy[i];
+    key[c] = 0u;
+}
+
+__global__ void mv_clear(uint4 *__restrict__ winner, uint32_t n4, uint32_t *__restrict__ counters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) counters[1] = 0u;
+    if (i < n4) winner[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// velocity_and_intent_kernel (velocity.rs:951-1157).
+__global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, const float *__restrict__ air_vx,
+                                                 const float *__restrict__ air_vy, uint32_t *__restrict__ winner,
+                                                 uint32_t slots, uint32_t seed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    s.intent[i] = NO_INTENT;
+    const uint32_t ty = s.ty[i];
+    if (ty == 0u || ty >= 256u) return;
+    const uint32_t flags = s.flags[i];
+    if ((flags & 0x10u) || (flags & 0x02u)) return;
+    const float px = s.x[i], py = s.y[i];
+    const int xi = tpt_f2i(px + 0.5f), yi = tpt_f2i(py + 0.5f);
+    if (xi < 0 || xi >= e.xr || yi < 0 || yi >= e.yr) {
+        s.ty[i] = 0u;
+        return;
+    }
+    const int cell = wcell(e, xi, yi);
+    const uint32_t meta = ty * 12u;
+    const uint32_t props = e.lut[meta], falldown = e.lut[meta + 1u], rank = e.lut[meta + 2u];
+    const float advection = __uint_as_float(e.lut[meta + 4u]), loss = __uint_as_float(e.lut[meta + 7u]);
+    const float collision = __uint_as_float(e.lut[meta + 8u]), gravity = __uint_as_float(e.lut[meta + 9u]);
+    const float diffusion = __uint_as_float(e.lut[meta + 10u]);
+    const uint32_t wall = e.wall[cell], em = e.emap[cell];
+    const bool in_margin = !(xi < 4 || yi < 4 || xi >= e.xr - 4 || yi >= e.yr - 4);
+    const bool stickman = ty == 59u || ty == 137u || ty == 159u;
+    if (!in_margin || (!stickman && wall_kills_particle(wall, em, props))) {
+        s.ty[i] = 0u;
+        return;
+    }
+    if (wall == 18u && (em & 0xFFu) < 8u) return;
+    float gx, gy;
+    base_gravity(e, gravity, (float)xi, (float)yi, gx, gy);
+    const float ax = air_vx[cell], ay = air_vy[cell];
+    const uint32_t h1 = pcg_hash(seed ^ (i * 2654435769u)), h2 = pcg_hash(h1);
+    const float loss_here = e.newton_grav ? 1.0f : loss;
+    const float rdx = (float)(h1 & 0xFFFFu) * (2.0f / 65535.0f) - 1.0f;
+    const float rdy = (float)(h2 & 0xFFFFu) * (2.0f / 65535.0f) - 1.0f;
+    float vx = s.vx[i] * loss_here + advection * ax + gx + diffusion * rdx;
+    float vy = s.vy[i] * loss_here + advection * ay + gy + diffusion * rdy;
+    float mv = fabsf(vx);
+    if (fabsf(vy) > mv) mv = fabsf(vy);
+    if (mv > MAX_VELOCITY) {
+        vx = vx * (MAX_VELOCITY / mv);
+        vy = vy * (MAX_VELOCITY / mv);
+    }
+    const int sx = step_axis(vx, h1), sy = step_axis(vy, h2);
+    uint32_t target;
+    if (falldown == 0u)
+        target = gas_destination(e, vx, vy, px, py, xi, yi, sx, sy, props, ty, i, collision, seed);
+    else
+        target = fall_down_destination(e, vx, vy, px, py, xi, yi, sx, sy, props, ty, falldown, flags, collision, h2,
+                                       gravity, s.tmp[i]);
+    if (target == DIE_INTENT) {
+        s.ty[i] = 0u;
+        return;
+    }
+    if (target != NO_INTENT) {
+        s.intent[i] = target;
+        if ((target & 0x80000000u) == 0u) {
+            const uint32_t key = ((rank & 0x1FFu) << 23) | ((i + 1u) & 0x7FFFFFu);
+            atomicMax(&winner[target & PIXEL_MASK], key);
+        }
+    }
+    s.vx[i] = vx;
+    s.vy[i] = vy;
+}
+
+__global__ void mv_saw(uint32_t *__restrict__ ty, const uint32_t *__restrict__ list, const uint32_t *__restrict__ counters) {
+    const uint32_t n = counters[1];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) ty[list[k]] = 180u;
+}
+
+struct CommitBufs {
+    uint32_t *ctype;
+    int32_t *life;
+    float *temp;
+    const uint32_t *photon_mask;
+    const float *air_pv;
+};
+
+__device__ __forceinline__ bool void_admits(uint32_t ty, uint32_t vctype, int32_t vtmp) {
+    if (vctype == 0u) return true;
+    const bool matches = vctype == ty, inv = (vtmp & 1) != 0;
+    return inv ? !matches : matches;
+}
+
+__device__ __forceinline__ uint32_t pmap_at(const uint32_t *pmap, int x, int y, int xr, int yr) {
+    return (x < 0 || y < 0 || x >= xr || y >= yr) ? 0u : pmap[y * xr + x];
+}
+
+// commit_move_kernel (commit.rs:181-535).
+__global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const uint32_t *__restrict__ pmap,
+                                                 const uint32_t *__restrict__ wall_a,
+                                                 const uint32_t *__restrict__ emap_a, const uint32_t *__restrict__ lut,
+                                                 const uint32_t *__restrict__ winner, uint32_t slots, int xres, int yres,
+                                                 int xc, uint32_t heat_enable) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    const uint32_t ty = s.ty[i];
+    if (ty == 0u || ty >= 256u) return;
+    const uint32_t raw = s.intent[i];
+    if (raw == NO_INTENT) {
+        if (lut[ty * 12u + 1u] >= 2u && (s.flags[i] & 0x10u) == 0u) s.flags[i] |= 0x01u;
+        return;
+    }
+    const bool half_mirror = (raw & HALF_MIRROR) != 0u;
+    const uint32_t target = raw & PIXEL_MASK;
+    if (raw & 0x80000000u) {
+        if (ty == 29u) {
+            const uint32_t hit = target & 0x7FFFFFFFu, r = pmap[hit];
+            if (r >= 512u) {
+                const uint32_t rty = r & 0x1FFu, rslot = (r >> 9) - 1u;
+                uint32_t mask = cb.photon_mask[rty];
+                if (rty == 190u) {
+                    const int bin = min(max((int)(cb.ctype[rslot] / 4u), 0), 25);
+                    mask = 0x1Fu << (uint32_t)bin;
+                } else if (rty == 194u) {
+                    const uint32_t colour = (cb.ctype[rslot] >> 6) & 0x3Fu;
+                    if ((colour & 48u) == 0u) mask |= 1u << 25;
+                    if ((colour & 12u) == 0u) mask |= 1u << 15;
+                    if ((colour & 3u) == 0u) mask |= 1u << 5;
+                }
+                cb.ctype[i] &= mask;
+                if (rty == 50u && cb.life[rslot] == 0) {
+                    const int hx = (int)(hit % (uint32_t)xres), hy = (int)(hit / (uint32_t)xres);
+                    if ((pmap_at(pmap, hx - 1, hy, xres, yres) & 0x1FFu) == 51u ||
+                        (pmap_at(pmap, hx + 1, hy, xres, yres) & 0x1FFu) == 51u ||
+                        (pmap_at(pmap, hx, hy - 1, xres, yres) & 0x1FFu) == 51u ||
+                        (pmap_at(pmap, hx, hy + 1, xres, yres) & 0x1FFu) == 51u) {
+                        cb.ctype[rslot] = 50u;
+                        s.ty[rslot] = 12u;
+                        cb.life[rslot] = 4;
+                    }
+                }
+                if (heat_enable) {
+                    if (rty == 62u || rty == 27u) cb.temp[rslot] = cb.temp[i];
+                    const uint32_t hc = lut[rty * 12u + 3u];
+                    const bool ins = hc == 0u || (rty == 68u && cb.life[rslot] != 10) ||
+                                     ((rty == 82u || rty == 162u) && (s.tmp[rslot] & 1) == 0);
+                    if (!ins && rty != 134u) {
+                        const float avg = tpt_clampf((cb.temp[rslot] + cb.temp[i]) * 0.5f, 0.0f, 9999.0f);
+                        cb.temp[i] = avg;
+                        cb.temp[rslot] = avg;
+                    }
+                }
+            }
+        }
+        return;
+    }
+    if (target >= (uint32_t)(xres * yres)) return;
+    s.flags[i] &= 0xFFFFFFFEu;
+    if ((winner[target] & 0x7FFFFFu) != i + 1u) return;
+    if (half_mirror) {
+        s.x[i] = (float)(target % (uint32_t)xres);
+        s.y[i] = (float)(target / (uint32_t)xres);
+        return;
+    }
+    const int nx = (int)(target % (uint32_t)xres), ny = (int)(target / (uint32_t)xres);
+    const int wc = (ny / TPT_CELL) * xc + nx / TPT_CELL;
+    const uint32_t wall = wall_a[wc];
+    const uint32_t dest = pmap[target], dty = dest & 0x1FFu, docc = dest >> 9;
+    const uint32_t mprops = lut[ty * 12u], dprops = lut[dty * 12u];
+    uint32_t can = 0u, other = i;
+    if (docc != 0u) {
+        other = docc - 1u;
+        can = lut[LUT_CAN + ty * 256u + dty];
+        if (can == 3u) {  // resolve_can (commit.rs:76-128)
+            if (dty == 58u && ty == 29u) can = cb.life[other] > 5 ? 2u : 0u;
+            else if (dty == 43u && ty == 29u) can = cb.life[other] < 10 ? 2u : 0u;
+            else if (dty == 124u) {
+                const float res = s.tmp[other] > 0 ? (float)s.tmp[other] : 4.0f;
+                const float pv = cb.air_pv[wc];
+                can = (pv < -res || pv > res) ? 2u : 0u;
+            } else if (dty == 75u)
+                can = (cb.life[other] == 10 && void_admits(ty, cb.ctype[other], s.tmp[other])) ? 1u : 0u;
+            else if (dty == 23u)
+                can = void_admits(ty, cb.ctype[other], s.tmp[other]) ? 1u : 0u;
+            else if (dty == 60u)
+                can = ty == 151u ? (cb.life[other] >= 10 ? 2u : 0u) : 1u;
+            else
+                can = 1u;
+        }
+    }
+    bool ehole_exit = false;
+    if (wall == 12u && (emap_a[wc] & 0xFFu) == 0u) {
+        const int cx = tpt_f2i(s.x[i] + 0.5f) / TPT_CELL, cy = tpt_f2i(s.y[i] + 0.5f) / TPT_CELL;
+        const int here = cy * xc + cx;
+        ehole_exit = wall_a[here] == 12u && (emap_a[here] & 0xFFu) == 0u;
+    }
+    if (ehole_exit) return;
+    if (docc != 0u && can == 0u) return;
+    if (docc != 0u && can == 1u && other != i) {
+        if (dty == 23u || dty == 75u) {
+            s.ty[i] = 0u;
+        } else if (dty == 91u || dty == 40u) {
+            cb.temp[other] = tpt_clampf(cb.temp[other] + cb.temp[i] / 2.0f, 0.0f, 9999.0f);
+            s.ty[i] = 0u;
+        } else if (dty == 92u || dty == 41u) {
+            if (ty == 122u) {
+                cb.temp[other] = tpt_clampf(cb.temp[other] - (9999.0f - cb.temp[i]) / 2.0f, 0.0f, 9999.0f);
+                s.ty[i] = 0u;
+            }
+        } else if (dty == 111u) {
+            if (ty == 31u) {
+                if (cb.life[other] < 6000) cb.life[other] += 1;
+                cb.temp[other] = 0.0f;
+                s.ty[i] = 0u;
+            }
+        } else if (dty == 165u || dty == 166u) {
+            if (mprops & 0x10u) {
+                s.tmp[other] += 20;
+                s.ty[i] = 0u;
+            }
+        } else if (ty == 30u && (dprops & 0x100u)) {
+            s.ty[i] = 0u;
+        } else if (ty == 30u) {
+            const int own_x = tpt_f2i(s.x[i] + 0.5f), own_y = tpt_f2i(s.y[i] + 0.5f);
+            const int own_cell = (own_y / TPT_CELL) * xc + own_x / TPT_CELL;
+            const uint32_t own = pmap[own_y * xres + own_x];
+            const bool drag = own != 0u && (lut[(own & 0x1FFu) * 12u] & 0x80u) != 0u;
+            if (drag && wall_a[own_cell] != 15u) {
+                const uint32_t under = (own >> 9) - 1u;
+                s.x[under] = (float)nx;
+                s.y[under] = (float)ny;
+                s.x[other] = (float)own_x;
+                s.y[other] = (float)own_y;
+            }
+            s.x[i] = (float)nx;
+            s.y[i] = (float)ny;
+        } else if (ty == 42u && cb.life[i] > 0) {
+        } else {
+            const uint32_t oraw = s.intent[other], otgt = oraw & PIXEL_MASK;
+            const bool other_moves = oraw != NO_INTENT && (oraw & 0x80000000u) == 0u &&
+                                     otgt < (uint32_t)(xres * yres) && (winner[otgt] & 0x7FFFFFu) == other + 1u;
+            if (!other_moves) {
+                const float ox = s.x[other], oy = s.y[other];
+                s.x[other] = s.x[i];
+                s.y[other] = s.y[i];
+                s.x[i] = ox;
+                s.y[i] = oy;
+            }
+        }
+        return;
+    }
+    s.x[i] = (float)nx;
+    s.y[i] = (float)ny;
+}
+
+__global__ void mv_map_clear(uint4 *__restrict__ pmap, uint4 *__restrict__ photons, uint32_t n4,
+                             uint32_t *__restrict__ counters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) counters[0] = 0u;
+    if (i >= n4) return;
+    pmap[i] = make_uint4(0u, 0u, 0u, 0u);
+    photons[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+__global__ __launch_bounds__(BLK) void mv_map_scatter(const uint32_t *__restrict__ ty_a, const float *__restrict__ px,
+                                                      const float *__restrict__ py,
+                                                      const uint32_t *__restrict__ elem_props,
+                                                      uint32_t *__restrict__ pmap, uint32_t *__restrict__ photons,
+                                                      uint32_t *__restrict__ invis_list,
+                                                      uint32_t *__restrict__ counters, uint32_t slots, int xres,
+                                                      int yres) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    const uint32_t ty = ty_a[i];
+    if (ty == 0u || ty >= 256u) return;
+    const int xi = tpt_f2i(px[i] + 0.5f), yi = tpt_f2i(py[i] + 0.5f);
+    if (xi < 0 || xi >= xres || yi < 0 || yi >= yres) return;
+    const uint32_t idx = (uint32_t)(yi * xres + xi), packed = ((i + 1u) << 9) | (ty & 0x1FFu);
+    if (elem_props[ty * 32u + 1u] & 0x10u) {
+        atomicMax(&photons[idx], packed);
+    } else if (ty == 124u || ty == 134u) {
+        invis_list[atomicAdd(&counters[0], 1u)] = i;
+    } else {
+        atomicMax(&pmap[idx], packed);
+    }
+}
+
+__global__ void mv_map_invis(const uint32_t *__restrict__ ty_a, const float *__restrict__ px,
+                             const float *__restrict__ py, uint32_t *__restrict__ pmap,
+                             const uint32_t *__restrict__ invis_list, const uint32_t *__restrict__ counters, int xres) {
+    const uint32_t n = counters[0];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t i = invis_list[k];
+        const int xi = tpt_f2i(px[i] + 0.5f), yi = tpt_f2i(py[i] + 0.5f);
+        const uint32_t idx = (uint32_t)(yi * xres + xi), packed = ((i + 1u) << 9) | (ty_a[i] & 0x1FFu);
+        uint32_t cur = pmap[idx];
+        for (;;) {
+            const uint32_t t = cur & 0x1FFu;
+            if ((cur != 0u && t != 124u && t != 134u) || (cur != 0u && cur <= packed)) break;
+            const uint32_t prev = atomicCAS(&pmap[idx], cur, packed);
+            if (prev == cur) break;
+            cur = prev;
+        }
+    }
+}
+
+}  // namespace
+
+int tpt_movement_step(tpt_ctx *ctx, uint32_t seed) {
+    const tpt_config &c = ctx->cfg;
+    const uint32_t slots = ctx->slot_bound, P = ctx->pixels;
+    const int xres = (int)c.xres, yres = (int)c.yres, xc = (int)ctx->xcells;
+    Slots s{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<float>(ctx, TPT_BUF_PART_X),
+            tpt_buf<float>(ctx, TPT_BUF_PART_Y),   tpt_buf<float>(ctx, TPT_BUF_PART_VX),
+            tpt_buf<float>(ctx, TPT_BUF_PART_VY),  tpt_buf<uint32_t>(ctx, TPT_BUF_PART_FLAGS),
+            tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP), tpt_buf<uint32_t>(ctx, TPT_BUF_INTENT_TARGET)};
+    const uint32_t *wall = tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), *emap = tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP);
+    const uint32_t *lut = tpt_buf<uint32_t>(ctx, TPT_BUF_MOVE_LUT);
+    uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), *photons = tpt_buf<uint32_t>(ctx, TPT_BUF_PHOTONS);
+    uint32_t *winner = tpt_buf<uint32_t>(ctx, TPT_BUF_CELL_WINNER);
+    float *air_vx = tpt_buf<float>(ctx, TPT_BUF_AIR_VX), *air_vy = tpt_buf<float>(ctx, TPT_BUF_AIR_VY);
+    const dim3 sg((slots + BLK - 1) / BLK), cg((ctx->cells + BLK - 1) / BLK), b(BLK);
+    const uint32_t n4 = P / 4;
+    const dim3 pg((n4 + BLK - 1) / BLK);
+
+    tpt_pass_begin(ctx, PASS_AIR_DEPOSIT);
+    hipLaunchKernelGGL(mv_air_deposit, sg, b, 0, ctx->stream, s, wall, emap, ctx->cell_scratch_u, slots, xres, yres,
+                       xc);
+    hipLaunchKernelGGL(mv_air_deposit_cells, cg, b, 0, ctx->stream, s, lut, air_vx, air_vy, ctx->cell_scratch_u,
+                       ctx->cells);
+    tpt_pass_end(ctx, PASS_AIR_DEPOSIT);
+    tpt_pass_begin(ctx, PASS_MOVE_CLEAR);
+    hipLaunchKernelGGL(mv_clear, pg, b, 0, ctx->stream, (uint4 *)winner, n4, ctx->counters);
+    tpt_pass_end(ctx, PASS_MOVE_CLEAR);
+    MoveEnv e{pmap, wall, emap, lut, xres, yres, xc, c.edge_mode, c.gravity_mode, c.custom_gravity_x,
+              c.custom_gravity_y, c.newton_grav, ctx->slot_scratch_u2, ctx->counters + 1};
+    tpt_pass_begin(ctx, PASS_MOVE_INTENT);
+    hipLaunchKernelGGL(mv_intent, sg, b, 0, ctx->stream, s, e, air_vx, air_vy, winner, slots, seed);
+    hipLaunchKernelGGL(mv_saw, dim3(16), b, 0, ctx->stream, s.ty, ctx->slot_scratch_u2, ctx->counters);
+    tpt_pass_end(ctx, PASS_MOVE_INTENT);
+    CommitBufs cb{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
+                  tpt_buf<float>(ctx, TPT_BUF_PART_TEMP), tpt_buf<uint32_t>(ctx, TPT_BUF_PHOTON_MASK),
+                  tpt_buf<float>(ctx, TPT_BUF_AIR_PV)};
+    tpt_pass_begin(ctx, PASS_MOVE_COMMIT);
+    hipLaunchKernelGGL(mv_commit, sg, b, 0, ctx->stream, s, cb, pmap, wall, emap, lut, winner, slots, xres, yres, xc,
+                       c.heat_enable ? 1u : 0u);
+    tpt_pass_end(ctx, PASS_MOVE_COMMIT);
+    tpt_pass_begin(ctx, PASS_MOVE_REBUILD);
+    hipLaunchKernelGGL(mv_map_clear, pg, b, 0, ctx->stream, (uint4 *)pmap, (uint4 *)photons, n4, ctx->counters);
+    hipLaunchKernelGGL(mv_map_scatter, sg, b, 0, ctx->stream, s.ty, s.x, s.y, tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS),
+                       pmap, photons, ctx->slot_scratch_u, ctx->counters, slots, xres, yres);
+    hipLaunchKernelGGL(mv_map_invis, dim3(64), b, 0, ctx->stream, s.ty, s.x, s.y, pmap, ctx->slot_scratch_u,
+                       ctx->counters, xres);
+    tpt_pass_end(ctx, PASS_MOVE_REBUILD);
+    return tpt_check_launch("movement pass");
+}

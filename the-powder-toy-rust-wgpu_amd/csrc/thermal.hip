@@ -1,0 +1,671 @@
+// Conduction and phase transitions for gfx950:
+// thermal_kernel::step_thermal_and_transitions_gpu
+// (crates/tpt-sim/src/thermal_kernel.rs:1574-1717).
+//
+// The reference's non-atomic read-modify-writes on shared cells and
+// particles are replaced by deterministic two-step forms (a slot pass that
+// records per-slot results plus a per-cell pass that resolves them):
+//   R2  air_hv: every gated particle reads the pre-pass value; the cell keeps
+//       the write of its highest gated slot (atomicMax on slot+1).
+//   R3  block_airh: gate failures are counted per cell (atomicAdd) and the
+//       saturating "+1 while bit 3 is clear" is applied n times at once.
+//   R5  air_pv / wall_emap: transitions read the pre-pass grids; the cell
+//       keeps the pv of its highest changed slot; spark charges land after.
+//   R6  liquid convection: one round of a priority matching over the
+//       proposals (i -> partner) built from pre-pass temperatures: a swap
+//       happens iff i is the lowest slot of every proposal touching i or its
+//       partner, so the swaps executed are disjoint.
+//
+// Kernels: th_conduct, th_conduct_cells, th_apply, th_convect_propose,
+// th_convect_touch, th_convect_swap, th_transition, th_transition_cells,
+// th_refresh.
+
+#include "tpt_internal.h"
+
+namespace {
+
+constexpr int BLK = 256;
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint32_t PARTICLE_HASH_SALT = 2654435761u;
+constexpr uint32_t TRANSITION_HASH_SALT = 0x51ED2701u;
+
+enum : uint32_t {
+    T_WATR = 2, T_FIRE = 4, T_LAVA = 6, T_STNE = 5, T_SPRK = 12, T_SNOW = 13, T_ICEI = 14, T_SAND = 17,
+    T_SALT = 18, T_SLTW = 19, T_ACID = 20, T_WTRV = 21, T_GLAS = 24, T_PLSM = 28, T_PHOT = 29, T_ELEC = 31,
+    T_DSTW = 35, T_LN2 = 38, T_BMTL = 46, T_BRMT = 47, T_HSWC = 68, T_RIME = 79, T_FOG = 80, T_PIPE = 82,
+    T_FRZW = 84, T_BIZR = 85, T_BIZRG = 86, T_PLUT = 89, T_BGLA = 95, T_THRM = 99, T_CFLM = 100, T_LIFE = 105,
+    T_DEUT = 111, T_FILT = 134, T_BRAY = 136, T_QRTZ = 141, T_PQRT = 142, T_GEL = 150, T_PPIP = 162,
+    T_VIBR = 165, T_BVBR = 166, T_TUNG = 171, T_CRMC = 178, T_HEAC = 179, T_LITH = 190, T_BASE = 193,
+    T_SEED = 194, T_BISM = 239
+};
+enum : int {
+    COL_PROPERTIES = 1, COL_FLAMMABLE = 12, COL_EXPLOSIVE = 13, COL_HEAT_CONDUCT = 15, COL_DEFAULT_TEMP = 16,
+    COL_LOW_PRESSURE = 18, COL_LOW_PRESSURE_TRANSITION = 19, COL_HIGH_PRESSURE = 20,
+    COL_HIGH_PRESSURE_TRANSITION = 21, COL_LOW_TEMP = 22, COL_LOW_TEMP_TRANSITION = 23, COL_HIGH_TEMP = 24,
+    COL_HIGH_TEMP_TRANSITION = 25
+};
+constexpr int32_t NT = -1, ST = 512;
+
+struct Parts {
+    uint32_t *ty;
+    int32_t *life;
+    uint32_t *ctype;
+    float *x, *y, *temp, *equil;
+    int32_t *tmp, *tmp2;
+};
+
+struct Dims {
+    int xres, yres, xc, yc;
+    uint32_t slots;
+};
+
+__device__ __forceinline__ uint32_t gpu_hash(uint32_t seed) {
+    uint32_t h = seed ^ 0x9E3779B9u;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    return h ^ (h >> 16);
+}
+
+__device__ __forceinline__ uint32_t EP(const uint32_t *__restrict__ p, uint32_t t, int col) { return p[t * 32u + col]; }
+__device__ __forceinline__ float EPF(const uint32_t *__restrict__ p, uint32_t t, int col) {
+    return __uint_as_float(p[t * 32u + col]);
+}
+
+__device__ __forceinline__ bool is_heat_insulator(uint32_t ty, int32_t life, int32_t tmp, uint32_t hc) {
+    return hc == 0u || (ty == T_HSWC && life != 10) || ((ty == T_PIPE || ty == T_PPIP) && (tmp & 1) == 0);
+}
+
+__device__ __forceinline__ float heat_capacity(uint32_t ty, uint32_t ctype) {
+    float hc = 1.0f;
+    if (ty == T_BASE) hc = 1.5f;
+    if ((ty == T_PIPE || ty == T_PPIP) && ctype > 0u && ctype < 256u) hc += ctype == T_BASE ? 1.5f : 1.0f;
+    return hc;
+}
+
+__device__ __forceinline__ bool pair_excluded(uint32_t t, uint32_t rt, int32_t st, int32_t rtt) {
+    return (t == T_FILT && (rt == T_BRAY || rt == T_BIZR || rt == T_BIZRG)) ||
+           (rt == T_FILT && (t == T_BRAY || t == T_PHOT || t == T_BIZR || t == T_BIZRG)) ||
+           (t == T_ELEC && rt == T_DEUT) || (t == T_DEUT && rt == T_ELEC) ||
+           (t == T_HSWC && rt == T_FILT && st == 1) || (t == T_FILT && rt == T_HSWC && rtt == 1);
+}
+
+__device__ __forceinline__ uint32_t slot_at(const uint32_t *__restrict__ pmap, int x, int y, Dims d) {
+    if (x < 0 || y < 0 || x >= d.xres || y >= d.yres) return NONE;
+    const uint32_t packed = pmap[y * d.xres + x];
+    return (packed >> 9) ? (packed >> 9) - 1u : NONE;
+}
+
+// thermal_conduction_kernel (thermal_kernel.rs:459-605), rules R2/R3.
+__global__ __launch_bounds__(BLK) void th_conduct(Parts p, const uint32_t *__restrict__ pmap,
+                                                  const float *__restrict__ air_hv,
+                                                  const uint32_t *__restrict__ props, float *__restrict__ hv_new,
+                                                  uint32_t *__restrict__ hv_key, uint32_t *__restrict__ fails,
+                                                  Dims d, uint32_t aheat, uint32_t seed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.slots) return;
+    const uint32_t ty = p.ty[i];
+    if (ty == 0u || ty >= 256u) return;
+    const uint32_t hcond = EP(props, ty, COL_HEAT_CONDUCT);
+    const int x = tpt_f2i(p.x[i]), y = tpt_f2i(p.y[i]);
+    if (x < 0 || x >= d.xres || y < 0 || y >= d.yres) return;
+    const int ci = (y / TPT_CELL) * d.xc + x / TPT_CELL;
+    const int32_t life = p.life[i], tmp = p.tmp[i];
+    const uint32_t ctype = p.ctype[i];
+    float gel = 1.0f;
+    if (ty == T_GEL) gel = (float)tmp * 2.55f;
+    float scaled = (float)hcond * gel;
+    if (scaled < 0.0f) scaled = 0.0f;
+    const uint32_t threshold = tpt_f2u(scaled);
+    const uint32_t roll = gpu_hash(seed ^ (i * PARTICLE_HASH_SALT) ^ ty) % 250u;
+    if (!is_heat_insulator(ty, life, tmp, hcond) && roll < threshold) {
+        const float t0 = p.temp[i];
+        float self_temp = t0;
+        if (aheat && (EP(props, ty, COL_PROPERTIES) & 0x00200000u) == 0u) {
+            const float hv = air_hv[ci];
+            const float dtemp = hv - t0;
+            const float hc = heat_capacity(ty, ctype);
+            float alpha = 0.4f * hc;
+            if (alpha > 0.04f) alpha = 0.04f;
+            self_temp = tpt_clampf(t0 + alpha * dtemp / hc, 0.0f, 9999.0f);
+            hv_new[i] = tpt_clampf(hv - alpha * dtemp, 0.0f, 9999.0f);
+            atomicMax(&hv_key[ci], i + 1u);
+        }
+        const float hc_self = heat_capacity(ty, ctype);
+        float c_heat = self_temp * hc_self, hc_total = hc_self;
+        for (int ny = 0; ny < 3; ++ny)
+            for (int nx = 0; nx < 3; ++nx) {
+                if (nx == 1 && ny == 1) continue;
+                const uint32_t nid = slot_at(pmap, x + nx - 1, y + ny - 1, d);
+                if (nid >= d.slots) continue;
+                const uint32_t nty = p.ty[nid];
+                if (nty == 0u || nty >= 256u) continue;
+                const int32_t ntmp = p.tmp[nid];
+                if (is_heat_insulator(nty, p.life[nid], ntmp, EP(props, nty, COL_HEAT_CONDUCT))) continue;
+                if (pair_excluded(ty, nty, tmp, ntmp)) continue;
+                const float n_hc = heat_capacity(nty, p.ctype[nid]);
+                c_heat += p.temp[nid] * n_hc;
+                hc_total += n_hc;
+            }
+        p.equil[i] = tpt_clampf(c_heat / hc_total, 0.0f, 9999.0f);
+    } else {
+        atomicAdd(&fails[ci], 1u);
+        p.equil[i] = -1.0f;
+    }
+}
+
+__global__ void th_conduct_cells(float *__restrict__ air_hv, uint32_t *__restrict__ block_airh,
+                                 const float *__restrict__ hv_new, uint32_t *__restrict__ hv_key,
+                                 uint32_t *__restrict__ fails, uint32_t cells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const uint32_t key = hv_key[c];
+    if (key) {
+        air_hv[c] = hv_new[key - 1u];
+        hv_key[c] = 0u;
+    }
+    const uint32_t n = fails[c];
+    if (n) {
+        uint32_t v = block_airh[c];
+        if ((v & 8u) == 0u) {
+            const uint32_t room = 8u - (v & 7u);
+            v += n < room ? n : room;
+            block_airh[c] = v;
+        }
+        fails[c] = 0u;
+    }
+}
+
+// thermal_apply_kernel (thermal_kernel.rs:616-708).
+__global__ __launch_bounds__(BLK) void th_apply(Parts p, const uint32_t *__restrict__ pmap,
+                                                const uint32_t *__restrict__ props, Dims d) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.slots) return;
+    const uint32_t jty = p.ty[j];
+    if (jty == 0u || jty >= 256u) return;
+    const int x = tpt_f2i(p.x[j]), y = tpt_f2i(p.y[j]);
+    if (x < 0 || x >= d.xres || y < 0 || y >= d.yres) return;
+    uint32_t best = j;
+    const float own = p.temp[j];
+    float best_temp = own;
+    bool has = false;
+    const float eq = p.equil[j];
+    if (eq >= 0.0f) {
+        best_temp = eq;
+        has = true;
+    }
+    const int32_t jtmp = p.tmp[j];
+    const bool j_ins = is_heat_insulator(jty, p.life[j], jtmp, EP(props, jty, COL_HEAT_CONDUCT));
+    if (!j_ins) {
+        for (int ny = 0; ny < 3; ++ny)
+            for (int nx = 0; nx < 3; ++nx) {
+                if (nx == 1 && ny == 1) continue;
+                const uint32_t nid = slot_at(pmap, x + nx - 1, y + ny - 1, d);
+                if (nid >= d.slots || nid == j) continue;
+                const uint32_t nty = p.ty[nid];
+                if (nty == 0u || nty >= 256u || !(!has || nid > best)) continue;
+                const float neq = p.equil[nid];
+                if (!(neq >= 0.0f)) continue;
+                const int32_t ntmp = p.tmp[nid];
+                if (is_heat_insulator(nty, p.life[nid], ntmp, EP(props, nty, COL_HEAT_CONDUCT))) continue;
+                if (pair_excluded(nty, jty, ntmp, jtmp)) continue;
+                best = nid;
+                best_temp = neq;
+                has = true;
+            }
+    }
+    p.temp[j] = tpt_clampf(has ? best_temp : own, 0.0f, 9999.0f);
+}
+
+struct GravCfg {
+    uint32_t mode;
+    float cgx, cgy;
+};
+
+// The liquid's convection partner (thermal_kernel.rs:755-820) from the
+// pre-pass state, or NONE.
+__device__ uint32_t convect_target(const Parts &p, const uint32_t *__restrict__ pmap,
+                                   const uint32_t *__restrict__ props, const float *__restrict__ gfx,
+                                   const float *__restrict__ gfy, Dims d, GravCfg g, uint32_t i, uint32_t seed) {
+    const uint32_t t = p.ty[i];
+    if (t == 0u || t >= 256u) return NONE;
+    const int x = tpt_f2i(p.x[i]), y = tpt_f2i(p.y[i]);
+    if (x < 0 || x >= d.xres || y < 0 || y >= d.yres) return NONE;
+    if ((EP(props, t, COL_PROPERTIES) & 0x02u) == 0u) return NONE;
+    if (t == T_GEL) {
+        const uint32_t roll = gpu_hash(seed ^ (i * PARTICLE_HASH_SALT) ^ t ^ TRANSITION_HASH_SALT);
+        const float gel = (float)p.tmp[i] * 2.55f;
+        const uint32_t gate_roll = gpu_hash(roll ^ 0x1B873593u) % 255u;
+        if (gel <= 1.0f + (float)gate_roll) return NONE;
+    }
+    const int ci = (y / TPT_CELL) * d.xc + x / TPT_CELL;
+    const float pg = -2.0f;
+    float gx = 0.0f, gy = 0.0f;
+    if (g.mode == 0u) {
+        gy = pg;
+    } else if (g.mode == 2u) {
+        const float dx = (float)x - (float)d.xres * 0.5f, dy = (float)y - (float)d.yres * 0.5f;
+        if (dx != 0.0f || dy != 0.0f) {
+            const float dd = 0.01f - sqrtf(dx * dx + dy * dy);
+            gx = pg * (dx / dd);
+            gy = pg * (dy / dd);
+        }
+    } else if (g.mode == 3u) {
+        gx = pg * g.cgx;
+        gy = pg * g.cgy;
+    }
+    gx += -2.0f * gfx[ci];
+    gy += -2.0f * gfy[ci];
+    const float rxf = gx + (float)x, ryf = gy + (float)y;
+    const int rx = rxf >= 0.0f ? tpt_f2i(rxf + 0.5f) : tpt_f2i(rxf - 0.5f);
+    const int ry = ryf >= 0.0f ? tpt_f2i(ryf + 0.5f) : tpt_f2i(ryf - 0.5f);
+    const int ox = min(max(rx, x - 1), x + 1), oy = min(max(ry, y - 1), y + 1);
+    if (ox == x && oy == y) return NONE;
+    const uint32_t nid = slot_at(pmap, ox, oy, d);
+    if (nid >= d.slots || p.ty[nid] != t || !(p.temp[i] > p.temp[nid])) return NONE;
+    return nid;
+}
+
+__global__ __launch_bounds__(BLK) void th_convect_propose(Parts p, const uint32_t *__restrict__ pmap,
+                                                          const uint32_t *__restrict__ props,
+                                                          const float *__restrict__ gfx,
+                                                          const float *__restrict__ gfy, uint32_t *__restrict__ tgt,
+                                                          uint32_t *__restrict__ touch, Dims d, GravCfg g,
+                                                          uint32_t seed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.slots) return;
+    tgt[i] = convect_target(p, pmap, props, gfx, gfy, d, g, i, seed);
+    touch[i] = NONE;
+}
+
+__global__ __launch_bounds__(BLK) void th_convect_touch(const uint32_t *__restrict__ tgt, uint32_t *__restrict__ touch,
+                                                        uint32_t slots) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    const uint32_t t = tgt[i];
+    if (t == NONE) return;
+    atomicMin(&touch[i], i);
+    atomicMin(&touch[t], i);
+}
+
+__global__ __launch_bounds__(BLK) void th_convect_swap(const uint32_t *__restrict__ tgt,
+                                                       const uint32_t *__restrict__ touch, float *__restrict__ temp,
+                                                       uint32_t slots) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    const uint32_t t = tgt[i];
+    if (t == NONE || touch[i] != i || touch[t] != i) return;
+    const float hot = temp[i];
+    temp[i] = temp[t];
+    temp[t] = hot;
+}
+
+__device__ float pres_sum(const float *__restrict__ pv, int x, int y, Dims d) {
+    const int cy = y / TPT_CELL, cx = x / TPT_CELL;
+    const int ym = max(y - 2, 0) / TPT_CELL, xm = max(x - 2, 0) / TPT_CELL;
+    const int yp = min(y + 2, d.yres - 1) / TPT_CELL, xp = min(x + 2, d.xres - 1) / TPT_CELL;
+    const float v = (pv[cy * d.xc + cx] + pv[ym * d.xc + cx] + pv[yp * d.xc + cx] + pv[cy * d.xc + xm] +
+                     pv[cy * d.xc + xp]) *
+                    2.0f;
+    return v > 0.0f ? v : 0.0f;
+}
+
+// phase_transition_kernel after the convection swap (thermal_kernel.rs:822-1315), rule R5.
+__global__ __launch_bounds__(BLK) void th_transition(Parts p, const float *__restrict__ air_pv,
+                                                     const float *__restrict__ gfx, const float *__restrict__ gfy,
+                                                     const uint32_t *__restrict__ wall_bmap,
+                                                     const uint32_t *__restrict__ wall_emap,
+                                                     const uint32_t *__restrict__ props, float *__restrict__ pv_new,
+                                                     uint32_t *__restrict__ pv_key, uint32_t *__restrict__ spark,
+                                                     Dims d, uint32_t legacy, uint32_t seed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.slots) return;
+    uint32_t t = p.ty[i];
+    if (t == 0u || t >= 256u) return;
+    const int x = tpt_f2i(p.x[i]), y = tpt_f2i(p.y[i]);
+    if (x < 0 || x >= d.xres || y < 0 || y >= d.yres) return;
+    const int ci = (y / TPT_CELL) * d.xc + x / TPT_CELL;
+    const float pv0 = air_pv[ci];
+    float pv = pv0;
+    const uint32_t roll = gpu_hash(seed ^ (i * PARTICLE_HASH_SALT) ^ t ^ TRANSITION_HASH_SALT);
+    if (!legacy) {
+        const uint32_t tprops0 = EP(props, t, COL_PROPERTIES);
+        uint32_t ctype = p.ctype[i];
+        const int32_t tmp = p.tmp[i], tmp2 = p.tmp2[i];
+        const float eq = p.equil[i];
+        if (eq >= 0.0f) {
+            const float pt = eq;
+            if ((t == T_ICEI || t == T_SNOW) && (ctype == 0u || ctype >= 256u || ctype == T_ICEI || ctype == T_SNOW)) {
+                ctype = T_WATR;
+                p.ctype[i] = ctype;
+            }
+            const float high_temp = EPF(props, t, COL_HIGH_TEMP);
+            const int32_t high_tr = (int32_t)EP(props, t, COL_HIGH_TEMP_TRANSITION);
+            const float low_temp = EPF(props, t, COL_LOW_TEMP);
+            const int32_t low_tr = (int32_t)EP(props, t, COL_LOW_TEMP_TRANSITION);
+            const float pv_shift = 2.0f * pv;
+            bool boil = (tprops0 & 0x02u) && high_tr >= 0 && high_tr < 256 &&
+                        (EP(props, (uint32_t)high_tr, COL_PROPERTIES) & 0x08u);
+            if (t == T_LN2 || t == T_SLTW) boil = true;
+            bool cond = (tprops0 & 0x08u) && low_tr >= 0 && low_tr < 256 &&
+                        (EP(props, (uint32_t)low_tr, COL_PROPERTIES) & 0x02u);
+            if (t == T_WTRV) cond = true;
+            const float ctemph = boil ? pt - pv_shift : pt;
+            const float ctempl = cond ? pt - pv_shift : pt;
+            uint32_t old_ty = t, nt = t, new_ctype = ctype;
+            int32_t new_life = p.life[i], new_tmp = tmp, new_tmp2 = tmp2;
+            bool s = false;
+            if (high_tr != NT && ctemph >= high_temp) {
+                if (high_tr != ST) {
+                    if (t == T_FOG) new_ctype = 0u;
+                    nt = (uint32_t)high_tr;
+                    s = true;
+                } else if (t == T_ICEI || t == T_SNOW) {
+                    bool melts = false;
+                    if (new_ctype > 0u && new_ctype < 256u && new_ctype != t) {
+                        const int32_t ct_tr = (int32_t)EP(props, new_ctype, COL_LOW_TEMP_TRANSITION);
+                        const float ct_low = EPF(props, new_ctype, COL_LOW_TEMP);
+                        if (ct_tr == (int32_t)T_ICEI || ct_tr == (int32_t)T_SNOW) {
+                            if (pt >= ct_low) melts = true;
+                        } else if (pt >= 273.15f) {
+                            melts = true;
+                        }
+                    }
+                    if (melts) {
+                        nt = new_ctype;
+                        new_ctype = 0u;
+                        new_life = 0;
+                        s = true;
+                    }
+                } else if (t == T_SLTW) {
+                    nt = (roll % 4u == 0u) ? T_SALT : T_WTRV;
+                    s = true;
+                } else if (t == T_BRMT) {
+                    if (new_ctype == T_TUNG) {
+                        if (ctemph >= EPF(props, T_TUNG, COL_HIGH_TEMP)) {
+                            nt = T_LAVA;
+                            old_ty = T_TUNG;
+                            s = true;
+                        }
+                    } else if (ctemph >= high_temp) {
+                        nt = T_LAVA;
+                        s = true;
+                    }
+                } else if (t == T_CRMC) {
+                    if (ctemph >= pres_sum(air_pv, x, y, d) + high_temp) {
+                        nt = T_LAVA;
+                        s = true;
+                    }
+                } else if (t == T_RIME) {
+                    if (new_tmp > 5) {
+                        nt = T_ACID;
+                        new_life = 25 + 5 * new_tmp;
+                        new_tmp = 0;
+                    } else {
+                        nt = new_ctype == T_DSTW ? T_DSTW : T_WATR;
+                        new_ctype = 0u;
+                    }
+                    s = true;
+                }
+            } else if (low_tr != NT && ctempl < low_temp) {
+                if (low_tr != ST) {
+                    nt = (uint32_t)low_tr;
+                    s = true;
+                } else if (t == T_WTRV) {
+                    nt = pt < 273.0f ? T_RIME : T_DSTW;
+                    s = true;
+                } else if (t == T_LAVA) {
+                    if (new_ctype > 0u && new_ctype < 256u && new_ctype != T_LAVA) {
+                        const float c_high = EPF(props, new_ctype, COL_HIGH_TEMP);
+                        const int32_t c_high_tr = (int32_t)EP(props, new_ctype, COL_HIGH_TEMP_TRANSITION);
+                        bool freeze = false;
+                        if (new_ctype == T_THRM) {
+                            freeze = pt < EPF(props, T_BMTL, COL_HIGH_TEMP);
+                        } else if (new_ctype == T_VIBR || new_ctype == T_BVBR) {
+                            freeze = pt < 273.15f;
+                        } else if (new_ctype == T_TUNG) {
+                            freeze = pt < c_high;
+                        } else if (new_ctype == T_CRMC) {
+                            freeze = !(ctemph >= pres_sum(air_pv, x, y, d) + c_high);
+                        } else if (c_high_tr == (int32_t)T_LAVA || new_ctype == T_HEAC) {
+                            freeze = pt < c_high && new_ctype != T_BISM;
+                        } else {
+                            freeze = pt < 973.0f;
+                        }
+                        if (freeze) {
+                            if (new_ctype == T_THRM) {
+                                nt = T_BMTL;
+                                new_tmp = 0;
+                            } else if (new_ctype == T_PLUT) {
+                                nt = T_LAVA;
+                                new_tmp = 0;
+                            } else {
+                                nt = new_ctype;
+                            }
+                            new_ctype = 0u;
+                            s = true;
+                        }
+                    } else if (pt < 973.0f) {
+                        nt = T_STNE;
+                        s = true;
+                    }
+                }
+            }
+            if (s) {
+                if (nt == T_ICEI || nt == T_LAVA || nt == T_SNOW) new_ctype = old_ty;
+                if (nt == T_RIME) new_ctype = T_DSTW;
+                if (!(nt == T_ICEI && new_ctype == T_FRZW) && nt != T_ACID) new_life = 0;
+                if (nt == T_FIRE) {
+                    new_tmp = 0;
+                    if (old_ty == T_SEED) {
+                        new_ctype = 0u;
+                        new_tmp2 = 0;
+                    }
+                }
+                if ((EP(props, nt, COL_PROPERTIES) & 0x08u) && !(EP(props, old_ty, COL_PROPERTIES) & 0x08u))
+                    pv += 0.50f;
+                if (nt == 0u) {
+                    p.ty[i] = 0u;
+                    t = 0u;
+                } else {
+                    p.ty[i] = nt;
+                    t = nt;
+                    if (nt == T_FIRE || nt == T_PLSM || nt == T_CFLM) new_life = 120 + (int32_t)(roll % 50u);
+                    if (nt == T_LAVA) {
+                        if (new_ctype == T_BRMT) new_ctype = T_BMTL;
+                        else if (new_ctype == T_SAND || new_ctype == T_BGLA || (new_ctype == T_LITH && new_tmp2 > 3))
+                            new_ctype = T_GLAS;
+                        else if (new_ctype == T_PQRT) new_ctype = T_QRTZ;
+                        new_life = 240 + (int32_t)(roll % 120u);
+                    }
+                }
+                p.ctype[i] = new_ctype;
+                p.life[i] = new_life;
+                p.tmp[i] = new_tmp;
+                p.tmp2[i] = new_tmp2;
+            }
+            float pt2 = tpt_clampf(pt, 0.0f, 9999.0f);
+            if (t == T_LAVA) {
+                p.life[i] = min(max(tpt_f2i((pt2 - 700.0f) / 7.0f), 0), 400);
+                if (p.ctype[i] == T_THRM && p.tmp[i] > 0) {
+                    p.tmp[i] -= 1;
+                    pt2 = 3500.0f;
+                }
+                if (p.ctype[i] == T_PLUT && p.tmp[i] > 0) {
+                    p.tmp[i] -= 1;
+                    pt2 = 9999.0f;
+                }
+                p.temp[i] = pt2;
+            }
+        } else {
+            p.temp[i] = tpt_clampf(p.temp[i], 0.0f, 9999.0f);
+        }
+    }
+    if (t != 0u && t < 256u) {
+        if (t == T_LIFE) p.temp[i] = tpt_clampf(p.temp[i] - 50.0f, 0.0f, 9999.0f);
+        const uint32_t tprops = EP(props, t, COL_PROPERTIES);
+        if ((tprops & 0x20u) || t == T_SPRK) {
+            int nx = x % TPT_CELL, ny = y % TPT_CELL;
+            nx = nx == 0 ? x / TPT_CELL - 1 : (nx == TPT_CELL - 1 ? x / TPT_CELL + 1 : x / TPT_CELL);
+            ny = ny == 0 ? y / TPT_CELL - 1 : (ny == TPT_CELL - 1 ? y / TPT_CELL + 1 : y / TPT_CELL);
+            if (nx >= 0 && ny >= 0 && nx < d.xc && ny < d.yc) {
+                const int wi = ny * d.xc + nx;
+                if (t != T_SPRK) {
+                    if ((wall_emap[wi] & 0xFFu) == 12u && p.life[i] == 0 && (wall_bmap[wi] & 0xFFu) != 18u) {
+                        p.ty[i] = T_SPRK;
+                        p.life[i] = 4;
+                        p.ctype[i] = t;
+                        t = T_SPRK;
+                    }
+                } else {
+                    const uint32_t wall = wall_bmap[wi] & 0xFFu;
+                    // WL_DETECT, WL_EWALL, WL_ALLOWLIQUID, WL_WALLELEC, WL_ALLOWALLELEC, WL_EHOLE
+                    if (wall == 3u || wall == 2u || wall == 6u || wall == 1u || wall == 11u || wall == 12u)
+                        spark[wi] = 1u;
+                }
+            }
+        }
+        const int32_t explosive = (int32_t)EP(props, t, COL_EXPLOSIVE);
+        if ((explosive & 2) && pv > 2.5f) {
+            const float fire_temp = EPF(props, T_FIRE, COL_DEFAULT_TEMP);
+            const int32_t flam = (int32_t)EP(props, t, COL_FLAMMABLE);
+            p.temp[i] = tpt_clampf(fire_temp + (float)(flam / 2), 0.0f, 9999.0f);
+            p.life[i] = 180 + (int32_t)(roll % 80u);
+            p.ty[i] = T_FIRE;
+            t = T_FIRE;
+            pv += 0.25f;
+        }
+        bool s = false;
+        uint32_t nt = t;
+        const float gx = gfx[ci], gy = gfy[ci];
+        const float gravtot = (gx < 0.0f ? 0.0f - gx : gx) + (gy < 0.0f ? 0.0f - gy : gy);
+        const int32_t hpt = (int32_t)EP(props, t, COL_HIGH_PRESSURE_TRANSITION);
+        const float hpv = EPF(props, t, COL_HIGH_PRESSURE);
+        const int32_t lpt = (int32_t)EP(props, t, COL_LOW_PRESSURE_TRANSITION);
+        const float lpv = EPF(props, t, COL_LOW_PRESSURE);
+        const int32_t tmp = p.tmp[i];
+        if (hpt != NT && pv > hpv) {
+            if (hpt != ST) {
+                nt = (uint32_t)hpt;
+                s = true;
+            } else if (t == T_BMTL && (pv > 2.5f || (pv > 1.0f && tmp == 1))) {
+                nt = T_BRMT;
+                s = true;
+            }
+        } else if (lpt != NT && pv < lpv && gravtot <= lpv / 4.0f) {
+            if (lpt != ST) {
+                nt = (uint32_t)lpt;
+                s = true;
+            }
+        } else if (hpt != NT && gravtot > hpv / 4.0f) {
+            if (hpt != ST) {
+                nt = (uint32_t)hpt;
+                s = true;
+            } else if (t == T_BMTL && (gravtot > 0.625f || (gravtot > 0.25f && tmp == 1))) {
+                nt = T_BRMT;
+                s = true;
+            }
+        }
+        if (s) {
+            if (nt == 0u) {
+                p.ty[i] = 0u;
+            } else {
+                p.life[i] = 0;
+                if (nt == T_BRMT) {
+                    p.ctype[i] = 0u;
+                    p.tmp[i] = 0;
+                    p.tmp2[i] = 0;
+                }
+                p.ty[i] = nt;
+                if (nt == T_FIRE) p.life[i] = 120 + (int32_t)(roll % 50u);
+            }
+        }
+    }
+    if (pv != pv0) {
+        pv_new[i] = pv;
+        atomicMax(&pv_key[ci], i + 1u);
+    }
+}
+
+__global__ void th_transition_cells(float *__restrict__ air_pv, uint32_t *__restrict__ wall_emap,
+                                    const float *__restrict__ pv_new, uint32_t *__restrict__ pv_key,
+                                    uint32_t *__restrict__ spark, uint32_t cells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const uint32_t key = pv_key[c];
+    if (key) {
+        air_pv[c] = pv_new[key - 1u];
+        pv_key[c] = 0u;
+    }
+    if (spark[c]) {
+        wall_emap[c] = 16u;
+        spark[c] = 0u;
+    }
+}
+
+// refresh_pmap_types_kernel (thermal_kernel.rs:1338-1387), pmap half.
+__global__ __launch_bounds__(BLK) void th_refresh(const uint32_t *__restrict__ ty_a, const float *__restrict__ px,
+                                                  const float *__restrict__ py, uint32_t *__restrict__ pmap, Dims d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.slots) return;
+    const uint32_t ty = ty_a[i];
+    if (ty == 0u || ty >= 256u) return;
+    const int x = tpt_f2i(px[i]), y = tpt_f2i(py[i]);
+    if (x < 0 || x >= d.xres || y < 0 || y >= d.yres) return;
+    const int idx = y * d.xres + x;
+    const uint32_t packed = pmap[idx];
+    if ((packed >> 9) == i + 1u && (packed & 0x1FFu) != ty) pmap[idx] = ((i + 1u) << 9) | (ty & 0x1FFu);
+}
+
+}  // namespace
+
+int tpt_thermal_step(tpt_ctx *ctx, uint32_t seed, int legacy) {
+    Parts p{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
+            tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE), tpt_buf<float>(ctx, TPT_BUF_PART_X),
+            tpt_buf<float>(ctx, TPT_BUF_PART_Y), tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),
+            tpt_buf<float>(ctx, TPT_BUF_PART_EQUIL), tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP),
+            tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP2)};
+    const Dims d{(int)ctx->cfg.xres, (int)ctx->cfg.yres, (int)ctx->xcells, (int)ctx->ycells, ctx->slot_bound};
+    const dim3 sg((d.slots + BLK - 1) / BLK), cg((ctx->cells + BLK - 1) / BLK), b(BLK);
+    const uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), *props = tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS);
+    const float *gfx = tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), *gfy = tpt_buf<float>(ctx, TPT_BUF_GRAV_FY);
+    if (!legacy) {
+        tpt_pass_begin(ctx, PASS_CONDUCT);
+        hipLaunchKernelGGL(th_conduct, sg, b, 0, ctx->stream, p, pmap, tpt_buf<float>(ctx, TPT_BUF_AIR_HV), props,
+                           ctx->slot_scratch_f, ctx->cell_scratch_u, ctx->cell_scratch_u2, d,
+                           ctx->cfg.aheat_enable ? 1u : 0u, seed);
+        tpt_pass_end(ctx, PASS_CONDUCT);
+        tpt_pass_begin(ctx, PASS_CONDUCT_CELL);
+        hipLaunchKernelGGL(th_conduct_cells, cg, b, 0, ctx->stream, tpt_buf<float>(ctx, TPT_BUF_AIR_HV),
+                           tpt_buf<uint32_t>(ctx, TPT_BUF_BLOCK_AIRH), ctx->slot_scratch_f, ctx->cell_scratch_u,
+                           ctx->cell_scratch_u2, ctx->cells);
+        tpt_pass_end(ctx, PASS_CONDUCT_CELL);
+        tpt_pass_begin(ctx, PASS_CONDUCT_APPLY);
+        hipLaunchKernelGGL(th_apply, sg, b, 0, ctx->stream, p, pmap, props, d);
+        tpt_pass_end(ctx, PASS_CONDUCT_APPLY);
+    }
+    tpt_pass_begin(ctx, PASS_TRANSITION);
+    if (!legacy) {
+        const GravCfg g{ctx->cfg.gravity_mode, ctx->cfg.custom_gravity_x, ctx->cfg.custom_gravity_y};
+        hipLaunchKernelGGL(th_convect_propose, sg, b, 0, ctx->stream, p, pmap, props, gfx, gfy, ctx->slot_scratch_u,
+                           ctx->slot_scratch_u2, d, g, seed);
+        hipLaunchKernelGGL(th_convect_touch, sg, b, 0, ctx->stream, ctx->slot_scratch_u, ctx->slot_scratch_u2,
+                           d.slots);
+        hipLaunchKernelGGL(th_convect_swap, sg, b, 0, ctx->stream, ctx->slot_scratch_u, ctx->slot_scratch_u2, p.temp,
+                           d.slots);
+    }
+    hipLaunchKernelGGL(th_transition, sg, b, 0, ctx->stream, p, tpt_buf<float>(ctx, TPT_BUF_AIR_PV), gfx, gfy,
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), props,
+                       ctx->slot_scratch_f, ctx->cell_scratch_u, ctx->cell_scratch_u2, d, legacy ? 1u : 0u, seed);
+    tpt_pass_end(ctx, PASS_TRANSITION);
+    tpt_pass_begin(ctx, PASS_TRANSITION_CELL);
+    hipLaunchKernelGGL(th_transition_cells, cg, b, 0, ctx->stream, tpt_buf<float>(ctx, TPT_BUF_AIR_PV),
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), ctx->slot_scratch_f, ctx->cell_scratch_u,
+                       ctx->cell_scratch_u2, ctx->cells);
+    tpt_pass_end(ctx, PASS_TRANSITION_CELL);
+    tpt_pass_begin(ctx, PASS_REFRESH_PMAP);
+    hipLaunchKernelGGL(th_refresh, sg, b, 0, ctx->stream, p.ty, p.x, p.y, tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), d);
+    tpt_pass_end(ctx, PASS_REFRESH_PMAP);
+    return tpt_check_launch("thermal pass");
+}

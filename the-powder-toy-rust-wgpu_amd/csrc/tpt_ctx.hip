@@ -78,6 +78,7 @@ static int alloc_buffers(tpt_ctx *c) {
         else if (id <= TPT_BUF_GRAV_FY) words = c->cells;
         else if (id == TPT_BUF_ELEM_PROPS) words = (size_t)TPT_PT_NUM * TPT_ELEM_STRIDE;
         else if (id == TPT_BUF_MOVE_LUT) words = TPT_MOVE_LUT_WORDS;
+        else if (id == TPT_BUF_PHOTON_MASK) words = TPT_PT_NUM;
         c->buf_bytes[id] = words * 4;
         if (hipMalloc(&c->buf[id], words * 4) != hipSuccess)
             return tpt_set_error(TPT_ENOMEM, "hipMalloc of buffer %d (%zu bytes) failed", id, words * 4);
@@ -90,7 +91,8 @@ static int alloc_buffers(tpt_ctx *c) {
     size_t cb = (size_t)c->cells * 4;
     if (hipMalloc(&c->cell_scratch_u, cb) != hipSuccess || hipMalloc(&c->cell_scratch_f, cb) != hipSuccess ||
         hipMalloc(&c->cell_scratch_u2, cb) != hipSuccess || hipMalloc(&c->cell_scratch_f2, cb) != hipSuccess ||
-        hipMalloc(&c->slot_scratch_u, cap * 4) != hipSuccess || hipMalloc(&c->counters, 256) != hipSuccess)
+        hipMalloc(&c->slot_scratch_u, cap * 4) != hipSuccess || hipMalloc(&c->slot_scratch_u2, cap * 4) != hipSuccess ||
+        hipMalloc(&c->slot_scratch_f, cap * 4) != hipSuccess || hipMalloc(&c->counters, 256) != hipSuccess)
         return tpt_set_error(TPT_ENOMEM, "scratch allocation failed");
     hipMemsetAsync(c->cell_scratch_u, 0, cb, c->stream);
     hipMemsetAsync(c->cell_scratch_u2, 0, cb, c->stream);
@@ -102,7 +104,7 @@ static void free_all(tpt_ctx *c) {
     for (int id = 0; id < TPT_BUF_COUNT; ++id)
         if (c->buf[id]) hipFree(c->buf[id]);
     void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
-                     c->slot_scratch_u, c->counters};
+                     c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters};
     for (void *p : extra)
         if (p) hipFree(p);
     tpt_gravity_free(c);

@@ -75,6 +75,8 @@ struct tpt_ctx {
     uint32_t *cell_scratch_u2 = nullptr;  // cells words
     float *cell_scratch_f2 = nullptr;     // cells words
     uint32_t *slot_scratch_u = nullptr;   // capacity words
+    uint32_t *slot_scratch_u2 = nullptr;  // capacity words
+    float *slot_scratch_f = nullptr;      // capacity words
     uint32_t *counters = nullptr;         // small device counters
     tpt_fft_state fft;
 

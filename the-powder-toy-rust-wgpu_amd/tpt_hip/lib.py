@@ -151,13 +151,15 @@ class Context:
     def load_state(self, st):
         """Upload every array of a SimState (sync_host_to_gpu of all categories)."""
         self.set_tables(st.arrays["elem_props"], st.arrays["move_lut"])
-        for name in S.BUFFER_ORDER[:-2]:
-            self.upload(name, st.arrays[name])
+        self.upload("photon_mask", st.arrays["photon_mask"])
+        for name in S.BUFFER_ORDER:
+            if name not in S.TABLES:
+                self.upload(name, st.arrays[name])
         self.set_slot_bound(st.slot_bound)
 
     def fetch(self, st, names=None):
         """Download arrays into a SimState (sync_gpu_to_host)."""
-        for name in names or S.BUFFER_ORDER[:-2]:
+        for name in names or [n for n in S.BUFFER_ORDER if n not in S.TABLES]:
             arr = st.arrays[name]
             arr[:] = self.download(name, arr.dtype, arr.size)
         return st

@@ -47,7 +47,8 @@ CELL_ARRAYS = [
 ]
 # Buffer ids of include/tpt_hip.h, in enum order.
 BUFFER_ORDER = [n for n, _ in SLOT_ARRAYS] + [n for n, _ in PIXEL_ARRAYS] + \
-    [n for n, _ in CELL_ARRAYS] + ["elem_props", "move_lut"]
+    [n for n, _ in CELL_ARRAYS] + ["elem_props", "move_lut", "photon_mask"]
+TABLES = ["elem_props", "move_lut", "photon_mask"]
 
 
 def default_config(xres, yres, capacity=None):
@@ -85,6 +86,7 @@ class SimState:
             self.arrays[n] = np.zeros(self.xcells * self.ycells, dtype=dt)
         self.arrays["elem_props"] = E.element_props_table()
         self.arrays["move_lut"] = E.move_lut()
+        self.arrays["photon_mask"] = E.photon_mask()
         self.slot_bound = 1
 
     def copy(self):
