@@ -23,23 +23,39 @@ for _p in (ROOT, PKG):
 
 HBM_PEAK_GBS = 8000.0
 
-# Passes the tick runs, in SimulationEngine::step order (lib.rs:334-409).
-TICK_PASSES = ["air"]
+# The whole tick: tpt_step (SimulationEngine::step order, lib.rs:334-409).
+TICK_PASSES = ["tick"]
 
-# Algorithmic bytes per launch of each kernel: every array the kernel must
-# read or write, once (stencil re-reads and LDS reuse are free).
-#   C = air cells, P = pixels.
-def kernel_bytes(name, C, P, N):
+
+def kernel_bytes(name, C, P, S, N, W, yc):
+    """Algorithmic bytes of one launch of each timed pass: every array the pass
+    must stream, once (stencil re-reads, LDS reuse and cache hits are free).
+    C = air cells, P = pixels, S = scanned slots, N = live particles,
+    W = padded FFT row length, yc = air-cell rows."""
     table = {
-        # pv, vx, vy, block_air in; opv, ovx, ovy out
-        "air_pressure_velocity": 28 * C,
-        # ovx, ovy, opv, block_air, wall_bmap, fan_vx, fan_vy in; vx, vy, pv out
-        "air_smooth_advect": 40 * C,
-        # hv, vx, vy, block_airh in; ohv, vx', vy' out
-        "air_heat_convect": 28 * C,
-        # ohv, vx', vy', wall_bmap, wall_emap, elem masks x2, pv, 16 pmap words in;
-        # hv, vx, vy, wall_emap, block_air, block_airh, pv out
-        "air_settle": 124 * C,
+        "air_pressure_velocity": 28 * C,      # pv,vx,vy,block_air in; opv,ovx,ovy out
+        "air_smooth_advect": 40 * C,          # ovx,ovy,opv,block_air,wall,fan x2 in; vx,vy,pv out
+        "air_heat_convect": 28 * C,           # hv,vx,vy,block_airh in; ohv,vx',vy' out
+        "air_settle": 124 * C,                # 9 grids + 16 pmap words in; 7 grids out
+        "grav_fft_rows": 8 * C + 8 * yc * W,  # mass,mask in; ycells padded rows out
+        "grav_fft_cols": None,                # filled in below (needs H)
+        "grav_fft_inverse_extract": 16 * yc * W + 12 * C,
+        "grav_apply": 4 * S + 36 * N,         # ty scan; x,y,vx,vy,flags,force gather in; vx,vy out
+        "grav_build_mass": 4 * S + 12 * C,    # ty scan; clear + copy of the mass grid
+        "recalc_clear": 12 * P,               # pmap, photons, pmap_count
+        "recalc_scatter": 4 * S + 20 * N,     # ty scan; life,x,y in; life out; pmap/count atomics
+        "check_stacking": 4 * P + 4 * S + 12 * N,
+        "thermal_conduction": 4 * S + 32 * N, # ty scan; x,y,life,tmp,ctype,temp in; equil, hv out
+        "thermal_conduction_cells": 16 * C,
+        "thermal_apply": 4 * S + 32 * N,
+        "phase_transition": 12 * S + 64 * N,
+        "phase_transition_cells": 16 * C,
+        "refresh_pmap_types": 4 * S + 16 * N,
+        "air_deposit": 4 * S + 20 * N + 16 * C,
+        "move_clear_intents": 4 * P,
+        "move_velocity_intent": 4 * S + 52 * N,  # ty scan; x,y,vx,vy,flags,tmp,air in; vx,vy,intent out; claim
+        "move_commit": 4 * S + 32 * N,
+        "move_rebuild_maps": 8 * P + 4 * S + 16 * N,
     }
     return table.get(name)
 
@@ -64,6 +80,8 @@ def run_pass(ctx, name, frame):
 def oracle_pass(o, name, frame):
     if name == "air":
         o.call("oracle_step_air")
+    elif name == "tick":
+        o.call("oracle_step", frame)
 
 
 def main():
@@ -96,6 +114,14 @@ def main():
     st = S.make_scene(args.workload)
     C, P = st.xcells * st.ycells, st.cfg["xres"] * st.cfg["yres"]
     N = int((st.arrays["part_ty"] != 0).sum())
+    S_ = st.slot_bound
+    W = 1
+    while W < 2 * st.xcells:
+        W *= 2
+    H = 1
+    while H < 2 * st.ycells:
+        H *= 2
+    yc = st.ycells
     passes = TICK_PASSES
     ctx = L.Context(st.cfg)
     ctx.load_state(st)
@@ -127,7 +153,9 @@ def main():
     per_kernel = {}
     for name, (ms, n) in times.items():
         if n:
-            b = kernel_bytes(name, C, P, N)
+            b = kernel_bytes(name, C, P, S_, N, W, yc)
+            if name == "grav_fft_cols":
+                b = 8 * yc * W * 3 + 16 * W * H   # work in, 2 spectra out, 2 kernel spectra in
             avg_us = ms / n * 1e3
             per_kernel[name] = {"avg_us": round(avg_us, 3), "launches": n,
                                 "alg_bytes": b,

@@ -138,9 +138,19 @@ void oracle_gravity_solve(oracle_state *s) {
     const uint32_t tw_bits = bit_count(w > h ? w : h), tw_len = (1u << tw_bits) / 2 ? (1u << tw_bits) / 2 : 1;
     float *tr = malloc(sizeof(float) * tw_len), *ti = malloc(sizeof(float) * tw_len);
     oracle_fft_twiddle_f32(1u << tw_bits, tr, ti);
-    float *kx = calloc((size_t)cells * 2, sizeof(float)), *ky = calloc((size_t)cells * 2, sizeof(float));
+    /* The kernel spectra depend on the grid only; like GravityFft (mod.rs:126-133)
+     * they are built once per grid size and reused. */
+    static float *kx = NULL, *ky = NULL;
+    static uint32_t kw = 0, kh = 0;
     float *work = calloc((size_t)cells * 2, sizeof(float));
     float *sx = malloc(sizeof(float) * cells * 2), *sy = malloc(sizeof(float) * cells * 2);
+    if (kx == NULL || kw != w || kh != h) {
+    free(kx);
+    free(ky);
+    kx = calloc((size_t)cells * 2, sizeof(float));
+    ky = calloc((size_t)cells * 2, sizeof(float));
+    kw = w;
+    kh = h;
     /* build_kernel_field_kernel (fft.rs:209-236), scale of mod.rs:180 */
     const float scale = -M_GRAV_F / (float)(w * h);
     for (uint32_t id = 0; id < cells; ++id) {
@@ -162,6 +172,7 @@ void oracle_gravity_solve(oracle_state *s) {
     oracle_fft_transform_f32(kx, w, h, 1, w, tr, ti, tw_bits, 1.0f);
     oracle_fft_transform_f32(ky, h, w, w, 1, tr, ti, tw_bits, 1.0f);
     oracle_fft_transform_f32(ky, w, h, 1, w, tr, ti, tw_bits, 1.0f);
+    }
     /* build_padded_mass_kernel (fft.rs:153-178) */
     const float inv_scale = 1.0f / MASS_SCALE;
     for (uint32_t py = 0; py < yc; ++py)
@@ -191,7 +202,7 @@ void oracle_gravity_solve(oracle_state *s) {
             s->grav_fx[id] = on ? sx[((size_t)py * w + px) * 2] : 0.0f;
             s->grav_fy[id] = on ? sy[((size_t)py * w + px) * 2] : 0.0f;
         }
-    free(tr); free(ti); free(kx); free(ky); free(work); free(sx); free(sy);
+    free(tr); free(ti); free(work); free(sx); free(sy);
 }
 
 /* The same convolution in double precision; writes fx/fy (xcells*ycells). */
