@@ -7,31 +7,10 @@ from tpt_hip import elements as E
 from tpt_hip import lib as L
 from tpt_hip import scene as S
 
+from edge_scenes import adversarial
+
 pytestmark = pytest.mark.gpu
 FIELDS = ["pmap", "photons", "pmap_count", "part_ty", "part_life", "part_tmp", "part_temp"]
-
-
-def adversarial(st, rng):
-    """Stacks, INVIS/FILT pairs, photons and FIRE with short life on top of a scene."""
-    a = st.arrays
-    free = np.flatnonzero(a["part_ty"] == 0)
-    base = st.slot_bound
-    add = []
-    for k in range(20):                      # stacks of 3..60 matter particles
-        x, y = int(rng.integers(8, st.cfg["xres"] - 8)), int(rng.integers(8, st.cfg["yres"] - 8))
-        for _ in range(int(rng.integers(3, 60))):
-            add.append(("DUST", x, y, 0))
-    for k in range(20):                      # INVIS/FILT sharing pixels with matter or each other
-        x, y = int(rng.integers(8, st.cfg["xres"] - 8)), int(rng.integers(8, st.cfg["yres"] - 8))
-        for name in rng.choice(["INVIS", "FILT", "WATR"], size=3):
-            add.append((str(name), x, y, 0))
-    for k in range(50):
-        add.append(("PHOT", int(rng.integers(8, 200)), int(rng.integers(8, 200)), 0))
-        add.append(("FIRE", int(rng.integers(8, 200)), int(rng.integers(8, 200)), int(rng.integers(0, 3))))
-    for j, (name, x, y, life) in enumerate(add):
-        s = base + j
-        a["part_ty"][s], a["part_x"][s], a["part_y"][s], a["part_life"][s] = E.pt(name), x, y, life
-    st.slot_bound = base + len(add)
 
 
 @pytest.mark.parametrize("workload,scale", [("612x384", 1.0), ("2048x2048", 0.25)])

@@ -56,3 +56,35 @@ def test_legacy_mode_transitions_only(gpu):
         ctx.fetch(st, FIELDS)
     for n in FIELDS:
         assert np.array_equal(ref.arrays[n].view(np.uint32), st.arrays[n].view(np.uint32)), n
+
+
+@pytest.mark.parametrize("pixel_major", [True, False], ids=["pixel-major", "per-slot"])
+def test_thermal_rare_branches(gpu, pixel_major):
+    """Stacks, INVIS/FILT, photons, HSWC/PIPE/GEL/FILT neighbourhoods and
+    off-pixel positions, through both conduction forms: right after
+    RecalcFreeParticles the pixel-major form runs; an upload in between
+    makes the library fall back to the per-slot form."""
+    from edge_scenes import adversarial, off_pixel_positions, thermal_specials
+    st = hot_scene("612x384", 0.5)
+    rng = np.random.default_rng(23)
+    adversarial(st, rng)
+    thermal_specials(st, rng)
+    off_pixel_positions(st, rng)
+    ref = st.copy()
+    o = O.OracleState(ref)
+    print(f"[thermal rare branches] pixel_major={pixel_major}", flush=True)
+    with L.Context(st.cfg) as ctx:
+        ctx.load_state(st)
+        for frame in range(4):
+            seed = L.frame_seed(frame)
+            stack = frame % 2 == 0
+            ctx.recalc_free_particles(seed, stack)
+            o.call("oracle_recalc_free_particles", seed, 1 if stack else 0)
+            if not pixel_major:
+                ctx.upload("part_tmp2", ctx.download("part_tmp2", np.int32))
+            ctx.step_thermal(seed)
+            o.call("oracle_step_thermal", seed, 0)
+        ctx.fetch(st, FIELDS)
+    for n in FIELDS:
+        a, b = ref.arrays[n].view(np.uint32), st.arrays[n].view(np.uint32)
+        assert np.array_equal(a, b), f"{n}: {np.count_nonzero(a != b)} differ"

@@ -983,6 +983,7 @@ __global__ void mv_map_invis(const uint32_t *__restrict__ ty_a, const float *__r
 }  // namespace
 
 int tpt_movement_step(tpt_ctx *ctx, uint32_t seed) {
+    ctx->maps_fresh = false;
     const tpt_config &c = ctx->cfg;
     const uint32_t slots = ctx->slot_bound, P = ctx->pixels;
     const int xres = (int)c.xres, yres = (int)c.yres, xc = (int)ctx->xcells;

@@ -1,16 +1,29 @@
 // RecalcFreeParticles for gfx950: SimBuffers::recalc_free_particles_device
 // (crates/tpt-sim/src/buffers/spatial.rs:371-472).
 //
-//   recalc_clear     zero pmap / photons / pmap_count (16-byte stores)
+//   recalc_clear     zero the pixel records / photons / pmap_count (16-byte
+//                    stores); pmap itself is rewritten whole by recalc_extract
 //   recalc_scatter   life decay + kills, map scatter, stack count.  The
 //                    reference's last-writer store on a shared pixel becomes
-//                    atomicMax on the packed word, which is the highest slot,
-//                    the serial order's last writer.  INVIS/FILT (which only
-//                    take an empty pixel) are queued instead.
+//                    a 64-bit atomicMax on the pixel record
+//                    ((slot+1)<<9 | flag<<8 | type) << 32 | temp bits, whose
+//                    high word orders by slot: the highest slot, the serial
+//                    order's last writer, keeps the pixel.  INVIS/FILT (which
+//                    only take an empty pixel) are queued instead.
 //   recalc_invis     the queued INVIS/FILT particles take their pixel only if
 //                    no other matter holds it; among several the lowest slot
 //                    wins (the first serial writer), by a CAS-min loop.
+//   recalc_extract   pmap = record high words without the flag; pmap_count
+//                    gains the owner's own +1 (losers counted themselves).
 //   stacking_mark / stacking_apply   CheckStacking (spatial.rs:190-275).
+//
+// The pixel record is library-internal: it lets the conduction pass run as
+// a pixel-major stencil.  Its flag bit says "this owner is handled through
+// the off-map list": every live slot whose own (truncated) pixel is on the
+// canvas ends up either as the unflagged owner of that pixel or in the list,
+// never both.  The list holds energy particles, INVIS/FILT, particles that
+// lose their pixel to a higher slot, and particles whose rounded and
+// truncated pixels differ.
 
 #include "tpt_internal.h"
 
@@ -27,82 +40,163 @@ __device__ __forceinline__ uint32_t lowbias_hash(uint32_t seed) {
     return h ^ (h >> 16);
 }
 
-__global__ void recalc_clear(uint4 *__restrict__ pmap, uint4 *__restrict__ photons, uint4 *__restrict__ count,
+// Appends `v` to a list for every lane where `pred` holds, one atomic per
+// wave.  Every lane of the wave must call it.
+__device__ __forceinline__ void wave_append(uint32_t *__restrict__ list, uint32_t *__restrict__ count, uint32_t v,
+                                            bool pred) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (pred) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
+__device__ __forceinline__ bool counted_type(uint32_t t) { return t != 96u && t != 153u && t != 159u && t != 28u; }
+
+__global__ void recalc_clear(uint4 *__restrict__ rec, uint4 *__restrict__ photons, uint4 *__restrict__ count,
                              uint32_t n4, uint32_t *__restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) counters[0] = 0u;
+    if (i == 0) {
+        counters[TPT_CTR_INVIS] = 0u;
+        counters[TPT_CTR_OFFMAP] = 0u;
+    }
     if (i >= n4) return;
     const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-    pmap[i] = z;
+    rec[2 * i] = z;
+    rec[2 * i + 1] = z;
     photons[i] = z;
     count[i] = z;
 }
 
 __global__ __launch_bounds__(BLK) void recalc_scatter(
     uint32_t *__restrict__ ty_a, int32_t *__restrict__ life_a, const float *__restrict__ px_a,
-    const float *__restrict__ py_a, const uint32_t *__restrict__ elem_props, const uint32_t *__restrict__ wall_bmap,
-    const uint32_t *__restrict__ wall_emap, uint32_t *__restrict__ pmap, uint32_t *__restrict__ photons,
-    uint32_t *__restrict__ pmap_count, uint32_t *__restrict__ invis_list, uint32_t *__restrict__ counters,
-    uint32_t slots, int xres, int yres, int xc) {
+    const float *__restrict__ py_a, const float *__restrict__ temp_a, const uint32_t *__restrict__ elem_props,
+    const uint32_t *__restrict__ wall_bmap, const uint32_t *__restrict__ wall_emap, unsigned long long *__restrict__ rec,
+    uint32_t *__restrict__ photons, uint32_t *__restrict__ pmap_count, uint32_t *__restrict__ invis_list,
+    uint32_t *__restrict__ offmap, uint32_t *__restrict__ counters, uint32_t slots, int xres, int yres, int xc) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= slots) return;
-    uint32_t ty = ty_a[i];
-    if (ty == 0u || ty >= (uint32_t)TPT_PT_NUM) return;
-    const int x = tpt_f2i(px_a[i] + 0.5f), y = tpt_f2i(py_a[i] + 0.5f);
-    const bool in_bounds = x >= 0 && y >= 0 && x < xres && y < yres;
-    bool stasis = false;
-    if (in_bounds) {
-        const int cell = (y / TPT_CELL) * xc + x / TPT_CELL;
-        stasis = (wall_bmap[cell] & 0xFFu) == 18u && (wall_emap[cell] & 0xFFu) < 8u;
-    }
-    const uint32_t props = elem_props[ty * 32u + 1u];
-    const int32_t life = life_a[i];
-    if (life > 0 && (props & 0x4000u) && (!in_bounds || !stasis)) {
-        const int32_t next = life - 1;
-        life_a[i] = next;
-        if (next <= 0 && (props & (0x10000u | 0x8000u))) {
+    bool app_invis = false, app_self = false, app_other = false;
+    uint32_t other = 0u;
+    uint32_t ty = i < slots ? ty_a[i] : 0u;
+    if (ty != 0u && ty < (uint32_t)TPT_PT_NUM) {
+        const float fx = px_a[i], fy = py_a[i];
+        const int x = tpt_f2i(fx + 0.5f), y = tpt_f2i(fy + 0.5f);
+        const bool in_bounds = x >= 0 && y >= 0 && x < xres && y < yres;
+        bool stasis = false;
+        if (in_bounds) {
+            const int cell = (y / TPT_CELL) * xc + x / TPT_CELL;
+            stasis = (wall_bmap[cell] & 0xFFu) == 18u && (wall_emap[cell] & 0xFFu) < 8u;
+        }
+        const uint32_t props = elem_props[ty * 32u + 1u];
+        const int32_t life = life_a[i];
+        if (life > 0 && (props & 0x4000u) && (!in_bounds || !stasis)) {
+            const int32_t next = life - 1;
+            life_a[i] = next;
+            if (next <= 0 && (props & (0x10000u | 0x8000u))) {
+                ty_a[i] = 0u;
+                ty = 0u;
+            }
+        } else if (life <= 0 && (props & 0x8000u) && (!in_bounds || !stasis)) {
             ty_a[i] = 0u;
             ty = 0u;
         }
-    } else if (life <= 0 && (props & 0x8000u) && (!in_bounds || !stasis)) {
-        ty_a[i] = 0u;
-        ty = 0u;
+        // The pixel the per-slot thermal kernels use (truncated position).
+        const int ox = tpt_f2i(fx), oy = tpt_f2i(fy);
+        const bool own_in = ox >= 0 && oy >= 0 && ox < xres && oy < yres;
+        const bool same_px = ox == x && oy == y;
+        if (ty != 0u && !in_bounds) {
+            app_self = own_in;
+        } else if (ty != 0u) {
+            const uint32_t idx = (uint32_t)y * (uint32_t)xres + (uint32_t)x;
+            const uint32_t packed = ((i + 1u) << 9) | (ty & 0x1FFu);
+            if (props & 0x10u) {
+                atomicMax(&photons[idx], packed);
+                app_self = own_in;
+            } else if (ty == 124u || ty == 134u) {  // INVIS / FILT
+                app_invis = true;
+                app_self = own_in;
+                atomicAdd(&pmap_count[idx], 1u);
+            } else {
+                const uint32_t key = packed | (same_px ? 0u : 0x100u);
+                const unsigned long long v =
+                    ((unsigned long long)key << 32) | (unsigned long long)__float_as_uint(temp_a[i]);
+                const unsigned long long old = atomicMax(&rec[idx], v);
+                const uint32_t okey = (uint32_t)(old >> 32);
+                if (!same_px) app_self = own_in;
+                if (okey > key) {
+                    // Lost to a higher slot: count self, list self.
+                    if (counted_type(ty)) atomicAdd(&pmap_count[idx], 1u);
+                    if (same_px) app_self = true;
+                } else if (okey != 0u) {
+                    // Displaced the previous holder: count it, list it.
+                    if (counted_type(okey & 0xFFu)) atomicAdd(&pmap_count[idx], 1u);
+                    if ((okey & 0x100u) == 0u) {
+                        app_other = true;
+                        other = (okey >> 9) - 1u;
+                    }
+                }
+            }
+        }
     }
-    if (ty == 0u || !in_bounds) return;
-    const uint32_t idx = (uint32_t)y * (uint32_t)xres + (uint32_t)x;
-    const uint32_t packed = ((i + 1u) << 9) | (ty & 0x1FFu);
-    if (props & 0x10u) {
-        atomicMax(&photons[idx], packed);
-        return;
-    }
-    if (ty == 124u || ty == 134u) {  // INVIS / FILT
-        invis_list[atomicAdd(&counters[0], 1u)] = i;
-    } else {
-        atomicMax(&pmap[idx], packed);
-    }
-    if (ty != 96u && ty != 153u && ty != 159u && ty != 28u) atomicAdd(&pmap_count[idx], 1u);
+    wave_append(invis_list, &counters[TPT_CTR_INVIS], i, app_invis);
+    wave_append(offmap, &counters[TPT_CTR_OFFMAP], i, app_self);
+    wave_append(offmap, &counters[TPT_CTR_OFFMAP], other, app_other);
 }
 
 __global__ __launch_bounds__(BLK) void recalc_invis(const uint32_t *__restrict__ ty_a, const float *__restrict__ px_a,
-                                                    const float *__restrict__ py_a, uint32_t *__restrict__ pmap,
+                                                    const float *__restrict__ py_a, const float *__restrict__ temp_a,
+                                                    unsigned long long *__restrict__ rec,
                                                     const uint32_t *__restrict__ invis_list,
                                                     const uint32_t *__restrict__ counters, int xres) {
-    const uint32_t n = counters[0];
+    const uint32_t n = counters[TPT_CTR_INVIS];
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t i = invis_list[k];
         const uint32_t ty = ty_a[i];
         const int x = tpt_f2i(px_a[i] + 0.5f), y = tpt_f2i(py_a[i] + 0.5f);
         const uint32_t idx = (uint32_t)y * (uint32_t)xres + (uint32_t)x;
         const uint32_t packed = ((i + 1u) << 9) | (ty & 0x1FFu);
-        uint32_t cur = pmap[idx];
+        // INVIS/FILT owners are always handled through the off-map list.
+        const unsigned long long v = ((unsigned long long)(packed | 0x100u) << 32) |
+                                     (unsigned long long)__float_as_uint(temp_a[i]);
+        unsigned long long cur = rec[idx];
         for (;;) {
-            const uint32_t t = cur & 0x1FFu;
-            const bool other_matter = cur != 0u && t != 124u && t != 134u;
-            if (other_matter || (cur != 0u && cur <= packed)) break;
-            const uint32_t prev = atomicCAS(&pmap[idx], cur, packed);
+            const uint32_t ck = (uint32_t)(cur >> 32);
+            const uint32_t t = ck & 0xFFu;
+            const bool other_matter = ck != 0u && t != 124u && t != 134u;
+            if (other_matter || (ck != 0u && (ck & ~0x100u) <= packed)) break;
+            const unsigned long long prev = atomicCAS(&rec[idx], cur, v);
             if (prev == cur) break;
             cur = prev;
         }
+    }
+}
+
+// pmap and the owners' own pmap_count contribution from the pixel records,
+// four pixels per thread.
+__global__ void recalc_extract(const ulonglong2 *__restrict__ rec, uint4 *__restrict__ pmap, uint4 *__restrict__ count,
+                               uint32_t n4) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n4) return;
+    const ulonglong2 a = rec[2 * i], b = rec[2 * i + 1];
+    const uint32_t k[4] = {(uint32_t)(a.x >> 32), (uint32_t)(a.y >> 32), (uint32_t)(b.x >> 32), (uint32_t)(b.y >> 32)};
+    uint32_t add[4];
+    bool any = false;
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t t = k[j] & 0xFFu;
+        add[j] = (k[j] != 0u && t != 124u && t != 134u && counted_type(t)) ? 1u : 0u;
+        any |= add[j] != 0u;
+    }
+    pmap[i] = make_uint4(k[0] & ~0x100u, k[1] & ~0x100u, k[2] & ~0x100u, k[3] & ~0x100u);
+    if (any) {
+        uint4 c = count[i];
+        c.x += add[0];
+        c.y += add[1];
+        c.z += add[2];
+        c.w += add[3];
+        count[i] = c;
     }
 }
 
@@ -125,6 +219,7 @@ __global__ void stacking_mark(uint32_t *__restrict__ pmap_count, const uint32_t 
 __global__ __launch_bounds__(BLK) void stacking_apply(uint32_t *__restrict__ ty_a, const float *__restrict__ px_a,
                                                       const float *__restrict__ py_a, float *__restrict__ temp_a,
                                                       int32_t *__restrict__ tmp_a, uint32_t *__restrict__ pmap,
+                                                      unsigned long long *__restrict__ rec,
                                                       uint32_t *__restrict__ pmap_count,
                                                       const uint32_t *__restrict__ elem_props, uint32_t slots, int xres,
                                                       int yres, uint32_t npart) {
@@ -148,6 +243,9 @@ __global__ __launch_bounds__(BLK) void stacking_apply(uint32_t *__restrict__ ty_
         tmp_a[i] = (int32_t)strength;
         __hip_atomic_store(&pmap_count[idx], npart, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         pmap[idx] = ((i + 1u) << 9) | 40u;
+        const uint32_t flag = (uint32_t)(rec[idx] >> 32) & 0x100u;
+        rec[idx] = ((unsigned long long)(((i + 1u) << 9) | flag | 40u) << 32) |
+                   (unsigned long long)__float_as_uint(9999.0f);
     } else {
         ty_a[i] = 0u;
     }
@@ -160,34 +258,44 @@ int tpt_recalc_step(tpt_ctx *ctx, uint32_t seed, int stacking) {
     const int xres = (int)ctx->cfg.xres, yres = (int)ctx->cfg.yres;
     uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), *photons = tpt_buf<uint32_t>(ctx, TPT_BUF_PHOTONS),
              *count = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP_COUNT);
+    unsigned long long *rec = (unsigned long long *)ctx->pix_rec;
+    ctx->maps_fresh = false;
     tpt_pass_begin(ctx, PASS_RECALC_CLEAR);
     // P is a multiple of 16 (xres and yres are multiples of 4).
     const uint32_t n4 = P / 4;
-    hipLaunchKernelGGL(recalc_clear, dim3((n4 + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, (uint4 *)pmap,
+    hipLaunchKernelGGL(recalc_clear, dim3((n4 + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, (uint4 *)rec,
                        (uint4 *)photons, (uint4 *)count, n4, ctx->counters);
     tpt_pass_end(ctx, PASS_RECALC_CLEAR);
     tpt_pass_begin(ctx, PASS_RECALC_SCATTER);
     hipLaunchKernelGGL(recalc_scatter, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
                        tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
                        tpt_buf<float>(ctx, TPT_BUF_PART_X), tpt_buf<float>(ctx, TPT_BUF_PART_Y),
-                       tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP),
-                       tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), pmap, photons, count, ctx->slot_scratch_u,
-                       ctx->counters, slots, xres, yres, (int)ctx->xcells);
+                       tpt_buf<float>(ctx, TPT_BUF_PART_TEMP), tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS),
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), rec,
+                       photons, count, ctx->slot_scratch_u, ctx->offmap, ctx->counters, slots, xres, yres,
+                       (int)ctx->xcells);
     hipLaunchKernelGGL(recalc_invis, dim3(64), dim3(BLK), 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY),
-                       tpt_buf<float>(ctx, TPT_BUF_PART_X), tpt_buf<float>(ctx, TPT_BUF_PART_Y), pmap,
-                       ctx->slot_scratch_u, ctx->counters, xres);
+                       tpt_buf<float>(ctx, TPT_BUF_PART_X), tpt_buf<float>(ctx, TPT_BUF_PART_Y),
+                       tpt_buf<float>(ctx, TPT_BUF_PART_TEMP), rec, ctx->slot_scratch_u, ctx->counters, xres);
+    hipLaunchKernelGGL(recalc_extract, dim3((n4 + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
+                       (const ulonglong2 *)rec, (uint4 *)pmap, (uint4 *)count, n4);
     tpt_pass_end(ctx, PASS_RECALC_SCATTER);
     int rc = tpt_check_launch("recalc_free_particles");
-    if (rc || !stacking) return rc;
-    const uint32_t npart = ctx->cfg.xres * ctx->cfg.yres;
-    tpt_pass_begin(ctx, PASS_STACKING);
-    hipLaunchKernelGGL(stacking_mark, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, count,
-                       tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), P, ctx->cfg.xres, ctx->xcells, npart, seed);
-    hipLaunchKernelGGL(stacking_apply, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
-                       tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<float>(ctx, TPT_BUF_PART_X),
-                       tpt_buf<float>(ctx, TPT_BUF_PART_Y), tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),
-                       tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP), pmap, count, tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS),
-                       slots, xres, yres, npart);
-    tpt_pass_end(ctx, PASS_STACKING);
-    return tpt_check_launch("check_stacking");
+    if (rc) return rc;
+    if (stacking) {
+        const uint32_t npart = ctx->cfg.xres * ctx->cfg.yres;
+        tpt_pass_begin(ctx, PASS_STACKING);
+        hipLaunchKernelGGL(stacking_mark, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, count,
+                           tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), P, ctx->cfg.xres, ctx->xcells, npart, seed);
+        hipLaunchKernelGGL(stacking_apply, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
+                           tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<float>(ctx, TPT_BUF_PART_X),
+                           tpt_buf<float>(ctx, TPT_BUF_PART_Y), tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),
+                           tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP), pmap, rec, count,
+                           tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS), slots, xres, yres, npart);
+        tpt_pass_end(ctx, PASS_STACKING);
+        rc = tpt_check_launch("check_stacking");
+        if (rc) return rc;
+    }
+    ctx->maps_fresh = true;
+    return TPT_OK;
 }

@@ -98,13 +98,12 @@ __device__ __forceinline__ uint32_t slot_at(const uint32_t *__restrict__ pmap, i
 }
 
 // thermal_conduction_kernel (thermal_kernel.rs:459-605), rules R2/R3.
-__global__ __launch_bounds__(BLK) void th_conduct(Parts p, const uint32_t *__restrict__ pmap,
-                                                  const float *__restrict__ air_hv,
-                                                  const uint32_t *__restrict__ props, float *__restrict__ hv_new,
-                                                  uint32_t *__restrict__ hv_key, uint32_t *__restrict__ fails,
-                                                  Dims d, uint32_t aheat, uint32_t seed) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.slots) return;
+// One slot, reading its neighbours through pmap and the slot arrays.
+__device__ __forceinline__ void conduct_slot(const Parts &p, const uint32_t *__restrict__ pmap,
+                                             const float *__restrict__ air_hv, const uint32_t *__restrict__ props,
+                                             float *__restrict__ hv_new, uint32_t *__restrict__ hv_key,
+                                             uint32_t *__restrict__ fails, const Dims &d, uint32_t aheat,
+                                             uint32_t seed, uint32_t i) {
     const uint32_t ty = p.ty[i];
     if (ty == 0u || ty >= 256u) return;
     const uint32_t hcond = EP(props, ty, COL_HEAT_CONDUCT);
@@ -155,6 +154,31 @@ __global__ __launch_bounds__(BLK) void th_conduct(Parts p, const uint32_t *__res
     }
 }
 
+__global__ __launch_bounds__(BLK) void th_conduct(Parts p, const uint32_t *__restrict__ pmap,
+                                                  const float *__restrict__ air_hv,
+                                                  const uint32_t *__restrict__ props, float *__restrict__ hv_new,
+                                                  uint32_t *__restrict__ hv_key, uint32_t *__restrict__ fails,
+                                                  Dims d, uint32_t aheat, uint32_t seed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < d.slots) conduct_slot(p, pmap, air_hv, props, hv_new, hv_key, fails, d, aheat, seed, i);
+}
+
+// The off-map list's slots (energy particles, INVIS/FILT, stacked losers,
+// off-pixel positions), same per-slot form.
+__global__ __launch_bounds__(BLK) void th_conduct_list(Parts p, const uint32_t *__restrict__ pmap,
+                                                       const float *__restrict__ air_hv,
+                                                       const uint32_t *__restrict__ props, float *__restrict__ hv_new,
+                                                       uint32_t *__restrict__ hv_key, uint32_t *__restrict__ fails,
+                                                       const uint32_t *__restrict__ list,
+                                                       const uint32_t *__restrict__ counters, Dims d, uint32_t aheat,
+                                                       uint32_t seed) {
+    const uint32_t n = counters[TPT_CTR_OFFMAP];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t i = list[k];
+        if (i < d.slots) conduct_slot(p, pmap, air_hv, props, hv_new, hv_key, fails, d, aheat, seed, i);
+    }
+}
+
 __global__ void th_conduct_cells(float *__restrict__ air_hv, uint32_t *__restrict__ block_airh,
                                  const float *__restrict__ hv_new, uint32_t *__restrict__ hv_key,
                                  uint32_t *__restrict__ fails, uint32_t cells) {
@@ -178,10 +202,8 @@ __global__ void th_conduct_cells(float *__restrict__ air_hv, uint32_t *__restric
 }
 
 // thermal_apply_kernel (thermal_kernel.rs:616-708).
-__global__ __launch_bounds__(BLK) void th_apply(Parts p, const uint32_t *__restrict__ pmap,
-                                                const uint32_t *__restrict__ props, Dims d) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= d.slots) return;
+__device__ __forceinline__ void apply_slot(const Parts &p, const uint32_t *__restrict__ pmap,
+                                           const uint32_t *__restrict__ props, const Dims &d, uint32_t j) {
     const uint32_t jty = p.ty[j];
     if (jty == 0u || jty >= 256u) return;
     const int x = tpt_f2i(p.x[j]), y = tpt_f2i(p.y[j]);
@@ -216,6 +238,236 @@ __global__ __launch_bounds__(BLK) void th_apply(Parts p, const uint32_t *__restr
             }
     }
     p.temp[j] = tpt_clampf(has ? best_temp : own, 0.0f, 9999.0f);
+}
+
+__global__ __launch_bounds__(BLK) void th_apply(Parts p, const uint32_t *__restrict__ pmap,
+                                                const uint32_t *__restrict__ props, Dims d) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < d.slots) apply_slot(p, pmap, props, d, j);
+}
+
+__global__ __launch_bounds__(BLK) void th_apply_list(Parts p, const uint32_t *__restrict__ pmap,
+                                                     const uint32_t *__restrict__ props,
+                                                     const uint32_t *__restrict__ list,
+                                                     const uint32_t *__restrict__ counters, Dims d) {
+    const uint32_t n = counters[TPT_CTR_OFFMAP];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t j = list[k];
+        if (j < d.slots) apply_slot(p, pmap, props, d, j);
+    }
+}
+
+// Pixel-major conduction and apply over the records RecalcFreeParticles
+// left (unflagged record owners only; the rest go through the off-map
+// list).  A 64x16 pixel tile (16x4 cells) per 256-thread block, each thread
+// one 4-pixel column of one cell, the tile plus a one-pixel halo in LDS.
+// Each cell's 16 pixels live in 4 adjacent lanes of one wave, so the R2/R3
+// cell results are reduced with lane shuffles and written by the block,
+// merged with what the off-map list recorded in hv_key / fails.
+constexpr int TW = 64, TH = 16, LW = TW + 2, LH = TH + 2;
+
+// Types whose conduction reads life / tmp / ctype besides the type itself.
+__device__ __forceinline__ bool special_type(uint32_t t) {
+    return t == T_HSWC || t == T_PIPE || t == T_PPIP || t == T_GEL;
+}
+
+__global__ __launch_bounds__(256) void th_conduct_px(Parts p, const unsigned long long *__restrict__ rec,
+                                                     float *__restrict__ air_hv, uint32_t *__restrict__ block_airh,
+                                                     const uint32_t *__restrict__ props,
+                                                     const float *__restrict__ hv_new, uint32_t *__restrict__ hv_key,
+                                                     uint32_t *__restrict__ fails, uint2 *__restrict__ heat, Dims d,
+                                                     uint32_t aheat, uint32_t seed) {
+    __shared__ uint32_t skey[LH][LW];
+    __shared__ float stemp[LH][LW];
+    const int tx = threadIdx.x, row = threadIdx.y, t = row * TW + tx;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    for (int e = t; e < LH * LW; e += TW * 4) {
+        const int ly = e / LW, lx = e - ly * LW;
+        const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+        unsigned long long v = 0ull;
+        if (gx >= 0 && gy >= 0 && gx < d.xres && gy < d.yres) v = rec[(size_t)gy * d.xres + gx];
+        skey[ly][lx] = (uint32_t)(v >> 32);
+        stemp[ly][lx] = __uint_as_float((uint32_t)v);
+    }
+    __syncthreads();
+    const int x = x0 + tx, cy = (y0 >> 2) + row;
+    const bool cell_in = x < d.xres && cy < d.yc;
+    const int ci = cy * d.xc + (x >> 2);
+    float hv0 = 0.0f;
+    if (cell_in && aheat) hv0 = air_hv[ci];  // pre-pass value (R2)
+    uint32_t best_key = 0u, nfail = 0u;
+    float best_hv = 0.0f;
+    for (int k = 0; k < 4 && cell_in; ++k) {
+        const int ly = row * 4 + k + 1, lx = tx + 1, y = y0 + row * 4 + k;
+        const uint32_t key = skey[ly][lx];
+        uint2 out = make_uint2(__float_as_uint(-1.0f), 0u);
+        if (key != 0u) {
+            const uint32_t s = (key >> 9) - 1u, ty = key & 0xFFu;
+            int32_t life = 0, tmp = 0;
+            uint32_t ctype = 0u;
+            if (special_type(ty)) {
+                life = p.life[s];
+                tmp = p.tmp[s];
+                ctype = p.ctype[s];
+            }
+            const uint32_t hcond = EP(props, ty, COL_HEAT_CONDUCT);
+            const bool ins = is_heat_insulator(ty, life, tmp, hcond);
+            out.y = (ins ? 1u : 0u) | (tmp == 1 ? 2u : 0u);
+            if (key & 0x100u) {
+                out.x = __float_as_uint(p.equil[s]);  // handled through the off-map list
+            } else if (s < d.slots) {
+                float gel = 1.0f;
+                if (ty == T_GEL) gel = (float)tmp * 2.55f;
+                float scaled = (float)hcond * gel;
+                if (scaled < 0.0f) scaled = 0.0f;
+                const uint32_t threshold = tpt_f2u(scaled);
+                const uint32_t roll = gpu_hash(seed ^ (s * PARTICLE_HASH_SALT) ^ ty) % 250u;
+                float eq = -1.0f;
+                if (!ins && roll < threshold) {
+                    const float t0 = stemp[ly][lx];
+                    float self_temp = t0;
+                    if (aheat && (EP(props, ty, COL_PROPERTIES) & 0x00200000u) == 0u) {
+                        const float dtemp = hv0 - t0;
+                        const float hc = heat_capacity(ty, ctype);
+                        float alpha = 0.4f * hc;
+                        if (alpha > 0.04f) alpha = 0.04f;
+                        self_temp = tpt_clampf(t0 + alpha * dtemp / hc, 0.0f, 9999.0f);
+                        const float hn = tpt_clampf(hv0 - alpha * dtemp, 0.0f, 9999.0f);
+                        if (s + 1u > best_key) {
+                            best_key = s + 1u;
+                            best_hv = hn;
+                        }
+                    }
+                    const float hc_self = heat_capacity(ty, ctype);
+                    float c_heat = self_temp * hc_self, hc_total = hc_self;
+                    for (int ny = 0; ny < 3; ++ny)
+                        for (int nx = 0; nx < 3; ++nx) {
+                            if (nx == 1 && ny == 1) continue;
+                            const uint32_t nk = skey[ly + ny - 1][lx + nx - 1];
+                            if (nk == 0u) continue;
+                            const uint32_t nid = (nk >> 9) - 1u, nty = nk & 0xFFu;
+                            if (nid >= d.slots || nty == 0u) continue;
+                            int32_t nlife = 0, ntmp = 0;
+                            uint32_t nctype = 0u;
+                            if (special_type(nty)) {
+                                nlife = p.life[nid];
+                                ntmp = p.tmp[nid];
+                                nctype = p.ctype[nid];
+                            }
+                            if (is_heat_insulator(nty, nlife, ntmp, EP(props, nty, COL_HEAT_CONDUCT))) continue;
+                            if (pair_excluded(ty, nty, tmp, ntmp)) continue;
+                            const float n_hc = heat_capacity(nty, nctype);
+                            c_heat += stemp[ly + ny - 1][lx + nx - 1] * n_hc;
+                            hc_total += n_hc;
+                        }
+                    eq = tpt_clampf(c_heat / hc_total, 0.0f, 9999.0f);
+                } else {
+                    ++nfail;
+                }
+                p.equil[s] = eq;
+                out.x = __float_as_uint(eq);
+            }
+        }
+        heat[(size_t)y * d.xres + x] = out;
+    }
+    for (int m = 1; m <= 2; m <<= 1) {
+        const uint32_t ok = __shfl_xor(best_key, m);
+        const float oh = __shfl_xor(best_hv, m);
+        nfail += __shfl_xor(nfail, m);
+        if (ok > best_key) {
+            best_key = ok;
+            best_hv = oh;
+        }
+    }
+    if (cell_in && (tx & 3) == 0) {
+        const uint32_t lk = hv_key[ci];
+        if (lk) {
+            if (lk > best_key) {
+                best_key = lk;
+                best_hv = hv_new[lk - 1u];
+            }
+            hv_key[ci] = 0u;
+        }
+        if (best_key) air_hv[ci] = best_hv;
+        const uint32_t lf = fails[ci];
+        if (lf) {
+            nfail += lf;
+            fails[ci] = 0u;
+        }
+        if (nfail) {
+            uint32_t v = block_airh[ci];
+            if ((v & 8u) == 0u) {
+                const uint32_t room = 8u - (v & 7u);
+                v += nfail < room ? nfail : room;
+                block_airh[ci] = v;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long long *__restrict__ rec,
+                                                   const uint2 *__restrict__ heat, Dims d) {
+    __shared__ uint32_t skey[LH][LW];
+    __shared__ float seq[LH][LW];
+    __shared__ uint32_t sinfo[LH][LW];
+    const int tx = threadIdx.x, row = threadIdx.y, t = row * TW + tx;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    for (int e = t; e < LH * LW; e += TW * 4) {
+        const int ly = e / LW, lx = e - ly * LW;
+        const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+        uint32_t key = 0u;
+        uint2 h = make_uint2(__float_as_uint(-1.0f), 0u);
+        if (gx >= 0 && gy >= 0 && gx < d.xres && gy < d.yres) {
+            const size_t pix = (size_t)gy * d.xres + gx;
+            key = (uint32_t)(rec[pix] >> 32);
+            if (key) h = heat[pix];
+        }
+        skey[ly][lx] = key;
+        seq[ly][lx] = __uint_as_float(h.x);
+        sinfo[ly][lx] = h.y;
+    }
+    __syncthreads();
+    const int x = x0 + tx;
+    if (x >= d.xres) return;
+    for (int k = 0; k < 4; ++k) {
+        const int ly = row * 4 + k + 1, lx = tx + 1, y = y0 + row * 4 + k;
+        if (y >= d.yres) break;
+        const uint32_t key = skey[ly][lx];
+        if (key == 0u || (key & 0x100u)) continue;
+        const uint32_t j = (key >> 9) - 1u, jty = key & 0xFFu;
+        if (j >= d.slots) continue;
+        const float own = __uint_as_float((uint32_t)rec[(size_t)y * d.xres + x]);
+        uint32_t best = j;
+        float best_temp = own;
+        bool has = false;
+        const float eq = seq[ly][lx];
+        if (eq >= 0.0f) {
+            best_temp = eq;
+            has = true;
+        }
+        const uint32_t jinfo = sinfo[ly][lx];
+        if ((jinfo & 1u) == 0u) {
+            const int32_t jtmp1 = (jinfo & 2u) ? 1 : 0;
+            for (int ny = 0; ny < 3; ++ny)
+                for (int nx = 0; nx < 3; ++nx) {
+                    if (nx == 1 && ny == 1) continue;
+                    const uint32_t nk = skey[ly + ny - 1][lx + nx - 1];
+                    if (nk == 0u) continue;
+                    const uint32_t nid = (nk >> 9) - 1u, nty = nk & 0xFFu;
+                    if (nid >= d.slots || nid == j) continue;
+                    if (nty == 0u || !(!has || nid > best)) continue;
+                    const float neq = seq[ly + ny - 1][lx + nx - 1];
+                    if (!(neq >= 0.0f)) continue;
+                    const uint32_t ninfo = sinfo[ly + ny - 1][lx + nx - 1];
+                    if (ninfo & 1u) continue;
+                    if (pair_excluded(nty, jty, (ninfo & 2u) ? 1 : 0, jtmp1)) continue;
+                    best = nid;
+                    best_temp = neq;
+                    has = true;
+                }
+        }
+        p.temp[j] = tpt_clampf(has ? best_temp : own, 0.0f, 9999.0f);
+    }
 }
 
 struct GravCfg {
@@ -630,7 +882,28 @@ int tpt_thermal_step(tpt_ctx *ctx, uint32_t seed, int legacy) {
     const dim3 sg((d.slots + BLK - 1) / BLK), cg((ctx->cells + BLK - 1) / BLK), b(BLK);
     const uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), *props = tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS);
     const float *gfx = tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), *gfy = tpt_buf<float>(ctx, TPT_BUF_GRAV_FY);
-    if (!legacy) {
+    const bool fresh = ctx->maps_fresh;
+    ctx->maps_fresh = false;
+    if (!legacy && fresh) {
+        // Pixel-major path over the records of the RecalcFreeParticles that
+        // ran just before; the off-map list first, so its cell writes and
+        // equilibria are in place for the tiles.
+        const dim3 lg(256), tb(TW, 4), tg((d.xres + TW - 1) / TW, (d.yres + TH - 1) / TH);
+        const unsigned long long *rec = (const unsigned long long *)ctx->pix_rec;
+        float *air_hv = tpt_buf<float>(ctx, TPT_BUF_AIR_HV);
+        const uint32_t aheat = ctx->cfg.aheat_enable ? 1u : 0u;
+        tpt_pass_begin(ctx, PASS_CONDUCT);
+        hipLaunchKernelGGL(th_conduct_list, lg, b, 0, ctx->stream, p, pmap, air_hv, props, ctx->slot_scratch_f,
+                           ctx->cell_scratch_u, ctx->cell_scratch_u2, ctx->offmap, ctx->counters, d, aheat, seed);
+        hipLaunchKernelGGL(th_conduct_px, tg, tb, 0, ctx->stream, p, rec, air_hv,
+                           tpt_buf<uint32_t>(ctx, TPT_BUF_BLOCK_AIRH), props, ctx->slot_scratch_f,
+                           ctx->cell_scratch_u, ctx->cell_scratch_u2, ctx->pix_heat, d, aheat, seed);
+        tpt_pass_end(ctx, PASS_CONDUCT);
+        tpt_pass_begin(ctx, PASS_CONDUCT_APPLY);
+        hipLaunchKernelGGL(th_apply_px, tg, tb, 0, ctx->stream, p, rec, (const uint2 *)ctx->pix_heat, d);
+        hipLaunchKernelGGL(th_apply_list, lg, b, 0, ctx->stream, p, pmap, props, ctx->offmap, ctx->counters, d);
+        tpt_pass_end(ctx, PASS_CONDUCT_APPLY);
+    } else if (!legacy) {
         tpt_pass_begin(ctx, PASS_CONDUCT);
         hipLaunchKernelGGL(th_conduct, sg, b, 0, ctx->stream, p, pmap, tpt_buf<float>(ctx, TPT_BUF_AIR_HV), props,
                            ctx->slot_scratch_f, ctx->cell_scratch_u, ctx->cell_scratch_u2, d,

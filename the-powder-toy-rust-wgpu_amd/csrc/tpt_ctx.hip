@@ -92,7 +92,9 @@ static int alloc_buffers(tpt_ctx *c) {
     if (hipMalloc(&c->cell_scratch_u, cb) != hipSuccess || hipMalloc(&c->cell_scratch_f, cb) != hipSuccess ||
         hipMalloc(&c->cell_scratch_u2, cb) != hipSuccess || hipMalloc(&c->cell_scratch_f2, cb) != hipSuccess ||
         hipMalloc(&c->slot_scratch_u, cap * 4) != hipSuccess || hipMalloc(&c->slot_scratch_u2, cap * 4) != hipSuccess ||
-        hipMalloc(&c->slot_scratch_f, cap * 4) != hipSuccess || hipMalloc(&c->counters, 256) != hipSuccess)
+        hipMalloc(&c->slot_scratch_f, cap * 4) != hipSuccess || hipMalloc(&c->counters, 256) != hipSuccess ||
+        hipMalloc(&c->pix_rec, (size_t)c->pixels * 8) != hipSuccess ||
+        hipMalloc(&c->pix_heat, (size_t)c->pixels * 8) != hipSuccess || hipMalloc(&c->offmap, cap * 4) != hipSuccess)
         return tpt_set_error(TPT_ENOMEM, "scratch allocation failed");
     hipMemsetAsync(c->cell_scratch_u, 0, cb, c->stream);
     hipMemsetAsync(c->cell_scratch_u2, 0, cb, c->stream);
@@ -104,7 +106,8 @@ static void free_all(tpt_ctx *c) {
     for (int id = 0; id < TPT_BUF_COUNT; ++id)
         if (c->buf[id]) hipFree(c->buf[id]);
     void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
-                     c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters};
+                     c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters,
+                     c->pix_rec, c->pix_heat, c->offmap};
     for (void *p : extra)
         if (p) hipFree(p);
     tpt_gravity_free(c);
@@ -194,6 +197,7 @@ extern "C" int tpt_upload(tpt_ctx *ctx, int buf, const void *src, size_t off, si
     if (rc) return rc;
     if (!src && bytes) return tpt_set_error(TPT_EINVAL, "null source");
     if (!bytes) return TPT_OK;
+    ctx->maps_fresh = false;
     TPT_HIP_CHECK(hipMemcpyAsync((char *)ctx->buf[buf] + off, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     TPT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     if (buf == TPT_BUF_ELEM_PROPS || buf == TPT_BUF_MOVE_LUT) ctx->tables_set = true;
@@ -221,6 +225,7 @@ extern "C" int tpt_set_slot_bound(tpt_ctx *ctx, uint32_t bound) {
     if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
     if (bound < 1) bound = 1;
     if (bound > ctx->cfg.capacity) bound = ctx->cfg.capacity;
+    if (bound != ctx->slot_bound) ctx->maps_fresh = false;
     ctx->slot_bound = bound;
     return TPT_OK;
 }

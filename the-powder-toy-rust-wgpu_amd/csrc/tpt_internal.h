@@ -15,6 +15,11 @@
 
 #define TPT_CELL 4
 
+// Slots of the small device counter block (tpt_ctx::counters).
+#define TPT_CTR_INVIS 0   // INVIS/FILT placement queue length
+#define TPT_CTR_SAW 1     // deferred WOOD->SAWD list length
+#define TPT_CTR_OFFMAP 2  // off-map list length (see spatial.hip)
+
 // Passes that carry their own timing slot (see tpt_get_pass_times).
 enum tpt_pass {
     PASS_AIR_PV = 0,      // pressure + velocity, fused
@@ -78,6 +83,17 @@ struct tpt_ctx {
     uint32_t *slot_scratch_u2 = nullptr;  // capacity words
     float *slot_scratch_f = nullptr;      // capacity words
     uint32_t *counters = nullptr;         // small device counters
+    // Pixel-major view of the maps built by RecalcFreeParticles: one 64-bit
+    // record per pixel, ((slot+1)<<9 | flag<<8 | type) << 32 | temp bits,
+    // the off-map slot list, and the conduction results per pixel
+    // (equilibrium temperature bits, insulator/tmp flags).
+    uint64_t *pix_rec = nullptr;          // pixels
+    uint2 *pix_heat = nullptr;            // pixels
+    uint32_t *offmap = nullptr;           // capacity words
+    // True while pix_rec / offmap describe the current particle state: set
+    // by RecalcFreeParticles, cleared by anything that may change slots,
+    // positions, temperatures or the maps afterwards.
+    bool maps_fresh = false;
     tpt_fft_state fft;
 
     // Profiling: a ring of event pairs per pass, folded lazily so that
