@@ -4,8 +4,9 @@
 //
 // The reference transforms the whole zero-padded W x H grid with one launch
 // per radix-2 stage (22 full-grid passes per 2-D transform at 1024^2, three
-// transforms per tick).  Here every 1-D transform runs inside LDS and the
-// solve is three launches:
+// transforms per tick).  Here every 1-D transform is a Stockham radix-4
+// transform (one radix-2 stage for odd lengths) inside LDS, with the
+// twiddle table staged in LDS, and the solve is three launches:
 //
 //   grav_rows_fwd   rows 0..ycells-1 only (the other rows of the padded mass
 //                   grid are zero, so is their transform): mass -> float,
@@ -29,125 +30,199 @@
 namespace {
 
 constexpr int FFT_THREADS = 256;
-constexpr int COL_TILE_ELEMS = 8192;  // complex values per column tile: 64 KiB of LDS
-constexpr int MAX_EPT = COL_TILE_ELEMS / FFT_THREADS;
+// Complex values one FFT workgroup holds in LDS: columns per workgroup in the
+// column pass (and sequences in the row passes) x transform length.
+constexpr int FFT_ELEMS = 4096;
+constexpr int R4_PER_THREAD = FFT_ELEMS / 4 / FFT_THREADS;  // radix-4 butterflies per thread
+constexpr int R2_PER_THREAD = FFT_ELEMS / 2 / FFT_THREADS;  // radix-2 butterflies per thread
+constexpr int EPT = FFT_ELEMS / FFT_THREADS;                // elements per thread
 constexpr float M_GRAV = 6.673e-1f;
 constexpr int GRAV_BLOCK = 256;
 
-__device__ __forceinline__ uint32_t bitrev(uint32_t i, int bits) {
-    return bits == 0 ? 0u : (__brev(i) >> (32 - bits));
+// exp(-2 pi i e / N) for e in [0, N) from the half table tab[t] =
+// exp(-2 pi i t / N), t < N/2 (the upper half is its negation); conjugated
+// for the inverse transform (sign = -1).
+__device__ __forceinline__ float2 twiddle(const float2 *tab, int e, int half, float sign) {
+    float2 w = e < half ? tab[e] : make_float2(-tab[e - half].x, -tab[e - half].y);
+    w.y *= sign;
+    return w;
 }
 
-// In-place radix-2 DIT transform of `cols` interleaved sequences of length n
-// (element i of sequence c at buf[i * cols + c]); input in bit-reversed order.
-__device__ void lds_fft(float2 *buf, int n, int logn, int cols, const float2 *__restrict__ tw, int tw_bits,
-                        float sign) {
-    const int butterflies = (n >> 1) * cols;
-    for (int s = 1; s <= logn; ++s) {
-        const int half = 1 << (s - 1);
-        const int shift = tw_bits - s;
-        for (int b = threadIdx.x; b < butterflies; b += blockDim.x) {
-            const int c = b % cols, k = b / cols;
-            const int pos = k & (half - 1);
-            const int i0 = ((k >> (s - 1)) << s) + pos, i1 = i0 + half;
-            float2 w = tw[pos << shift];
-            w.y *= sign;
-            const float2 x = buf[i0 * cols + c], y = buf[i1 * cols + c];
-            const float tr = w.x * y.x - w.y * y.y;
-            const float ti = w.x * y.y + w.y * y.x;
-            buf[i0 * cols + c] = make_float2(x.x + tr, x.y + ti);
-            buf[i1 * cols + c] = make_float2(x.x - tr, x.y - ti);
+__device__ __forceinline__ float2 cmul(float2 a, float2 w) {
+    return make_float2(w.x * a.x - w.y * a.y, w.x * a.y + w.y * a.x);
+}
+
+// Copies the half twiddle table for length n = 2^logn out of the global
+// table for length 2^tw_bits.
+__device__ void load_twiddles(float2 *tab, const float2 *__restrict__ tw, int logn, int tw_bits) {
+    const int half = (1 << logn) >> 1, shift = tw_bits - logn;
+    for (int t = threadIdx.x; t < half; t += blockDim.x) tab[t] = tw[t << shift];
+}
+
+// In-place Stockham FFT of C = 2^logc interleaved sequences of length
+// n = 2^logn held in LDS (element i of sequence c at buf[i * C + c]), natural
+// order in and out: radix-4 stages, then one radix-2 stage when logn is odd.
+// Each stage reads its butterflies' inputs into registers, waits, and writes
+// the outputs where the next stage expects them.  C * n <= FFT_ELEMS.
+__device__ void lds_fft(float2 *buf, const float2 *tab, int logn, int logc, float sign) {
+    const int n = 1 << logn, C = 1 << logc, half = n >> 1;
+    int s = 0;
+    for (; s + 2 <= logn; s += 2) {
+        const int L = 1 << s, quarter = (n >> 2) * C, nb = quarter;
+        float2 y[R4_PER_THREAD][4];
+#pragma unroll
+        for (int u = 0; u < R4_PER_THREAD; ++u) {
+            const int b = threadIdx.x + u * FFT_THREADS;
+            if (b < nb) {
+                const int k = (b >> logc) & (L - 1);
+                const float2 a0 = buf[b], a1r = buf[b + quarter], a2r = buf[b + 2 * quarter],
+                             a3r = buf[b + 3 * quarter];
+                const int e = k << (logn - s - 2);
+                const float2 a1 = cmul(a1r, twiddle(tab, e, half, sign));
+                const float2 a2 = cmul(a2r, twiddle(tab, 2 * e, half, sign));
+                const float2 a3 = cmul(a3r, twiddle(tab, 3 * e, half, sign));
+                const float2 t0 = make_float2(a0.x + a2.x, a0.y + a2.y), t1 = make_float2(a0.x - a2.x, a0.y - a2.y);
+                const float2 t2 = make_float2(a1.x + a3.x, a1.y + a3.y), t3 = make_float2(a1.x - a3.x, a1.y - a3.y);
+                y[u][0] = make_float2(t0.x + t2.x, t0.y + t2.y);
+                y[u][2] = make_float2(t0.x - t2.x, t0.y - t2.y);
+                y[u][1] = make_float2(t1.x + sign * t3.y, t1.y - sign * t3.x);
+                y[u][3] = make_float2(t1.x - sign * t3.y, t1.y + sign * t3.x);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < R4_PER_THREAD; ++u) {
+            const int b = threadIdx.x + u * FFT_THREADS;
+            if (b < nb) {
+                const int c = b & (C - 1), j = b >> logc, k = j & (L - 1);
+                const int base = (((j - k) << 2) + k) * C + c;
+                buf[base] = y[u][0];
+                buf[base + L * C] = y[u][1];
+                buf[base + 2 * L * C] = y[u][2];
+                buf[base + 3 * L * C] = y[u][3];
+            }
+        }
+        __syncthreads();
+    }
+    if (s < logn) {
+        const int L = 1 << s, nb = half * C;
+        float2 y[R2_PER_THREAD][2];
+#pragma unroll
+        for (int u = 0; u < R2_PER_THREAD; ++u) {
+            const int b = threadIdx.x + u * FFT_THREADS;
+            if (b < nb) {
+                const int k = (b >> logc) & (L - 1);
+                const float2 a0 = buf[b];
+                const float2 a1 = cmul(buf[b + nb], twiddle(tab, k << (logn - s - 1), half, sign));
+                y[u][0] = make_float2(a0.x + a1.x, a0.y + a1.y);
+                y[u][1] = make_float2(a0.x - a1.x, a0.y - a1.y);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < R2_PER_THREAD; ++u) {
+            const int b = threadIdx.x + u * FFT_THREADS;
+            if (b < nb) {
+                const int c = b & (C - 1), j = b >> logc, k = j & (L - 1);
+                const int base = (((j - k) << 1) + k) * C + c;
+                buf[base] = y[u][0];
+                buf[base + L * C] = y[u][1];
+            }
         }
         __syncthreads();
     }
 }
 
 struct FftDims {
-    int xc, yc, W, H, logW, logH, tw_bits, cols;
+    int xc, yc, W, H, logW, logH, tw_bits, logcols;
 };
 
-// build_padded_mass_kernel (fft.rs:153-178) + forward row transform.
+// build_padded_mass_kernel (fft.rs:153-178) + forward row transform, one
+// row per workgroup.
 __global__ __launch_bounds__(FFT_THREADS) void grav_rows_fwd(const int32_t *__restrict__ mass,
                                                              const uint32_t *__restrict__ mask,
                                                              float2 *__restrict__ work,
                                                              const float2 *__restrict__ tw, FftDims d) {
-    extern __shared__ float2 sm[];
+    __shared__ float2 sm[FFT_ELEMS];
+    __shared__ float2 tab[FFT_ELEMS / 2];
     const int row = blockIdx.x;
-    for (int i = threadIdx.x; i < d.W; i += blockDim.x) {
+    load_twiddles(tab, tw, d.logW, d.tw_bits);
+    for (int i = threadIdx.x; i < d.W; i += FFT_THREADS) {
         float v = 0.0f;
         if (i < d.xc) {
             const int c = row * d.xc + i;
             if (mask[c] != 0u) v = (float)mass[c] * (1.0f / 65536.0f);
         }
-        sm[bitrev(i, d.logW)] = make_float2(v, 0.0f);
+        sm[i] = make_float2(v, 0.0f);
     }
     __syncthreads();
-    lds_fft(sm, d.W, d.logW, 1, tw, d.tw_bits, 1.0f);
-    for (int i = threadIdx.x; i < d.W; i += blockDim.x) work[(size_t)row * d.W + i] = sm[i];
+    lds_fft(sm, tab, d.logW, 0, 1.0f);
+    for (int i = threadIdx.x; i < d.W; i += FFT_THREADS) work[(size_t)row * d.W + i] = sm[i];
 }
 
 // Forward column transform, products with both kernel spectra
-// (complex_mul_kernel, fft.rs:129-140) and both inverse column transforms.
+// (complex_mul_kernel, fft.rs:129-140) and both inverse column transforms,
+// for 2^logcols adjacent columns per workgroup.  Only rows < rows_in of the
+// input are non-zero and only rows < rows_out of the outputs are stored.
 __global__ __launch_bounds__(FFT_THREADS) void grav_cols(const float2 *__restrict__ work,
                                                          const float2 *__restrict__ kx,
                                                          const float2 *__restrict__ ky, float2 *__restrict__ sx,
                                                          float2 *__restrict__ sy, const float2 *__restrict__ tw,
                                                          FftDims d, int rows_in, int rows_out) {
-    extern __shared__ float2 sm[];
-    const int cols = d.cols, c0 = blockIdx.x * cols, total = d.H * cols;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int r = e / cols, c = e % cols;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (r < rows_in) v = work[(size_t)r * d.W + c0 + c];
-        sm[bitrev(r, d.logH) * cols + c] = v;
+    __shared__ float2 sm[FFT_ELEMS];
+    __shared__ float2 tab[FFT_ELEMS / 2];
+    const int logc = d.logcols, cols = 1 << logc, c0 = blockIdx.x * cols, total = d.H << logc;
+    load_twiddles(tab, tw, d.logH, d.tw_bits);
+    for (int e = threadIdx.x; e < total; e += FFT_THREADS) {
+        const int r = e >> logc, c = e & (cols - 1);
+        sm[e] = r < rows_in ? work[(size_t)r * d.W + c0 + c] : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
-    lds_fft(sm, d.H, d.logH, cols, tw, d.tw_bits, 1.0f);
-    float2 reg[MAX_EPT];
+    lds_fft(sm, tab, d.logH, logc, 1.0f);
+    float2 spec[EPT];
 #pragma unroll
-    for (int j = 0; j < MAX_EPT; ++j) {
+    for (int j = 0; j < EPT; ++j) {
         const int e = threadIdx.x + j * FFT_THREADS;
-        if (e < total) reg[j] = sm[e];
+        if (e < total) spec[j] = sm[e];
     }
     for (int pass = 0; pass < 2; ++pass) {
         const float2 *__restrict__ K = pass ? ky : kx;
         float2 *__restrict__ out = pass ? sy : sx;
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < MAX_EPT; ++j) {
+        for (int j = 0; j < EPT; ++j) {
             const int e = threadIdx.x + j * FFT_THREADS;
             if (e < total) {
-                const int r = e / cols, c = e % cols;
-                const float2 k = K[(size_t)r * d.W + c0 + c];
-                const float2 a = reg[j];
-                sm[bitrev(r, d.logH) * cols + c] = make_float2(a.x * k.x - a.y * k.y, a.x * k.y + a.y * k.x);
+                const float2 k = K[(size_t)(e >> logc) * d.W + c0 + (e & (cols - 1))];
+                const float2 a = spec[j];
+                sm[e] = make_float2(a.x * k.x - a.y * k.y, a.x * k.y + a.y * k.x);
             }
         }
         __syncthreads();
-        lds_fft(sm, d.H, d.logH, cols, tw, d.tw_bits, -1.0f);
-        for (int e = threadIdx.x; e < rows_out * cols; e += blockDim.x) {
-            const int r = e / cols, c = e % cols;
-            out[(size_t)r * d.W + c0 + c] = sm[e];
-        }
+        lds_fft(sm, tab, d.logH, logc, -1.0f);
+        for (int e = threadIdx.x; e < (rows_out << logc); e += FFT_THREADS)
+            out[(size_t)(e >> logc) * d.W + c0 + (e & (cols - 1))] = sm[e];
     }
 }
 
-// Inverse row transforms + extract_force_kernel (fft.rs:182-200).
+// Inverse row transforms of both spectra (two interleaved sequences) +
+// extract_force_kernel (fft.rs:182-200), one row per workgroup.
 __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__restrict__ sx,
                                                              const float2 *__restrict__ sy,
                                                              const uint32_t *__restrict__ mask,
                                                              float *__restrict__ fx, float *__restrict__ fy,
                                                              const float2 *__restrict__ tw, FftDims d) {
-    extern __shared__ float2 sm[];
+    __shared__ float2 sm[FFT_ELEMS];
+    __shared__ float2 tab[FFT_ELEMS / 2];
     const int row = blockIdx.x;
-    for (int i = threadIdx.x; i < d.W; i += blockDim.x) {
-        const uint32_t r = bitrev(i, d.logW);
-        sm[r * 2 + 0] = sx[(size_t)row * d.W + i];
-        sm[r * 2 + 1] = sy[(size_t)row * d.W + i];
+    load_twiddles(tab, tw, d.logW, d.tw_bits);
+    for (int i = threadIdx.x; i < d.W; i += FFT_THREADS) {
+        sm[i * 2 + 0] = sx[(size_t)row * d.W + i];
+        sm[i * 2 + 1] = sy[(size_t)row * d.W + i];
     }
     __syncthreads();
-    lds_fft(sm, d.W, d.logW, 2, tw, d.tw_bits, -1.0f);
-    for (int i = threadIdx.x; i < d.xc; i += blockDim.x) {
+    lds_fft(sm, tab, d.logW, 1, -1.0f);
+    for (int i = threadIdx.x; i < d.xc; i += FFT_THREADS) {
         const int c = row * d.xc + i;
         const bool on = mask[c] != 0u;
         fx[c] = on ? sm[i * 2 + 0].x : 0.0f;
@@ -176,28 +251,28 @@ __global__ void grav_kernel_field(float2 *__restrict__ kx, float2 *__restrict__ 
 
 __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inplace(float2 *__restrict__ data,
                                                                  const float2 *__restrict__ tw, FftDims d) {
-    extern __shared__ float2 sm[];
+    __shared__ float2 sm[FFT_ELEMS];
+    __shared__ float2 tab[FFT_ELEMS / 2];
     const size_t base = (size_t)blockIdx.x * d.W;
-    for (int i = threadIdx.x; i < d.W; i += blockDim.x) sm[bitrev(i, d.logW)] = data[base + i];
+    load_twiddles(tab, tw, d.logW, d.tw_bits);
+    for (int i = threadIdx.x; i < d.W; i += FFT_THREADS) sm[i] = data[base + i];
     __syncthreads();
-    lds_fft(sm, d.W, d.logW, 1, tw, d.tw_bits, 1.0f);
-    for (int i = threadIdx.x; i < d.W; i += blockDim.x) data[base + i] = sm[i];
+    lds_fft(sm, tab, d.logW, 0, 1.0f);
+    for (int i = threadIdx.x; i < d.W; i += FFT_THREADS) data[base + i] = sm[i];
 }
 
 __global__ __launch_bounds__(FFT_THREADS) void grav_cols_inplace(float2 *__restrict__ data,
                                                                  const float2 *__restrict__ tw, FftDims d) {
-    extern __shared__ float2 sm[];
-    const int cols = d.cols, c0 = blockIdx.x * cols, total = d.H * cols;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int r = e / cols, c = e % cols;
-        sm[bitrev(r, d.logH) * cols + c] = data[(size_t)r * d.W + c0 + c];
-    }
+    __shared__ float2 sm[FFT_ELEMS];
+    __shared__ float2 tab[FFT_ELEMS / 2];
+    const int logc = d.logcols, cols = 1 << logc, c0 = blockIdx.x * cols, total = d.H << logc;
+    load_twiddles(tab, tw, d.logH, d.tw_bits);
+    for (int e = threadIdx.x; e < total; e += FFT_THREADS)
+        sm[e] = data[(size_t)(e >> logc) * d.W + c0 + (e & (cols - 1))];
     __syncthreads();
-    lds_fft(sm, d.H, d.logH, cols, tw, d.tw_bits, 1.0f);
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int r = e / cols, c = e % cols;
-        data[(size_t)r * d.W + c0 + c] = sm[e];
-    }
+    lds_fft(sm, tab, d.logH, logc, 1.0f);
+    for (int e = threadIdx.x; e < total; e += FFT_THREADS)
+        data[(size_t)(e >> logc) * d.W + c0 + (e & (cols - 1))] = sm[e];
 }
 
 // apply_gravity_kernel (gravity_kernel.rs:434-507), NewtonianGravity = 1.
@@ -323,10 +398,11 @@ FftDims dims_of(const tpt_ctx *ctx) {
     d.logW = ilog2(ctx->fft.width);
     d.logH = ilog2(ctx->fft.height);
     d.tw_bits = ilog2(ctx->fft.width > ctx->fft.height ? ctx->fft.width : ctx->fft.height);
-    int cols = COL_TILE_ELEMS / d.H;
-    if (cols < 1) cols = 1;
-    if (cols > d.W) cols = d.W;
-    d.cols = cols;
+    // Columns per workgroup: FFT_ELEMS values, at least one column, at
+    // most the width.
+    int logc = ilog2((uint32_t)(FFT_ELEMS / d.H));
+    if (logc > d.logW) logc = d.logW;
+    d.logcols = logc;
     return d;
 }
 
@@ -366,10 +442,9 @@ int ensure_fft(tpt_ctx *ctx) {
     hipLaunchKernelGGL(grav_kernel_field, dim3((grid + 255) / 256), dim3(256), 0, ctx->stream, f.kernel_x,
                        f.kernel_y, d, scale);
     for (float2 *k : {f.kernel_x, f.kernel_y}) {
-        hipLaunchKernelGGL(grav_rows_inplace, dim3(f.height), dim3(FFT_THREADS), f.width * sizeof(float2),
-                           ctx->stream, k, f.twiddle, d);
-        hipLaunchKernelGGL(grav_cols_inplace, dim3(f.width / d.cols), dim3(FFT_THREADS),
-                           (size_t)d.H * d.cols * sizeof(float2), ctx->stream, k, f.twiddle, d);
+        hipLaunchKernelGGL(grav_rows_inplace, dim3(f.height), dim3(FFT_THREADS), 0, ctx->stream, k, f.twiddle, d);
+        hipLaunchKernelGGL(grav_cols_inplace, dim3(f.width >> d.logcols), dim3(FFT_THREADS), 0, ctx->stream, k,
+                           f.twiddle, d);
     }
     int rc = tpt_check_launch("gravity kernel spectra");
     if (rc) return rc;
@@ -383,16 +458,15 @@ int solve(tpt_ctx *ctx) {
     const tpt_fft_state &f = ctx->fft;
     const FftDims d = dims_of(ctx);
     tpt_pass_begin(ctx, PASS_GRAV_FFT_ROWS);
-    hipLaunchKernelGGL(grav_rows_fwd, dim3(d.yc), dim3(FFT_THREADS), d.W * sizeof(float2), ctx->stream,
+    hipLaunchKernelGGL(grav_rows_fwd, dim3(d.yc), dim3(FFT_THREADS), 0, ctx->stream,
                        tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_PREV), tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK),
                        f.work, f.twiddle, d);
     tpt_pass_end(ctx, PASS_GRAV_FFT_ROWS);
     tpt_pass_begin(ctx, PASS_GRAV_FFT_COLS);
-    hipLaunchKernelGGL(grav_cols, dim3(d.W / d.cols), dim3(FFT_THREADS), (size_t)d.H * d.cols * sizeof(float2),
-                       ctx->stream, f.work, f.kernel_x, f.kernel_y, f.spec_x, f.spec_y, f.twiddle, d, d.yc, d.yc);
+    hipLaunchKernelGGL(grav_cols, dim3(d.W >> d.logcols), dim3(FFT_THREADS), 0, ctx->stream, f.work, f.kernel_x, f.kernel_y, f.spec_x, f.spec_y, f.twiddle, d, d.yc, d.yc);
     tpt_pass_end(ctx, PASS_GRAV_FFT_COLS);
     tpt_pass_begin(ctx, PASS_GRAV_FFT_INV);
-    hipLaunchKernelGGL(grav_rows_inv, dim3(d.yc), dim3(FFT_THREADS), 2 * d.W * sizeof(float2), ctx->stream,
+    hipLaunchKernelGGL(grav_rows_inv, dim3(d.yc), dim3(FFT_THREADS), 0, ctx->stream,
                        f.spec_x, f.spec_y, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK),
                        tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), tpt_buf<float>(ctx, TPT_BUF_GRAV_FY), f.twiddle, d);
     tpt_pass_end(ctx, PASS_GRAV_FFT_INV);

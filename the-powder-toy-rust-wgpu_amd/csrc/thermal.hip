@@ -279,8 +279,10 @@ __global__ __launch_bounds__(256) void th_conduct_px(Parts p, const unsigned lon
                                                      uint32_t aheat, uint32_t seed) {
     __shared__ uint32_t skey[LH][LW];
     __shared__ float stemp[LH][LW];
+    __shared__ uint32_t shc[256];  // HeatConduct per type
     const int tx = threadIdx.x, row = threadIdx.y, t = row * TW + tx;
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    shc[t] = EP(props, (uint32_t)t, COL_HEAT_CONDUCT);
     for (int e = t; e < LH * LW; e += TW * 4) {
         const int ly = e / LW, lx = e - ly * LW;
         const int gx = x0 + lx - 1, gy = y0 + ly - 1;
@@ -310,7 +312,7 @@ __global__ __launch_bounds__(256) void th_conduct_px(Parts p, const unsigned lon
                 tmp = p.tmp[s];
                 ctype = p.ctype[s];
             }
-            const uint32_t hcond = EP(props, ty, COL_HEAT_CONDUCT);
+            const uint32_t hcond = shc[ty];
             const bool ins = is_heat_insulator(ty, life, tmp, hcond);
             out.y = (ins ? 1u : 0u) | (tmp == 1 ? 2u : 0u);
             if (key & 0x100u) {
@@ -354,7 +356,7 @@ __global__ __launch_bounds__(256) void th_conduct_px(Parts p, const unsigned lon
                                 ntmp = p.tmp[nid];
                                 nctype = p.ctype[nid];
                             }
-                            if (is_heat_insulator(nty, nlife, ntmp, EP(props, nty, COL_HEAT_CONDUCT))) continue;
+                            if (is_heat_insulator(nty, nlife, ntmp, shc[nty])) continue;
                             if (pair_excluded(ty, nty, tmp, ntmp)) continue;
                             const float n_hc = heat_capacity(nty, nctype);
                             c_heat += stemp[ly + ny - 1][lx + nx - 1] * n_hc;
@@ -410,18 +412,21 @@ __global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long 
     __shared__ uint32_t skey[LH][LW];
     __shared__ float seq[LH][LW];
     __shared__ uint32_t sinfo[LH][LW];
+    __shared__ float sown[TH][TW];  // record temperatures of the tile itself
     const int tx = threadIdx.x, row = threadIdx.y, t = row * TW + tx;
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
     for (int e = t; e < LH * LW; e += TW * 4) {
         const int ly = e / LW, lx = e - ly * LW;
         const int gx = x0 + lx - 1, gy = y0 + ly - 1;
-        uint32_t key = 0u;
+        unsigned long long v = 0ull;
         uint2 h = make_uint2(__float_as_uint(-1.0f), 0u);
         if (gx >= 0 && gy >= 0 && gx < d.xres && gy < d.yres) {
             const size_t pix = (size_t)gy * d.xres + gx;
-            key = (uint32_t)(rec[pix] >> 32);
-            if (key) h = heat[pix];
+            v = rec[pix];
+            if (v >> 32) h = heat[pix];
         }
+        const uint32_t key = (uint32_t)(v >> 32);
+        if (ly >= 1 && ly <= TH && lx >= 1 && lx <= TW) sown[ly - 1][lx - 1] = __uint_as_float((uint32_t)v);
         skey[ly][lx] = key;
         seq[ly][lx] = __uint_as_float(h.x);
         sinfo[ly][lx] = h.y;
@@ -436,7 +441,7 @@ __global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long 
         if (key == 0u || (key & 0x100u)) continue;
         const uint32_t j = (key >> 9) - 1u, jty = key & 0xFFu;
         if (j >= d.slots) continue;
-        const float own = __uint_as_float((uint32_t)rec[(size_t)y * d.xres + x]);
+        const float own = sown[ly - 1][tx];
         uint32_t best = j;
         float best_temp = own;
         bool has = false;
