@@ -40,6 +40,10 @@ constexpr uint32_t LUT_CAN = 256u * 12u;
 
 struct MoveEnv {
     const uint32_t *__restrict__ pmap;
+    // pmap's type bits as bytes (types are below 256), built just before
+    // the intent kernel: the planner's probes mostly need only the
+    // occupant's type, and the byte map is a quarter of pmap's footprint.
+    const uint8_t *__restrict__ tmap;
     const uint32_t *__restrict__ wall;
     const uint32_t *__restrict__ emap;
     const uint32_t *__restrict__ lut;
@@ -86,7 +90,7 @@ __device__ __forceinline__ int wcell(const MoveEnv &e, int x, int y) { return (y
 // eval_move_at (mod.rs:838-891).
 __device__ uint32_t eval_move_at(const MoveEnv &e, uint32_t props, uint32_t ty, int nx, int ny) {
     if (!(nx >= 0 && nx < e.xr && ny >= 0 && ny < e.yr)) return 0u;
-    const uint32_t d_ty = e.pmap[ny * e.xr + nx] & 0x1FFu;
+    const uint32_t d_ty = (uint32_t)e.tmap[ny * e.xr + nx];
     const int wc = wcell(e, nx, ny);
     const uint32_t wall = e.wall[wc], em = e.emap[wc];
     uint32_t can = e.lut[LUT_CAN + ty * 256u + d_ty];
@@ -145,6 +149,7 @@ __device__ uint32_t do_move_at(const MoveEnv &e, uint32_t props, uint32_t ty, in
 
 __device__ void saw_wood_at(const MoveEnv &e, int nx, int ny, float vx, float vy) {
     if (nx >= 0 && ny >= 0 && nx < e.xr && ny < e.yr) {
+        if (e.tmap[ny * e.xr + nx] != 15u) return;
         const uint32_t v = e.pmap[ny * e.xr + nx];
         if (v >= 512u && (v & 0x1FFu) == 15u && vx * vx + vy * vy > 25.0f)
             e.saw_list[atomicAdd(e.saw_count, 1u)] = (v >> 9) - 1u;
@@ -156,14 +161,13 @@ __device__ __forceinline__ uint32_t pix(const MoveEnv &e, int x, int y) { return
 // plan_sample_blocked (liquid.rs:80-118).
 __device__ bool plan_sample_blocked(const MoveEnv &e, uint32_t props, uint32_t ty, int nx, int ny, bool closed_start) {
     if (nx < 0 || ny < 0 || nx >= e.xr || ny >= e.yr) return true;
-    const uint32_t packed = e.pmap[ny * e.xr + nx];
-    const uint32_t d_ty = packed & 0x1FFu;
+    const uint32_t d_ty = e.tmap[ny * e.xr + nx];
     const uint32_t raw_can = e.lut[LUT_CAN + ty * 256u + d_ty];
     const uint32_t ev = eval_move_at(e, props, ty, nx, ny);
     const int wc = wcell(e, nx, ny);
     const uint32_t wall = e.wall[wc];
     const bool closed_now = wall == 12u && (e.emap[wc] & 0xFFu) == 0u;
-    return ev == 0u || (raw_can == 3u && ev == 1u) || (ty == 29u && packed != 0u) || wall == 7u ||
+    return ev == 0u || (raw_can == 3u && ev == 1u) || (ty == 29u && d_ty != 0u) || wall == 7u ||
            closed_now != closed_start;
 }
 
@@ -296,7 +300,7 @@ __device__ uint32_t scan_liquid_vertical(const MoveEnv &e, int xi, int yi, int f
             stop = true;
         } else {
             if (fin_y >= 0 && fin_y < e.yr) {
-                const uint32_t d_ty = e.pmap[pix(e, j, fin_y)] & 0x1FFu;
+                const uint32_t d_ty = (uint32_t)e.tmap[pix(e, j, fin_y)];
                 if (d_ty != ty || e.wall[wcell(e, j, fin_y)] != 0u) {
                     int ox, oy;
                     const uint32_t d = do_move_at(e, props, ty, j, fin_y, ox, oy);
@@ -313,7 +317,7 @@ __device__ uint32_t scan_liquid_vertical(const MoveEnv &e, int xi, int yi, int f
                 }
             }
             if (!stop && fin_y != clear_y && clear_y >= 0 && clear_y < e.yr) {
-                const uint32_t d_ty = e.pmap[pix(e, j, clear_y)] & 0x1FFu;
+                const uint32_t d_ty = (uint32_t)e.tmap[pix(e, j, clear_y)];
                 if (d_ty != ty || e.wall[wcell(e, j, clear_y)] != 0u) {
                     int ox, oy;
                     const uint32_t d = do_move_at(e, props, ty, j, clear_y, ox, oy);
@@ -330,7 +334,7 @@ __device__ uint32_t scan_liquid_vertical(const MoveEnv &e, int xi, int yi, int f
                 }
             }
             if (!stop && clear_y >= 0 && clear_y < e.yr) {
-                const uint32_t d_ty = e.pmap[pix(e, j, clear_y)] & 0x1FFu;
+                const uint32_t d_ty = (uint32_t)e.tmap[pix(e, j, clear_y)];
                 const uint32_t wall = e.wall[wcell(e, j, clear_y)];
                 if (d_ty != ty || (wall != 0u && wall != 4u)) stop = true;
             }
@@ -356,7 +360,7 @@ __device__ uint32_t scroll_liquid_down(const MoveEnv &e, int scan_x, int scan_y,
         if (jj < 0 || jj >= e.yr || jj < scan_y - rt || jj >= scan_y + rt) {
             stop = true;
         } else {
-            const uint32_t d_ty = e.pmap[pix(e, scan_x, jj)] & 0x1FFu;
+            const uint32_t d_ty = (uint32_t)e.tmap[pix(e, scan_x, jj)];
             const uint32_t wall = e.wall[wcell(e, scan_x, jj)];
             if (d_ty != ty || wall != 0u) {
                 int ox, oy;
@@ -411,7 +415,7 @@ __device__ uint32_t scan_liquid_perpendicular(const MoveEnv &e, float cxf, float
             if (nx < 0 || ny < 0 || nx >= e.xr || ny >= e.yr) {
                 stop = true;
             } else {
-                const uint32_t d_ty = e.pmap[pix(e, nx, ny)] & 0x1FFu;
+                const uint32_t d_ty = (uint32_t)e.tmap[pix(e, nx, ny)];
                 const uint32_t wall = e.wall[wcell(e, nx, ny)];
                 if (d_ty != ty || wall != 0u) {
                     int ox, oy;
@@ -450,7 +454,7 @@ __device__ uint32_t scan_liquid_perpendicular(const MoveEnv &e, float cxf, float
                 if (nnx < 0 || nny < 0 || nnx >= e.xr || nny >= e.yr) {
                     stop2 = true;
                 } else {
-                    const uint32_t d_ty = e.pmap[pix(e, nnx, nny)] & 0x1FFu;
+                    const uint32_t d_ty = (uint32_t)e.tmap[pix(e, nnx, nny)];
                     const uint32_t wall = e.wall[wcell(e, nnx, nny)];
                     if (d_ty != ty || wall != 0u) {
                         int ox, oy;
@@ -514,7 +518,7 @@ __device__ uint32_t fall_down_destination(const MoveEnv &e, float &vx, float &vy
             if (ox2 == 1 && oy2 == 1) continue;
             const int nx = xi + ox2 - 1, ny = yi + oy2 - 1;
             uint32_t nty = 0u;
-            if (nx >= 0 && nx < e.xr && ny >= 0 && ny < e.yr) nty = e.pmap[ny * e.xr + nx] & 0x1FFu;
+            if (nx >= 0 && nx < e.xr && ny >= 0 && ny < e.yr) nty = (uint32_t)e.tmap[ny * e.xr + nx];
             if (nty == 0u) space++;
             if (nty != ty) nt++;
         }
@@ -589,8 +593,8 @@ __device__ uint32_t gas_destination(const MoveEnv &e, float &vx, float &vy, floa
             blocked = true;
             if (ty == 29u) {
                 const uint32_t dest_px = pix(e, ox, oy);
-                const uint32_t dest_ty = e.pmap[dest_px] & 0x1FFu;
-                const uint32_t src_ty = e.pmap[pix(e, xi, yi)] & 0x1FFu;
+                const uint32_t dest_ty = (uint32_t)e.tmap[dest_px];
+                const uint32_t src_ty = (uint32_t)e.tmap[pix(e, xi, yi)];
                 const uint32_t roll = pcg_hash(seed ^ (slot * 0x9E3779B9u) ^ (dest_px * 0x85EBCA6Bu));
                 if ((dest_ty == 46u && (roll & 1u) == 0u) || src_ty == 46u) {
                     blocked = false;
@@ -672,6 +676,89 @@ __global__ void mv_air_deposit_cells(Slots s, const uint32_t *__restrict__ lut, 
     air_vx[c] = air_vx[c] * loss + drag * s.vx[i];
     air_vy[c] = air_vy[c] * loss + drag * s.vy[i];
     key[c] = 0u;
+}
+
+// The same reduction over the pixel records: one thread per cell looks at
+// the owners of its 16 pixels from the highest slot down and stops at the
+// first eligible one; slots that own no pixel (the off-map list) were put
+// into key[] by mv_air_deposit_list first.  A record's pixel is the rounded
+// position the eligibility test uses, and cells within one cell of the
+// border lie inside the 4-pixel margin entirely, as do stasis cells.
+__global__ __launch_bounds__(BLK) void mv_air_deposit_list(Slots s, const uint32_t *__restrict__ wall,
+                                                           const uint32_t *__restrict__ emap,
+                                                           uint32_t *__restrict__ key,
+                                                           const uint32_t *__restrict__ list,
+                                                           const uint32_t *__restrict__ counters, uint32_t slots,
+                                                           int xres, int yres, int xc) {
+    const uint32_t n = counters[TPT_CTR_OFFMAP];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t i = list[k];
+        if (i >= slots) continue;
+        const uint32_t ty = s.ty[i];
+        if (ty == 0u || ty >= 256u) continue;
+        const uint32_t flags = s.flags[i];
+        if ((flags & 0x10u) || (flags & 0x02u)) continue;
+        const int xi = tpt_f2i(s.x[i] + 0.5f), yi = tpt_f2i(s.y[i] + 0.5f);
+        if (xi < 4 || yi < 4 || xi >= xres - 4 || yi >= yres - 4) continue;
+        const int cell = (yi / TPT_CELL) * xc + xi / TPT_CELL;
+        if (wall[cell] == 18u && (emap[cell] & 0xFFu) < 8u) continue;
+        atomicMax(&key[cell], i + 1u);
+    }
+}
+
+__global__ __launch_bounds__(BLK) void mv_air_deposit_px(Slots s, const ulonglong2 *__restrict__ rec,
+                                                         const uint32_t *__restrict__ wall,
+                                                         const uint32_t *__restrict__ emap,
+                                                         const uint32_t *__restrict__ lut, float *__restrict__ air_vx,
+                                                         float *__restrict__ air_vy, uint32_t *__restrict__ key,
+                                                         uint32_t slots, int xres, int xc, int yc) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= xc * yc) return;
+    const int cx = c % xc, cy = c / xc;
+    uint32_t best = key[c];
+    if (best) key[c] = 0u;
+    if (cx >= 1 && cy >= 1 && cx <= xc - 2 && cy <= yc - 2 && !(wall[c] == 18u && (emap[c] & 0xFFu) < 8u)) {
+        uint32_t cand[16];
+        for (int r = 0; r < 4; ++r) {
+            const size_t p2 = ((size_t)(cy * 4 + r) * xres + cx * 4) >> 1;
+            const ulonglong2 a = rec[p2], b2 = rec[p2 + 1];
+            cand[r * 4 + 0] = (uint32_t)(a.x >> 41);
+            cand[r * 4 + 1] = (uint32_t)(a.y >> 41);
+            cand[r * 4 + 2] = (uint32_t)(b2.x >> 41);
+            cand[r * 4 + 3] = (uint32_t)(b2.y >> 41);
+        }
+        // cand = slot + 1 of each pixel's owner (0: none).
+        for (;;) {
+            int m = -1;
+            uint32_t mv = best;
+            for (int q = 0; q < 16; ++q)
+                if (cand[q] > mv) {
+                    mv = cand[q];
+                    m = q;
+                }
+            if (m < 0) break;
+            cand[m] = 0u;
+            const uint32_t i = mv - 1u;
+            if (i >= slots) continue;
+            const uint32_t ty = s.ty[i];
+            if (ty == 0u || ty >= 256u) continue;
+            if (s.flags[i] & 0x12u) continue;
+            best = mv;
+            break;
+        }
+    }
+    if (!best) return;
+    const uint32_t i = best - 1u, base = s.ty[i] * 12u + 4u;
+    const float drag = __uint_as_float(lut[base + 1u]), loss = __uint_as_float(lut[base + 2u]);
+    air_vx[c] = air_vx[c] * loss + drag * s.vx[i];
+    air_vy[c] = air_vy[c] * loss + drag * s.vy[i];
+}
+
+__global__ void mv_type_map(const uint4 *__restrict__ pmap, uint32_t *__restrict__ tmap, uint32_t n4) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n4) return;
+    const uint4 v = pmap[i];
+    tmap[i] = (v.x & 0xFFu) | ((v.y & 0xFFu) << 8) | ((v.z & 0xFFu) << 16) | ((v.w & 0xFFu) << 24);
 }
 
 __global__ void mv_clear(uint4 *__restrict__ winner, uint32_t n4, uint32_t *__restrict__ counters) {
@@ -1001,17 +1088,26 @@ int tpt_movement_step(tpt_ctx *ctx, uint32_t seed) {
     const dim3 pg((n4 + BLK - 1) / BLK);
 
     tpt_pass_begin(ctx, PASS_AIR_DEPOSIT);
-    hipLaunchKernelGGL(mv_air_deposit, sg, b, 0, ctx->stream, s, wall, emap, ctx->cell_scratch_u, slots, xres, yres,
-                       xc);
-    hipLaunchKernelGGL(mv_air_deposit_cells, cg, b, 0, ctx->stream, s, lut, air_vx, air_vy, ctx->cell_scratch_u,
-                       ctx->cells);
+    if (ctx->rec_cover) {
+        hipLaunchKernelGGL(mv_air_deposit_list, dim3(256), b, 0, ctx->stream, s, wall, emap, ctx->cell_scratch_u,
+                           ctx->offmap, ctx->counters, slots, xres, yres, xc);
+        hipLaunchKernelGGL(mv_air_deposit_px, cg, b, 0, ctx->stream, s, (const ulonglong2 *)ctx->pix_rec, wall, emap,
+                           lut, air_vx, air_vy, ctx->cell_scratch_u, slots, xres, xc, (int)ctx->ycells);
+    } else {
+        hipLaunchKernelGGL(mv_air_deposit, sg, b, 0, ctx->stream, s, wall, emap, ctx->cell_scratch_u, slots, xres,
+                           yres, xc);
+        hipLaunchKernelGGL(mv_air_deposit_cells, cg, b, 0, ctx->stream, s, lut, air_vx, air_vy, ctx->cell_scratch_u,
+                           ctx->cells);
+    }
+    ctx->rec_cover = false;
     tpt_pass_end(ctx, PASS_AIR_DEPOSIT);
     tpt_pass_begin(ctx, PASS_MOVE_CLEAR);
     hipLaunchKernelGGL(mv_clear, pg, b, 0, ctx->stream, (uint4 *)winner, n4, ctx->counters);
     tpt_pass_end(ctx, PASS_MOVE_CLEAR);
-    MoveEnv e{pmap, wall, emap, lut, xres, yres, xc, c.edge_mode, c.gravity_mode, c.custom_gravity_x,
-              c.custom_gravity_y, c.newton_grav, ctx->slot_scratch_u2, ctx->counters + 1};
+    MoveEnv e{pmap, ctx->pix_type, wall, emap, lut, xres, yres, xc, c.edge_mode, c.gravity_mode,
+              c.custom_gravity_x, c.custom_gravity_y, c.newton_grav, ctx->slot_scratch_u2, ctx->counters + 1};
     tpt_pass_begin(ctx, PASS_MOVE_INTENT);
+    hipLaunchKernelGGL(mv_type_map, pg, b, 0, ctx->stream, (const uint4 *)pmap, (uint32_t *)ctx->pix_type, n4);
     hipLaunchKernelGGL(mv_intent, sg, b, 0, ctx->stream, s, e, air_vx, air_vy, winner, slots, seed);
     hipLaunchKernelGGL(mv_saw, dim3(16), b, 0, ctx->stream, s.ty, ctx->slot_scratch_u2, ctx->counters);
     tpt_pass_end(ctx, PASS_MOVE_INTENT);

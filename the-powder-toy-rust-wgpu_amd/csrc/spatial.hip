@@ -259,7 +259,7 @@ int tpt_recalc_step(tpt_ctx *ctx, uint32_t seed, int stacking) {
     uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), *photons = tpt_buf<uint32_t>(ctx, TPT_BUF_PHOTONS),
              *count = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP_COUNT);
     unsigned long long *rec = (unsigned long long *)ctx->pix_rec;
-    ctx->maps_fresh = false;
+    ctx->maps_fresh = ctx->rec_cover = false;
     tpt_pass_begin(ctx, PASS_RECALC_CLEAR);
     // P is a multiple of 16 (xres and yres are multiples of 4).
     const uint32_t n4 = P / 4;
@@ -296,6 +296,6 @@ int tpt_recalc_step(tpt_ctx *ctx, uint32_t seed, int stacking) {
         rc = tpt_check_launch("check_stacking");
         if (rc) return rc;
     }
-    ctx->maps_fresh = true;
+    ctx->maps_fresh = ctx->rec_cover = true;
     return TPT_OK;
 }

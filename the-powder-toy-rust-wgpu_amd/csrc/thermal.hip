@@ -202,8 +202,12 @@ __global__ void th_conduct_cells(float *__restrict__ air_hv, uint32_t *__restric
 }
 
 // thermal_apply_kernel (thermal_kernel.rs:616-708).
+// With `heat` set, the neighbours' equilibria come from the per-pixel
+// conduction results (the pixel-major path leaves them there) instead of
+// the slot array.
 __device__ __forceinline__ void apply_slot(const Parts &p, const uint32_t *__restrict__ pmap,
-                                           const uint32_t *__restrict__ props, const Dims &d, uint32_t j) {
+                                           const uint32_t *__restrict__ props, const Dims &d, uint32_t j,
+                                           const uint2 *__restrict__ heat = nullptr) {
     const uint32_t jty = p.ty[j];
     if (jty == 0u || jty >= 256u) return;
     const int x = tpt_f2i(p.x[j]), y = tpt_f2i(p.y[j]);
@@ -227,7 +231,7 @@ __device__ __forceinline__ void apply_slot(const Parts &p, const uint32_t *__res
                 if (nid >= d.slots || nid == j) continue;
                 const uint32_t nty = p.ty[nid];
                 if (nty == 0u || nty >= 256u || !(!has || nid > best)) continue;
-                const float neq = p.equil[nid];
+                const float neq = heat ? __uint_as_float(heat[(y + ny - 1) * d.xres + (x + nx - 1)].x) : p.equil[nid];
                 if (!(neq >= 0.0f)) continue;
                 const int32_t ntmp = p.tmp[nid];
                 if (is_heat_insulator(nty, p.life[nid], ntmp, EP(props, nty, COL_HEAT_CONDUCT))) continue;
@@ -246,20 +250,32 @@ __global__ __launch_bounds__(BLK) void th_apply(Parts p, const uint32_t *__restr
     if (j < d.slots) apply_slot(p, pmap, props, d, j);
 }
 
+// The off-map list's slots; an off-map slot that owns the record of its
+// rounded pixel (a flagged owner) also refreshes the record's temperature.
 __global__ __launch_bounds__(BLK) void th_apply_list(Parts p, const uint32_t *__restrict__ pmap,
                                                      const uint32_t *__restrict__ props,
                                                      const uint32_t *__restrict__ list,
-                                                     const uint32_t *__restrict__ counters, Dims d) {
+                                                     const uint32_t *__restrict__ counters,
+                                                     const uint2 *__restrict__ heat, uint4 *__restrict__ res, Dims d) {
     const uint32_t n = counters[TPT_CTR_OFFMAP];
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t j = list[k];
-        if (j < d.slots) apply_slot(p, pmap, props, d, j);
+        if (j >= d.slots) continue;
+        apply_slot(p, pmap, props, d, j, heat);
+        const int rx = tpt_f2i(p.x[j] + 0.5f), ry = tpt_f2i(p.y[j] + 0.5f);
+        if (rx < 0 || ry < 0 || rx >= d.xres || ry >= d.yres) continue;
+        const size_t pix = (size_t)ry * d.xres + rx;
+        if ((res[pix].x >> 9) == j + 1u) res[pix].z = __float_as_uint(p.temp[j]);
     }
 }
 
 // Pixel-major conduction and apply over the records RecalcFreeParticles
 // left (unflagged record owners only; the rest go through the off-map
-// list).  A 64x16 pixel tile (16x4 cells) per 256-thread block, each thread
+// list).  Results stay pixel-major (heat: equilibrium + flags; res: owner
+// key, equilibrium, new temperature); the slot arrays are written by the
+// slot-major convection pass that follows, which reads each slot's own
+// result with one 16-byte load: a random load costs about half of a random
+// 4-byte store here.  A 64x16 pixel tile (16x4 cells) per 256-thread block, each thread
 // one 4-pixel column of one cell, the tile plus a one-pixel halo in LDS.
 // Each cell's 16 pixels live in 4 adjacent lanes of one wave, so the R2/R3
 // cell results are reduced with lane shuffles and written by the block,
@@ -366,7 +382,6 @@ __global__ __launch_bounds__(256) void th_conduct_px(Parts p, const unsigned lon
                 } else {
                     ++nfail;
                 }
-                p.equil[s] = eq;
                 out.x = __float_as_uint(eq);
             }
         }
@@ -407,8 +422,8 @@ __global__ __launch_bounds__(256) void th_conduct_px(Parts p, const unsigned lon
     }
 }
 
-__global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long long *__restrict__ rec,
-                                                   const uint2 *__restrict__ heat, Dims d) {
+__global__ __launch_bounds__(256) void th_apply_px(const unsigned long long *__restrict__ rec,
+                                                   const uint2 *__restrict__ heat, uint4 *__restrict__ res, Dims d) {
     __shared__ uint32_t skey[LH][LW];
     __shared__ float seq[LH][LW];
     __shared__ uint32_t sinfo[LH][LW];
@@ -438,10 +453,13 @@ __global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long 
         const int ly = row * 4 + k + 1, lx = tx + 1, y = y0 + row * 4 + k;
         if (y >= d.yres) break;
         const uint32_t key = skey[ly][lx];
+        const float own = sown[ly - 1][tx];
+        const size_t pix = (size_t)y * d.xres + x;
+        // Flagged owners keep their record temperature until the list pass.
+        res[pix] = make_uint4(key, __float_as_uint(seq[ly][lx]), __float_as_uint(own), 0u);
         if (key == 0u || (key & 0x100u)) continue;
         const uint32_t j = (key >> 9) - 1u, jty = key & 0xFFu;
         if (j >= d.slots) continue;
-        const float own = sown[ly - 1][tx];
         uint32_t best = j;
         float best_temp = own;
         bool has = false;
@@ -471,7 +489,7 @@ __global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long 
                     has = true;
                 }
         }
-        p.temp[j] = tpt_clampf(has ? best_temp : own, 0.0f, 9999.0f);
+        res[pix].z = __float_as_uint(tpt_clampf(has ? best_temp : own, 0.0f, 9999.0f));
     }
 }
 
@@ -522,6 +540,75 @@ __device__ uint32_t convect_target(const Parts &p, const uint32_t *__restrict__ 
     const uint32_t nid = slot_at(pmap, ox, oy, d);
     if (nid >= d.slots || p.ty[nid] != t || !(p.temp[i] > p.temp[nid])) return NONE;
     return nid;
+}
+
+// Slot-major pass after the pixel-major apply: each slot that owns its own
+// pixel (unflagged owner) takes its equilibrium and new temperature from the
+// per-pixel results and stores them; then convect_target with the partner
+// read from the same results (owner key, type bits, post-apply temperature:
+// one 16-byte load instead of pmap plus two slot-array gathers; types are
+// still those RecalcFreeParticles saw).
+__global__ __launch_bounds__(BLK) void th_convect_propose_px(Parts p, const uint4 *__restrict__ res,
+                                                             const uint32_t *__restrict__ props,
+                                                             const float *__restrict__ gfx,
+                                                             const float *__restrict__ gfy,
+                                                             uint32_t *__restrict__ tgt, uint32_t *__restrict__ touch,
+                                                             Dims d, GravCfg g, uint32_t seed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.slots) return;
+    touch[i] = NONE;
+    uint32_t target = NONE;
+    const uint32_t t = p.ty[i];
+    if (t != 0u && t < 256u) {
+        const int x = tpt_f2i(p.x[i]), y = tpt_f2i(p.y[i]);
+        if (x >= 0 && x < d.xres && y >= 0 && y < d.yres) {
+            const uint4 r = res[(size_t)y * d.xres + x];
+            float own;
+            if ((r.x >> 9) == i + 1u && (r.x & 0x100u) == 0u) {
+                p.equil[i] = __uint_as_float(r.y);
+                p.temp[i] = own = __uint_as_float(r.z);
+            } else {
+                own = p.temp[i];
+            }
+            bool go = (EP(props, t, COL_PROPERTIES) & 0x02u) != 0u;
+            if (go && t == T_GEL) {
+                const uint32_t roll = gpu_hash(seed ^ (i * PARTICLE_HASH_SALT) ^ t ^ TRANSITION_HASH_SALT);
+                const float gel = (float)p.tmp[i] * 2.55f;
+                const uint32_t gate_roll = gpu_hash(roll ^ 0x1B873593u) % 255u;
+                if (gel <= 1.0f + (float)gate_roll) go = false;
+            }
+            if (go) {
+                const int ci = (y / TPT_CELL) * d.xc + x / TPT_CELL;
+                const float pg = -2.0f;
+                float gx = 0.0f, gy = 0.0f;
+                if (g.mode == 0u) {
+                    gy = pg;
+                } else if (g.mode == 2u) {
+                    const float dx = (float)x - (float)d.xres * 0.5f, dy = (float)y - (float)d.yres * 0.5f;
+                    if (dx != 0.0f || dy != 0.0f) {
+                        const float dd = 0.01f - sqrtf(dx * dx + dy * dy);
+                        gx = pg * (dx / dd);
+                        gy = pg * (dy / dd);
+                    }
+                } else if (g.mode == 3u) {
+                    gx = pg * g.cgx;
+                    gy = pg * g.cgy;
+                }
+                gx += -2.0f * gfx[ci];
+                gy += -2.0f * gfy[ci];
+                const float rxf = gx + (float)x, ryf = gy + (float)y;
+                const int rx = rxf >= 0.0f ? tpt_f2i(rxf + 0.5f) : tpt_f2i(rxf - 0.5f);
+                const int ry = ryf >= 0.0f ? tpt_f2i(ryf + 0.5f) : tpt_f2i(ryf - 0.5f);
+                const int ox = min(max(rx, x - 1), x + 1), oy = min(max(ry, y - 1), y + 1);
+                if ((ox != x || oy != y) && ox >= 0 && oy >= 0 && ox < d.xres && oy < d.yres) {
+                    const uint4 q = res[(size_t)oy * d.xres + ox];
+                    const uint32_t nid = (q.x >> 9) - 1u;
+                    if (q.x != 0u && nid < d.slots && (q.x & 0xFFu) == t && own > __uint_as_float(q.z)) target = nid;
+                }
+            }
+        }
+    }
+    tgt[i] = target;
 }
 
 __global__ __launch_bounds__(BLK) void th_convect_propose(Parts p, const uint32_t *__restrict__ pmap,
@@ -575,11 +662,13 @@ __global__ __launch_bounds__(BLK) void th_transition(Parts p, const float *__res
                                                      const uint32_t *__restrict__ wall_emap,
                                                      const uint32_t *__restrict__ props, float *__restrict__ pv_new,
                                                      uint32_t *__restrict__ pv_key, uint32_t *__restrict__ spark,
-                                                     Dims d, uint32_t legacy, uint32_t seed) {
+                                                     uint32_t *__restrict__ pmap_refresh, Dims d, uint32_t legacy,
+                                                     uint32_t seed) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.slots) return;
     uint32_t t = p.ty[i];
     if (t == 0u || t >= 256u) return;
+    const uint32_t t_entry = t;
     const int x = tpt_f2i(p.x[i]), y = tpt_f2i(p.y[i]);
     if (x < 0 || x >= d.xres || y < 0 || y >= d.yres) return;
     const int ci = (y / TPT_CELL) * d.xc + x / TPT_CELL;
@@ -843,6 +932,17 @@ __global__ __launch_bounds__(BLK) void th_transition(Parts p, const float *__res
         pv_new[i] = pv;
         atomicMax(&pv_key[ci], i + 1u);
     }
+    // refresh_pmap_types for this slot, when the maps were built from the
+    // types this pass started with: only a changed type can be stale, and
+    // nothing in this kernel reads pmap.
+    if (pmap_refresh) {
+        const uint32_t tf = p.ty[i];
+        if (tf != t_entry && tf != 0u) {
+            const int idx = y * d.xres + x;
+            const uint32_t packed = pmap_refresh[idx];
+            if ((packed >> 9) == i + 1u && (packed & 0x1FFu) != tf) pmap_refresh[idx] = ((i + 1u) << 9) | (tf & 0x1FFu);
+        }
+    }
 }
 
 __global__ void th_transition_cells(float *__restrict__ air_pv, uint32_t *__restrict__ wall_emap,
@@ -905,8 +1005,9 @@ int tpt_thermal_step(tpt_ctx *ctx, uint32_t seed, int legacy) {
                            ctx->cell_scratch_u, ctx->cell_scratch_u2, ctx->pix_heat, d, aheat, seed);
         tpt_pass_end(ctx, PASS_CONDUCT);
         tpt_pass_begin(ctx, PASS_CONDUCT_APPLY);
-        hipLaunchKernelGGL(th_apply_px, tg, tb, 0, ctx->stream, p, rec, (const uint2 *)ctx->pix_heat, d);
-        hipLaunchKernelGGL(th_apply_list, lg, b, 0, ctx->stream, p, pmap, props, ctx->offmap, ctx->counters, d);
+        hipLaunchKernelGGL(th_apply_px, tg, tb, 0, ctx->stream, rec, (const uint2 *)ctx->pix_heat, ctx->pix_res, d);
+        hipLaunchKernelGGL(th_apply_list, lg, b, 0, ctx->stream, p, pmap, props, ctx->offmap, ctx->counters,
+                           (const uint2 *)ctx->pix_heat, ctx->pix_res, d);
         tpt_pass_end(ctx, PASS_CONDUCT_APPLY);
     } else if (!legacy) {
         tpt_pass_begin(ctx, PASS_CONDUCT);
@@ -926,8 +1027,12 @@ int tpt_thermal_step(tpt_ctx *ctx, uint32_t seed, int legacy) {
     tpt_pass_begin(ctx, PASS_TRANSITION);
     if (!legacy) {
         const GravCfg g{ctx->cfg.gravity_mode, ctx->cfg.custom_gravity_x, ctx->cfg.custom_gravity_y};
-        hipLaunchKernelGGL(th_convect_propose, sg, b, 0, ctx->stream, p, pmap, props, gfx, gfy, ctx->slot_scratch_u,
-                           ctx->slot_scratch_u2, d, g, seed);
+        if (fresh)
+            hipLaunchKernelGGL(th_convect_propose_px, sg, b, 0, ctx->stream, p, (const uint4 *)ctx->pix_res, props,
+                               gfx, gfy, ctx->slot_scratch_u, ctx->slot_scratch_u2, d, g, seed);
+        else
+            hipLaunchKernelGGL(th_convect_propose, sg, b, 0, ctx->stream, p, pmap, props, gfx, gfy,
+                               ctx->slot_scratch_u, ctx->slot_scratch_u2, d, g, seed);
         hipLaunchKernelGGL(th_convect_touch, sg, b, 0, ctx->stream, ctx->slot_scratch_u, ctx->slot_scratch_u2,
                            d.slots);
         hipLaunchKernelGGL(th_convect_swap, sg, b, 0, ctx->stream, ctx->slot_scratch_u, ctx->slot_scratch_u2, p.temp,
@@ -935,15 +1040,19 @@ int tpt_thermal_step(tpt_ctx *ctx, uint32_t seed, int legacy) {
     }
     hipLaunchKernelGGL(th_transition, sg, b, 0, ctx->stream, p, tpt_buf<float>(ctx, TPT_BUF_AIR_PV), gfx, gfy,
                        tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), props,
-                       ctx->slot_scratch_f, ctx->cell_scratch_u, ctx->cell_scratch_u2, d, legacy ? 1u : 0u, seed);
+                       ctx->slot_scratch_f, ctx->cell_scratch_u, ctx->cell_scratch_u2,
+                       fresh ? tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP) : nullptr, d, legacy ? 1u : 0u, seed);
     tpt_pass_end(ctx, PASS_TRANSITION);
     tpt_pass_begin(ctx, PASS_TRANSITION_CELL);
     hipLaunchKernelGGL(th_transition_cells, cg, b, 0, ctx->stream, tpt_buf<float>(ctx, TPT_BUF_AIR_PV),
                        tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), ctx->slot_scratch_f, ctx->cell_scratch_u,
                        ctx->cell_scratch_u2, ctx->cells);
     tpt_pass_end(ctx, PASS_TRANSITION_CELL);
-    tpt_pass_begin(ctx, PASS_REFRESH_PMAP);
-    hipLaunchKernelGGL(th_refresh, sg, b, 0, ctx->stream, p.ty, p.x, p.y, tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), d);
-    tpt_pass_end(ctx, PASS_REFRESH_PMAP);
+    if (!fresh) {
+        tpt_pass_begin(ctx, PASS_REFRESH_PMAP);
+        hipLaunchKernelGGL(th_refresh, sg, b, 0, ctx->stream, p.ty, p.x, p.y, tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP),
+                           d);
+        tpt_pass_end(ctx, PASS_REFRESH_PMAP);
+    }
     return tpt_check_launch("thermal pass");
 }

@@ -94,7 +94,9 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMalloc(&c->slot_scratch_u, cap * 4) != hipSuccess || hipMalloc(&c->slot_scratch_u2, cap * 4) != hipSuccess ||
         hipMalloc(&c->slot_scratch_f, cap * 4) != hipSuccess || hipMalloc(&c->counters, 256) != hipSuccess ||
         hipMalloc(&c->pix_rec, (size_t)c->pixels * 8) != hipSuccess ||
-        hipMalloc(&c->pix_heat, (size_t)c->pixels * 8) != hipSuccess || hipMalloc(&c->offmap, cap * 4) != hipSuccess)
+        hipMalloc(&c->pix_heat, (size_t)c->pixels * 8) != hipSuccess || hipMalloc(&c->offmap, cap * 4) != hipSuccess ||
+        hipMalloc(&c->pix_res, (size_t)c->pixels * 16) != hipSuccess ||
+        hipMalloc(&c->pix_type, (size_t)c->pixels) != hipSuccess)
         return tpt_set_error(TPT_ENOMEM, "scratch allocation failed");
     hipMemsetAsync(c->cell_scratch_u, 0, cb, c->stream);
     hipMemsetAsync(c->cell_scratch_u2, 0, cb, c->stream);
@@ -107,7 +109,7 @@ static void free_all(tpt_ctx *c) {
         if (c->buf[id]) hipFree(c->buf[id]);
     void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
                      c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters,
-                     c->pix_rec, c->pix_heat, c->offmap};
+                     c->pix_rec, c->pix_heat, c->offmap, c->pix_res, c->pix_type};
     for (void *p : extra)
         if (p) hipFree(p);
     tpt_gravity_free(c);
@@ -198,6 +200,7 @@ extern "C" int tpt_upload(tpt_ctx *ctx, int buf, const void *src, size_t off, si
     if (!src && bytes) return tpt_set_error(TPT_EINVAL, "null source");
     if (!bytes) return TPT_OK;
     ctx->maps_fresh = false;
+    ctx->rec_cover = false;
     TPT_HIP_CHECK(hipMemcpyAsync((char *)ctx->buf[buf] + off, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     TPT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     if (buf == TPT_BUF_ELEM_PROPS || buf == TPT_BUF_MOVE_LUT) ctx->tables_set = true;
@@ -225,7 +228,7 @@ extern "C" int tpt_set_slot_bound(tpt_ctx *ctx, uint32_t bound) {
     if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
     if (bound < 1) bound = 1;
     if (bound > ctx->cfg.capacity) bound = ctx->cfg.capacity;
-    if (bound != ctx->slot_bound) ctx->maps_fresh = false;
+    if (bound != ctx->slot_bound) ctx->maps_fresh = ctx->rec_cover = false;
     ctx->slot_bound = bound;
     return TPT_OK;
 }

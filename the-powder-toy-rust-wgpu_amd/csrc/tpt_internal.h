@@ -89,11 +89,18 @@ struct tpt_ctx {
     // (equilibrium temperature bits, insulator/tmp flags).
     uint64_t *pix_rec = nullptr;          // pixels
     uint2 *pix_heat = nullptr;            // pixels
+    uint4 *pix_res = nullptr;             // pixels: owner key, equilibrium, new temperature
+    uint8_t *pix_type = nullptr;          // pixels: pmap type byte for the movement planner
     uint32_t *offmap = nullptr;           // capacity words
     // True while pix_rec / offmap describe the current particle state: set
     // by RecalcFreeParticles, cleared by anything that may change slots,
     // positions, temperatures or the maps afterwards.
     bool maps_fresh = false;
+    // True while every live slot is either the owner of its rounded pixel's
+    // record or on the off-map list, with positions unchanged since
+    // RecalcFreeParticles (types may have changed since): set there,
+    // cleared by movement, uploads and slot-bound changes.
+    bool rec_cover = false;
     tpt_fft_state fft;
 
     // Profiling: a ring of event pairs per pass, folded lazily so that
