@@ -137,14 +137,28 @@ struct FftDims {
     int xc, yc, W, H, logW, logH, tw_bits, logcols;
 };
 
+// Column tile of this workgroup.  Workgroups are dealt to the 8 XCDs
+// round-robin; numbering the tiles so that each XCD gets a contiguous run
+// of columns keeps a row's 64-byte segment within one XCD's L2.
+__device__ __forceinline__ int col_tile(const FftDims &d) {
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    return (nb % 8) ? b : (b % 8) * (nb / 8) + b / 8;
+}
+
+// Dynamic LDS of the FFT kernels: the data plus the half twiddle table.
+static inline size_t rows_lds(const FftDims &d, int seqs) { return ((size_t)d.W * seqs + d.W / 2) * sizeof(float2); }
+static inline size_t cols_lds(const FftDims &d) {
+    return (((size_t)d.H << d.logcols) + d.H / 2) * sizeof(float2);
+}
+
 // build_padded_mass_kernel (fft.rs:153-178) + forward row transform, one
 // row per workgroup.
 __global__ __launch_bounds__(FFT_THREADS) void grav_rows_fwd(const int32_t *__restrict__ mass,
                                                              const uint32_t *__restrict__ mask,
                                                              float2 *__restrict__ work,
                                                              const float2 *__restrict__ tw, FftDims d) {
-    __shared__ float2 sm[FFT_ELEMS];
-    __shared__ float2 tab[FFT_ELEMS / 2];
+    extern __shared__ float2 dyn[];
+    float2 *sm = dyn, *tab = dyn + d.W;
     const int row = blockIdx.x;
     load_twiddles(tab, tw, d.logW, d.tw_bits);
     for (int i = threadIdx.x; i < d.W; i += FFT_THREADS) {
@@ -169,9 +183,9 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_cols(const float2 *__restric
                                                          const float2 *__restrict__ ky, float2 *__restrict__ sx,
                                                          float2 *__restrict__ sy, const float2 *__restrict__ tw,
                                                          FftDims d, int rows_in, int rows_out) {
-    __shared__ float2 sm[FFT_ELEMS];
-    __shared__ float2 tab[FFT_ELEMS / 2];
-    const int logc = d.logcols, cols = 1 << logc, c0 = blockIdx.x * cols, total = d.H << logc;
+    extern __shared__ float2 dyn[];
+    const int logc = d.logcols, cols = 1 << logc, c0 = col_tile(d) * cols, total = d.H << logc;
+    float2 *sm = dyn, *tab = dyn + total;
     load_twiddles(tab, tw, d.logH, d.tw_bits);
     for (int e = threadIdx.x; e < total; e += FFT_THREADS) {
         const int r = e >> logc, c = e & (cols - 1);
@@ -212,8 +226,8 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__res
                                                              const uint32_t *__restrict__ mask,
                                                              float *__restrict__ fx, float *__restrict__ fy,
                                                              const float2 *__restrict__ tw, FftDims d) {
-    __shared__ float2 sm[FFT_ELEMS];
-    __shared__ float2 tab[FFT_ELEMS / 2];
+    extern __shared__ float2 dyn[];
+    float2 *sm = dyn, *tab = dyn + 2 * d.W;
     const int row = blockIdx.x;
     load_twiddles(tab, tw, d.logW, d.tw_bits);
     for (int i = threadIdx.x; i < d.W; i += FFT_THREADS) {
@@ -251,8 +265,8 @@ __global__ void grav_kernel_field(float2 *__restrict__ kx, float2 *__restrict__ 
 
 __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inplace(float2 *__restrict__ data,
                                                                  const float2 *__restrict__ tw, FftDims d) {
-    __shared__ float2 sm[FFT_ELEMS];
-    __shared__ float2 tab[FFT_ELEMS / 2];
+    extern __shared__ float2 dyn[];
+    float2 *sm = dyn, *tab = dyn + d.W;
     const size_t base = (size_t)blockIdx.x * d.W;
     load_twiddles(tab, tw, d.logW, d.tw_bits);
     for (int i = threadIdx.x; i < d.W; i += FFT_THREADS) sm[i] = data[base + i];
@@ -263,9 +277,9 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inplace(float2 *__restr
 
 __global__ __launch_bounds__(FFT_THREADS) void grav_cols_inplace(float2 *__restrict__ data,
                                                                  const float2 *__restrict__ tw, FftDims d) {
-    __shared__ float2 sm[FFT_ELEMS];
-    __shared__ float2 tab[FFT_ELEMS / 2];
-    const int logc = d.logcols, cols = 1 << logc, c0 = blockIdx.x * cols, total = d.H << logc;
+    extern __shared__ float2 dyn[];
+    const int logc = d.logcols, cols = 1 << logc, c0 = col_tile(d) * cols, total = d.H << logc;
+    float2 *sm = dyn, *tab = dyn + total;
     load_twiddles(tab, tw, d.logH, d.tw_bits);
     for (int e = threadIdx.x; e < total; e += FFT_THREADS)
         sm[e] = data[(size_t)(e >> logc) * d.W + c0 + (e & (cols - 1))];
@@ -398,11 +412,10 @@ FftDims dims_of(const tpt_ctx *ctx) {
     d.logW = ilog2(ctx->fft.width);
     d.logH = ilog2(ctx->fft.height);
     d.tw_bits = ilog2(ctx->fft.width > ctx->fft.height ? ctx->fft.width : ctx->fft.height);
-    // Columns per workgroup: FFT_ELEMS values, at least one column, at
-    // most the width.
-    int logc = ilog2((uint32_t)(FFT_ELEMS / d.H));
-    if (logc > d.logW) logc = d.logW;
-    d.logcols = logc;
+    // One column per workgroup: W workgroups (1024 at the largest canvas)
+    // keep four or more in flight per CU, which the latency of the LDS
+    // stages needs more than wider row segments.
+    d.logcols = 0;
     return d;
 }
 
@@ -442,9 +455,10 @@ int ensure_fft(tpt_ctx *ctx) {
     hipLaunchKernelGGL(grav_kernel_field, dim3((grid + 255) / 256), dim3(256), 0, ctx->stream, f.kernel_x,
                        f.kernel_y, d, scale);
     for (float2 *k : {f.kernel_x, f.kernel_y}) {
-        hipLaunchKernelGGL(grav_rows_inplace, dim3(f.height), dim3(FFT_THREADS), 0, ctx->stream, k, f.twiddle, d);
-        hipLaunchKernelGGL(grav_cols_inplace, dim3(f.width >> d.logcols), dim3(FFT_THREADS), 0, ctx->stream, k,
+        hipLaunchKernelGGL(grav_rows_inplace, dim3(f.height), dim3(FFT_THREADS), rows_lds(d, 1), ctx->stream, k,
                            f.twiddle, d);
+        hipLaunchKernelGGL(grav_cols_inplace, dim3(f.width >> d.logcols), dim3(FFT_THREADS), cols_lds(d), ctx->stream,
+                           k, f.twiddle, d);
     }
     int rc = tpt_check_launch("gravity kernel spectra");
     if (rc) return rc;
@@ -458,15 +472,15 @@ int solve(tpt_ctx *ctx) {
     const tpt_fft_state &f = ctx->fft;
     const FftDims d = dims_of(ctx);
     tpt_pass_begin(ctx, PASS_GRAV_FFT_ROWS);
-    hipLaunchKernelGGL(grav_rows_fwd, dim3(d.yc), dim3(FFT_THREADS), 0, ctx->stream,
+    hipLaunchKernelGGL(grav_rows_fwd, dim3(d.yc), dim3(FFT_THREADS), rows_lds(d, 1), ctx->stream,
                        tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_PREV), tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK),
                        f.work, f.twiddle, d);
     tpt_pass_end(ctx, PASS_GRAV_FFT_ROWS);
     tpt_pass_begin(ctx, PASS_GRAV_FFT_COLS);
-    hipLaunchKernelGGL(grav_cols, dim3(d.W >> d.logcols), dim3(FFT_THREADS), 0, ctx->stream, f.work, f.kernel_x, f.kernel_y, f.spec_x, f.spec_y, f.twiddle, d, d.yc, d.yc);
+    hipLaunchKernelGGL(grav_cols, dim3(d.W >> d.logcols), dim3(FFT_THREADS), cols_lds(d), ctx->stream, f.work, f.kernel_x, f.kernel_y, f.spec_x, f.spec_y, f.twiddle, d, d.yc, d.yc);
     tpt_pass_end(ctx, PASS_GRAV_FFT_COLS);
     tpt_pass_begin(ctx, PASS_GRAV_FFT_INV);
-    hipLaunchKernelGGL(grav_rows_inv, dim3(d.yc), dim3(FFT_THREADS), 0, ctx->stream,
+    hipLaunchKernelGGL(grav_rows_inv, dim3(d.yc), dim3(FFT_THREADS), rows_lds(d, 2), ctx->stream,
                        f.spec_x, f.spec_y, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK),
                        tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), tpt_buf<float>(ctx, TPT_BUF_GRAV_FY), f.twiddle, d);
     tpt_pass_end(ctx, PASS_GRAV_FFT_INV);

@@ -256,7 +256,8 @@ __global__ __launch_bounds__(BLK) void th_apply_list(Parts p, const uint32_t *__
                                                      const uint32_t *__restrict__ props,
                                                      const uint32_t *__restrict__ list,
                                                      const uint32_t *__restrict__ counters,
-                                                     const uint2 *__restrict__ heat, uint4 *__restrict__ res, Dims d) {
+                                                     const uint2 *__restrict__ heat,
+                                                     unsigned long long *__restrict__ rec, Dims d) {
     const uint32_t n = counters[TPT_CTR_OFFMAP];
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t j = list[k];
@@ -265,17 +266,17 @@ __global__ __launch_bounds__(BLK) void th_apply_list(Parts p, const uint32_t *__
         const int rx = tpt_f2i(p.x[j] + 0.5f), ry = tpt_f2i(p.y[j] + 0.5f);
         if (rx < 0 || ry < 0 || rx >= d.xres || ry >= d.yres) continue;
         const size_t pix = (size_t)ry * d.xres + rx;
-        if ((res[pix].x >> 9) == j + 1u) res[pix].z = __float_as_uint(p.temp[j]);
+        const unsigned long long v = rec[pix];
+        if ((uint32_t)(v >> 41) == j + 1u)
+            rec[pix] = (v & 0xFFFFFFFF00000000ull) | (unsigned long long)__float_as_uint(p.temp[j]);
     }
 }
 
 // Pixel-major conduction and apply over the records RecalcFreeParticles
 // left (unflagged record owners only; the rest go through the off-map
-// list).  Results stay pixel-major (heat: equilibrium + flags; res: owner
-// key, equilibrium, new temperature); the slot arrays are written by the
-// slot-major convection pass that follows, which reads each slot's own
-// result with one 16-byte load: a random load costs about half of a random
-// 4-byte store here.  A 64x16 pixel tile (16x4 cells) per 256-thread block, each thread
+// list).  Conduction leaves each pixel's equilibrium and flags in `heat`
+// for the apply tile; apply leaves the new temperature in the record for
+// the convection pass.  A 64x16 pixel tile (16x4 cells) per 256-thread block, each thread
 // one 4-pixel column of one cell, the tile plus a one-pixel halo in LDS.
 // Each cell's 16 pixels live in 4 adjacent lanes of one wave, so the R2/R3
 // cell results are reduced with lane shuffles and written by the block,
@@ -287,6 +288,44 @@ __device__ __forceinline__ bool special_type(uint32_t t) {
     return t == T_HSWC || t == T_PIPE || t == T_PPIP || t == T_GEL;
 }
 
+// Per-type flags the tiles look up instead of re-deriving them per
+// neighbour.  Types without TF_SPECIAL conduct unless HeatConduct is 0, with
+// heat capacity 1 (1.5 for BASE); the pair exclusions only ever pair two
+// TF_PAIR types.
+constexpr uint32_t TF_HC0 = 1u, TF_SPECIAL = 2u, TF_PAIR = 4u, TF_BASE = 8u, TF_NOAMB = 16u;
+
+__device__ __forceinline__ uint32_t type_flags(const uint32_t *__restrict__ props, uint32_t t) {
+    uint32_t f = 0u;
+    if (EP(props, t, COL_HEAT_CONDUCT) == 0u) f |= TF_HC0;
+    if (special_type(t)) f |= TF_SPECIAL;
+    if (t == T_FILT || t == T_BRAY || t == T_PHOT || t == T_BIZR || t == T_BIZRG || t == T_ELEC || t == T_DEUT ||
+        t == T_HSWC)
+        f |= TF_PAIR;
+    if (t == T_BASE) f |= TF_BASE;
+    if (EP(props, t, COL_PROPERTIES) & 0x00200000u) f |= TF_NOAMB;
+    return f;
+}
+
+// Appends v to an LDS list for the lanes where pred holds; every lane of
+// the wave calls it.
+__device__ __forceinline__ void lds_append(uint32_t *list, uint32_t *count, uint32_t v, bool pred) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (pred) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
+// Conduction for the unflagged owners of a 64x16 tile.  Phase 1 stages the
+// records (+halo), the type tables and the 16x4 cells' pre-pass air_hv;
+// phase 2 writes the pixels with nothing to compute and compacts the rest
+// into an LDS list (wave ballot + prefix count); phase 3 runs the owners,
+// two or so per thread, reducing R2 (highest gated slot carries its hv,
+// a 64-bit LDS max) and R3 (failure count) per cell in LDS; phase 4 merges
+// the off-map list's cell results and writes the cells.
 __global__ __launch_bounds__(256) void th_conduct_px(Parts p, const unsigned long long *__restrict__ rec,
                                                      float *__restrict__ air_hv, uint32_t *__restrict__ block_airh,
                                                      const uint32_t *__restrict__ props,
@@ -295,10 +334,23 @@ __global__ __launch_bounds__(256) void th_conduct_px(Parts p, const unsigned lon
                                                      uint32_t aheat, uint32_t seed) {
     __shared__ uint32_t skey[LH][LW];
     __shared__ float stemp[LH][LW];
-    __shared__ uint32_t shc[256];  // HeatConduct per type
+    __shared__ uint32_t shc[256], sflag[256];
+    __shared__ uint32_t list[TW * TH];
+    __shared__ unsigned long long cbest[64];
+    __shared__ uint32_t cfail[64];
+    __shared__ float chv[64];
+    __shared__ uint32_t count;
     const int tx = threadIdx.x, row = threadIdx.y, t = row * TW + tx;
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
     shc[t] = EP(props, (uint32_t)t, COL_HEAT_CONDUCT);
+    sflag[t] = type_flags(props, (uint32_t)t);
+    if (t < 64) {
+        const int cx = (x0 >> 2) + (t & 15), cy = (y0 >> 2) + (t >> 4);
+        cbest[t] = 0ull;
+        cfail[t] = 0u;
+        chv[t] = (aheat && cx < d.xc && cy < d.yc) ? air_hv[cy * d.xc + cx] : 0.0f;  // pre-pass (R2)
+    }
+    if (t == 0) count = 0u;
     for (int e = t; e < LH * LW; e += TW * 4) {
         const int ly = e / LW, lx = e - ly * LW;
         const int gx = x0 + lx - 1, gy = y0 + ly - 1;
@@ -308,128 +360,148 @@ __global__ __launch_bounds__(256) void th_conduct_px(Parts p, const unsigned lon
         stemp[ly][lx] = __uint_as_float((uint32_t)v);
     }
     __syncthreads();
-    const int x = x0 + tx, cy = (y0 >> 2) + row;
-    const bool cell_in = x < d.xres && cy < d.yc;
-    const int ci = cy * d.xc + (x >> 2);
-    float hv0 = 0.0f;
-    if (cell_in && aheat) hv0 = air_hv[ci];  // pre-pass value (R2)
-    uint32_t best_key = 0u, nfail = 0u;
-    float best_hv = 0.0f;
-    for (int k = 0; k < 4 && cell_in; ++k) {
-        const int ly = row * 4 + k + 1, lx = tx + 1, y = y0 + row * 4 + k;
-        const uint32_t key = skey[ly][lx];
-        uint2 out = make_uint2(__float_as_uint(-1.0f), 0u);
-        if (key != 0u) {
-            const uint32_t s = (key >> 9) - 1u, ty = key & 0xFFu;
-            int32_t life = 0, tmp = 0;
-            uint32_t ctype = 0u;
-            if (special_type(ty)) {
-                life = p.life[s];
-                tmp = p.tmp[s];
-                ctype = p.ctype[s];
-            }
-            const uint32_t hcond = shc[ty];
-            const bool ins = is_heat_insulator(ty, life, tmp, hcond);
-            out.y = (ins ? 1u : 0u) | (tmp == 1 ? 2u : 0u);
-            if (key & 0x100u) {
-                out.x = __float_as_uint(p.equil[s]);  // handled through the off-map list
-            } else if (s < d.slots) {
-                float gel = 1.0f;
-                if (ty == T_GEL) gel = (float)tmp * 2.55f;
-                float scaled = (float)hcond * gel;
-                if (scaled < 0.0f) scaled = 0.0f;
-                const uint32_t threshold = tpt_f2u(scaled);
-                const uint32_t roll = gpu_hash(seed ^ (s * PARTICLE_HASH_SALT) ^ ty) % 250u;
-                float eq = -1.0f;
-                if (!ins && roll < threshold) {
-                    const float t0 = stemp[ly][lx];
-                    float self_temp = t0;
-                    if (aheat && (EP(props, ty, COL_PROPERTIES) & 0x00200000u) == 0u) {
-                        const float dtemp = hv0 - t0;
-                        const float hc = heat_capacity(ty, ctype);
-                        float alpha = 0.4f * hc;
-                        if (alpha > 0.04f) alpha = 0.04f;
-                        self_temp = tpt_clampf(t0 + alpha * dtemp / hc, 0.0f, 9999.0f);
-                        const float hn = tpt_clampf(hv0 - alpha * dtemp, 0.0f, 9999.0f);
-                        if (s + 1u > best_key) {
-                            best_key = s + 1u;
-                            best_hv = hn;
-                        }
-                    }
-                    const float hc_self = heat_capacity(ty, ctype);
-                    float c_heat = self_temp * hc_self, hc_total = hc_self;
-                    for (int ny = 0; ny < 3; ++ny)
-                        for (int nx = 0; nx < 3; ++nx) {
-                            if (nx == 1 && ny == 1) continue;
-                            const uint32_t nk = skey[ly + ny - 1][lx + nx - 1];
-                            if (nk == 0u) continue;
-                            const uint32_t nid = (nk >> 9) - 1u, nty = nk & 0xFFu;
-                            if (nid >= d.slots || nty == 0u) continue;
-                            int32_t nlife = 0, ntmp = 0;
-                            uint32_t nctype = 0u;
-                            if (special_type(nty)) {
-                                nlife = p.life[nid];
-                                ntmp = p.tmp[nid];
-                                nctype = p.ctype[nid];
-                            }
-                            if (is_heat_insulator(nty, nlife, ntmp, shc[nty])) continue;
-                            if (pair_excluded(ty, nty, tmp, ntmp)) continue;
-                            const float n_hc = heat_capacity(nty, nctype);
-                            c_heat += stemp[ly + ny - 1][lx + nx - 1] * n_hc;
-                            hc_total += n_hc;
-                        }
-                    eq = tpt_clampf(c_heat / hc_total, 0.0f, 9999.0f);
-                } else {
-                    ++nfail;
+    const int x = x0 + tx;
+    for (int k = 0; k < 4; ++k) {
+        const int ly = row * 4 + k + 1, y = y0 + row * 4 + k;
+        const bool on = x < d.xres && y < d.yres;
+        const uint32_t key = on ? skey[ly][tx + 1] : 0u;
+        const bool work = key != 0u && (key & 0x100u) == 0u;
+        if (on && !work) {
+            uint2 out = make_uint2(__float_as_uint(-1.0f), 0u);
+            if (key != 0u) {  // flagged owner: its equilibrium came from the off-map list
+                const uint32_t s = (key >> 9) - 1u, ty = key & 0xFFu;
+                int32_t life = 0, tmp = 0;
+                if (sflag[ty] & TF_SPECIAL) {
+                    life = p.life[s];
+                    tmp = p.tmp[s];
                 }
-                out.x = __float_as_uint(eq);
+                const bool ins = is_heat_insulator(ty, life, tmp, shc[ty]);
+                out = make_uint2(__float_as_uint(p.equil[s]), (ins ? 1u : 0u) | (tmp == 1 ? 2u : 0u));
+            }
+            heat[(size_t)y * d.xres + x] = out;
+        }
+        lds_append(list, &count, (uint32_t)((ly - 1) * TW + tx), work);
+    }
+    __syncthreads();
+    const uint32_t n = count;
+    for (uint32_t j = (uint32_t)t; j < n; j += TW * 4) {
+        const uint32_t li = list[j];
+        const int ly = (int)(li / TW) + 1, lx = (int)(li % TW) + 1;
+        const int px = x0 + lx - 1, py = y0 + ly - 1;
+        const int cl = ((ly - 1) >> 2) * 16 + ((lx - 1) >> 2);
+        const uint32_t key = skey[ly][lx];
+        const uint32_t s = (key >> 9) - 1u, ty = key & 0xFFu, fo = sflag[ty];
+        int32_t life = 0, tmp = 0;
+        uint32_t ctype = 0u;
+        if (fo & TF_SPECIAL) {
+            life = p.life[s];
+            tmp = p.tmp[s];
+            ctype = p.ctype[s];
+        }
+        const uint32_t hcond = shc[ty];
+        const bool ins = (fo & TF_SPECIAL) ? is_heat_insulator(ty, life, tmp, hcond) : (fo & TF_HC0) != 0u;
+        float eq = -1.0f;
+        if (s < d.slots) {
+            float gel = 1.0f;
+            if (ty == T_GEL) gel = (float)tmp * 2.55f;
+            float scaled = (float)hcond * gel;
+            if (scaled < 0.0f) scaled = 0.0f;
+            const uint32_t threshold = tpt_f2u(scaled);
+            const uint32_t roll = gpu_hash(seed ^ (s * PARTICLE_HASH_SALT) ^ ty) % 250u;
+            if (!ins && roll < threshold) {
+                const float hc = (fo & TF_SPECIAL) ? heat_capacity(ty, ctype) : ((fo & TF_BASE) ? 1.5f : 1.0f);
+                const float t0 = stemp[ly][lx];
+                float self_temp = t0;
+                if (aheat && (fo & TF_NOAMB) == 0u) {
+                    const float hv0 = chv[cl];
+                    const float dtemp = hv0 - t0;
+                    float alpha = 0.4f * hc;
+                    if (alpha > 0.04f) alpha = 0.04f;
+                    self_temp = tpt_clampf(t0 + alpha * dtemp / hc, 0.0f, 9999.0f);
+                    const float hn = tpt_clampf(hv0 - alpha * dtemp, 0.0f, 9999.0f);
+                    atomicMax(&cbest[cl], ((unsigned long long)(s + 1u) << 32) | (unsigned long long)__float_as_uint(hn));
+                }
+                float c_heat = self_temp * hc, hc_total = hc;
+                for (int ny = 0; ny < 3; ++ny)
+                    for (int nx = 0; nx < 3; ++nx) {
+                        if (nx == 1 && ny == 1) continue;
+                        const uint32_t nk = skey[ly + ny - 1][lx + nx - 1];
+                        if (nk == 0u) continue;
+                        const uint32_t nid = (nk >> 9) - 1u, nty = nk & 0xFFu;
+                        if (nid >= d.slots) continue;
+                        const uint32_t fn = sflag[nty];
+                        float n_hc;
+                        if (fn & TF_SPECIAL) {
+                            const int32_t nlife = p.life[nid], ntmp = p.tmp[nid];
+                            if (is_heat_insulator(nty, nlife, ntmp, shc[nty])) continue;
+                            if ((fo & fn & TF_PAIR) && pair_excluded(ty, nty, tmp, ntmp)) continue;
+                            n_hc = heat_capacity(nty, p.ctype[nid]);
+                        } else {
+                            if (fn & TF_HC0) continue;
+                            if ((fo & fn & TF_PAIR) && pair_excluded(ty, nty, tmp, 0)) continue;
+                            n_hc = (fn & TF_BASE) ? 1.5f : 1.0f;
+                        }
+                        c_heat += stemp[ly + ny - 1][lx + nx - 1] * n_hc;
+                        hc_total += n_hc;
+                    }
+                eq = tpt_clampf(c_heat / hc_total, 0.0f, 9999.0f);
+            } else {
+                atomicAdd(&cfail[cl], 1u);
             }
         }
-        heat[(size_t)y * d.xres + x] = out;
+        if (s < d.slots) p.equil[s] = eq;
+        heat[(size_t)py * d.xres + px] = make_uint2(__float_as_uint(eq), (ins ? 1u : 0u) | (tmp == 1 ? 2u : 0u));
     }
-    for (int m = 1; m <= 2; m <<= 1) {
-        const uint32_t ok = __shfl_xor(best_key, m);
-        const float oh = __shfl_xor(best_hv, m);
-        nfail += __shfl_xor(nfail, m);
-        if (ok > best_key) {
-            best_key = ok;
-            best_hv = oh;
-        }
-    }
-    if (cell_in && (tx & 3) == 0) {
-        const uint32_t lk = hv_key[ci];
-        if (lk) {
-            if (lk > best_key) {
-                best_key = lk;
-                best_hv = hv_new[lk - 1u];
+    __syncthreads();
+    if (t < 64) {
+        const int cx = (x0 >> 2) + (t & 15), cy = (y0 >> 2) + (t >> 4);
+        if (cx < d.xc && cy < d.yc) {
+            const int ci = cy * d.xc + cx;
+            uint32_t best_key = (uint32_t)(cbest[t] >> 32);
+            float best_hv = __uint_as_float((uint32_t)cbest[t]);
+            const uint32_t lk = hv_key[ci];
+            if (lk) {
+                if (lk > best_key) {
+                    best_key = lk;
+                    best_hv = hv_new[lk - 1u];
+                }
+                hv_key[ci] = 0u;
             }
-            hv_key[ci] = 0u;
-        }
-        if (best_key) air_hv[ci] = best_hv;
-        const uint32_t lf = fails[ci];
-        if (lf) {
-            nfail += lf;
-            fails[ci] = 0u;
-        }
-        if (nfail) {
-            uint32_t v = block_airh[ci];
-            if ((v & 8u) == 0u) {
-                const uint32_t room = 8u - (v & 7u);
-                v += nfail < room ? nfail : room;
-                block_airh[ci] = v;
+            if (best_key) air_hv[ci] = best_hv;
+            uint32_t nfail = cfail[t];
+            const uint32_t lf = fails[ci];
+            if (lf) {
+                nfail += lf;
+                fails[ci] = 0u;
+            }
+            if (nfail) {
+                uint32_t v = block_airh[ci];
+                if ((v & 8u) == 0u) {
+                    const uint32_t room = 8u - (v & 7u);
+                    v += nfail < room ? nfail : room;
+                    block_airh[ci] = v;
+                }
             }
         }
     }
 }
 
-__global__ __launch_bounds__(256) void th_apply_px(const unsigned long long *__restrict__ rec,
-                                                   const uint2 *__restrict__ heat, uint4 *__restrict__ res, Dims d) {
+// Apply for the unflagged owners of a tile: phase 2 compacts the owners,
+// phase 3 computes them.
+__global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long long *__restrict__ rec,
+                                                   uint32_t *__restrict__ rec_temp, const uint2 *__restrict__ heat,
+                                                   const uint32_t *__restrict__ props, Dims d) {
     __shared__ uint32_t skey[LH][LW];
     __shared__ float seq[LH][LW];
     __shared__ uint32_t sinfo[LH][LW];
     __shared__ float sown[TH][TW];  // record temperatures of the tile itself
+    __shared__ uint32_t sflag[256];
+    __shared__ uint32_t list[TW * TH];
+    __shared__ uint32_t count;
     const int tx = threadIdx.x, row = threadIdx.y, t = row * TW + tx;
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    sflag[t] = type_flags(props, (uint32_t)t);
+    if (t == 0) count = 0u;
     for (int e = t; e < LH * LW; e += TW * 4) {
         const int ly = e / LW, lx = e - ly * LW;
         const int gx = x0 + lx - 1, gy = y0 + ly - 1;
@@ -440,26 +512,28 @@ __global__ __launch_bounds__(256) void th_apply_px(const unsigned long long *__r
             v = rec[pix];
             if (v >> 32) h = heat[pix];
         }
-        const uint32_t key = (uint32_t)(v >> 32);
         if (ly >= 1 && ly <= TH && lx >= 1 && lx <= TW) sown[ly - 1][lx - 1] = __uint_as_float((uint32_t)v);
-        skey[ly][lx] = key;
+        skey[ly][lx] = (uint32_t)(v >> 32);
         seq[ly][lx] = __uint_as_float(h.x);
         sinfo[ly][lx] = h.y;
     }
     __syncthreads();
     const int x = x0 + tx;
-    if (x >= d.xres) return;
     for (int k = 0; k < 4; ++k) {
-        const int ly = row * 4 + k + 1, lx = tx + 1, y = y0 + row * 4 + k;
-        if (y >= d.yres) break;
+        const int ly = row * 4 + k + 1, y = y0 + row * 4 + k;
+        const bool on = x < d.xres && y < d.yres;
+        const uint32_t key = on ? skey[ly][tx + 1] : 0u;
+        const bool work = key != 0u && (key & 0x100u) == 0u && ((key >> 9) - 1u) < d.slots;
+        lds_append(list, &count, (uint32_t)((ly - 1) * TW + tx), work);
+    }
+    __syncthreads();
+    const uint32_t n = count;
+    for (uint32_t jj = (uint32_t)t; jj < n; jj += TW * 4) {
+        const uint32_t li = list[jj];
+        const int ly = (int)(li / TW) + 1, lx = (int)(li % TW) + 1;
         const uint32_t key = skey[ly][lx];
-        const float own = sown[ly - 1][tx];
-        const size_t pix = (size_t)y * d.xres + x;
-        // Flagged owners keep their record temperature until the list pass.
-        res[pix] = make_uint4(key, __float_as_uint(seq[ly][lx]), __float_as_uint(own), 0u);
-        if (key == 0u || (key & 0x100u)) continue;
-        const uint32_t j = (key >> 9) - 1u, jty = key & 0xFFu;
-        if (j >= d.slots) continue;
+        const uint32_t j = (key >> 9) - 1u, jty = key & 0xFFu, fo = sflag[jty];
+        const float own = sown[ly - 1][lx - 1];
         uint32_t best = j;
         float best_temp = own;
         bool has = false;
@@ -478,18 +552,21 @@ __global__ __launch_bounds__(256) void th_apply_px(const unsigned long long *__r
                     if (nk == 0u) continue;
                     const uint32_t nid = (nk >> 9) - 1u, nty = nk & 0xFFu;
                     if (nid >= d.slots || nid == j) continue;
-                    if (nty == 0u || !(!has || nid > best)) continue;
+                    if (!(!has || nid > best)) continue;
                     const float neq = seq[ly + ny - 1][lx + nx - 1];
                     if (!(neq >= 0.0f)) continue;
                     const uint32_t ninfo = sinfo[ly + ny - 1][lx + nx - 1];
                     if (ninfo & 1u) continue;
-                    if (pair_excluded(nty, jty, (ninfo & 2u) ? 1 : 0, jtmp1)) continue;
+                    if ((fo & sflag[nty] & TF_PAIR) && pair_excluded(nty, jty, (ninfo & 2u) ? 1 : 0, jtmp1)) continue;
                     best = nid;
                     best_temp = neq;
                     has = true;
                 }
         }
-        res[pix].z = __float_as_uint(tpt_clampf(has ? best_temp : own, 0.0f, 9999.0f));
+        const float nt = tpt_clampf(has ? best_temp : own, 0.0f, 9999.0f);
+        p.temp[j] = nt;
+        // The record keeps the post-apply temperature for the convection pass.
+        rec_temp[2 * ((size_t)(y0 + ly - 1) * d.xres + (x0 + lx - 1))] = __float_as_uint(nt);
     }
 }
 
@@ -542,69 +619,58 @@ __device__ uint32_t convect_target(const Parts &p, const uint32_t *__restrict__ 
     return nid;
 }
 
-// Slot-major pass after the pixel-major apply: each slot that owns its own
-// pixel (unflagged owner) takes its equilibrium and new temperature from the
-// per-pixel results and stores them; then convect_target with the partner
-// read from the same results (owner key, type bits, post-apply temperature:
-// one 16-byte load instead of pmap plus two slot-array gathers; types are
-// still those RecalcFreeParticles saw).
-__global__ __launch_bounds__(BLK) void th_convect_propose_px(Parts p, const uint4 *__restrict__ res,
-                                                             const uint32_t *__restrict__ props,
-                                                             const float *__restrict__ gfx,
-                                                             const float *__restrict__ gfy,
-                                                             uint32_t *__restrict__ tgt, uint32_t *__restrict__ touch,
-                                                             Dims d, GravCfg g, uint32_t seed) {
+// convect_target with the partner read from the pixel records (slot, type
+// and the post-apply temperature in one 8-byte load) instead of pmap plus
+// two slot-array gathers.  Valid while the records are current: types as
+// at RecalcFreeParticles, temperatures as written by the apply kernels.
+__global__ __launch_bounds__(BLK) void th_convect_propose_rec(Parts p, const unsigned long long *__restrict__ rec,
+                                                              const uint32_t *__restrict__ props,
+                                                              const float *__restrict__ gfx,
+                                                              const float *__restrict__ gfy,
+                                                              uint32_t *__restrict__ tgt, uint32_t *__restrict__ touch,
+                                                              Dims d, GravCfg g, uint32_t seed) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.slots) return;
     touch[i] = NONE;
     uint32_t target = NONE;
     const uint32_t t = p.ty[i];
-    if (t != 0u && t < 256u) {
+    if (t != 0u && t < 256u && (EP(props, t, COL_PROPERTIES) & 0x02u) != 0u) {
         const int x = tpt_f2i(p.x[i]), y = tpt_f2i(p.y[i]);
-        if (x >= 0 && x < d.xres && y >= 0 && y < d.yres) {
-            const uint4 r = res[(size_t)y * d.xres + x];
-            float own;
-            if ((r.x >> 9) == i + 1u && (r.x & 0x100u) == 0u) {
-                p.equil[i] = __uint_as_float(r.y);
-                p.temp[i] = own = __uint_as_float(r.z);
-            } else {
-                own = p.temp[i];
-            }
-            bool go = (EP(props, t, COL_PROPERTIES) & 0x02u) != 0u;
-            if (go && t == T_GEL) {
-                const uint32_t roll = gpu_hash(seed ^ (i * PARTICLE_HASH_SALT) ^ t ^ TRANSITION_HASH_SALT);
-                const float gel = (float)p.tmp[i] * 2.55f;
-                const uint32_t gate_roll = gpu_hash(roll ^ 0x1B873593u) % 255u;
-                if (gel <= 1.0f + (float)gate_roll) go = false;
-            }
-            if (go) {
-                const int ci = (y / TPT_CELL) * d.xc + x / TPT_CELL;
-                const float pg = -2.0f;
-                float gx = 0.0f, gy = 0.0f;
-                if (g.mode == 0u) {
-                    gy = pg;
-                } else if (g.mode == 2u) {
-                    const float dx = (float)x - (float)d.xres * 0.5f, dy = (float)y - (float)d.yres * 0.5f;
-                    if (dx != 0.0f || dy != 0.0f) {
-                        const float dd = 0.01f - sqrtf(dx * dx + dy * dy);
-                        gx = pg * (dx / dd);
-                        gy = pg * (dy / dd);
-                    }
-                } else if (g.mode == 3u) {
-                    gx = pg * g.cgx;
-                    gy = pg * g.cgy;
+        bool go = x >= 0 && x < d.xres && y >= 0 && y < d.yres;
+        if (go && t == T_GEL) {
+            const uint32_t roll = gpu_hash(seed ^ (i * PARTICLE_HASH_SALT) ^ t ^ TRANSITION_HASH_SALT);
+            const float gel = (float)p.tmp[i] * 2.55f;
+            const uint32_t gate_roll = gpu_hash(roll ^ 0x1B873593u) % 255u;
+            if (gel <= 1.0f + (float)gate_roll) go = false;
+        }
+        if (go) {
+            const int ci = (y / TPT_CELL) * d.xc + x / TPT_CELL;
+            const float pg = -2.0f;
+            float gx = 0.0f, gy = 0.0f;
+            if (g.mode == 0u) {
+                gy = pg;
+            } else if (g.mode == 2u) {
+                const float dx = (float)x - (float)d.xres * 0.5f, dy = (float)y - (float)d.yres * 0.5f;
+                if (dx != 0.0f || dy != 0.0f) {
+                    const float dd = 0.01f - sqrtf(dx * dx + dy * dy);
+                    gx = pg * (dx / dd);
+                    gy = pg * (dy / dd);
                 }
-                gx += -2.0f * gfx[ci];
-                gy += -2.0f * gfy[ci];
-                const float rxf = gx + (float)x, ryf = gy + (float)y;
-                const int rx = rxf >= 0.0f ? tpt_f2i(rxf + 0.5f) : tpt_f2i(rxf - 0.5f);
-                const int ry = ryf >= 0.0f ? tpt_f2i(ryf + 0.5f) : tpt_f2i(ryf - 0.5f);
-                const int ox = min(max(rx, x - 1), x + 1), oy = min(max(ry, y - 1), y + 1);
-                if ((ox != x || oy != y) && ox >= 0 && oy >= 0 && ox < d.xres && oy < d.yres) {
-                    const uint4 q = res[(size_t)oy * d.xres + ox];
-                    const uint32_t nid = (q.x >> 9) - 1u;
-                    if (q.x != 0u && nid < d.slots && (q.x & 0xFFu) == t && own > __uint_as_float(q.z)) target = nid;
-                }
+            } else if (g.mode == 3u) {
+                gx = pg * g.cgx;
+                gy = pg * g.cgy;
+            }
+            gx += -2.0f * gfx[ci];
+            gy += -2.0f * gfy[ci];
+            const float rxf = gx + (float)x, ryf = gy + (float)y;
+            const int rx = rxf >= 0.0f ? tpt_f2i(rxf + 0.5f) : tpt_f2i(rxf - 0.5f);
+            const int ry = ryf >= 0.0f ? tpt_f2i(ryf + 0.5f) : tpt_f2i(ryf - 0.5f);
+            const int ox = min(max(rx, x - 1), x + 1), oy = min(max(ry, y - 1), y + 1);
+            if ((ox != x || oy != y) && ox >= 0 && oy >= 0 && ox < d.xres && oy < d.yres) {
+                const unsigned long long v = rec[(size_t)oy * d.xres + ox];
+                const uint32_t key = (uint32_t)(v >> 32), nid = (key >> 9) - 1u;
+                if (key != 0u && nid < d.slots && (key & 0xFFu) == t && p.temp[i] > __uint_as_float((uint32_t)v))
+                    target = nid;
             }
         }
     }
@@ -1005,9 +1071,10 @@ int tpt_thermal_step(tpt_ctx *ctx, uint32_t seed, int legacy) {
                            ctx->cell_scratch_u, ctx->cell_scratch_u2, ctx->pix_heat, d, aheat, seed);
         tpt_pass_end(ctx, PASS_CONDUCT);
         tpt_pass_begin(ctx, PASS_CONDUCT_APPLY);
-        hipLaunchKernelGGL(th_apply_px, tg, tb, 0, ctx->stream, rec, (const uint2 *)ctx->pix_heat, ctx->pix_res, d);
+        hipLaunchKernelGGL(th_apply_px, tg, tb, 0, ctx->stream, p, rec, (uint32_t *)ctx->pix_rec,
+                           (const uint2 *)ctx->pix_heat, props, d);
         hipLaunchKernelGGL(th_apply_list, lg, b, 0, ctx->stream, p, pmap, props, ctx->offmap, ctx->counters,
-                           (const uint2 *)ctx->pix_heat, ctx->pix_res, d);
+                           (const uint2 *)ctx->pix_heat, (unsigned long long *)ctx->pix_rec, d);
         tpt_pass_end(ctx, PASS_CONDUCT_APPLY);
     } else if (!legacy) {
         tpt_pass_begin(ctx, PASS_CONDUCT);
@@ -1028,8 +1095,9 @@ int tpt_thermal_step(tpt_ctx *ctx, uint32_t seed, int legacy) {
     if (!legacy) {
         const GravCfg g{ctx->cfg.gravity_mode, ctx->cfg.custom_gravity_x, ctx->cfg.custom_gravity_y};
         if (fresh)
-            hipLaunchKernelGGL(th_convect_propose_px, sg, b, 0, ctx->stream, p, (const uint4 *)ctx->pix_res, props,
-                               gfx, gfy, ctx->slot_scratch_u, ctx->slot_scratch_u2, d, g, seed);
+            hipLaunchKernelGGL(th_convect_propose_rec, sg, b, 0, ctx->stream, p,
+                               (const unsigned long long *)ctx->pix_rec, props, gfx, gfy, ctx->slot_scratch_u,
+                               ctx->slot_scratch_u2, d, g, seed);
         else
             hipLaunchKernelGGL(th_convect_propose, sg, b, 0, ctx->stream, p, pmap, props, gfx, gfy,
                                ctx->slot_scratch_u, ctx->slot_scratch_u2, d, g, seed);

@@ -95,7 +95,6 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMalloc(&c->slot_scratch_f, cap * 4) != hipSuccess || hipMalloc(&c->counters, 256) != hipSuccess ||
         hipMalloc(&c->pix_rec, (size_t)c->pixels * 8) != hipSuccess ||
         hipMalloc(&c->pix_heat, (size_t)c->pixels * 8) != hipSuccess || hipMalloc(&c->offmap, cap * 4) != hipSuccess ||
-        hipMalloc(&c->pix_res, (size_t)c->pixels * 16) != hipSuccess ||
         hipMalloc(&c->pix_type, (size_t)c->pixels) != hipSuccess)
         return tpt_set_error(TPT_ENOMEM, "scratch allocation failed");
     hipMemsetAsync(c->cell_scratch_u, 0, cb, c->stream);
@@ -109,7 +108,7 @@ static void free_all(tpt_ctx *c) {
         if (c->buf[id]) hipFree(c->buf[id]);
     void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
                      c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters,
-                     c->pix_rec, c->pix_heat, c->offmap, c->pix_res, c->pix_type};
+                     c->pix_rec, c->pix_heat, c->offmap, c->pix_type};
     for (void *p : extra)
         if (p) hipFree(p);
     tpt_gravity_free(c);

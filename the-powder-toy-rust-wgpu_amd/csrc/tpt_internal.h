@@ -89,7 +89,6 @@ struct tpt_ctx {
     // (equilibrium temperature bits, insulator/tmp flags).
     uint64_t *pix_rec = nullptr;          // pixels
     uint2 *pix_heat = nullptr;            // pixels
-    uint4 *pix_res = nullptr;             // pixels: owner key, equilibrium, new temperature
     uint8_t *pix_type = nullptr;          // pixels: pmap type byte for the movement planner
     uint32_t *offmap = nullptr;           // capacity words
     // True while pix_rec / offmap describe the current particle state: set
