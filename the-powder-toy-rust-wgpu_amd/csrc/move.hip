@@ -4,6 +4,10 @@
 //   mv_air_deposit(+_cells)  particle -> air drag (mod.rs:297-346); the racy
 //                            per-cell store becomes "highest eligible slot
 //                            writes" via atomicMax on slot+1 and a cell pass.
+//   mv_air_deposit_list/_px  the same rule over the pixel records of the
+//                            RecalcFreeParticles that ran before: one thread
+//                            per cell takes its pixels' owners from the
+//                            highest slot down; the off-map list first.
 //   mv_clear                 cell_winner = 0 (mod.rs:268-273), 16-byte stores.
 //   mv_intent                velocity integration, PlanMove and the per-
 //                            Falldown destination chain, then the claim
@@ -13,10 +17,11 @@
 //   mv_saw                   WOOD -> SAWD of rejected probes, after the pass.
 //   mv_commit                winner check, consume/swap/move, FLAG_STAGNANT
 //                            (commit.rs:181-535).
-//   mv_map_clear / mv_map_scatter / mv_map_invis
-//                            pmap/photons rebuild (commit.rs:541-603): the
-//                            highest matter slot owns a pixel (atomicMax on the
-//                            packed word), INVIS/FILT only take an empty one.
+//   map rebuild              pmap/photons rebuild (commit.rs:541-603): the
+//                            highest matter slot owns a pixel, INVIS/FILT only
+//                            take an empty one; built by the map builder of
+//                            RecalcFreeParticles (spatial.hip) without life
+//                            decay, so that the next recalc can update it.
 //
 // Not built: photon refraction at glass (velocity.rs:262-647), the e == 2 pair
 // pass (reaction_kernel/energy_entry.rs) and WL_DETECT charging
@@ -774,11 +779,12 @@ __global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, const float
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= slots) return;
     s.intent[i] = NO_INTENT;
-    const uint32_t ty = s.ty[i];
+    // Slot fields loaded together rather than one latency per test.
+    const uint32_t ty = s.ty[i], flags = s.flags[i];
+    const float px = s.x[i], py = s.y[i], vx0 = s.vx[i], vy0 = s.vy[i];
+    const int32_t tmp0 = s.tmp[i];
     if (ty == 0u || ty >= 256u) return;
-    const uint32_t flags = s.flags[i];
     if ((flags & 0x10u) || (flags & 0x02u)) return;
-    const float px = s.x[i], py = s.y[i];
     const int xi = tpt_f2i(px + 0.5f), yi = tpt_f2i(py + 0.5f);
     if (xi < 0 || xi >= e.xr || yi < 0 || yi >= e.yr) {
         s.ty[i] = 0u;
@@ -805,8 +811,8 @@ __global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, const float
     const float loss_here = e.newton_grav ? 1.0f : loss;
     const float rdx = (float)(h1 & 0xFFFFu) * (2.0f / 65535.0f) - 1.0f;
     const float rdy = (float)(h2 & 0xFFFFu) * (2.0f / 65535.0f) - 1.0f;
-    float vx = s.vx[i] * loss_here + advection * ax + gx + diffusion * rdx;
-    float vy = s.vy[i] * loss_here + advection * ay + gy + diffusion * rdy;
+    float vx = vx0 * loss_here + advection * ax + gx + diffusion * rdx;
+    float vy = vy0 * loss_here + advection * ay + gy + diffusion * rdy;
     float mv = fabsf(vx);
     if (fabsf(vy) > mv) mv = fabsf(vy);
     if (mv > MAX_VELOCITY) {
@@ -819,7 +825,7 @@ __global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, const float
         target = gas_destination(e, vx, vy, px, py, xi, yi, sx, sy, props, ty, i, collision, seed);
     else
         target = fall_down_destination(e, vx, vy, px, py, xi, yi, sx, sy, props, ty, falldown, flags, collision, h2,
-                                       gravity, s.tmp[i]);
+                                       gravity, tmp0);
     if (target == DIE_INTENT) {
         s.ty[i] = 0u;
         return;
@@ -866,11 +872,11 @@ __global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const u
                                                  int xc, uint32_t heat_enable) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= slots) return;
-    const uint32_t ty = s.ty[i];
+    // Slot fields loaded together (only this thread writes flags[i]).
+    const uint32_t ty = s.ty[i], raw = s.intent[i], flags = s.flags[i];
     if (ty == 0u || ty >= 256u) return;
-    const uint32_t raw = s.intent[i];
     if (raw == NO_INTENT) {
-        if (lut[ty * 12u + 1u] >= 2u && (s.flags[i] & 0x10u) == 0u) s.flags[i] |= 0x01u;
+        if (lut[ty * 12u + 1u] >= 2u && (flags & 0x10u) == 0u) s.flags[i] = flags | 0x01u;
         return;
     }
     const bool half_mirror = (raw & HALF_MIRROR) != 0u;
@@ -918,17 +924,18 @@ __global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const u
         return;
     }
     if (target >= (uint32_t)(xres * yres)) return;
-    s.flags[i] &= 0xFFFFFFFEu;
-    if ((winner[target] & 0x7FFFFFu) != i + 1u) return;
-    if (half_mirror) {
-        s.x[i] = (float)(target % (uint32_t)xres);
-        s.y[i] = (float)(target / (uint32_t)xres);
-        return;
-    }
+    s.flags[i] = flags & 0xFFFFFFFEu;
+    // The claim, the destination's occupant and its wall in one round trip.
     const int nx = (int)(target % (uint32_t)xres), ny = (int)(target / (uint32_t)xres);
     const int wc = (ny / TPT_CELL) * xc + nx / TPT_CELL;
-    const uint32_t wall = wall_a[wc];
-    const uint32_t dest = pmap[target], dty = dest & 0x1FFu, docc = dest >> 9;
+    const uint32_t claim = winner[target], dest = pmap[target], wall = wall_a[wc];
+    if ((claim & 0x7FFFFFu) != i + 1u) return;
+    if (half_mirror) {
+        s.x[i] = (float)nx;
+        s.y[i] = (float)ny;
+        return;
+    }
+    const uint32_t dty = dest & 0x1FFu, docc = dest >> 9;
     const uint32_t mprops = lut[ty * 12u], dprops = lut[dty * 12u];
     uint32_t can = 0u, other = i;
     if (docc != 0u) {
@@ -1016,61 +1023,10 @@ __global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const u
     s.y[i] = (float)ny;
 }
 
-__global__ void mv_map_clear(uint4 *__restrict__ pmap, uint4 *__restrict__ photons, uint32_t n4,
-                             uint32_t *__restrict__ counters) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) counters[0] = 0u;
-    if (i >= n4) return;
-    pmap[i] = make_uint4(0u, 0u, 0u, 0u);
-    photons[i] = make_uint4(0u, 0u, 0u, 0u);
-}
-
-__global__ __launch_bounds__(BLK) void mv_map_scatter(const uint32_t *__restrict__ ty_a, const float *__restrict__ px,
-                                                      const float *__restrict__ py,
-                                                      const uint32_t *__restrict__ elem_props,
-                                                      uint32_t *__restrict__ pmap, uint32_t *__restrict__ photons,
-                                                      uint32_t *__restrict__ invis_list,
-                                                      uint32_t *__restrict__ counters, uint32_t slots, int xres,
-                                                      int yres) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= slots) return;
-    const uint32_t ty = ty_a[i];
-    if (ty == 0u || ty >= 256u) return;
-    const int xi = tpt_f2i(px[i] + 0.5f), yi = tpt_f2i(py[i] + 0.5f);
-    if (xi < 0 || xi >= xres || yi < 0 || yi >= yres) return;
-    const uint32_t idx = (uint32_t)(yi * xres + xi), packed = ((i + 1u) << 9) | (ty & 0x1FFu);
-    if (elem_props[ty * 32u + 1u] & 0x10u) {
-        atomicMax(&photons[idx], packed);
-    } else if (ty == 124u || ty == 134u) {
-        invis_list[atomicAdd(&counters[0], 1u)] = i;
-    } else {
-        atomicMax(&pmap[idx], packed);
-    }
-}
-
-__global__ void mv_map_invis(const uint32_t *__restrict__ ty_a, const float *__restrict__ px,
-                             const float *__restrict__ py, uint32_t *__restrict__ pmap,
-                             const uint32_t *__restrict__ invis_list, const uint32_t *__restrict__ counters, int xres) {
-    const uint32_t n = counters[0];
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const uint32_t i = invis_list[k];
-        const int xi = tpt_f2i(px[i] + 0.5f), yi = tpt_f2i(py[i] + 0.5f);
-        const uint32_t idx = (uint32_t)(yi * xres + xi), packed = ((i + 1u) << 9) | (ty_a[i] & 0x1FFu);
-        uint32_t cur = pmap[idx];
-        for (;;) {
-            const uint32_t t = cur & 0x1FFu;
-            if ((cur != 0u && t != 124u && t != 134u) || (cur != 0u && cur <= packed)) break;
-            const uint32_t prev = atomicCAS(&pmap[idx], cur, packed);
-            if (prev == cur) break;
-            cur = prev;
-        }
-    }
-}
-
 }  // namespace
 
 int tpt_movement_step(tpt_ctx *ctx, uint32_t seed) {
-    ctx->maps_fresh = false;
+    ctx->maps_fresh = ctx->rebuild_fresh = false;
     const tpt_config &c = ctx->cfg;
     const uint32_t slots = ctx->slot_bound, P = ctx->pixels;
     const int xres = (int)c.xres, yres = (int)c.yres, xc = (int)ctx->xcells;
@@ -1080,7 +1036,7 @@ int tpt_movement_step(tpt_ctx *ctx, uint32_t seed) {
             tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP), tpt_buf<uint32_t>(ctx, TPT_BUF_INTENT_TARGET)};
     const uint32_t *wall = tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), *emap = tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP);
     const uint32_t *lut = tpt_buf<uint32_t>(ctx, TPT_BUF_MOVE_LUT);
-    uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), *photons = tpt_buf<uint32_t>(ctx, TPT_BUF_PHOTONS);
+    uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP);
     uint32_t *winner = tpt_buf<uint32_t>(ctx, TPT_BUF_CELL_WINNER);
     float *air_vx = tpt_buf<float>(ctx, TPT_BUF_AIR_VX), *air_vy = tpt_buf<float>(ctx, TPT_BUF_AIR_VY);
     const dim3 sg((slots + BLK - 1) / BLK), cg((ctx->cells + BLK - 1) / BLK), b(BLK);
@@ -1118,12 +1074,9 @@ int tpt_movement_step(tpt_ctx *ctx, uint32_t seed) {
     hipLaunchKernelGGL(mv_commit, sg, b, 0, ctx->stream, s, cb, pmap, wall, emap, lut, winner, slots, xres, yres, xc,
                        c.heat_enable ? 1u : 0u);
     tpt_pass_end(ctx, PASS_MOVE_COMMIT);
-    tpt_pass_begin(ctx, PASS_MOVE_REBUILD);
-    hipLaunchKernelGGL(mv_map_clear, pg, b, 0, ctx->stream, (uint4 *)pmap, (uint4 *)photons, n4, ctx->counters);
-    hipLaunchKernelGGL(mv_map_scatter, sg, b, 0, ctx->stream, s.ty, s.x, s.y, tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS),
-                       pmap, photons, ctx->slot_scratch_u, ctx->counters, slots, xres, yres);
-    hipLaunchKernelGGL(mv_map_invis, dim3(64), b, 0, ctx->stream, s.ty, s.x, s.y, pmap, ctx->slot_scratch_u,
-                       ctx->counters, xres);
-    tpt_pass_end(ctx, PASS_MOVE_REBUILD);
-    return tpt_check_launch("movement pass");
+    int rc = tpt_check_launch("movement pass");
+    if (rc) return rc;
+    // clear_spatial_maps_kernel + scatter_spatial_maps_kernel (commit.rs:541-603),
+    // built as the pixel records RecalcFreeParticles uses.
+    return tpt_move_rebuild_maps(ctx);
 }

@@ -19,11 +19,19 @@
 //
 // The pixel record is library-internal: it lets the conduction pass run as
 // a pixel-major stencil.  Its flag bit says "this owner is handled through
-// the off-map list": every live slot whose own (truncated) pixel is on the
-// canvas ends up either as the unflagged owner of that pixel or in the list,
-// never both.  The list holds energy particles, INVIS/FILT, particles that
-// lose their pixel to a higher slot, and particles whose rounded and
-// truncated pixels differ.
+// the off-map list": every live slot whose rounded or truncated pixel is on
+// the canvas ends up either as the unflagged owner of that pixel or in the
+// list, never both.  The list holds energy particles, INVIS/FILT, particles
+// that lose their pixel to a higher slot, and particles whose rounded and
+// truncated pixels differ; per-slot consumers re-check their own bounds.
+//
+// The movement pass rebuilds the same records, list and a private count map
+// after its commit (tpt_move_rebuild_maps).  Until the next
+// RecalcFreeParticles only air and gravity run, which move no particle, so
+// that recalc is incremental (recalc_decay / recalc_relist /
+// recalc_invis_list): life decay kills, a killed owner clears its record and
+// every listed particle is offered its pixel again; pmap_count is the
+// private map minus the killed.
 
 #include "tpt_internal.h"
 
@@ -76,7 +84,8 @@ __global__ __launch_bounds__(BLK) void recalc_scatter(
     const float *__restrict__ py_a, const float *__restrict__ temp_a, const uint32_t *__restrict__ elem_props,
     const uint32_t *__restrict__ wall_bmap, const uint32_t *__restrict__ wall_emap, unsigned long long *__restrict__ rec,
     uint32_t *__restrict__ photons, uint32_t *__restrict__ pmap_count, uint32_t *__restrict__ invis_list,
-    uint32_t *__restrict__ offmap, uint32_t *__restrict__ counters, uint32_t slots, int xres, int yres, int xc) {
+    uint32_t *__restrict__ offmap, uint32_t *__restrict__ counters, uint32_t slots, int xres, int yres, int xc,
+    uint32_t decay) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool app_invis = false, app_self = false, app_other = false;
     uint32_t other = 0u;
@@ -91,8 +100,9 @@ __global__ __launch_bounds__(BLK) void recalc_scatter(
             stasis = (wall_bmap[cell] & 0xFFu) == 18u && (wall_emap[cell] & 0xFFu) < 8u;
         }
         const uint32_t props = elem_props[ty * 32u + 1u];
-        const int32_t life = life_a[i];
-        if (life > 0 && (props & 0x4000u) && (!in_bounds || !stasis)) {
+        const int32_t life = decay ? life_a[i] : 0;
+        if (!decay) {
+        } else if (life > 0 && (props & 0x4000u) && (!in_bounds || !stasis)) {
             const int32_t next = life - 1;
             life_a[i] = next;
             if (next <= 0 && (props & (0x10000u | 0x8000u))) {
@@ -114,10 +124,10 @@ __global__ __launch_bounds__(BLK) void recalc_scatter(
             const uint32_t packed = ((i + 1u) << 9) | (ty & 0x1FFu);
             if (props & 0x10u) {
                 atomicMax(&photons[idx], packed);
-                app_self = own_in;
+                app_self = true;
             } else if (ty == 124u || ty == 134u) {  // INVIS / FILT
                 app_invis = true;
-                app_self = own_in;
+                app_self = true;
                 atomicAdd(&pmap_count[idx], 1u);
             } else {
                 const uint32_t key = packed | (same_px ? 0u : 0x100u);
@@ -125,7 +135,7 @@ __global__ __launch_bounds__(BLK) void recalc_scatter(
                     ((unsigned long long)key << 32) | (unsigned long long)__float_as_uint(temp_a[i]);
                 const unsigned long long old = atomicMax(&rec[idx], v);
                 const uint32_t okey = (uint32_t)(old >> 32);
-                if (!same_px) app_self = own_in;
+                if (!same_px) app_self = true;
                 if (okey > key) {
                     // Lost to a higher slot: count self, list self.
                     if (counted_type(ty)) atomicAdd(&pmap_count[idx], 1u);
@@ -190,13 +200,117 @@ __global__ void recalc_extract(const ulonglong2 *__restrict__ rec, uint4 *__rest
         any |= add[j] != 0u;
     }
     pmap[i] = make_uint4(k[0] & ~0x100u, k[1] & ~0x100u, k[2] & ~0x100u, k[3] & ~0x100u);
-    if (any) {
+    if (any && count) {
         uint4 c = count[i];
         c.x += add[0];
         c.y += add[1];
         c.z += add[2];
         c.w += add[3];
         count[i] = c;
+    }
+}
+
+// Incremental RecalcFreeParticles, step 1: life decay and kills (the
+// decay rule of recalc_scatter); a killed particle leaves its pixel record
+// or photons entry if it held it, and the count of its pixel.
+__global__ __launch_bounds__(BLK) void recalc_decay(
+    uint32_t *__restrict__ ty_a, int32_t *__restrict__ life_a, const float *__restrict__ px_a,
+    const float *__restrict__ py_a, const uint32_t *__restrict__ elem_props, const uint32_t *__restrict__ wall_bmap,
+    const uint32_t *__restrict__ wall_emap, unsigned long long *__restrict__ rec, uint32_t *__restrict__ photons,
+    uint32_t *__restrict__ pmap_count, uint32_t slots, int xres, int yres, int xc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    const uint32_t ty = ty_a[i];
+    if (ty == 0u || ty >= (uint32_t)TPT_PT_NUM) return;
+    const uint32_t props = elem_props[ty * 32u + 1u];
+    if ((props & (0x4000u | 0x8000u)) == 0u) return;
+    const int x = tpt_f2i(px_a[i] + 0.5f), y = tpt_f2i(py_a[i] + 0.5f);
+    const bool in_bounds = x >= 0 && y >= 0 && x < xres && y < yres;
+    bool stasis = false;
+    if (in_bounds) {
+        const int cell = (y / TPT_CELL) * xc + x / TPT_CELL;
+        stasis = (wall_bmap[cell] & 0xFFu) == 18u && (wall_emap[cell] & 0xFFu) < 8u;
+    }
+    const int32_t life = life_a[i];
+    bool killed = false;
+    if (life > 0 && (props & 0x4000u) && (!in_bounds || !stasis)) {
+        const int32_t next = life - 1;
+        life_a[i] = next;
+        killed = next <= 0 && (props & (0x10000u | 0x8000u));
+    } else if (life <= 0 && (props & 0x8000u) && (!in_bounds || !stasis)) {
+        killed = true;
+    }
+    if (!killed) return;
+    ty_a[i] = 0u;
+    if (!in_bounds) return;
+    const uint32_t idx = (uint32_t)y * (uint32_t)xres + (uint32_t)x;
+    if (props & 0x10u) {
+        if ((photons[idx] >> 9) == i + 1u) photons[idx] = 0u;
+        return;
+    }
+    if ((uint32_t)(rec[idx] >> 41) == i + 1u) rec[idx] = 0ull;
+    if (counted_type(ty)) atomicSub(&pmap_count[idx], 1u);
+}
+
+// Step 2: every live listed particle is offered its pixel again (a no-op
+// unless the holder was killed); matter takes the record with the flag set,
+// since it stays on the list.  INVIS/FILT follow in recalc_invis_list.
+__global__ __launch_bounds__(BLK) void recalc_relist(const uint32_t *__restrict__ ty_a, const float *__restrict__ px_a,
+                                                     const float *__restrict__ py_a, const float *__restrict__ temp_a,
+                                                     const uint32_t *__restrict__ elem_props,
+                                                     unsigned long long *__restrict__ rec,
+                                                     uint32_t *__restrict__ photons, const uint32_t *__restrict__ list,
+                                                     const uint32_t *__restrict__ counters, uint32_t slots, int xres,
+                                                     int yres) {
+    const uint32_t n = counters[TPT_CTR_OFFMAP];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t i = list[k];
+        if (i >= slots) continue;
+        const uint32_t ty = ty_a[i];
+        if (ty == 0u || ty >= (uint32_t)TPT_PT_NUM || ty == 124u || ty == 134u) continue;
+        const int x = tpt_f2i(px_a[i] + 0.5f), y = tpt_f2i(py_a[i] + 0.5f);
+        if (x < 0 || y < 0 || x >= xres || y >= yres) continue;
+        const uint32_t idx = (uint32_t)y * (uint32_t)xres + (uint32_t)x;
+        const uint32_t packed = ((i + 1u) << 9) | ty;
+        if (elem_props[ty * 32u + 1u] & 0x10u)
+            atomicMax(&photons[idx], packed);
+        else
+            atomicMax(&rec[idx], ((unsigned long long)(packed | 0x100u) << 32) |
+                                     (unsigned long long)__float_as_uint(temp_a[i]));
+    }
+}
+
+// Step 3: the listed INVIS/FILT run the placement of recalc_invis again.
+__global__ __launch_bounds__(BLK) void recalc_invis_list(const uint32_t *__restrict__ ty_a,
+                                                         const float *__restrict__ px_a,
+                                                         const float *__restrict__ py_a,
+                                                         const float *__restrict__ temp_a,
+                                                         unsigned long long *__restrict__ rec,
+                                                         const uint32_t *__restrict__ list,
+                                                         const uint32_t *__restrict__ counters, uint32_t slots,
+                                                         int xres, int yres) {
+    const uint32_t n = counters[TPT_CTR_OFFMAP];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t i = list[k];
+        if (i >= slots) continue;
+        const uint32_t ty = ty_a[i];
+        if (ty != 124u && ty != 134u) continue;
+        const int x = tpt_f2i(px_a[i] + 0.5f), y = tpt_f2i(py_a[i] + 0.5f);
+        if (x < 0 || y < 0 || x >= xres || y >= yres) continue;
+        const uint32_t idx = (uint32_t)y * (uint32_t)xres + (uint32_t)x;
+        const uint32_t packed = ((i + 1u) << 9) | ty;
+        const unsigned long long v = ((unsigned long long)(packed | 0x100u) << 32) |
+                                     (unsigned long long)__float_as_uint(temp_a[i]);
+        unsigned long long cur = rec[idx];
+        for (;;) {
+            const uint32_t ck = (uint32_t)(cur >> 32);
+            const uint32_t t = ck & 0xFFu;
+            const bool other_matter = ck != 0u && t != 124u && t != 134u;
+            if (other_matter || (ck != 0u && (ck & ~0x100u) <= packed)) break;
+            const unsigned long long prev = atomicCAS(&rec[idx], cur, v);
+            if (prev == cur) break;
+            cur = prev;
+        }
     }
 }
 
@@ -253,34 +367,82 @@ __global__ __launch_bounds__(BLK) void stacking_apply(uint32_t *__restrict__ ty_
 
 }  // namespace
 
-int tpt_recalc_step(tpt_ctx *ctx, uint32_t seed, int stacking) {
+// Full map build: clear, (decay +) scatter, INVIS/FILT placement, pmap
+// extraction; counts into `count`.
+static int map_build(tpt_ctx *ctx, int decay, uint32_t *count, int pass_clear, int pass_scatter) {
     const uint32_t P = ctx->pixels, slots = ctx->slot_bound;
     const int xres = (int)ctx->cfg.xres, yres = (int)ctx->cfg.yres;
-    uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), *photons = tpt_buf<uint32_t>(ctx, TPT_BUF_PHOTONS),
-             *count = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP_COUNT);
+    uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), *photons = tpt_buf<uint32_t>(ctx, TPT_BUF_PHOTONS);
     unsigned long long *rec = (unsigned long long *)ctx->pix_rec;
-    ctx->maps_fresh = ctx->rec_cover = false;
-    tpt_pass_begin(ctx, PASS_RECALC_CLEAR);
     // P is a multiple of 16 (xres and yres are multiples of 4).
     const uint32_t n4 = P / 4;
+    tpt_pass_begin(ctx, pass_clear);
     hipLaunchKernelGGL(recalc_clear, dim3((n4 + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, (uint4 *)rec,
                        (uint4 *)photons, (uint4 *)count, n4, ctx->counters);
-    tpt_pass_end(ctx, PASS_RECALC_CLEAR);
-    tpt_pass_begin(ctx, PASS_RECALC_SCATTER);
+    tpt_pass_end(ctx, pass_clear);
+    tpt_pass_begin(ctx, pass_scatter);
     hipLaunchKernelGGL(recalc_scatter, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
                        tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
                        tpt_buf<float>(ctx, TPT_BUF_PART_X), tpt_buf<float>(ctx, TPT_BUF_PART_Y),
                        tpt_buf<float>(ctx, TPT_BUF_PART_TEMP), tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS),
                        tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), rec,
                        photons, count, ctx->slot_scratch_u, ctx->offmap, ctx->counters, slots, xres, yres,
-                       (int)ctx->xcells);
+                       (int)ctx->xcells, decay ? 1u : 0u);
     hipLaunchKernelGGL(recalc_invis, dim3(64), dim3(BLK), 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY),
                        tpt_buf<float>(ctx, TPT_BUF_PART_X), tpt_buf<float>(ctx, TPT_BUF_PART_Y),
                        tpt_buf<float>(ctx, TPT_BUF_PART_TEMP), rec, ctx->slot_scratch_u, ctx->counters, xres);
     hipLaunchKernelGGL(recalc_extract, dim3((n4 + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
                        (const ulonglong2 *)rec, (uint4 *)pmap, (uint4 *)count, n4);
+    tpt_pass_end(ctx, pass_scatter);
+    return TPT_OK;
+}
+
+// Incremental form (see the file comment), valid while rebuild_fresh.
+static int map_update(tpt_ctx *ctx) {
+    const uint32_t P = ctx->pixels, slots = ctx->slot_bound;
+    const int xres = (int)ctx->cfg.xres, yres = (int)ctx->cfg.yres;
+    uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), *photons = tpt_buf<uint32_t>(ctx, TPT_BUF_PHOTONS),
+             *count = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP_COUNT);
+    unsigned long long *rec = (unsigned long long *)ctx->pix_rec;
+    const uint32_t *ty = tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY);
+    const float *px = tpt_buf<float>(ctx, TPT_BUF_PART_X), *py = tpt_buf<float>(ctx, TPT_BUF_PART_Y),
+                *temp = tpt_buf<float>(ctx, TPT_BUF_PART_TEMP);
+    const uint32_t *props = tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS);
+    const uint32_t n4 = P / 4;
+    tpt_pass_begin(ctx, PASS_RECALC_CLEAR);
+    TPT_HIP_CHECK(hipMemcpyAsync(count, ctx->pix_count_mv, (size_t)P * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    tpt_pass_end(ctx, PASS_RECALC_CLEAR);
+    tpt_pass_begin(ctx, PASS_RECALC_SCATTER);
+    hipLaunchKernelGGL(recalc_decay, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE), px, py,
+                       props, tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), rec,
+                       photons, count, slots, xres, yres, (int)ctx->xcells);
+    hipLaunchKernelGGL(recalc_relist, dim3(256), dim3(BLK), 0, ctx->stream, ty, px, py, temp, props, rec, photons,
+                       ctx->offmap, ctx->counters, slots, xres, yres);
+    hipLaunchKernelGGL(recalc_invis_list, dim3(64), dim3(BLK), 0, ctx->stream, ty, px, py, temp, rec, ctx->offmap,
+                       ctx->counters, slots, xres, yres);
+    hipLaunchKernelGGL(recalc_extract, dim3((n4 + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
+                       (const ulonglong2 *)rec, (uint4 *)pmap, (uint4 *)nullptr, n4);
     tpt_pass_end(ctx, PASS_RECALC_SCATTER);
-    int rc = tpt_check_launch("recalc_free_particles");
+    return TPT_OK;
+}
+
+int tpt_move_rebuild_maps(tpt_ctx *ctx) {
+    int rc = map_build(ctx, 0, ctx->pix_count_mv, PASS_MOVE_REBUILD, PASS_MOVE_REBUILD);
+    if (rc == TPT_OK) rc = tpt_check_launch("map rebuild");
+    ctx->rebuild_fresh = rc == TPT_OK;
+    return rc;
+}
+
+int tpt_recalc_step(tpt_ctx *ctx, uint32_t seed, int stacking) {
+    const uint32_t P = ctx->pixels, slots = ctx->slot_bound;
+    const int xres = (int)ctx->cfg.xres, yres = (int)ctx->cfg.yres;
+    uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), *count = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP_COUNT);
+    unsigned long long *rec = (unsigned long long *)ctx->pix_rec;
+    const bool incremental = ctx->rebuild_fresh;
+    ctx->maps_fresh = ctx->rec_cover = ctx->rebuild_fresh = false;
+    int rc = incremental ? map_update(ctx) : map_build(ctx, 1, count, PASS_RECALC_CLEAR, PASS_RECALC_SCATTER);
+    if (rc == TPT_OK) rc = tpt_check_launch("recalc_free_particles");
     if (rc) return rc;
     if (stacking) {
         const uint32_t npart = ctx->cfg.xres * ctx->cfg.yres;

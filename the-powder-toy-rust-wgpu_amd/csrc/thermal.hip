@@ -282,6 +282,7 @@ __global__ __launch_bounds__(BLK) void th_apply_list(Parts p, const uint32_t *__
 // cell results are reduced with lane shuffles and written by the block,
 // merged with what the off-map list recorded in hv_key / fails.
 constexpr int TW = 64, TH = 16, LW = TW + 2, LH = TH + 2;
+constexpr int LOAD_ITERS = (LH * LW + TW * 4 - 1) / (TW * 4);  // halo-tile loads per thread
 
 // Types whose conduction reads life / tmp / ctype besides the type itself.
 __device__ __forceinline__ bool special_type(uint32_t t) {
@@ -351,13 +352,24 @@ __global__ __launch_bounds__(256) void th_conduct_px(Parts p, const unsigned lon
         chv[t] = (aheat && cx < d.xc && cy < d.yc) ? air_hv[cy * d.xc + cx] : 0.0f;  // pre-pass (R2)
     }
     if (t == 0) count = 0u;
-    for (int e = t; e < LH * LW; e += TW * 4) {
-        const int ly = e / LW, lx = e - ly * LW;
-        const int gx = x0 + lx - 1, gy = y0 + ly - 1;
-        unsigned long long v = 0ull;
-        if (gx >= 0 && gy >= 0 && gx < d.xres && gy < d.yres) v = rec[(size_t)gy * d.xres + gx];
-        skey[ly][lx] = (uint32_t)(v >> 32);
-        stemp[ly][lx] = __uint_as_float((uint32_t)v);
+    {
+        // All halo loads in flight before the first LDS store.
+        unsigned long long v[LOAD_ITERS];
+#pragma unroll
+        for (int u = 0; u < LOAD_ITERS; ++u) {
+            const int e = t + u * TW * 4, ly = e / LW, lx = e - ly * LW;
+            const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+            v[u] = 0ull;
+            if (e < LH * LW && gx >= 0 && gy >= 0 && gx < d.xres && gy < d.yres) v[u] = rec[(size_t)gy * d.xres + gx];
+        }
+#pragma unroll
+        for (int u = 0; u < LOAD_ITERS; ++u) {
+            const int e = t + u * TW * 4, ly = e / LW, lx = e - ly * LW;
+            if (e < LH * LW) {
+                skey[ly][lx] = (uint32_t)(v[u] >> 32);
+                stemp[ly][lx] = __uint_as_float((uint32_t)v[u]);
+            }
+        }
     }
     __syncthreads();
     const int x = x0 + tx;
@@ -502,20 +514,33 @@ __global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long 
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
     sflag[t] = type_flags(props, (uint32_t)t);
     if (t == 0) count = 0u;
-    for (int e = t; e < LH * LW; e += TW * 4) {
-        const int ly = e / LW, lx = e - ly * LW;
-        const int gx = x0 + lx - 1, gy = y0 + ly - 1;
-        unsigned long long v = 0ull;
-        uint2 h = make_uint2(__float_as_uint(-1.0f), 0u);
-        if (gx >= 0 && gy >= 0 && gx < d.xres && gy < d.yres) {
-            const size_t pix = (size_t)gy * d.xres + gx;
-            v = rec[pix];
-            if (v >> 32) h = heat[pix];
+    {
+        // All halo loads in flight before the first LDS store (every canvas
+        // pixel's heat entry was written by the conduction tile).
+        unsigned long long v[LOAD_ITERS];
+        uint2 h[LOAD_ITERS];
+#pragma unroll
+        for (int u = 0; u < LOAD_ITERS; ++u) {
+            const int e = t + u * TW * 4, ly = e / LW, lx = e - ly * LW;
+            const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+            v[u] = 0ull;
+            h[u] = make_uint2(__float_as_uint(-1.0f), 0u);
+            if (e < LH * LW && gx >= 0 && gy >= 0 && gx < d.xres && gy < d.yres) {
+                const size_t pix = (size_t)gy * d.xres + gx;
+                v[u] = rec[pix];
+                h[u] = heat[pix];
+            }
         }
-        if (ly >= 1 && ly <= TH && lx >= 1 && lx <= TW) sown[ly - 1][lx - 1] = __uint_as_float((uint32_t)v);
-        skey[ly][lx] = (uint32_t)(v >> 32);
-        seq[ly][lx] = __uint_as_float(h.x);
-        sinfo[ly][lx] = h.y;
+#pragma unroll
+        for (int u = 0; u < LOAD_ITERS; ++u) {
+            const int e = t + u * TW * 4, ly = e / LW, lx = e - ly * LW;
+            if (e < LH * LW) {
+                if (ly >= 1 && ly <= TH && lx >= 1 && lx <= TW) sown[ly - 1][lx - 1] = __uint_as_float((uint32_t)v[u]);
+                skey[ly][lx] = (uint32_t)(v[u] >> 32);
+                seq[ly][lx] = __uint_as_float(h[u].x);
+                sinfo[ly][lx] = h[u].y;
+            }
+        }
     }
     __syncthreads();
     const int x = x0 + tx;
@@ -732,10 +757,16 @@ __global__ __launch_bounds__(BLK) void th_transition(Parts p, const float *__res
                                                      uint32_t seed) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.slots) return;
+    // The slot's fields are loaded together up front; testing them one by
+    // one would chain a memory latency per branch.
     uint32_t t = p.ty[i];
+    const float fx = p.x[i], fy = p.y[i];
+    const uint32_t ctype0 = p.ctype[i];
+    const int32_t tmp0 = p.tmp[i], tmp20 = p.tmp2[i], life0 = p.life[i];
+    const float eq0 = p.equil[i];
     if (t == 0u || t >= 256u) return;
     const uint32_t t_entry = t;
-    const int x = tpt_f2i(p.x[i]), y = tpt_f2i(p.y[i]);
+    const int x = tpt_f2i(fx), y = tpt_f2i(fy);
     if (x < 0 || x >= d.xres || y < 0 || y >= d.yres) return;
     const int ci = (y / TPT_CELL) * d.xc + x / TPT_CELL;
     const float pv0 = air_pv[ci];
@@ -743,9 +774,9 @@ __global__ __launch_bounds__(BLK) void th_transition(Parts p, const float *__res
     const uint32_t roll = gpu_hash(seed ^ (i * PARTICLE_HASH_SALT) ^ t ^ TRANSITION_HASH_SALT);
     if (!legacy) {
         const uint32_t tprops0 = EP(props, t, COL_PROPERTIES);
-        uint32_t ctype = p.ctype[i];
-        const int32_t tmp = p.tmp[i], tmp2 = p.tmp2[i];
-        const float eq = p.equil[i];
+        uint32_t ctype = ctype0;
+        const int32_t tmp = tmp0, tmp2 = tmp20;
+        const float eq = eq0;
         if (eq >= 0.0f) {
             const float pt = eq;
             if ((t == T_ICEI || t == T_SNOW) && (ctype == 0u || ctype >= 256u || ctype == T_ICEI || ctype == T_SNOW)) {
@@ -766,7 +797,7 @@ __global__ __launch_bounds__(BLK) void th_transition(Parts p, const float *__res
             const float ctemph = boil ? pt - pv_shift : pt;
             const float ctempl = cond ? pt - pv_shift : pt;
             uint32_t old_ty = t, nt = t, new_ctype = ctype;
-            int32_t new_life = p.life[i], new_tmp = tmp, new_tmp2 = tmp2;
+            int32_t new_life = life0, new_tmp = tmp, new_tmp2 = tmp2;
             bool s = false;
             if (high_tr != NT && ctemph >= high_temp) {
                 if (high_tr != ST) {
@@ -1054,7 +1085,7 @@ int tpt_thermal_step(tpt_ctx *ctx, uint32_t seed, int legacy) {
     const uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), *props = tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS);
     const float *gfx = tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), *gfy = tpt_buf<float>(ctx, TPT_BUF_GRAV_FY);
     const bool fresh = ctx->maps_fresh;
-    ctx->maps_fresh = false;
+    ctx->maps_fresh = ctx->rebuild_fresh = false;
     if (!legacy && fresh) {
         // Pixel-major path over the records of the RecalcFreeParticles that
         // ran just before; the off-map list first, so its cell writes and

@@ -95,7 +95,8 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMalloc(&c->slot_scratch_f, cap * 4) != hipSuccess || hipMalloc(&c->counters, 256) != hipSuccess ||
         hipMalloc(&c->pix_rec, (size_t)c->pixels * 8) != hipSuccess ||
         hipMalloc(&c->pix_heat, (size_t)c->pixels * 8) != hipSuccess || hipMalloc(&c->offmap, cap * 4) != hipSuccess ||
-        hipMalloc(&c->pix_type, (size_t)c->pixels) != hipSuccess)
+        hipMalloc(&c->pix_type, (size_t)c->pixels) != hipSuccess ||
+        hipMalloc(&c->pix_count_mv, (size_t)c->pixels * 4) != hipSuccess)
         return tpt_set_error(TPT_ENOMEM, "scratch allocation failed");
     hipMemsetAsync(c->cell_scratch_u, 0, cb, c->stream);
     hipMemsetAsync(c->cell_scratch_u2, 0, cb, c->stream);
@@ -108,7 +109,7 @@ static void free_all(tpt_ctx *c) {
         if (c->buf[id]) hipFree(c->buf[id]);
     void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
                      c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters,
-                     c->pix_rec, c->pix_heat, c->offmap, c->pix_type};
+                     c->pix_rec, c->pix_heat, c->offmap, c->pix_type, c->pix_count_mv};
     for (void *p : extra)
         if (p) hipFree(p);
     tpt_gravity_free(c);
@@ -176,6 +177,7 @@ extern "C" int tpt_set_element_table(tpt_ctx *ctx, const uint32_t *props, size_t
     if (props_words != (size_t)TPT_PT_NUM * TPT_ELEM_STRIDE || lut_words != (size_t)TPT_MOVE_LUT_WORDS)
         return tpt_set_error(TPT_EINVAL, "table sizes %zu/%zu, expected %u/%u", props_words, lut_words,
                              TPT_PT_NUM * TPT_ELEM_STRIDE, TPT_MOVE_LUT_WORDS);
+    ctx->maps_fresh = ctx->rec_cover = ctx->rebuild_fresh = false;
     TPT_HIP_CHECK(hipMemcpyAsync(ctx->buf[TPT_BUF_ELEM_PROPS], props, props_words * 4, hipMemcpyHostToDevice, ctx->stream));
     TPT_HIP_CHECK(hipMemcpyAsync(ctx->buf[TPT_BUF_MOVE_LUT], move_lut, lut_words * 4, hipMemcpyHostToDevice, ctx->stream));
     TPT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -198,8 +200,7 @@ extern "C" int tpt_upload(tpt_ctx *ctx, int buf, const void *src, size_t off, si
     if (rc) return rc;
     if (!src && bytes) return tpt_set_error(TPT_EINVAL, "null source");
     if (!bytes) return TPT_OK;
-    ctx->maps_fresh = false;
-    ctx->rec_cover = false;
+    ctx->maps_fresh = ctx->rec_cover = ctx->rebuild_fresh = false;
     TPT_HIP_CHECK(hipMemcpyAsync((char *)ctx->buf[buf] + off, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     TPT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     if (buf == TPT_BUF_ELEM_PROPS || buf == TPT_BUF_MOVE_LUT) ctx->tables_set = true;
@@ -227,7 +228,7 @@ extern "C" int tpt_set_slot_bound(tpt_ctx *ctx, uint32_t bound) {
     if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
     if (bound < 1) bound = 1;
     if (bound > ctx->cfg.capacity) bound = ctx->cfg.capacity;
-    if (bound != ctx->slot_bound) ctx->maps_fresh = ctx->rec_cover = false;
+    if (bound != ctx->slot_bound) ctx->maps_fresh = ctx->rec_cover = ctx->rebuild_fresh = false;
     ctx->slot_bound = bound;
     return TPT_OK;
 }

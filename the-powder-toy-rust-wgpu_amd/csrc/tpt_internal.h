@@ -100,6 +100,11 @@ struct tpt_ctx {
     // RecalcFreeParticles (types may have changed since): set there,
     // cleared by movement, uploads and slot-bound changes.
     bool rec_cover = false;
+    // True from the movement pass's map rebuild until anything other than
+    // air or gravity runs: the next RecalcFreeParticles may then update the
+    // rebuilt maps instead of rebuilding them.
+    bool rebuild_fresh = false;
+    uint32_t *pix_count_mv = nullptr;     // pixels: particle counts of the movement rebuild
     tpt_fft_state fft;
 
     // Profiling: a ring of event pairs per pass, folded lazily so that
@@ -141,6 +146,7 @@ int tpt_gravity_free(tpt_ctx *ctx);
 int tpt_recalc_step(tpt_ctx *ctx, uint32_t seed, int stacking);
 int tpt_thermal_step(tpt_ctx *ctx, uint32_t seed, int legacy);
 int tpt_movement_step(tpt_ctx *ctx, uint32_t seed);
+int tpt_move_rebuild_maps(tpt_ctx *ctx);
 
 // Device helpers ------------------------------------------------------------
 __device__ __forceinline__ float tpt_clampf(float v, float lo, float hi) {
