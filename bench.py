@@ -42,22 +42,39 @@ def kernel_bytes(name, C, P, S, N, W, yc):
         "grav_fft_inverse_extract": 16 * yc * W + 12 * C,
         "grav_apply": 4 * S + 36 * N,         # ty scan; x,y,vx,vy,flags,force gather in; vx,vy out
         "grav_build_mass": 4 * S + 12 * C,    # ty scan; clear + copy of the mass grid
-        "recalc_clear": 12 * P,               # pmap, photons, pmap_count
-        "recalc_scatter": 4 * S + 20 * N,     # ty scan; life,x,y in; life out; pmap/count atomics
+        "recalc_clear": 8 * P,                # pmap_count from the movement rebuild's counts
+        "recalc_scatter": 4 * S + 12 * P,     # ty scan (life decay); records in, pmap out
         "check_stacking": 4 * P + 4 * S + 12 * N,
-        "thermal_conduction": 4 * S + 32 * N, # ty scan; x,y,life,tmp,ctype,temp in; equil, hv out
+        "thermal_conduction": 16 * P + 4 * N + 16 * C,  # records in, per-pixel results out; equil out
         "thermal_conduction_cells": 16 * C,
-        "thermal_apply": 4 * S + 32 * N,
-        "phase_transition": 12 * S + 64 * N,
+        "thermal_apply": 16 * P + 8 * N,      # records + results in; temp out, record temp out
+        "phase_transition": 12 * S + 72 * N,  # convection (4 words + partner record) + transitions
         "phase_transition_cells": 16 * C,
         "refresh_pmap_types": 4 * S + 16 * N,
-        "air_deposit": 4 * S + 20 * N + 16 * C,
-        "move_clear_intents": 4 * P,
+        "air_deposit": 8 * P + 16 * C,        # records in; per-cell winner gathers; air_vx,vy RW
+        "move_clear_intents": 9 * P,          # cell_winner clear; pmap in, type bytes out
         "move_velocity_intent": 4 * S + 52 * N,  # ty scan; x,y,vx,vy,flags,tmp,air in; vx,vy,intent out; claim
+        "move_saw": 0,
         "move_commit": 4 * S + 32 * N,
-        "move_rebuild_maps": 8 * P + 4 * S + 16 * N,
+        "move_rebuild_maps": 36 * P + 4 * S + 20 * N,  # records/photons/counts clear; scatter; pmap + counts
     }
     return table.get(name)
+
+
+# Timed passes that are one kernel, for the roofline's HBM traffic (the
+# rocprofv3 summary is per kernel).
+PASS_KERNEL = {"move_velocity_intent": "mv_intent", "move_commit": "mv_commit", "grav_apply": "grav_apply",
+               "grav_fft_cols": "grav_cols", "thermal_apply": "th_apply_px"}
+
+
+def traffic_table(workload):
+    """Calibrated HBM bytes per dispatch by kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, if any."""
+    path = os.path.join(ROOT, "profiles", f"tick_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        return json.load(fh)["kernels"], os.path.relpath(path, ROOT)
 
 
 def run_pass(ctx, name, frame):
@@ -163,9 +180,13 @@ def main():
     dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches"])
     d = per_kernel[dom]
     achieved = d["gbs"]
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                "traffic": None, "alg_bytes_per_launch": d["alg_bytes"], "avg_us": d["avg_us"]}
+    traffic, traffic_src = traffic_table(args.workload)
+    kern = PASS_KERNEL.get(dom)
+    measured = traffic.get(kern, {}).get("hbm_bytes") if (traffic and kern) else None
+    roofline = {"bound": "hbm", "kernel": kern or dom, "pass": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                "traffic": measured, "traffic_source": traffic_src if measured else None,
+                "alg_bytes_per_launch": d["alg_bytes"], "avg_us": d["avg_us"]}
     ctx.close()
 
     cpu = None

@@ -577,7 +577,133 @@ static uint32_t fall_down_destination(mctx *m, float *pvx, float *pvy, float px,
     return target;
 }
 
-/* velocity.rs:755-928 without the glass refraction */
+/* Photon refraction at glass: velocity.rs:262-647 (Simulation.cpp:1626-1750,
+ * 886-922, 2966-3005). Only GLAS and BGLA bend light. */
+static int refract_blocking(const mctx *m, int x, int y) {
+    if (x < 0 || y < 0 || x >= m->xr || y >= m->yr) return 0;
+    uint32_t t = m->s->pmap[y * m->xr + x] & 0x1FFu;
+    return t == 24u || t == 95u;
+}
+
+static int refract_boundary(const mctx *m, int x, int y) {
+    return refract_blocking(m, x, y) &&
+           !(refract_blocking(m, x, y - 1) && refract_blocking(m, x, y + 1) && refract_blocking(m, x - 1, y) &&
+             refract_blocking(m, x + 1, y));
+}
+
+static const int ring_dx[8] = {1, 1, 0, -1, -1, -1, 0, 1};
+static const int ring_dy[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+static const uint32_t ring_mask[8] = {0x83u, 0x07u, 0x0Eu, 0x1Cu, 0x38u, 0x70u, 0xE0u, 0xC1u};
+
+/* find_next_boundary (velocity.rs:343-389): the next boundary step of a walk
+ * around the glass; returns 1 and moves (x, y, em) when one exists. */
+static int next_boundary(const mctx *m, int *x, int *y, uint32_t dm, int *em, int reverse) {
+    if (!(*x > 0 && *x < m->xr - 1 && *y > 0 && *y < m->yr - 1)) return 0;
+    if (*em != -1) dm &= ring_mask[*em];
+    uint32_t mask = 0;
+    for (int i = 0; i < 8; ++i)
+        if ((dm & (1u << i)) && refract_blocking(m, *x + ring_dx[i], *y + ring_dy[i])) mask |= 1u << i;
+    for (int i = 0; i < 8; ++i) {
+        int n = (i + (reverse ? 1 : 7)) & 7;
+        if ((mask & (1u << i)) && !(mask & (1u << n))) {
+            *x += ring_dx[i];
+            *y += ring_dy[i];
+            *em = i;
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* get_normal (velocity.rs:401-456) */
+static int surface_normal(const mctx *m, int x, int y, float dx, float dy, float *nx, float *ny) {
+    if (!((dx != 0.0f || dy != 0.0f) && refract_boundary(m, x, y))) return 0;
+    int lx = x, ly = y, rx = x, ry = y, lm = -1, rm = -1, lv = 1, rv = 1, j = 0;
+    for (int k = 0; k < 10; ++k) {
+        if (lv) lv = next_boundary(m, &lx, &ly, 0xFFu, &lm, 1);
+        if (rv) rv = next_boundary(m, &rx, &ry, 0xFFu, &rm, 0);
+        j += lv + rv;
+        if (!lv && !rv) break;
+    }
+    if (!(j >= 3 && (lx != rx || ly != ry))) return 0;
+    float ex = (float)(rx - lx), ey = (float)(ry - ly);
+    float inv = 1.0f / sqrtf(ex * ex + ey * ey);
+    *nx = ey * inv;
+    *ny = -ex * inv;
+    return 1;
+}
+
+/* get_normal_interp (velocity.rs:462-516) */
+static int surface_normal_along(const mctx *m, float x0, float y0, float dx, float dy, float *nx, float *ny) {
+    dx = dx / 16.0f;
+    dy = dy / 16.0f;
+    for (int k = 0; k < 20; ++k) {
+        int x = ora_f2i(x0 + 0.5f), y = ora_f2i(y0 + 0.5f);
+        if (x < 0 || y < 0 || x >= m->xr || y >= m->yr) return 0;
+        if (refract_boundary(m, x, y)) return surface_normal(m, x, y, dx, dy, nx, ny);
+        x0 += dx;
+        y0 += dy;
+    }
+    return 0;
+}
+
+/* get_wavelength_bin (velocity.rs:523-560); -1: no wavelength left. */
+static int wavelength_bin(uint32_t wm, uint32_t seed, uint32_t *out) {
+    uint32_t masked = wm & 0x3FFFFFFFu;
+    *out = wm;
+    if (!masked) return -1;
+    uint32_t lo = 30, hi = 0;
+    for (uint32_t i = 0; i < 30; ++i)
+        if (masked & (1u << i)) {
+            if (i < lo) lo = i;
+            if (i > hi) hi = i;
+        }
+    if (hi - lo < 5u) return (int)(hi + lo);
+    uint32_t r = oracle_pcg_hash(seed ^ (lo * 2654435769u));
+    uint32_t i = ((r >> 1) % (hi - lo - 4u)) + lo;
+    if (r & 1u) {
+        *out = wm & (0x1Fu << i);
+        return (int)((i + 2u) * 2u);
+    }
+    *out = wm & (0xFu << i);
+    return (int)((i + 2u) * 2u - 1u);
+}
+
+/* refract_photon (velocity.rs:577-637): 0 = passes, 1 = dies, 2 = total
+ * internal reflection (stays this tick). */
+static int refract_photon(mctx *m, float px, float py, int xi, int yi, float *vx, float *vy, int fx, int fy,
+                          uint32_t slot, uint32_t seed) {
+    const oracle_state *s = m->s;
+    uint32_t rt = s->pmap[fy * m->xr + fx] & 0x1FFu, lt = s->pmap[yi * m->xr + xi] & 0x1FFu;
+    int r_glass = rt == 24u || rt == 95u, l_glass = lt == 24u || lt == 95u;
+    if (r_glass == l_glass) return 0;
+    float nrx, nry;
+    if (!surface_normal_along(m, px, py, *vx, *vy, &nrx, &nry)) return 1;
+    uint32_t narrowed;
+    int bin = wavelength_bin(s->part_ctype[slot], seed, &narrowed);
+    if (bin == -1) return 1;
+    s->part_ctype[slot] = narrowed;
+    float nn = 1.9f - 0.07f * ((float)bin - 30.0f) / 30.0f;
+    nn *= nn;
+    if (r_glass) { /* entering the glass */
+        nrx = -nrx;
+        nry = -nry;
+        nn = 1.0f / nn;
+    }
+    float ct1 = *vx * nrx + *vy * nry;
+    float ct2 = 1.0f - (nn * nn) * (1.0f - (ct1 * ct1));
+    if (ct2 < 0.0f) {
+        *vx -= 2.0f * ct1 * nrx;
+        *vy -= 2.0f * ct1 * nry;
+        return 2;
+    }
+    float ct2s = sqrtf(ct2) - nn * ct1;
+    *vx = nn * *vx + ct2s * nrx;
+    *vy = nn * *vy + ct2s * nry;
+    return 0;
+}
+
+/* velocity.rs:755-928 */
 static uint32_t gas_destination(mctx *m, float *pvx, float *pvy, float px, float py, int xi, int yi, int sx, int sy,
                                 uint32_t props, uint32_t ty, uint32_t slot, float collision, uint32_t seed) {
     const oracle_state *s = m->s;
@@ -589,12 +715,17 @@ static uint32_t gas_destination(mctx *m, float *pvx, float *pvy, float px, float
         fin_y = yi + sy;
     }
     uint32_t target = NO_INTENT;
-    int blocked = 0, died = 0;
+    int blocked = 0, died = 0, tir = 0;
     if (fin_x != xi || fin_y != yi) {
         int ox, oy;
         uint32_t d = do_move_at(m, props, ty, fin_x, fin_y, &ox, &oy);
         if (d == DO_OK) {
-            target = pix(m, ox, oy);
+            if (ty == 29) {
+                int r = refract_photon(m, px, py, xi, yi, &vx, &vy, ox, oy, slot, seed);
+                died = r == 1;
+                tir = r == 2;
+            }
+            if (!died && !tir) target = pix(m, ox, oy);
         } else if (d == DO_DIE) {
             died = 1;
         } else {
@@ -943,9 +1074,119 @@ void oracle_rebuild_maps(oracle_state *s) {
     }
 }
 
-/* step_movement_gpu (mod.rs:1176-1227) without the pair pass and WL_DETECT. */
+/* WL_DETECT charging and the set_emap closure (mod.rs:348-828,
+ * Simulation.cpp:928-983, 2330-2331). Emap values live in the low 6 bits;
+ * bits 7/6 mark the left/right ends of a span filled in this pass. */
+#define EMAP_CHARGED 16u
+#define EMAP_LEFT 0x80u
+#define EMAP_RIGHT 0x40u
+
+static int is_wire(uint32_t w) { return w == 3 || w == 2 || w == 6 || w == 1 || w == 5 || w == 12 || w == 18; }
+static int wire_off(const oracle_state *s, uint32_t c) { return is_wire(s->wall_bmap[c]) && (s->wall_emap[c] & 0x3Fu) < 8u; }
+static int wire_at(const oracle_state *s, int x, int y, int xc, int yc) {
+    return x >= 0 && y >= 0 && x < xc && y < yc && is_wire(s->wall_bmap[y * xc + x]);
+}
+
+/* Charges the uncharged wire run through (x, y) and marks its ends. */
+static void fill_run(oracle_state *s, uint32_t row, uint32_t x1, uint32_t x2) {
+    for (uint32_t x = x1; x <= x2; ++x) s->wall_emap[row + x] = EMAP_CHARGED;
+    s->wall_emap[row + x1] |= EMAP_LEFT;
+    s->wall_emap[row + x2] |= EMAP_RIGHT;
+}
+
+/* ladder shortcuts of set_emap (mod.rs:412-456, Simulation.cpp:955-985):
+ * dir -1 looks up (cap row below the target), +1 looks down. */
+static int ladder(const oracle_state *s, int singleton, int x, int target, int dir, int xc, int yc) {
+    int cap = target - dir;
+    return singleton && wire_at(s, x - 1, cap, xc, yc) && wire_at(s, x, cap, xc, yc) && wire_at(s, x + 1, cap, xc, yc) &&
+           !wire_at(s, x - 1, target, xc, yc) && wire_at(s, x, target, xc, yc) && !wire_at(s, x + 1, target, xc, yc);
+}
+
+/* detect_charge_kernel (mod.rs:474-545) in slot order. */
+void oracle_detect_charge(oracle_state *s) {
+    const uint32_t xres = s->xres, yres = s->yres, xc = xres / 4;
+    for (uint32_t i = 0; i < s->slot_bound; ++i) {
+        uint32_t ty = s->part_ty[i];
+        if (ty == 0 || ty >= 256 || (s->part_flags[i] & 0x10u)) continue;
+        int xi = ora_f2i(s->part_x[i] + 0.5f), yi = ora_f2i(s->part_y[i] + 0.5f);
+        if (xi < 4 || yi < 4 || xi >= (int)xres - 4 || yi >= (int)yres - 4) continue;
+        uint32_t cx = (uint32_t)xi / 4, row = ((uint32_t)yi / 4) * xc, cell = row + cx;
+        if (s->wall_bmap[cell] != 3u || (s->wall_emap[cell] & 0x3Fu) >= 8u) continue;
+        uint32_t x1 = cx, x2 = cx;
+        while (x1 > 0 && wire_off(s, row + x1 - 1)) --x1;
+        while (x2 + 1 < xc && wire_off(s, row + x2 + 1)) ++x2;
+        fill_run(s, row, x1, x2);
+    }
+}
+
+/* emap_gate_kernel + emap_closure_kernel + strip (mod.rs:556-828). */
+void oracle_emap_closure(oracle_state *s) {
+    const int xc = (int)(s->xres / 4), yc = (int)(s->yres / 4);
+    const uint32_t cells = (uint32_t)(xc * yc);
+    int any_off = 0, any_charged = 0;
+    for (uint32_t c = 0; c < cells; ++c) {
+        any_off |= wire_off(s, c);
+        any_charged |= (s->wall_emap[c] & 0x3Fu) == EMAP_CHARGED;
+    }
+    if (any_off && any_charged) {
+        int changed = 1;
+        for (int pass = 0; changed && pass < yc + 2; ++pass) {
+            changed = 0;
+            for (int y = 0; y < yc; ++y) {
+                uint32_t row = (uint32_t)(y * xc);
+                for (int x = 0; x < xc;) {
+                    if (!wire_off(s, row + (uint32_t)x)) {
+                        ++x;
+                        continue;
+                    }
+                    int x1 = x, x2 = x;
+                    while (x2 + 1 < xc && wire_off(s, row + (uint32_t)x2 + 1)) ++x2;
+                    int act = 0;
+                    /* child above: a charged span one row below */
+                    for (int xx = x1; xx <= x2 && !act && y + 1 < yc; ++xx) {
+                        uint32_t se = s->wall_emap[(uint32_t)((y + 1) * xc + xx)];
+                        int sy = y + 1, single = (se & (EMAP_LEFT | EMAP_RIGHT)) == (EMAP_LEFT | EMAP_RIGHT);
+                        if ((se & 0x3Fu) == EMAP_CHARGED && !ladder(s, single, xx, sy - 2, -1, xc, yc) &&
+                            ((se & (EMAP_LEFT | EMAP_RIGHT)) || y + 1 == yc - 1 || wire_at(s, xx - 1, y, xc, yc) ||
+                             wire_at(s, xx + 1, y, xc, yc) || wire_at(s, xx - 1, sy + 1, xc, yc) ||
+                             !wire_at(s, xx, sy + 1, xc, yc) || wire_at(s, xx + 1, sy + 1, xc, yc)))
+                            act = 1;
+                    }
+                    /* child below: a charged span one row above */
+                    for (int xx = x1; xx <= x2 && !act && y > 0; ++xx) {
+                        uint32_t se = s->wall_emap[(uint32_t)((y - 1) * xc + xx)];
+                        int sy = y - 1, single = (se & (EMAP_LEFT | EMAP_RIGHT)) == (EMAP_LEFT | EMAP_RIGHT);
+                        if ((se & 0x3Fu) == EMAP_CHARGED && !ladder(s, single, xx, y + 1, 1, xc, yc) &&
+                            ((se & (EMAP_LEFT | EMAP_RIGHT)) || wire_at(s, xx - 1, y, xc, yc) ||
+                             wire_at(s, xx + 1, y, xc, yc) || wire_at(s, xx - 1, sy - 1, xc, yc) ||
+                             !wire_at(s, xx, sy - 1, xc, yc) || wire_at(s, xx + 1, sy - 1, xc, yc)))
+                            act = 1;
+                    }
+                    /* ladder jumps from a singleton two rows away */
+                    for (int xx = x1; xx <= x2 && !act; ++xx) {
+                        int up = y + 2 < yc && (s->wall_emap[(uint32_t)((y + 2) * xc + xx)] & (EMAP_LEFT | EMAP_RIGHT)) ==
+                                                   (EMAP_LEFT | EMAP_RIGHT);
+                        int down = y >= 2 && (s->wall_emap[(uint32_t)((y - 2) * xc + xx)] & (EMAP_LEFT | EMAP_RIGHT)) ==
+                                                 (EMAP_LEFT | EMAP_RIGHT);
+                        if (ladder(s, up, xx, y, -1, xc, yc) || ladder(s, down, xx, y, 1, xc, yc)) act = 1;
+                    }
+                    if (act) {
+                        fill_run(s, row, (uint32_t)x1, (uint32_t)x2);
+                        changed = 1;
+                    }
+                    x = x2 + 1;
+                }
+            }
+        }
+    }
+    for (uint32_t c = 0; c < cells; ++c) s->wall_emap[c] &= ~(EMAP_LEFT | EMAP_RIGHT);
+}
+
+/* step_movement_gpu (mod.rs:1176-1227) without the pair pass. */
 void oracle_step_movement(oracle_state *s, uint32_t seed) {
     oracle_air_deposit(s);
+    oracle_detect_charge(s);
+    oracle_emap_closure(s);
     oracle_velocity_intent(s, seed);
     oracle_commit(s);
     oracle_rebuild_maps(s);

@@ -93,7 +93,8 @@ def lib():
         _lib.oracle_recalc_free_particles.restype = None
         _lib.oracle_pcg_hash.argtypes = [ctypes.c_uint32]
         _lib.oracle_pcg_hash.restype = ctypes.c_uint32
-        for name, extra in (("oracle_air_deposit", []), ("oracle_velocity_intent", [ctypes.c_uint32]),
+        for name, extra in (("oracle_air_deposit", []), ("oracle_detect_charge", []), ("oracle_emap_closure", []),
+                            ("oracle_velocity_intent", [ctypes.c_uint32]),
                             ("oracle_commit", []), ("oracle_rebuild_maps", []),
                             ("oracle_step_movement", [ctypes.c_uint32]), ("oracle_step", [ctypes.c_uint64])):
             f = getattr(_lib, name)

@@ -95,6 +95,8 @@ void oracle_step_thermal(oracle_state *s, uint32_t seed, int legacy);
 /* Movement (move_kernel/{mod,velocity,liquid,commit}.rs). */
 uint32_t oracle_pcg_hash(uint32_t seed);
 void oracle_air_deposit(oracle_state *s);
+void oracle_detect_charge(oracle_state *s);
+void oracle_emap_closure(oracle_state *s);
 void oracle_velocity_intent(oracle_state *s, uint32_t seed);
 void oracle_commit(oracle_state *s);
 void oracle_rebuild_maps(oracle_state *s);

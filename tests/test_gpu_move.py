@@ -112,3 +112,78 @@ def test_full_size_2048_invariants(gpu):
     assert np.isfinite(st.arrays["air_pv"]).all() and np.abs(st.arrays["air_pv"]).max() <= 256
     t = st.arrays["part_temp"][live]
     assert t.min() >= 0 and t.max() <= 9999
+
+
+def test_photon_refraction_bit_exact(gpu):
+    """Glass blocks in a photon swarm: refraction, total internal reflection,
+    dispersion draws and the glass walk, HIP vs the oracle."""
+    st = S.make_scene("612x384", scale_particles=0.1, newton_grav=False)
+    rng = np.random.default_rng(41)
+    a = st.arrays
+    occupied = set(zip(a["part_x"][: st.slot_bound].astype(int), a["part_y"][: st.slot_bound].astype(int)))
+    base, add = st.slot_bound, []
+    for _ in range(6):  # glass blocks, some with holes in their faces
+        x0, y0 = int(rng.integers(30, 500)), int(rng.integers(30, 300))
+        w, h = int(rng.integers(10, 60)), int(rng.integers(10, 50))
+        for y in range(y0, y0 + h):
+            for x in range(x0, x0 + w):
+                if (x, y) not in occupied and rng.random() > 0.02:
+                    occupied.add((x, y))
+                    add.append(("GLAS" if rng.random() > 0.3 else "BGLA", x, y, 0.0, 0.0, 0))
+    for _ in range(3000):
+        x, y = int(rng.integers(10, 600)), int(rng.integers(10, 370))
+        th = rng.uniform(0, 2 * np.pi)
+        mask = int(rng.choice([0x3FFFFFFF, (1 << 14) | (1 << 15), 0x3FF << 10, 0]))
+        add.append(("PHOT", x, y, 3.0 * np.cos(th), 3.0 * np.sin(th), mask))
+    for j, (name, x, y, vx, vy, ctype) in enumerate(add):
+        s = base + j
+        a["part_ty"][s], a["part_x"][s], a["part_y"][s] = E.pt(name), x, y
+        a["part_vx"][s], a["part_vy"][s], a["part_ctype"][s], a["part_temp"][s] = vx, vy, ctype, 300.0
+        a["part_life"][s] = 680 if name == "PHOT" else 0   # PHOT's default life
+    st.slot_bound = base + len(add)
+    ref = st.copy()
+    o = O.OracleState(ref)
+    print("[move parity] photons and glass", flush=True)
+    with L.Context(st.cfg) as ctx:
+        ctx.load_state(st)
+        for frame in range(8):
+            seed = L.frame_seed(frame)
+            ctx.recalc_free_particles(seed, False)
+            o.call("oracle_recalc_free_particles", seed, 0)
+            ctx.step_movement(seed)
+            o.call("oracle_step_movement", seed)
+        ctx.fetch(st)
+    compare(ref, st, INT_FIELDS + FLOAT_FIELDS, "photons and glass")
+
+
+def test_wire_walls_whole_tick_bit_exact(gpu):
+    """Detector walls under falling particles, runs and columns of every
+    conductor wall type, some charged at the start: WL_DETECT charging, the
+    set_emap closure and emap decay through whole ticks, HIP vs the oracle."""
+    st = S.make_scene("612x384", scale_particles=0.3, newton_grav=False)
+    rng = np.random.default_rng(53)
+    xc, yc = st.xcells, st.ycells
+    bmap, emap = st.arrays["wall_bmap"], st.arrays["wall_emap"]
+    wires = [3, 3, 3, 1, 2, 6, 5, 12, 18]
+    for _ in range(40):
+        w = int(rng.choice(wires))
+        x0, y0 = int(rng.integers(2, xc - 20)), int(rng.integers(2, yc - 12))
+        if rng.random() < 0.6:
+            cells = [(x, y0) for x in range(x0, x0 + int(rng.integers(2, 18)))]
+        else:
+            cells = [(x0, y) for y in range(y0, y0 + int(rng.integers(2, 10)))]
+        for (x, y) in cells:
+            bmap[y * xc + x] = w
+            if rng.random() < 0.05:
+                emap[y * xc + x] = 16
+    S.derive_block_masks(st)
+    ref = st.copy()
+    o = O.OracleState(ref)
+    print("[tick parity] wire walls, 10 ticks", flush=True)
+    with L.Context(st.cfg) as ctx:
+        ctx.load_state(st)
+        for frame in range(10):
+            ctx.step(frame)
+            o.call("oracle_step", frame)
+        ctx.fetch(st)
+    compare(ref, st, INT_FIELDS + FLOAT_FIELDS + ["wall_emap", "block_air", "block_airh"], "wire walls tick 10")

@@ -111,3 +111,141 @@ def test_sub_pixel_step_matches_hash():
         assert abs(tx - 64) <= 1 and abs(ty_ - 64) <= 1
     # the stochastic step fires iff r < |v| per axis (velocity.rs:98-111)
     assert abs(vx) < 1.0 and abs(vy) < 1.0
+
+
+# --- photon refraction at glass (velocity.rs:262-647) -----------------------
+
+def glass_scene(photon, glass_rows=(64, 90), glass_cols=(20, 108)):
+    """A 128x128 walled box with a GLAS slab and one photon (x, y, vx, vy, ctype)."""
+    c = S.default_config(128, 128, capacity=4096)
+    st = S.SimState(c)
+    S.wall_ring(st)
+    S.derive_block_masks(st)
+    a = st.arrays
+    k = 0
+    for y in range(*glass_rows):
+        for x in range(*glass_cols):
+            a["part_ty"][k], a["part_x"][k], a["part_y"][k], a["part_temp"][k] = E.pt("GLAS"), x, y, 300.0
+            k += 1
+    x, y, vx, vy, ctype = photon
+    a["part_ty"][k], a["part_x"][k], a["part_y"][k] = E.pt("PHOT"), x, y
+    a["part_vx"][k], a["part_vy"][k], a["part_ctype"][k], a["part_temp"][k] = vx, vy, ctype, 300.0
+    st.slot_bound = k + 1
+    S.rebuild_pmap(st)
+    return st, k
+
+
+# A two-bit wavelength mask (bits 14, 15) is narrow enough that its bin is
+# deterministic: lo + hi = 29 (Simulation.cpp:886-922).
+NARROW = (1 << 14) | (1 << 15)
+N0 = np.float32(1.9) - np.float32(0.07) * (np.float32(29) - np.float32(30)) / np.float32(30)
+RATIO_IN = 1.0 / float(N0 * N0)   # glass index as the reference applies it, entering
+
+
+def run_intent(st):
+    o = O.OracleState(st)
+    o.call("oracle_velocity_intent", 7)
+    return st
+
+
+def test_refraction_normal_incidence_keeps_velocity():
+    st, k = glass_scene((64, 63, 0.0, 1.0, NARROW))
+    run_intent(st)
+    assert st.arrays["part_vx"][k] == 0.0
+    assert st.arrays["part_vy"][k] == pytest.approx(1.0, abs=1e-6)
+    assert st.arrays["intent_target"][k] == 64 * 128 + 64          # enters the glass pixel
+    assert st.arrays["part_ctype"][k] == NARROW                    # a <5-bit span is kept whole
+
+
+def test_refraction_entering_glass_obeys_the_index_ratio():
+    """Unit speed in: the tangential component scales by the ratio, the
+    speed stays 1 (ct2 = 1 - r^2 sin^2), and the photon keeps heading in."""
+    st, k = glass_scene((64, 63, 0.6, 0.8, NARROW))
+    run_intent(st)
+    vx, vy = float(st.arrays["part_vx"][k]), float(st.arrays["part_vy"][k])
+    assert vx == pytest.approx(0.6 * RATIO_IN, rel=1e-5)
+    assert vx * vx + vy * vy == pytest.approx(1.0, abs=1e-5)
+    assert vy > 0.0
+    assert st.arrays["intent_target"][k] == 64 * 128 + 65
+
+
+def test_refraction_total_internal_reflection_mirrors_and_stays():
+    """Leaving the glass at 53 degrees from the normal: ratio^2 sin^2 > 1, so
+    the velocity is mirrored about the surface and the photon stays put."""
+    st, k = glass_scene((64, 64, 0.8, -0.6, NARROW))
+    run_intent(st)
+    assert float(st.arrays["part_vx"][k]) == pytest.approx(0.8, abs=1e-6)
+    assert float(st.arrays["part_vy"][k]) == pytest.approx(0.6, abs=1e-6)
+    assert st.arrays["intent_target"][k] == 0xFFFFFFFF
+
+
+def test_refraction_without_wavelength_kills():
+    st, k = glass_scene((64, 63, 0.0, 1.0, 0))
+    run_intent(st)
+    assert st.arrays["part_ty"][k] == 0
+
+
+def test_refraction_draws_a_contiguous_band_from_a_wide_mask():
+    full = 0x3FFFFFFF
+    for seed_slot in range(6):
+        st, k = glass_scene((64, 63, 0.3 + 0.05 * seed_slot, 0.9, full))
+        run_intent(st)
+        got = int(st.arrays["part_ctype"][k])
+        assert got != 0 and (got & ~full) == 0
+        low = (got & -got).bit_length() - 1
+        width = got.bit_length() - low
+        assert width in (4, 5) and got == ((1 << width) - 1) << low
+
+
+# --- WL_DETECT charging and the set_emap closure (mod.rs:348-828) -----------
+
+def wire_scene(cells, particles):
+    """128x128 px canvas (32x32 cells): `cells` maps (cx, cy) -> wall id,
+    `particles` are (name, x, y)."""
+    c = S.default_config(128, 128, capacity=256)
+    st = S.SimState(c)
+    for (cx, cy), w in cells.items():
+        st.arrays["wall_bmap"][cy * 32 + cx] = w
+    S.derive_block_masks(st)
+    for i, (name, x, y) in enumerate(particles):
+        st.arrays["part_ty"][i], st.arrays["part_x"][i], st.arrays["part_y"][i] = E.pt(name), x, y
+        st.arrays["part_temp"][i] = 300.0
+    st.slot_bound = len(particles)
+    S.rebuild_pmap(st)
+    return st
+
+
+def charge(st):
+    o = O.OracleState(st)
+    o.call("oracle_detect_charge")
+    o.call("oracle_emap_closure")
+    return st.arrays["wall_emap"].reshape(32, 32)
+
+
+WL_DETECT, WL_WALLELEC = 3, 1
+
+
+def test_detector_charges_its_run_only():
+    cells = {(x, 10): WL_DETECT for x in range(5, 13)}
+    cells.update({(x, 10): WL_DETECT for x in range(14, 17)})   # a separate run after a gap
+    em = charge(wire_scene(cells, [("DUST", 7 * 4 + 1, 10 * 4 + 2)]))
+    assert (em[10, 5:13] == 16).all()
+    assert (em[10, 14:17] == 0).all()
+    assert (em & 0xC0).sum() == 0                                # no end markers survive
+
+
+def test_closure_floods_connected_wire_only():
+    cells = {(x, 10): WL_DETECT for x in range(5, 13)}
+    cells.update({(12, y): WL_WALLELEC for y in range(11, 15)})  # a one-cell-wide column down
+    cells.update({(x, 15): WL_WALLELEC for x in range(12, 19)})  # into another run
+    cells.update({(x, 20): WL_WALLELEC for x in range(3, 9)})    # not connected
+    em = charge(wire_scene(cells, [("DUST", 6 * 4, 10 * 4)]))
+    for (cx, cy) in cells:
+        assert em[cy, cx] == (0 if cy == 20 else 16), (cx, cy)
+    assert (em & 0xC0).sum() == 0
+
+
+def test_no_trigger_no_charge():
+    cells = {(x, 10): WL_DETECT for x in range(5, 13)}
+    em = charge(wire_scene(cells, [("DUST", 40, 80)]))         # particle outside the detector
+    assert (em == 0).all()

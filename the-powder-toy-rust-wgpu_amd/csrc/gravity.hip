@@ -301,28 +301,24 @@ __global__ __launch_bounds__(GRAV_BLOCK) void grav_apply(const uint32_t *__restr
                                                          int yres, int xc) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= slots) return;
-    // Branch-free up to the store: every slot field is loaded in one round
-    // trip and the cell/element lookups in a second, instead of one memory
-    // latency per early-out test.
-    const uint32_t ty = ty_a[i], flags = flags_a[i];
-    const float fxp = px_a[i], fyp = py_a[i], vx0 = vx_a[i], vy0 = vy_a[i];
-    const int xi = tpt_f2i(fxp + 0.5f), yi = tpt_f2i(fyp + 0.5f);
-    bool ok = (ty != 0u) & (ty < (uint32_t)TPT_PT_NUM) & ((flags & 0x10u) == 0u) & (xi >= 4) & (yi >= 4) &
-              (xi < xres - 4) & (yi < yres - 4);
-    const int cell = ok ? (yi / TPT_CELL) * xc + xi / TPT_CELL : 0;
-    const uint32_t tyc = ok ? ty : 0u;
-    const uint32_t wb = wall_bmap[cell], we = wall_emap[cell], props = lut[tyc * 12u];
-    const float gfx = fx[cell], gfy = fy[cell], loss = __uint_as_float(lut[tyc * 12u + 7u]);
-    ok = ok & !(wb == 18u && (we & 0xFFu) < 8u);
-    if (!ok) return;
-    float vx = vx0, vy = vy0;
+    const uint32_t ty = ty_a[i];
+    if (ty == 0u || ty >= (uint32_t)TPT_PT_NUM) return;
+    const uint32_t flags = flags_a[i];
+    if (flags & 0x10u) return;
+    const int xi = tpt_f2i(px_a[i] + 0.5f), yi = tpt_f2i(py_a[i] + 0.5f);
+    if (xi < 4 || yi < 4 || xi >= xres - 4 || yi >= yres - 4) return;
+    const int cell = (yi / TPT_CELL) * xc + xi / TPT_CELL;
+    if (wall_bmap[cell] == 18u && (wall_emap[cell] & 0xFFu) < 8u) return;
+    const uint32_t props = lut[ty * 12u];
+    const float loss = __uint_as_float(lut[ty * 12u + 7u]);
+    float vx = vx_a[i], vy = vy_a[i];
     if (!(ty == 78u && (flags & 0x08u))) {  // PT_SPNG with FLAG_MOVABLE keeps its velocity
         vx *= loss;
         vy *= loss;
     }
     if ((props & 0x04u) == 0u) {  // TYPE_SOLID receives no force
-        vx += 1.0f * gfx;
-        vy += 1.0f * gfy;
+        vx += 1.0f * fx[cell];
+        vy += 1.0f * fy[cell];
     }
     vx_a[i] = vx;
     vy_a[i] = vy;

@@ -23,9 +23,11 @@
 //                            RecalcFreeParticles (spatial.hip) without life
 //                            decay, so that the next recalc can update it.
 //
-// Not built: photon refraction at glass (velocity.rs:262-647), the e == 2 pair
-// pass (reaction_kernel/energy_entry.rs) and WL_DETECT charging
-// (mod.rs:473-828); a photon that enters a pixel is moved without refraction.
+//   mv_detect_* / mv_emap_*  WL_DETECT charging and the set_emap closure
+//                            (mod.rs:348-828), on canvases with wire walls.
+//
+// Not built: the e == 2 pair pass (reaction_kernel/energy_entry.rs), which
+// needs the reaction creation queue.
 
 #include "tpt_internal.h"
 
@@ -59,6 +61,7 @@ struct MoveEnv {
     uint32_t newton_grav;
     uint32_t *__restrict__ saw_list;
     uint32_t *__restrict__ saw_count;
+    uint32_t *__restrict__ ctype;  // photon wavelength masks, narrowed by refraction
 };
 
 __device__ __forceinline__ uint32_t pcg_hash(uint32_t seed) {
@@ -576,6 +579,130 @@ __device__ uint32_t fall_down_destination(const MoveEnv &e, float &vx, float &vy
 }
 
 // gas_destination (velocity.rs:755-928) without the glass refraction.
+// Photon refraction at glass (velocity.rs:262-647; Simulation.cpp:1626-1750,
+// 886-922, 2966-3005).  Only GLAS (24) and BGLA (95) bend light.
+__device__ __forceinline__ bool refract_blocking(const MoveEnv &e, int x, int y) {
+    if (x < 0 || y < 0 || x >= e.xr || y >= e.yr) return false;
+    const uint32_t t = e.tmap[y * e.xr + x];
+    return t == 24u || t == 95u;
+}
+
+__device__ bool refract_boundary(const MoveEnv &e, int x, int y) {
+    return refract_blocking(e, x, y) && !(refract_blocking(e, x, y - 1) && refract_blocking(e, x, y + 1) &&
+                                          refract_blocking(e, x - 1, y) && refract_blocking(e, x + 1, y));
+}
+
+// The 8-neighbour ring (x right, y down, counter-clockwise from +x) and the
+// directions each step may continue in.
+__device__ __forceinline__ int ring_dx(int i) { return (i == 0 || i == 1 || i == 7) ? 1 : ((i >= 3 && i <= 5) ? -1 : 0); }
+__device__ __forceinline__ int ring_dy(int i) { return (i >= 1 && i <= 3) ? 1 : ((i >= 5) ? -1 : 0); }
+__device__ __forceinline__ uint32_t ring_mask(int i) {
+    const uint32_t m = 0x07u << ((i + 7) & 7);  // bits i-1, i, i+1 (mod 8)
+    return (m | (m >> 8)) & 0xFFu;
+}
+
+// find_next_boundary (velocity.rs:343-389).
+__device__ bool next_boundary(const MoveEnv &e, int &x, int &y, uint32_t dm, int &em, bool reverse) {
+    if (!(x > 0 && x < e.xr - 1 && y > 0 && y < e.yr - 1)) return false;
+    if (em != -1) dm &= ring_mask(em);
+    uint32_t mask = 0u;
+    for (int i = 0; i < 8; ++i)
+        if ((dm & (1u << i)) && refract_blocking(e, x + ring_dx(i), y + ring_dy(i))) mask |= 1u << i;
+    for (int i = 0; i < 8; ++i) {
+        const int n = (i + (reverse ? 1 : 7)) & 7;
+        if ((mask & (1u << i)) && !(mask & (1u << n))) {
+            x += ring_dx(i);
+            y += ring_dy(i);
+            em = i;
+            return true;
+        }
+    }
+    return false;
+}
+
+// get_normal (velocity.rs:401-456).
+__device__ bool surface_normal(const MoveEnv &e, int x, int y, float dx, float dy, float &nx, float &ny) {
+    if (!((dx != 0.0f || dy != 0.0f) && refract_boundary(e, x, y))) return false;
+    int lx = x, ly = y, rx = x, ry = y, lm = -1, rm = -1, j = 0;
+    bool lv = true, rv = true;
+    for (int k = 0; k < 10; ++k) {
+        if (lv) lv = next_boundary(e, lx, ly, 0xFFu, lm, true);
+        if (rv) rv = next_boundary(e, rx, ry, 0xFFu, rm, false);
+        j += (lv ? 1 : 0) + (rv ? 1 : 0);
+        if (!lv && !rv) break;
+    }
+    if (!(j >= 3 && (lx != rx || ly != ry))) return false;
+    const float ex = (float)(rx - lx), ey = (float)(ry - ly);
+    const float inv = 1.0f / sqrtf(ex * ex + ey * ey);
+    nx = ey * inv;
+    ny = -ex * inv;
+    return true;
+}
+
+// get_normal_interp (velocity.rs:462-516).
+__device__ bool surface_normal_along(const MoveEnv &e, float x0, float y0, float dx, float dy, float &nx, float &ny) {
+    dx = dx / 16.0f;
+    dy = dy / 16.0f;
+    for (int k = 0; k < 20; ++k) {
+        const int x = tpt_f2i(x0 + 0.5f), y = tpt_f2i(y0 + 0.5f);
+        if (x < 0 || y < 0 || x >= e.xr || y >= e.yr) return false;
+        if (refract_boundary(e, x, y)) return surface_normal(e, x, y, dx, dy, nx, ny);
+        x0 += dx;
+        y0 += dy;
+    }
+    return false;
+}
+
+// get_wavelength_bin (velocity.rs:523-560); -1: no wavelength left.
+__device__ int wavelength_bin(uint32_t wm, uint32_t seed, uint32_t &out) {
+    const uint32_t masked = wm & 0x3FFFFFFFu;
+    out = wm;
+    if (!masked) return -1;
+    const uint32_t lo = (uint32_t)__ffs(masked) - 1u, hi = 31u - (uint32_t)__clz(masked);
+    if (hi - lo < 5u) return (int)(hi + lo);
+    const uint32_t r = pcg_hash(seed ^ (lo * 2654435769u));
+    const uint32_t i = ((r >> 1) % (hi - lo - 4u)) + lo;
+    if (r & 1u) {
+        out = wm & (0x1Fu << i);
+        return (int)((i + 2u) * 2u);
+    }
+    out = wm & (0xFu << i);
+    return (int)((i + 2u) * 2u - 1u);
+}
+
+// refract_photon (velocity.rs:577-637): 0 passes, 1 dies, 2 total internal
+// reflection (stays this tick).
+__device__ int refract_photon(const MoveEnv &e, float px, float py, int xi, int yi, float &vx, float &vy, int fx,
+                              int fy, uint32_t slot, uint32_t seed) {
+    const uint32_t rt = e.tmap[fy * e.xr + fx], lt = e.tmap[yi * e.xr + xi];
+    const bool r_glass = rt == 24u || rt == 95u, l_glass = lt == 24u || lt == 95u;
+    if (r_glass == l_glass) return 0;
+    float nrx, nry;
+    if (!surface_normal_along(e, px, py, vx, vy, nrx, nry)) return 1;
+    uint32_t narrowed;
+    const int bin = wavelength_bin(e.ctype[slot], seed, narrowed);
+    if (bin == -1) return 1;
+    e.ctype[slot] = narrowed;
+    float nn = 1.9f - 0.07f * ((float)bin - 30.0f) / 30.0f;
+    nn *= nn;
+    if (r_glass) {  // entering the glass
+        nrx = -nrx;
+        nry = -nry;
+        nn = 1.0f / nn;
+    }
+    const float ct1 = vx * nrx + vy * nry;
+    const float ct2 = 1.0f - (nn * nn) * (1.0f - (ct1 * ct1));
+    if (ct2 < 0.0f) {
+        vx -= 2.0f * ct1 * nrx;
+        vy -= 2.0f * ct1 * nry;
+        return 2;
+    }
+    const float ct2s = sqrtf(ct2) - nn * ct1;
+    vx = nn * vx + ct2s * nrx;
+    vy = nn * vy + ct2s * nry;
+    return 0;
+}
+
 __device__ uint32_t gas_destination(const MoveEnv &e, float &vx, float &vy, float px, float py, int xi, int yi, int sx,
                                     int sy, uint32_t props, uint32_t ty, uint32_t slot, float collision,
                                     uint32_t seed) {
@@ -586,12 +713,17 @@ __device__ uint32_t gas_destination(const MoveEnv &e, float &vx, float &vy, floa
         fin_y = yi + sy;
     }
     uint32_t target = NO_INTENT;
-    bool blocked = false, died = false;
+    bool blocked = false, died = false, tir = false;
     if (fin_x != xi || fin_y != yi) {
         int ox, oy;
         const uint32_t d = do_move_at(e, props, ty, fin_x, fin_y, ox, oy);
         if (d == DO_OK) {
-            target = pix(e, ox, oy);
+            if (ty == 29u) {
+                const int r = refract_photon(e, px, py, xi, yi, vx, vy, ox, oy, slot, seed);
+                died = r == 1;
+                tir = r == 2;
+            }
+            if (!died && !tir) target = pix(e, ox, oy);
         } else if (d == DO_DIE) {
             died = true;
         } else {
@@ -757,6 +889,142 @@ __global__ __launch_bounds__(BLK) void mv_air_deposit_px(Slots s, const ulonglon
     const float drag = __uint_as_float(lut[base + 1u]), loss = __uint_as_float(lut[base + 2u]);
     air_vx[c] = air_vx[c] * loss + drag * s.vx[i];
     air_vy[c] = air_vy[c] * loss + drag * s.vy[i];
+}
+
+// WL_DETECT charging and the set_emap closure (mod.rs:348-828,
+// Simulation.cpp:928-983, 2330-2331).  Emap values are the low 6 bits;
+// bits 7/6 mark the left/right ends of a span filled this tick.
+//
+// A particle in an uncharged detector charges the whole uncharged wire run
+// through its cell.  The reference fills runs from concurrent threads that
+// read and write emap at once; here every trigger computes its run from the
+// untouched emap (mv_detect_span) and a second kernel fills the runs
+// (mv_detect_fill): each cell gets one value whichever trigger wrote it, the
+// union of the triggered runs, as the serial slot order gives.
+constexpr uint32_t EMAP_CHARGED = 16u, EMAP_LEFT = 0x80u, EMAP_RIGHT = 0x40u, EMAP_ENDS = 0xC0u;
+
+__device__ __forceinline__ bool is_wire(uint32_t w) {
+    return w == 3u || w == 2u || w == 6u || w == 1u || w == 5u || w == 12u || w == 18u;
+}
+__device__ __forceinline__ bool wire_off(const uint32_t *bmap, const uint32_t *emap, uint32_t c) {
+    return is_wire(bmap[c]) && (emap[c] & 0x3Fu) < 8u;
+}
+__device__ __forceinline__ bool wire_at(const uint32_t *bmap, int x, int y, int xc, int yc) {
+    return x >= 0 && y >= 0 && x < xc && y < yc && is_wire(bmap[y * xc + x]);
+}
+// set_emap's ladder shortcuts (mod.rs:412-456): dir -1 looks up (the cap row
+// is below the target row), +1 looks down.
+__device__ bool ladder(const uint32_t *bmap, bool single, int x, int target, int dir, int xc, int yc) {
+    const int cap = target - dir;
+    return single && wire_at(bmap, x - 1, cap, xc, yc) && wire_at(bmap, x, cap, xc, yc) &&
+           wire_at(bmap, x + 1, cap, xc, yc) && !wire_at(bmap, x - 1, target, xc, yc) &&
+           wire_at(bmap, x, target, xc, yc) && !wire_at(bmap, x + 1, target, xc, yc);
+}
+
+__global__ __launch_bounds__(BLK) void mv_detect_span(Slots s, const uint32_t *__restrict__ bmap,
+                                                      const uint32_t *__restrict__ emap, uint32_t *__restrict__ span,
+                                                      uint32_t slots, int xres, int yres, int xc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    const uint32_t ty = s.ty[i];
+    if (ty == 0u || ty >= 256u || (s.flags[i] & 0x10u)) return;
+    const int xi = tpt_f2i(s.x[i] + 0.5f), yi = tpt_f2i(s.y[i] + 0.5f);
+    if (xi < 4 || yi < 4 || xi >= xres - 4 || yi >= yres - 4) return;
+    const uint32_t cx = (uint32_t)xi / 4u, row = ((uint32_t)yi / 4u) * (uint32_t)xc, cell = row + cx;
+    if (bmap[cell] != 3u || (emap[cell] & 0x3Fu) >= 8u) return;
+    uint32_t x1 = cx, x2 = cx;
+    while (x1 > 0u && wire_off(bmap, emap, row + x1 - 1u)) --x1;
+    while (x2 + 1u < (uint32_t)xc && wire_off(bmap, emap, row + x2 + 1u)) ++x2;
+    span[cell] = ((x1 + 1u) << 16) | (x2 + 1u);  // every trigger of the run stores the same value
+}
+
+__global__ void mv_detect_fill(uint32_t *__restrict__ emap, uint32_t *__restrict__ span, int xc, uint32_t cells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const uint32_t v = span[c];
+    if (!v) return;
+    span[c] = 0u;
+    const uint32_t row = (c / (uint32_t)xc) * (uint32_t)xc, x1 = (v >> 16) - 1u, x2 = (v & 0xFFFFu) - 1u;
+    for (uint32_t x = x1; x <= x2; ++x)
+        emap[row + x] = EMAP_CHARGED | (x == x1 ? EMAP_LEFT : 0u) | (x == x2 ? EMAP_RIGHT : 0u);
+}
+
+// emap_gate_kernel (mod.rs:556-575): bit 0 an uncharged wire, bit 1 a cell
+// charged this tick.
+__global__ void mv_emap_gate(const uint32_t *__restrict__ bmap, const uint32_t *__restrict__ emap,
+                             uint32_t *__restrict__ gate, uint32_t cells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t bits = 0u;
+    if (c < cells) bits = (wire_off(bmap, emap, c) ? 1u : 0u) | ((emap[c] & 0x3Fu) == EMAP_CHARGED ? 2u : 0u);
+    const uint64_t b0 = __ballot(bits & 1u), b1 = __ballot(bits & 2u);
+    if (__lane_id() == 0 && (b0 || b1)) atomicOr(gate, (b0 ? 1u : 0u) | (b1 ? 2u : 0u));
+}
+
+// emap_closure_kernel (mod.rs:594-810): the recursive half of set_emap as
+// repeated top-to-bottom sweeps, one thread, as the reference runs it; it
+// returns at once unless the gate saw both an uncharged and a charged wire.
+__global__ void mv_emap_closure(const uint32_t *__restrict__ bmap, uint32_t *__restrict__ emap,
+                                uint32_t *__restrict__ gate, int xc, int yc) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const uint32_t g = *gate;
+    *gate = 0u;
+    if (g != 3u) return;
+    bool changed = true;
+    for (int pass = 0; changed && pass < yc + 2; ++pass) {
+        changed = false;
+        for (int y = 0; y < yc; ++y) {
+            const uint32_t row = (uint32_t)(y * xc);
+            for (int x = 0; x < xc;) {
+                if (!wire_off(bmap, emap, row + (uint32_t)x)) {
+                    ++x;
+                    continue;
+                }
+                int x1 = x, x2 = x;
+                while (x2 + 1 < xc && wire_off(bmap, emap, row + (uint32_t)x2 + 1u)) ++x2;
+                bool act = false;
+                for (int xx = x1; xx <= x2 && !act && y + 1 < yc; ++xx) {  // charged span below
+                    const uint32_t se = emap[(y + 1) * xc + xx];
+                    const int sy = y + 1;
+                    const bool single = (se & EMAP_ENDS) == EMAP_ENDS;
+                    if ((se & 0x3Fu) == EMAP_CHARGED && !ladder(bmap, single, xx, sy - 2, -1, xc, yc) &&
+                        ((se & EMAP_ENDS) || y + 1 == yc - 1 || wire_at(bmap, xx - 1, y, xc, yc) ||
+                         wire_at(bmap, xx + 1, y, xc, yc) || wire_at(bmap, xx - 1, sy + 1, xc, yc) ||
+                         !wire_at(bmap, xx, sy + 1, xc, yc) || wire_at(bmap, xx + 1, sy + 1, xc, yc)))
+                        act = true;
+                }
+                for (int xx = x1; xx <= x2 && !act && y > 0; ++xx) {  // charged span above
+                    const uint32_t se = emap[(y - 1) * xc + xx];
+                    const int sy = y - 1;
+                    const bool single = (se & EMAP_ENDS) == EMAP_ENDS;
+                    if ((se & 0x3Fu) == EMAP_CHARGED && !ladder(bmap, single, xx, y + 1, 1, xc, yc) &&
+                        ((se & EMAP_ENDS) || wire_at(bmap, xx - 1, y, xc, yc) || wire_at(bmap, xx + 1, y, xc, yc) ||
+                         wire_at(bmap, xx - 1, sy - 1, xc, yc) || !wire_at(bmap, xx, sy - 1, xc, yc) ||
+                         wire_at(bmap, xx + 1, sy - 1, xc, yc)))
+                        act = true;
+                }
+                for (int xx = x1; xx <= x2 && !act; ++xx) {  // ladder jumps from a singleton
+                    const bool up = y + 2 < yc && (emap[(y + 2) * xc + xx] & EMAP_ENDS) == EMAP_ENDS;
+                    const bool down = y >= 2 && (emap[(y - 2) * xc + xx] & EMAP_ENDS) == EMAP_ENDS;
+                    if (ladder(bmap, up, xx, y, -1, xc, yc) || ladder(bmap, down, xx, y, 1, xc, yc)) act = true;
+                }
+                if (act) {
+                    for (int xx = x1; xx <= x2; ++xx)
+                        emap[row + (uint32_t)xx] =
+                            EMAP_CHARGED | (xx == x1 ? EMAP_LEFT : 0u) | (xx == x2 ? EMAP_RIGHT : 0u);
+                    changed = true;
+                }
+                x = x2 + 1;
+            }
+        }
+    }
+}
+
+__global__ void mv_emap_strip(uint32_t *__restrict__ emap, uint32_t cells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < cells) {
+        const uint32_t v = emap[c];
+        if (v & EMAP_ENDS) emap[c] = v & ~EMAP_ENDS;
+    }
 }
 
 __global__ void mv_type_map(const uint4 *__restrict__ pmap, uint32_t *__restrict__ tmap, uint32_t n4) {
@@ -1057,16 +1325,37 @@ int tpt_movement_step(tpt_ctx *ctx, uint32_t seed) {
     }
     ctx->rec_cover = false;
     tpt_pass_end(ctx, PASS_AIR_DEPOSIT);
+    // WL_DETECT charging + closure + marker strip, on canvases with wire
+    // walls (the host sees every wall edit) or with end markers uploaded.
+    if (ctx->has_wire || ctx->emap_markers) {
+        uint32_t *emap_w = tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP);
+        tpt_pass_begin(ctx, PASS_MOVE_DETECT);
+        if (ctx->has_wire) {
+            hipLaunchKernelGGL(mv_detect_span, sg, b, 0, ctx->stream, s, wall, emap_w, ctx->cell_scratch_u2, slots,
+                               xres, yres, xc);
+            hipLaunchKernelGGL(mv_detect_fill, cg, b, 0, ctx->stream, emap_w, ctx->cell_scratch_u2, xc, ctx->cells);
+            hipLaunchKernelGGL(mv_emap_gate, cg, b, 0, ctx->stream, wall, emap_w, ctx->counters + TPT_CTR_EMAP_GATE,
+                               ctx->cells);
+            hipLaunchKernelGGL(mv_emap_closure, dim3(1), dim3(64), 0, ctx->stream, wall, emap_w,
+                               ctx->counters + TPT_CTR_EMAP_GATE, xc, (int)ctx->ycells);
+        }
+        hipLaunchKernelGGL(mv_emap_strip, cg, b, 0, ctx->stream, emap_w, ctx->cells);
+        ctx->emap_markers = false;
+        tpt_pass_end(ctx, PASS_MOVE_DETECT);
+    }
     tpt_pass_begin(ctx, PASS_MOVE_CLEAR);
     hipLaunchKernelGGL(mv_clear, pg, b, 0, ctx->stream, (uint4 *)winner, n4, ctx->counters);
+    hipLaunchKernelGGL(mv_type_map, pg, b, 0, ctx->stream, (const uint4 *)pmap, (uint32_t *)ctx->pix_type, n4);
     tpt_pass_end(ctx, PASS_MOVE_CLEAR);
     MoveEnv e{pmap, ctx->pix_type, wall, emap, lut, xres, yres, xc, c.edge_mode, c.gravity_mode,
-              c.custom_gravity_x, c.custom_gravity_y, c.newton_grav, ctx->slot_scratch_u2, ctx->counters + 1};
+              c.custom_gravity_x, c.custom_gravity_y, c.newton_grav, ctx->slot_scratch_u2, ctx->counters + 1,
+              tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE)};
     tpt_pass_begin(ctx, PASS_MOVE_INTENT);
-    hipLaunchKernelGGL(mv_type_map, pg, b, 0, ctx->stream, (const uint4 *)pmap, (uint32_t *)ctx->pix_type, n4);
     hipLaunchKernelGGL(mv_intent, sg, b, 0, ctx->stream, s, e, air_vx, air_vy, winner, slots, seed);
-    hipLaunchKernelGGL(mv_saw, dim3(16), b, 0, ctx->stream, s.ty, ctx->slot_scratch_u2, ctx->counters);
     tpt_pass_end(ctx, PASS_MOVE_INTENT);
+    tpt_pass_begin(ctx, PASS_MOVE_SAW);
+    hipLaunchKernelGGL(mv_saw, dim3(16), b, 0, ctx->stream, s.ty, ctx->slot_scratch_u2, ctx->counters);
+    tpt_pass_end(ctx, PASS_MOVE_SAW);
     CommitBufs cb{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
                   tpt_buf<float>(ctx, TPT_BUF_PART_TEMP), tpt_buf<uint32_t>(ctx, TPT_BUF_PHOTON_MASK),
                   tpt_buf<float>(ctx, TPT_BUF_AIR_PV)};

@@ -18,8 +18,8 @@ const char *const tpt_pass_names[PASS_COUNT] = {
     "grav_mass_seed", "grav_fft_rows", "grav_fft_cols", "grav_fft_inverse_extract",
     "grav_apply", "grav_build_mass", "recalc_clear", "recalc_scatter", "check_stacking",
     "thermal_conduction", "thermal_conduction_cells", "thermal_apply", "phase_transition",
-    "phase_transition_cells", "refresh_pmap_types", "air_deposit", "air_deposit_cells",
-    "move_clear_intents", "move_velocity_intent", "move_commit", "move_rebuild_maps",
+    "phase_transition_cells", "refresh_pmap_types", "air_deposit", "air_deposit_cells", "move_detect",
+    "move_clear_intents", "move_velocity_intent", "move_saw", "move_commit", "move_rebuild_maps",
 };
 
 static thread_local char g_err[512];
@@ -201,6 +201,18 @@ extern "C" int tpt_upload(tpt_ctx *ctx, int buf, const void *src, size_t off, si
     if (!src && bytes) return tpt_set_error(TPT_EINVAL, "null source");
     if (!bytes) return TPT_OK;
     ctx->maps_fresh = ctx->rec_cover = ctx->rebuild_fresh = false;
+    if (buf == TPT_BUF_WALL_BMAP || buf == TPT_BUF_WALL_EMAP) {
+        const uint32_t *w = (const uint32_t *)src;
+        bool found = false;
+        for (size_t k = 0; k < bytes / 4 && !found; ++k) {
+            const uint32_t v = w[k];
+            found = buf == TPT_BUF_WALL_BMAP
+                        ? (v == 1u || v == 2u || v == 3u || v == 5u || v == 6u || v == 12u || v == 18u)
+                        : (v & 0xC0u) != 0u;
+        }
+        bool &flag = buf == TPT_BUF_WALL_BMAP ? ctx->has_wire : ctx->emap_markers;
+        flag = (off == 0 && bytes == ctx->buf_bytes[buf]) ? found : (flag || found);
+    }
     TPT_HIP_CHECK(hipMemcpyAsync((char *)ctx->buf[buf] + off, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     TPT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     if (buf == TPT_BUF_ELEM_PROPS || buf == TPT_BUF_MOVE_LUT) ctx->tables_set = true;

@@ -19,6 +19,7 @@
 #define TPT_CTR_INVIS 0   // INVIS/FILT placement queue length
 #define TPT_CTR_SAW 1     // deferred WOOD->SAWD list length
 #define TPT_CTR_OFFMAP 2  // off-map list length (see spatial.hip)
+#define TPT_CTR_EMAP_GATE 3  // wire closure gate bits (move.hip)
 
 // Passes that carry their own timing slot (see tpt_get_pass_times).
 enum tpt_pass {
@@ -43,8 +44,10 @@ enum tpt_pass {
     PASS_REFRESH_PMAP,    // pmap type refresh
     PASS_AIR_DEPOSIT,     // particle -> air drag
     PASS_AIR_DEPOSIT_CELL,
-    PASS_MOVE_CLEAR,      // cell_winner clear
+    PASS_MOVE_DETECT,     // WL_DETECT charging + wire closure
+    PASS_MOVE_CLEAR,      // cell_winner clear + planner type map
     PASS_MOVE_INTENT,     // velocity integration + destination + claim
+    PASS_MOVE_SAW,        // deferred WOOD -> SAWD
     PASS_MOVE_COMMIT,     // winner check + move/swap
     PASS_MOVE_REBUILD,    // pmap/photons rebuild
     PASS_COUNT
@@ -104,6 +107,11 @@ struct tpt_ctx {
     // air or gravity runs: the next RecalcFreeParticles may then update the
     // rebuilt maps instead of rebuilding them.
     bool rebuild_fresh = false;
+    // Wall facts the host learns from its uploads (the device never edits
+    // wall_bmap): the canvas has conductor walls; an uploaded emap carries
+    // span-end marker bits.
+    bool has_wire = false;
+    bool emap_markers = false;
     uint32_t *pix_count_mv = nullptr;     // pixels: particle counts of the movement rebuild
     tpt_fft_state fft;
 
