@@ -916,6 +916,52 @@ static uint32_t pmap_at(const oracle_state *s, int x, int y) {
 }
 
 /* commit_move_kernel (commit.rs:181-535) for all slots. */
+/* A blocked photon's hit on the pixel it could not enter (commit.rs:263-320):
+ * wavelength filter, PSCN/NSCN sparking, temperature exchange. The reference
+ * runs these read-modify-writes of the hit particle from concurrent threads;
+ * the canonical order (R10) applies every hit after the other commits of the
+ * pass, in ascending photon slot order. */
+static void photon_hit(oracle_state *s, uint32_t i, uint32_t target) {
+    const uint32_t xres = s->xres;
+    uint32_t hit = target & 0x7FFFFFFFu, r = s->pmap[hit];
+    if (r >= 512u) {
+        uint32_t rty = r & 0x1FFu, rslot = (r >> 9) - 1;
+        uint32_t mask = s->photon_mask[rty];
+        if (rty == 190) {
+            int bin = (int)(s->part_ctype[rslot] / 4u);
+            if (bin < 0) bin = 0;
+            if (bin > 25) bin = 25;
+            mask = 0x1Fu << (uint32_t)bin;
+        } else if (rty == 194) {
+            uint32_t colour = (s->part_ctype[rslot] >> 6) & 0x3Fu;
+            if ((colour & 48u) == 0) mask |= 1u << 25;
+            if ((colour & 12u) == 0) mask |= 1u << 15;
+            if ((colour & 3u) == 0) mask |= 1u << 5;
+        }
+        s->part_ctype[i] &= mask;
+        if (rty == 50 && s->part_life[rslot] == 0) {
+            int hx = (int)(hit % xres), hy = (int)(hit / xres);
+            if ((pmap_at(s, hx - 1, hy) & 0x1FFu) == 51 || (pmap_at(s, hx + 1, hy) & 0x1FFu) == 51 ||
+                (pmap_at(s, hx, hy - 1) & 0x1FFu) == 51 || (pmap_at(s, hx, hy + 1) & 0x1FFu) == 51) {
+                s->part_ctype[rslot] = 50;
+                s->part_ty[rslot] = 12;
+                s->part_life[rslot] = 4;
+            }
+        }
+        if (s->heat_enable) {
+            if (rty == 62 || rty == 27) s->part_temp[rslot] = s->part_temp[i];
+            uint32_t hc = s->move_lut[rty * 12 + 3];
+            if (!is_heat_insulator_c(rty, s->part_life[rslot], s->part_tmp[rslot], hc) && rty != 134) {
+                float avg = (s->part_temp[rslot] + s->part_temp[i]) * 0.5f;
+                if (avg > 9999.0f) avg = 9999.0f;
+                if (avg < 0.0f) avg = 0.0f;
+                s->part_temp[i] = avg;
+                s->part_temp[rslot] = avg;
+            }
+        }
+    }
+}
+
 void oracle_commit(oracle_state *s) {
     const uint32_t xres = s->xres, yres = s->yres, xc = xres / 4;
     for (uint32_t i = 0; i < s->slot_bound; ++i) {
@@ -928,48 +974,7 @@ void oracle_commit(oracle_state *s) {
         }
         int half_mirror = (raw & INTENT_HALF_MIRROR) != 0;
         uint32_t target = raw & INTENT_PIXEL_MASK;
-        if (raw & 0x80000000u) {
-            if (ty == 29) {
-                uint32_t hit = target & 0x7FFFFFFFu, r = s->pmap[hit];
-                if (r >= 512u) {
-                    uint32_t rty = r & 0x1FFu, rslot = (r >> 9) - 1;
-                    uint32_t mask = s->photon_mask[rty];
-                    if (rty == 190) {
-                        int bin = (int)(s->part_ctype[rslot] / 4u);
-                        if (bin < 0) bin = 0;
-                        if (bin > 25) bin = 25;
-                        mask = 0x1Fu << (uint32_t)bin;
-                    } else if (rty == 194) {
-                        uint32_t colour = (s->part_ctype[rslot] >> 6) & 0x3Fu;
-                        if ((colour & 48u) == 0) mask |= 1u << 25;
-                        if ((colour & 12u) == 0) mask |= 1u << 15;
-                        if ((colour & 3u) == 0) mask |= 1u << 5;
-                    }
-                    s->part_ctype[i] &= mask;
-                    if (rty == 50 && s->part_life[rslot] == 0) {
-                        int hx = (int)(hit % xres), hy = (int)(hit / xres);
-                        if ((pmap_at(s, hx - 1, hy) & 0x1FFu) == 51 || (pmap_at(s, hx + 1, hy) & 0x1FFu) == 51 ||
-                            (pmap_at(s, hx, hy - 1) & 0x1FFu) == 51 || (pmap_at(s, hx, hy + 1) & 0x1FFu) == 51) {
-                            s->part_ctype[rslot] = 50;
-                            s->part_ty[rslot] = 12;
-                            s->part_life[rslot] = 4;
-                        }
-                    }
-                    if (s->heat_enable) {
-                        if (rty == 62 || rty == 27) s->part_temp[rslot] = s->part_temp[i];
-                        uint32_t hc = s->move_lut[rty * 12 + 3];
-                        if (!is_heat_insulator_c(rty, s->part_life[rslot], s->part_tmp[rslot], hc) && rty != 134) {
-                            float avg = (s->part_temp[rslot] + s->part_temp[i]) * 0.5f;
-                            if (avg > 9999.0f) avg = 9999.0f;
-                            if (avg < 0.0f) avg = 0.0f;
-                            s->part_temp[i] = avg;
-                            s->part_temp[rslot] = avg;
-                        }
-                    }
-                }
-            }
-            continue;
-        }
+        if (raw & 0x80000000u) continue; /* photon hits: after the loop (R10) */
         if (target >= xres * yres) continue;
         s->part_flags[i] &= 0xFFFFFFFEu;
         uint32_t key = s->cell_winner[target];
@@ -1055,6 +1060,11 @@ void oracle_commit(oracle_state *s) {
             s->part_x[i] = (float)nx;
             s->part_y[i] = (float)ny;
         }
+    }
+    for (uint32_t i = 0; i < s->slot_bound; ++i) {
+        uint32_t raw = s->intent_target[i];
+        if (s->part_ty[i] == 29 && raw != NO_INTENT && (raw & 0x80000000u))
+            photon_hit(s, i, raw & INTENT_PIXEL_MASK);
     }
 }
 

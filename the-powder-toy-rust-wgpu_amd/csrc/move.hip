@@ -1136,7 +1136,8 @@ __device__ __forceinline__ uint32_t pmap_at(const uint32_t *pmap, int x, int y, 
 __global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const uint32_t *__restrict__ pmap,
                                                  const uint32_t *__restrict__ wall_a,
                                                  const uint32_t *__restrict__ emap_a, const uint32_t *__restrict__ lut,
-                                                 const uint32_t *__restrict__ winner, uint32_t slots, int xres, int yres,
+                                                 const uint32_t *__restrict__ winner, uint32_t *__restrict__ photon_list,
+                                                 uint32_t *__restrict__ counters, uint32_t slots, int xres, int yres,
                                                  int xc, uint32_t heat_enable) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= slots) return;
@@ -1150,45 +1151,8 @@ __global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const u
     const bool half_mirror = (raw & HALF_MIRROR) != 0u;
     const uint32_t target = raw & PIXEL_MASK;
     if (raw & 0x80000000u) {
-        if (ty == 29u) {
-            const uint32_t hit = target & 0x7FFFFFFFu, r = pmap[hit];
-            if (r >= 512u) {
-                const uint32_t rty = r & 0x1FFu, rslot = (r >> 9) - 1u;
-                uint32_t mask = cb.photon_mask[rty];
-                if (rty == 190u) {
-                    const int bin = min(max((int)(cb.ctype[rslot] / 4u), 0), 25);
-                    mask = 0x1Fu << (uint32_t)bin;
-                } else if (rty == 194u) {
-                    const uint32_t colour = (cb.ctype[rslot] >> 6) & 0x3Fu;
-                    if ((colour & 48u) == 0u) mask |= 1u << 25;
-                    if ((colour & 12u) == 0u) mask |= 1u << 15;
-                    if ((colour & 3u) == 0u) mask |= 1u << 5;
-                }
-                cb.ctype[i] &= mask;
-                if (rty == 50u && cb.life[rslot] == 0) {
-                    const int hx = (int)(hit % (uint32_t)xres), hy = (int)(hit / (uint32_t)xres);
-                    if ((pmap_at(pmap, hx - 1, hy, xres, yres) & 0x1FFu) == 51u ||
-                        (pmap_at(pmap, hx + 1, hy, xres, yres) & 0x1FFu) == 51u ||
-                        (pmap_at(pmap, hx, hy - 1, xres, yres) & 0x1FFu) == 51u ||
-                        (pmap_at(pmap, hx, hy + 1, xres, yres) & 0x1FFu) == 51u) {
-                        cb.ctype[rslot] = 50u;
-                        s.ty[rslot] = 12u;
-                        cb.life[rslot] = 4;
-                    }
-                }
-                if (heat_enable) {
-                    if (rty == 62u || rty == 27u) cb.temp[rslot] = cb.temp[i];
-                    const uint32_t hc = lut[rty * 12u + 3u];
-                    const bool ins = hc == 0u || (rty == 68u && cb.life[rslot] != 10) ||
-                                     ((rty == 82u || rty == 162u) && (s.tmp[rslot] & 1) == 0);
-                    if (!ins && rty != 134u) {
-                        const float avg = tpt_clampf((cb.temp[rslot] + cb.temp[i]) * 0.5f, 0.0f, 9999.0f);
-                        cb.temp[i] = avg;
-                        cb.temp[rslot] = avg;
-                    }
-                }
-            }
-        }
+        // A blocked photon's hit: deferred to mv_photon_hits (canonical R10).
+        if (ty == 29u) photon_list[atomicAdd(&counters[TPT_CTR_PHOT], 1u)] = i;
         return;
     }
     if (target >= (uint32_t)(xres * yres)) return;
@@ -1291,6 +1255,93 @@ __global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const u
     s.y[i] = (float)ny;
 }
 
+// The blocked photons' hits on the pixels they could not enter
+// (commit.rs:263-320): wavelength filter, PSCN/NSCN sparking, temperature
+// exchange with the hit particle.  The reference runs these read-modify-
+// writes from concurrent threads; here (canonical R10) they follow the rest
+// of the commit, and hits on one particle apply in ascending photon slot
+// order: one workgroup runs rounds in which the lowest pending photon of
+// each hit particle (atomicMin into `owner`, all ones at rest) applies its
+// hit and leaves the list.
+__device__ void photon_hit(Slots &s, CommitBufs &cb, const uint32_t *__restrict__ pmap,
+                           const uint32_t *__restrict__ lut, uint32_t i, uint32_t target, int xres, int yres,
+                           uint32_t heat_enable) {
+    const uint32_t hit = target & 0x7FFFFFFFu, r = pmap[hit];
+    if (r >= 512u) {
+        const uint32_t rty = r & 0x1FFu, rslot = (r >> 9) - 1u;
+        uint32_t mask = cb.photon_mask[rty];
+        if (rty == 190u) {
+            const int bin = min(max((int)(cb.ctype[rslot] / 4u), 0), 25);
+            mask = 0x1Fu << (uint32_t)bin;
+        } else if (rty == 194u) {
+            const uint32_t colour = (cb.ctype[rslot] >> 6) & 0x3Fu;
+            if ((colour & 48u) == 0u) mask |= 1u << 25;
+            if ((colour & 12u) == 0u) mask |= 1u << 15;
+            if ((colour & 3u) == 0u) mask |= 1u << 5;
+        }
+        cb.ctype[i] &= mask;
+        if (rty == 50u && cb.life[rslot] == 0) {
+            const int hx = (int)(hit % (uint32_t)xres), hy = (int)(hit / (uint32_t)xres);
+            if ((pmap_at(pmap, hx - 1, hy, xres, yres) & 0x1FFu) == 51u ||
+                (pmap_at(pmap, hx + 1, hy, xres, yres) & 0x1FFu) == 51u ||
+                (pmap_at(pmap, hx, hy - 1, xres, yres) & 0x1FFu) == 51u ||
+                (pmap_at(pmap, hx, hy + 1, xres, yres) & 0x1FFu) == 51u) {
+                cb.ctype[rslot] = 50u;
+                s.ty[rslot] = 12u;
+                cb.life[rslot] = 4;
+            }
+        }
+        if (heat_enable) {
+            if (rty == 62u || rty == 27u) cb.temp[rslot] = cb.temp[i];
+            const uint32_t hc = lut[rty * 12u + 3u];
+            const bool ins = hc == 0u || (rty == 68u && cb.life[rslot] != 10) ||
+                             ((rty == 82u || rty == 162u) && (s.tmp[rslot] & 1) == 0);
+            if (!ins && rty != 134u) {
+                const float avg = tpt_clampf((cb.temp[rslot] + cb.temp[i]) * 0.5f, 0.0f, 9999.0f);
+                cb.temp[i] = avg;
+                cb.temp[rslot] = avg;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void mv_photon_hits(Slots s, CommitBufs cb, const uint32_t *__restrict__ pmap,
+                                                       const uint32_t *__restrict__ lut, uint32_t *__restrict__ list,
+                                                       uint32_t *__restrict__ counters, uint32_t *__restrict__ owner,
+                                                       int xres, int yres, uint32_t heat_enable) {
+    __shared__ int pending;
+    const uint32_t n = counters[TPT_CTR_PHOT];
+    for (;;) {
+        if (threadIdx.x == 0) pending = 0;
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+            const uint32_t i = list[k];
+            if (i == NO_INTENT) continue;
+            const uint32_t r = pmap[(s.intent[i] & PIXEL_MASK) & 0x7FFFFFFFu];
+            if (r < 512u) {
+                list[k] = NO_INTENT;  // nothing there to hit
+                continue;
+            }
+            atomicMin(&owner[(r >> 9) - 1u], i);
+            pending = 1;
+        }
+        __syncthreads();
+        if (!pending) break;
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+            const uint32_t i = list[k];
+            if (i == NO_INTENT) continue;
+            const uint32_t target = s.intent[i] & PIXEL_MASK;
+            const uint32_t rslot = (pmap[target & 0x7FFFFFFFu] >> 9) - 1u;
+            if (owner[rslot] != i) continue;
+            photon_hit(s, cb, pmap, lut, i, target, xres, yres, heat_enable);
+            owner[rslot] = 0xFFFFFFFFu;
+            list[k] = NO_INTENT;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counters[TPT_CTR_PHOT] = 0u;
+}
+
 }  // namespace
 
 int tpt_movement_step(tpt_ctx *ctx, uint32_t seed) {
@@ -1360,8 +1411,10 @@ int tpt_movement_step(tpt_ctx *ctx, uint32_t seed) {
                   tpt_buf<float>(ctx, TPT_BUF_PART_TEMP), tpt_buf<uint32_t>(ctx, TPT_BUF_PHOTON_MASK),
                   tpt_buf<float>(ctx, TPT_BUF_AIR_PV)};
     tpt_pass_begin(ctx, PASS_MOVE_COMMIT);
-    hipLaunchKernelGGL(mv_commit, sg, b, 0, ctx->stream, s, cb, pmap, wall, emap, lut, winner, slots, xres, yres, xc,
-                       c.heat_enable ? 1u : 0u);
+    hipLaunchKernelGGL(mv_commit, sg, b, 0, ctx->stream, s, cb, pmap, wall, emap, lut, winner, ctx->slot_scratch_u2,
+                       ctx->counters, slots, xres, yres, xc, c.heat_enable ? 1u : 0u);
+    hipLaunchKernelGGL(mv_photon_hits, dim3(1), dim3(1024), 0, ctx->stream, s, cb, pmap, lut, ctx->slot_scratch_u2,
+                       ctx->counters, ctx->slot_owner, xres, yres, c.heat_enable ? 1u : 0u);
     tpt_pass_end(ctx, PASS_MOVE_COMMIT);
     int rc = tpt_check_launch("movement pass");
     if (rc) return rc;

@@ -20,6 +20,7 @@
 #define TPT_CTR_SAW 1     // deferred WOOD->SAWD list length
 #define TPT_CTR_OFFMAP 2  // off-map list length (see spatial.hip)
 #define TPT_CTR_EMAP_GATE 3  // wire closure gate bits (move.hip)
+#define TPT_CTR_PHOT 4       // blocked-photon hit list length (move.hip)
 
 // Passes that carry their own timing slot (see tpt_get_pass_times).
 enum tpt_pass {
@@ -113,6 +114,7 @@ struct tpt_ctx {
     bool has_wire = false;
     bool emap_markers = false;
     uint32_t *pix_count_mv = nullptr;     // pixels: particle counts of the movement rebuild
+    uint32_t *slot_owner = nullptr;       // capacity words, all ones at rest (photon hit rounds)
     tpt_fft_state fft;
 
     // Profiling: a ring of event pairs per pass, folded lazily so that
