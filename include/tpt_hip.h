@@ -166,6 +166,30 @@ TPT_API int tpt_step_movement(tpt_ctx *ctx, uint32_t seed);
  * (lib.rs:315-318) and CheckStacking on every 10th frame (lib.rs:359). */
 TPT_API int tpt_step(tpt_ctx *ctx, uint64_t frame_count);
 
+/* `ticks` calls of tpt_step for frames first_frame, first_frame+1, ...
+ * (the loop of SimulationEngine::step callers, tpt-app main.rs:656-672).
+ * The first tick runs as tpt_step; the steady-state tick is then captured
+ * once as a hipGraph (re-captured after any upload, table or slot-bound
+ * change) and replayed, with the frame counter, seed and CheckStacking
+ * flag kept in device memory.  Results equal the tpt_step loop. */
+TPT_API int tpt_run(tpt_ctx *ctx, uint64_t first_frame, uint32_t ticks);
+
+/* The batched form (no counterpart in the reference, which steps one
+ * canvas per SimulationEngine): n independent canvases, each a tpt_ctx
+ * with its own buffers and stream, advanced together.  tpt_batch_step runs
+ * one eager tick per canvas, then replays one hipGraph holding every
+ * canvas's tick as a parallel branch (fork/join across the canvases'
+ * streams).  Canvases are loaded and read through tpt_batch_context and
+ * the single-context calls. */
+typedef struct tpt_batch tpt_batch;
+TPT_API int tpt_batch_create(uint32_t n, const tpt_config *cfgs, tpt_batch **out);
+TPT_API int tpt_batch_destroy(tpt_batch *batch);
+TPT_API int tpt_batch_context(tpt_batch *batch, uint32_t k, tpt_ctx **ctx);
+TPT_API int tpt_batch_step(tpt_batch *batch, uint64_t first_frame, uint32_t ticks);
+/* The same ticks as per-canvas tpt_step launches (the comparison point). */
+TPT_API int tpt_batch_step_launches(tpt_batch *batch, uint64_t first_frame, uint32_t ticks);
+TPT_API int tpt_batch_synchronize(tpt_batch *batch);
+
 /* The tick seed of lib.rs:315-318. */
 TPT_API uint32_t tpt_frame_seed(uint64_t frame_count);
 

@@ -1043,10 +1043,11 @@ __global__ void mv_clear(uint4 *__restrict__ winner, uint32_t n4, uint32_t *__re
 // velocity_and_intent_kernel (velocity.rs:951-1157).
 __global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, const float *__restrict__ air_vx,
                                                  const float *__restrict__ air_vy, uint32_t *__restrict__ winner,
-                                                 uint32_t slots, uint32_t seed) {
+                                                 uint32_t slots, const uint32_t *__restrict__ seedp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= slots) return;
     s.intent[i] = NO_INTENT;
+    const uint32_t seed = *seedp;
     // Slot fields loaded together rather than one latency per test.
     const uint32_t ty = s.ty[i], flags = s.flags[i];
     const float px = s.x[i], py = s.y[i], vx0 = s.vx[i], vy0 = s.vy[i];
@@ -1344,7 +1345,8 @@ __global__ __launch_bounds__(1024) void mv_photon_hits(Slots s, CommitBufs cb, c
 
 }  // namespace
 
-int tpt_movement_step(tpt_ctx *ctx, uint32_t seed) {
+int tpt_movement_step(tpt_ctx *ctx) {
+    const uint32_t *seed = ctx->counters + TPT_CTR_SEED;
     ctx->maps_fresh = ctx->rebuild_fresh = false;
     const tpt_config &c = ctx->cfg;
     const uint32_t slots = ctx->slot_bound, P = ctx->pixels;

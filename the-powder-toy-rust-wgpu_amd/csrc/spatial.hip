@@ -314,10 +314,18 @@ __global__ __launch_bounds__(BLK) void recalc_invis_list(const uint32_t *__restr
     }
 }
 
+__global__ void copy_words(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint32_t n4) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n4) dst[i] = src[i];
+}
+
+// CheckStacking runs when the device frame says so (every 10th frame), so
+// that a captured tick carries it; counters[TPT_CTR_STACK] is the flag.
 __global__ void stacking_mark(uint32_t *__restrict__ pmap_count, const uint32_t *__restrict__ wall_bmap, uint32_t n,
-                              uint32_t xres, uint32_t xc, uint32_t npart, uint32_t seed) {
+                              uint32_t xres, uint32_t xc, uint32_t npart, const uint32_t *__restrict__ counters) {
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
+    if (idx >= n || !counters[TPT_CTR_STACK]) return;
+    const uint32_t seed = counters[TPT_CTR_SEED];
     const uint32_t count = pmap_count[idx];
     if (count <= 5u) return;
     const uint32_t x = idx % xres, y = idx / xres;
@@ -336,9 +344,9 @@ __global__ __launch_bounds__(BLK) void stacking_apply(uint32_t *__restrict__ ty_
                                                       unsigned long long *__restrict__ rec,
                                                       uint32_t *__restrict__ pmap_count,
                                                       const uint32_t *__restrict__ elem_props, uint32_t slots, int xres,
-                                                      int yres, uint32_t npart) {
+                                                      int yres, uint32_t npart, const uint32_t *__restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= slots) return;
+    if (i >= slots || !counters[TPT_CTR_STACK]) return;
     const uint32_t ty = ty_a[i];
     if (ty == 0u || ty >= (uint32_t)TPT_PT_NUM) return;
     const int x = tpt_f2i(px_a[i] + 0.5f), y = tpt_f2i(py_a[i] + 0.5f);
@@ -410,7 +418,8 @@ static int map_update(tpt_ctx *ctx) {
     const uint32_t *props = tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS);
     const uint32_t n4 = P / 4;
     tpt_pass_begin(ctx, PASS_RECALC_CLEAR);
-    TPT_HIP_CHECK(hipMemcpyAsync(count, ctx->pix_count_mv, (size_t)P * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    hipLaunchKernelGGL(copy_words, dim3((n4 + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
+                       (const uint4 *)ctx->pix_count_mv, (uint4 *)count, n4);
     tpt_pass_end(ctx, PASS_RECALC_CLEAR);
     tpt_pass_begin(ctx, PASS_RECALC_SCATTER);
     hipLaunchKernelGGL(recalc_decay, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
@@ -434,7 +443,10 @@ int tpt_move_rebuild_maps(tpt_ctx *ctx) {
     return rc;
 }
 
-int tpt_recalc_step(tpt_ctx *ctx, uint32_t seed, int stacking) {
+// `stacking`: CheckStacking this call (the device flag must agree); while a
+// tick is captured the stacking kernels are always recorded and gate
+// themselves on the device flag.
+int tpt_recalc_step(tpt_ctx *ctx, int stacking) {
     const uint32_t P = ctx->pixels, slots = ctx->slot_bound;
     const int xres = (int)ctx->cfg.xres, yres = (int)ctx->cfg.yres;
     uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), *count = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP_COUNT);
@@ -444,16 +456,17 @@ int tpt_recalc_step(tpt_ctx *ctx, uint32_t seed, int stacking) {
     int rc = incremental ? map_update(ctx) : map_build(ctx, 1, count, PASS_RECALC_CLEAR, PASS_RECALC_SCATTER);
     if (rc == TPT_OK) rc = tpt_check_launch("recalc_free_particles");
     if (rc) return rc;
-    if (stacking) {
+    if (stacking || ctx->capturing) {
         const uint32_t npart = ctx->cfg.xres * ctx->cfg.yres;
         tpt_pass_begin(ctx, PASS_STACKING);
         hipLaunchKernelGGL(stacking_mark, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, count,
-                           tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), P, ctx->cfg.xres, ctx->xcells, npart, seed);
+                           tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), P, ctx->cfg.xres, ctx->xcells, npart,
+                           ctx->counters);
         hipLaunchKernelGGL(stacking_apply, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
                            tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<float>(ctx, TPT_BUF_PART_X),
                            tpt_buf<float>(ctx, TPT_BUF_PART_Y), tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),
                            tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP), pmap, rec, count,
-                           tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS), slots, xres, yres, npart);
+                           tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS), slots, xres, yres, npart, ctx->counters);
         tpt_pass_end(ctx, PASS_STACKING);
         rc = tpt_check_launch("check_stacking");
         if (rc) return rc;

@@ -158,8 +158,9 @@ __global__ __launch_bounds__(BLK) void th_conduct(Parts p, const uint32_t *__res
                                                   const float *__restrict__ air_hv,
                                                   const uint32_t *__restrict__ props, float *__restrict__ hv_new,
                                                   uint32_t *__restrict__ hv_key, uint32_t *__restrict__ fails,
-                                                  Dims d, uint32_t aheat, uint32_t seed) {
+                                                  Dims d, uint32_t aheat, const uint32_t *__restrict__ seedp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t seed = *seedp;
     if (i < d.slots) conduct_slot(p, pmap, air_hv, props, hv_new, hv_key, fails, d, aheat, seed, i);
 }
 
@@ -171,8 +172,8 @@ __global__ __launch_bounds__(BLK) void th_conduct_list(Parts p, const uint32_t *
                                                        uint32_t *__restrict__ hv_key, uint32_t *__restrict__ fails,
                                                        const uint32_t *__restrict__ list,
                                                        const uint32_t *__restrict__ counters, Dims d, uint32_t aheat,
-                                                       uint32_t seed) {
-    const uint32_t n = counters[TPT_CTR_OFFMAP];
+                                                       const uint32_t *__restrict__ seedp) {
+    const uint32_t n = counters[TPT_CTR_OFFMAP], seed = *seedp;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t i = list[k];
         if (i < d.slots) conduct_slot(p, pmap, air_hv, props, hv_new, hv_key, fails, d, aheat, seed, i);
@@ -332,7 +333,8 @@ __global__ __launch_bounds__(256) void th_conduct_px(Parts p, const unsigned lon
                                                      const uint32_t *__restrict__ props,
                                                      const float *__restrict__ hv_new, uint32_t *__restrict__ hv_key,
                                                      uint32_t *__restrict__ fails, uint2 *__restrict__ heat, Dims d,
-                                                     uint32_t aheat, uint32_t seed) {
+                                                     uint32_t aheat, const uint32_t *__restrict__ seedp) {
+    const uint32_t seed = *seedp;
     __shared__ uint32_t skey[LH][LW];
     __shared__ float stemp[LH][LW];
     __shared__ uint32_t shc[256], sflag[256];
@@ -653,9 +655,10 @@ __global__ __launch_bounds__(BLK) void th_convect_propose_rec(Parts p, const uns
                                                               const float *__restrict__ gfx,
                                                               const float *__restrict__ gfy,
                                                               uint32_t *__restrict__ tgt, uint32_t *__restrict__ touch,
-                                                              Dims d, GravCfg g, uint32_t seed) {
+                                                              Dims d, GravCfg g, const uint32_t *__restrict__ seedp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.slots) return;
+    const uint32_t seed = *seedp;
     touch[i] = NONE;
     uint32_t target = NONE;
     const uint32_t t = p.ty[i];
@@ -707,10 +710,10 @@ __global__ __launch_bounds__(BLK) void th_convect_propose(Parts p, const uint32_
                                                           const float *__restrict__ gfx,
                                                           const float *__restrict__ gfy, uint32_t *__restrict__ tgt,
                                                           uint32_t *__restrict__ touch, Dims d, GravCfg g,
-                                                          uint32_t seed) {
+                                                          const uint32_t *__restrict__ seedp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.slots) return;
-    tgt[i] = convect_target(p, pmap, props, gfx, gfy, d, g, i, seed);
+    tgt[i] = convect_target(p, pmap, props, gfx, gfy, d, g, i, *seedp);
     touch[i] = NONE;
 }
 
@@ -754,8 +757,9 @@ __global__ __launch_bounds__(BLK) void th_transition(Parts p, const float *__res
                                                      const uint32_t *__restrict__ props, float *__restrict__ pv_new,
                                                      uint32_t *__restrict__ pv_key, uint32_t *__restrict__ spark,
                                                      uint32_t *__restrict__ pmap_refresh, Dims d, uint32_t legacy,
-                                                     uint32_t seed) {
+                                                     const uint32_t *__restrict__ seedp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t seed = *seedp;
     if (i >= d.slots) return;
     // The slot's fields are loaded together up front; testing them one by
     // one would chain a memory latency per branch.
@@ -1074,7 +1078,8 @@ __global__ __launch_bounds__(BLK) void th_refresh(const uint32_t *__restrict__ t
 
 }  // namespace
 
-int tpt_thermal_step(tpt_ctx *ctx, uint32_t seed, int legacy) {
+int tpt_thermal_step(tpt_ctx *ctx, int legacy) {
+    const uint32_t *seed = ctx->counters + TPT_CTR_SEED;
     Parts p{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
             tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE), tpt_buf<float>(ctx, TPT_BUF_PART_X),
             tpt_buf<float>(ctx, TPT_BUF_PART_Y), tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),

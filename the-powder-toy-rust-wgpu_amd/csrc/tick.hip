@@ -1,4 +1,5 @@
-// Whole-tick driver and the per-pass C entry points.
+// Whole-tick driver, the per-pass C entry points, graph replay and the
+// batched form.
 //
 // tpt_step follows SimulationEngine::step (crates/tpt-sim/src/lib.rs:303-459)
 // for the passes this library implements: air -> Newtonian gravity (or the
@@ -6,14 +7,105 @@
 // frame) -> conduction/transitions -> movement.  The element reactions
 // (reaction_kernel::step_reactions_gpu, lib.rs:379) are not part of this
 // library.
+//
+// The frame counter, the tick seed and the CheckStacking flag live in the
+// context's device counter block, so that a captured tick replays with the
+// right seed every frame: tick_begin derives them from the frame, and every
+// seeded kernel reads the seed from there.
 
 #include "tpt_internal.h"
 
-static int require(tpt_ctx *ctx) {
+#include <new>
+#include <vector>
+
+namespace {
+
+__device__ __forceinline__ uint32_t frame_seed_dev(uint64_t f) {
+    uint32_t v = (uint32_t)f * 0x9E3779B9u;
+    v = (v << 13) | (v >> 19);
+    return v ^ 0x85EBCA6Bu;
+}
+
+// The tick's seed and CheckStacking flag from frame `f` (the argument, or
+// the device counter when `from_device`), and the counter advanced.
+__global__ void tick_begin(uint32_t *__restrict__ counters, uint32_t lo, uint32_t hi, uint32_t from_device) {
+    if (threadIdx.x != 0) return;
+    uint64_t f = from_device ? (((uint64_t)counters[TPT_CTR_FRAME_HI] << 32) | counters[TPT_CTR_FRAME_LO])
+                             : (((uint64_t)hi << 32) | lo);
+    counters[TPT_CTR_SEED] = frame_seed_dev(f);
+    counters[TPT_CTR_STACK] = (f % 10u) == 0u ? 1u : 0u;
+    f += 1u;
+    counters[TPT_CTR_FRAME_LO] = (uint32_t)f;
+    counters[TPT_CTR_FRAME_HI] = (uint32_t)(f >> 32);
+}
+
+// A per-pass call's seed (and CheckStacking flag).
+__global__ void set_seed(uint32_t *__restrict__ counters, uint32_t seed, uint32_t stacking) {
+    if (threadIdx.x != 0) return;
+    counters[TPT_CTR_SEED] = seed;
+    counters[TPT_CTR_STACK] = stacking;
+}
+
+int require(tpt_ctx *ctx) {
     if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
     if (!ctx->tables_set) return tpt_set_error(TPT_ESTATE, "element table not set");
     return TPT_OK;
 }
+
+int put_seed(tpt_ctx *ctx, uint32_t seed, int stacking) {
+    hipLaunchKernelGGL(set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, seed, stacking ? 1u : 0u);
+    return tpt_check_launch("seed");
+}
+
+// One tick.  `from_device`: take the frame from the device counter (a
+// captured tick); otherwise from `frame`.
+int tick(tpt_ctx *ctx, uint64_t frame, bool from_device) {
+    hipLaunchKernelGGL(tick_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, (uint32_t)frame,
+                       (uint32_t)(frame >> 32), from_device ? 1u : 0u);
+    int rc = tpt_check_launch("tick_begin");
+    if (rc) return rc;
+    if ((rc = tpt_air_step(ctx))) return rc;
+    if ((rc = tpt_gravity_step(ctx, 0u))) return rc;
+    if ((rc = tpt_recalc_step(ctx, !from_device && (frame % 10u) == 0u))) return rc;
+    if ((rc = tpt_thermal_step(ctx, ctx->cfg.heat_enable ? 0 : 1))) return rc;
+    return tpt_movement_step(ctx);
+}
+
+// Records the steady-state tick of `ctx` on its stream, which is already
+// capturing.  Profiling events are not recorded into graphs.
+int record_tick(tpt_ctx *ctx) {
+    const bool prof = ctx->profiling;
+    ctx->profiling = false;
+    ctx->capturing = true;
+    const int rc = tick(ctx, 0, true);
+    ctx->capturing = false;
+    ctx->profiling = prof;
+    return rc;
+}
+
+int capture(tpt_ctx *ctx) {
+    if (ctx->graph) {
+        (void)hipGraphExecDestroy(ctx->graph);
+        ctx->graph = nullptr;
+    }
+    TPT_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = record_tick(ctx);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return tpt_set_error(TPT_EDEVICE, "tick capture: %s", hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return tpt_set_error(TPT_EDEVICE, "graph instantiate: %s", hipGetErrorString(ei));
+    ctx->graph_epoch = ctx->epoch;
+    ctx->graph_slots = ctx->slot_bound;
+    return TPT_OK;
+}
+
+}  // namespace
 
 extern "C" int tpt_step_gravity(tpt_ctx *ctx, uint32_t seed) {
     int rc = require(ctx);
@@ -27,26 +119,181 @@ extern "C" int tpt_gravity_solve(tpt_ctx *ctx) {
 
 extern "C" int tpt_recalc_free_particles(tpt_ctx *ctx, uint32_t seed, int stacking) {
     int rc = require(ctx);
-    return rc ? rc : tpt_recalc_step(ctx, seed, stacking);
+    if (!rc) rc = put_seed(ctx, seed, stacking);
+    return rc ? rc : tpt_recalc_step(ctx, stacking);
 }
 
 extern "C" int tpt_step_thermal(tpt_ctx *ctx, uint32_t seed, int legacy) {
     int rc = require(ctx);
-    return rc ? rc : tpt_thermal_step(ctx, seed, legacy);
+    if (!rc) rc = put_seed(ctx, seed, 0);
+    return rc ? rc : tpt_thermal_step(ctx, legacy);
 }
 
 extern "C" int tpt_step_movement(tpt_ctx *ctx, uint32_t seed) {
     int rc = require(ctx);
-    return rc ? rc : tpt_movement_step(ctx, seed);
+    if (!rc) rc = put_seed(ctx, seed, 0);
+    return rc ? rc : tpt_movement_step(ctx);
 }
 
 extern "C" int tpt_step(tpt_ctx *ctx, uint64_t frame_count) {
     int rc = require(ctx);
-    if (rc) return rc;
-    const uint32_t seed = tpt_frame_seed(frame_count);
-    if ((rc = tpt_air_step(ctx))) return rc;
-    if ((rc = tpt_gravity_step(ctx, seed))) return rc;
-    if ((rc = tpt_recalc_step(ctx, seed, (frame_count % 10u) == 0u))) return rc;
-    if ((rc = tpt_thermal_step(ctx, seed, ctx->cfg.heat_enable ? 0 : 1))) return rc;
-    return tpt_movement_step(ctx, seed);
+    return rc ? rc : tick(ctx, frame_count, false);
+}
+
+extern "C" int tpt_run(tpt_ctx *ctx, uint64_t first_frame, uint32_t ticks) {
+    int rc = require(ctx);
+    if (rc || ticks == 0) return rc;
+    // The eager first tick brings the context to the steady state the
+    // captured tick assumes (maps rebuilt by the previous movement pass).
+    if ((rc = tick(ctx, first_frame, false))) return rc;
+    if (ticks == 1) return TPT_OK;
+    if (!ctx->graph || ctx->graph_epoch != ctx->epoch || ctx->graph_slots != ctx->slot_bound)
+        if ((rc = capture(ctx))) return rc;
+    for (uint32_t t = 1; t < ticks; ++t) TPT_HIP_CHECK(hipGraphLaunch(ctx->graph, ctx->stream));
+    return TPT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched form.
+
+struct tpt_batch {
+    std::vector<tpt_ctx *> ctx;
+    hipStream_t main = nullptr;
+    hipEvent_t fork = nullptr, done = nullptr;  // fork: recorded inside the capture only
+    std::vector<hipEvent_t> join, pre;           // join: inside the capture; pre: outside
+    hipGraphExec_t graph = nullptr;
+    std::vector<uint64_t> epochs;
+    std::vector<uint32_t> slots;
+};
+
+static void batch_free(tpt_batch *b) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->pre)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->main) (void)hipStreamDestroy(b->main);
+    for (tpt_ctx *c : b->ctx)
+        if (c) tpt_destroy(c);
+    delete b;
+}
+
+extern "C" int tpt_batch_create(uint32_t n, const tpt_config *cfgs, tpt_batch **out) {
+    if (!out || !cfgs || n == 0) return tpt_set_error(TPT_EINVAL, "batch needs n >= 1 configs and an output");
+    *out = nullptr;
+    tpt_batch *b = new (std::nothrow) tpt_batch();
+    if (!b) return tpt_set_error(TPT_ENOMEM, "host allocation failed");
+    b->ctx.assign(n, nullptr);
+    b->join.assign(n, nullptr);
+    b->pre.assign(n, nullptr);
+    int rc = TPT_OK;
+    for (uint32_t k = 0; k < n && rc == TPT_OK; ++k) rc = tpt_create(&cfgs[k], &b->ctx[k]);
+    if (rc == TPT_OK && (hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking) != hipSuccess ||
+                         hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+                         hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess))
+        rc = tpt_set_error(TPT_EDEVICE, "batch stream/event creation failed");
+    for (uint32_t k = 0; k < n && rc == TPT_OK; ++k)
+        if (hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b->pre[k], hipEventDisableTiming) != hipSuccess)
+            rc = tpt_set_error(TPT_EDEVICE, "batch event creation failed");
+    if (rc != TPT_OK) {
+        batch_free(b);
+        return rc;
+    }
+    *out = b;
+    return TPT_OK;
+}
+
+extern "C" int tpt_batch_destroy(tpt_batch *b) {
+    if (!b) return tpt_set_error(TPT_EINVAL, "null batch");
+    (void)tpt_batch_synchronize(b);
+    batch_free(b);
+    return TPT_OK;
+}
+
+extern "C" int tpt_batch_context(tpt_batch *b, uint32_t k, tpt_ctx **ctx) {
+    if (!b || !ctx || k >= b->ctx.size()) return tpt_set_error(TPT_EINVAL, "batch context %u out of range", k);
+    *ctx = b->ctx[k];
+    return TPT_OK;
+}
+
+// Makes `waiter` wait for everything queued on `on` so far.
+static int stream_after(hipStream_t waiter, hipStream_t on, hipEvent_t ev) {
+    TPT_HIP_CHECK(hipEventRecord(ev, on));
+    TPT_HIP_CHECK(hipStreamWaitEvent(waiter, ev, 0));
+    return TPT_OK;
+}
+
+static int batch_capture(tpt_batch *b) {
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    TPT_HIP_CHECK(hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
+    int rc = TPT_OK;
+    hipError_t e = hipEventRecord(b->fork, b->main);
+    for (size_t k = 0; k < b->ctx.size() && rc == TPT_OK && e == hipSuccess; ++k) {
+        tpt_ctx *c = b->ctx[k];
+        e = hipStreamWaitEvent(c->stream, b->fork, 0);
+        if (e == hipSuccess) rc = record_tick(c);
+        if (rc == TPT_OK && e == hipSuccess) e = hipEventRecord(b->join[k], c->stream);
+        if (rc == TPT_OK && e == hipSuccess) e = hipStreamWaitEvent(b->main, b->join[k], 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(b->main, &g);
+    if (rc != TPT_OK || e != hipSuccess || ee != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc) return rc;
+        return tpt_set_error(TPT_EDEVICE, "batch capture: %s", hipGetErrorString(e != hipSuccess ? e : ee));
+    }
+    const hipError_t ei = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return tpt_set_error(TPT_EDEVICE, "batch graph instantiate: %s", hipGetErrorString(ei));
+    b->epochs.resize(b->ctx.size());
+    b->slots.resize(b->ctx.size());
+    for (size_t k = 0; k < b->ctx.size(); ++k) {
+        b->epochs[k] = b->ctx[k]->epoch;
+        b->slots[k] = b->ctx[k]->slot_bound;
+    }
+    return TPT_OK;
+}
+
+extern "C" int tpt_batch_step(tpt_batch *b, uint64_t first_frame, uint32_t ticks) {
+    if (!b) return tpt_set_error(TPT_EINVAL, "null batch");
+    if (ticks == 0) return TPT_OK;
+    int rc = TPT_OK;
+    for (tpt_ctx *c : b->ctx)
+        if ((rc = require(c)) || (rc = tick(c, first_frame, false))) return rc;
+    if (ticks == 1) return TPT_OK;
+    bool stale = !b->graph || b->epochs.size() != b->ctx.size();
+    for (size_t k = 0; k < b->ctx.size() && !stale; ++k)
+        stale = b->epochs[k] != b->ctx[k]->epoch || b->slots[k] != b->ctx[k]->slot_bound;
+    // The replays on the batch stream follow each canvas's eager tick ...
+    for (size_t k = 0; k < b->ctx.size(); ++k)
+        if ((rc = stream_after(b->main, b->ctx[k]->stream, b->pre[k]))) return rc;
+    if (stale && (rc = batch_capture(b))) return rc;
+    for (uint32_t t = 1; t < ticks; ++t) TPT_HIP_CHECK(hipGraphLaunch(b->graph, b->main));
+    // ... and later work on a canvas's own stream follows the replays.
+    for (tpt_ctx *c : b->ctx)
+        if ((rc = stream_after(c->stream, b->main, b->done))) return rc;
+    return TPT_OK;
+}
+
+extern "C" int tpt_batch_step_launches(tpt_batch *b, uint64_t first_frame, uint32_t ticks) {
+    if (!b) return tpt_set_error(TPT_EINVAL, "null batch");
+    for (uint32_t t = 0; t < ticks; ++t)
+        for (tpt_ctx *c : b->ctx) {
+            int rc = require(c);
+            if (rc || (rc = tick(c, first_frame + t, false))) return rc;
+        }
+    return TPT_OK;
+}
+
+extern "C" int tpt_batch_synchronize(tpt_batch *b) {
+    if (!b) return tpt_set_error(TPT_EINVAL, "null batch");
+    TPT_HIP_CHECK(hipStreamSynchronize(b->main));
+    for (tpt_ctx *c : b->ctx) TPT_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return TPT_OK;
 }

@@ -53,13 +53,13 @@ static void fold_oldest(tpt_ctx *ctx, int pass) {
 }
 
 void tpt_pass_begin(tpt_ctx *ctx, int pass) {
-    if (!ctx->profiling) return;
+    if (!ctx->profiling || ctx->capturing) return;
     if (ctx->ev_pending[pass] == tpt_ctx::RING) fold_oldest(ctx, pass);
     hipEventRecord(ctx->ev_start[pass][ctx->ev_head[pass]], ctx->stream);
 }
 
 void tpt_pass_end(tpt_ctx *ctx, int pass) {
-    if (!ctx->profiling) return;
+    if (!ctx->profiling || ctx->capturing) return;
     int slot = ctx->ev_head[pass];
     hipEventRecord(ctx->ev_stop[pass][slot], ctx->stream);
     ctx->ev_head[pass] = (slot + 1) % tpt_ctx::RING;
@@ -116,6 +116,7 @@ static void free_all(tpt_ctx *c) {
     for (void *p : extra)
         if (p) hipFree(p);
     tpt_gravity_free(c);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (int p = 0; p < PASS_COUNT; ++p)
         for (int r = 0; r < tpt_ctx::RING; ++r) {
             if (c->ev_start[p][r]) hipEventDestroy(c->ev_start[p][r]);
@@ -149,12 +150,6 @@ extern "C" int tpt_create(const tpt_config *cfg, tpt_ctx **out) {
         return tpt_set_error(TPT_EDEVICE, "no HIP device or stream creation failed");
     }
     int rc = alloc_buffers(c);
-    if (rc == TPT_OK) {
-        for (int p = 0; p < PASS_COUNT && rc == TPT_OK; ++p)
-            for (int r = 0; r < tpt_ctx::RING && rc == TPT_OK; ++r)
-                if (hipEventCreate(&c->ev_start[p][r]) != hipSuccess || hipEventCreate(&c->ev_stop[p][r]) != hipSuccess)
-                    rc = tpt_set_error(TPT_EDEVICE, "event creation failed");
-    }
     if (rc == TPT_OK && hipStreamSynchronize(c->stream) != hipSuccess)
         rc = tpt_set_error(TPT_EDEVICE, "initial clear failed");
     if (rc != TPT_OK) {
@@ -181,6 +176,7 @@ extern "C" int tpt_set_element_table(tpt_ctx *ctx, const uint32_t *props, size_t
         return tpt_set_error(TPT_EINVAL, "table sizes %zu/%zu, expected %u/%u", props_words, lut_words,
                              TPT_PT_NUM * TPT_ELEM_STRIDE, TPT_MOVE_LUT_WORDS);
     ctx->maps_fresh = ctx->rec_cover = ctx->rebuild_fresh = false;
+    ctx->epoch++;
     TPT_HIP_CHECK(hipMemcpyAsync(ctx->buf[TPT_BUF_ELEM_PROPS], props, props_words * 4, hipMemcpyHostToDevice, ctx->stream));
     TPT_HIP_CHECK(hipMemcpyAsync(ctx->buf[TPT_BUF_MOVE_LUT], move_lut, lut_words * 4, hipMemcpyHostToDevice, ctx->stream));
     TPT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -204,6 +200,7 @@ extern "C" int tpt_upload(tpt_ctx *ctx, int buf, const void *src, size_t off, si
     if (!src && bytes) return tpt_set_error(TPT_EINVAL, "null source");
     if (!bytes) return TPT_OK;
     ctx->maps_fresh = ctx->rec_cover = ctx->rebuild_fresh = false;
+    ctx->epoch++;
     if (buf == TPT_BUF_WALL_BMAP || buf == TPT_BUF_WALL_EMAP) {
         const uint32_t *w = (const uint32_t *)src;
         bool found = false;
@@ -243,7 +240,10 @@ extern "C" int tpt_set_slot_bound(tpt_ctx *ctx, uint32_t bound) {
     if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
     if (bound < 1) bound = 1;
     if (bound > ctx->cfg.capacity) bound = ctx->cfg.capacity;
-    if (bound != ctx->slot_bound) ctx->maps_fresh = ctx->rec_cover = ctx->rebuild_fresh = false;
+    if (bound != ctx->slot_bound) {
+        ctx->maps_fresh = ctx->rec_cover = ctx->rebuild_fresh = false;
+        ctx->epoch++;
+    }
     ctx->slot_bound = bound;
     return TPT_OK;
 }
@@ -268,6 +268,14 @@ extern "C" int tpt_step_air(tpt_ctx *ctx) {
 
 extern "C" int tpt_set_profiling(tpt_ctx *ctx, int enable) {
     if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
+    // Events are created on first use: a batch of many contexts needs none.
+    if (enable && !ctx->ev_start[0][0]) {
+        for (int p = 0; p < PASS_COUNT; ++p)
+            for (int r = 0; r < tpt_ctx::RING; ++r)
+                if (hipEventCreate(&ctx->ev_start[p][r]) != hipSuccess ||
+                    hipEventCreate(&ctx->ev_stop[p][r]) != hipSuccess)
+                    return tpt_set_error(TPT_EDEVICE, "event creation failed");
+    }
     ctx->profiling = enable != 0;
     return TPT_OK;
 }

@@ -21,6 +21,10 @@
 #define TPT_CTR_OFFMAP 2  // off-map list length (see spatial.hip)
 #define TPT_CTR_EMAP_GATE 3  // wire closure gate bits (move.hip)
 #define TPT_CTR_PHOT 4       // blocked-photon hit list length (move.hip)
+#define TPT_CTR_FRAME_LO 8   // next frame_count, low / high word (tick.hip)
+#define TPT_CTR_FRAME_HI 9
+#define TPT_CTR_SEED 10      // seed of the pass(es) running now
+#define TPT_CTR_STACK 11     // 1: CheckStacking this tick
 
 // Passes that carry their own timing slot (see tpt_get_pass_times).
 enum tpt_pass {
@@ -113,6 +117,15 @@ struct tpt_ctx {
     // span-end marker bits.
     bool has_wire = false;
     bool emap_markers = false;
+
+    // Graph capture of the steady-state tick (tick.hip).  `capturing` makes
+    // the passes record every kernel whose launch is otherwise decided on
+    // the host per frame; `epoch` counts changes that invalidate a capture.
+    bool capturing = false;
+    uint64_t epoch = 0;
+    uint64_t graph_epoch = ~0ull;
+    uint32_t graph_slots = 0;
+    hipGraphExec_t graph = nullptr;
     uint32_t *pix_count_mv = nullptr;     // pixels: particle counts of the movement rebuild
     uint32_t *slot_owner = nullptr;       // capacity words, all ones at rest (photon hit rounds)
     tpt_fft_state fft;
@@ -153,9 +166,9 @@ int tpt_air_step(tpt_ctx *ctx);
 int tpt_gravity_step(tpt_ctx *ctx, uint32_t seed);
 int tpt_gravity_solve_only(tpt_ctx *ctx);
 int tpt_gravity_free(tpt_ctx *ctx);
-int tpt_recalc_step(tpt_ctx *ctx, uint32_t seed, int stacking);
-int tpt_thermal_step(tpt_ctx *ctx, uint32_t seed, int legacy);
-int tpt_movement_step(tpt_ctx *ctx, uint32_t seed);
+int tpt_recalc_step(tpt_ctx *ctx, int stacking);
+int tpt_thermal_step(tpt_ctx *ctx, int legacy);
+int tpt_movement_step(tpt_ctx *ctx);
 int tpt_move_rebuild_maps(tpt_ctx *ctx);
 
 // Device helpers ------------------------------------------------------------

@@ -54,6 +54,13 @@ SIGNATURES = {
     "tpt_step_thermal": ([_vp, ctypes.c_uint32, ctypes.c_int], ctypes.c_int),
     "tpt_step_movement": ([_vp, ctypes.c_uint32], ctypes.c_int),
     "tpt_step": ([_vp, ctypes.c_uint64], ctypes.c_int),
+    "tpt_run": ([_vp, ctypes.c_uint64, ctypes.c_uint32], ctypes.c_int),
+    "tpt_batch_create": ([ctypes.c_uint32, _vp, ctypes.POINTER(_vp)], ctypes.c_int),
+    "tpt_batch_destroy": ([_vp], ctypes.c_int),
+    "tpt_batch_context": ([_vp, ctypes.c_uint32, ctypes.POINTER(_vp)], ctypes.c_int),
+    "tpt_batch_step": ([_vp, ctypes.c_uint64, ctypes.c_uint32], ctypes.c_int),
+    "tpt_batch_step_launches": ([_vp, ctypes.c_uint64, ctypes.c_uint32], ctypes.c_int),
+    "tpt_batch_synchronize": ([_vp], ctypes.c_int),
     "tpt_frame_seed": ([ctypes.c_uint64], ctypes.c_uint32),
     "tpt_synchronize": ([_vp], ctypes.c_int),
     "tpt_set_profiling": ([_vp, ctypes.c_int], ctypes.c_int),
@@ -96,22 +103,29 @@ def frame_seed(frame_count):
     return load().tpt_frame_seed(frame_count)
 
 
+def _config(cfg):
+    c = TptConfig()
+    for name, _ in TptConfig._fields_:
+        setattr(c, name, cfg[name])
+    return c
+
+
 class Context:
     """One tpt_ctx: device buffers for one canvas plus its HIP stream."""
 
-    def __init__(self, cfg):
-        lib = load()
-        c = TptConfig()
-        for name, _ in TptConfig._fields_:
-            setattr(c, name, cfg[name])
+    def __init__(self, cfg, handle=None):
         self.cfg = dict(cfg)
+        self._owned = handle is None
         self._h = _vp()
-        _check(lib.tpt_create(ctypes.byref(c), ctypes.byref(self._h)))
+        if handle is None:
+            _check(load().tpt_create(ctypes.byref(_config(cfg)), ctypes.byref(self._h)))
+        else:
+            self._h = handle
 
     def close(self):
-        if self._h:
+        if self._h and self._owned:
             _check(load().tpt_destroy(self._h))
-            self._h = _vp()
+        self._h = _vp()
 
     def __del__(self):
         try:
@@ -186,6 +200,10 @@ class Context:
     def step(self, frame_count):
         _check(load().tpt_step(self._h, frame_count))
 
+    def run(self, first_frame, ticks):
+        """`ticks` ticks from `first_frame`: one eager tick, then graph replays."""
+        _check(load().tpt_run(self._h, first_frame, ticks))
+
     def synchronize(self):
         _check(load().tpt_synchronize(self._h))
 
@@ -203,3 +221,46 @@ class Context:
         n = ctypes.c_int()
         _check(load().tpt_get_pass_times(self._h, names, ms, n_l, PASS_SLOTS, ctypes.byref(n)))
         return {names[i].decode(): (ms[i], n_l[i]) for i in range(n.value)}
+
+
+class Batch:
+    """tpt_batch: independent canvases advanced together (one graph replay
+    per tick); `contexts[k]` loads and reads canvas k."""
+
+    def __init__(self, cfgs):
+        arr = (TptConfig * len(cfgs))(*[_config(c) for c in cfgs])
+        self._h = _vp()
+        _check(load().tpt_batch_create(len(cfgs), arr, ctypes.byref(self._h)))
+        self.contexts = []
+        for k, c in enumerate(cfgs):
+            h = _vp()
+            _check(load().tpt_batch_context(self._h, k, ctypes.byref(h)))
+            self.contexts.append(Context(c, handle=h))
+
+    def step(self, first_frame, ticks):
+        _check(load().tpt_batch_step(self._h, first_frame, ticks))
+
+    def step_launches(self, first_frame, ticks):
+        _check(load().tpt_batch_step_launches(self._h, first_frame, ticks))
+
+    def synchronize(self):
+        _check(load().tpt_batch_synchronize(self._h))
+
+    def close(self):
+        if self._h:
+            for c in self.contexts:
+                c.close()
+            _check(load().tpt_batch_destroy(self._h))
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
