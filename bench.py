@@ -101,6 +101,71 @@ def oracle_pass(o, name, frame):
         o.call("oracle_step", frame)
 
 
+def batched(args):
+    """The batched form: N independent canvases with sides drawn from
+    128..2048 in steps of 4 (seed 2), each with its own seed, a fill drawn
+    from 10-50%, the 612x384 workload's element mix and init rules.  Times
+    the same ticks as per-canvas tpt_step launches and as tpt_batch_step
+    (one eager tick, then replays of one graph holding every canvas)."""
+    import numpy as np
+    from tpt_hip import lib as L
+    from tpt_hip import scene as S
+
+    rng = np.random.default_rng(2)
+    n = args.batched
+    shapes = [(4 * int(rng.integers(32, 513)), 4 * int(rng.integers(32, 513))) for _ in range(n)]
+    fills = rng.uniform(0.10, 0.50, n)
+    cfgs, particles = [], 0
+    t0 = time.perf_counter()
+    for k, (w, h) in enumerate(shapes):
+        cfgs.append(S.default_config(w, h))
+    batch = L.Batch(cfgs)
+    for k, (w, h) in enumerate(shapes):
+        st = S.make_scene(xres=w, yres=h, seed=1000 + k, particles=int(fills[k] * w * h))
+        batch.contexts[k].load_state(st)
+        particles += int((st.arrays["part_ty"] != 0).sum())
+        del st
+        if k % 32 == 31:
+            print(f"[batched] {k + 1}/{n} canvases loaded ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+    frame = 0
+    batch.step_launches(frame, args.warmup)
+    frame += args.warmup
+    batch.synchronize()
+    t0 = time.perf_counter()
+    batch.step_launches(frame, args.steps)
+    batch.synchronize()
+    t_launch = time.perf_counter() - t0
+    frame += args.steps
+    batch.step(frame, 2)  # capture
+    batch.synchronize()
+    frame += 2
+    t0 = time.perf_counter()
+    batch.step(frame, args.steps)
+    batch.synchronize()
+    t_graph = time.perf_counter() - t0
+    batch.close()
+    px = sum(w * h for w, h in shapes)
+    out = {
+        "metric": "aggregate canvas-ticks/s, batched form: graph replay vs per-canvas launches",
+        "value": round(n * args.steps / t_graph, 2),
+        "unit": "canvas-ticks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_graph / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{n} canvases, sides 128..2048 step 4 (seed 2), fill 10-50%",
+                   "pixels": px, "particles": particles, "parallelism": "single-gpu"},
+        "per_canvas_launches": {"value": round(n * args.steps / t_launch, 2), "ms_per_step": round(t_launch / args.steps * 1e3, 3)},
+        "speedup_vs_launches": round(t_launch / t_graph, 3),
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -111,6 +176,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--batched", type=int, default=0, metavar="N",
+                    help="the batched form instead: N mixed-size canvases, graph replay vs per-canvas launches")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -124,6 +191,8 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.batched:
+        return batched(args)
 
     from tpt_hip import lib as L
     from tpt_hip import scene as S
@@ -156,6 +225,15 @@ def main():
     ctx.synchronize()
     dt = time.perf_counter() - t0
     ticks_per_s = args.steps / dt
+    # The same ticks through tpt_run: one eager tick, then graph replays.
+    ctx.run(frame, 2)
+    frame += 2
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.run(frame, args.steps)
+    ctx.synchronize()
+    dt_graph = time.perf_counter() - t0
+    frame += args.steps
 
     # Per-kernel durations from HIP events on the library's stream, over a
     # second run of the same number of ticks.
@@ -222,6 +300,7 @@ def main():
                    "air_cells": C, "pixels": P, "particles": N, "parallelism": "single-gpu"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "graph_replay": {"value": round(args.steps / dt_graph, 2), "ms_per_step": round(dt_graph / args.steps * 1e3, 4)},
         "per_kernel": per_kernel,
     }
     print(json.dumps(out))
