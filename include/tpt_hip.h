@@ -48,6 +48,10 @@ extern "C" {
 #define TPT_MOVE_LUT_WORDS (TPT_PT_NUM * TPT_MOVE_META_WORDS + TPT_PT_NUM * TPT_PT_NUM)
 #define TPT_MAX_CAPACITY 0x7FFFFFu /* 23-bit pmap slot field (layout.rs:21-31) */
 #define TPT_MAX_DIM 2896u          /* largest canvas side (layout.rs:33-40)    */
+#define TPT_SPAWN_DEFAULT_WORDS 8  /* u32 words per spawn-defaults row          */
+#define TPT_GPU_SPAWN_START 1024u  /* device allocation window, slots           */
+#define TPT_GPU_SPAWN_END 17408u   /* [START, END) (buffers/mod.rs:502-514)     */
+#define TPT_GPU_CREATE_COUNT 16384u /* creations drained per tick (mod.rs:514)  */
 
 /* Mirror of the simulation-relevant fields of SimulationConfig
  * (crates/tpt-core/src/particle.rs:166-346).  Enums use the reference's
@@ -101,6 +105,11 @@ typedef enum tpt_buf_id {
     TPT_BUF_MOVE_LUT,    /* TPT_MOVE_LUT_WORDS words            */
     TPT_BUF_PHOTON_MASK, /* TPT_PT_NUM words: PhotonReflectWavelengths
                             (build_photon_mask, move_kernel/mod.rs:1592-1606) */
+    TPT_BUF_SPAWN_DEFAULTS, /* TPT_PT_NUM*TPT_SPAWN_DEFAULT_WORDS words:
+                            DefaultProperties rows {temp bits, life, ctype,
+                            tmp, tmp2, tmp3, tmp4, 0}
+                            (ElementRegistry::to_gpu_spawn_defaults_u32,
+                            tpt-core/src/elements/mod.rs:905-936) */
     TPT_BUF_COUNT
 } tpt_buf_id;
 
@@ -134,6 +143,11 @@ TPT_API int tpt_buffer_bytes(tpt_ctx *ctx, int buf, size_t *bytes);
  * slot passes scan [0, bound).  The host owns slot allocation, so it sets
  * the bound after loading particles; clamped to [1, capacity]. */
 TPT_API int tpt_set_slot_bound(tpt_ctx *ctx, uint32_t bound);
+/* The current bound.  A movement pass raises it to cover the device
+ * allocation window [TPT_GPU_SPAWN_START, TPT_GPU_SPAWN_END) (clamped to the
+ * capacity), as the reference's `gpu_spawn_used` does once the movement
+ * step has armed its creation queue (move_kernel/mod.rs:1212-1216). */
+TPT_API int tpt_get_slot_bound(tpt_ctx *ctx, uint32_t *bound);
 
 /* Per-pass entry points, in tick order. */
 
@@ -158,7 +172,12 @@ TPT_API int tpt_recalc_free_particles(tpt_ctx *ctx, uint32_t seed, int stacking)
  * (crates/tpt-sim/src/thermal_kernel.rs:1574-1717). */
 TPT_API int tpt_step_thermal(tpt_ctx *ctx, uint32_t seed, int legacy);
 
-/* move_kernel::step_movement_gpu (crates/tpt-sim/src/move_kernel/mod.rs:1176-1227). */
+/* move_kernel::step_movement_gpu (crates/tpt-sim/src/move_kernel/mod.rs:1176-1227),
+ * including the e == 2 pair pass between the intent and commit launches
+ * (reaction_kernel/energy_entry.rs:90-393) and the drain of the photons it
+ * creates (begin/drain_pending_creates, reaction_kernel/mod.rs:138-157,
+ * create_alloc/create_fields_kernel, reaction_kernel/shared.rs:516-912).
+ * The pair pass needs TPT_BUF_SPAWN_DEFAULTS. */
 TPT_API int tpt_step_movement(tpt_ctx *ctx, uint32_t seed);
 
 /* SimulationEngine::step (crates/tpt-sim/src/lib.rs:303-459) for the passes

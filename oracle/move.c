@@ -15,9 +15,14 @@
  *      take an empty one (the lowest such slot), energy: highest slot.
  *  SAW the WOOD -> SAWD conversion of a rejected probe (mod.rs:213-232) is
  *      applied after the intent pass (the reference writes it mid-pass).
- * Not restated (no counterpart in this library): photon refraction at glass
- * (velocity.rs:262-647), the e == 2 pair pass (reaction_kernel/energy_entry.rs),
- * WL_DETECT charging and its emap closure (mod.rs:473-828).
+ *  R10 blocked-photon hits are applied after the commit loop, in ascending
+ *      photon slot order per hit particle.
+ *  DET WL_DETECT charging: the union of the runs through every triggered
+ *      cell, computed from the pre-pass emap.
+ *  R11/R12 the e == 2 pair pass and its creations (see pair_resolve).
+ * Also restated: photon refraction at glass (velocity.rs:262-647), the
+ * e == 2 pair pass (reaction_kernel/energy_entry.rs:90-393) with the drain of
+ * its photons, WL_DETECT charging and its emap closure (mod.rs:473-828).
  */
 #include "tpt_oracle.h"
 
@@ -1192,13 +1197,314 @@ void oracle_emap_closure(oracle_state *s) {
     for (uint32_t c = 0; c < cells; ++c) s->wall_emap[c] &= ~(EMAP_LEFT | EMAP_RIGHT);
 }
 
-/* step_movement_gpu (mod.rs:1176-1227) without the pair pass. */
+/* ---- e == 2 pair pass (reaction_kernel/energy_entry.rs:90-393) ----------
+ *
+ * Canonical form, shared with the HIP kernels:
+ *  R11 movers are resolved in ascending slot order against the types the
+ *      intent pass left (the reference's transient PAIR_CLAIMED type of an
+ *      occupant is not modelled: each occupant has at most one mover);
+ *  R12 creation records are allocated in ascending requester slot order, the
+ *      k-th record taking the k-th free slot of the device window
+ *      [1024, 17408) in ascending order; at most 16384 records. */
+
+#define PT_GLAS 24u
+#define PT_PHOT 29u
+#define PT_NEUT 30u
+#define PT_ELEC 31u
+#define PT_H2 32u
+#define PT_GRVT 39u
+#define PT_GPMP 43u
+#define PT_GLOW 66u
+#define PT_BIZR 85u
+#define PT_BIZRG 86u
+#define PT_BIZRS 87u
+#define PT_BGLA 95u
+#define PT_INVIS 124u
+#define PT_FILT 134u
+#define PT_C5 139u
+#define PT_PROT 173u
+#define SPAWN_START 1024u
+#define SPAWN_END 17408u
+#define CREATE_COUNT 16384u
+
+/* hash_u32 (reaction_kernel/shared.rs:1015-1023). */
+uint32_t oracle_hash_u32(uint32_t x) {
+    uint32_t h = x;
+    h ^= h >> 16;
+    h *= 2246822507u;
+    h ^= h >> 13;
+    h *= 3266489909u;
+    h ^= h >> 16;
+    return h;
+}
+
+/* Element_FILT_getWavelengths (reaction_kernel/rays/emitter.rs:10-26). */
+static uint32_t filt_wavelengths(uint32_t ctype, float temp) {
+    if ((ctype & 0x3FFFFFFFu) != 0u) return ctype & 0x3FFFFFFFu;
+    int bin = ora_f2i((temp - 273.0f) * 0.025f);
+    if (bin < 0) bin = 0;
+    if (bin > 25) bin = 25;
+    return 0x1Fu << (uint32_t)bin;
+}
+
+/* Element_FILT_interactWavelengths (reaction_kernel/rays/emitter.rs:28-80);
+ * shift counts keep their low five bits, as u32 shifts in WGSL do. */
+uint32_t oracle_interact_wavelengths(int32_t filt_tmp, uint32_t filt_ctype, float filt_temp, uint32_t orig_wl) {
+    const uint32_t mask = 0x3FFFFFFFu;
+    uint32_t filt_wl = filt_wavelengths(filt_ctype, filt_temp);
+    if (filt_tmp == 0) return filt_wl;
+    if (filt_tmp == 1) return orig_wl & filt_wl;
+    if (filt_tmp == 2) return orig_wl | filt_wl;
+    if (filt_tmp == 3) return orig_wl & ~filt_wl;
+    if (filt_tmp == 4 || filt_tmp == 5) {
+        int shift = ora_f2i((filt_temp - 273.0f) * 0.025f);
+        if (shift <= 0) shift = 1;
+        uint32_t sh = (uint32_t)shift & 31u;
+        return (filt_tmp == 4 ? orig_wl << sh : orig_wl >> sh) & mask;
+    }
+    if (filt_tmp == 6) return orig_wl;
+    if (filt_tmp == 7) return orig_wl ^ filt_wl;
+    if (filt_tmp == 8) return ~orig_wl & mask;
+    if (filt_tmp == 10) {
+        uint32_t lsb = filt_wl & (~filt_wl + 1u);
+        return (orig_wl * lsb) & mask;
+    }
+    if (filt_tmp == 11) {
+        uint32_t lsb = filt_wl & (~filt_wl + 1u);
+        return lsb != 0u ? (orig_wl / lsb) & mask : orig_wl;
+    }
+    return filt_wl;
+}
+
+typedef struct {
+    uint32_t slot, x, y, ctype;
+    float temp, vx, vy;
+} create_rec;
+
+typedef struct {
+    create_rec *rec;
+    uint32_t n, cap;
+    uint32_t *free_slots; /* ascending */
+    uint32_t n_free;
+} pair_queue;
+
+static void queue_photon(pair_queue *q, uint32_t slot, int x, int y, uint32_t ctype, float temp, float vx, float vy) {
+    if (q->n >= q->cap) return; /* requesters come in ascending slot order: the lowest ones are kept */
+    create_rec r = {slot, (uint32_t)x, (uint32_t)y, ctype, temp, vx, vy};
+    q->rec[q->n++] = r;
+}
+
+/* begin_pending_creates' free-slot collection (collect_gpu_free_slots_kernel,
+ * reaction_kernel/shared.rs:1197-1222): empty, not portal-held window slots. */
+static void collect_free(const oracle_state *s, pair_queue *q) {
+    q->n_free = 0;
+    for (uint32_t slot = SPAWN_START; slot < SPAWN_END && slot < s->capacity; ++slot)
+        if (s->part_ty[slot] == 0u && (s->part_flags[slot] & 0x04u) == 0u) q->free_slots[q->n_free++] = slot;
+}
+
+/* pair_resolve_kernel (energy_entry.rs:90-393) for every slot, in order. */
+static void pair_resolve(oracle_state *s, uint32_t frame_seed, pair_queue *q) {
+    const int xres = (int)s->xres, yres = (int)s->yres;
+    const uint32_t xc = s->xres / 4, pixels = s->xres * s->yres;
+    const uint32_t *def_elec = s->spawn_defaults + PT_ELEC * 8u;
+    for (uint32_t i = 0; i < s->slot_bound; ++i) {
+        uint32_t ty = s->part_ty[i];
+        if (ty == 0u || ty >= 256u) continue;
+        if (!(ty == PT_PHOT || ty == PT_NEUT || ty == PT_ELEC || ty == PT_PROT || ty == PT_BIZR || ty == PT_BIZRG))
+            continue;
+        uint32_t raw = s->intent_target[i];
+        if (raw == NO_INTENT || (raw & 0x80000000u) != 0u) continue;
+        uint32_t dest = raw & INTENT_PIXEL_MASK;
+        if (dest >= pixels) continue;
+        float xf = s->part_x[i], yf = s->part_y[i];
+        int xi = ora_f2i(xf + 0.5f), yi = ora_f2i(yf + 0.5f);
+        uint32_t own = (uint32_t)(yi * xres + xi);
+        if (own == dest) continue;
+        uint32_t m = s->pmap[dest];
+        if (m < 512u) continue;
+        uint32_t rty = m & 0x1FFu, r = (m >> 9) - 1u;
+        if (r >= s->slot_bound) continue;
+        /* energy_entry.rs:171-175: a 27-bit mask, which also takes in the low
+         * four bits of the winner's weight rank. */
+        if ((s->cell_winner[dest] & 0x07FFFFFFu) != ((i + 1u) & 0x07FFFFFFu)) continue;
+        /* The occupant claim (a CAS from the map's type) fails on a slot the
+         * intent pass killed or a WOOD it sawed. */
+        if (s->part_ty[r] != rty) continue;
+        uint32_t seed = oracle_hash_u32(frame_seed ^ oracle_hash_u32(i));
+        uint32_t final_ty = rty;
+        if (ty == PT_PHOT) {
+            if (rty == PT_GLOW) { /* Simulation.cpp:1263-1272 */
+                if (s->part_life[r] == 0 && seed % 30u == 0u) {
+                    s->part_life[r] = 120;
+                    int lr = -1;
+                    if (oracle_hash_u32(seed ^ 0x68E31DA9u) % 2u == 0u) lr = 1;
+                    float fx = xf - (float)lr * 0.3f * s->part_vy[i];
+                    float fy = yf + (float)lr * 0.3f * s->part_vx[i];
+                    int gx = ora_f2i(fx + 0.5f), gy = ora_f2i(fy + 0.5f);
+                    if (gx >= 0 && gx < xres && gy >= 0 && gy < yres) {
+                        uint32_t g = s->pmap[gy * xres + gx];
+                        uint32_t gslot = (g >> 9) - 1u;
+                        if (g >= 512u && (g & 0x1FFu) == PT_GLOW && gslot < s->slot_bound) {
+                            int bin = ora_f2i((s->part_temp[gslot] - 273.0f) * 0.25f);
+                            if (bin < 0) bin = 0;
+                            if (bin > 25) bin = 25;
+                            queue_photon(q, i, gx, gy, 0x1Fu << (uint32_t)bin, s->part_temp[gslot], s->part_vx[i],
+                                         s->part_vy[i]);
+                        }
+                    }
+                }
+            } else if (rty == PT_FILT) { /* :1273-1275 */
+                s->part_ctype[i] = oracle_interact_wavelengths(s->part_tmp[r], s->part_ctype[r], s->part_temp[r],
+                                                               s->part_ctype[i]);
+            } else if (rty == PT_C5) {
+                uint32_t rc = s->part_ctype[r], ic = s->part_ctype[i];
+                if (s->part_life[r] > 0 && (rc & ic & 0xFFFFFFC0u) != 0u) { /* :1277-1303 */
+                    int32_t rt = s->part_tmp[r];
+                    float cvx = (float)((int32_t)((uint32_t)rt << 16) >> 16) / 255.0f;
+                    float cvy = (float)(rt >> 16) / 255.0f;
+                    float ovx = s->part_vx[i], ovy = s->part_vy[i];
+                    if (ovx + cvx == 0.0f && ovy + cvy == 0.0f) {
+                        s->part_vx[i] = cvy;
+                        s->part_vy[i] = -cvx;
+                    } else {
+                        s->part_ctype[i] = (rc & ic) >> 6;
+                        s->part_vx[i] = ovx + cvx;
+                        s->part_vy[i] = ovy + cvy;
+                        float speed = s->part_vx[i] * s->part_vx[i] + s->part_vy[i] * s->part_vy[i];
+                        float scale = sqrtf((ovx * ovx + ovy * ovy) / speed);
+                        s->part_vx[i] *= scale;
+                        s->part_vy[i] *= scale;
+                    }
+                    s->part_life[r] = 0;
+                    s->part_ctype[r] = 0u;
+                } else if (rc == 0u && (ic & 0xFFFFFFC0u) != 0u) { /* :1304-1314 */
+                    s->part_life[r] = 1;
+                    s->part_ctype[r] = ic;
+                    s->part_tmp[r] = (int32_t)((0xFFFFu & (uint32_t)ora_f2i(s->part_vx[i] * 255.0f)) |
+                                               (0xFFFF0000u & (uint32_t)ora_f2i(s->part_vy[i] * 16711680.0f)));
+                    s->part_tmp2[r] = (int32_t)((0xFFFFu & (uint32_t)ora_f2i((xf - (float)xi) * 255.0f)) |
+                                                (0xFFFF0000u & (uint32_t)ora_f2i((yf - (float)yi) * 16711680.0f)));
+                    s->part_ty[i] = 0u;
+                }
+            } else if (rty == PT_INVIS) { /* :1315-1326 */
+                float resistance = 4.0f;
+                if (s->part_tmp[r] > 0) resistance = (float)s->part_tmp[r];
+                uint32_t px = dest % s->xres, py = dest / s->xres;
+                float pv = s->air_pv[(py / 4u) * xc + px / 4u];
+                if (pv >= -resistance && pv <= resistance) {
+                    s->part_ty[i] = PT_NEUT;
+                    s->part_ctype[i] = 0u;
+                }
+            } else if (rty == PT_BIZR || rty == PT_BIZRG || rty == PT_BIZRS) { /* :1326-1332 */
+                s->part_ty[i] = PT_ELEC;
+                s->part_ctype[i] = 0u;
+            } else if (rty == PT_H2) { /* :1333-1343 */
+                if ((s->part_tmp[i] & 1) == 0) {
+                    s->part_ty[i] = PT_PROT;
+                    s->part_ctype[i] = 0u;
+                    s->part_tmp2[i] = 1;
+                    final_ty = PT_ELEC;
+                    s->part_x[r] = (float)xi;
+                    s->part_y[r] = (float)yi;
+                    s->intent_target[r] = NO_INTENT;
+                    /* create_part(ID(r), x, y, PT_ELEC): DefaultProperties, then
+                     * ELEC's Create hook (ELEC.cpp:159-165). */
+                    s->part_temp[r] = ora_bits_f(def_elec[0]);
+                    s->part_life[r] = 680;
+                    s->part_ctype[r] = def_elec[2];
+                    s->part_tmp[r] = (int32_t)def_elec[3];
+                    s->part_tmp2[r] = (int32_t)def_elec[4];
+                    uint32_t roll = oracle_hash_u32(frame_seed ^ ((r + 1u) << 7) ^ (PT_ELEC << 3));
+                    float a = (float)(roll % 360u) / 360.0f * 6.28318530717958647692f;
+                    s->part_vx[r] = 2.0f * (float)cos((double)a);
+                    s->part_vy[r] = 2.0f * (float)sin((double)a);
+                }
+            } else if (rty == PT_GPMP && s->part_life[r] == 0) { /* :1344-1351 */
+                s->part_ty[i] = PT_GRVT;
+                s->part_tmp[i] = ora_f2i(s->part_temp[r] - 273.15f);
+            }
+        } else if (ty == PT_NEUT) { /* :1353-1358 */
+            if ((rty == PT_GLAS || rty == PT_BGLA) && seed % 10u == 0u) {
+                uint32_t g = s->pmap[own];
+                uint32_t gty = g & 0x1FFu, gslot = (g >> 9) - 1u;
+                if (g >= 512u && (gty == PT_GLAS || gty == PT_BGLA) && gslot < s->slot_bound) {
+                    float vx = s->part_vx[i], vy = s->part_vy[i];
+                    if (vx * vx + vy * vy >= 1.44f * 1.44f) {
+                        /* cherenkov_photon (reaction_kernel/neut/mod.rs:335-369) */
+                        uint32_t roll = oracle_hash_u32(seed ^ 0x9E3779B9u);
+                        int lr = (int)(roll % 2u) * 2 - 1;
+                        float rvx = vx - (float)lr * 2.5f * vy;
+                        float rvy = vy + (float)lr * 2.5f * vx;
+                        float speed = sqrtf(rvx * rvx + rvy * rvy);
+                        float scale = 1.269f / speed;
+                        queue_photon(q, i, xi, yi, 0x00000F80u, s->part_temp[gslot], rvx * scale, rvy * scale);
+                    }
+                }
+            }
+        } else if (ty == PT_ELEC) { /* :1362-1368 */
+            if (rty == PT_GLOW) {
+                s->part_ty[i] = PT_PHOT;
+                s->part_ctype[i] = 0x3FFFFFFFu;
+            }
+        } else if (ty == PT_PROT) { /* :1370-1374 */
+            if (rty == PT_INVIS) s->part_ty[i] = PT_NEUT;
+        } else if (rty == PT_FILT) { /* BIZR / BIZRG + FILT, :1375-1379 */
+            s->part_ctype[i] = oracle_interact_wavelengths(s->part_tmp[r], s->part_ctype[r], s->part_temp[r],
+                                                           s->part_ctype[i]);
+        }
+        s->part_ty[r] = final_ty;
+    }
+}
+
+/* drain_pending_creates for the pair pass's photons (create_alloc_kernel +
+ * create_fields_kernel, reaction_kernel/shared.rs:516-912) with the record
+ * mask both creators use (CR_NO_OCCUPANCY | CR_SET_CTYPE | CR_SET_TEMP |
+ * CR_SET_VX | CR_SET_VY): no occupancy test; life and tmp..tmp4 from PHOT's
+ * DefaultProperties, dcolour and flags cleared.  The map words the
+ * reference writes are rebuilt by the scatter that follows. */
+static void drain_creates(oracle_state *s, const pair_queue *q) {
+    const uint32_t *def = s->spawn_defaults + PT_PHOT * 8u;
+    uint32_t cursor = 0;
+    for (uint32_t k = 0; k < q->n; ++k) {
+        const create_rec *r = &q->rec[k];
+        if (r->x >= s->xres || r->y >= s->yres) continue;
+        uint32_t a = cursor++;
+        if (a >= q->n_free) continue;
+        uint32_t slot = q->free_slots[a];
+        s->part_ty[slot] = PT_PHOT;
+        s->part_x[slot] = (float)r->x;
+        s->part_y[slot] = (float)r->y;
+        s->part_flags[slot] = 0u;
+        s->part_life[slot] = (int32_t)def[1];
+        s->part_ctype[slot] = r->ctype;
+        s->part_temp[slot] = r->temp;
+        s->part_tmp[slot] = (int32_t)def[3];
+        s->part_tmp2[slot] = (int32_t)def[4];
+        s->part_tmp3[slot] = (int32_t)def[5];
+        s->part_tmp4[slot] = (int32_t)def[6];
+        s->part_dcolour[slot] = 0u;
+        s->part_vx[slot] = r->vx;
+        s->part_vy[slot] = r->vy;
+    }
+}
+
+/* step_movement_gpu (mod.rs:1176-1227). */
 void oracle_step_movement(oracle_state *s, uint32_t seed) {
+    /* gpu_spawn_used (mod.rs:1212-1216): the bound covers the window. */
+    uint32_t spawn_end = s->capacity < SPAWN_END ? s->capacity : SPAWN_END;
+    if (s->slot_bound < spawn_end) s->slot_bound = spawn_end;
     oracle_air_deposit(s);
     oracle_detect_charge(s);
     oracle_emap_closure(s);
     oracle_velocity_intent(s, seed);
+    pair_queue q = {calloc(CREATE_COUNT, sizeof(create_rec)), 0, CREATE_COUNT,
+                    calloc(SPAWN_END - SPAWN_START, sizeof(uint32_t)), 0};
+    collect_free(s, &q);
+    pair_resolve(s, seed, &q);
     oracle_commit(s);
+    drain_creates(s, &q);
+    free(q.rec);
+    free(q.free_slots);
     oracle_rebuild_maps(s);
 }
 

@@ -53,7 +53,7 @@ ARRAYS = [
     ("grav_mass_prev", np.int32, "cell"), ("grav_fx", np.float32, "cell"),
     ("grav_fy", np.float32, "cell"),
     ("elem_props", np.uint32, "props"), ("move_lut", np.uint32, "lut"),
-    ("photon_mask", np.uint32, "table"),
+    ("photon_mask", np.uint32, "table"), ("spawn_defaults", np.uint32, "table"),
 ]
 _PTR = {np.uint32: _u32p, np.int32: _i32p, np.float32: _f32p}
 
@@ -93,6 +93,10 @@ def lib():
         _lib.oracle_recalc_free_particles.restype = None
         _lib.oracle_pcg_hash.argtypes = [ctypes.c_uint32]
         _lib.oracle_pcg_hash.restype = ctypes.c_uint32
+        _lib.oracle_hash_u32.argtypes = [ctypes.c_uint32]
+        _lib.oracle_hash_u32.restype = ctypes.c_uint32
+        _lib.oracle_interact_wavelengths.argtypes = [ctypes.c_int32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32]
+        _lib.oracle_interact_wavelengths.restype = ctypes.c_uint32
         for name, extra in (("oracle_air_deposit", []), ("oracle_detect_charge", []), ("oracle_emap_closure", []),
                             ("oracle_velocity_intent", [ctypes.c_uint32]),
                             ("oracle_commit", []), ("oracle_rebuild_maps", []),
@@ -164,7 +168,9 @@ class OracleState:
 
     def call(self, fname, *args):
         self.c.slot_bound = self.state.slot_bound
-        return getattr(lib(), fname)(ctypes.byref(self.c), *args)
+        out = getattr(lib(), fname)(ctypes.byref(self.c), *args)
+        self.state.slot_bound = self.c.slot_bound  # a movement pass may raise it
+        return out
 
 
 def gravity_solve_f64(state):

@@ -51,6 +51,7 @@ typedef struct oracle_state {
     const uint32_t *elem_props;   /* 256 x 32 */
     const uint32_t *move_lut;     /* 256 x 12 + 256 x 256 */
     const uint32_t *photon_mask;  /* 256 */
+    const uint32_t *spawn_defaults; /* 256 x 8 (to_gpu_spawn_defaults_u32) */
 } oracle_state;
 
 /* air_kernel::step_air_gpu (air_kernel/mod.rs:120-299). */
@@ -101,6 +102,11 @@ void oracle_velocity_intent(oracle_state *s, uint32_t seed);
 void oracle_commit(oracle_state *s);
 void oracle_rebuild_maps(oracle_state *s);
 void oracle_step_movement(oracle_state *s, uint32_t seed);
+/* hash_u32 (reaction_kernel/shared.rs:1015-1023) and
+ * Element_FILT_interactWavelengths (reaction_kernel/rays/emitter.rs:28-80),
+ * used by the e == 2 pair pass; exposed for known-answer tests. */
+uint32_t oracle_hash_u32(uint32_t x);
+uint32_t oracle_interact_wavelengths(int32_t filt_tmp, uint32_t filt_ctype, float filt_temp, uint32_t orig_wl);
 
 /* SimulationEngine::step (lib.rs:303-459) over the passes above. */
 void oracle_step(oracle_state *s, uint64_t frame_count);

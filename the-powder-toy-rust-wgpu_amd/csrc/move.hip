@@ -26,8 +26,10 @@
 //   mv_detect_* / mv_emap_*  WL_DETECT charging and the set_emap closure
 //                            (mod.rs:348-828), on canvases with wire walls.
 //
-// Not built: the e == 2 pair pass (reaction_kernel/energy_entry.rs), which
-// needs the reaction creation queue.
+//   mv_pair                  the e == 2 pair pass (reaction_kernel/energy_entry.rs)
+//                            over the tracked movers the intent pass listed.
+//   mv_drain                 the photons the pair pass creates, allocated
+//                            from the device window (reaction_kernel/shared.rs).
 
 #include "tpt_internal.h"
 
@@ -44,6 +46,15 @@ constexpr float MAX_VELOCITY = 1.0e4f;
 constexpr uint32_t MAX_PLAN_STEPS = 5001u;
 constexpr uint32_t EDGE_VOID = 0u, EDGE_LOOP = 2u;
 constexpr uint32_t LUT_CAN = 256u * 12u;
+
+// Element ids the e == 2 pair pass names (tpt-core/src/constants.rs:63-424).
+constexpr uint32_t PT_GLAS = 24u, PT_PHOT = 29u, PT_NEUT = 30u, PT_ELEC = 31u, PT_H2 = 32u, PT_GRVT = 39u,
+                   PT_GPMP = 43u, PT_GLOW = 66u, PT_BIZR = 85u, PT_BIZRG = 86u, PT_BIZRS = 87u, PT_BGLA = 95u,
+                   PT_INVIS = 124u, PT_FILT = 134u, PT_C5 = 139u, PT_PROT = 173u;
+
+__device__ __forceinline__ bool pair_tracked(uint32_t ty) {
+    return ty == PT_PHOT || ty == PT_NEUT || ty == PT_ELEC || ty == PT_PROT || ty == PT_BIZR || ty == PT_BIZRG;
+}
 
 struct MoveEnv {
     const uint32_t *__restrict__ pmap;
@@ -62,6 +73,8 @@ struct MoveEnv {
     uint32_t *__restrict__ saw_list;
     uint32_t *__restrict__ saw_count;
     uint32_t *__restrict__ ctype;  // photon wavelength masks, narrowed by refraction
+    uint32_t *__restrict__ pair_list;   // movers the e == 2 pair pass examines
+    uint32_t *__restrict__ counters;
 };
 
 __device__ __forceinline__ uint32_t pcg_hash(uint32_t seed) {
@@ -1036,7 +1049,11 @@ __global__ void mv_type_map(const uint4 *__restrict__ pmap, uint32_t *__restrict
 
 __global__ void mv_clear(uint4 *__restrict__ winner, uint32_t n4, uint32_t *__restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) counters[1] = 0u;
+    if (i == 0) {
+        counters[TPT_CTR_SAW] = 0u;
+        counters[TPT_CTR_PAIR] = 0u;
+        counters[TPT_CTR_CREATE] = 0u;
+    }
     if (i < n4) winner[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
@@ -1104,15 +1121,361 @@ __global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, const float
         if ((target & 0x80000000u) == 0u) {
             const uint32_t key = ((rank & 0x1FFu) << 23) | ((i + 1u) & 0x7FFFFFu);
             atomicMax(&winner[target & PIXEL_MASK], key);
+            if (pair_tracked(ty)) e.pair_list[atomicAdd(&e.counters[TPT_CTR_PAIR], 1u)] = i;
         }
     }
     s.vx[i] = vx;
     s.vy[i] = vy;
 }
 
-__global__ void mv_saw(uint32_t *__restrict__ ty, const uint32_t *__restrict__ list, const uint32_t *__restrict__ counters) {
-    const uint32_t n = counters[1];
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) ty[list[k]] = 180u;
+// Deferred WOOD -> SAWD of rejected probes, and the free bitmap of the
+// device allocation window for the pair pass's creations: a window slot is
+// free when empty and not portal-held (collect_gpu_free_slots_kernel,
+// reaction_kernel/shared.rs:1197-1222), sampled here, after the intent pass
+// (which kills out-of-bounds particles) and before the pair pass, as
+// begin_pending_creates is (move_kernel/mod.rs:1203).  SAWD and WOOD are
+// both non-empty, so the two halves do not interact.  Launched as
+// TPT_SPAWN_BITMAP_WORDS * 32 / BLK blocks of BLK threads.
+__global__ __launch_bounds__(BLK) void mv_saw(uint32_t *__restrict__ ty, const uint32_t *__restrict__ flags,
+                                              const uint32_t *__restrict__ list,
+                                              const uint32_t *__restrict__ counters,
+                                              uint32_t *__restrict__ spawn_free, uint32_t capacity) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t slot = TPT_GPU_SPAWN_START + t;
+    const bool free_slot = slot < capacity && ty[slot] == 0u && (flags[slot] & 0x04u) == 0u;
+    const uint64_t bits = __ballot(free_slot);
+    if ((threadIdx.x & 63u) == 0u) {
+        spawn_free[t >> 5] = (uint32_t)bits;
+        spawn_free[(t >> 5) + 1u] = (uint32_t)(bits >> 32);
+    }
+    const uint32_t n = counters[TPT_CTR_SAW];
+    for (uint32_t k = t; k < n; k += gridDim.x * blockDim.x) ty[list[k]] = 180u;
+}
+
+// ---- e == 2 pair pass (reaction_kernel/energy_entry.rs:90-393) ------------
+//
+// Canonical form (the reference resolves pairs concurrently):
+//  * every mover reads its own type as the intent pass left it: the
+//    reference's transient claim of the occupant's type (PAIR_CLAIMED) is
+//    not modelled, since each occupant has at most one mover (one winner per
+//    destination pixel, one pixel per occupant);
+//  * creation records are allocated in ascending requester slot order, the
+//    k-th record taking the k-th free slot of the window in ascending order;
+//    at most TPT_GPU_CREATE_COUNT records, the lowest requester slots.
+// The arms touch disjoint data otherwise, so no other order matters.
+
+// hash_u32 (reaction_kernel/shared.rs:1015-1023): MurmurHash3's fmix32.
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 2246822507u;
+    h ^= h >> 13;
+    h *= 3266489909u;
+    h ^= h >> 16;
+    return h;
+}
+
+// Element_FILT_getWavelengths (reaction_kernel/rays/emitter.rs:10-26).
+__device__ __forceinline__ uint32_t filt_wavelengths(uint32_t ctype, float temp) {
+    if ((ctype & 0x3FFFFFFFu) != 0u) return ctype & 0x3FFFFFFFu;
+    int bin = tpt_f2i((temp - 273.0f) * 0.025f);
+    bin = bin < 0 ? 0 : (bin > 25 ? 25 : bin);
+    return 0x1Fu << bin;
+}
+
+// Element_FILT_interactWavelengths (reaction_kernel/rays/emitter.rs:28-80).
+// Shift counts use their low five bits, as the reference's u32 shifts do.
+__device__ uint32_t interact_wavelengths(int32_t filt_tmp, uint32_t filt_ctype, float filt_temp, uint32_t wl) {
+    const uint32_t mask = 0x3FFFFFFFu;
+    const uint32_t fw = filt_wavelengths(filt_ctype, filt_temp);
+    switch (filt_tmp) {
+    case 0: return fw;
+    case 1: return wl & fw;
+    case 2: return wl | fw;
+    case 3: return wl & ~fw;
+    case 4:
+    case 5: {
+        int shift = tpt_f2i((filt_temp - 273.0f) * 0.025f);
+        if (shift <= 0) shift = 1;
+        return (filt_tmp == 4 ? wl << ((uint32_t)shift & 31u) : wl >> ((uint32_t)shift & 31u)) & mask;
+    }
+    case 6: return wl;
+    case 7: return wl ^ fw;
+    case 8: return ~wl & mask;
+    case 10: return (wl * (fw & (~fw + 1u))) & mask;
+    case 11: {
+        const uint32_t lsb = fw & (~fw + 1u);
+        return lsb != 0u ? (wl / lsb) & mask : wl;
+    }
+    default: return fw;
+    }
+}
+
+struct PairBufs {
+    uint32_t *ctype;
+    int32_t *life, *tmp2;
+    float *temp;
+    const float *air_pv;
+    const uint32_t *defaults;  // spawn defaults, TPT_SPAWN_DEFAULT_WORDS per element
+    uint32_t *rec;             // creation records
+    uint32_t rec_cap;
+};
+
+__device__ __forceinline__ void queue_photon(const PairBufs &pb, uint32_t *counters, uint32_t slot, int x, int y,
+                                             uint32_t ctype, float temp, float vx, float vy) {
+    const uint32_t k = atomicAdd(&counters[TPT_CTR_CREATE], 1u);
+    if (k >= pb.rec_cap) return;
+    uint32_t *r = pb.rec + (size_t)k * TPT_CREATE_REC_WORDS;
+    r[0] = slot;
+    r[1] = (uint32_t)x | ((uint32_t)y << 16);
+    r[2] = ctype;
+    r[3] = __float_as_uint(temp);
+    r[4] = __float_as_uint(vx);
+    r[5] = __float_as_uint(vy);
+}
+
+// pair_resolve_kernel's body for mover i (energy_entry.rs:111-390).
+__device__ void pair_one(Slots s, const PairBufs &pb, const uint32_t *__restrict__ pmap,
+                         const uint32_t *__restrict__ winner, uint32_t *counters, uint32_t i, uint32_t slots,
+                         int xres, int yres, int xc, uint32_t frame_seed) {
+    const uint32_t ty = s.ty[i];
+    if (ty == 0u || ty >= 256u || !pair_tracked(ty)) return;
+    const uint32_t raw = s.intent[i];
+    if (raw == NO_INTENT || (raw & 0x80000000u) != 0u) return;
+    const uint32_t dest = raw & PIXEL_MASK;
+    if (dest >= (uint32_t)(xres * yres)) return;
+    const float xf = s.x[i], yf = s.y[i];
+    const int xi = tpt_f2i(xf + 0.5f), yi = tpt_f2i(yf + 0.5f);
+    const uint32_t own = (uint32_t)(yi * xres + xi);
+    if (own == dest) return;
+    const uint32_t m = pmap[dest];
+    if (m < 512u) return;
+    const uint32_t rty = m & 0x1FFu, r = (m >> 9) - 1u;
+    if (r >= slots) return;
+    // The winner test masks 27 bits (energy_entry.rs:171-175), so it also
+    // requires the low four bits of the mover's weight rank to be zero.
+    if ((winner[dest] & 0x07FFFFFFu) != ((i + 1u) & 0x07FFFFFFu)) return;
+    // The occupant claim (a CAS from the map's type, energy_entry.rs:181-185)
+    // fails on a slot the intent pass killed or a WOOD it sawed.  Matter
+    // types change in this pass only through the H2 arm, on the thread's
+    // own occupant, so the read is stable.
+    if (s.ty[r] != rty) return;
+    const uint32_t seed = fmix32(frame_seed ^ fmix32(i));
+    uint32_t final_ty = rty;
+    if (ty == PT_PHOT) {
+        if (rty == PT_GLOW) {
+            if (pb.life[r] == 0 && seed % 30u == 0u) {
+                pb.life[r] = 120;
+                int lr = -1;
+                if (fmix32(seed ^ 0x68E31DA9u) % 2u == 0u) lr = 1;
+                const float fx = xf - (float)lr * 0.3f * s.vy[i];
+                const float fy = yf + (float)lr * 0.3f * s.vx[i];
+                const int gx = tpt_f2i(fx + 0.5f), gy = tpt_f2i(fy + 0.5f);
+                if (gx >= 0 && gx < xres && gy >= 0 && gy < yres) {
+                    const uint32_t g = pmap[gy * xres + gx];
+                    const uint32_t gslot = (g >> 9) - 1u;
+                    if (g >= 512u && (g & 0x1FFu) == PT_GLOW && gslot < slots) {
+                        int bin = tpt_f2i((pb.temp[gslot] - 273.0f) * 0.25f);
+                        bin = bin < 0 ? 0 : (bin > 25 ? 25 : bin);
+                        queue_photon(pb, counters, i, gx, gy, 0x1Fu << bin, pb.temp[gslot], s.vx[i], s.vy[i]);
+                    }
+                }
+            }
+        } else if (rty == PT_FILT) {
+            pb.ctype[i] = interact_wavelengths(s.tmp[r], pb.ctype[r], pb.temp[r], pb.ctype[i]);
+        } else if (rty == PT_C5) {
+            const uint32_t rc = pb.ctype[r], ic = pb.ctype[i];
+            if (pb.life[r] > 0 && (rc & ic & 0xFFFFFFC0u) != 0u) {
+                const int32_t rt = s.tmp[r];
+                const float cvx = (float)((int32_t)((uint32_t)rt << 16) >> 16) / 255.0f;
+                const float cvy = (float)(rt >> 16) / 255.0f;
+                const float ovx = s.vx[i], ovy = s.vy[i];
+                if (ovx + cvx == 0.0f && ovy + cvy == 0.0f) {
+                    s.vx[i] = cvy;
+                    s.vy[i] = -cvx;
+                } else {
+                    pb.ctype[i] = (rc & ic) >> 6;
+                    const float nvx = ovx + cvx, nvy = ovy + cvy;
+                    const float speed = nvx * nvx + nvy * nvy;
+                    const float scale = sqrtf((ovx * ovx + ovy * ovy) / speed);
+                    s.vx[i] = nvx * scale;
+                    s.vy[i] = nvy * scale;
+                }
+                pb.life[r] = 0;
+                pb.ctype[r] = 0u;
+            } else if (rc == 0u && (ic & 0xFFFFFFC0u) != 0u) {
+                const float vx = s.vx[i], vy = s.vy[i];
+                pb.life[r] = 1;
+                pb.ctype[r] = ic;
+                s.tmp[r] = (int32_t)((0xFFFFu & (uint32_t)tpt_f2i(vx * 255.0f)) |
+                                     (0xFFFF0000u & (uint32_t)tpt_f2i(vy * 16711680.0f)));
+                pb.tmp2[r] = (int32_t)((0xFFFFu & (uint32_t)tpt_f2i((xf - (float)xi) * 255.0f)) |
+                                       (0xFFFF0000u & (uint32_t)tpt_f2i((yf - (float)yi) * 16711680.0f)));
+                s.ty[i] = 0u;
+            }
+        } else if (rty == PT_INVIS) {
+            float resistance = 4.0f;
+            if (s.tmp[r] > 0) resistance = (float)s.tmp[r];
+            const uint32_t px = dest % (uint32_t)xres, py = dest / (uint32_t)xres;
+            const float pv = pb.air_pv[(py / 4u) * (uint32_t)xc + px / 4u];
+            if (pv >= -resistance && pv <= resistance) {
+                s.ty[i] = PT_NEUT;
+                pb.ctype[i] = 0u;
+            }
+        } else if (rty == PT_BIZR || rty == PT_BIZRG || rty == PT_BIZRS) {
+            s.ty[i] = PT_ELEC;
+            pb.ctype[i] = 0u;
+        } else if (rty == PT_H2) {
+            if ((s.tmp[i] & 1) == 0) {
+                s.ty[i] = PT_PROT;
+                pb.ctype[i] = 0u;
+                pb.tmp2[i] = 1;
+                final_ty = PT_ELEC;
+                s.x[r] = (float)xi;
+                s.y[r] = (float)yi;
+                s.intent[r] = NO_INTENT;
+                // create_part(ID(r), x, y, PT_ELEC): DefaultProperties, then
+                // ELEC's Create hook (life 680, speed 2 at a hashed angle).
+                const uint32_t *d = pb.defaults + PT_ELEC * TPT_SPAWN_DEFAULT_WORDS;
+                pb.temp[r] = __uint_as_float(d[0]);
+                pb.life[r] = 680;
+                pb.ctype[r] = d[2];
+                s.tmp[r] = (int32_t)d[3];
+                pb.tmp2[r] = (int32_t)d[4];
+                const uint32_t roll = fmix32(frame_seed ^ ((r + 1u) << 7) ^ (PT_ELEC << 3));
+                const float a = (float)(roll % 360u) / 360.0f * 6.28318530717958647692f;
+                // Double-precision cos/sin rounded to float: the same value
+                // on every IEEE platform for these 360 angles.
+                s.vx[r] = 2.0f * (float)cos((double)a);
+                s.vy[r] = 2.0f * (float)sin((double)a);
+            }
+        } else if (rty == PT_GPMP && pb.life[r] == 0) {
+            s.ty[i] = PT_GRVT;
+            s.tmp[i] = tpt_f2i(pb.temp[r] - 273.15f);
+        }
+    } else if (ty == PT_NEUT) {
+        if ((rty == PT_GLAS || rty == PT_BGLA) && seed % 10u == 0u) {
+            const uint32_t g = pmap[own];
+            const uint32_t gty = g & 0x1FFu, gslot = (g >> 9) - 1u;
+            if (g >= 512u && (gty == PT_GLAS || gty == PT_BGLA) && gslot < slots) {
+                const float vx = s.vx[i], vy = s.vy[i];
+                if (vx * vx + vy * vy >= 1.44f * 1.44f) {
+                    // cherenkov_photon (reaction_kernel/neut/mod.rs:335-369).
+                    const uint32_t roll = fmix32(seed ^ 0x9E3779B9u);
+                    const int lr = (int)(roll % 2u) * 2 - 1;
+                    const float rvx = vx - (float)lr * 2.5f * vy;
+                    const float rvy = vy + (float)lr * 2.5f * vx;
+                    const float speed = sqrtf(rvx * rvx + rvy * rvy);
+                    const float scale = 1.269f / speed;
+                    queue_photon(pb, counters, i, xi, yi, 0x00000F80u, pb.temp[gslot], rvx * scale, rvy * scale);
+                }
+            }
+        }
+    } else if (ty == PT_ELEC) {
+        if (rty == PT_GLOW) {
+            s.ty[i] = PT_PHOT;
+            pb.ctype[i] = 0x3FFFFFFFu;
+        }
+    } else if (ty == PT_PROT) {
+        if (rty == PT_INVIS) s.ty[i] = PT_NEUT;
+    } else if (rty == PT_FILT) {  // BIZR / BIZRG
+        pb.ctype[i] = interact_wavelengths(s.tmp[r], pb.ctype[r], pb.temp[r], pb.ctype[i]);
+    }
+    if (final_ty != rty) s.ty[r] = final_ty;
+}
+
+__global__ __launch_bounds__(BLK) void mv_pair(Slots s, PairBufs pb, const uint32_t *__restrict__ pmap,
+                                               const uint32_t *__restrict__ winner, const uint32_t *__restrict__ list,
+                                               uint32_t *__restrict__ counters, uint32_t slots, int xres, int yres,
+                                               int xc, const uint32_t *__restrict__ seedp) {
+    const uint32_t n = counters[TPT_CTR_PAIR];
+    if (n == 0u) return;
+    const uint32_t frame_seed = *seedp;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        pair_one(s, pb, pmap, winner, counters, list[k], slots, xres, yres, xc, frame_seed);
+}
+
+struct DrainBufs {
+    int32_t *life, *tmp2, *tmp3, *tmp4;
+    uint32_t *ctype, *dcolour;
+    float *temp;
+    const uint32_t *defaults;
+    const uint32_t *rec;
+    const uint32_t *spawn_free;
+};
+
+constexpr int DRAIN_BLK = 1024;
+constexpr uint32_t DRAIN_CHUNK = 4096u;
+
+// drain_pending_creates for the pair pass's photons (create_alloc_kernel +
+// create_fields_kernel, reaction_kernel/shared.rs:516-912, with the record
+// mask CR_NO_OCCUPANCY | CR_SET_CTYPE | CR_SET_TEMP | CR_SET_VX | CR_SET_VY
+// both creators use): every record takes a slot from the window without an
+// occupancy test; life, tmp..tmp4 come from PHOT's DefaultProperties,
+// dcolour and flags are cleared.  The map words the reference writes here
+// are rebuilt by the scatter that follows, and the active-element mask it
+// updates has no reader in this library.  One workgroup: a record's
+// allocation index is its rank by requester slot (slots are unique).
+__global__ __launch_bounds__(DRAIN_BLK) void mv_drain(Slots s, DrainBufs d, const uint32_t *__restrict__ counters,
+                                                      uint32_t rec_cap, int xres, int yres) {
+    __shared__ uint32_t keys[DRAIN_CHUNK];
+    __shared__ uint32_t prefix[TPT_SPAWN_BITMAP_WORDS + 1];
+    const uint32_t total = counters[TPT_CTR_CREATE];
+    const uint32_t n = total < rec_cap ? total : rec_cap;
+    if (n == 0u) return;
+    const uint32_t tid = threadIdx.x;
+    // Exclusive prefix popcount of the free bitmap (sequential over 512
+    // words by one thread: negligible next to the rank loop).
+    if (tid == 0u) {
+        uint32_t acc = 0u;
+        for (uint32_t w = 0; w < TPT_SPAWN_BITMAP_WORDS; ++w) {
+            prefix[w] = acc;
+            acc += __popc(d.spawn_free[w]);
+        }
+        prefix[TPT_SPAWN_BITMAP_WORDS] = acc;
+    }
+    __syncthreads();
+    const uint32_t avail = prefix[TPT_SPAWN_BITMAP_WORDS];
+    for (uint32_t base = 0; base < n; base += DRAIN_BLK) {
+        const uint32_t k = base + tid;
+        const uint32_t mine = k < n ? d.rec[(size_t)k * TPT_CREATE_REC_WORDS] : 0xFFFFFFFFu;
+        uint32_t rank = 0u;
+        for (uint32_t c0 = 0; c0 < n; c0 += DRAIN_CHUNK) {
+            const uint32_t len = n - c0 < DRAIN_CHUNK ? n - c0 : DRAIN_CHUNK;
+            __syncthreads();
+            for (uint32_t j = tid; j < len; j += DRAIN_BLK) keys[j] = d.rec[(size_t)(c0 + j) * TPT_CREATE_REC_WORDS];
+            __syncthreads();
+            for (uint32_t j = 0; j < len; ++j) rank += keys[j] < mine ? 1u : 0u;
+        }
+        if (k >= n || rank >= TPT_GPU_CREATE_COUNT || rank >= avail) continue;
+        const uint32_t *r = d.rec + (size_t)k * TPT_CREATE_REC_WORDS;
+        const int x = (int)(r[1] & 0xFFFFu), y = (int)(r[1] >> 16);
+        if (x >= xres || y >= yres) continue;
+        // The rank-th set bit of the bitmap, in ascending slot order.
+        uint32_t lo = 0u, hi = TPT_SPAWN_BITMAP_WORDS - 1u;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1u) >> 1;
+            if (prefix[mid] <= rank) lo = mid;
+            else hi = mid - 1u;
+        }
+        uint32_t word = d.spawn_free[lo], skip = rank - prefix[lo];
+        for (; skip > 0u; --skip) word &= word - 1u;
+        const uint32_t slot = TPT_GPU_SPAWN_START + lo * 32u + (uint32_t)__ffs(word) - 1u;
+        const uint32_t *def = d.defaults + PT_PHOT * TPT_SPAWN_DEFAULT_WORDS;
+        s.ty[slot] = PT_PHOT;
+        s.x[slot] = (float)x;
+        s.y[slot] = (float)y;
+        s.flags[slot] = 0u;
+        d.life[slot] = (int32_t)def[1];
+        d.ctype[slot] = r[2];
+        d.temp[slot] = __uint_as_float(r[3]);
+        s.tmp[slot] = (int32_t)def[3];
+        d.tmp2[slot] = (int32_t)def[4];
+        d.tmp3[slot] = (int32_t)def[5];
+        d.tmp4[slot] = (int32_t)def[6];
+        d.dcolour[slot] = 0u;
+        s.vx[slot] = __uint_as_float(r[4]);
+        s.vy[slot] = __uint_as_float(r[5]);
+    }
 }
 
 struct CommitBufs {
@@ -1349,6 +1712,12 @@ int tpt_movement_step(tpt_ctx *ctx) {
     const uint32_t *seed = ctx->counters + TPT_CTR_SEED;
     ctx->maps_fresh = ctx->rebuild_fresh = false;
     const tpt_config &c = ctx->cfg;
+    // The pair pass arms the creation queue, after which the slot bound
+    // covers the device allocation window (gpu_spawn_used,
+    // move_kernel/mod.rs:1212-1216).  The added slots are empty until the
+    // drain fills them.
+    const uint32_t spawn_end = c.capacity < TPT_GPU_SPAWN_END ? c.capacity : TPT_GPU_SPAWN_END;
+    if (ctx->slot_bound < spawn_end) ctx->slot_bound = spawn_end;
     const uint32_t slots = ctx->slot_bound, P = ctx->pixels;
     const int xres = (int)c.xres, yres = (int)c.yres, xc = (int)ctx->xcells;
     Slots s{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<float>(ctx, TPT_BUF_PART_X),
@@ -1401,14 +1770,24 @@ int tpt_movement_step(tpt_ctx *ctx) {
     hipLaunchKernelGGL(mv_type_map, pg, b, 0, ctx->stream, (const uint4 *)pmap, (uint32_t *)ctx->pix_type, n4);
     tpt_pass_end(ctx, PASS_MOVE_CLEAR);
     MoveEnv e{pmap, ctx->pix_type, wall, emap, lut, xres, yres, xc, c.edge_mode, c.gravity_mode,
-              c.custom_gravity_x, c.custom_gravity_y, c.newton_grav, ctx->slot_scratch_u2, ctx->counters + 1,
-              tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE)};
+              c.custom_gravity_x, c.custom_gravity_y, c.newton_grav, ctx->slot_scratch_u2, ctx->counters + TPT_CTR_SAW,
+              tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE), ctx->slot_scratch_u, ctx->counters};
     tpt_pass_begin(ctx, PASS_MOVE_INTENT);
     hipLaunchKernelGGL(mv_intent, sg, b, 0, ctx->stream, s, e, air_vx, air_vy, winner, slots, seed);
     tpt_pass_end(ctx, PASS_MOVE_INTENT);
     tpt_pass_begin(ctx, PASS_MOVE_SAW);
-    hipLaunchKernelGGL(mv_saw, dim3(16), b, 0, ctx->stream, s.ty, ctx->slot_scratch_u2, ctx->counters);
+    uint32_t *spawn_free = ctx->create_rec + (size_t)c.capacity * TPT_CREATE_REC_WORDS;
+    hipLaunchKernelGGL(mv_saw, dim3(TPT_SPAWN_BITMAP_WORDS * 32u / BLK), b, 0, ctx->stream, s.ty, s.flags,
+                       ctx->slot_scratch_u2, ctx->counters, spawn_free, c.capacity);
     tpt_pass_end(ctx, PASS_MOVE_SAW);
+    const uint32_t *defaults = tpt_buf<uint32_t>(ctx, TPT_BUF_SPAWN_DEFAULTS);
+    PairBufs pb{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
+                tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP2), tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),
+                tpt_buf<float>(ctx, TPT_BUF_AIR_PV), defaults, ctx->create_rec, c.capacity};
+    tpt_pass_begin(ctx, PASS_MOVE_PAIR);
+    hipLaunchKernelGGL(mv_pair, dim3(64), b, 0, ctx->stream, s, pb, pmap, winner, ctx->slot_scratch_u, ctx->counters,
+                       slots, xres, yres, xc, seed);
+    tpt_pass_end(ctx, PASS_MOVE_PAIR);
     CommitBufs cb{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
                   tpt_buf<float>(ctx, TPT_BUF_PART_TEMP), tpt_buf<uint32_t>(ctx, TPT_BUF_PHOTON_MASK),
                   tpt_buf<float>(ctx, TPT_BUF_AIR_PV)};
@@ -1418,6 +1797,15 @@ int tpt_movement_step(tpt_ctx *ctx) {
     hipLaunchKernelGGL(mv_photon_hits, dim3(1), dim3(1024), 0, ctx->stream, s, cb, pmap, lut, ctx->slot_scratch_u2,
                        ctx->counters, ctx->slot_owner, xres, yres, c.heat_enable ? 1u : 0u);
     tpt_pass_end(ctx, PASS_MOVE_COMMIT);
+    DrainBufs db{tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),    tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP2),
+                 tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP3),    tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP4),
+                 tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE),  tpt_buf<uint32_t>(ctx, TPT_BUF_PART_DCOLOUR),
+                 tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),      defaults,
+                 ctx->create_rec,                              spawn_free};
+    tpt_pass_begin(ctx, PASS_MOVE_DRAIN);
+    hipLaunchKernelGGL(mv_drain, dim3(1), dim3(DRAIN_BLK), 0, ctx->stream, s, db, ctx->counters, c.capacity, xres,
+                       yres);
+    tpt_pass_end(ctx, PASS_MOVE_DRAIN);
     int rc = tpt_check_launch("movement pass");
     if (rc) return rc;
     // clear_spatial_maps_kernel + scatter_spatial_maps_kernel (commit.rs:541-603),

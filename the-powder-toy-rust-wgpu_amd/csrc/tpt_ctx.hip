@@ -19,7 +19,8 @@ const char *const tpt_pass_names[PASS_COUNT] = {
     "grav_apply", "grav_build_mass", "recalc_clear", "recalc_scatter", "check_stacking",
     "thermal_conduction", "thermal_conduction_cells", "thermal_apply", "phase_transition",
     "phase_transition_cells", "refresh_pmap_types", "air_deposit", "air_deposit_cells", "move_detect",
-    "move_clear_intents", "move_velocity_intent", "move_saw", "move_commit", "move_rebuild_maps",
+    "move_clear_intents", "move_velocity_intent", "move_saw", "move_pair", "move_commit", "move_drain",
+    "move_rebuild_maps",
 };
 
 static thread_local char g_err[512];
@@ -79,6 +80,7 @@ static int alloc_buffers(tpt_ctx *c) {
         else if (id == TPT_BUF_ELEM_PROPS) words = (size_t)TPT_PT_NUM * TPT_ELEM_STRIDE;
         else if (id == TPT_BUF_MOVE_LUT) words = TPT_MOVE_LUT_WORDS;
         else if (id == TPT_BUF_PHOTON_MASK) words = TPT_PT_NUM;
+        else if (id == TPT_BUF_SPAWN_DEFAULTS) words = (size_t)TPT_PT_NUM * TPT_SPAWN_DEFAULT_WORDS;
         c->buf_bytes[id] = words * 4;
         if (hipMalloc(&c->buf[id], words * 4) != hipSuccess)
             return tpt_set_error(TPT_ENOMEM, "hipMalloc of buffer %d (%zu bytes) failed", id, words * 4);
@@ -97,7 +99,8 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMalloc(&c->pix_heat, (size_t)c->pixels * 8) != hipSuccess || hipMalloc(&c->offmap, cap * 4) != hipSuccess ||
         hipMalloc(&c->pix_type, (size_t)c->pixels) != hipSuccess ||
         hipMalloc(&c->pix_count_mv, (size_t)c->pixels * 4) != hipSuccess ||
-        hipMalloc(&c->slot_owner, cap * 4) != hipSuccess)
+        hipMalloc(&c->slot_owner, cap * 4) != hipSuccess ||
+        hipMalloc(&c->create_rec, (cap * TPT_CREATE_REC_WORDS + TPT_SPAWN_BITMAP_WORDS) * 4) != hipSuccess)
         return tpt_set_error(TPT_ENOMEM, "scratch allocation failed");
     hipMemsetAsync(c->cell_scratch_u, 0, cb, c->stream);
     hipMemsetAsync(c->cell_scratch_u2, 0, cb, c->stream);
@@ -112,7 +115,7 @@ static void free_all(tpt_ctx *c) {
         if (c->buf[id]) hipFree(c->buf[id]);
     void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
                      c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters,
-                     c->pix_rec, c->pix_heat, c->offmap, c->pix_type, c->pix_count_mv, c->slot_owner};
+                     c->pix_rec, c->pix_heat, c->offmap, c->pix_type, c->pix_count_mv, c->slot_owner, c->create_rec};
     for (void *p : extra)
         if (p) hipFree(p);
     tpt_gravity_free(c);
@@ -245,6 +248,12 @@ extern "C" int tpt_set_slot_bound(tpt_ctx *ctx, uint32_t bound) {
         ctx->epoch++;
     }
     ctx->slot_bound = bound;
+    return TPT_OK;
+}
+
+extern "C" int tpt_get_slot_bound(tpt_ctx *ctx, uint32_t *bound) {
+    if (!ctx || !bound) return tpt_set_error(TPT_EINVAL, "null argument");
+    *bound = ctx->slot_bound;
     return TPT_OK;
 }
 

@@ -14,6 +14,8 @@
 #include "../../include/tpt_hip.h"
 
 #define TPT_CELL 4
+#define TPT_CREATE_REC_WORDS 6  // requester slot, x | y << 16, ctype, temp, vx, vy
+#define TPT_SPAWN_BITMAP_WORDS ((TPT_GPU_SPAWN_END - TPT_GPU_SPAWN_START) / 32u)
 
 // Slots of the small device counter block (tpt_ctx::counters).
 #define TPT_CTR_INVIS 0   // INVIS/FILT placement queue length
@@ -21,6 +23,8 @@
 #define TPT_CTR_OFFMAP 2  // off-map list length (see spatial.hip)
 #define TPT_CTR_EMAP_GATE 3  // wire closure gate bits (move.hip)
 #define TPT_CTR_PHOT 4       // blocked-photon hit list length (move.hip)
+#define TPT_CTR_PAIR 5       // e == 2 pair candidate list length (move.hip)
+#define TPT_CTR_CREATE 6     // e == 2 creation records queued this tick (move.hip)
 #define TPT_CTR_FRAME_LO 8   // next frame_count, low / high word (tick.hip)
 #define TPT_CTR_FRAME_HI 9
 #define TPT_CTR_SEED 10      // seed of the pass(es) running now
@@ -52,8 +56,10 @@ enum tpt_pass {
     PASS_MOVE_DETECT,     // WL_DETECT charging + wire closure
     PASS_MOVE_CLEAR,      // cell_winner clear + planner type map
     PASS_MOVE_INTENT,     // velocity integration + destination + claim
-    PASS_MOVE_SAW,        // deferred WOOD -> SAWD
+    PASS_MOVE_SAW,        // deferred WOOD -> SAWD + spawn-window free bitmap
+    PASS_MOVE_PAIR,       // e == 2 pair resolution
     PASS_MOVE_COMMIT,     // winner check + move/swap
+    PASS_MOVE_DRAIN,      // creations queued by the pair pass
     PASS_MOVE_REBUILD,    // pmap/photons rebuild
     PASS_COUNT
 };
@@ -128,6 +134,10 @@ struct tpt_ctx {
     hipGraphExec_t graph = nullptr;
     uint32_t *pix_count_mv = nullptr;     // pixels: particle counts of the movement rebuild
     uint32_t *slot_owner = nullptr;       // capacity words, all ones at rest (photon hit rounds)
+    // e == 2 creations: TPT_CREATE_REC_WORDS words per record, one record
+    // at most per pair mover (so `capacity` records), then the free bitmap
+    // of the device allocation window (TPT_SPAWN_BITMAP_WORDS words).
+    uint32_t *create_rec = nullptr;
     tpt_fft_state fft;
 
     // Profiling: a ring of event pairs per pass, folded lazily so that

@@ -182,3 +182,19 @@ def photon_mask(elements=None):
     """``build_photon_mask`` (move_kernel/mod.rs:1592-1606) -> uint32[PT_NUM]."""
     E = ELEMENTS if elements is None else elements
     return np.array([e["photon_reflect"] & 0xFFFFFFFF for e in E], dtype=np.uint32)
+
+
+SPAWN_DEFAULT_WORDS = 8
+
+
+def spawn_defaults(elements=None):
+    """``to_gpu_spawn_defaults_u32`` (tpt-core/src/elements/mod.rs:900-936) ->
+    uint32[PT_NUM * 8]: {temp bits, life, ctype, tmp, tmp2, tmp3, tmp4, 0}."""
+    E = ELEMENTS if elements is None else elements
+    out = np.zeros(PT_NUM * SPAWN_DEFAULT_WORDS, dtype=np.uint32)
+    for t, e in enumerate(E):
+        out[t * SPAWN_DEFAULT_WORDS: (t + 1) * SPAWN_DEFAULT_WORDS] = [
+            _f32_bits(e["default_temp"]), _i32_bits(e["default_life"]), e["default_ctype"] & 0xFFFFFFFF,
+            _i32_bits(e["default_tmp"]), _i32_bits(e["default_tmp2"]), _i32_bits(e["default_tmp3"]),
+            _i32_bits(e["default_tmp4"]), 0]
+    return out

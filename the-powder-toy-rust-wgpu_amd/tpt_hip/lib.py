@@ -47,6 +47,7 @@ SIGNATURES = {
     "tpt_download": ([_vp, ctypes.c_int, _vp, ctypes.c_size_t, ctypes.c_size_t], ctypes.c_int),
     "tpt_buffer_bytes": ([_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "tpt_set_slot_bound": ([_vp, ctypes.c_uint32], ctypes.c_int),
+    "tpt_get_slot_bound": ([_vp, ctypes.POINTER(ctypes.c_uint32)], ctypes.c_int),
     "tpt_step_air": ([_vp], ctypes.c_int),
     "tpt_step_gravity": ([_vp, ctypes.c_uint32], ctypes.c_int),
     "tpt_gravity_solve": ([_vp], ctypes.c_int),
@@ -166,16 +167,23 @@ class Context:
         """Upload every array of a SimState (sync_host_to_gpu of all categories)."""
         self.set_tables(st.arrays["elem_props"], st.arrays["move_lut"])
         self.upload("photon_mask", st.arrays["photon_mask"])
+        self.upload("spawn_defaults", st.arrays["spawn_defaults"])
         for name in S.BUFFER_ORDER:
             if name not in S.TABLES:
                 self.upload(name, st.arrays[name])
         self.set_slot_bound(st.slot_bound)
 
+    def slot_bound(self):
+        out = ctypes.c_uint32()
+        _check(load().tpt_get_slot_bound(self._h, ctypes.byref(out)))
+        return out.value
+
     def fetch(self, st, names=None):
-        """Download arrays into a SimState (sync_gpu_to_host)."""
+        """Download arrays into a SimState (sync_gpu_to_host), with the slot bound."""
         for name in names or [n for n in S.BUFFER_ORDER if n not in S.TABLES]:
             arr = st.arrays[name]
             arr[:] = self.download(name, arr.dtype, arr.size)
+        st.slot_bound = self.slot_bound()
         return st
 
     # passes ---------------------------------------------------------------

@@ -47,8 +47,8 @@ CELL_ARRAYS = [
 ]
 # Buffer ids of include/tpt_hip.h, in enum order.
 BUFFER_ORDER = [n for n, _ in SLOT_ARRAYS] + [n for n, _ in PIXEL_ARRAYS] + \
-    [n for n, _ in CELL_ARRAYS] + ["elem_props", "move_lut", "photon_mask"]
-TABLES = ["elem_props", "move_lut", "photon_mask"]
+    [n for n, _ in CELL_ARRAYS] + ["elem_props", "move_lut", "photon_mask", "spawn_defaults"]
+TABLES = ["elem_props", "move_lut", "photon_mask", "spawn_defaults"]
 
 
 def default_config(xres, yres, capacity=None):
@@ -87,6 +87,7 @@ class SimState:
         self.arrays["elem_props"] = E.element_props_table()
         self.arrays["move_lut"] = E.move_lut()
         self.arrays["photon_mask"] = E.photon_mask()
+        self.arrays["spawn_defaults"] = E.spawn_defaults()
         self.slot_bound = 1
 
     def copy(self):
