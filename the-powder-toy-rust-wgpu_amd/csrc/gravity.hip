@@ -221,10 +221,14 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_cols(const float2 *__restric
 
 // Inverse row transforms of both spectra (two interleaved sequences) +
 // extract_force_kernel (fft.rs:182-200), one row per workgroup.
+// Also packs {fx, fy, wall_bmap, wall_emap} per cell for grav_apply.
 __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__restrict__ sx,
                                                              const float2 *__restrict__ sy,
                                                              const uint32_t *__restrict__ mask,
                                                              float *__restrict__ fx, float *__restrict__ fy,
+                                                             const uint32_t *__restrict__ wall,
+                                                             const uint32_t *__restrict__ emap,
+                                                             uint4 *__restrict__ rec,
                                                              const float2 *__restrict__ tw, FftDims d) {
     extern __shared__ float2 dyn[];
     float2 *sm = dyn, *tab = dyn + 2 * d.W;
@@ -239,8 +243,10 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__res
     for (int i = threadIdx.x; i < d.xc; i += FFT_THREADS) {
         const int c = row * d.xc + i;
         const bool on = mask[c] != 0u;
-        fx[c] = on ? sm[i * 2 + 0].x : 0.0f;
-        fy[c] = on ? sm[i * 2 + 1].x : 0.0f;
+        const float gx = on ? sm[i * 2 + 0].x : 0.0f, gy = on ? sm[i * 2 + 1].x : 0.0f;
+        fx[c] = gx;
+        fy[c] = gy;
+        rec[c] = make_uint4(__float_as_uint(gx), __float_as_uint(gy), wall[c], emap[c]);
     }
 }
 
@@ -290,13 +296,12 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_cols_inplace(float2 *__restr
 }
 
 // apply_gravity_kernel (gravity_kernel.rs:434-507), NewtonianGravity = 1.
+// rec: the per-cell {fx, fy, wall_bmap, wall_emap} grav_rows_inv packed.
 __global__ __launch_bounds__(GRAV_BLOCK) void grav_apply(const uint32_t *__restrict__ ty_a,
                                                          const float *__restrict__ px_a,
                                                          const float *__restrict__ py_a, float *__restrict__ vx_a,
                                                          float *__restrict__ vy_a, const uint32_t *__restrict__ flags_a,
-                                                         const float *__restrict__ fx, const float *__restrict__ fy,
-                                                         const uint32_t *__restrict__ wall_bmap,
-                                                         const uint32_t *__restrict__ wall_emap,
+                                                         const uint4 *__restrict__ rec,
                                                          const uint32_t *__restrict__ lut, uint32_t slots, int xres,
                                                          int yres, int xc) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -308,7 +313,8 @@ __global__ __launch_bounds__(GRAV_BLOCK) void grav_apply(const uint32_t *__restr
     const int xi = tpt_f2i(px_a[i] + 0.5f), yi = tpt_f2i(py_a[i] + 0.5f);
     if (xi < 4 || yi < 4 || xi >= xres - 4 || yi >= yres - 4) return;
     const int cell = (yi / TPT_CELL) * xc + xi / TPT_CELL;
-    if (wall_bmap[cell] == 18u && (wall_emap[cell] & 0xFFu) < 8u) return;
+    const uint4 r = rec[cell];
+    if (r.z == 18u && (r.w & 0xFFu) < 8u) return;
     const uint32_t props = lut[ty * 12u];
     const float loss = __uint_as_float(lut[ty * 12u + 7u]);
     float vx = vx_a[i], vy = vy_a[i];
@@ -317,8 +323,8 @@ __global__ __launch_bounds__(GRAV_BLOCK) void grav_apply(const uint32_t *__restr
         vy *= loss;
     }
     if ((props & 0x04u) == 0u) {  // TYPE_SOLID receives no force
-        vx += 1.0f * fx[cell];
-        vy += 1.0f * fy[cell];
+        vx += 1.0f * __uint_as_float(r.x);
+        vy += 1.0f * __uint_as_float(r.y);
     }
     vx_a[i] = vx;
     vy_a[i] = vy;
@@ -482,7 +488,9 @@ int solve(tpt_ctx *ctx) {
     tpt_pass_begin(ctx, PASS_GRAV_FFT_INV);
     hipLaunchKernelGGL(grav_rows_inv, dim3(d.yc), dim3(FFT_THREADS), rows_lds(d, 2), ctx->stream,
                        f.spec_x, f.spec_y, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK),
-                       tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), tpt_buf<float>(ctx, TPT_BUF_GRAV_FY), f.twiddle, d);
+                       tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), tpt_buf<float>(ctx, TPT_BUF_GRAV_FY),
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP),
+                       ctx->cell_rec, f.twiddle, d);
     tpt_pass_end(ctx, PASS_GRAV_FFT_INV);
     return tpt_check_launch("gravity solve");
 }
@@ -508,9 +516,8 @@ int tpt_gravity_step(tpt_ctx *ctx, uint32_t) {
     hipLaunchKernelGGL(grav_apply, sgrid, dim3(GRAV_BLOCK), 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY),
                        tpt_buf<float>(ctx, TPT_BUF_PART_X), tpt_buf<float>(ctx, TPT_BUF_PART_Y),
                        tpt_buf<float>(ctx, TPT_BUF_PART_VX), tpt_buf<float>(ctx, TPT_BUF_PART_VY),
-                       tpt_buf<uint32_t>(ctx, TPT_BUF_PART_FLAGS), tpt_buf<float>(ctx, TPT_BUF_GRAV_FX),
-                       tpt_buf<float>(ctx, TPT_BUF_GRAV_FY), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP),
-                       tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_MOVE_LUT), slots,
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_PART_FLAGS), ctx->cell_rec,
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_MOVE_LUT), slots,
                        (int)ctx->cfg.xres, (int)ctx->cfg.yres, (int)ctx->xcells);
     tpt_pass_end(ctx, PASS_GRAV_APPLY);
     tpt_pass_begin(ctx, PASS_GRAV_BUILD);

@@ -62,8 +62,10 @@ struct MoveEnv {
     // the intent kernel: the planner's probes mostly need only the
     // occupant's type, and the byte map is a quarter of pmap's footprint.
     const uint8_t *__restrict__ tmap;
-    const uint32_t *__restrict__ wall;
-    const uint32_t *__restrict__ emap;
+    // Per-cell record {air_vx bits, air_vy bits, wall_bmap, wall_emap},
+    // packed once per movement pass: a particle's own cell and its probes'
+    // cells cost one random line instead of one per array.
+    const uint4 *__restrict__ cellrec;
     const uint32_t *__restrict__ lut;
     int xr, yr, xc;
     uint32_t edge;
@@ -113,7 +115,7 @@ __device__ uint32_t eval_move_at(const MoveEnv &e, uint32_t props, uint32_t ty, 
     if (!(nx >= 0 && nx < e.xr && ny >= 0 && ny < e.yr)) return 0u;
     const uint32_t d_ty = (uint32_t)e.tmap[ny * e.xr + nx];
     const int wc = wcell(e, nx, ny);
-    const uint32_t wall = e.wall[wc], em = e.emap[wc];
+    const uint32_t wall = e.cellrec[wc].z, em = e.cellrec[wc].w;
     uint32_t can = e.lut[LUT_CAN + ty * 256u + d_ty];
     if (can == 3u) {
         if (d_ty == 23u || d_ty == 75u) can = 1u;
@@ -186,8 +188,8 @@ __device__ bool plan_sample_blocked(const MoveEnv &e, uint32_t props, uint32_t t
     const uint32_t raw_can = e.lut[LUT_CAN + ty * 256u + d_ty];
     const uint32_t ev = eval_move_at(e, props, ty, nx, ny);
     const int wc = wcell(e, nx, ny);
-    const uint32_t wall = e.wall[wc];
-    const bool closed_now = wall == 12u && (e.emap[wc] & 0xFFu) == 0u;
+    const uint32_t wall = e.cellrec[wc].z;
+    const bool closed_now = wall == 12u && (e.cellrec[wc].w & 0xFFu) == 0u;
     return ev == 0u || (raw_can == 3u && ev == 1u) || (ty == 29u && d_ty != 0u) || wall == 7u ||
            closed_now != closed_start;
 }
@@ -218,7 +220,7 @@ __device__ Plan plan_move(const MoveEnv &e, float vx, float vy, float px, float 
     p.clear_y = yi;
     if (mv >= ISTP) {
         const int sc = wcell(e, xi, yi);
-        const bool start_closed = e.wall[sc] == 12u && (e.emap[sc] & 0xFFu) == 0u;
+        const bool start_closed = e.cellrec[sc].z == 12u && (e.cellrec[sc].w & 0xFFu) == 0u;
         const float step_xf = vx * ISTP / mv, step_yf = vy * ISTP / mv;
         float walk_xf = px, walk_yf = py, remaining = mv;
         for (uint32_t guard = 0; guard < MAX_PLAN_STEPS; ++guard) {
@@ -322,7 +324,7 @@ __device__ uint32_t scan_liquid_vertical(const MoveEnv &e, int xi, int yi, int f
         } else {
             if (fin_y >= 0 && fin_y < e.yr) {
                 const uint32_t d_ty = (uint32_t)e.tmap[pix(e, j, fin_y)];
-                if (d_ty != ty || e.wall[wcell(e, j, fin_y)] != 0u) {
+                if (d_ty != ty || e.cellrec[wcell(e, j, fin_y)].z != 0u) {
                     int ox, oy;
                     const uint32_t d = do_move_at(e, props, ty, j, fin_y, ox, oy);
                     if (d == DO_DIE) {
@@ -339,7 +341,7 @@ __device__ uint32_t scan_liquid_vertical(const MoveEnv &e, int xi, int yi, int f
             }
             if (!stop && fin_y != clear_y && clear_y >= 0 && clear_y < e.yr) {
                 const uint32_t d_ty = (uint32_t)e.tmap[pix(e, j, clear_y)];
-                if (d_ty != ty || e.wall[wcell(e, j, clear_y)] != 0u) {
+                if (d_ty != ty || e.cellrec[wcell(e, j, clear_y)].z != 0u) {
                     int ox, oy;
                     const uint32_t d = do_move_at(e, props, ty, j, clear_y, ox, oy);
                     if (d == DO_DIE) {
@@ -356,7 +358,7 @@ __device__ uint32_t scan_liquid_vertical(const MoveEnv &e, int xi, int yi, int f
             }
             if (!stop && clear_y >= 0 && clear_y < e.yr) {
                 const uint32_t d_ty = (uint32_t)e.tmap[pix(e, j, clear_y)];
-                const uint32_t wall = e.wall[wcell(e, j, clear_y)];
+                const uint32_t wall = e.cellrec[wcell(e, j, clear_y)].z;
                 if (d_ty != ty || (wall != 0u && wall != 4u)) stop = true;
             }
             if (!stop) {
@@ -382,7 +384,7 @@ __device__ uint32_t scroll_liquid_down(const MoveEnv &e, int scan_x, int scan_y,
             stop = true;
         } else {
             const uint32_t d_ty = (uint32_t)e.tmap[pix(e, scan_x, jj)];
-            const uint32_t wall = e.wall[wcell(e, scan_x, jj)];
+            const uint32_t wall = e.cellrec[wcell(e, scan_x, jj)].z;
             if (d_ty != ty || wall != 0u) {
                 int ox, oy;
                 const uint32_t d = do_move_at(e, props, ty, scan_x, jj, ox, oy);
@@ -437,7 +439,7 @@ __device__ uint32_t scan_liquid_perpendicular(const MoveEnv &e, float cxf, float
                 stop = true;
             } else {
                 const uint32_t d_ty = (uint32_t)e.tmap[pix(e, nx, ny)];
-                const uint32_t wall = e.wall[wcell(e, nx, ny)];
+                const uint32_t wall = e.cellrec[wcell(e, nx, ny)].z;
                 if (d_ty != ty || wall != 0u) {
                     int ox, oy;
                     const uint32_t d = do_move_at(e, props, ty, nx, ny, ox, oy);
@@ -476,7 +478,7 @@ __device__ uint32_t scan_liquid_perpendicular(const MoveEnv &e, float cxf, float
                     stop2 = true;
                 } else {
                     const uint32_t d_ty = (uint32_t)e.tmap[pix(e, nnx, nny)];
-                    const uint32_t wall = e.wall[wcell(e, nnx, nny)];
+                    const uint32_t wall = e.cellrec[wcell(e, nnx, nny)].z;
                     if (d_ty != ty || wall != 0u) {
                         int ox, oy;
                         const uint32_t d = do_move_at(e, props, ty, nnx, nny, ox, oy);
@@ -1047,6 +1049,15 @@ __global__ void mv_type_map(const uint4 *__restrict__ pmap, uint32_t *__restrict
     tmap[i] = (v.x & 0xFFu) | ((v.y & 0xFFu) << 8) | ((v.z & 0xFFu) << 16) | ((v.w & 0xFFu) << 24);
 }
 
+// The movement pass's per-cell record (MoveEnv::cellrec), after the air
+// deposit and the wire passes have written their grids.
+__global__ void mv_cell_pack(const uint32_t *__restrict__ air_vx, const uint32_t *__restrict__ air_vy,
+                             const uint32_t *__restrict__ wall, const uint32_t *__restrict__ emap,
+                             uint4 *__restrict__ rec, uint32_t cells) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cells) rec[i] = make_uint4(air_vx[i], air_vy[i], wall[i], emap[i]);
+}
+
 __global__ void mv_clear(uint4 *__restrict__ winner, uint32_t n4, uint32_t *__restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
@@ -1058,9 +1069,8 @@ __global__ void mv_clear(uint4 *__restrict__ winner, uint32_t n4, uint32_t *__re
 }
 
 // velocity_and_intent_kernel (velocity.rs:951-1157).
-__global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, const float *__restrict__ air_vx,
-                                                 const float *__restrict__ air_vy, uint32_t *__restrict__ winner,
-                                                 uint32_t slots, const uint32_t *__restrict__ seedp) {
+__global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, uint32_t *__restrict__ winner, uint32_t slots,
+                                                 const uint32_t *__restrict__ seedp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= slots) return;
     s.intent[i] = NO_INTENT;
@@ -1082,7 +1092,8 @@ __global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, const float
     const float advection = __uint_as_float(e.lut[meta + 4u]), loss = __uint_as_float(e.lut[meta + 7u]);
     const float collision = __uint_as_float(e.lut[meta + 8u]), gravity = __uint_as_float(e.lut[meta + 9u]);
     const float diffusion = __uint_as_float(e.lut[meta + 10u]);
-    const uint32_t wall = e.wall[cell], em = e.emap[cell];
+    const uint4 crec = e.cellrec[cell];
+    const uint32_t wall = crec.z, em = crec.w;
     const bool in_margin = !(xi < 4 || yi < 4 || xi >= e.xr - 4 || yi >= e.yr - 4);
     const bool stickman = ty == 59u || ty == 137u || ty == 159u;
     if (!in_margin || (!stickman && wall_kills_particle(wall, em, props))) {
@@ -1092,7 +1103,7 @@ __global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, const float
     if (wall == 18u && (em & 0xFFu) < 8u) return;
     float gx, gy;
     base_gravity(e, gravity, (float)xi, (float)yi, gx, gy);
-    const float ax = air_vx[cell], ay = air_vy[cell];
+    const float ax = __uint_as_float(crec.x), ay = __uint_as_float(crec.y);
     const uint32_t h1 = pcg_hash(seed ^ (i * 2654435769u)), h2 = pcg_hash(h1);
     const float loss_here = e.newton_grav ? 1.0f : loss;
     const float rdx = (float)(h1 & 0xFFFFu) * (2.0f / 65535.0f) - 1.0f;
@@ -1498,8 +1509,7 @@ __device__ __forceinline__ uint32_t pmap_at(const uint32_t *pmap, int x, int y, 
 
 // commit_move_kernel (commit.rs:181-535).
 __global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const uint32_t *__restrict__ pmap,
-                                                 const uint32_t *__restrict__ wall_a,
-                                                 const uint32_t *__restrict__ emap_a, const uint32_t *__restrict__ lut,
+                                                 const uint4 *__restrict__ cellrec, const uint32_t *__restrict__ lut,
                                                  const uint32_t *__restrict__ winner, uint32_t *__restrict__ photon_list,
                                                  uint32_t *__restrict__ counters, uint32_t slots, int xres, int yres,
                                                  int xc, uint32_t heat_enable) {
@@ -1524,7 +1534,7 @@ __global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const u
     // The claim, the destination's occupant and its wall in one round trip.
     const int nx = (int)(target % (uint32_t)xres), ny = (int)(target / (uint32_t)xres);
     const int wc = (ny / TPT_CELL) * xc + nx / TPT_CELL;
-    const uint32_t claim = winner[target], dest = pmap[target], wall = wall_a[wc];
+    const uint32_t claim = winner[target], dest = pmap[target], wall = cellrec[wc].z;
     if ((claim & 0x7FFFFFu) != i + 1u) return;
     if (half_mirror) {
         s.x[i] = (float)nx;
@@ -1555,10 +1565,10 @@ __global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const u
         }
     }
     bool ehole_exit = false;
-    if (wall == 12u && (emap_a[wc] & 0xFFu) == 0u) {
+    if (wall == 12u && (cellrec[wc].w & 0xFFu) == 0u) {
         const int cx = tpt_f2i(s.x[i] + 0.5f) / TPT_CELL, cy = tpt_f2i(s.y[i] + 0.5f) / TPT_CELL;
         const int here = cy * xc + cx;
-        ehole_exit = wall_a[here] == 12u && (emap_a[here] & 0xFFu) == 0u;
+        ehole_exit = cellrec[here].z == 12u && (cellrec[here].w & 0xFFu) == 0u;
     }
     if (ehole_exit) return;
     if (docc != 0u && can == 0u) return;
@@ -1591,7 +1601,7 @@ __global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const u
             const int own_cell = (own_y / TPT_CELL) * xc + own_x / TPT_CELL;
             const uint32_t own = pmap[own_y * xres + own_x];
             const bool drag = own != 0u && (lut[(own & 0x1FFu) * 12u] & 0x80u) != 0u;
-            if (drag && wall_a[own_cell] != 15u) {
+            if (drag && cellrec[own_cell].z != 15u) {
                 const uint32_t under = (own >> 9) - 1u;
                 s.x[under] = (float)nx;
                 s.y[under] = (float)ny;
@@ -1766,14 +1776,16 @@ int tpt_movement_step(tpt_ctx *ctx) {
         tpt_pass_end(ctx, PASS_MOVE_DETECT);
     }
     tpt_pass_begin(ctx, PASS_MOVE_CLEAR);
+    hipLaunchKernelGGL(mv_cell_pack, cg, b, 0, ctx->stream, (const uint32_t *)air_vx, (const uint32_t *)air_vy, wall,
+                       emap, ctx->cell_rec, ctx->cells);
     hipLaunchKernelGGL(mv_clear, pg, b, 0, ctx->stream, (uint4 *)winner, n4, ctx->counters);
     hipLaunchKernelGGL(mv_type_map, pg, b, 0, ctx->stream, (const uint4 *)pmap, (uint32_t *)ctx->pix_type, n4);
     tpt_pass_end(ctx, PASS_MOVE_CLEAR);
-    MoveEnv e{pmap, ctx->pix_type, wall, emap, lut, xres, yres, xc, c.edge_mode, c.gravity_mode,
+    MoveEnv e{pmap, ctx->pix_type, ctx->cell_rec, lut, xres, yres, xc, c.edge_mode, c.gravity_mode,
               c.custom_gravity_x, c.custom_gravity_y, c.newton_grav, ctx->slot_scratch_u2, ctx->counters + TPT_CTR_SAW,
               tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE), ctx->slot_scratch_u, ctx->counters};
     tpt_pass_begin(ctx, PASS_MOVE_INTENT);
-    hipLaunchKernelGGL(mv_intent, sg, b, 0, ctx->stream, s, e, air_vx, air_vy, winner, slots, seed);
+    hipLaunchKernelGGL(mv_intent, sg, b, 0, ctx->stream, s, e, winner, slots, seed);
     tpt_pass_end(ctx, PASS_MOVE_INTENT);
     tpt_pass_begin(ctx, PASS_MOVE_SAW);
     uint32_t *spawn_free = ctx->create_rec + (size_t)c.capacity * TPT_CREATE_REC_WORDS;
@@ -1792,7 +1804,7 @@ int tpt_movement_step(tpt_ctx *ctx) {
                   tpt_buf<float>(ctx, TPT_BUF_PART_TEMP), tpt_buf<uint32_t>(ctx, TPT_BUF_PHOTON_MASK),
                   tpt_buf<float>(ctx, TPT_BUF_AIR_PV)};
     tpt_pass_begin(ctx, PASS_MOVE_COMMIT);
-    hipLaunchKernelGGL(mv_commit, sg, b, 0, ctx->stream, s, cb, pmap, wall, emap, lut, winner, ctx->slot_scratch_u2,
+    hipLaunchKernelGGL(mv_commit, sg, b, 0, ctx->stream, s, cb, pmap, ctx->cell_rec, lut, winner, ctx->slot_scratch_u2,
                        ctx->counters, slots, xres, yres, xc, c.heat_enable ? 1u : 0u);
     hipLaunchKernelGGL(mv_photon_hits, dim3(1), dim3(1024), 0, ctx->stream, s, cb, pmap, lut, ctx->slot_scratch_u2,
                        ctx->counters, ctx->slot_owner, xres, yres, c.heat_enable ? 1u : 0u);

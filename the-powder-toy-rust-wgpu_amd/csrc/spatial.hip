@@ -387,8 +387,10 @@ static int map_build(tpt_ctx *ctx, int decay, uint32_t *count, int pass_clear, i
     tpt_pass_begin(ctx, pass_clear);
     hipLaunchKernelGGL(recalc_clear, dim3((n4 + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, (uint4 *)rec,
                        (uint4 *)photons, (uint4 *)count, n4, ctx->counters);
-    tpt_pass_end(ctx, pass_clear);
-    tpt_pass_begin(ctx, pass_scatter);
+    if (pass_scatter != pass_clear) {  // one timing slot for both halves otherwise
+        tpt_pass_end(ctx, pass_clear);
+        tpt_pass_begin(ctx, pass_scatter);
+    }
     hipLaunchKernelGGL(recalc_scatter, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
                        tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
                        tpt_buf<float>(ctx, TPT_BUF_PART_X), tpt_buf<float>(ctx, TPT_BUF_PART_Y),

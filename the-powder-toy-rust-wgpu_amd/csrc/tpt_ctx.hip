@@ -100,7 +100,8 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMalloc(&c->pix_type, (size_t)c->pixels) != hipSuccess ||
         hipMalloc(&c->pix_count_mv, (size_t)c->pixels * 4) != hipSuccess ||
         hipMalloc(&c->slot_owner, cap * 4) != hipSuccess ||
-        hipMalloc(&c->create_rec, (cap * TPT_CREATE_REC_WORDS + TPT_SPAWN_BITMAP_WORDS) * 4) != hipSuccess)
+        hipMalloc(&c->create_rec, (cap * TPT_CREATE_REC_WORDS + TPT_SPAWN_BITMAP_WORDS) * 4) != hipSuccess ||
+        hipMalloc(&c->cell_rec, (size_t)c->cells * sizeof(uint4)) != hipSuccess)
         return tpt_set_error(TPT_ENOMEM, "scratch allocation failed");
     hipMemsetAsync(c->cell_scratch_u, 0, cb, c->stream);
     hipMemsetAsync(c->cell_scratch_u2, 0, cb, c->stream);
@@ -115,7 +116,7 @@ static void free_all(tpt_ctx *c) {
         if (c->buf[id]) hipFree(c->buf[id]);
     void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
                      c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters,
-                     c->pix_rec, c->pix_heat, c->offmap, c->pix_type, c->pix_count_mv, c->slot_owner, c->create_rec};
+                     c->pix_rec, c->pix_heat, c->offmap, c->pix_type, c->pix_count_mv, c->slot_owner, c->create_rec, c->cell_rec};
     for (void *p : extra)
         if (p) hipFree(p);
     tpt_gravity_free(c);

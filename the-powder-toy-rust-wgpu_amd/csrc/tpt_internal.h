@@ -138,6 +138,11 @@ struct tpt_ctx {
     // at most per pair mover (so `capacity` records), then the free bitmap
     // of the device allocation window (TPT_SPAWN_BITMAP_WORDS words).
     uint32_t *create_rec = nullptr;
+    // Per-cell 16-byte records, so that a random cell access costs one line:
+    // {grav_fx, grav_fy, wall_bmap, wall_emap} from the gravity solve to
+    // grav_apply, then {air_vx, air_vy, wall_bmap, wall_emap} for the
+    // movement planner (rebuilt by each pass before use).
+    uint4 *cell_rec = nullptr;
     tpt_fft_state fft;
 
     // Profiling: a ring of event pairs per pass, folded lazily so that
