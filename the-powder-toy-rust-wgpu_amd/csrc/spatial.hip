@@ -94,22 +94,25 @@ __global__ __launch_bounds__(BLK) void recalc_scatter(
         const float fx = px_a[i], fy = py_a[i];
         const int x = tpt_f2i(fx + 0.5f), y = tpt_f2i(fy + 0.5f);
         const bool in_bounds = x >= 0 && y >= 0 && x < xres && y < yres;
+        const uint32_t props = elem_props[ty * 32u + 1u];
+        const int32_t life = decay ? life_a[i] : 0;
+        // Life decay and kills, unless the particle sits in an unpowered
+        // stasis wall cell; the wall grids are read only when it matters.
+        const bool dec = decay && life > 0 && (props & 0x4000u);
+        const bool kill = decay && !dec && life <= 0 && (props & 0x8000u);
         bool stasis = false;
-        if (in_bounds) {
+        if ((dec || kill) && in_bounds) {
             const int cell = (y / TPT_CELL) * xc + x / TPT_CELL;
             stasis = (wall_bmap[cell] & 0xFFu) == 18u && (wall_emap[cell] & 0xFFu) < 8u;
         }
-        const uint32_t props = elem_props[ty * 32u + 1u];
-        const int32_t life = decay ? life_a[i] : 0;
-        if (!decay) {
-        } else if (life > 0 && (props & 0x4000u) && (!in_bounds || !stasis)) {
+        if (dec && !stasis) {
             const int32_t next = life - 1;
             life_a[i] = next;
             if (next <= 0 && (props & (0x10000u | 0x8000u))) {
                 ty_a[i] = 0u;
                 ty = 0u;
             }
-        } else if (life <= 0 && (props & 0x8000u) && (!in_bounds || !stasis)) {
+        } else if (kill && !stasis) {
             ty_a[i] = 0u;
             ty = 0u;
         }

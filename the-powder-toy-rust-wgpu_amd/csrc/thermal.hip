@@ -652,8 +652,7 @@ __device__ uint32_t convect_target(const Parts &p, const uint32_t *__restrict__ 
 // at RecalcFreeParticles, temperatures as written by the apply kernels.
 __global__ __launch_bounds__(BLK) void th_convect_propose_rec(Parts p, const unsigned long long *__restrict__ rec,
                                                               const uint32_t *__restrict__ props,
-                                                              const float *__restrict__ gfx,
-                                                              const float *__restrict__ gfy,
+                                                              const uint4 *__restrict__ crec,
                                                               uint32_t *__restrict__ tgt, uint32_t *__restrict__ touch,
                                                               Dims d, GravCfg g, const uint32_t *__restrict__ seedp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -688,8 +687,9 @@ __global__ __launch_bounds__(BLK) void th_convect_propose_rec(Parts p, const uns
                 gx = pg * g.cgx;
                 gy = pg * g.cgy;
             }
-            gx += -2.0f * gfx[ci];
-            gy += -2.0f * gfy[ci];
+            const uint2 gf = *(const uint2 *)&crec[ci];
+            gx += -2.0f * __uint_as_float(gf.x);
+            gy += -2.0f * __uint_as_float(gf.y);
             const float rxf = gx + (float)x, ryf = gy + (float)y;
             const int rx = rxf >= 0.0f ? tpt_f2i(rxf + 0.5f) : tpt_f2i(rxf - 0.5f);
             const int ry = ryf >= 0.0f ? tpt_f2i(ryf + 0.5f) : tpt_f2i(ryf - 0.5f);
@@ -750,8 +750,17 @@ __device__ float pres_sum(const float *__restrict__ pv, int x, int y, Dims d) {
 }
 
 // phase_transition_kernel after the convection swap (thermal_kernel.rs:822-1315), rule R5.
+// Per-cell {grav_fx, grav_fy, air_pv, 0} for the convection proposal and
+// the transitions: one random line per particle instead of three.
+__global__ void th_cell_pack(const uint32_t *__restrict__ gfx, const uint32_t *__restrict__ gfy,
+                             const uint32_t *__restrict__ pv, uint4 *__restrict__ rec, uint32_t cells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < cells) rec[c] = make_uint4(gfx[c], gfy[c], pv[c], 0u);
+}
+
+// crec: the per-cell {grav_fx, grav_fy, air_pv} record th_cell_pack wrote.
 __global__ __launch_bounds__(BLK) void th_transition(Parts p, const float *__restrict__ air_pv,
-                                                     const float *__restrict__ gfx, const float *__restrict__ gfy,
+                                                     const uint4 *__restrict__ crec,
                                                      const uint32_t *__restrict__ wall_bmap,
                                                      const uint32_t *__restrict__ wall_emap,
                                                      const uint32_t *__restrict__ props, float *__restrict__ pv_new,
@@ -773,7 +782,8 @@ __global__ __launch_bounds__(BLK) void th_transition(Parts p, const float *__res
     const int x = tpt_f2i(fx), y = tpt_f2i(fy);
     if (x < 0 || x >= d.xres || y < 0 || y >= d.yres) return;
     const int ci = (y / TPT_CELL) * d.xc + x / TPT_CELL;
-    const float pv0 = air_pv[ci];
+    const uint4 cr = crec[ci];
+    const float pv0 = __uint_as_float(cr.z);
     float pv = pv0;
     const uint32_t roll = gpu_hash(seed ^ (i * PARTICLE_HASH_SALT) ^ t ^ TRANSITION_HASH_SALT);
     if (!legacy) {
@@ -985,7 +995,7 @@ __global__ __launch_bounds__(BLK) void th_transition(Parts p, const float *__res
         }
         bool s = false;
         uint32_t nt = t;
-        const float gx = gfx[ci], gy = gfy[ci];
+        const float gx = __uint_as_float(cr.x), gy = __uint_as_float(cr.y);
         const float gravtot = (gx < 0.0f ? 0.0f - gx : gx) + (gy < 0.0f ? 0.0f - gy : gy);
         const int32_t hpt = (int32_t)EP(props, t, COL_HIGH_PRESSURE_TRANSITION);
         const float hpv = EPF(props, t, COL_HIGH_PRESSURE);
@@ -1128,11 +1138,13 @@ int tpt_thermal_step(tpt_ctx *ctx, int legacy) {
         tpt_pass_end(ctx, PASS_CONDUCT_APPLY);
     }
     tpt_pass_begin(ctx, PASS_TRANSITION);
+    hipLaunchKernelGGL(th_cell_pack, cg, b, 0, ctx->stream, (const uint32_t *)gfx, (const uint32_t *)gfy,
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_AIR_PV), ctx->cell_rec, ctx->cells);
     if (!legacy) {
         const GravCfg g{ctx->cfg.gravity_mode, ctx->cfg.custom_gravity_x, ctx->cfg.custom_gravity_y};
         if (fresh)
             hipLaunchKernelGGL(th_convect_propose_rec, sg, b, 0, ctx->stream, p,
-                               (const unsigned long long *)ctx->pix_rec, props, gfx, gfy, ctx->slot_scratch_u,
+                               (const unsigned long long *)ctx->pix_rec, props, ctx->cell_rec, ctx->slot_scratch_u,
                                ctx->slot_scratch_u2, d, g, seed);
         else
             hipLaunchKernelGGL(th_convect_propose, sg, b, 0, ctx->stream, p, pmap, props, gfx, gfy,
@@ -1142,7 +1154,7 @@ int tpt_thermal_step(tpt_ctx *ctx, int legacy) {
         hipLaunchKernelGGL(th_convect_swap, sg, b, 0, ctx->stream, ctx->slot_scratch_u, ctx->slot_scratch_u2, p.temp,
                            d.slots);
     }
-    hipLaunchKernelGGL(th_transition, sg, b, 0, ctx->stream, p, tpt_buf<float>(ctx, TPT_BUF_AIR_PV), gfx, gfy,
+    hipLaunchKernelGGL(th_transition, sg, b, 0, ctx->stream, p, tpt_buf<float>(ctx, TPT_BUF_AIR_PV), ctx->cell_rec,
                        tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), props,
                        ctx->slot_scratch_f, ctx->cell_scratch_u, ctx->cell_scratch_u2,
                        fresh ? tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP) : nullptr, d, legacy ? 1u : 0u, seed);

@@ -140,7 +140,8 @@ struct tpt_ctx {
     uint32_t *create_rec = nullptr;
     // Per-cell 16-byte records, so that a random cell access costs one line:
     // {grav_fx, grav_fy, wall_bmap, wall_emap} from the gravity solve to
-    // grav_apply, then {air_vx, air_vy, wall_bmap, wall_emap} for the
+    // grav_apply, {grav_fx, grav_fy, air_pv, 0} for convection and the
+    // transitions, then {air_vx, air_vy, wall_bmap, wall_emap} for the
     // movement planner (rebuilt by each pass before use).
     uint4 *cell_rec = nullptr;
     tpt_fft_state fft;
