@@ -52,9 +52,9 @@ def kernel_bytes(name, C, P, S, N, W, yc):
         "phase_transition_cells": 16 * C,
         "refresh_pmap_types": 4 * S + 16 * N,
         "air_deposit": 8 * P + 16 * C,        # records in; per-cell winner gathers; air_vx,vy RW
-        "move_clear_intents": 9 * P,          # cell_winner clear; pmap in, type bytes out
+        "move_prepare": 9 * P + 32 * C,       # cell_winner clear; pmap in, type bytes out; cell records
         "move_velocity_intent": 4 * S + 52 * N,  # ty scan; x,y,vx,vy,flags,tmp,air in; vx,vy,intent out; claim
-        "move_saw": 0,
+        "move_saw_pair": 0,
         "move_commit": 4 * S + 32 * N,
         "move_rebuild_maps": 36 * P + 4 * S + 20 * N,  # records/photons/counts clear; scatter; pmap + counts
     }

@@ -221,14 +221,15 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_cols(const float2 *__restric
 
 // Inverse row transforms of both spectra (two interleaved sequences) +
 // extract_force_kernel (fft.rs:182-200), one row per workgroup.
-// Also packs {fx, fy, wall_bmap, wall_emap} per cell for grav_apply.
+// Also packs {fx, fy, wall_bmap, wall_emap} per cell for grav_apply, and
+// zeroes the mass grid for the tick's deposit when asked to.
 __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__restrict__ sx,
                                                              const float2 *__restrict__ sy,
                                                              const uint32_t *__restrict__ mask,
                                                              float *__restrict__ fx, float *__restrict__ fy,
                                                              const uint32_t *__restrict__ wall,
                                                              const uint32_t *__restrict__ emap,
-                                                             uint4 *__restrict__ rec,
+                                                             uint4 *__restrict__ rec, int32_t *__restrict__ mass_clear,
                                                              const float2 *__restrict__ tw, FftDims d) {
     extern __shared__ float2 dyn[];
     float2 *sm = dyn, *tab = dyn + 2 * d.W;
@@ -247,6 +248,9 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__res
         fx[c] = gx;
         fy[c] = gy;
         rec[c] = make_uint4(__float_as_uint(gx), __float_as_uint(gy), wall[c], emap[c]);
+        // The mass grid this tick's deposit accumulates into (the solve
+        // reads the previous tick's snapshot); null for a bare solve.
+        if (mass_clear) mass_clear[c] = 0;
     }
 }
 
@@ -472,7 +476,7 @@ int ensure_fft(tpt_ctx *ctx) {
     return TPT_OK;
 }
 
-int solve(tpt_ctx *ctx) {
+int solve(tpt_ctx *ctx, bool clear_mass) {
     int rc = ensure_fft(ctx);
     if (rc) return rc;
     const tpt_fft_state &f = ctx->fft;
@@ -490,14 +494,15 @@ int solve(tpt_ctx *ctx) {
                        f.spec_x, f.spec_y, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK),
                        tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), tpt_buf<float>(ctx, TPT_BUF_GRAV_FY),
                        tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP),
-                       ctx->cell_rec, f.twiddle, d);
+                       ctx->cell_rec, clear_mass ? tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS) : nullptr, f.twiddle,
+                       d);
     tpt_pass_end(ctx, PASS_GRAV_FFT_INV);
     return tpt_check_launch("gravity solve");
 }
 
 }  // namespace
 
-int tpt_gravity_solve_only(tpt_ctx *ctx) { return solve(ctx); }
+int tpt_gravity_solve_only(tpt_ctx *ctx) { return solve(ctx, false); }
 
 int tpt_gravity_step(tpt_ctx *ctx, uint32_t) {
     const uint32_t cells = ctx->cells;
@@ -508,7 +513,7 @@ int tpt_gravity_step(tpt_ctx *ctx, uint32_t) {
         hipLaunchKernelGGL(fill_u32, cgrid, blk, 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_FY), 0u, cells);
         return tpt_check_launch("gravity force clear");
     }
-    int rc = solve(ctx);
+    int rc = solve(ctx, true);
     if (rc) return rc;
     const uint32_t slots = ctx->slot_bound;
     const dim3 sgrid((slots + GRAV_BLOCK - 1) / GRAV_BLOCK);
@@ -521,7 +526,6 @@ int tpt_gravity_step(tpt_ctx *ctx, uint32_t) {
                        (int)ctx->cfg.xres, (int)ctx->cfg.yres, (int)ctx->xcells);
     tpt_pass_end(ctx, PASS_GRAV_APPLY);
     tpt_pass_begin(ctx, PASS_GRAV_BUILD);
-    hipLaunchKernelGGL(fill_u32, cgrid, blk, 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASS), 0u, cells);
     hipLaunchKernelGGL(grav_build_mass, sgrid, dim3(GRAV_BLOCK), 0, ctx->stream,
                        tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<float>(ctx, TPT_BUF_PART_X),
                        tpt_buf<float>(ctx, TPT_BUF_PART_Y), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),

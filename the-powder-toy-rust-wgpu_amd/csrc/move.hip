@@ -8,13 +8,16 @@
 //                            RecalcFreeParticles that ran before: one thread
 //                            per cell takes its pixels' owners from the
 //                            highest slot down; the off-map list first.
-//   mv_clear                 cell_winner = 0 (mod.rs:268-273), 16-byte stores.
+//   mv_prepare               cell_winner = 0 (mod.rs:268-273), the planner's
+//                            byte type map and per-cell record, in one launch.
 //   mv_intent                velocity integration, PlanMove and the per-
 //                            Falldown destination chain, then the claim
 //                            atomicMax(cell_winner[pixel],
 //                            (weight_rank << 23) | (slot + 1))
 //                            (velocity.rs:951-1157, liquid.rs:137-963).
-//   mv_saw                   WOOD -> SAWD of rejected probes, after the pass.
+//   mv_saw_pair              WOOD -> SAWD of rejected probes, after the pass,
+//                            and the e == 2 pair pass (reaction_kernel/
+//                            energy_entry.rs) over the movers mv_intent listed.
 //   mv_commit                winner check, consume/swap/move, FLAG_STAGNANT
 //                            (commit.rs:181-535).
 //   map rebuild              pmap/photons rebuild (commit.rs:541-603): the
@@ -26,10 +29,9 @@
 //   mv_detect_* / mv_emap_*  WL_DETECT charging and the set_emap closure
 //                            (mod.rs:348-828), on canvases with wire walls.
 //
-//   mv_pair                  the e == 2 pair pass (reaction_kernel/energy_entry.rs)
-//                            over the tracked movers the intent pass listed.
-//   mv_drain                 the photons the pair pass creates, allocated
-//                            from the device window (reaction_kernel/shared.rs).
+//   mv_after_commit          the deferred blocked-photon hits, then the drain
+//                            of the photons the pair pass creates into the
+//                            device window (reaction_kernel/shared.rs).
 
 #include "tpt_internal.h"
 
@@ -1042,39 +1044,42 @@ __global__ void mv_emap_strip(uint32_t *__restrict__ emap, uint32_t cells) {
     }
 }
 
-__global__ void mv_type_map(const uint4 *__restrict__ pmap, uint32_t *__restrict__ tmap, uint32_t n4) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n4) return;
-    const uint4 v = pmap[i];
-    tmap[i] = (v.x & 0xFFu) | ((v.y & 0xFFu) << 8) | ((v.z & 0xFFu) << 16) | ((v.w & 0xFFu) << 24);
-}
-
-// The movement pass's per-cell record (MoveEnv::cellrec), after the air
-// deposit and the wire passes have written their grids.
-__global__ void mv_cell_pack(const uint32_t *__restrict__ air_vx, const uint32_t *__restrict__ air_vy,
-                             const uint32_t *__restrict__ wall, const uint32_t *__restrict__ emap,
-                             uint4 *__restrict__ rec, uint32_t cells) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < cells) rec[i] = make_uint4(air_vx[i], air_vy[i], wall[i], emap[i]);
-}
-
-__global__ void mv_clear(uint4 *__restrict__ winner, uint32_t n4, uint32_t *__restrict__ counters) {
+// Everything the movement pass needs rebuilt before the intent kernel, in
+// one launch of max(P/4, 512) threads:
+//  * cell_winner = 0 (clear_move_intents_kernel, mod.rs:268-273), 16-byte
+//    stores;
+//  * the planner's byte type map from pmap;
+//  * the per-cell record MoveEnv::cellrec, after the air deposit and the
+//    wire passes have written their grids;
+//  * the allocation window's free bitmap cleared (the intent kernel fills
+//    the words of the slots it covers) and the pass's list counters reset.
+__global__ __launch_bounds__(BLK) void mv_prepare(uint4 *__restrict__ winner, const uint4 *__restrict__ pmap,
+                                                  uint32_t *__restrict__ tmap, uint32_t n4,
+                                                  const uint32_t *__restrict__ air_vx,
+                                                  const uint32_t *__restrict__ air_vy,
+                                                  const uint32_t *__restrict__ wall, const uint32_t *__restrict__ emap,
+                                                  uint4 *__restrict__ rec, uint32_t cells,
+                                                  uint32_t *__restrict__ spawn_free, uint32_t *__restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         counters[TPT_CTR_SAW] = 0u;
         counters[TPT_CTR_PAIR] = 0u;
         counters[TPT_CTR_CREATE] = 0u;
     }
-    if (i < n4) winner[i] = make_uint4(0u, 0u, 0u, 0u);
+    if (i < TPT_SPAWN_BITMAP_WORDS) spawn_free[i] = 0u;
+    if (i < n4) {
+        winner[i] = make_uint4(0u, 0u, 0u, 0u);
+        const uint4 v = pmap[i];
+        tmap[i] = (v.x & 0xFFu) | ((v.y & 0xFFu) << 8) | ((v.z & 0xFFu) << 16) | ((v.w & 0xFFu) << 24);
+    }
+    if (i < cells) rec[i] = make_uint4(air_vx[i], air_vy[i], wall[i], emap[i]);
 }
 
 // velocity_and_intent_kernel (velocity.rs:951-1157).
-__global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, uint32_t *__restrict__ winner, uint32_t slots,
-                                                 const uint32_t *__restrict__ seedp) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void intent_one(Slots &s, const MoveEnv &e, uint32_t *__restrict__ winner, uint32_t i,
+                                           uint32_t slots, uint32_t seed) {
     if (i >= slots) return;
     s.intent[i] = NO_INTENT;
-    const uint32_t seed = *seedp;
     // Slot fields loaded together rather than one latency per test.
     const uint32_t ty = s.ty[i], flags = s.flags[i];
     const float px = s.x[i], py = s.y[i], vx0 = s.vx[i], vy0 = s.vy[i];
@@ -1139,28 +1144,29 @@ __global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, uint32_t *_
     s.vy[i] = vy;
 }
 
-// Deferred WOOD -> SAWD of rejected probes, and the free bitmap of the
-// device allocation window for the pair pass's creations: a window slot is
-// free when empty and not portal-held (collect_gpu_free_slots_kernel,
-// reaction_kernel/shared.rs:1197-1222), sampled here, after the intent pass
-// (which kills out-of-bounds particles) and before the pair pass, as
-// begin_pending_creates is (move_kernel/mod.rs:1203).  SAWD and WOOD are
-// both non-empty, so the two halves do not interact.  Launched as
-// TPT_SPAWN_BITMAP_WORDS * 32 / BLK blocks of BLK threads.
-__global__ __launch_bounds__(BLK) void mv_saw(uint32_t *__restrict__ ty, const uint32_t *__restrict__ flags,
-                                              const uint32_t *__restrict__ list,
-                                              const uint32_t *__restrict__ counters,
-                                              uint32_t *__restrict__ spawn_free, uint32_t capacity) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t slot = TPT_GPU_SPAWN_START + t;
-    const bool free_slot = slot < capacity && ty[slot] == 0u && (flags[slot] & 0x04u) == 0u;
+// After its slots' intents (which kill out-of-bounds particles), a block
+// over the device allocation window records which of them are free: empty
+// and not portal-held (collect_gpu_free_slots_kernel,
+// reaction_kernel/shared.rs:1197-1222), as begin_pending_creates samples
+// them between the intent and pair launches (move_kernel/mod.rs:1203).
+// A slot's type is written only by its own lane in this kernel.
+__global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, uint32_t *__restrict__ winner, uint32_t slots,
+                                                 const uint32_t *__restrict__ seedp, uint32_t capacity,
+                                                 uint32_t *__restrict__ spawn_free) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    intent_one(s, e, winner, i, slots, *seedp);
+    const uint32_t first = blockIdx.x * blockDim.x;
+    if (first + blockDim.x <= TPT_GPU_SPAWN_START || first >= TPT_GPU_SPAWN_END) return;
+    __syncthreads();
+    const bool in_window = i >= TPT_GPU_SPAWN_START && i < TPT_GPU_SPAWN_END;
+    const bool free_slot = in_window && i < capacity && s.ty[i] == 0u && (s.flags[i] & 0x04u) == 0u;
     const uint64_t bits = __ballot(free_slot);
-    if ((threadIdx.x & 63u) == 0u) {
-        spawn_free[t >> 5] = (uint32_t)bits;
-        spawn_free[(t >> 5) + 1u] = (uint32_t)(bits >> 32);
+    const uint32_t lane0 = i & ~63u;
+    if ((threadIdx.x & 63u) == 0u && lane0 >= TPT_GPU_SPAWN_START && lane0 < TPT_GPU_SPAWN_END) {
+        const uint32_t w = (lane0 - TPT_GPU_SPAWN_START) >> 5;
+        spawn_free[w] = (uint32_t)bits;
+        spawn_free[w + 1u] = (uint32_t)(bits >> 32);
     }
-    const uint32_t n = counters[TPT_CTR_SAW];
-    for (uint32_t k = t; k < n; k += gridDim.x * blockDim.x) ty[list[k]] = 180u;
 }
 
 // ---- e == 2 pair pass (reaction_kernel/energy_entry.rs:90-393) ------------
@@ -1394,15 +1400,23 @@ __device__ void pair_one(Slots s, const PairBufs &pb, const uint32_t *__restrict
     if (final_ty != rty) s.ty[r] = final_ty;
 }
 
-__global__ __launch_bounds__(BLK) void mv_pair(Slots s, PairBufs pb, const uint32_t *__restrict__ pmap,
-                                               const uint32_t *__restrict__ winner, const uint32_t *__restrict__ list,
-                                               uint32_t *__restrict__ counters, uint32_t slots, int xres, int yres,
-                                               int xc, const uint32_t *__restrict__ seedp) {
-    const uint32_t n = counters[TPT_CTR_PAIR];
+// The deferred WOOD -> SAWD of rejected probes, then the pair pass over the
+// movers the intent kernel listed.  The two lists touch disjoint state: a
+// sawed WOOD has no pair arm, so whether a pair test sees it as WOOD or
+// SAWD changes nothing.
+__global__ __launch_bounds__(BLK) void mv_saw_pair(Slots s, PairBufs pb, const uint32_t *__restrict__ pmap,
+                                                   const uint32_t *__restrict__ winner,
+                                                   const uint32_t *__restrict__ saw_list,
+                                                   const uint32_t *__restrict__ pair_list,
+                                                   uint32_t *__restrict__ counters, uint32_t slots, int xres, int yres,
+                                                   int xc, const uint32_t *__restrict__ seedp) {
+    const uint32_t n_saw = counters[TPT_CTR_SAW], n = counters[TPT_CTR_PAIR];
+    const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (uint32_t k = t0; k < n_saw; k += stride) s.ty[saw_list[k]] = 180u;
     if (n == 0u) return;
     const uint32_t frame_seed = *seedp;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-        pair_one(s, pb, pmap, winner, counters, list[k], slots, xres, yres, xc, frame_seed);
+    for (uint32_t k = t0; k < n; k += stride)
+        pair_one(s, pb, pmap, winner, counters, pair_list[k], slots, xres, yres, xc, frame_seed);
 }
 
 struct DrainBufs {
@@ -1426,8 +1440,8 @@ constexpr uint32_t DRAIN_CHUNK = 4096u;
 // are rebuilt by the scatter that follows, and the active-element mask it
 // updates has no reader in this library.  One workgroup: a record's
 // allocation index is its rank by requester slot (slots are unique).
-__global__ __launch_bounds__(DRAIN_BLK) void mv_drain(Slots s, DrainBufs d, const uint32_t *__restrict__ counters,
-                                                      uint32_t rec_cap, int xres, int yres) {
+__device__ void drain_creates(Slots &s, const DrainBufs &d, const uint32_t *__restrict__ counters,
+                              uint32_t rec_cap, int xres, int yres) {
     __shared__ uint32_t keys[DRAIN_CHUNK];
     __shared__ uint32_t prefix[TPT_SPAWN_BITMAP_WORDS + 1];
     const uint32_t total = counters[TPT_CTR_CREATE];
@@ -1525,7 +1539,7 @@ __global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const u
     const bool half_mirror = (raw & HALF_MIRROR) != 0u;
     const uint32_t target = raw & PIXEL_MASK;
     if (raw & 0x80000000u) {
-        // A blocked photon's hit: deferred to mv_photon_hits (canonical R10).
+        // A blocked photon's hit: deferred to mv_after_commit (canonical R10).
         if (ty == 29u) photon_list[atomicAdd(&counters[TPT_CTR_PHOT], 1u)] = i;
         return;
     }
@@ -1679,10 +1693,10 @@ __device__ void photon_hit(Slots &s, CommitBufs &cb, const uint32_t *__restrict_
     }
 }
 
-__global__ __launch_bounds__(1024) void mv_photon_hits(Slots s, CommitBufs cb, const uint32_t *__restrict__ pmap,
-                                                       const uint32_t *__restrict__ lut, uint32_t *__restrict__ list,
-                                                       uint32_t *__restrict__ counters, uint32_t *__restrict__ owner,
-                                                       int xres, int yres, uint32_t heat_enable) {
+__device__ void photon_hits(Slots &s, CommitBufs &cb, const uint32_t *__restrict__ pmap,
+                            const uint32_t *__restrict__ lut, uint32_t *__restrict__ list,
+                            uint32_t *__restrict__ counters, uint32_t *__restrict__ owner, int xres, int yres,
+                            uint32_t heat_enable) {
     __shared__ int pending;
     const uint32_t n = counters[TPT_CTR_PHOT];
     for (;;) {
@@ -1714,6 +1728,22 @@ __global__ __launch_bounds__(1024) void mv_photon_hits(Slots s, CommitBufs cb, c
         __syncthreads();
     }
     if (threadIdx.x == 0) counters[TPT_CTR_PHOT] = 0u;
+}
+
+// One workgroup after the commit pass: the deferred blocked-photon hits,
+// then the drain of the pair pass's creations.  The drain writes only
+// window slots that were free before the pair pass; the hits touch only
+// particles that were live then.
+__global__ __launch_bounds__(DRAIN_BLK) void mv_after_commit(Slots s, CommitBufs cb, DrainBufs d,
+                                                             const uint32_t *__restrict__ pmap,
+                                                             const uint32_t *__restrict__ lut,
+                                                             uint32_t *__restrict__ list,
+                                                             uint32_t *__restrict__ counters,
+                                                             uint32_t *__restrict__ owner, uint32_t rec_cap, int xres,
+                                                             int yres, uint32_t heat_enable) {
+    photon_hits(s, cb, pmap, lut, list, counters, owner, xres, yres, heat_enable);
+    __syncthreads();
+    drain_creates(s, d, counters, rec_cap, xres, yres);
 }
 
 }  // namespace
@@ -1775,39 +1805,33 @@ int tpt_movement_step(tpt_ctx *ctx) {
         ctx->emap_markers = false;
         tpt_pass_end(ctx, PASS_MOVE_DETECT);
     }
+    uint32_t *spawn_free = ctx->create_rec + (size_t)c.capacity * TPT_CREATE_REC_WORDS;
     tpt_pass_begin(ctx, PASS_MOVE_CLEAR);
-    hipLaunchKernelGGL(mv_cell_pack, cg, b, 0, ctx->stream, (const uint32_t *)air_vx, (const uint32_t *)air_vy, wall,
-                       emap, ctx->cell_rec, ctx->cells);
-    hipLaunchKernelGGL(mv_clear, pg, b, 0, ctx->stream, (uint4 *)winner, n4, ctx->counters);
-    hipLaunchKernelGGL(mv_type_map, pg, b, 0, ctx->stream, (const uint4 *)pmap, (uint32_t *)ctx->pix_type, n4);
+    const uint32_t n_prep = n4 > TPT_SPAWN_BITMAP_WORDS ? n4 : TPT_SPAWN_BITMAP_WORDS;
+    hipLaunchKernelGGL(mv_prepare, dim3((n_prep + BLK - 1) / BLK), b, 0, ctx->stream, (uint4 *)winner,
+                       (const uint4 *)pmap, (uint32_t *)ctx->pix_type, n4, (const uint32_t *)air_vx,
+                       (const uint32_t *)air_vy, wall, emap, ctx->cell_rec, ctx->cells, spawn_free, ctx->counters);
     tpt_pass_end(ctx, PASS_MOVE_CLEAR);
     MoveEnv e{pmap, ctx->pix_type, ctx->cell_rec, lut, xres, yres, xc, c.edge_mode, c.gravity_mode,
               c.custom_gravity_x, c.custom_gravity_y, c.newton_grav, ctx->slot_scratch_u2, ctx->counters + TPT_CTR_SAW,
               tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE), ctx->slot_scratch_u, ctx->counters};
     tpt_pass_begin(ctx, PASS_MOVE_INTENT);
-    hipLaunchKernelGGL(mv_intent, sg, b, 0, ctx->stream, s, e, winner, slots, seed);
+    hipLaunchKernelGGL(mv_intent, sg, b, 0, ctx->stream, s, e, winner, slots, seed, c.capacity, spawn_free);
     tpt_pass_end(ctx, PASS_MOVE_INTENT);
-    tpt_pass_begin(ctx, PASS_MOVE_SAW);
-    uint32_t *spawn_free = ctx->create_rec + (size_t)c.capacity * TPT_CREATE_REC_WORDS;
-    hipLaunchKernelGGL(mv_saw, dim3(TPT_SPAWN_BITMAP_WORDS * 32u / BLK), b, 0, ctx->stream, s.ty, s.flags,
-                       ctx->slot_scratch_u2, ctx->counters, spawn_free, c.capacity);
-    tpt_pass_end(ctx, PASS_MOVE_SAW);
     const uint32_t *defaults = tpt_buf<uint32_t>(ctx, TPT_BUF_SPAWN_DEFAULTS);
     PairBufs pb{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
                 tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP2), tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),
                 tpt_buf<float>(ctx, TPT_BUF_AIR_PV), defaults, ctx->create_rec, c.capacity};
-    tpt_pass_begin(ctx, PASS_MOVE_PAIR);
-    hipLaunchKernelGGL(mv_pair, dim3(64), b, 0, ctx->stream, s, pb, pmap, winner, ctx->slot_scratch_u, ctx->counters,
-                       slots, xres, yres, xc, seed);
-    tpt_pass_end(ctx, PASS_MOVE_PAIR);
+    tpt_pass_begin(ctx, PASS_MOVE_SAW);
+    hipLaunchKernelGGL(mv_saw_pair, dim3(64), b, 0, ctx->stream, s, pb, pmap, winner, ctx->slot_scratch_u2,
+                       ctx->slot_scratch_u, ctx->counters, slots, xres, yres, xc, seed);
+    tpt_pass_end(ctx, PASS_MOVE_SAW);
     CommitBufs cb{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
                   tpt_buf<float>(ctx, TPT_BUF_PART_TEMP), tpt_buf<uint32_t>(ctx, TPT_BUF_PHOTON_MASK),
                   tpt_buf<float>(ctx, TPT_BUF_AIR_PV)};
     tpt_pass_begin(ctx, PASS_MOVE_COMMIT);
     hipLaunchKernelGGL(mv_commit, sg, b, 0, ctx->stream, s, cb, pmap, ctx->cell_rec, lut, winner, ctx->slot_scratch_u2,
                        ctx->counters, slots, xres, yres, xc, c.heat_enable ? 1u : 0u);
-    hipLaunchKernelGGL(mv_photon_hits, dim3(1), dim3(1024), 0, ctx->stream, s, cb, pmap, lut, ctx->slot_scratch_u2,
-                       ctx->counters, ctx->slot_owner, xres, yres, c.heat_enable ? 1u : 0u);
     tpt_pass_end(ctx, PASS_MOVE_COMMIT);
     DrainBufs db{tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),    tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP2),
                  tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP3),    tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP4),
@@ -1815,8 +1839,9 @@ int tpt_movement_step(tpt_ctx *ctx) {
                  tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),      defaults,
                  ctx->create_rec,                              spawn_free};
     tpt_pass_begin(ctx, PASS_MOVE_DRAIN);
-    hipLaunchKernelGGL(mv_drain, dim3(1), dim3(DRAIN_BLK), 0, ctx->stream, s, db, ctx->counters, c.capacity, xres,
-                       yres);
+    hipLaunchKernelGGL(mv_after_commit, dim3(1), dim3(DRAIN_BLK), 0, ctx->stream, s, cb, db, pmap, lut,
+                       ctx->slot_scratch_u2, ctx->counters, ctx->slot_owner, c.capacity, xres, yres,
+                       c.heat_enable ? 1u : 0u);
     tpt_pass_end(ctx, PASS_MOVE_DRAIN);
     int rc = tpt_check_launch("movement pass");
     if (rc) return rc;

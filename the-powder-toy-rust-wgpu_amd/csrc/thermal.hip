@@ -166,13 +166,22 @@ __global__ __launch_bounds__(BLK) void th_conduct(Parts p, const uint32_t *__res
 
 // The off-map list's slots (energy particles, INVIS/FILT, stacked losers,
 // off-pixel positions), same per-slot form.
+// Also writes the per-cell {grav_fx, grav_fy, air_pv, 0} records the
+// convection proposal and the transitions read (th_cell_pack's job on the
+// other paths): conduction changes none of the three grids.
 __global__ __launch_bounds__(BLK) void th_conduct_list(Parts p, const uint32_t *__restrict__ pmap,
                                                        const float *__restrict__ air_hv,
                                                        const uint32_t *__restrict__ props, float *__restrict__ hv_new,
                                                        uint32_t *__restrict__ hv_key, uint32_t *__restrict__ fails,
                                                        const uint32_t *__restrict__ list,
                                                        const uint32_t *__restrict__ counters, Dims d, uint32_t aheat,
-                                                       const uint32_t *__restrict__ seedp) {
+                                                       const uint32_t *__restrict__ seedp,
+                                                       const uint32_t *__restrict__ gfx,
+                                                       const uint32_t *__restrict__ gfy,
+                                                       const uint32_t *__restrict__ pv, uint4 *__restrict__ crec,
+                                                       uint32_t cells) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x)
+        crec[c] = make_uint4(gfx[c], gfy[c], pv[c], 0u);
     const uint32_t n = counters[TPT_CTR_OFFMAP], seed = *seedp;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t i = list[k];
@@ -1111,7 +1120,9 @@ int tpt_thermal_step(tpt_ctx *ctx, int legacy) {
         const uint32_t aheat = ctx->cfg.aheat_enable ? 1u : 0u;
         tpt_pass_begin(ctx, PASS_CONDUCT);
         hipLaunchKernelGGL(th_conduct_list, lg, b, 0, ctx->stream, p, pmap, air_hv, props, ctx->slot_scratch_f,
-                           ctx->cell_scratch_u, ctx->cell_scratch_u2, ctx->offmap, ctx->counters, d, aheat, seed);
+                           ctx->cell_scratch_u, ctx->cell_scratch_u2, ctx->offmap, ctx->counters, d, aheat, seed,
+                           (const uint32_t *)gfx, (const uint32_t *)gfy, tpt_buf<uint32_t>(ctx, TPT_BUF_AIR_PV),
+                           ctx->cell_rec, ctx->cells);
         hipLaunchKernelGGL(th_conduct_px, tg, tb, 0, ctx->stream, p, rec, air_hv,
                            tpt_buf<uint32_t>(ctx, TPT_BUF_BLOCK_AIRH), props, ctx->slot_scratch_f,
                            ctx->cell_scratch_u, ctx->cell_scratch_u2, ctx->pix_heat, d, aheat, seed);
@@ -1138,8 +1149,9 @@ int tpt_thermal_step(tpt_ctx *ctx, int legacy) {
         tpt_pass_end(ctx, PASS_CONDUCT_APPLY);
     }
     tpt_pass_begin(ctx, PASS_TRANSITION);
-    hipLaunchKernelGGL(th_cell_pack, cg, b, 0, ctx->stream, (const uint32_t *)gfx, (const uint32_t *)gfy,
-                       tpt_buf<uint32_t>(ctx, TPT_BUF_AIR_PV), ctx->cell_rec, ctx->cells);
+    if (legacy || !fresh)  // the fast path's th_conduct_list wrote the records
+        hipLaunchKernelGGL(th_cell_pack, cg, b, 0, ctx->stream, (const uint32_t *)gfx, (const uint32_t *)gfy,
+                           tpt_buf<uint32_t>(ctx, TPT_BUF_AIR_PV), ctx->cell_rec, ctx->cells);
     if (!legacy) {
         const GravCfg g{ctx->cfg.gravity_mode, ctx->cfg.custom_gravity_x, ctx->cfg.custom_gravity_y};
         if (fresh)

@@ -19,7 +19,7 @@ const char *const tpt_pass_names[PASS_COUNT] = {
     "grav_apply", "grav_build_mass", "recalc_clear", "recalc_scatter", "check_stacking",
     "thermal_conduction", "thermal_conduction_cells", "thermal_apply", "phase_transition",
     "phase_transition_cells", "refresh_pmap_types", "air_deposit", "air_deposit_cells", "move_detect",
-    "move_clear_intents", "move_velocity_intent", "move_saw", "move_pair", "move_commit", "move_drain",
+    "move_prepare", "move_velocity_intent", "move_saw_pair", "move_commit", "move_photon_hits_drain",
     "move_rebuild_maps",
 };
 

@@ -54,12 +54,11 @@ enum tpt_pass {
     PASS_AIR_DEPOSIT,     // particle -> air drag
     PASS_AIR_DEPOSIT_CELL,
     PASS_MOVE_DETECT,     // WL_DETECT charging + wire closure
-    PASS_MOVE_CLEAR,      // cell_winner clear + planner type map
+    PASS_MOVE_CLEAR,      // cell_winner clear + planner type map + cell records
     PASS_MOVE_INTENT,     // velocity integration + destination + claim
-    PASS_MOVE_SAW,        // deferred WOOD -> SAWD + spawn-window free bitmap
-    PASS_MOVE_PAIR,       // e == 2 pair resolution
+    PASS_MOVE_SAW,        // deferred WOOD -> SAWD + e == 2 pair resolution
     PASS_MOVE_COMMIT,     // winner check + move/swap
-    PASS_MOVE_DRAIN,      // creations queued by the pair pass
+    PASS_MOVE_DRAIN,      // blocked-photon hits + creations queued by the pair pass
     PASS_MOVE_REBUILD,    // pmap/photons rebuild
     PASS_COUNT
 };
