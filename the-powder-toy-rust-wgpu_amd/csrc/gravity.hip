@@ -178,7 +178,10 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_rows_fwd(const int32_t *__re
 // (complex_mul_kernel, fft.rs:129-140) and both inverse column transforms,
 // for 2^logcols adjacent columns per workgroup.  Only rows < rows_in of the
 // input are non-zero and only rows < rows_out of the outputs are stored.
-__global__ __launch_bounds__(FFT_THREADS) void grav_cols(const float2 *__restrict__ work,
+// Four waves per SIMD (128 VGPRs): the 1024 column workgroups then fit in
+// one round on 256 CUs instead of one and a third (measured 44 -> 33 us at
+// 2048x2048).
+__global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void grav_cols(const float2 *__restrict__ work,
                                                          const float2 *__restrict__ kx,
                                                          const float2 *__restrict__ ky, float2 *__restrict__ sx,
                                                          float2 *__restrict__ sy, const float2 *__restrict__ tw,

@@ -1150,7 +1150,10 @@ __device__ __forceinline__ void intent_one(Slots &s, const MoveEnv &e, uint32_t 
 // reaction_kernel/shared.rs:1197-1222), as begin_pending_creates samples
 // them between the intent and pair launches (move_kernel/mod.rs:1203).
 // A slot's type is written only by its own lane in this kernel.
-__global__ __launch_bounds__(BLK) void mv_intent(Slots s, MoveEnv e, uint32_t *__restrict__ winner, uint32_t slots,
+// Eight waves per SIMD (64 VGPRs): the kernel waits on dependent random
+// loads, and the extra waves hide more of that than the few spilled
+// registers cost (measured).
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8))) void mv_intent(Slots s, MoveEnv e, uint32_t *__restrict__ winner, uint32_t slots,
                                                  const uint32_t *__restrict__ seedp, uint32_t capacity,
                                                  uint32_t *__restrict__ spawn_free) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
