@@ -509,6 +509,31 @@ __global__ __launch_bounds__(256) void th_conduct_px(Parts p, const unsigned lon
     }
 }
 
+// A pixel as an apply candidate, packed in one word from its record key,
+// its conduction equilibrium and the conduction flags (bit 0 insulator,
+// bit 1 tmp == 1): slot + 1 (bits 0-22) when it can lend its equilibrium
+// (a slot below the bound, equilibrium >= 0, not an insulator), else 0;
+// TF_PAIR (bit 23) with the type's index among the TF_PAIR types (bits
+// 24-26); tmp == 1 (bit 27) and insulator (bit 28) for the pixel's own use.
+constexpr uint32_t CAND_ID = 0x7FFFFFu, CAND_PAIR = 1u << 23, CAND_TMP1 = 1u << 27, CAND_INS = 1u << 28;
+__constant__ uint32_t cand_pair_types[8] = {T_FILT, T_BRAY, T_PHOT, T_BIZR, T_BIZRG, T_ELEC, T_DEUT, T_HSWC};
+
+__device__ __forceinline__ uint32_t cand_pair_type(uint32_t c) { return cand_pair_types[(c >> 24) & 7u]; }
+
+__device__ __forceinline__ uint32_t cand_info(uint32_t nk, float eq, uint32_t hinfo, const uint32_t *sflag,
+                                              uint32_t slots) {
+    uint32_t c = ((hinfo & 2u) ? CAND_TMP1 : 0u) | ((hinfo & 1u) ? CAND_INS : 0u);
+    if (nk == 0u) return c;
+    const uint32_t nid = (nk >> 9) - 1u, nty = nk & 0xFFu;
+    if (nid < slots && eq >= 0.0f && (hinfo & 1u) == 0u) c |= nid + 1u;
+    if (sflag[nty] & TF_PAIR) {
+        uint32_t k = 0u;
+        while (cand_pair_types[k] != nty) ++k;
+        c |= CAND_PAIR | (k << 24);
+    }
+    return c;
+}
+
 // Apply for the unflagged owners of a tile: phase 2 compacts the owners,
 // phase 3 computes them.
 __global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long long *__restrict__ rec,
@@ -516,7 +541,7 @@ __global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long 
                                                    const uint32_t *__restrict__ props, Dims d) {
     __shared__ uint32_t skey[LH][LW];
     __shared__ float seq[LH][LW];
-    __shared__ uint32_t sinfo[LH][LW];
+    __shared__ uint32_t scand[LH][LW];  // each pixel as an apply candidate: cand_info()
     __shared__ float sown[TH][TW];  // record temperatures of the tile itself
     __shared__ uint32_t sflag[256];
     __shared__ uint32_t list[TW * TH];
@@ -542,14 +567,17 @@ __global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long 
                 h[u] = heat[pix];
             }
         }
+        __syncthreads();  // sflag
 #pragma unroll
         for (int u = 0; u < LOAD_ITERS; ++u) {
             const int e = t + u * TW * 4, ly = e / LW, lx = e - ly * LW;
             if (e < LH * LW) {
                 if (ly >= 1 && ly <= TH && lx >= 1 && lx <= TW) sown[ly - 1][lx - 1] = __uint_as_float((uint32_t)v[u]);
-                skey[ly][lx] = (uint32_t)(v[u] >> 32);
-                seq[ly][lx] = __uint_as_float(h[u].x);
-                sinfo[ly][lx] = h[u].y;
+                const uint32_t nk = (uint32_t)(v[u] >> 32);
+                const float eq = __uint_as_float(h[u].x);
+                skey[ly][lx] = nk;
+                seq[ly][lx] = eq;
+                scand[ly][lx] = cand_info(nk, eq, h[u].y, sflag, d.slots);
             }
         }
     }
@@ -578,24 +606,22 @@ __global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long 
             best_temp = eq;
             has = true;
         }
-        const uint32_t jinfo = sinfo[ly][lx];
-        if ((jinfo & 1u) == 0u) {
-            const int32_t jtmp1 = (jinfo & 2u) ? 1 : 0;
+        const uint32_t jinfo = scand[ly][lx];
+        if ((jinfo & CAND_INS) == 0u) {
+            const int32_t jtmp1 = (jinfo & CAND_TMP1) ? 1 : 0;
             for (int ny = 0; ny < 3; ++ny)
                 for (int nx = 0; nx < 3; ++nx) {
                     if (nx == 1 && ny == 1) continue;
-                    const uint32_t nk = skey[ly + ny - 1][lx + nx - 1];
-                    if (nk == 0u) continue;
-                    const uint32_t nid = (nk >> 9) - 1u, nty = nk & 0xFFu;
-                    if (nid >= d.slots || nid == j) continue;
+                    const uint32_t c = scand[ly + ny - 1][lx + nx - 1];
+                    const uint32_t nid1 = c & CAND_ID;
+                    if (nid1 == 0u || nid1 - 1u == j) continue;
+                    const uint32_t nid = nid1 - 1u;
                     if (!(!has || nid > best)) continue;
-                    const float neq = seq[ly + ny - 1][lx + nx - 1];
-                    if (!(neq >= 0.0f)) continue;
-                    const uint32_t ninfo = sinfo[ly + ny - 1][lx + nx - 1];
-                    if (ninfo & 1u) continue;
-                    if ((fo & sflag[nty] & TF_PAIR) && pair_excluded(nty, jty, (ninfo & 2u) ? 1 : 0, jtmp1)) continue;
+                    if ((fo & TF_PAIR) && (c & CAND_PAIR) &&
+                        pair_excluded(cand_pair_type(c), jty, (c & CAND_TMP1) ? 1 : 0, jtmp1))
+                        continue;
                     best = nid;
-                    best_temp = neq;
+                    best_temp = seq[ly + ny - 1][lx + nx - 1];
                     has = true;
                 }
         }
