@@ -5,7 +5,10 @@ import ctypes
 import os
 import re
 
+import pytest
+
 from tpt_hip import lib as L
+from tpt_hip import scene as S
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -45,3 +48,35 @@ def test_config_struct_matches_header():
     body = re.search(r"typedef struct tpt_config \{(.*?)\} tpt_config;", src, re.S).group(1)
     fields = re.findall(r"(?:uint32_t|float)\s+(\w+);", body)
     assert fields == [f for f, _ in L.TptConfig._fields_]
+
+
+def _create(**over):
+    cfg = S.default_config(64, 64)
+    cfg.update(over)
+    return L.Context(cfg)
+
+
+@pytest.mark.parametrize("over", [dict(xres=66), dict(yres=2), dict(xres=2900), dict(yres=0), dict(capacity=0),
+                                  dict(capacity=0x800000), dict(edge_mode=3), dict(air_mode=5),
+                                  dict(gravity_mode=4), dict(convection_mode=3), dict(grav_mass_mode=2)])
+def test_create_rejects_bad_configs(over):
+    """Validation happens before any device call, so this runs without a GPU:
+    the reference silently no-ops on such canvases, the ABI says why."""
+    with pytest.raises(L.TptError) as e:
+        _create(**over)
+    assert e.value.code == L.TPT_EINVAL
+    assert "tpt error -1: " in str(e.value) and len(str(e.value)) > len("tpt error -1: ")
+
+
+def test_null_arguments_are_rejected():
+    lib = L.load()
+    assert lib.tpt_step(None, 0) == L.TPT_EINVAL
+    assert lib.tpt_step_movement(None, 1) == L.TPT_EINVAL
+    assert lib.tpt_upload(None, 0, None, 0, 0) == L.TPT_EINVAL
+    assert lib.tpt_download(None, 0, None, 0, 0) == L.TPT_EINVAL
+    assert lib.tpt_set_slot_bound(None, 1) == L.TPT_EINVAL
+    assert lib.tpt_get_slot_bound(None, None) == L.TPT_EINVAL
+    assert lib.tpt_run(None, 0, 1) == L.TPT_EINVAL
+    assert lib.tpt_create(None, None) == L.TPT_EINVAL
+    assert lib.tpt_batch_create(0, None, None) == L.TPT_EINVAL
+    assert lib.tpt_last_error().decode()

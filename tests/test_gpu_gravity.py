@@ -21,9 +21,16 @@ def rel_err(got, want):
     return float(np.abs(got.astype(np.float64) - want).max() / scale) if scale else float(np.abs(got).max())
 
 
-@pytest.mark.parametrize("workload,scale", [("612x384", 1.0), ("1600x900", 0.1), ("2048x2048", 0.02)])
+@pytest.mark.parametrize("workload,scale", [("612x384", 1.0), ("1600x900", 0.1), ("2048x2048", 0.02),
+                                            ("2896x2896", 0.0), ("100x36", 0.0)])
 def test_solve_within_1e4_of_fp64(gpu, workload, scale):
-    st = S.make_scene(workload, scale_particles=scale, grav_mass_mode=1)
+    """Named workloads, the largest canvas (FFT grid 2048x2048) and a small
+    canvas whose padded grid is 64x32."""
+    if workload in ("2896x2896", "100x36"):
+        X, Y = (int(v) for v in workload.split("x"))
+        st = S.make_scene("custom", xres=X, yres=Y, seed=3, particles=0, grav_mass_mode=1)
+    else:
+        st = S.make_scene(workload, scale_particles=scale, grav_mass_mode=1)
     rng = np.random.default_rng(7)
     n = st.xcells * st.ycells
     st.arrays["grav_mass_prev"][:] = (rng.uniform(0, 100, n) * 65536).astype(np.int32) * (rng.random(n) < 0.3)
