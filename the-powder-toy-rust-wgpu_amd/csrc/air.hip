@@ -406,8 +406,12 @@ __global__ __launch_bounds__(TX *TY) void air_settle_kernel(
     float *__restrict__ hv, float *__restrict__ vx, float *__restrict__ vy, const uint32_t *__restrict__ wall_bmap,
     uint32_t *__restrict__ wall_emap, uint32_t *__restrict__ ba, uint32_t *__restrict__ bh,
     const uint32_t *__restrict__ eba, const uint32_t *__restrict__ ebh, const uint32_t *__restrict__ pmap,
-    const uint32_t *__restrict__ elem_props, float *__restrict__ pv, AirDims d, uint32_t xres, uint32_t copy_heat) {
+    const uint32_t *__restrict__ elem_props, float *__restrict__ pv, AirDims d, uint32_t xres, uint32_t copy_heat,
+    tpt_tick_begin tb) {
     const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
+    // No kernel of the air pass reads the counter block, so the tick can
+    // open here instead of in a launch of its own.
+    if (tb.counters && x == 0 && y == 0) tpt_tick_begin_dev(tb);
     if (x >= d.xc || y >= d.yc) return;
     const int idx = y * d.xc + x;
     if (copy_heat) {
@@ -454,7 +458,7 @@ __global__ __launch_bounds__(TX *TY) void air_settle_kernel(
 
 }  // namespace
 
-int tpt_air_step(tpt_ctx *ctx) {
+int tpt_air_step(tpt_ctx *ctx, const tpt_tick_begin *tb) {
     const AirDims d{(int)ctx->xcells, (int)ctx->ycells};
     const dim3 block(TX, TY);
     const dim3 grid((ctx->xcells + TX - 1) / TX, (ctx->ycells + TY - 1) / TY);
@@ -491,7 +495,7 @@ int tpt_air_step(tpt_ctx *ctx) {
                        tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), ba, bh,
                        tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_BLOCK_AIR), tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_BLOCK_AIRH),
                        tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS), pv, d,
-                       c.xres, c.aheat_enable ? 1u : 0u);
+                       c.xres, c.aheat_enable ? 1u : 0u, tb ? *tb : tpt_tick_begin{nullptr, 0u, 0u, 0u});
     tpt_pass_end(ctx, PASS_AIR_SETTLE);
     return tpt_check_launch("air_settle_kernel");
 }

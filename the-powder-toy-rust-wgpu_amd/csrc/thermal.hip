@@ -695,9 +695,12 @@ __global__ __launch_bounds__(BLK) void th_convect_propose_rec(Parts p, const uns
     const uint32_t seed = *seedp;
     touch[i] = NONE;
     uint32_t target = NONE;
+    // Position and temperature loaded with the type (coherent), so that a
+    // liquid's two dependent random loads are the only latency it waits on.
     const uint32_t t = p.ty[i];
+    const float fx = p.x[i], fy = p.y[i], own_temp = p.temp[i];
     if (t != 0u && t < 256u && (EP(props, t, COL_PROPERTIES) & 0x02u) != 0u) {
-        const int x = tpt_f2i(p.x[i]), y = tpt_f2i(p.y[i]);
+        const int x = tpt_f2i(fx), y = tpt_f2i(fy);
         bool go = x >= 0 && x < d.xres && y >= 0 && y < d.yres;
         if (go && t == T_GEL) {
             const uint32_t roll = gpu_hash(seed ^ (i * PARTICLE_HASH_SALT) ^ t ^ TRANSITION_HASH_SALT);
@@ -732,7 +735,7 @@ __global__ __launch_bounds__(BLK) void th_convect_propose_rec(Parts p, const uns
             if ((ox != x || oy != y) && ox >= 0 && oy >= 0 && ox < d.xres && oy < d.yres) {
                 const unsigned long long v = rec[(size_t)oy * d.xres + ox];
                 const uint32_t key = (uint32_t)(v >> 32), nid = (key >> 9) - 1u;
-                if (key != 0u && nid < d.slots && (key & 0xFFu) == t && p.temp[i] > __uint_as_float((uint32_t)v))
+                if (key != 0u && nid < d.slots && (key & 0xFFu) == t && own_temp > __uint_as_float((uint32_t)v))
                     target = nid;
             }
         }

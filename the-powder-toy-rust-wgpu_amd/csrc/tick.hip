@@ -10,8 +10,9 @@
 //
 // The frame counter, the tick seed and the CheckStacking flag live in the
 // context's device counter block, so that a captured tick replays with the
-// right seed every frame: tick_begin derives them from the frame, and every
-// seeded kernel reads the seed from there.
+// right seed every frame: the tick's first kernel derives them from the
+// frame (tpt_tick_begin_dev), and every seeded kernel reads the seed from
+// there.
 
 #include "tpt_internal.h"
 
@@ -19,25 +20,6 @@
 #include <vector>
 
 namespace {
-
-__device__ __forceinline__ uint32_t frame_seed_dev(uint64_t f) {
-    uint32_t v = (uint32_t)f * 0x9E3779B9u;
-    v = (v << 13) | (v >> 19);
-    return v ^ 0x85EBCA6Bu;
-}
-
-// The tick's seed and CheckStacking flag from frame `f` (the argument, or
-// the device counter when `from_device`), and the counter advanced.
-__global__ void tick_begin(uint32_t *__restrict__ counters, uint32_t lo, uint32_t hi, uint32_t from_device) {
-    if (threadIdx.x != 0) return;
-    uint64_t f = from_device ? (((uint64_t)counters[TPT_CTR_FRAME_HI] << 32) | counters[TPT_CTR_FRAME_LO])
-                             : (((uint64_t)hi << 32) | lo);
-    counters[TPT_CTR_SEED] = frame_seed_dev(f);
-    counters[TPT_CTR_STACK] = (f % 10u) == 0u ? 1u : 0u;
-    f += 1u;
-    counters[TPT_CTR_FRAME_LO] = (uint32_t)f;
-    counters[TPT_CTR_FRAME_HI] = (uint32_t)(f >> 32);
-}
 
 // A per-pass call's seed (and CheckStacking flag).
 __global__ void set_seed(uint32_t *__restrict__ counters, uint32_t seed, uint32_t stacking) {
@@ -60,11 +42,9 @@ int put_seed(tpt_ctx *ctx, uint32_t seed, int stacking) {
 // One tick.  `from_device`: take the frame from the device counter (a
 // captured tick); otherwise from `frame`.
 int tick(tpt_ctx *ctx, uint64_t frame, bool from_device) {
-    hipLaunchKernelGGL(tick_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, (uint32_t)frame,
-                       (uint32_t)(frame >> 32), from_device ? 1u : 0u);
-    int rc = tpt_check_launch("tick_begin");
-    if (rc) return rc;
-    if ((rc = tpt_air_step(ctx))) return rc;
+    const tpt_tick_begin tb{ctx->counters, (uint32_t)frame, (uint32_t)(frame >> 32), from_device ? 1u : 0u};
+    int rc;
+    if ((rc = tpt_air_step(ctx, &tb))) return rc;
     if ((rc = tpt_gravity_step(ctx, 0u))) return rc;
     if ((rc = tpt_recalc_step(ctx, !from_device && (frame % 10u) == 0u))) return rc;
     if ((rc = tpt_thermal_step(ctx, ctx->cfg.heat_enable ? 0 : 1))) return rc;

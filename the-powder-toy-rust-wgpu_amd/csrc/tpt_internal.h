@@ -177,7 +177,36 @@ template <typename T> static inline T *tpt_buf(tpt_ctx *ctx, int id) {
 }
 
 // Pass drivers implemented per translation unit.
-int tpt_air_step(tpt_ctx *ctx);
+// The start of a tick (frame counter, seed, CheckStacking flag in the
+// counter block), run by the first kernel of the tick's air pass.
+struct tpt_tick_begin {
+    uint32_t *counters;  // null: not a tick
+    uint32_t lo, hi;     // frame_count, when not taken from the device
+    uint32_t from_device;
+};
+
+__device__ __forceinline__ uint32_t tpt_frame_seed_dev(uint64_t f) {
+    uint32_t v = (uint32_t)f * 0x9E3779B9u;
+    v = (v << 13) | (v >> 19);
+    return v ^ 0x85EBCA6Bu;
+}
+
+// The tick's seed and CheckStacking flag from frame `f` (the argument, or
+// the device counter when `from_device`), and the counter advanced; one
+// thread.
+__device__ __forceinline__ void tpt_tick_begin_dev(const tpt_tick_begin &tb) {
+    uint32_t *c = tb.counters;
+    uint64_t f = tb.from_device ? (((uint64_t)c[TPT_CTR_FRAME_HI] << 32) | c[TPT_CTR_FRAME_LO])
+                                : (((uint64_t)tb.hi << 32) | tb.lo);
+    c[TPT_CTR_SEED] = tpt_frame_seed_dev(f);
+    c[TPT_CTR_STACK] = (f % 10u) == 0u ? 1u : 0u;
+    f += 1u;
+    c[TPT_CTR_FRAME_LO] = (uint32_t)f;
+    c[TPT_CTR_FRAME_HI] = (uint32_t)(f >> 32);
+}
+
+// air_kernel::step_air_gpu; `tb` set when the pass opens a tick.
+int tpt_air_step(tpt_ctx *ctx, const tpt_tick_begin *tb = nullptr);
 int tpt_gravity_step(tpt_ctx *ctx, uint32_t seed);
 int tpt_gravity_solve_only(tpt_ctx *ctx);
 int tpt_gravity_free(tpt_ctx *ctx);
