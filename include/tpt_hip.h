@@ -81,6 +81,9 @@ typedef struct tpt_config {
      * 1: extension -- every live particle deposits its element weight
      *    (GpuElementProps.weight) as mass.  No reference counterpart. */
     uint32_t grav_mass_mode;
+    /* SimulationConfig::water_equal_test (particle.rs:209): run the water
+     * equalisation pass after movement (lib.rs:418-420). */
+    uint32_t water_equal_test;
 } tpt_config;
 
 /* Device buffers, named after the SimBuffers fields they mirror. */
@@ -179,6 +182,12 @@ TPT_API int tpt_step_thermal(tpt_ctx *ctx, uint32_t seed, int legacy);
  * create_alloc/create_fields_kernel, reaction_kernel/shared.rs:516-912).
  * The pair pass needs TPT_BUF_SPAWN_DEFAULTS. */
 TPT_API int tpt_step_movement(tpt_ctx *ctx, uint32_t seed);
+
+/* water_equal_gpu::step_water_equal_gpu (crates/tpt-sim/src/water_equal_gpu/mod.rs:67-166):
+ * the communicating-vessel pass after movement, a no-op unless
+ * water_equal_test is set.  Reads the cell grids as the preceding movement
+ * pass left them. */
+TPT_API int tpt_step_water_equal(tpt_ctx *ctx, uint32_t seed);
 
 /* SimulationEngine::step (crates/tpt-sim/src/lib.rs:303-459) for the passes
  * above, with seed = rotl(frame_count*0x9E3779B9, 13) ^ 0x85EBCA6B

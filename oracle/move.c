@@ -1508,6 +1508,140 @@ void oracle_step_movement(oracle_state *s, uint32_t seed) {
     oracle_rebuild_maps(s);
 }
 
+/* ---- water equalisation (water_equal_gpu/, lib.rs:418-420) ----------------
+ *
+ * Canonical rule (shared with the HIP kernels):
+ *  WEQ the per-column target that several rows write (row.rs:135-139) keeps
+ *      the write of the highest-numbered row, as processing the rows in
+ *      increasing y does. */
+
+static int is_liquid_word(const oracle_state *s, uint32_t p) {
+    if (p == 0u) return 0;
+    uint32_t ty = p & 0x1FFu;
+    return ty < 256u && s->elem_props[ty * 32u + 3u] == 2u;
+}
+
+/* hash_u32-based 1-in-200 gate (intent.rs:46-49). */
+static int flood_chance(uint32_t frame_seed, uint32_t id) {
+    return oracle_hash_u32(frame_seed ^ (id * 0x9E3779B9u)) % 200u == 0u;
+}
+
+void oracle_water_equal(oracle_state *s, uint32_t seed) {
+    const uint32_t xres = s->xres, yres = s->yres, xc = xres / 4, P = xres * yres;
+    uint32_t *col_surf = malloc(sizeof(uint32_t) * xres), *col_target = calloc(xres, sizeof(uint32_t));
+    uint32_t *winner = calloc(P, sizeof(uint32_t));
+    mctx m = {s, (int)xres, (int)yres, xc, s->edge_mode, NULL};
+    /* Pass 1: column surfaces (column.rs:20-82). */
+    for (uint32_t x = 0; x < xres; ++x) {
+        int found = 0;
+        uint32_t min_y = yres, max_y = 0;
+        for (uint32_t y = 0; y < yres; ++y)
+            if (is_liquid_word(s, s->pmap[y * xres + x])) {
+                if (!found) {
+                    found = 1;
+                    min_y = y;
+                }
+                max_y = y;
+            }
+        if (found) {
+            int free_above = 0;
+            if (min_y > 0 && s->pmap[(min_y - 1) * xres + x] == 0u) {
+                uint32_t wc = ((min_y - 1) / 4) * xc + x / 4;
+                if (!wall_blocks_particle(s->wall_bmap[wc], s->wall_emap[wc], 0x2u)) free_above = 1;
+            }
+            col_surf[x] = (min_y << 16) | (free_above ? 1u << 15 : 0u) | (max_y & 0x7FFFu);
+        } else {
+            col_surf[x] = 0xFFFFFFFFu;
+        }
+    }
+    /* Pass 2: spans of liquid per row, in increasing y (row.rs:22-150). */
+    for (uint32_t y = 0; y < yres; ++y) {
+        uint32_t x = 0;
+        while (x < xres) {
+            if (!is_liquid_word(s, s->pmap[y * xres + x])) {
+                ++x;
+                continue;
+            }
+            uint32_t x_start = x, x_end = x;
+            for (uint32_t nx = x + 1; nx < xres; ++nx) {
+                uint32_t wc = (y / 4) * xc + nx / 4;
+                if (is_liquid_word(s, s->pmap[y * xres + nx]) &&
+                    !wall_blocks_particle(s->wall_bmap[wc], s->wall_emap[wc], 0x2u))
+                    x_end = nx;
+                else
+                    break;
+            }
+            uint32_t min_top = yres, max_free = 0, dmin = xres, dmax = 0;
+            for (uint32_t k = x_start; k <= x_end; ++k) {
+                uint32_t sf = col_surf[k];
+                if (sf == 0xFFFFFFFFu) continue;
+                uint32_t top = sf >> 16, bot = sf & 0x7FFFu;
+                int fr = (sf & (1u << 15)) != 0;
+                if (y >= top && y <= bot) {
+                    if (top < min_top) min_top = top;
+                    if (fr) {
+                        if (top > max_free) {
+                            max_free = top;
+                            dmin = dmax = k;
+                        } else if (top == max_free) {
+                            if (k < dmin) dmin = k;
+                            if (k > dmax) dmax = k;
+                        }
+                    }
+                }
+            }
+            if (max_free > min_top + 1u) {
+                uint32_t dest_row = max_free - 1u, span_len = dmax - dmin + 1u;
+                for (uint32_t k = x_start; k <= x_end; ++k) {
+                    uint32_t sf = col_surf[k];
+                    if (sf == 0xFFFFFFFFu) continue;
+                    if ((sf >> 16) < dest_row) {
+                        uint32_t dest_x = dmin + (k + seed) % span_len, dest = dest_row * xres + dest_x;
+                        col_target[k] = dest + 1u;
+                        winner[dest] = 0xFFFFFFFFu;
+                    }
+                }
+            }
+            x = x_end + 1u;
+        }
+    }
+    /* Pass 3: intents, the lowest slot taking a target (intent.rs:50-129). */
+    for (uint32_t id = 0; id < s->slot_bound; ++id) {
+        uint32_t ty = s->part_ty[id];
+        if (ty == 0u || ty >= 256u || s->elem_props[ty * 32u + 3u] != 2u) continue;
+        if (s->part_flags[id] & (0x10u | 0x02u)) continue;
+        if (!flood_chance(seed, id)) continue;
+        int xi = ora_f2i(s->part_x[id] + 0.5f), yi = ora_f2i(s->part_y[id] + 0.5f);
+        if (xi < 0 || xi >= (int)xres || yi < 0 || yi >= (int)yres) continue;
+        if ((s->pmap[(uint32_t)yi * xres + (uint32_t)xi] >> 9) != id + 1u) continue;
+        uint32_t packed = col_target[xi];
+        if (packed == 0u) continue;
+        uint32_t target = packed - 1u, dest_y = target / xres, dest_x = target % xres;
+        int can_enter = eval_move_at(&m, s->elem_props[ty * 32u + 1u], ty, (int)dest_x, (int)dest_y) != 0u;
+        if ((uint32_t)yi < dest_y && s->pmap[target] == 0u && can_enter && id + 1u < winner[target])
+            winner[target] = id + 1u;
+    }
+    /* Pass 4: commits (commit.rs:15-63). */
+    for (uint32_t id = 0; id < s->slot_bound; ++id) {
+        uint32_t ty = s->part_ty[id];
+        if (ty == 0u || ty >= 256u) continue;
+        int xi = ora_f2i(s->part_x[id] + 0.5f), yi = ora_f2i(s->part_y[id] + 0.5f);
+        if (xi < 0 || xi >= (int)xres || yi < 0 || yi >= (int)yres) continue;
+        uint32_t old_idx = (uint32_t)yi * xres + (uint32_t)xi, packed = col_target[xi];
+        if (packed == 0u) continue;
+        uint32_t target = packed - 1u, dest_y = target / xres;
+        if ((uint32_t)yi < dest_y && winner[target] == id + 1u) {
+            if ((s->pmap[old_idx] >> 9) == id + 1u) s->pmap[old_idx] = 0u;
+            s->pmap[target] = ((id + 1u) << 9) | (ty & 0x1FFu);
+            s->part_x[id] = (float)(target % xres);
+            s->part_y[id] = (float)dest_y;
+        }
+    }
+    free(col_surf);
+    free(col_target);
+    free(winner);
+}
+
 /* SimulationEngine::step (lib.rs:303-459) over the passes this build covers. */
 void oracle_step(oracle_state *s, uint64_t frame_count) {
     uint32_t v = (uint32_t)frame_count * 0x9E3779B9u;
@@ -1517,4 +1651,5 @@ void oracle_step(oracle_state *s, uint64_t frame_count) {
     oracle_recalc_free_particles(s, seed, (frame_count % 10u) == 0);
     oracle_step_thermal(s, seed, s->heat_enable ? 0 : 1);
     oracle_step_movement(s, seed);
+    if (s->water_equal_test) oracle_water_equal(s, seed);
 }

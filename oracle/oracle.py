@@ -23,7 +23,7 @@ _CFG = [
     ("heat_enable", ctypes.c_uint32), ("newton_grav", ctypes.c_uint32),
     ("edge_mode", ctypes.c_uint32), ("gravity_mode", ctypes.c_uint32),
     ("air_mode", ctypes.c_uint32), ("convection_mode", ctypes.c_uint32),
-    ("grav_mass_mode", ctypes.c_uint32),
+    ("grav_mass_mode", ctypes.c_uint32), ("water_equal_test", ctypes.c_uint32),
     ("ambient_air_temp", ctypes.c_float), ("edge_pressure", ctypes.c_float),
     ("edge_velocity_x", ctypes.c_float), ("edge_velocity_y", ctypes.c_float),
     ("custom_gravity_x", ctypes.c_float), ("custom_gravity_y", ctypes.c_float),
@@ -100,7 +100,8 @@ def lib():
         for name, extra in (("oracle_air_deposit", []), ("oracle_detect_charge", []), ("oracle_emap_closure", []),
                             ("oracle_velocity_intent", [ctypes.c_uint32]),
                             ("oracle_commit", []), ("oracle_rebuild_maps", []),
-                            ("oracle_step_movement", [ctypes.c_uint32]), ("oracle_step", [ctypes.c_uint64])):
+                            ("oracle_step_movement", [ctypes.c_uint32]), ("oracle_step", [ctypes.c_uint64]),
+                            ("oracle_water_equal", [ctypes.c_uint32])):
             f = getattr(_lib, name)
             f.argtypes = [ctypes.POINTER(OracleStateC)] + extra
             f.restype = None

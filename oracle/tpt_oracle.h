@@ -31,7 +31,7 @@ typedef struct oracle_state {
     /* config (SimulationConfig, crates/tpt-core/src/particle.rs:166-346) */
     uint32_t xres, yres, capacity, slot_bound;
     uint32_t aheat_enable, heat_enable, newton_grav;
-    uint32_t edge_mode, gravity_mode, air_mode, convection_mode, grav_mass_mode;
+    uint32_t edge_mode, gravity_mode, air_mode, convection_mode, grav_mass_mode, water_equal_test;
     float ambient_air_temp, edge_pressure, edge_velocity_x, edge_velocity_y;
     float custom_gravity_x, custom_gravity_y, vorticity_coeff;
     /* slot arrays */
@@ -102,6 +102,8 @@ void oracle_velocity_intent(oracle_state *s, uint32_t seed);
 void oracle_commit(oracle_state *s);
 void oracle_rebuild_maps(oracle_state *s);
 void oracle_step_movement(oracle_state *s, uint32_t seed);
+/* water_equal_gpu::step_water_equal_gpu (water_equal_gpu/mod.rs:67-166). */
+void oracle_water_equal(oracle_state *s, uint32_t seed);
 /* hash_u32 (reaction_kernel/shared.rs:1015-1023) and
  * Element_FILT_interactWavelengths (reaction_kernel/rays/emitter.rs:28-80),
  * used by the e == 2 pair pass; exposed for known-answer tests. */

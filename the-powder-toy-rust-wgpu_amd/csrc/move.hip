@@ -113,9 +113,8 @@ __device__ __forceinline__ bool wall_kills_particle(uint32_t wall, uint32_t emap
 __device__ __forceinline__ int wcell(const MoveEnv &e, int x, int y) { return (y / TPT_CELL) * e.xc + x / TPT_CELL; }
 
 // eval_move_at (mod.rs:838-891).
-__device__ uint32_t eval_move_at(const MoveEnv &e, uint32_t props, uint32_t ty, int nx, int ny) {
-    if (!(nx >= 0 && nx < e.xr && ny >= 0 && ny < e.yr)) return 0u;
-    const uint32_t d_ty = (uint32_t)e.tmap[ny * e.xr + nx];
+// The body of eval_move_at for an occupant type the caller supplies.
+__device__ uint32_t eval_move_ty(const MoveEnv &e, uint32_t props, uint32_t ty, int nx, int ny, uint32_t d_ty) {
     const int wc = wcell(e, nx, ny);
     const uint32_t wall = e.cellrec[wc].z, em = e.cellrec[wc].w;
     uint32_t can = e.lut[LUT_CAN + ty * 256u + d_ty];
@@ -134,6 +133,11 @@ __device__ uint32_t eval_move_at(const MoveEnv &e, uint32_t props, uint32_t ty, 
         }
     }
     return result;
+}
+
+__device__ uint32_t eval_move_at(const MoveEnv &e, uint32_t props, uint32_t ty, int nx, int ny) {
+    if (!(nx >= 0 && nx < e.xr && ny >= 0 && ny < e.yr)) return 0u;
+    return eval_move_ty(e, props, ty, nx, ny, (uint32_t)e.tmap[ny * e.xr + nx]);
 }
 
 __device__ __forceinline__ float remainder_p(float x, float y) {
@@ -1749,7 +1753,265 @@ __global__ __launch_bounds__(DRAIN_BLK) void mv_after_commit(Slots s, CommitBufs
     drain_creates(s, d, counters, rec_cap, xres, yres);
 }
 
+// ---- water equalisation (water_equal_gpu/, lib.rs:418-420) --------------
+//
+// The reference scans each column and each row with one thread; here the
+// column pass folds 64-row chunks into per-column atomics, and each row is
+// one workgroup that finds its liquid spans with a block-wide max-scan and
+// reduces them in LDS.  Canonical rule WEQ: a column target that several
+// rows assign keeps the highest row's (64-bit atomicMax on (row, target)).
+
+constexpr int WEQ_CHUNK = 64;            // rows per column-pass thread
+constexpr int WEQ_MAX_X = (int)TPT_MAX_DIM;
+constexpr int WEQ_BLK = 256;
+constexpr int WEQ_PER = (WEQ_MAX_X + WEQ_BLK - 1) / WEQ_BLK;  // columns per thread in a row pass
+
+__device__ __forceinline__ bool weq_liquid(const uint32_t *__restrict__ props, uint32_t p) {
+    const uint32_t ty = p & 0x1FFu;
+    return p != 0u && ty < 256u && props[ty * 32u + 3u] == 2u;
+}
+
+// Column pass, part 1: the first and last liquid row of each (column,
+// 64-row chunk), folded into the column's min / max (max stored as row + 1).
+__global__ __launch_bounds__(64) void weq_column_chunks(const uint32_t *__restrict__ pmap,
+                                                        const uint32_t *__restrict__ props,
+                                                        uint32_t *__restrict__ col_min, uint32_t *__restrict__ col_max,
+                                                        int xres, int yres) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    if (x >= xres) return;
+    const int y0 = blockIdx.y * WEQ_CHUNK, y1 = min(y0 + WEQ_CHUNK, yres);
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    for (int y = y0; y < y1; ++y)
+        if (weq_liquid(props, pmap[y * xres + x])) {
+            if (lo == 0xFFFFFFFFu) lo = (uint32_t)y;
+            hi = (uint32_t)y + 1u;
+        }
+    if (hi) {
+        atomicMin(&col_min[x], lo);
+        atomicMax(&col_max[x], hi);
+    }
+}
+
+// Column pass, part 2 (water_column_analyze_kernel, column.rs:20-82): the
+// column's surface word, the target and accumulators reset for the next use.
+__global__ void weq_column_surfaces(const uint32_t *__restrict__ pmap, const uint4 *__restrict__ cellrec,
+                                    uint32_t *__restrict__ col_min, uint32_t *__restrict__ col_max,
+                                    uint32_t *__restrict__ col_surf, unsigned long long *__restrict__ col_target,
+                                    uint32_t *__restrict__ counters, int xres, int xc) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x == 0) counters[TPT_CTR_WEQ] = 0u;
+    if (x >= xres) return;
+    const uint32_t lo = col_min[x], hi = col_max[x];
+    col_min[x] = 0xFFFFFFFFu;
+    col_max[x] = 0u;
+    col_target[x] = 0ull;
+    if (hi == 0u) {
+        col_surf[x] = 0xFFFFFFFFu;
+        return;
+    }
+    bool free_above = false;
+    if (lo > 0u && pmap[(lo - 1u) * (uint32_t)xres + (uint32_t)x] == 0u) {
+        const uint4 r = cellrec[((lo - 1u) / 4u) * (uint32_t)xc + (uint32_t)x / 4u];
+        free_above = !wall_blocks_particle(r.z, r.w, 0x2u);
+    }
+    col_surf[x] = (lo << 16) | (free_above ? 1u << 15 : 0u) | ((hi - 1u) & 0x7FFFu);
+}
+
+// Row pass (water_row_connect_kernel, row.rs:22-150), one workgroup per row.
+__global__ __launch_bounds__(WEQ_BLK) void weq_rows(const uint32_t *__restrict__ pmap,
+                                                    const uint32_t *__restrict__ props,
+                                                    const uint4 *__restrict__ cellrec,
+                                                    const uint32_t *__restrict__ col_surf,
+                                                    unsigned long long *__restrict__ col_target,
+                                                    uint32_t *__restrict__ winner, int xres, int yres, int xc,
+                                                    const uint32_t *__restrict__ seedp) {
+    __shared__ uint16_t start[WEQ_MAX_X];  // span start of each liquid column, 0xFFFF elsewhere
+    __shared__ uint32_t min_top[WEQ_MAX_X], max_free[WEQ_MAX_X], dmin[WEQ_MAX_X], dmax[WEQ_MAX_X];
+    __shared__ uint32_t carry[WEQ_BLK];
+    const int y = blockIdx.x, t = threadIdx.x, x0 = t * WEQ_PER;
+    const uint32_t seed = *seedp;
+    // Spans: a liquid pixel continues its left neighbour's span when that
+    // neighbour is liquid and its own cell does not block liquids.
+    bool liq[WEQ_PER];
+    int last_start = -1;
+    bool prev = x0 > 0 && x0 - 1 < xres && weq_liquid(props, pmap[y * xres + x0 - 1]);
+#pragma unroll
+    for (int u = 0; u < WEQ_PER; ++u) {
+        const int x = x0 + u;
+        liq[u] = x < xres && weq_liquid(props, pmap[y * xres + x]);
+        if (liq[u]) {
+            const uint4 r = cellrec[(y / 4) * xc + x / 4];
+            const bool link = prev && !wall_blocks_particle(r.z, r.w, 0x2u);
+            if (!link) last_start = x;
+        }
+        prev = liq[u];
+        if (x < xres) {
+            min_top[x] = (uint32_t)yres;
+            max_free[x] = 0u;
+            dmin[x] = (uint32_t)xres;
+            dmax[x] = 0u;
+        }
+    }
+    // Inclusive max-scan of the chunks' last span starts (Hillis-Steele).
+    carry[t] = (uint32_t)(last_start + 1);
+    __syncthreads();
+    for (int off = 1; off < WEQ_BLK; off <<= 1) {
+        const uint32_t v = t >= off ? carry[t - off] : 0u;
+        __syncthreads();
+        if (v > carry[t]) carry[t] = v;
+        __syncthreads();
+    }
+    int cur = t > 0 ? (int)carry[t - 1] - 1 : -1;
+    prev = x0 > 0 && x0 - 1 < xres && weq_liquid(props, pmap[y * xres + x0 - 1]);
+#pragma unroll
+    for (int u = 0; u < WEQ_PER; ++u) {
+        const int x = x0 + u;
+        if (x >= xres) break;
+        if (liq[u]) {
+            const uint4 r = cellrec[(y / 4) * xc + x / 4];
+            if (!(prev && !wall_blocks_particle(r.z, r.w, 0x2u))) cur = x;
+            start[x] = (uint16_t)cur;
+        } else {
+            start[x] = 0xFFFFu;
+        }
+        prev = liq[u];
+    }
+    __syncthreads();
+    // Per span: the highest surface, then the lowest free surface and the
+    // columns that have it.
+#pragma unroll
+    for (int u = 0; u < WEQ_PER; ++u) {
+        const int k = x0 + u;
+        if (k >= xres || start[k] == 0xFFFFu) continue;
+        const uint32_t sf = col_surf[k];
+        if (sf == 0xFFFFFFFFu) continue;
+        const uint32_t top = sf >> 16, bot = sf & 0x7FFFu;
+        if ((uint32_t)y >= top && (uint32_t)y <= bot) {
+            atomicMin(&min_top[start[k]], top);
+            if (sf & (1u << 15)) atomicMax(&max_free[start[k]], top);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < WEQ_PER; ++u) {
+        const int k = x0 + u;
+        if (k >= xres || start[k] == 0xFFFFu) continue;
+        const uint32_t sf = col_surf[k];
+        if (sf == 0xFFFFFFFFu) continue;
+        const uint32_t top = sf >> 16, bot = sf & 0x7FFFu, s0 = start[k];
+        if ((uint32_t)y >= top && (uint32_t)y <= bot && (sf & (1u << 15)) && top == max_free[s0]) {
+            atomicMin(&dmin[s0], (uint32_t)k);
+            atomicMax(&dmax[s0], (uint32_t)k);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < WEQ_PER; ++u) {
+        const int k = x0 + u;
+        if (k >= xres || start[k] == 0xFFFFu) continue;
+        const uint32_t s0 = start[k], mf = max_free[s0];
+        if (!(mf > min_top[s0] + 1u)) continue;
+        const uint32_t sf = col_surf[k];
+        if (sf == 0xFFFFFFFFu) continue;
+        const uint32_t dest_row = mf - 1u;
+        if ((sf >> 16) >= dest_row) continue;
+        const uint32_t span_len = dmax[s0] - dmin[s0] + 1u;
+        const uint32_t dest_x = dmin[s0] + ((uint32_t)k + seed) % span_len;
+        const uint32_t dest = dest_row * (uint32_t)xres + dest_x;
+        atomicMax(&col_target[k], ((unsigned long long)(y + 1) << 32) | (unsigned long long)(dest + 1u));
+        winner[dest] = 0xFFFFFFFFu;
+    }
+}
+
+// The 1-in-200 gate (flood_chance, intent.rs:46-49).
+__device__ __forceinline__ bool weq_chance(uint32_t frame_seed, uint32_t id) {
+    return fmix32(frame_seed ^ (id * 0x9E3779B9u)) % 200u == 0u;
+}
+
+// water_intent_kernel (intent.rs:50-129): eligible liquids bid for their
+// column's target with the lowest slot winning; bidders are listed for the
+// commit.
+__global__ __launch_bounds__(BLK) void weq_intent(Slots s, MoveEnv e, const uint32_t *__restrict__ props,
+                                                  const unsigned long long *__restrict__ col_target,
+                                                  uint32_t *__restrict__ winner, uint32_t *__restrict__ list,
+                                                  uint32_t *__restrict__ counters, uint32_t slots,
+                                                  const uint32_t *__restrict__ seedp) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= slots) return;
+    const uint32_t ty = s.ty[id];
+    if (ty == 0u || ty >= 256u || props[ty * 32u + 3u] != 2u) return;
+    if (s.flags[id] & (0x10u | 0x02u)) return;
+    if (!weq_chance(*seedp, id)) return;
+    const int xi = tpt_f2i(s.x[id] + 0.5f), yi = tpt_f2i(s.y[id] + 0.5f);
+    if (xi < 0 || xi >= e.xr || yi < 0 || yi >= e.yr) return;
+    if ((e.pmap[yi * e.xr + xi] >> 9) != id + 1u) return;
+    const uint32_t packed = (uint32_t)col_target[xi];
+    if (packed == 0u) return;
+    const uint32_t target = packed - 1u, dest_y = target / (uint32_t)e.xr, dest_x = target % (uint32_t)e.xr;
+    const uint32_t occ = e.pmap[target];
+    const bool can_enter = eval_move_ty(e, props[ty * 32u + 1u], ty, (int)dest_x, (int)dest_y, occ & 0x1FFu) != 0u;
+    if ((uint32_t)yi < dest_y && occ == 0u && can_enter) {
+        atomicMin(&winner[target], id + 1u);
+        list[atomicAdd(&counters[TPT_CTR_WEQ], 1u)] = id;
+    }
+}
+
+// water_commit_kernel (commit.rs:15-63) over the bidders: only a bidder
+// can hold a target.  Targets are empty pixels, so no winner's old pixel is
+// another's target.
+__global__ __launch_bounds__(BLK) void weq_commit(Slots s, uint32_t *__restrict__ pmap,
+                                                  const unsigned long long *__restrict__ col_target,
+                                                  const uint32_t *__restrict__ winner,
+                                                  const uint32_t *__restrict__ list,
+                                                  const uint32_t *__restrict__ counters, int xres) {
+    const uint32_t n = counters[TPT_CTR_WEQ];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t id = list[k], ty = s.ty[id];
+        const int xi = tpt_f2i(s.x[id] + 0.5f), yi = tpt_f2i(s.y[id] + 0.5f);
+        const uint32_t target = (uint32_t)col_target[xi] - 1u, dest_y = target / (uint32_t)xres;
+        if (winner[target] != id + 1u) continue;
+        const uint32_t old_idx = (uint32_t)yi * (uint32_t)xres + (uint32_t)xi;
+        if ((pmap[old_idx] >> 9) == id + 1u) pmap[old_idx] = 0u;
+        pmap[target] = ((id + 1u) << 9) | (ty & 0x1FFu);
+        s.x[id] = (float)(target % (uint32_t)xres);
+        s.y[id] = (float)dest_y;
+    }
+}
+
 }  // namespace
+
+int tpt_water_equal_step(tpt_ctx *ctx) {
+    const tpt_config &c = ctx->cfg;
+    const int xres = (int)c.xres, yres = (int)c.yres, xc = (int)ctx->xcells;
+    const uint32_t slots = ctx->slot_bound;
+    uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP);
+    const uint32_t *props = tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS);
+    const uint32_t *seed = ctx->counters + TPT_CTR_SEED;
+    tpt_water_bufs &w = ctx->water;
+    // The pass moves particles and rewrites pmap: the movement rebuild's
+    // pixel records no longer describe the state.
+    ctx->maps_fresh = ctx->rebuild_fresh = ctx->rec_cover = false;
+    tpt_pass_begin(ctx, PASS_WATER_EQUAL);
+    hipLaunchKernelGGL(weq_column_chunks, dim3((xres + 63) / 64, (yres + WEQ_CHUNK - 1) / WEQ_CHUNK), dim3(64), 0,
+                       ctx->stream, pmap, props, w.col_min, w.col_max, xres, yres);
+    hipLaunchKernelGGL(weq_column_surfaces, dim3((xres + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, pmap,
+                       ctx->cell_rec, w.col_min, w.col_max, w.col_surf, w.col_target, ctx->counters, xres, xc);
+    hipLaunchKernelGGL(weq_rows, dim3(yres), dim3(WEQ_BLK), 0, ctx->stream, pmap, props, ctx->cell_rec, w.col_surf,
+                       w.col_target, w.winner, xres, yres, xc, seed);
+    Slots s{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<float>(ctx, TPT_BUF_PART_X),
+            tpt_buf<float>(ctx, TPT_BUF_PART_Y),   tpt_buf<float>(ctx, TPT_BUF_PART_VX),
+            tpt_buf<float>(ctx, TPT_BUF_PART_VY),  tpt_buf<uint32_t>(ctx, TPT_BUF_PART_FLAGS),
+            tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP), tpt_buf<uint32_t>(ctx, TPT_BUF_INTENT_TARGET)};
+    MoveEnv e{pmap, ctx->pix_type, ctx->cell_rec, tpt_buf<uint32_t>(ctx, TPT_BUF_MOVE_LUT), xres, yres, xc,
+              c.edge_mode, c.gravity_mode, c.custom_gravity_x, c.custom_gravity_y, c.newton_grav, nullptr, nullptr,
+              nullptr, nullptr, nullptr};
+    hipLaunchKernelGGL(weq_intent, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, s, e, props, w.col_target,
+                       w.winner, ctx->slot_scratch_u, ctx->counters, slots, seed);
+    hipLaunchKernelGGL(weq_commit, dim3(64), dim3(BLK), 0, ctx->stream, s, pmap, w.col_target, w.winner,
+                       ctx->slot_scratch_u, ctx->counters, xres);
+    tpt_pass_end(ctx, PASS_WATER_EQUAL);
+    return tpt_check_launch("water equalisation");
+}
 
 int tpt_movement_step(tpt_ctx *ctx) {
     const uint32_t *seed = ctx->counters + TPT_CTR_SEED;

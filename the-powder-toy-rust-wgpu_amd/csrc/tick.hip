@@ -4,7 +4,8 @@
 // tpt_step follows SimulationEngine::step (crates/tpt-sim/src/lib.rs:303-459)
 // for the passes this library implements: air -> Newtonian gravity (or the
 // force-grid clear) -> RecalcFreeParticles (+ CheckStacking every 10th
-// frame) -> conduction/transitions -> movement.  The element reactions
+// frame) -> conduction/transitions -> movement -> water equalisation (when
+// enabled).  The element reactions
 // (reaction_kernel::step_reactions_gpu, lib.rs:379) are not part of this
 // library.
 //
@@ -48,7 +49,8 @@ int tick(tpt_ctx *ctx, uint64_t frame, bool from_device) {
     if ((rc = tpt_gravity_step(ctx, 0u))) return rc;
     if ((rc = tpt_recalc_step(ctx, !from_device && (frame % 10u) == 0u))) return rc;
     if ((rc = tpt_thermal_step(ctx, ctx->cfg.heat_enable ? 0 : 1))) return rc;
-    return tpt_movement_step(ctx);
+    if ((rc = tpt_movement_step(ctx))) return rc;
+    return ctx->cfg.water_equal_test ? tpt_water_equal_step(ctx) : TPT_OK;
 }
 
 // Records the steady-state tick of `ctx` on its stream, which is already
@@ -113,6 +115,13 @@ extern "C" int tpt_step_movement(tpt_ctx *ctx, uint32_t seed) {
     int rc = require(ctx);
     if (!rc) rc = put_seed(ctx, seed, 0);
     return rc ? rc : tpt_movement_step(ctx);
+}
+
+extern "C" int tpt_step_water_equal(tpt_ctx *ctx, uint32_t seed) {
+    int rc = require(ctx);
+    if (rc || !ctx->cfg.water_equal_test) return rc;
+    if ((rc = put_seed(ctx, seed, 0))) return rc;
+    return tpt_water_equal_step(ctx);
 }
 
 extern "C" int tpt_step(tpt_ctx *ctx, uint64_t frame_count) {

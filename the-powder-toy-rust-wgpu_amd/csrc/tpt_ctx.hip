@@ -20,7 +20,7 @@ const char *const tpt_pass_names[PASS_COUNT] = {
     "thermal_conduction", "thermal_conduction_cells", "thermal_apply", "phase_transition",
     "phase_transition_cells", "refresh_pmap_types", "air_deposit", "air_deposit_cells", "move_detect",
     "move_prepare", "move_velocity_intent", "move_saw_pair", "move_commit", "move_photon_hits_drain",
-    "move_rebuild_maps",
+    "move_rebuild_maps", "water_equal",
 };
 
 static thread_local char g_err[512];
@@ -101,8 +101,15 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMalloc(&c->pix_count_mv, (size_t)c->pixels * 4) != hipSuccess ||
         hipMalloc(&c->slot_owner, cap * 4) != hipSuccess ||
         hipMalloc(&c->create_rec, (cap * TPT_CREATE_REC_WORDS + TPT_SPAWN_BITMAP_WORDS) * 4) != hipSuccess ||
-        hipMalloc(&c->cell_rec, (size_t)c->cells * sizeof(uint4)) != hipSuccess)
+        hipMalloc(&c->cell_rec, (size_t)c->cells * sizeof(uint4)) != hipSuccess ||
+        hipMalloc(&c->water.col_min, (size_t)c->cfg.xres * 4) != hipSuccess ||
+        hipMalloc(&c->water.col_max, (size_t)c->cfg.xres * 4) != hipSuccess ||
+        hipMalloc(&c->water.col_surf, (size_t)c->cfg.xres * 4) != hipSuccess ||
+        hipMalloc(&c->water.col_target, (size_t)c->cfg.xres * 8) != hipSuccess ||
+        hipMalloc(&c->water.winner, (size_t)c->pixels * 4) != hipSuccess)
         return tpt_set_error(TPT_ENOMEM, "scratch allocation failed");
+    hipMemsetAsync(c->water.col_min, 0xFF, (size_t)c->cfg.xres * 4, c->stream);
+    hipMemsetAsync(c->water.col_max, 0, (size_t)c->cfg.xres * 4, c->stream);
     hipMemsetAsync(c->cell_scratch_u, 0, cb, c->stream);
     hipMemsetAsync(c->cell_scratch_u2, 0, cb, c->stream);
     hipMemsetAsync(c->counters, 0, 256, c->stream);
@@ -116,7 +123,9 @@ static void free_all(tpt_ctx *c) {
         if (c->buf[id]) hipFree(c->buf[id]);
     void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
                      c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters,
-                     c->pix_rec, c->pix_heat, c->offmap, c->pix_type, c->pix_count_mv, c->slot_owner, c->create_rec, c->cell_rec};
+                     c->pix_rec, c->pix_heat, c->offmap, c->pix_type, c->pix_count_mv, c->slot_owner, c->create_rec, c->cell_rec,
+                     c->water.col_min, c->water.col_max, c->water.col_surf, c->water.col_target,
+                     c->water.winner};
     for (void *p : extra)
         if (p) hipFree(p);
     tpt_gravity_free(c);
@@ -140,7 +149,7 @@ extern "C" int tpt_create(const tpt_config *cfg, tpt_ctx **out) {
     if (cfg->capacity == 0 || cfg->capacity > TPT_MAX_CAPACITY)
         return tpt_set_error(TPT_EINVAL, "capacity %u outside [1, 0x7FFFFF]", cfg->capacity);
     if (cfg->edge_mode > 2 || cfg->gravity_mode > 3 || cfg->air_mode > 4 || cfg->convection_mode > 2 ||
-        cfg->grav_mass_mode > 1)
+        cfg->grav_mass_mode > 1 || cfg->water_equal_test > 1)
         return tpt_set_error(TPT_EINVAL, "enum field out of range");
     tpt_ctx *c = new (std::nothrow) tpt_ctx();
     if (!c) return tpt_set_error(TPT_ENOMEM, "host allocation failed");

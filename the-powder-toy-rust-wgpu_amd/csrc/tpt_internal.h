@@ -25,6 +25,7 @@
 #define TPT_CTR_PHOT 4       // blocked-photon hit list length (move.hip)
 #define TPT_CTR_PAIR 5       // e == 2 pair candidate list length (move.hip)
 #define TPT_CTR_CREATE 6     // e == 2 creation records queued this tick (move.hip)
+#define TPT_CTR_WEQ 7        // water equalisation bidder list length (move.hip)
 #define TPT_CTR_FRAME_LO 8   // next frame_count, low / high word (tick.hip)
 #define TPT_CTR_FRAME_HI 9
 #define TPT_CTR_SEED 10      // seed of the pass(es) running now
@@ -60,6 +61,7 @@ enum tpt_pass {
     PASS_MOVE_COMMIT,     // winner check + move/swap
     PASS_MOVE_DRAIN,      // blocked-photon hits + creations queued by the pair pass
     PASS_MOVE_REBUILD,    // pmap/photons rebuild
+    PASS_WATER_EQUAL,     // water equalisation (when enabled)
     PASS_COUNT
 };
 
@@ -75,6 +77,14 @@ struct tpt_fft_state {
     float2 *kernel_y = nullptr;
     float2 *twiddle = nullptr;        // exp(-2 pi i k / N), N = max(width,height)
     bool kernels_ready = false;
+};
+
+// Scratch of the water equalisation pass.
+struct tpt_water_bufs {
+    uint32_t *col_min = nullptr, *col_max = nullptr;  // xres: chunk accumulators, kept reset
+    uint32_t *col_surf = nullptr;                      // xres
+    unsigned long long *col_target = nullptr;          // xres: (row + 1) << 32 | (target pixel + 1)
+    uint32_t *winner = nullptr;                        // pixels
 };
 
 struct tpt_ctx {
@@ -143,6 +153,7 @@ struct tpt_ctx {
     // transitions, then {air_vx, air_vy, wall_bmap, wall_emap} for the
     // movement planner (rebuilt by each pass before use).
     uint4 *cell_rec = nullptr;
+    tpt_water_bufs water;
     tpt_fft_state fft;
 
     // Profiling: a ring of event pairs per pass, folded lazily so that
@@ -214,6 +225,7 @@ int tpt_recalc_step(tpt_ctx *ctx, int stacking);
 int tpt_thermal_step(tpt_ctx *ctx, int legacy);
 int tpt_movement_step(tpt_ctx *ctx);
 int tpt_move_rebuild_maps(tpt_ctx *ctx);
+int tpt_water_equal_step(tpt_ctx *ctx);
 
 // Device helpers ------------------------------------------------------------
 __device__ __forceinline__ float tpt_clampf(float v, float lo, float hi) {

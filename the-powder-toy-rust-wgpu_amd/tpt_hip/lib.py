@@ -33,6 +33,7 @@ class TptConfig(ctypes.Structure):
         ("edge_velocity_y", ctypes.c_float), ("custom_gravity_x", ctypes.c_float),
         ("custom_gravity_y", ctypes.c_float), ("vorticity_coeff", ctypes.c_float),
         ("grav_mass_mode", ctypes.c_uint32),
+        ("water_equal_test", ctypes.c_uint32),
     ]
 
 
@@ -53,6 +54,7 @@ SIGNATURES = {
     "tpt_gravity_solve": ([_vp], ctypes.c_int),
     "tpt_recalc_free_particles": ([_vp, ctypes.c_uint32, ctypes.c_int], ctypes.c_int),
     "tpt_step_thermal": ([_vp, ctypes.c_uint32, ctypes.c_int], ctypes.c_int),
+    "tpt_step_water_equal": ([_vp, ctypes.c_uint32], ctypes.c_int),
     "tpt_step_movement": ([_vp, ctypes.c_uint32], ctypes.c_int),
     "tpt_step": ([_vp, ctypes.c_uint64], ctypes.c_int),
     "tpt_run": ([_vp, ctypes.c_uint64, ctypes.c_uint32], ctypes.c_int),
@@ -204,6 +206,9 @@ class Context:
 
     def step_movement(self, seed):
         _check(load().tpt_step_movement(self._h, seed))
+
+    def step_water_equal(self, seed):
+        _check(load().tpt_step_water_equal(self._h, seed))
 
     def step(self, frame_count):
         _check(load().tpt_step(self._h, frame_count))

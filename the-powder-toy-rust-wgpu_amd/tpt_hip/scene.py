@@ -63,7 +63,7 @@ def default_config(xres, yres, capacity=None):
         "edge_mode": 0, "gravity_mode": 0, "air_mode": 0, "convection_mode": 2,
         "ambient_air_temp": 295.15, "edge_pressure": 0.0, "edge_velocity_x": 0.0,
         "edge_velocity_y": 0.0, "custom_gravity_x": 0.0, "custom_gravity_y": 1.0,
-        "vorticity_coeff": 0.1, "grav_mass_mode": 0,
+        "vorticity_coeff": 0.1, "grav_mass_mode": 0, "water_equal_test": 0,
     }
 
 
