@@ -178,6 +178,8 @@ def main():
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     ap.add_argument("--batched", type=int, default=0, metavar="N",
                     help="the batched form instead: N mixed-size canvases, graph replay vs per-canvas launches")
+    ap.add_argument("--water-equal", action="store_true",
+                    help="run the workload with water_equal_test set (the optional pass after movement)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -198,6 +200,8 @@ def main():
     from tpt_hip import scene as S
 
     st = S.make_scene(args.workload)
+    if args.water_equal:
+        st.cfg["water_equal_test"] = 1
     C, P = st.xcells * st.ycells, st.cfg["xres"] * st.cfg["yres"]
     N = int((st.arrays["part_ty"] != 0).sum())
     S_ = st.slot_bound
@@ -299,7 +303,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload} plain tick, passes: {'+'.join(passes)}",
+        "config": {"workload": f"{args.workload} plain tick{' with water equalisation' if args.water_equal else ''}, "
+                               f"passes: {'+'.join(passes)}",
                    "air_cells": C, "pixels": P, "particles": N, "parallelism": "single-gpu"},
         "roofline": roofline,
         "cpu_baseline": cpu,

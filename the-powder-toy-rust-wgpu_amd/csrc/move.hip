@@ -1781,11 +1781,18 @@ __global__ __launch_bounds__(64) void weq_column_chunks(const uint32_t *__restri
     if (x >= xres) return;
     const int y0 = blockIdx.y * WEQ_CHUNK, y1 = min(y0 + WEQ_CHUNK, yres);
     uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-    for (int y = y0; y < y1; ++y)
-        if (weq_liquid(props, pmap[y * xres + x])) {
-            if (lo == 0xFFFFFFFFu) lo = (uint32_t)y;
-            hi = (uint32_t)y + 1u;
-        }
+    // Sixteen rows' loads in flight at a time.
+    for (int yb = y0; yb < y1; yb += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = yb + u < y1 ? pmap[(yb + u) * xres + x] : 0u;
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (weq_liquid(props, v[u])) {
+                if (lo == 0xFFFFFFFFu) lo = (uint32_t)(yb + u);
+                hi = (uint32_t)(yb + u) + 1u;
+            }
+    }
     if (hi) {
         atomicMin(&col_min[x], lo);
         atomicMax(&col_max[x], hi);
