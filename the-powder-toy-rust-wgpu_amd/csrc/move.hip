@@ -1766,10 +1766,28 @@ constexpr int WEQ_MAX_X = (int)TPT_MAX_DIM;
 constexpr int WEQ_BLK = 256;
 constexpr int WEQ_PER = (WEQ_MAX_X + WEQ_BLK - 1) / WEQ_BLK;  // columns per thread in a row pass
 
-__device__ __forceinline__ bool weq_liquid(const uint32_t *__restrict__ props, uint32_t p) {
-    const uint32_t ty = p & 0x1FFu;
-    return p != 0u && ty < 256u && props[ty * 32u + 3u] == 2u;
-}
+// Whether a pmap word holds a liquid (element state 2, props word 3), off a
+// 256-bit table of liquid types that every wave builds with four ballots, so
+// the per-pixel test needs no dependent load.  Every lane of the wave must
+// construct it.
+struct WeqLiquidTable {
+    unsigned long long w0, w1, w2, w3;  // separate values: an array would be indexed through scratch
+    __device__ explicit WeqLiquidTable(const uint32_t *__restrict__ props) {
+        const uint32_t lane = threadIdx.x & 63u;
+        w0 = __ballot(props[lane * 32u + 3u] == 2u);
+        w1 = __ballot(props[(64u + lane) * 32u + 3u] == 2u);
+        w2 = __ballot(props[(128u + lane) * 32u + 3u] == 2u);
+        w3 = __ballot(props[(192u + lane) * 32u + 3u] == 2u);
+    }
+    __device__ bool operator()(uint32_t p) const {
+        const uint32_t ty = p & 0x1FFu;
+        if (p == 0u || ty >= 256u) return false;
+        const uint32_t q = ty >> 6;
+        const unsigned long long word = (w0 & (0ull - (q == 0u))) | (w1 & (0ull - (q == 1u))) |
+                                        (w2 & (0ull - (q == 2u))) | (w3 & (0ull - (q == 3u)));
+        return (word >> (ty & 63u)) & 1ull;
+    }
+};
 
 // Column pass, part 1: the first and last liquid row of each (column,
 // 64-row chunk), folded into the column's min / max (max stored as row + 1).
@@ -1777,6 +1795,7 @@ __global__ __launch_bounds__(64) void weq_column_chunks(const uint32_t *__restri
                                                         const uint32_t *__restrict__ props,
                                                         uint32_t *__restrict__ col_min, uint32_t *__restrict__ col_max,
                                                         int xres, int yres) {
+    const WeqLiquidTable liquid(props);
     const int x = blockIdx.x * 64 + threadIdx.x;
     if (x >= xres) return;
     const int y0 = blockIdx.y * WEQ_CHUNK, y1 = min(y0 + WEQ_CHUNK, yres);
@@ -1788,7 +1807,7 @@ __global__ __launch_bounds__(64) void weq_column_chunks(const uint32_t *__restri
         for (int u = 0; u < 16; ++u) v[u] = yb + u < y1 ? pmap[(yb + u) * xres + x] : 0u;
 #pragma unroll
         for (int u = 0; u < 16; ++u)
-            if (weq_liquid(props, v[u])) {
+            if (liquid(v[u])) {
                 if (lo == 0xFFFFFFFFu) lo = (uint32_t)(yb + u);
                 hi = (uint32_t)(yb + u) + 1u;
             }
@@ -1825,6 +1844,15 @@ __global__ void weq_column_surfaces(const uint32_t *__restrict__ pmap, const uin
 }
 
 // Row pass (water_row_connect_kernel, row.rs:22-150), one workgroup per row.
+// Lane t of wave v owns pixels x = 256 u + 64 v + t, so each (wave, u) pair
+// holds one 64-pixel word of the row: a ballot gives the word's liquid and
+// linkable bits, one wave turns them into span-start bits and an exclusive
+// max-scan of the words' last starts, and every liquid pixel reads its span
+// start off those.  The span reductions are LDS atomics indexed by the
+// span's start column; only start columns are initialised.
+constexpr int WEQ_WORDS = WEQ_PER * (WEQ_BLK / 64);
+static_assert(WEQ_WORDS <= 64, "one wave scans the row's words");
+
 __global__ __launch_bounds__(WEQ_BLK) void weq_rows(const uint32_t *__restrict__ pmap,
                                                     const uint32_t *__restrict__ props,
                                                     const uint4 *__restrict__ cellrec,
@@ -1832,96 +1860,108 @@ __global__ __launch_bounds__(WEQ_BLK) void weq_rows(const uint32_t *__restrict__
                                                     unsigned long long *__restrict__ col_target,
                                                     uint32_t *__restrict__ winner, int xres, int yres, int xc,
                                                     const uint32_t *__restrict__ seedp) {
-    __shared__ uint16_t start[WEQ_MAX_X];  // span start of each liquid column, 0xFFFF elsewhere
-    __shared__ uint32_t min_top[WEQ_MAX_X], max_free[WEQ_MAX_X], dmin[WEQ_MAX_X], dmax[WEQ_MAX_X];
-    __shared__ uint32_t carry[WEQ_BLK];
-    const int y = blockIdx.x, t = threadIdx.x, x0 = t * WEQ_PER;
-    const uint32_t seed = *seedp;
-    // Spans: a liquid pixel continues its left neighbour's span when that
-    // neighbour is liquid and its own cell does not block liquids.
-    bool liq[WEQ_PER];
-    int last_start = -1;
-    bool prev = x0 > 0 && x0 - 1 < xres && weq_liquid(props, pmap[y * xres + x0 - 1]);
+    extern __shared__ uint32_t weq_lds[];  // min_top, max_free, dmin, dmax: xres words each
+    uint32_t *min_top = weq_lds, *max_free = weq_lds + xres, *dmin = weq_lds + 2 * xres, *dmax = weq_lds + 3 * xres;
+    __shared__ unsigned long long word_liq[WEQ_WORDS], word_link[WEQ_WORDS], word_start[WEQ_WORDS];
+    __shared__ int word_carry[WEQ_WORDS], row_any;
+    const int y = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int nwords = (xres + 63) >> 6;
+    // All of the row's loads are issued before any is used.
+    uint32_t pix[WEQ_PER], sf[WEQ_PER];
+    uint4 rec[WEQ_PER];
 #pragma unroll
     for (int u = 0; u < WEQ_PER; ++u) {
-        const int x = x0 + u;
-        liq[u] = x < xres && weq_liquid(props, pmap[y * xres + x]);
-        if (liq[u]) {
-            const uint4 r = cellrec[(y / 4) * xc + x / 4];
-            const bool link = prev && !wall_blocks_particle(r.z, r.w, 0x2u);
-            if (!link) last_start = x;
+        const int x = u * WEQ_BLK + t;
+        const bool in = x < xres;
+        pix[u] = in ? pmap[y * xres + x] : 0u;
+        sf[u] = in ? col_surf[x] : 0xFFFFFFFFu;
+        rec[u] = in ? cellrec[(y >> 2) * xc + (x >> 2)] : make_uint4(0u, 0u, 0u, 0u);
+    }
+    const WeqLiquidTable liquid(props);
+    bool liq[WEQ_PER];
+#pragma unroll
+    for (int u = 0; u < WEQ_PER; ++u) {
+        liq[u] = liquid(pix[u]);
+        const bool link = !wall_blocks_particle(rec[u].z, rec[u].w, 0x2u);
+        const unsigned long long lw = __ballot(liq[u]), kw = __ballot(link);
+        const int w = u * (WEQ_BLK / 64) + wv;
+        if (lane == 0 && w < nwords) {
+            word_liq[w] = lw;
+            word_link[w] = kw;
         }
-        prev = liq[u];
-        if (x < xres) {
+    }
+    __syncthreads();
+    // A liquid pixel starts a span unless its left neighbour is liquid and
+    // its own cell lets liquids through.
+    if (wv == 0) {
+        const int w = lane;
+        unsigned long long st = 0ull;
+        int last = -1;
+        if (w < nwords) {
+            const unsigned long long l = word_liq[w];
+            const unsigned long long prev = (l << 1) | (w > 0 ? word_liq[w - 1] >> 63 : 0ull);
+            st = l & ~(prev & word_link[w]);
+            if (st) last = w * 64 + 63 - __clzll((long long)st);
+            word_start[w] = st;
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(last, off);
+            if (lane >= off && v > last) last = v;
+        }
+        const int excl = __shfl_up(last, 1);
+        if (w < nwords) word_carry[w] = lane > 0 ? excl : -1;
+        if (lane == 63) row_any = last >= 0;
+    }
+    __syncthreads();
+    if (!row_any) return;
+    int start[WEQ_PER];
+#pragma unroll
+    for (int u = 0; u < WEQ_PER; ++u) {
+        const int x = u * WEQ_BLK + t, w = u * (WEQ_BLK / 64) + wv;
+        start[u] = -1;
+        if (!liq[u]) continue;
+        const unsigned long long below = word_start[w] & ((2ull << lane) - 1ull);
+        start[u] = below ? w * 64 + 63 - __clzll((long long)below) : word_carry[w];
+        if (start[u] == x) {
             min_top[x] = (uint32_t)yres;
             max_free[x] = 0u;
             dmin[x] = (uint32_t)xres;
             dmax[x] = 0u;
         }
     }
-    // Inclusive max-scan of the chunks' last span starts (Hillis-Steele).
-    carry[t] = (uint32_t)(last_start + 1);
     __syncthreads();
-    for (int off = 1; off < WEQ_BLK; off <<= 1) {
-        const uint32_t v = t >= off ? carry[t - off] : 0u;
-        __syncthreads();
-        if (v > carry[t]) carry[t] = v;
-        __syncthreads();
-    }
-    int cur = t > 0 ? (int)carry[t - 1] - 1 : -1;
-    prev = x0 > 0 && x0 - 1 < xres && weq_liquid(props, pmap[y * xres + x0 - 1]);
-#pragma unroll
-    for (int u = 0; u < WEQ_PER; ++u) {
-        const int x = x0 + u;
-        if (x >= xres) break;
-        if (liq[u]) {
-            const uint4 r = cellrec[(y / 4) * xc + x / 4];
-            if (!(prev && !wall_blocks_particle(r.z, r.w, 0x2u))) cur = x;
-            start[x] = (uint16_t)cur;
-        } else {
-            start[x] = 0xFFFFu;
-        }
-        prev = liq[u];
-    }
-    __syncthreads();
+    const uint32_t seed = *seedp;
     // Per span: the highest surface, then the lowest free surface and the
     // columns that have it.
+    bool inrow[WEQ_PER];
 #pragma unroll
     for (int u = 0; u < WEQ_PER; ++u) {
-        const int k = x0 + u;
-        if (k >= xres || start[k] == 0xFFFFu) continue;
-        const uint32_t sf = col_surf[k];
-        if (sf == 0xFFFFFFFFu) continue;
-        const uint32_t top = sf >> 16, bot = sf & 0x7FFFu;
-        if ((uint32_t)y >= top && (uint32_t)y <= bot) {
-            atomicMin(&min_top[start[k]], top);
-            if (sf & (1u << 15)) atomicMax(&max_free[start[k]], top);
+        const uint32_t top = sf[u] >> 16, bot = sf[u] & 0x7FFFu;
+        inrow[u] = start[u] >= 0 && sf[u] != 0xFFFFFFFFu && (uint32_t)y >= top && (uint32_t)y <= bot;
+        if (inrow[u]) {
+            atomicMin(&min_top[start[u]], top);
+            if (sf[u] & (1u << 15)) atomicMax(&max_free[start[u]], top);
         }
     }
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < WEQ_PER; ++u) {
-        const int k = x0 + u;
-        if (k >= xres || start[k] == 0xFFFFu) continue;
-        const uint32_t sf = col_surf[k];
-        if (sf == 0xFFFFFFFFu) continue;
-        const uint32_t top = sf >> 16, bot = sf & 0x7FFFu, s0 = start[k];
-        if ((uint32_t)y >= top && (uint32_t)y <= bot && (sf & (1u << 15)) && top == max_free[s0]) {
-            atomicMin(&dmin[s0], (uint32_t)k);
-            atomicMax(&dmax[s0], (uint32_t)k);
+        const int k = u * WEQ_BLK + t;
+        if (inrow[u] && (sf[u] & (1u << 15)) && (sf[u] >> 16) == max_free[start[u]]) {
+            atomicMin(&dmin[start[u]], (uint32_t)k);
+            atomicMax(&dmax[start[u]], (uint32_t)k);
         }
     }
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < WEQ_PER; ++u) {
-        const int k = x0 + u;
-        if (k >= xres || start[k] == 0xFFFFu) continue;
-        const uint32_t s0 = start[k], mf = max_free[s0];
+        const int k = u * WEQ_BLK + t;
+        if (start[u] < 0 || sf[u] == 0xFFFFFFFFu) continue;
+        const uint32_t s0 = (uint32_t)start[u], mf = max_free[s0];
         if (!(mf > min_top[s0] + 1u)) continue;
-        const uint32_t sf = col_surf[k];
-        if (sf == 0xFFFFFFFFu) continue;
         const uint32_t dest_row = mf - 1u;
-        if ((sf >> 16) >= dest_row) continue;
+        if ((sf[u] >> 16) >= dest_row) continue;
         const uint32_t span_len = dmax[s0] - dmin[s0] + 1u;
         const uint32_t dest_x = dmin[s0] + ((uint32_t)k + seed) % span_len;
         const uint32_t dest = dest_row * (uint32_t)xres + dest_x;
@@ -2003,7 +2043,8 @@ int tpt_water_equal_step(tpt_ctx *ctx) {
                        ctx->stream, pmap, props, w.col_min, w.col_max, xres, yres);
     hipLaunchKernelGGL(weq_column_surfaces, dim3((xres + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, pmap,
                        ctx->cell_rec, w.col_min, w.col_max, w.col_surf, w.col_target, ctx->counters, xres, xc);
-    hipLaunchKernelGGL(weq_rows, dim3(yres), dim3(WEQ_BLK), 0, ctx->stream, pmap, props, ctx->cell_rec, w.col_surf,
+    hipLaunchKernelGGL(weq_rows, dim3(yres), dim3(WEQ_BLK), 16u * (uint32_t)xres, ctx->stream, pmap, props,
+                       ctx->cell_rec, w.col_surf,
                        w.col_target, w.winner, xres, yres, xc, seed);
     Slots s{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<float>(ctx, TPT_BUF_PART_X),
             tpt_buf<float>(ctx, TPT_BUF_PART_Y),   tpt_buf<float>(ctx, TPT_BUF_PART_VX),
