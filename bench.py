@@ -1,6 +1,7 @@
 """Benchmark: plain HIP tick throughput on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2048x2048]
+                    [--water-equal] [--scene SAVE.tptrs] [--batched N]
 
 Prints ONE JSON line.  A step is one tick of the built passes over the
 synthetic 2048x2048 workload (512x512 air cells, ~2.1M particles, 64 wall
@@ -157,7 +158,7 @@ def batched(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic",
+        "data": "saved scene" if args.scene else "synthetic",
         "config": {"workload": f"{n} canvases, sides 128..2048 step 4 (seed 2), fill 10-50%",
                    "pixels": px, "particles": particles, "parallelism": "single-gpu"},
         "per_canvas_launches": {"value": round(n * args.steps / t_launch, 2), "ms_per_step": round(t_launch / args.steps * 1e3, 3)},
@@ -180,6 +181,8 @@ def main():
                     help="the batched form instead: N mixed-size canvases, graph replay vs per-canvas launches")
     ap.add_argument("--water-equal", action="store_true",
                     help="run the workload with water_equal_test set (the optional pass after movement)")
+    ap.add_argument("--scene", default=None, metavar="PATH",
+                    help="tick a .tptrs save instead of the synthetic workload")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -199,7 +202,15 @@ def main():
     from tpt_hip import lib as L
     from tpt_hip import scene as S
 
-    st = S.make_scene(args.workload)
+    def scene():
+        if args.scene:
+            from tpt_hip import save as V
+            return V.restore(V.load(args.scene))
+        return S.make_scene(args.workload)
+
+    st = scene()
+    if args.scene:
+        args.workload = os.path.basename(args.scene)
     if args.water_equal:
         st.cfg["water_equal_test"] = 1
     C, P = st.xcells * st.ycells, st.cfg["xres"] * st.cfg["yres"]
@@ -277,7 +288,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
-        ref = S.make_scene(args.workload)
+        ref = scene()
+        ref.cfg["water_equal_test"] = st.cfg["water_equal_test"]
         o = O.OracleState(ref)
         n_cpu, t0 = 0, time.perf_counter()
         while True:
@@ -302,7 +314,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic",
+        "data": "saved scene" if args.scene else "synthetic",
         "config": {"workload": f"{args.workload} plain tick{' with water equalisation' if args.water_equal else ''}, "
                                f"passes: {'+'.join(passes)}",
                    "air_cells": C, "pixels": P, "particles": N, "parallelism": "single-gpu"},

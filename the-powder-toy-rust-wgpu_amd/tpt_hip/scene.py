@@ -158,6 +158,18 @@ def derive_block_masks(st):
         st.arrays["elem_block_airh"]
 
 
+def gravity_mask(st):
+    """rebuild_gravity_mask (gravity_kernel.rs:658-710): cells reachable from
+    the canvas border through 4-neighbours without entering a WL_GRAV (14)
+    cell get 0xFFFFFFFF, every other cell 0."""
+    from scipy import ndimage
+    open_ = (st.arrays["wall_bmap"].reshape(st.ycells, st.xcells) & 0xFF) != 14
+    labels, _ = ndimage.label(open_, structure=[[0, 1, 0], [1, 1, 1], [0, 1, 0]])
+    border = np.unique(np.concatenate([labels[0], labels[-1], labels[:, 0], labels[:, -1]]))
+    reach = np.isin(labels, border[border != 0])
+    st.arrays["grav_mask"][:] = np.where(reach.reshape(-1), 0xFFFFFFFF, 0).astype(np.uint32)
+
+
 def gaussian_pressure(st, rng, blobs=8, sigma=6.0, amp=20.0):
     yy, xx = np.mgrid[0:st.ycells, 0:st.xcells].astype(np.float64)
     pv = np.zeros((st.ycells, st.xcells))
