@@ -41,8 +41,8 @@ def kernel_bytes(name, C, P, S, N, W, yc):
         "grav_fft_rows": 8 * C + 8 * yc * W,  # mass,mask in; ycells padded rows out
         "grav_fft_cols": None,                # filled in below (needs H)
         "grav_fft_inverse_extract": 16 * yc * W + 12 * C,
-        "grav_apply": 4 * S + 36 * N,         # ty scan; x,y,vx,vy,flags,force gather in; vx,vy out
-        "grav_build_mass": 4 * S + 12 * C,    # ty scan; clear + copy of the mass grid
+        "grav_apply": 4 * S + 36 * N,         # ty scan; x,y,vx,vy,flags,force gather in; vx,vy out; mass deposit
+        "grav_build_mass": 8 * C,             # mass -> snapshot copy (element-weight mass mode only)
         "recalc_clear": 8 * P,                # pmap_count from the movement rebuild's counts
         "recalc_scatter": 4 * S + 12 * P,     # ty scan (life decay); records in, pmap out
         "check_stacking": 4 * P + 4 * S + 12 * N,
@@ -64,7 +64,7 @@ def kernel_bytes(name, C, P, S, N, W, yc):
 
 # Timed passes that are one kernel, for the roofline's HBM traffic (the
 # rocprofv3 summary is per kernel).
-PASS_KERNEL = {"move_velocity_intent": "mv_intent", "move_commit": "mv_commit", "grav_apply": "grav_apply",
+PASS_KERNEL = {"move_velocity_intent": "mv_intent", "move_commit": "mv_commit", "grav_apply": "grav_apply_mass",
                "grav_fft_cols": "grav_cols", "thermal_apply": "th_apply_px"}
 
 

@@ -233,6 +233,7 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__res
                                                              const uint32_t *__restrict__ wall,
                                                              const uint32_t *__restrict__ emap,
                                                              uint4 *__restrict__ rec, int32_t *__restrict__ mass_clear,
+                                                             int32_t *__restrict__ prev_clear,
                                                              const float2 *__restrict__ tw, FftDims d) {
     extern __shared__ float2 dyn[];
     float2 *sm = dyn, *tab = dyn + 2 * d.W;
@@ -252,8 +253,12 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__res
         fy[c] = gy;
         rec[c] = make_uint4(__float_as_uint(gx), __float_as_uint(gy), wall[c], emap[c]);
         // The mass grid this tick's deposit accumulates into (the solve
-        // reads the previous tick's snapshot); null for a bare solve.
-        if (mass_clear) mass_clear[c] = 0;
+        // reads the previous tick's snapshot, consumed by now, which the
+        // deposit refills alongside); null for a bare solve.
+        if (mass_clear) {
+            mass_clear[c] = 0;
+            prev_clear[c] = 0;
+        }
     }
 }
 
@@ -304,20 +309,15 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_cols_inplace(float2 *__restr
 
 // apply_gravity_kernel (gravity_kernel.rs:434-507), NewtonianGravity = 1.
 // rec: the per-cell {fx, fy, wall_bmap, wall_emap} grav_rows_inv packed.
-__global__ __launch_bounds__(GRAV_BLOCK) void grav_apply(const uint32_t *__restrict__ ty_a,
-                                                         const float *__restrict__ px_a,
-                                                         const float *__restrict__ py_a, float *__restrict__ vx_a,
-                                                         float *__restrict__ vy_a, const uint32_t *__restrict__ flags_a,
-                                                         const uint4 *__restrict__ rec,
-                                                         const uint32_t *__restrict__ lut, uint32_t slots, int xres,
-                                                         int yres, int xc) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= slots) return;
-    const uint32_t ty = ty_a[i];
-    if (ty == 0u || ty >= (uint32_t)TPT_PT_NUM) return;
+// The force step of step_gravity_gpu for one slot (gravity_kernel.rs).
+__device__ __forceinline__ void apply_one(uint32_t i, uint32_t ty, float x, float y, float *__restrict__ vx_a,
+                                          float *__restrict__ vy_a, const uint32_t *__restrict__ flags_a,
+                                          const uint4 *__restrict__ rec, const uint32_t *__restrict__ lut, int xres,
+                                          int yres, int xc) {
+    if (ty >= (uint32_t)TPT_PT_NUM) return;
     const uint32_t flags = flags_a[i];
     if (flags & 0x10u) return;
-    const int xi = tpt_f2i(px_a[i] + 0.5f), yi = tpt_f2i(py_a[i] + 0.5f);
+    const int xi = tpt_f2i(x + 0.5f), yi = tpt_f2i(y + 0.5f);
     if (xi < 4 || yi < 4 || xi >= xres - 4 || yi >= yres - 4) return;
     const int cell = (yi / TPT_CELL) * xc + xi / TPT_CELL;
     const uint4 r = rec[cell];
@@ -337,17 +337,14 @@ __global__ __launch_bounds__(GRAV_BLOCK) void grav_apply(const uint32_t *__restr
     vy_a[i] = vy;
 }
 
-// build_gravity_mass_kernel (gravity_kernel.rs:275-362).  mode 1 deposits
-// the element weight of every live particle.
-__global__ __launch_bounds__(GRAV_BLOCK) void grav_build_mass(
-    const uint32_t *__restrict__ ty_a, const float *__restrict__ px_a, const float *__restrict__ py_a,
-    const int32_t *__restrict__ life_a, const int32_t *__restrict__ tmp_a, const float *__restrict__ temp_a,
-    const uint32_t *__restrict__ mask, const uint32_t *__restrict__ props, int32_t *__restrict__ mass,
-    uint32_t slots, int xres, int yres, int xc, int yc, uint32_t mode) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= slots) return;
-    const uint32_t ty = ty_a[i];
-    if (ty == 0u) return;
+// build_gravity_mass_kernel (gravity_kernel.rs:275-362) for one slot.  mode
+// 1 deposits the element weight of every live particle.  `prev`, when set,
+// receives the same deposit (the snapshot the next tick's solve reads).
+__device__ __forceinline__ void mass_one(uint32_t i, uint32_t ty, float x, float y, const int32_t *__restrict__ life_a,
+                                         const int32_t *__restrict__ tmp_a, const float *__restrict__ temp_a,
+                                         const uint32_t *__restrict__ mask, const uint32_t *__restrict__ props,
+                                         int32_t *__restrict__ mass, int32_t *__restrict__ prev, int xres, int yres,
+                                         int xc, int yc, uint32_t mode) {
     float m = 0.0f;
     bool contributes = false;
     if (mode == 1u) {
@@ -381,7 +378,7 @@ __global__ __launch_bounds__(GRAV_BLOCK) void grav_build_mass(
         }
     }
     if (!contributes) return;
-    const int xi = tpt_f2i(px_a[i] + 0.5f), yi = tpt_f2i(py_a[i] + 0.5f);
+    const int xi = tpt_f2i(x + 0.5f), yi = tpt_f2i(y + 0.5f);
     if (xi < 0 || yi < 0 || xi >= xres || yi >= yres) return;
     const int cx = xi / TPT_CELL, cy = yi / TPT_CELL;
     if (cx >= xc || cy >= yc) return;
@@ -391,7 +388,38 @@ __global__ __launch_bounds__(GRAV_BLOCK) void grav_build_mass(
     if (scaled >= 0.0f) scaled += 0.5f;
     else scaled -= 0.5f;
     const int32_t q = tpt_f2i(scaled);
-    if (q != 0) atomicAdd(&mass[cell], q);
+    if (q != 0) {
+        atomicAdd(&mass[cell], q);
+        if (prev) atomicAdd(&prev[cell], q);
+    }
+}
+
+// The force step and the next mass deposit in one pass over the slots: they
+// read the same type and position and write disjoint arrays.
+struct MassArgs {
+    const int32_t *life, *tmp;
+    const float *temp;
+    const uint32_t *mask, *props;
+    int32_t *mass, *prev;
+    int yc;
+    uint32_t mode;
+};
+
+__global__ __launch_bounds__(GRAV_BLOCK) void grav_apply_mass(const uint32_t *__restrict__ ty_a,
+                                                              const float *__restrict__ px_a,
+                                                              const float *__restrict__ py_a, float *__restrict__ vx_a,
+                                                              float *__restrict__ vy_a,
+                                                              const uint32_t *__restrict__ flags_a,
+                                                              const uint4 *__restrict__ rec,
+                                                              const uint32_t *__restrict__ lut, MassArgs m,
+                                                              uint32_t slots, int xres, int yres, int xc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    const uint32_t ty = ty_a[i];
+    if (ty == 0u) return;
+    const float x = px_a[i], y = py_a[i];
+    apply_one(i, ty, x, y, vx_a, vy_a, flags_a, rec, lut, xres, yres, xc);
+    mass_one(i, ty, x, y, m.life, m.tmp, m.temp, m.mask, m.props, m.mass, m.prev, xres, yres, xc, m.yc, m.mode);
 }
 
 __global__ void fill_u32(uint32_t *__restrict__ p, uint32_t v, uint32_t n) {
@@ -497,8 +525,8 @@ int solve(tpt_ctx *ctx, bool clear_mass) {
                        f.spec_x, f.spec_y, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK),
                        tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), tpt_buf<float>(ctx, TPT_BUF_GRAV_FY),
                        tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP),
-                       ctx->cell_rec, clear_mass ? tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS) : nullptr, f.twiddle,
-                       d);
+                       ctx->cell_rec, clear_mass ? tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS) : nullptr,
+                       tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_PREV), f.twiddle, d);
     tpt_pass_end(ctx, PASS_GRAV_FFT_INV);
     return tpt_check_launch("gravity solve");
 }
@@ -520,26 +548,32 @@ int tpt_gravity_step(tpt_ctx *ctx, uint32_t) {
     if (rc) return rc;
     const uint32_t slots = ctx->slot_bound;
     const dim3 sgrid((slots + GRAV_BLOCK - 1) / GRAV_BLOCK);
+    // The force step and the deposit of this tick's mass in one launch.  In
+    // the reference's mass mode the deposits are rare and go to both grids,
+    // which the solve has just cleared; with every particle depositing
+    // (mode 1) the snapshot is one copy instead of twice the atomics.
+    const bool dual = ctx->cfg.grav_mass_mode == 0u;
+    int32_t *mass = tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS);
+    const MassArgs m{tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE), tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP),
+                     tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),   tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK),
+                     tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS), mass,
+                     dual ? tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_PREV) : nullptr, (int)ctx->ycells,
+                     ctx->cfg.grav_mass_mode};
     tpt_pass_begin(ctx, PASS_GRAV_APPLY);
-    hipLaunchKernelGGL(grav_apply, sgrid, dim3(GRAV_BLOCK), 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY),
-                       tpt_buf<float>(ctx, TPT_BUF_PART_X), tpt_buf<float>(ctx, TPT_BUF_PART_Y),
-                       tpt_buf<float>(ctx, TPT_BUF_PART_VX), tpt_buf<float>(ctx, TPT_BUF_PART_VY),
-                       tpt_buf<uint32_t>(ctx, TPT_BUF_PART_FLAGS), ctx->cell_rec,
-                       tpt_buf<uint32_t>(ctx, TPT_BUF_MOVE_LUT), slots,
-                       (int)ctx->cfg.xres, (int)ctx->cfg.yres, (int)ctx->xcells);
-    tpt_pass_end(ctx, PASS_GRAV_APPLY);
-    tpt_pass_begin(ctx, PASS_GRAV_BUILD);
-    hipLaunchKernelGGL(grav_build_mass, sgrid, dim3(GRAV_BLOCK), 0, ctx->stream,
+    hipLaunchKernelGGL(grav_apply_mass, sgrid, dim3(GRAV_BLOCK), 0, ctx->stream,
                        tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<float>(ctx, TPT_BUF_PART_X),
-                       tpt_buf<float>(ctx, TPT_BUF_PART_Y), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
-                       tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP), tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),
-                       tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK), tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS),
-                       tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS), slots, (int)ctx->cfg.xres, (int)ctx->cfg.yres,
-                       (int)ctx->xcells, (int)ctx->ycells, ctx->cfg.grav_mass_mode);
-    // prev = current: the snapshot the next tick's solve reads.
-    hipLaunchKernelGGL(copy_u32, cgrid, blk, 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASS),
-                       tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASS_PREV), cells);
-    tpt_pass_end(ctx, PASS_GRAV_BUILD);
+                       tpt_buf<float>(ctx, TPT_BUF_PART_Y), tpt_buf<float>(ctx, TPT_BUF_PART_VX),
+                       tpt_buf<float>(ctx, TPT_BUF_PART_VY), tpt_buf<uint32_t>(ctx, TPT_BUF_PART_FLAGS),
+                       ctx->cell_rec, tpt_buf<uint32_t>(ctx, TPT_BUF_MOVE_LUT), m, slots, (int)ctx->cfg.xres,
+                       (int)ctx->cfg.yres, (int)ctx->xcells);
+    tpt_pass_end(ctx, PASS_GRAV_APPLY);
+    if (!dual) {
+        // prev = current: the snapshot the next tick's solve reads.
+        tpt_pass_begin(ctx, PASS_GRAV_BUILD);
+        hipLaunchKernelGGL(copy_u32, cgrid, blk, 0, ctx->stream, (const uint32_t *)mass,
+                           tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASS_PREV), cells);
+        tpt_pass_end(ctx, PASS_GRAV_BUILD);
+    }
     return tpt_check_launch("gravity apply/build");
 }
 

@@ -42,7 +42,7 @@ enum tpt_pass {
     PASS_GRAV_FFT_COLS,   // forward column FFT * kernel spectra, inverse column FFT
     PASS_GRAV_FFT_INV,    // inverse row FFT + force extraction
     PASS_GRAV_APPLY,      // Loss + Newtonian force on particles
-    PASS_GRAV_BUILD,      // fixed-point mass deposit
+    PASS_GRAV_BUILD,      // mass snapshot copy (element-weight mass mode; the deposit runs in grav_apply)
     PASS_RECALC_CLEAR,    // pmap/photons/pmap_count clear
     PASS_RECALC_SCATTER,  // life decay + map scatter
     PASS_STACKING,        // CheckStacking
