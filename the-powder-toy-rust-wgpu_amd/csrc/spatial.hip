@@ -189,10 +189,13 @@ __global__ __launch_bounds__(BLK) void recalc_invis(const uint32_t *__restrict__
 
 // pmap and the owners' own pmap_count contribution from the pixel records,
 // four pixels per thread.
+// With `count_src` (the incremental recalc) pmap_count is that private
+// count map, already net of the killed, instead of gaining the owners.
 __global__ void recalc_extract(const ulonglong2 *__restrict__ rec, uint4 *__restrict__ pmap, uint4 *__restrict__ count,
-                               uint32_t n4) {
+                               const uint4 *__restrict__ count_src, uint32_t n4) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n4) return;
+    if (count_src) count[i] = count_src[i];
     const ulonglong2 a = rec[2 * i], b = rec[2 * i + 1];
     const uint32_t k[4] = {(uint32_t)(a.x >> 32), (uint32_t)(a.y >> 32), (uint32_t)(b.x >> 32), (uint32_t)(b.y >> 32)};
     uint32_t add[4];
@@ -203,7 +206,7 @@ __global__ void recalc_extract(const ulonglong2 *__restrict__ rec, uint4 *__rest
         any |= add[j] != 0u;
     }
     pmap[i] = make_uint4(k[0] & ~0x100u, k[1] & ~0x100u, k[2] & ~0x100u, k[3] & ~0x100u);
-    if (any && count) {
+    if (any && count && !count_src) {
         uint4 c = count[i];
         c.x += add[0];
         c.y += add[1];
@@ -317,11 +320,6 @@ __global__ __launch_bounds__(BLK) void recalc_invis_list(const uint32_t *__restr
     }
 }
 
-__global__ void copy_words(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint32_t n4) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n4) dst[i] = src[i];
-}
-
 // CheckStacking runs when the device frame says so (every 10th frame), so
 // that a captured tick carries it; counters[TPT_CTR_STACK] is the flag.
 __global__ void stacking_mark(uint32_t *__restrict__ pmap_count, const uint32_t *__restrict__ wall_bmap, uint32_t n,
@@ -405,7 +403,7 @@ static int map_build(tpt_ctx *ctx, int decay, uint32_t *count, int pass_clear, i
                        tpt_buf<float>(ctx, TPT_BUF_PART_X), tpt_buf<float>(ctx, TPT_BUF_PART_Y),
                        tpt_buf<float>(ctx, TPT_BUF_PART_TEMP), rec, ctx->slot_scratch_u, ctx->counters, xres);
     hipLaunchKernelGGL(recalc_extract, dim3((n4 + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
-                       (const ulonglong2 *)rec, (uint4 *)pmap, (uint4 *)count, n4);
+                       (const ulonglong2 *)rec, (uint4 *)pmap, (uint4 *)count, (const uint4 *)nullptr, n4);
     tpt_pass_end(ctx, pass_scatter);
     return TPT_OK;
 }
@@ -422,21 +420,20 @@ static int map_update(tpt_ctx *ctx) {
                 *temp = tpt_buf<float>(ctx, TPT_BUF_PART_TEMP);
     const uint32_t *props = tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS);
     const uint32_t n4 = P / 4;
-    tpt_pass_begin(ctx, PASS_RECALC_CLEAR);
-    hipLaunchKernelGGL(copy_words, dim3((n4 + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
-                       (const uint4 *)ctx->pix_count_mv, (uint4 *)count, n4);
-    tpt_pass_end(ctx, PASS_RECALC_CLEAR);
+    // The kills come off the movement rebuild's private count map, which
+    // the extract then publishes as pmap_count.
+    uint32_t *count_mv = ctx->pix_count_mv;
     tpt_pass_begin(ctx, PASS_RECALC_SCATTER);
     hipLaunchKernelGGL(recalc_decay, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
                        tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE), px, py,
                        props, tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), rec,
-                       photons, count, slots, xres, yres, (int)ctx->xcells);
+                       photons, count_mv, slots, xres, yres, (int)ctx->xcells);
     hipLaunchKernelGGL(recalc_relist, dim3(256), dim3(BLK), 0, ctx->stream, ty, px, py, temp, props, rec, photons,
                        ctx->offmap, ctx->counters, slots, xres, yres);
     hipLaunchKernelGGL(recalc_invis_list, dim3(64), dim3(BLK), 0, ctx->stream, ty, px, py, temp, rec, ctx->offmap,
                        ctx->counters, slots, xres, yres);
     hipLaunchKernelGGL(recalc_extract, dim3((n4 + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
-                       (const ulonglong2 *)rec, (uint4 *)pmap, (uint4 *)nullptr, n4);
+                       (const ulonglong2 *)rec, (uint4 *)pmap, (uint4 *)count, (const uint4 *)count_mv, n4);
     tpt_pass_end(ctx, PASS_RECALC_SCATTER);
     return TPT_OK;
 }

@@ -262,14 +262,15 @@ __global__ __launch_bounds__(BLK) void th_apply(Parts p, const uint32_t *__restr
 
 // The off-map list's slots; an off-map slot that owns the record of its
 // rounded pixel (a flagged owner) also refreshes the record's temperature.
-__global__ __launch_bounds__(BLK) void th_apply_list(Parts p, const uint32_t *__restrict__ pmap,
-                                                     const uint32_t *__restrict__ props,
-                                                     const uint32_t *__restrict__ list,
-                                                     const uint32_t *__restrict__ counters,
-                                                     const uint2 *__restrict__ heat,
-                                                     unsigned long long *__restrict__ rec, Dims d) {
+// The apply of the off-map list: entries first, first + stride, ...
+__device__ __forceinline__ void apply_list_part(const Parts &p, const uint32_t *__restrict__ pmap,
+                                                const uint32_t *__restrict__ props,
+                                                const uint32_t *__restrict__ list,
+                                                const uint32_t *__restrict__ counters,
+                                                const uint2 *__restrict__ heat, unsigned long long *rec,
+                                                const Dims &d, uint32_t first, uint32_t stride) {
     const uint32_t n = counters[TPT_CTR_OFFMAP];
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    for (uint32_t k = first; k < n; k += stride) {
         const uint32_t j = list[k];
         if (j >= d.slots) continue;
         apply_slot(p, pmap, props, d, j, heat);
@@ -536,9 +537,24 @@ __device__ __forceinline__ uint32_t cand_info(uint32_t nk, float eq, uint32_t hi
 
 // Apply for the unflagged owners of a tile: phase 2 compacts the owners,
 // phase 3 computes them.
+// Blocks past the tile rows apply the off-map list: the listed particles
+// and the unflagged owners the tiles handle are disjoint, and neither side
+// reads what the other writes (a listed particle's temperature, or its
+// record's temperature word).
 __global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long long *__restrict__ rec,
                                                    uint32_t *__restrict__ rec_temp, const uint2 *__restrict__ heat,
-                                                   const uint32_t *__restrict__ props, Dims d) {
+                                                   const uint32_t *__restrict__ props, Dims d,
+                                                   const uint32_t *__restrict__ pmap,
+                                                   const uint32_t *__restrict__ offmap,
+                                                   const uint32_t *__restrict__ counters,
+                                                   unsigned long long *rec_list, int tile_rows) {
+    if ((int)blockIdx.y >= tile_rows) {
+        const uint32_t nb = (gridDim.y - (uint32_t)tile_rows) * gridDim.x;
+        const uint32_t b = (blockIdx.y - (uint32_t)tile_rows) * gridDim.x + blockIdx.x;
+        apply_list_part(p, pmap, props, offmap, counters, heat, rec_list, d,
+                        b * 256u + threadIdx.y * TW + threadIdx.x, nb * 256u);
+        return;
+    }
     __shared__ uint32_t skey[LH][LW];
     __shared__ float seq[LH][LW];
     __shared__ uint32_t scand[LH][LW];  // each pixel as an apply candidate: cand_info()
@@ -1157,10 +1173,12 @@ int tpt_thermal_step(tpt_ctx *ctx, int legacy) {
                            ctx->cell_scratch_u, ctx->cell_scratch_u2, ctx->pix_heat, d, aheat, seed);
         tpt_pass_end(ctx, PASS_CONDUCT);
         tpt_pass_begin(ctx, PASS_CONDUCT_APPLY);
-        hipLaunchKernelGGL(th_apply_px, tg, tb, 0, ctx->stream, p, rec, (uint32_t *)ctx->pix_rec,
-                           (const uint2 *)ctx->pix_heat, props, d);
-        hipLaunchKernelGGL(th_apply_list, lg, b, 0, ctx->stream, p, pmap, props, ctx->offmap, ctx->counters,
-                           (const uint2 *)ctx->pix_heat, (unsigned long long *)ctx->pix_rec, d);
+        // The tiles plus rows of blocks for the off-map list (about 256).
+        const int list_rows = (int)((256u + tg.x - 1u) / tg.x);
+        const dim3 ag(tg.x, tg.y + (uint32_t)list_rows);
+        hipLaunchKernelGGL(th_apply_px, ag, tb, 0, ctx->stream, p, rec, (uint32_t *)ctx->pix_rec,
+                           (const uint2 *)ctx->pix_heat, props, d, pmap, ctx->offmap, ctx->counters,
+                           (unsigned long long *)ctx->pix_rec, (int)tg.y);
         tpt_pass_end(ctx, PASS_CONDUCT_APPLY);
     } else if (!legacy) {
         tpt_pass_begin(ctx, PASS_CONDUCT);
