@@ -47,8 +47,9 @@ static void fold_oldest(tpt_ctx *ctx, int pass) {
     const int R = tpt_ctx::RING;
     int slot = (ctx->ev_head[pass] - ctx->ev_pending[pass] + R) % R;
     float ms = 0.0f;
-    hipEventSynchronize(ctx->ev_stop[pass][slot]);
-    if (hipEventElapsedTime(&ms, ctx->ev_start[pass][slot], ctx->ev_stop[pass][slot]) == hipSuccess)
+    // A failed wait or query drops the measurement; timing never fails a pass.
+    if (hipEventSynchronize(ctx->ev_stop[pass][slot]) == hipSuccess &&
+        hipEventElapsedTime(&ms, ctx->ev_start[pass][slot], ctx->ev_stop[pass][slot]) == hipSuccess)
         ctx->pass_ms[pass] += ms;
     ctx->ev_pending[pass] -= 1;
 }
@@ -56,13 +57,13 @@ static void fold_oldest(tpt_ctx *ctx, int pass) {
 void tpt_pass_begin(tpt_ctx *ctx, int pass) {
     if (!ctx->profiling || ctx->capturing) return;
     if (ctx->ev_pending[pass] == tpt_ctx::RING) fold_oldest(ctx, pass);
-    hipEventRecord(ctx->ev_start[pass][ctx->ev_head[pass]], ctx->stream);
+    (void)hipEventRecord(ctx->ev_start[pass][ctx->ev_head[pass]], ctx->stream);
 }
 
 void tpt_pass_end(tpt_ctx *ctx, int pass) {
     if (!ctx->profiling || ctx->capturing) return;
     int slot = ctx->ev_head[pass];
-    hipEventRecord(ctx->ev_stop[pass][slot], ctx->stream);
+    (void)hipEventRecord(ctx->ev_stop[pass][slot], ctx->stream);
     ctx->ev_head[pass] = (slot + 1) % tpt_ctx::RING;
     ctx->ev_pending[pass] += 1;
     ctx->pass_launches[pass] += 1;
@@ -108,34 +109,34 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMalloc(&c->water.col_target, (size_t)c->cfg.xres * 8) != hipSuccess ||
         hipMalloc(&c->water.winner, (size_t)c->pixels * 4) != hipSuccess)
         return tpt_set_error(TPT_ENOMEM, "scratch allocation failed");
-    hipMemsetAsync(c->water.col_min, 0xFF, (size_t)c->cfg.xres * 4, c->stream);
-    hipMemsetAsync(c->water.col_max, 0, (size_t)c->cfg.xres * 4, c->stream);
-    hipMemsetAsync(c->cell_scratch_u, 0, cb, c->stream);
-    hipMemsetAsync(c->cell_scratch_u2, 0, cb, c->stream);
-    hipMemsetAsync(c->counters, 0, 256, c->stream);
-    if (hipMemsetAsync(c->slot_owner, 0xFF, cap * 4, c->stream) != hipSuccess)
-        return tpt_set_error(TPT_EDEVICE, "owner clear failed");
+    if (hipMemsetAsync(c->water.col_min, 0xFF, (size_t)c->cfg.xres * 4, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->water.col_max, 0, (size_t)c->cfg.xres * 4, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->cell_scratch_u, 0, cb, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->cell_scratch_u2, 0, cb, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->counters, 0, 256, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->slot_owner, 0xFF, cap * 4, c->stream) != hipSuccess)
+        return tpt_set_error(TPT_EDEVICE, "scratch clear failed");
     return TPT_OK;
 }
 
 static void free_all(tpt_ctx *c) {
     for (int id = 0; id < TPT_BUF_COUNT; ++id)
-        if (c->buf[id]) hipFree(c->buf[id]);
+        if (c->buf[id]) (void)hipFree(c->buf[id]);
     void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
                      c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters,
                      c->pix_rec, c->pix_heat, c->offmap, c->pix_type, c->pix_count_mv, c->slot_owner, c->create_rec, c->cell_rec,
                      c->water.col_min, c->water.col_max, c->water.col_surf, c->water.col_target,
                      c->water.winner};
     for (void *p : extra)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     tpt_gravity_free(c);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (int p = 0; p < PASS_COUNT; ++p)
         for (int r = 0; r < tpt_ctx::RING; ++r) {
-            if (c->ev_start[p][r]) hipEventDestroy(c->ev_start[p][r]);
-            if (c->ev_stop[p][r]) hipEventDestroy(c->ev_stop[p][r]);
+            if (c->ev_start[p][r]) (void)hipEventDestroy(c->ev_start[p][r]);
+            if (c->ev_stop[p][r]) (void)hipEventDestroy(c->ev_stop[p][r]);
         }
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
 extern "C" int tpt_create(const tpt_config *cfg, tpt_ctx **out) {
@@ -176,10 +177,11 @@ extern "C" int tpt_create(const tpt_config *cfg, tpt_ctx **out) {
 
 extern "C" int tpt_destroy(tpt_ctx *ctx) {
     if (!ctx) return tpt_set_error(TPT_EINVAL, "null context");
-    hipStreamSynchronize(ctx->stream);
+    // The context is released either way; a failed wait is still reported.
+    const bool synced = hipStreamSynchronize(ctx->stream) == hipSuccess;
     free_all(ctx);
     delete ctx;
-    return TPT_OK;
+    return synced ? TPT_OK : tpt_set_error(TPT_EDEVICE, "stream synchronize before destroy failed");
 }
 
 extern "C" int tpt_set_element_table(tpt_ctx *ctx, const uint32_t *props, size_t props_words,
