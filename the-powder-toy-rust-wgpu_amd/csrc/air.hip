@@ -436,10 +436,9 @@ __global__ __launch_bounds__(TX *TY) void air_settle_kernel(
 #pragma unroll
     for (int dy = 0; dy < 4; ++dy) {
         const uint4 row = *reinterpret_cast<const uint4 *>(pmap + (size_t)(y * 4 + dy) * xres + x * 4);
-        const uint32_t e4[4] = {row.x, row.y, row.z, row.w};
 #pragma unroll
         for (int dx = 0; dx < 4; ++dx) {
-            const uint32_t e = e4[dx];
+            const uint32_t e = dx == 0 ? row.x : dx == 1 ? row.y : dx == 2 ? row.z : row.w;
             if (e == 0u) continue;
             const uint32_t ty = e & 0x1FFu;
             const float hot = __uint_as_float(elem_props[ty * 32u + 11u]);

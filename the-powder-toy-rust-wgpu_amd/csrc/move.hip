@@ -712,9 +712,14 @@ __device__ uint32_t gas_destination(const MoveEnv &e, float &vx, float &vy, floa
     if (died) {
         target = DIE_INTENT;
     } else if (blocked) {
+        // Each axis is scaled once, after the probes (which read the
+        // unscaled velocity), by `collision` or by 1 (exact).  Scaling in
+        // the branches let the compiler merge the stores through a selected
+        // address, which put vx/vy in scratch memory for the whole kernel.
+        float kx = 1.0f, ky = 1.0f;
         if (ty == 29u) {
-            if (fin_x != xi) vx *= collision;
-            if (fin_y != yi) vy *= collision;
+            if (fin_x != xi) kx = collision;
+            if (fin_y != yi) ky = collision;
         } else {
             const int bx = min(max(fin_x, xi - 2), xi + 2), by = min(max(fin_y, yi - 2), yi + 2);
             const int rx = 2 * xi - bx, ry = 2 * yi - by;
@@ -723,21 +728,23 @@ __device__ uint32_t gas_destination(const MoveEnv &e, float &vx, float &vy, floa
             if (d1 != DO_BLOCKED) {
                 if (d1 == DO_DIE) target = DIE_INTENT;
                 else if (ox != xi || oy != yi) target = pix(e, ox, oy);
-                vx *= collision;
+                kx = collision;
             } else {
                 saw_wood_at(e, ox, oy, vx, vy);
                 const uint32_t d2 = do_move_at(e, props, ty, bx, ry, ox, oy);
                 if (d2 != DO_BLOCKED) {
                     if (d2 == DO_DIE) target = DIE_INTENT;
                     else if (ox != xi || oy != yi) target = pix(e, ox, oy);
-                    vy *= collision;
+                    ky = collision;
                 } else {
                     saw_wood_at(e, ox, oy, vx, vy);
-                    vx *= collision;
-                    vy *= collision;
+                    kx = collision;
+                    ky = collision;
                 }
             }
         }
+        vx *= kx;
+        vy *= ky;
     }
     return target;
 }

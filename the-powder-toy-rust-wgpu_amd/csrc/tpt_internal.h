@@ -207,8 +207,12 @@ __device__ __forceinline__ uint32_t tpt_frame_seed_dev(uint64_t f) {
 // thread.
 __device__ __forceinline__ void tpt_tick_begin_dev(const tpt_tick_begin &tb) {
     uint32_t *c = tb.counters;
-    uint64_t f = tb.from_device ? (((uint64_t)c[TPT_CTR_FRAME_HI] << 32) | c[TPT_CTR_FRAME_LO])
-                                : (((uint64_t)tb.hi << 32) | tb.lo);
+    // Both frames are formed and one is selected by value: a selected
+    // address (the counters or the by-value argument) would put the argument
+    // in scratch memory for every thread of the calling kernel.
+    const uint64_t on_device = ((uint64_t)c[TPT_CTR_FRAME_HI] << 32) | c[TPT_CTR_FRAME_LO];
+    const uint64_t given = ((uint64_t)tb.hi << 32) | tb.lo;
+    uint64_t f = tb.from_device ? on_device : given;
     c[TPT_CTR_SEED] = tpt_frame_seed_dev(f);
     c[TPT_CTR_STACK] = (f % 10u) == 0u ? 1u : 0u;
     f += 1u;
