@@ -18,7 +18,7 @@ constexpr int BLK = 256;
 // reduces them in LDS.  Canonical rule WEQ: a column target that several
 // rows assign keeps the highest row's (64-bit atomicMax on (row, target)).
 
-constexpr int WEQ_CHUNK = 64;            // rows per column-pass thread
+constexpr int WEQ_CHUNK = 64;            // rows per column-pass block (16 per wave)
 constexpr int WEQ_MAX_X = (int)TPT_MAX_DIM;
 constexpr int WEQ_BLK = 256;
 constexpr int WEQ_PER = (WEQ_MAX_X + WEQ_BLK - 1) / WEQ_BLK;  // columns per thread in a row pass
@@ -48,26 +48,38 @@ struct WeqLiquidTable {
 
 // Column pass, part 1: the first and last liquid row of each (column,
 // 64-row chunk), folded into the column's min / max (max stored as row + 1).
-__global__ __launch_bounds__(64) void weq_column_chunks(const uint32_t *__restrict__ pmap,
-                                                        const uint32_t *__restrict__ props,
-                                                        uint32_t *__restrict__ col_min, uint32_t *__restrict__ col_max,
-                                                        int xres, int yres) {
+// Each of a block's four waves takes 16 rows of the chunk, all 16 loads in
+// flight at once; the waves combine in LDS and one atomic pair per column
+// and chunk goes out.
+__global__ __launch_bounds__(256) void weq_column_chunks(const uint32_t *__restrict__ pmap,
+                                                         const uint32_t *__restrict__ props,
+                                                         uint32_t *__restrict__ col_min, uint32_t *__restrict__ col_max,
+                                                         int xres, int yres) {
+    __shared__ uint32_t s_lo[4][64], s_hi[4][64];
     const WeqLiquidTable liquid(props);
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    if (x >= xres) return;
-    const int y0 = blockIdx.y * WEQ_CHUNK, y1 = min(y0 + WEQ_CHUNK, yres);
+    const int tx = threadIdx.x, w = threadIdx.y;
+    const int x = blockIdx.x * 64 + tx;
+    const int y0 = blockIdx.y * WEQ_CHUNK + w * 16, y1 = min(y0 + 16, yres);
     uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-    // Sixteen rows' loads in flight at a time.
-    for (int yb = y0; yb < y1; yb += 16) {
+    if (x < xres) {
         uint32_t v[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = yb + u < y1 ? pmap[(yb + u) * xres + x] : 0u;
+        for (int u = 0; u < 16; ++u) v[u] = y0 + u < y1 ? pmap[(y0 + u) * xres + x] : 0u;
 #pragma unroll
         for (int u = 0; u < 16; ++u)
             if (liquid(v[u])) {
-                if (lo == 0xFFFFFFFFu) lo = (uint32_t)(yb + u);
-                hi = (uint32_t)(yb + u) + 1u;
+                if (lo == 0xFFFFFFFFu) lo = (uint32_t)(y0 + u);
+                hi = (uint32_t)(y0 + u) + 1u;
             }
+    }
+    s_lo[w][tx] = lo;
+    s_hi[w][tx] = hi;
+    __syncthreads();
+    if (w != 0 || x >= xres) return;
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        lo = min(lo, s_lo[k][tx]);
+        hi = max(hi, s_hi[k][tx]);
     }
     if (hi) {
         atomicMin(&col_min[x], lo);
@@ -124,22 +136,26 @@ __global__ __launch_bounds__(WEQ_BLK) void weq_rows(const uint32_t *__restrict__
     const int y = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int nwords = (xres + 63) >> 6;
     // All of the row's loads are issued before any is used.
+    // Four adjacent lanes share a cell: its wall and emap words are loaded
+    // by the first of them and passed on by a shuffle.
     uint32_t pix[WEQ_PER], sf[WEQ_PER];
-    uint4 rec[WEQ_PER];
+    uint2 we[WEQ_PER];
+    const uint2 *__restrict__ cell_we = reinterpret_cast<const uint2 *>(cellrec);
 #pragma unroll
     for (int u = 0; u < WEQ_PER; ++u) {
         const int x = u * WEQ_BLK + t;
         const bool in = x < xres;
         pix[u] = in ? pmap[y * xres + x] : 0u;
         sf[u] = in ? col_surf[x] : 0xFFFFFFFFu;
-        rec[u] = in ? cellrec[(y >> 2) * xc + (x >> 2)] : make_uint4(0u, 0u, 0u, 0u);
+        we[u] = in && (lane & 3) == 0 ? cell_we[((y >> 2) * xc + (x >> 2)) * 2 + 1] : make_uint2(0u, 0u);
     }
     const WeqLiquidTable liquid(props);
     bool liq[WEQ_PER];
 #pragma unroll
     for (int u = 0; u < WEQ_PER; ++u) {
         liq[u] = liquid(pix[u]);
-        const bool link = !wall_blocks_particle(rec[u].z, rec[u].w, 0x2u);
+        const uint32_t wall = (uint32_t)__shfl((int)we[u].x, lane & ~3), emap = (uint32_t)__shfl((int)we[u].y, lane & ~3);
+        const bool link = !wall_blocks_particle(wall, emap, 0x2u);
         const unsigned long long lw = __ballot(liq[u]), kw = __ballot(link);
         const int w = u * (WEQ_BLK / 64) + wv;
         if (lane == 0 && w < nwords) {
@@ -296,8 +312,8 @@ int tpt_water_equal_step(tpt_ctx *ctx) {
     // pixel records no longer describe the state.
     ctx->maps_fresh = ctx->rebuild_fresh = ctx->rec_cover = false;
     tpt_pass_begin(ctx, PASS_WATER_EQUAL);
-    hipLaunchKernelGGL(weq_column_chunks, dim3((xres + 63) / 64, (yres + WEQ_CHUNK - 1) / WEQ_CHUNK), dim3(64), 0,
-                       ctx->stream, pmap, props, w.col_min, w.col_max, xres, yres);
+    hipLaunchKernelGGL(weq_column_chunks, dim3((xres + 63) / 64, (yres + WEQ_CHUNK - 1) / WEQ_CHUNK), dim3(64, 4),
+                       0, ctx->stream, pmap, props, w.col_min, w.col_max, xres, yres);
     hipLaunchKernelGGL(weq_column_surfaces, dim3((xres + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, pmap,
                        ctx->cell_rec, w.col_min, w.col_max, w.col_surf, w.col_target, ctx->counters, xres, xc);
     hipLaunchKernelGGL(weq_rows, dim3(yres), dim3(WEQ_BLK), 16u * (uint32_t)xres, ctx->stream, pmap, props,

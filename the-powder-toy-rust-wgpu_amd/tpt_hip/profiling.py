@@ -7,7 +7,9 @@
 writes `<out>.md` (per-kernel durations from the kernel trace, FETCH_SIZE and
 WRITE_SIZE per dispatch, calibrated HBM bytes) and `<out>_traffic.json` (the
 calibrated HBM bytes per dispatch by kernel, which bench.py reports as the
-roofline's `traffic`).
+roofline's `traffic`).  With only `--trace` (and `--commands`, `--out`) it
+writes the kernel-time table alone, `--match` keeping the kernels whose name
+contains one of the given substrings.
 
 Calibration: the same counters are collected for a device-to-device copy of
 `--copy-bytes` bytes (bench.py --calibrate-copy); each counter's scale is the
@@ -56,17 +58,29 @@ def copy_scale(db, counter, copy_bytes):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--trace", required=True)
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
-    ap.add_argument("--copy-fetch", required=True)
-    ap.add_argument("--copy-write", required=True)
-    ap.add_argument("--copy-bytes", type=int, required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-fetch")
+    ap.add_argument("--copy-write")
+    ap.add_argument("--copy-bytes", type=int)
     ap.add_argument("--commands", required=True, help="text file with the rocprofv3 commands that were run")
     ap.add_argument("--title", default="Whole tick, 2048x2048 canvas")
+    ap.add_argument("--match", nargs="*", default=None, help="keep kernels whose name contains one of these")
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
 
     trace = kernel_stats(a.trace)
+    if a.match:
+        trace = [t for t in trace if any(m in t[0] for m in a.match)]
+    if not a.fetch:
+        lines = [f"# {a.title}", "", "Command (run from /tmp with TMPDIR=/tmp):", "", "```",
+                 open(a.commands).read().strip(), "```", "",
+                 "| kernel | calls | avg us | total us | % GPU time |", "|---|---|---|---|---|"]
+        for name, calls, total, avg, pct in trace:
+            lines.append(f"| {name} | {calls} | {avg:.1f} | {total:.0f} | {pct:.1f} |")
+        with open(a.out + ".md", "w") as fh:
+            fh.write("\n".join(lines) + "\n")
+        return
     fetch = counter_means(a.fetch, "FETCH_SIZE")
     write = counter_means(a.write, "WRITE_SIZE")
     fscale = copy_scale(a.copy_fetch, "FETCH_SIZE", a.copy_bytes)
