@@ -279,6 +279,12 @@ def main():
     traffic, traffic_src = traffic_table(args.workload)
     kern = PASS_KERNEL.get(dom)
     measured = traffic.get(kern, {}).get("hbm_bytes") if (traffic and kern) else None
+    # The whole tick against the same roofline: the algorithmic bytes of
+    # every timed pass with a byte model, per tick, over the timed tick.
+    tick_bytes = sum(v["alg_bytes"] * v["launches"] for v in per_kernel.values() if v["alg_bytes"]) / args.steps
+    tick_gbs = tick_bytes / (dt / args.steps) / 1e9
+    tick_roofline = {"alg_bytes_per_tick": round(tick_bytes), "achieved": round(tick_gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(tick_gbs / HBM_PEAK_GBS, 4)}
     roofline = {"bound": "hbm", "kernel": kern or dom, "pass": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                 "traffic": measured, "traffic_source": traffic_src if measured else None,
@@ -321,6 +327,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "graph_replay": {"value": round(args.steps / dt_graph, 2), "ms_per_step": round(dt_graph / args.steps * 1e3, 4)},
+        "tick_roofline": tick_roofline,
         "per_kernel": per_kernel,
     }
     print(json.dumps(out))
