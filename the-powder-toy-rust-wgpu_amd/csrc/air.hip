@@ -36,11 +36,20 @@ __device__ __forceinline__ bool interior(int x, int y, AirDims d) {
     return x >= 1 && y >= 1 && x + 1 < d.xc && y + 1 < d.yc;
 }
 
-__device__ __forceinline__ uint32_t blocked_at(const uint32_t *__restrict__ b, int x, int y, AirDims d) {
-    return (x >= 0 && y >= 0 && x < d.xc && y < d.yc) ? b[y * d.xc + x] : 0u;
+// A grid read through a window: element (x, y) of the canvas grid sits at
+// p[(y - oy) * s + (x - ox)].  The pressure/velocity kernel reads its
+// inputs through LDS copies of its tile plus a 3-cell halo.
+template <typename T> struct GridView {
+    const T *p;
+    int ox, oy, s;
+    __device__ __forceinline__ T at(int x, int y) const { return p[(y - oy) * s + (x - ox)]; }
+};
+
+__device__ __forceinline__ uint32_t blocked_at(const GridView<uint32_t> &b, int x, int y, AirDims d) {
+    return (x >= 0 && y >= 0 && x < d.xc && y < d.yc) ? b.at(x, y) : 0u;
 }
 
-__device__ __forceinline__ bool wall_cleared_vx(const uint32_t *__restrict__ b, int x, int y, AirDims d) {
+__device__ __forceinline__ bool wall_cleared_vx(const GridView<uint32_t> &b, int x, int y, AirDims d) {
     bool out = false;
     if (y >= 1 && y + 1 < d.yc) {
         if (x >= 1 && x + 1 < d.xc && blocked_at(b, x, y, d)) out = true;
@@ -50,7 +59,7 @@ __device__ __forceinline__ bool wall_cleared_vx(const uint32_t *__restrict__ b, 
     return out;
 }
 
-__device__ __forceinline__ bool wall_cleared_vy(const uint32_t *__restrict__ b, int x, int y, AirDims d) {
+__device__ __forceinline__ bool wall_cleared_vy(const GridView<uint32_t> &b, int x, int y, AirDims d) {
     bool out = false;
     if (x >= 1 && x + 1 < d.xc) {
         if (y >= 1 && y + 1 < d.yc && blocked_at(b, x, y, d)) out = true;
@@ -60,19 +69,19 @@ __device__ __forceinline__ bool wall_cleared_vy(const uint32_t *__restrict__ b, 
     return out;
 }
 
-__device__ __forceinline__ float effective_v(const float *__restrict__ v, const uint32_t *__restrict__ b, int x,
-                                             int y, AirDims d, float edge, bool along_x) {
-    float val = v[y * d.xc + x];
+__device__ __forceinline__ float effective_v(const GridView<float> &v, const GridView<uint32_t> &b, int x, int y,
+                                             AirDims d, float edge, bool along_x) {
+    float val = v.at(x, y);
     if (in_border(x, y, d)) val = edge + (val - edge) * 0.9f;
     if (along_x ? wall_cleared_vx(b, x, y, d) : wall_cleared_vy(b, x, y, d)) val = 0.0f;
     return val;
 }
 
 // Pressure of pressure.rs:186-236 at (x, y).
-__device__ __forceinline__ float pressure_at(const float *__restrict__ pv, const float *__restrict__ vx,
-                                             const float *__restrict__ vy, const uint32_t *__restrict__ ba,
-                                             int x, int y, AirDims d, float ep, float evx, float evy) {
-    float cur = pv[y * d.xc + x];
+__device__ __forceinline__ float pressure_at(const GridView<float> &pv, const GridView<float> &vx,
+                                             const GridView<float> &vy, const GridView<uint32_t> &ba, int x, int y,
+                                             AirDims d, float ep, float evx, float evy) {
+    float cur = pv.at(x, y);
     if (in_border(x, y, d)) cur = ep + (cur - ep) * 0.8f;
     float next = cur;
     if (interior(x, y, d)) {
@@ -83,20 +92,60 @@ __device__ __forceinline__ float pressure_at(const float *__restrict__ pv, const
     return next;
 }
 
+// pressure.rs:174-238 then pressure.rs:248-299.  The tile's inputs plus a
+// 3-cell halo (the velocity's pressure gradient reads the pressure one cell
+// out, which reads velocity one more cell out, whose wall test reads the
+// mask one more) are staged in LDS with every load in flight at once; the
+// tile's pressure plus a one-cell halo is then computed into LDS, and the
+// velocity reads its gradient there.
+constexpr int PH = 3, SW = TX + 2 * PH, SH = TY + 2 * PH;
+
 __global__ __launch_bounds__(TX *TY) void air_pv_kernel(const float *__restrict__ pv, const float *__restrict__ vx,
                                                         const float *__restrict__ vy,
                                                         const uint32_t *__restrict__ ba, float *__restrict__ opv,
                                                         float *__restrict__ ovx, float *__restrict__ ovy, AirDims d,
                                                         float ep, float evx, float evy) {
+    __shared__ float s_in_pv[SH * SW], s_in_vx[SH * SW], s_in_vy[SH * SW];
+    __shared__ uint32_t s_in_ba[SH * SW];
     __shared__ float s_pv[TY + 2][TX + 2];
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
     const int tid = threadIdx.y * TX + threadIdx.x;
+    constexpr int STAGE = (SH * SW + TX * TY - 1) / (TX * TY);
+    {
+        float a[STAGE], b[STAGE], c[STAGE];
+        uint32_t m[STAGE];
+#pragma unroll
+        for (int k = 0; k < STAGE; ++k) {
+            const int i = tid + k * TX * TY, ly = i / SW, lx = i - ly * SW;
+            const int gx = x0 + lx - PH, gy = y0 + ly - PH;
+            const bool in = i < SH * SW && gx >= 0 && gy >= 0 && gx < d.xc && gy < d.yc;
+            const int g = in ? gy * d.xc + gx : 0;
+            a[k] = in ? pv[g] : 0.0f;
+            b[k] = in ? vx[g] : 0.0f;
+            c[k] = in ? vy[g] : 0.0f;
+            m[k] = in ? ba[g] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < STAGE; ++k) {
+            const int i = tid + k * TX * TY;
+            if (i < SH * SW) {
+                s_in_pv[i] = a[k];
+                s_in_vx[i] = b[k];
+                s_in_vy[i] = c[k];
+                s_in_ba[i] = m[k];
+            }
+        }
+    }
+    __syncthreads();
+    const GridView<float> P{s_in_pv, x0 - PH, y0 - PH, SW}, VX{s_in_vx, x0 - PH, y0 - PH, SW},
+        VY{s_in_vy, x0 - PH, y0 - PH, SW};
+    const GridView<uint32_t> B{s_in_ba, x0 - PH, y0 - PH, SW};
     // Pressure for the tile and its one-cell halo.
     for (int i = tid; i < (TY + 2) * (TX + 2); i += TX * TY) {
         int ly = i / (TX + 2), lx = i % (TX + 2);
         int gx = x0 + lx - 1, gy = y0 + ly - 1;
         float p = 0.0f;
-        if (gx >= 0 && gy >= 0 && gx < d.xc && gy < d.yc) p = pressure_at(pv, vx, vy, ba, gx, gy, d, ep, evx, evy);
+        if (gx >= 0 && gy >= 0 && gx < d.xc && gy < d.yc) p = pressure_at(P, VX, VY, B, gx, gy, d, ep, evx, evy);
         s_pv[ly][lx] = p;
     }
     __syncthreads();
@@ -106,21 +155,21 @@ __global__ __launch_bounds__(TX *TY) void air_pv_kernel(const float *__restrict_
     const int lx = threadIdx.x + 1, ly = threadIdx.y + 1;
     opv[idx] = s_pv[ly][lx];
 
-    float cvx = vx[idx], cvy = vy[idx];
+    float cvx = VX.at(x, y), cvy = VY.at(x, y);
     if (in_border(x, y, d)) {
         cvx = evx + (cvx - evx) * 0.9f;
         cvy = evy + (cvy - evy) * 0.9f;
     }
-    if (wall_cleared_vx(ba, x, y, d)) cvx = 0.0f;
-    if (wall_cleared_vy(ba, x, y, d)) cvy = 0.0f;
+    if (wall_cleared_vx(B, x, y, d)) cvx = 0.0f;
+    if (wall_cleared_vy(B, x, y, d)) cvy = 0.0f;
     float nvx = cvx, nvy = cvy;
     if (interior(x, y, d)) {
         float dx = s_pv[ly][lx - 1] - s_pv[ly][lx + 1];
         float dy = s_pv[ly - 1][lx] - s_pv[ly + 1][lx];
         nvx = evx + (cvx - evx) * 0.999f + dx * 0.2f;
         nvy = evy + (cvy - evy) * 0.999f + dy * 0.2f;
-        if (blocked_at(ba, x - 1, y, d) || blocked_at(ba, x, y, d) || blocked_at(ba, x + 1, y, d)) nvx = 0.0f;
-        if (blocked_at(ba, x, y - 1, d) || blocked_at(ba, x, y, d) || blocked_at(ba, x, y + 1, d)) nvy = 0.0f;
+        if (blocked_at(B, x - 1, y, d) || blocked_at(B, x, y, d) || blocked_at(B, x + 1, y, d)) nvx = 0.0f;
+        if (blocked_at(B, x, y - 1, d) || blocked_at(B, x, y, d) || blocked_at(B, x, y + 1, d)) nvy = 0.0f;
     }
     ovx[idx] = nvx;
     ovy[idx] = nvy;
