@@ -45,6 +45,49 @@ template <typename T> struct GridView {
     __device__ __forceinline__ T at(int x, int y) const { return p[(y - oy) * s + (x - ox)]; }
 };
 
+// A canvas grid in global memory, read as at(x, y).
+template <typename T> struct GlobalView {
+    const T *p;
+    int xc;
+    __device__ __forceinline__ T at(int x, int y) const { return p[y * xc + x]; }
+};
+
+// Stages the window of `n` grids (origin x0 - h, y0 - h, (TX + 2h) x
+// (TY + 2h)) into LDS, all loads in flight before the first store;
+// cells off the canvas are staged as zero and never read.
+template <int H, typename T0, typename T1, typename T2, typename T3>
+__device__ __forceinline__ void stage4(const T0 *__restrict__ g0, const T1 *__restrict__ g1,
+                                       const T2 *__restrict__ g2, const T3 *__restrict__ g3, T0 *s0, T1 *s1, T2 *s2,
+                                       T3 *s3, int x0, int y0, AirDims d) {
+    constexpr int W = TX + 2 * H, N = W * (TY + 2 * H), K = (N + TX * TY - 1) / (TX * TY);
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    T0 a[K];
+    T1 b[K];
+    T2 c[K];
+    T3 e[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = tid + k * TX * TY, ly = i / W, lx = i - ly * W;
+        const int gx = x0 + lx - H, gy = y0 + ly - H;
+        const bool in = i < N && gx >= 0 && gy >= 0 && gx < d.xc && gy < d.yc;
+        const int g = in ? gy * d.xc + gx : 0;
+        a[k] = in ? g0[g] : T0(0);
+        b[k] = in ? g1[g] : T1(0);
+        c[k] = in ? g2[g] : T2(0);
+        e[k] = in ? g3[g] : T3(0);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = tid + k * TX * TY;
+        if (i < N) {
+            s0[i] = a[k];
+            s1[i] = b[k];
+            s2[i] = c[k];
+            s3[i] = e[k];
+        }
+    }
+}
+
 __device__ __forceinline__ uint32_t blocked_at(const GridView<uint32_t> &b, int x, int y, AirDims d) {
     return (x >= 0 && y >= 0 && x < d.xc && y < d.yc) ? b.at(x, y) : 0u;
 }
@@ -176,8 +219,9 @@ __global__ __launch_bounds__(TX *TY) void air_pv_kernel(const float *__restrict_
 }
 
 // advect.rs:66-152.
-__device__ void far_advect_target(int x, int y, float dx, float dy, const uint32_t *__restrict__ block, AirDims d,
-                                  bool heat, float &otx, float &oty) {
+template <typename B>
+__device__ __forceinline__ void far_advect_target(int x, int y, float dx, float dy, const B &block, AirDims d,
+                                                  bool heat, float &otx, float &oty) {
     float tx = (float)x - dx * ADVECT_DIST;
     float ty = (float)y - dy * ADVECT_DIST;
     float adx = dx * ADVECT_DIST;
@@ -208,7 +252,7 @@ __device__ void far_advect_target(int x, int y, float dx, float dy, const uint32
             int cx = tpt_f2i(tx + 0.5f), cy = tpt_f2i(ty + 0.5f);
             bool hit = true;
             if (cx >= 0 && cy >= 0 && cx < d.xc && cy < d.yc) {
-                uint32_t b = block[cy * d.xc + cx];
+                uint32_t b = block.at(cx, cy);
                 hit = (heat ? (b & 0x8u) : b) != 0u;
             }
             if (hit) {
@@ -236,16 +280,15 @@ __device__ __forceinline__ float gauss_w(int dx, int dy) {
 }
 
 // advect.rs:10-42.
-__device__ __forceinline__ float vorticity_bmap(const float *__restrict__ vx, const float *__restrict__ vy,
-                                                const uint32_t *__restrict__ b, AirDims d, int x, int y) {
+__device__ __forceinline__ float vorticity_bmap(const GridView<float> &vx, const GridView<float> &vy,
+                                                const GridView<uint32_t> &b, AirDims d, int x, int y) {
     float out = 0.0f;
     if (x > 1 && x + 2 < d.xc && y > 1 && y + 2 < d.yc) {
-        int row = y * d.xc, rm = (y - 1) * d.xc, rp = (y + 1) * d.xc;
-        bool here = b[row + x] != 0u;
-        float dvydx = vy[row + x + 1] - vy[row + x - 1];
-        if (here || b[row + x + 1] != 0u || b[row + x - 1] != 0u) dvydx = 0.0f;
-        float dvxdy = vx[rp + x] - vx[rm + x];
-        if (here || b[rp + x] != 0u || b[rm + x] != 0u) dvxdy = 0.0f;
+        bool here = b.at(x, y) != 0u;
+        float dvydx = vy.at(x + 1, y) - vy.at(x - 1, y);
+        if (here || b.at(x + 1, y) != 0u || b.at(x - 1, y) != 0u) dvydx = 0.0f;
+        float dvxdy = vx.at(x, y + 1) - vx.at(x, y - 1);
+        if (here || b.at(x, y + 1) != 0u || b.at(x, y - 1) != 0u) dvxdy = 0.0f;
         out = (dvydx - dvxdy) * 0.5f;
     }
     return out;
@@ -256,10 +299,21 @@ __global__ __launch_bounds__(TX *TY) void air_advect_kernel(
     const uint32_t *__restrict__ ba, const uint32_t *__restrict__ wall, const float *__restrict__ fan_vx,
     const float *__restrict__ fan_vy, float *__restrict__ vx, float *__restrict__ vy, float *__restrict__ pv,
     AirDims d, float vorticity_coeff, uint32_t air_mode) {
-    const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
+    constexpr int H = 2, W = TX + 2 * H, N = W * (TY + 2 * H);
+    __shared__ float s_vx[N], s_vy[N], s_pv[N];
+    __shared__ uint32_t s_ba[N];
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    stage4<H>(ovx, ovy, opv, ba, s_vx, s_vy, s_pv, s_ba, x0, y0, d);
+    __syncthreads();
+    // The smoothing, the vorticity and the cell itself read the LDS window;
+    // the far-advection walk and its bilinear taps, which can leave it, read
+    // global memory.
+    const GridView<float> OVX{s_vx, x0 - H, y0 - H, W}, OVY{s_vy, x0 - H, y0 - H, W}, OPV{s_pv, x0 - H, y0 - H, W};
+    const GridView<uint32_t> BA{s_ba, x0 - H, y0 - H, W};
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
     if (x >= d.xc || y >= d.yc) return;
     const int idx = y * d.xc + x;
-    const float cvx = ovx[idx], cvy = ovy[idx], cpv = opv[idx];
+    const float cvx = OVX.at(x, y), cvy = OVY.at(x, y), cpv = OPV.at(x, y);
     float sx = 0.0f, sy = 0.0f, sp = 0.0f, ws = 0.0f;
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy)
@@ -271,11 +325,10 @@ __global__ __launch_bounds__(TX *TY) void air_advect_kernel(
             const float w = gauss_w(dx, dy);
             float a = cvx, b = cvy, c = cpv;
             if (nx >= 1 && ny >= 1 && nx + 1 < d.xc && ny + 1 < d.yc) {
-                int ni = ny * d.xc + nx;
-                if (ba[ni] == 0u) {
-                    a = ovx[ni];
-                    b = ovy[ni];
-                    c = opv[ni];
+                if (BA.at(nx, ny) == 0u) {
+                    a = OVX.at(nx, ny);
+                    b = OVY.at(nx, ny);
+                    c = OPV.at(nx, ny);
                 }
             }
             sx += a * w;
@@ -286,10 +339,10 @@ __global__ __launch_bounds__(TX *TY) void air_advect_kernel(
     float nvx = sx / ws, nvy = sy / ws, npv = sp / ws;
 
     float tx0, ty0;
-    far_advect_target(x, y, nvx, nvy, ba, d, false, tx0, ty0);
+    far_advect_target(x, y, nvx, nvy, GlobalView<uint32_t>{ba, d.xc}, d, false, tx0, ty0);
     const int i0 = tpt_f2i(tx0), j0 = tpt_f2i(ty0);
     const float fx = tx0 - (float)i0, fy = ty0 - (float)j0;
-    if (ba[idx] == 0u && i0 >= 2 && i0 < d.xc - 3 && j0 >= 2 && j0 < d.yc - 3 && fx >= 0.0f && fy >= 0.0f) {
+    if (BA.at(x, y) == 0u && i0 >= 2 && i0 < d.xc - 3 && j0 >= 2 && j0 < d.yc - 3 && fx >= 0.0f && fy >= 0.0f) {
         const int r0 = j0 * d.xc, r1 = (j0 + 1) * d.xc;
         const float w00 = (1.0f - fx) * (1.0f - fy);
         const float w10 = fx * (1.0f - fy);
@@ -302,11 +355,11 @@ __global__ __launch_bounds__(TX *TY) void air_advect_kernel(
     }
 
     if (vorticity_coeff > 0.0f && x > 1 && x + 2 < d.xc && y > 1 && y + 2 < d.yc) {
-        const float w = vorticity_bmap(ovx, ovy, ba, d, x, y);
-        float wxp = vorticity_bmap(ovx, ovy, ba, d, x + 1, y);
-        float wxm = vorticity_bmap(ovx, ovy, ba, d, x - 1, y);
-        float wyp = vorticity_bmap(ovx, ovy, ba, d, x, y + 1);
-        float wym = vorticity_bmap(ovx, ovy, ba, d, x, y - 1);
+        const float w = vorticity_bmap(OVX, OVY, BA, d, x, y);
+        float wxp = vorticity_bmap(OVX, OVY, BA, d, x + 1, y);
+        float wxm = vorticity_bmap(OVX, OVY, BA, d, x - 1, y);
+        float wyp = vorticity_bmap(OVX, OVY, BA, d, x, y + 1);
+        float wym = vorticity_bmap(OVX, OVY, BA, d, x, y - 1);
         if (wxp < 0.0f) wxp = 0.0f - wxp;
         if (wxm < 0.0f) wxm = 0.0f - wxm;
         if (wyp < 0.0f) wyp = 0.0f - wyp;
@@ -354,10 +407,20 @@ __global__ __launch_bounds__(TX *TY) void air_heat_kernel(const float *__restric
                                                           const uint32_t *__restrict__ bh, float *__restrict__ ohv,
                                                           float *__restrict__ nvx_out, float *__restrict__ nvy_out,
                                                           AirDims d, HeatParams p) {
-    const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
+    constexpr int H = 2, W = TX + 2 * H, N = W * (TY + 2 * H);
+    __shared__ float s_hv[N], s_vx[N], s_vy[N];
+    __shared__ uint32_t s_bh[N];
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    stage4<H>(hv, vx, vy, bh, s_hv, s_vx, s_vy, s_bh, x0, y0, d);
+    __syncthreads();
+    // The smoothing, the convection taps and the cell itself read the LDS
+    // window; the far-advection walk and its bilinear taps read global memory.
+    const GridView<float> HV{s_hv, x0 - H, y0 - H, W}, VX{s_vx, x0 - H, y0 - H, W}, VY{s_vy, x0 - H, y0 - H, W};
+    const GridView<uint32_t> BH{s_bh, x0 - H, y0 - H, W};
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
     if (x >= d.xc || y >= d.yc) return;
     const int idx = y * d.xc + x;
-    const float center = hv[idx], cvx = vx[idx], cvy = vy[idx];
+    const float center = HV.at(x, y), cvx = VX.at(x, y), cvy = VY.at(x, y);
     float sh = 0.0f, sx = 0.0f, sy = 0.0f, ws = 0.0f;
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy)
@@ -369,11 +432,10 @@ __global__ __launch_bounds__(TX *TY) void air_heat_kernel(const float *__restric
             const float w = gauss_w(dx, dy);
             float a = center, b = cvx, c = cvy;
             if (nx >= 1 && ny >= 1 && nx + 1 < d.xc && ny + 1 < d.yc) {
-                int ni = ny * d.xc + nx;
-                if ((bh[ni] & 0x8u) == 0u) {
-                    a = hv[ni];
-                    b = vx[ni];
-                    c = vy[ni];
+                if ((BH.at(nx, ny) & 0x8u) == 0u) {
+                    a = HV.at(nx, ny);
+                    b = VX.at(nx, ny);
+                    c = VY.at(nx, ny);
                 }
             }
             sh += a * w;
@@ -384,10 +446,10 @@ __global__ __launch_bounds__(TX *TY) void air_heat_kernel(const float *__restric
     float dh = sh / ws;
     const float svx = sx / ws, svy = sy / ws;
     float tx, ty;
-    far_advect_target(x, y, svx, svy, bh, d, true, tx, ty);
+    far_advect_target(x, y, svx, svy, GlobalView<uint32_t>{bh, d.xc}, d, true, tx, ty);
     const int i0 = tpt_f2i(tx), j0 = tpt_f2i(ty);
     const float fx = tx - (float)i0, fy = ty - (float)j0;
-    const uint32_t own = bh[idx];
+    const uint32_t own = BH.at(x, y);
     if ((own & 0x8u) == 0u && i0 >= 0 && i0 < d.xc - 1 && j0 >= 0 && j0 < d.yc - 1 && fx >= 0.0f && fy >= 0.0f) {
         const int r0 = j0 * d.xc, r1 = (j0 + 1) * d.xc;
         const float s00 = (bh[r0 + i0] & 0x8u) ? dh : hv[r0 + i0];
@@ -425,9 +487,8 @@ __global__ __launch_bounds__(TX *TY) void air_heat_kernel(const float *__restric
         }
         float cx = 0.0f - gx, cy = 0.0f - gy;
         if (p.convection_mode == 1u) {
-            const int rm = (y - 1) * d.xc;
-            const float weight = ((center - hv[idx - 1]) * cx + (center - hv[rm + x]) * cy) / 5000.0f;
-            if (weight > 0.0f && (bh[rm + x] & 0x8u) == 0u) {
+            const float weight = ((center - HV.at(x - 1, y)) * cx + (center - HV.at(x, y - 1)) * cy) / 5000.0f;
+            if (weight > 0.0f && (BH.at(x, y - 1) & 0x8u) == 0u) {
                 nvx += weight * cx;
                 nvy += weight * cy;
             }
