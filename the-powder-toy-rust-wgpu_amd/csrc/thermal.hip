@@ -547,12 +547,24 @@ __global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long 
                                                    const uint32_t *__restrict__ pmap,
                                                    const uint32_t *__restrict__ offmap,
                                                    const uint32_t *__restrict__ counters,
-                                                   unsigned long long *rec_list, int tile_rows) {
+                                                   unsigned long long *rec_list, int tile_rows, int list_rows,
+                                                   uint32_t *__restrict__ conv_tgt, uint32_t *__restrict__ conv_touch) {
     if ((int)blockIdx.y >= tile_rows) {
-        const uint32_t nb = (gridDim.y - (uint32_t)tile_rows) * gridDim.x;
-        const uint32_t b = (blockIdx.y - (uint32_t)tile_rows) * gridDim.x + blockIdx.x;
-        apply_list_part(p, pmap, props, offmap, counters, heat, rec_list, d,
-                        b * 256u + threadIdx.y * TW + threadIdx.x, nb * 256u);
+        const int extra = (int)blockIdx.y - tile_rows;
+        const uint32_t t = threadIdx.y * TW + threadIdx.x;
+        if (extra < list_rows) {
+            const uint32_t nb = (uint32_t)list_rows * gridDim.x, b = (uint32_t)extra * gridDim.x + blockIdx.x;
+            apply_list_part(p, pmap, props, offmap, counters, heat, rec_list, d, b * 256u + t, nb * 256u);
+        } else {
+            // The convection pass's proposal and touch words start at NONE
+            // for every slot (neither buffer is read by the apply).
+            const uint32_t nb = (gridDim.y - (uint32_t)(tile_rows + list_rows)) * gridDim.x;
+            const uint32_t b = (uint32_t)(extra - list_rows) * gridDim.x + blockIdx.x;
+            for (uint32_t k = b * 256u + t; k < d.slots; k += nb * 256u) {
+                conv_tgt[k] = NONE;
+                conv_touch[k] = NONE;
+            }
+        }
         return;
     }
     __shared__ uint32_t skey[LH][LW];
@@ -701,62 +713,91 @@ __device__ uint32_t convect_target(const Parts &p, const uint32_t *__restrict__ 
 // and the post-apply temperature in one 8-byte load) instead of pmap plus
 // two slot-array gathers.  Valid while the records are current: types as
 // at RecalcFreeParticles, temperatures as written by the apply kernels.
-__global__ __launch_bounds__(BLK) void th_convect_propose_rec(Parts p, const unsigned long long *__restrict__ rec,
-                                                              const uint32_t *__restrict__ props,
-                                                              const uint4 *__restrict__ crec,
-                                                              uint32_t *__restrict__ tgt, uint32_t *__restrict__ touch,
-                                                              Dims d, GravCfg g, const uint32_t *__restrict__ seedp) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.slots) return;
-    const uint32_t seed = *seedp;
-    touch[i] = NONE;
-    uint32_t target = NONE;
-    // Position and temperature loaded with the type (coherent), so that a
-    // liquid's two dependent random loads are the only latency it waits on.
-    const uint32_t t = p.ty[i];
-    const float fx = p.x[i], fy = p.y[i], own_temp = p.temp[i];
-    if (t != 0u && t < 256u && (EP(props, t, COL_PROPERTIES) & 0x02u) != 0u) {
-        const int x = tpt_f2i(fx), y = tpt_f2i(fy);
-        bool go = x >= 0 && x < d.xres && y >= 0 && y < d.yres;
-        if (go && t == T_GEL) {
-            const uint32_t roll = gpu_hash(seed ^ (i * PARTICLE_HASH_SALT) ^ t ^ TRANSITION_HASH_SALT);
-            const float gel = (float)p.tmp[i] * 2.55f;
-            const uint32_t gate_roll = gpu_hash(roll ^ 0x1B873593u) % 255u;
-            if (gel <= 1.0f + (float)gate_roll) go = false;
-        }
-        if (go) {
-            const int ci = (y / TPT_CELL) * d.xc + x / TPT_CELL;
-            const float pg = -2.0f;
-            float gx = 0.0f, gy = 0.0f;
-            if (g.mode == 0u) {
-                gy = pg;
-            } else if (g.mode == 2u) {
-                const float dx = (float)x - (float)d.xres * 0.5f, dy = (float)y - (float)d.yres * 0.5f;
-                if (dx != 0.0f || dy != 0.0f) {
-                    const float dd = 0.01f - sqrtf(dx * dx + dy * dy);
-                    gx = pg * (dx / dd);
-                    gy = pg * (dy / dd);
-                }
-            } else if (g.mode == 3u) {
-                gx = pg * g.cgx;
-                gy = pg * g.cgy;
-            }
-            const uint2 gf = *(const uint2 *)&crec[ci];
-            gx += -2.0f * __uint_as_float(gf.x);
-            gy += -2.0f * __uint_as_float(gf.y);
-            const float rxf = gx + (float)x, ryf = gy + (float)y;
-            const int rx = rxf >= 0.0f ? tpt_f2i(rxf + 0.5f) : tpt_f2i(rxf - 0.5f);
-            const int ry = ryf >= 0.0f ? tpt_f2i(ryf + 0.5f) : tpt_f2i(ryf - 0.5f);
-            const int ox = min(max(rx, x - 1), x + 1), oy = min(max(ry, y - 1), y + 1);
-            if ((ox != x || oy != y) && ox >= 0 && oy >= 0 && ox < d.xres && oy < d.yres) {
-                const unsigned long long v = rec[(size_t)oy * d.xres + ox];
-                const uint32_t key = (uint32_t)(v >> 32), nid = (key >> 9) - 1u;
-                if (key != 0u && nid < d.slots && (key & 0xFFu) == t && own_temp > __uint_as_float((uint32_t)v))
-                    target = nid;
-            }
-        }
+// The convection proposal of liquid slot i of type t at pixel (x, y) with
+// temperature own_temp, against the pixel records (convect_target with the
+// partner read from one 8-byte record).
+__device__ __forceinline__ uint32_t propose_rec(const Parts &p, const unsigned long long *__restrict__ rec,
+                                                const uint4 *__restrict__ crec, const Dims &d, const GravCfg &g,
+                                                uint32_t i, uint32_t t, int x, int y, float own_temp,
+                                                uint32_t seed) {
+    if (t == T_GEL) {
+        const uint32_t roll = gpu_hash(seed ^ (i * PARTICLE_HASH_SALT) ^ t ^ TRANSITION_HASH_SALT);
+        const float gel = (float)p.tmp[i] * 2.55f;
+        const uint32_t gate_roll = gpu_hash(roll ^ 0x1B873593u) % 255u;
+        if (gel <= 1.0f + (float)gate_roll) return NONE;
     }
-    tgt[i] = target;
+    const int ci = (y / TPT_CELL) * d.xc + x / TPT_CELL;
+    const float pg = -2.0f;
+    float gx = 0.0f, gy = 0.0f;
+    if (g.mode == 0u) {
+        gy = pg;
+    } else if (g.mode == 2u) {
+        const float dx = (float)x - (float)d.xres * 0.5f, dy = (float)y - (float)d.yres * 0.5f;
+        if (dx != 0.0f || dy != 0.0f) {
+            const float dd = 0.01f - sqrtf(dx * dx + dy * dy);
+            gx = pg * (dx / dd);
+            gy = pg * (dy / dd);
+        }
+    } else if (g.mode == 3u) {
+        gx = pg * g.cgx;
+        gy = pg * g.cgy;
+    }
+    const uint2 gf = *(const uint2 *)&crec[ci];
+    gx += -2.0f * __uint_as_float(gf.x);
+    gy += -2.0f * __uint_as_float(gf.y);
+    const float rxf = gx + (float)x, ryf = gy + (float)y;
+    const int rx = rxf >= 0.0f ? tpt_f2i(rxf + 0.5f) : tpt_f2i(rxf - 0.5f);
+    const int ry = ryf >= 0.0f ? tpt_f2i(ryf + 0.5f) : tpt_f2i(ryf - 0.5f);
+    const int ox = min(max(rx, x - 1), x + 1), oy = min(max(ry, y - 1), y + 1);
+    if ((ox != x || oy != y) && ox >= 0 && oy >= 0 && ox < d.xres && oy < d.yres) {
+        const unsigned long long v = rec[(size_t)oy * d.xres + ox];
+        const uint32_t key = (uint32_t)(v >> 32), nid = (key >> 9) - 1u;
+        if (key != 0u && nid < d.slots && (key & 0xFFu) == t && own_temp > __uint_as_float((uint32_t)v))
+            return nid;
+    }
+    return NONE;
+}
+
+// Pixel-major convection proposals over the records RecalcFreeParticles left:
+// one thread per pixel, its unflagged owner proposing (the record holds its
+// type, and its temperature as the apply left it; for such owners the
+// rounded pixel is the truncated one the proposal uses).  Blocks past the
+// pixels take the off-map list's slots.  Every slot's tgt / touch was set
+// to NONE by the apply launch, so only proposals are stored.
+__global__ __launch_bounds__(BLK) void th_convect_propose_px(Parts p, const unsigned long long *__restrict__ rec,
+                                                             const uint32_t *__restrict__ props,
+                                                             const uint4 *__restrict__ crec,
+                                                             const uint32_t *__restrict__ offmap,
+                                                             const uint32_t *__restrict__ counters,
+                                                             uint32_t *__restrict__ tgt, Dims d, GravCfg g,
+                                                             const uint32_t *__restrict__ seedp,
+                                                             uint32_t pixel_blocks) {
+    const uint32_t seed = *seedp;
+    if (blockIdx.x >= pixel_blocks) {
+        const uint32_t n = counters[TPT_CTR_OFFMAP];
+        const uint32_t stride = (gridDim.x - pixel_blocks) * blockDim.x;
+        for (uint32_t k = (blockIdx.x - pixel_blocks) * blockDim.x + threadIdx.x; k < n; k += stride) {
+            const uint32_t i = offmap[k];
+            if (i >= d.slots) continue;
+            const uint32_t t = p.ty[i];
+            if (t == 0u || t >= 256u || (EP(props, t, COL_PROPERTIES) & 0x02u) == 0u) continue;
+            const int x = tpt_f2i(p.x[i]), y = tpt_f2i(p.y[i]);
+            if (x < 0 || x >= d.xres || y < 0 || y >= d.yres) continue;
+            const uint32_t target = propose_rec(p, rec, crec, d, g, i, t, x, y, p.temp[i], seed);
+            if (target != NONE) tgt[i] = target;
+        }
+        return;
+    }
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint32_t)(d.xres * d.yres)) return;
+    const unsigned long long v = rec[idx];
+    const uint32_t key = (uint32_t)(v >> 32);
+    if (key == 0u || (key & 0x100u)) return;
+    const uint32_t i = (key >> 9) - 1u, t = key & 0xFFu;
+    if (i >= d.slots || (EP(props, t, COL_PROPERTIES) & 0x02u) == 0u) return;
+    const int x = (int)(idx % (uint32_t)d.xres), y = (int)(idx / (uint32_t)d.xres);
+    const uint32_t target = propose_rec(p, rec, crec, d, g, i, t, x, y, __uint_as_float((uint32_t)v), seed);
+    if (target != NONE) tgt[i] = target;
 }
 
 __global__ __launch_bounds__(BLK) void th_convect_propose(Parts p, const uint32_t *__restrict__ pmap,
@@ -1154,6 +1195,7 @@ int tpt_thermal_step(tpt_ctx *ctx, int legacy) {
     const uint32_t *pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP), *props = tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS);
     const float *gfx = tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), *gfy = tpt_buf<float>(ctx, TPT_BUF_GRAV_FY);
     const bool fresh = ctx->maps_fresh;
+    const uint32_t pixel_blocks = (ctx->pixels + BLK - 1) / BLK;
     ctx->maps_fresh = ctx->rebuild_fresh = false;
     if (!legacy && fresh) {
         // Pixel-major path over the records of the RecalcFreeParticles that
@@ -1173,12 +1215,14 @@ int tpt_thermal_step(tpt_ctx *ctx, int legacy) {
                            ctx->cell_scratch_u, ctx->cell_scratch_u2, ctx->pix_heat, d, aheat, seed);
         tpt_pass_end(ctx, PASS_CONDUCT);
         tpt_pass_begin(ctx, PASS_CONDUCT_APPLY);
-        // The tiles plus rows of blocks for the off-map list (about 256).
-        const int list_rows = (int)((256u + tg.x - 1u) / tg.x);
-        const dim3 ag(tg.x, tg.y + (uint32_t)list_rows);
+        // The tiles, rows of blocks for the off-map list (about 256 blocks)
+        // and rows that reset the convection words (about 1024 blocks).
+        const int list_rows = (int)((256u + tg.x - 1u) / tg.x), fill_rows = (int)((1024u + tg.x - 1u) / tg.x);
+        const dim3 ag(tg.x, tg.y + (uint32_t)(list_rows + fill_rows));
         hipLaunchKernelGGL(th_apply_px, ag, tb, 0, ctx->stream, p, rec, (uint32_t *)ctx->pix_rec,
                            (const uint2 *)ctx->pix_heat, props, d, pmap, ctx->offmap, ctx->counters,
-                           (unsigned long long *)ctx->pix_rec, (int)tg.y);
+                           (unsigned long long *)ctx->pix_rec, (int)tg.y, list_rows, ctx->slot_scratch_u,
+                           ctx->slot_scratch_u2);
         tpt_pass_end(ctx, PASS_CONDUCT_APPLY);
     } else if (!legacy) {
         tpt_pass_begin(ctx, PASS_CONDUCT);
@@ -1202,9 +1246,9 @@ int tpt_thermal_step(tpt_ctx *ctx, int legacy) {
     if (!legacy) {
         const GravCfg g{ctx->cfg.gravity_mode, ctx->cfg.custom_gravity_x, ctx->cfg.custom_gravity_y};
         if (fresh)
-            hipLaunchKernelGGL(th_convect_propose_rec, sg, b, 0, ctx->stream, p,
-                               (const unsigned long long *)ctx->pix_rec, props, ctx->cell_rec, ctx->slot_scratch_u,
-                               ctx->slot_scratch_u2, d, g, seed);
+            hipLaunchKernelGGL(th_convect_propose_px, dim3(pixel_blocks + 256u), b, 0, ctx->stream, p,
+                               (const unsigned long long *)ctx->pix_rec, props, ctx->cell_rec, ctx->offmap,
+                               ctx->counters, ctx->slot_scratch_u, d, g, seed, pixel_blocks);
         else
             hipLaunchKernelGGL(th_convect_propose, sg, b, 0, ctx->stream, p, pmap, props, gfx, gfy,
                                ctx->slot_scratch_u, ctx->slot_scratch_u2, d, g, seed);
