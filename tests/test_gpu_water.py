@@ -73,3 +73,23 @@ def test_whole_ticks_with_water_bit_exact(gpu, scene, ticks):
             o.call("oracle_step", frame)
         ctx.fetch(st)
     compare(ref, st, INT_FIELDS + FLOAT_FIELDS + ["pmap_count"], f"{scene} with water")
+
+
+def test_water_ticks_through_graph_replay(gpu):
+    """tpt_run with the water pass on: the first tick eager, the rest graph
+    replays whose recalc is the full one (the water pass leaves no
+    incremental maps), against the oracle's ticks."""
+    st = S.make_scene("612x384", scale_particles=0.5, newton_grav=False)
+    st.cfg["water_equal_test"] = 1
+    ref = st.copy()
+    o = O.OracleState(ref)
+    print("[water parity] 612x384 via tpt_run, 24 ticks", flush=True)
+    with L.Context(st.cfg) as ctx:
+        ctx.load_state(st)
+        ctx.run(0, 13)
+        ctx.run(13, 11)
+        ctx.fetch(st)
+    for frame in range(24):
+        o.call("oracle_step", frame)
+    compare(ref, st, INT_FIELDS + FLOAT_FIELDS + ["pmap_count"], "612x384 with water, graph replay")
+    assert st.slot_bound == ref.slot_bound
