@@ -14,6 +14,13 @@
  * Races in the reference (non-atomic read-modify-writes) are replaced by one
  * canonical serialisation shared with the HIP kernels; each affected function
  * names the rule it implements.
+ *
+ * Pinning: every function is checked by tests/test_oracle_*.py against the
+ * reference's own test values, closed forms, invariants or brute force on
+ * small inputs.  Parity unpinned: the outcome of the reference's racy
+ * kernels (which these canonical rules replace), its CubeCL lowering
+ * details (contraction, division ULP), and the f32 twiddles of its FFT
+ * (the solve here is FP64 and the HIP FFT is held to it within 1e-4).
  */
 #ifndef TPT_ORACLE_H
 #define TPT_ORACLE_H
