@@ -230,6 +230,10 @@ __global__ __launch_bounds__(BLK) void recalc_decay(
     if (ty == 0u || ty >= (uint32_t)TPT_PT_NUM) return;
     const uint32_t props = elem_props[ty * 32u + 1u];
     if ((props & (0x4000u | 0x8000u)) == 0u) return;
+    // Neither rule can apply (a LIFE_DEC particle with no life left and no
+    // LIFE_KILL, the common case): nothing to do, whatever the stasis test.
+    const int32_t life = life_a[i];
+    if (!(life > 0 && (props & 0x4000u)) && !(life <= 0 && (props & 0x8000u))) return;
     const int x = tpt_f2i(px_a[i] + 0.5f), y = tpt_f2i(py_a[i] + 0.5f);
     const bool in_bounds = x >= 0 && y >= 0 && x < xres && y < yres;
     bool stasis = false;
@@ -237,7 +241,6 @@ __global__ __launch_bounds__(BLK) void recalc_decay(
         const int cell = (y / TPT_CELL) * xc + x / TPT_CELL;
         stasis = (wall_bmap[cell] & 0xFFu) == 18u && (wall_emap[cell] & 0xFFu) < 8u;
     }
-    const int32_t life = life_a[i];
     bool killed = false;
     if (life > 0 && (props & 0x4000u) && (!in_bounds || !stasis)) {
         const int32_t next = life - 1;
