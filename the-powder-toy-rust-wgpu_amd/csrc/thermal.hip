@@ -654,7 +654,9 @@ __global__ __launch_bounds__(256) void th_apply_px(Parts p, const unsigned long 
                 }
         }
         const float nt = tpt_clampf(has ? best_temp : own, 0.0f, 9999.0f);
-        p.temp[j] = nt;
+        // The record's temperature is the slot's current one, so an unchanged
+        // value needs no store (a random store costs a whole line).
+        if (__float_as_uint(nt) != __float_as_uint(own)) p.temp[j] = nt;
         // The record keeps the post-apply temperature for the convection pass.
         rec_temp[2 * ((size_t)(y0 + ly - 1) * d.xres + (x0 + lx - 1))] = __float_as_uint(nt);
     }
