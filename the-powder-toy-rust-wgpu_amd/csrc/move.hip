@@ -1472,7 +1472,7 @@ __global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const u
     const uint32_t ty = s.ty[i], raw = s.intent[i], flags = s.flags[i];
     if (ty == 0u || ty >= 256u) return;
     if (raw == NO_INTENT) {
-        if (lut[ty * 12u + 1u] >= 2u && (flags & 0x10u) == 0u) s.flags[i] = flags | 0x01u;
+        if (lut[ty * 12u + 1u] >= 2u && (flags & 0x11u) == 0u) s.flags[i] = flags | 0x01u;
         return;
     }
     const bool half_mirror = (raw & HALF_MIRROR) != 0u;
@@ -1483,7 +1483,7 @@ __global__ __launch_bounds__(BLK) void mv_commit(Slots s, CommitBufs cb, const u
         return;
     }
     if (target >= (uint32_t)(xres * yres)) return;
-    s.flags[i] = flags & 0xFFFFFFFEu;
+    if (flags & 0x01u) s.flags[i] = flags & 0xFFFFFFFEu;
     // The claim, the destination's occupant and its wall in one round trip.
     const int nx = (int)(target % (uint32_t)xres), ny = (int)(target / (uint32_t)xres);
     const int wc = (ny / TPT_CELL) * xc + nx / TPT_CELL;
