@@ -98,19 +98,29 @@ def test_tick_invariants_on_walled_scene():
 
 
 def test_sub_pixel_step_matches_hash():
-    # A lone WTRV (gas, vy ~ -0.1 +- diffusion) moves by at most one pixel per axis.
-    st = box([("WTRV", 64, 64, 0.0, 0.0)])
-    o = O.OracleState(st)
-    o.call("oracle_velocity_intent", 99)
-    h1 = pcg(99 ^ 0)
-    h2 = pcg(h1)
-    vx, vy = st.arrays["part_vx"][0], st.arrays["part_vy"][0]
-    it = st.arrays["intent_target"][0]
-    if it != 0xFFFFFFFF:
-        tx, ty_ = int(it) % 128, int(it) // 128
-        assert abs(tx - 64) <= 1 and abs(ty_ - 64) <= 1
-    # the stochastic step fires iff r < |v| per axis (velocity.rs:98-111)
-    assert abs(vx) < 1.0 and abs(vy) < 1.0
+    """A lone WTRV at rest: v = g + diffusion * ((h & 0xFFFF) * 2/65535 - 1)
+    with h1 = pcg(seed ^ 0), h2 = pcg(h1) (velocity.rs:160-181), then the
+    stochastic unit step fires on each axis iff (h & 0xFFFF)/65535 < |v|
+    (velocity.rs:98-111).  Every value is recomputed here in float32."""
+    f = np.float32
+    lut = E.move_lut()
+    t = E.pt("WTRV")
+    grav, diff = lut[t * 12 + 9:t * 12 + 11].view(np.float32)
+    for seed in (99, 4242, 0x85EBCA6B, 7):
+        st = box([("WTRV", 64, 64, 0.0, 0.0)])
+        O.OracleState(st).call("oracle_velocity_intent", seed)
+        h1 = pcg(seed ^ 0)
+        h2 = pcg(h1)
+        k = f(2.0) / f(65535.0)
+        vx = f(0.0) * lut[t * 12 + 7:t * 12 + 8].view(np.float32)[0] + f(0.0) + f(0.0) + diff * (f(h1 & 0xFFFF) * k - f(1.0))
+        vy = f(0.0) + f(0.0) + grav + diff * (f(h2 & 0xFFFF) * k - f(1.0))
+        assert st.arrays["part_vx"][0].view(np.uint32) == f(vx).view(np.uint32)
+        assert st.arrays["part_vy"][0].view(np.uint32) == f(vy).view(np.uint32)
+        unit = f(1.0) / f(65535.0)
+        sx = int(np.sign(vx)) if f(h1 & 0xFFFF) * unit < abs(vx) else 0
+        sy = int(np.sign(vy)) if f(h2 & 0xFFFF) * unit < abs(vy) else 0
+        want = 0xFFFFFFFF if (sx, sy) == (0, 0) else (64 + sy) * 128 + 64 + sx
+        assert st.arrays["intent_target"][0] == want, seed
 
 
 # --- photon refraction at glass (velocity.rs:262-647) -----------------------
