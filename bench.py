@@ -68,6 +68,17 @@ PASS_KERNEL = {"move_velocity_intent": "mv_intent", "move_commit": "mv_commit", 
                "grav_fft_cols": "grav_cols", "thermal_apply": "th_apply_px"}
 
 
+def reference_tick_bytes(C, P, N, G):
+    """Algorithmic bytes of one tick of the reference's passes, every logical
+    array read once and written once per pass (C air cells, P pixels, N live
+    particles, G cells of the padded FFT grid): air 140C; gravity FFT and
+    mass 96G + 32C + 36N; conduction and apply 32N + 4P + 16C; movement
+    68N + 20P + 24C; RecalcFreeParticles 12P + 32N; phase transitions and
+    the pmap refresh 68N + 28C."""
+    return (140 * C + (96 * G + 32 * C + 36 * N) + (32 * N + 4 * P + 16 * C) + (68 * N + 20 * P + 24 * C)
+            + (12 * P + 32 * N) + (68 * N + 28 * C))
+
+
 def traffic_table(workload):
     """Calibrated HBM bytes per dispatch by kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, if any."""
@@ -281,32 +292,59 @@ def main():
     measured = traffic.get(kern, {}).get("hbm_bytes") if (traffic and kern) else None
     # The whole tick against the same roofline: the algorithmic bytes of
     # every timed pass with a byte model, per tick, over the timed tick.
-    tick_bytes = sum(v["alg_bytes"] * v["launches"] for v in per_kernel.values() if v["alg_bytes"]) / args.steps
+    design_bytes = sum(v["alg_bytes"] * v["launches"] for v in per_kernel.values() if v["alg_bytes"]) / args.steps
+    tick_bytes = reference_tick_bytes(C, P, N, W * H)
     tick_gbs = tick_bytes / (dt / args.steps) / 1e9
     tick_roofline = {"alg_bytes_per_tick": round(tick_bytes), "achieved": round(tick_gbs, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(tick_gbs / HBM_PEAK_GBS, 4)}
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(tick_gbs / HBM_PEAK_GBS, 4),
+                     "byte_model": "reference passes, each array streamed once: air 140C, gravity 96G+32C+36N, "
+                                   "conduction 32N+4P+16C, movement 68N+20P+24C, recalc 12P+32N, "
+                                   "transitions 68N+28C",
+                     "design_bytes_per_tick": round(design_bytes),
+                     "design_frac": round(design_bytes / (dt / args.steps) / 1e9 / HBM_PEAK_GBS, 4)}
     roofline = {"bound": "hbm", "kernel": kern or dom, "pass": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                 "traffic": measured, "traffic_source": traffic_src if measured else None,
                 "alg_bytes_per_launch": d["alg_bytes"], "avg_us": d["avg_us"]}
     ctx.close()
 
-    cpu = None
+    cpu, cpu_configs = None, None
     if not args.no_cpu:
         from oracle import oracle as O
+        olib = O.lib()
+        all_threads = olib.oracle_max_threads()
+
+        def oracle_rate(state, threads, seconds, max_ticks):
+            """Ticks/s of the C oracle over a bounded sample of whole ticks."""
+            olib.oracle_set_threads(threads)
+            o = O.OracleState(state)
+            n_cpu, t0 = 0, time.perf_counter()
+            while True:
+                for p in passes:
+                    oracle_pass(o, p, n_cpu)
+                n_cpu += 1
+                el = time.perf_counter() - t0
+                if el >= seconds or n_cpu >= max_ticks:
+                    return n_cpu, n_cpu / el
+
         ref = scene()
         ref.cfg["water_equal_test"] = st.cfg["water_equal_test"]
-        o = O.OracleState(ref)
-        n_cpu, t0 = 0, time.perf_counter()
-        while True:
-            for p in passes:
-                oracle_pass(o, p, n_cpu)
-            n_cpu += 1
-            el = time.perf_counter() - t0
-            if el >= args.cpu_seconds or n_cpu >= args.steps:
-                break
-        cpu = {"value": round(n_cpu / el, 3), "unit": "ticks/s", "cores": 1, "kind": "port",
-               "sample": f"{n_cpu} ticks of the same {args.workload} workload and passes, single-threaded C oracle"}
+        n_cpu, rate = oracle_rate(ref, all_threads, args.cpu_seconds, args.steps)
+        cpu = {"value": round(rate, 3), "unit": "ticks/s", "cores": all_threads, "kind": "port",
+               "sample": f"{n_cpu} ticks of the same {args.workload} workload and passes, C oracle, "
+                         f"OpenMP over {all_threads} threads (air stencils, FFT rows, gravity and "
+                         f"thermal apply, conduction; the passes with ordered rules serial)"}
+        # Configs 0-2 on all threads and on one, a few seconds each.
+        cpu_configs = {}
+        if not args.scene:
+            for wl in ("612x384", "1600x900", "2048x2048"):
+                row = {}
+                for threads in (all_threads, 1):
+                    state = S.make_scene(wl)
+                    n_cpu, rate = oracle_rate(state, threads, args.cpu_seconds / 3.0, 1000)
+                    row[f"threads_{threads}"] = {"ticks_per_s": round(rate, 3), "ticks": n_cpu}
+                cpu_configs[wl] = row
+        olib.oracle_set_threads(all_threads)
 
     out = {
         "metric": "ticks/s @2048x2048 plain vs batched vs air-surrogate (relL2<=1e-2); HBM GB/s",
@@ -326,6 +364,7 @@ def main():
                    "air_cells": C, "pixels": P, "particles": N, "parallelism": "single-gpu"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_configs": cpu_configs,
         "graph_replay": {"value": round(args.steps / dt_graph, 2), "ms_per_step": round(dt_graph / args.steps * 1e3, 4)},
         "tick_roofline": tick_roofline,
         "per_kernel": per_kernel,

@@ -103,6 +103,19 @@ typedef enum tpt_buf_id {
     TPT_BUF_BLOCK_AIR, TPT_BUF_BLOCK_AIRH, TPT_BUF_ELEM_BLOCK_AIR,
     TPT_BUF_ELEM_BLOCK_AIRH, TPT_BUF_GRAV_MASK, TPT_BUF_GRAV_MASS,
     TPT_BUF_GRAV_MASS_PREV, TPT_BUF_GRAV_FX, TPT_BUF_GRAV_FY,
+    /* PGRV/NGRV tool wells, f32 mass per cell (SimBuffers::grav_tool_mass,
+       buffers/mod.rs:316-325).  tpt_step_gravity folds every non-zero cell
+       into the snapshot its solve reads and zeroes it
+       (seed_gravity_tool_mass_kernel, gravity_kernel.rs:241-262). */
+    TPT_BUF_GRAV_TOOL_MASS,
+    /* One-shot element mass assignments (CEXP, GSNS), i32 in 1/65536 units,
+       and their write flags (gpu_grav_mass_override{,_set},
+       buffers/mod.rs:326-347).  tpt_step_gravity writes each flagged value
+       over the mass it built (apply_grav_mass_override_kernel,
+       gravity_kernel.rs:379-391); tpt_step zeroes both grids where the
+       reference's element pass clears them, between the thermal and
+       movement passes (reaction_kernel/modext/mod.rs:53-99). */
+    TPT_BUF_GRAV_MASS_OVERRIDE, TPT_BUF_GRAV_MASS_OVERRIDE_SET,
     /* tables */
     TPT_BUF_ELEM_PROPS,  /* TPT_PT_NUM*TPT_ELEM_STRIDE words   */
     TPT_BUF_MOVE_LUT,    /* TPT_MOVE_LUT_WORDS words            */

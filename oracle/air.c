@@ -87,6 +87,7 @@ static float effective_vy(const float *vy, const uint32_t *b, uint32_t x, uint32
 void oracle_air_pressure(oracle_state *s) {
     const uint32_t xc = s->xres / 4, yc = s->yres / 4;
     const float ep = s->edge_pressure;
+#pragma omp parallel for schedule(static)
     for (uint32_t y = 0; y < yc; ++y)
         for (uint32_t x = 0; x < xc; ++x) {
             uint32_t idx = y * xc + x;
@@ -108,6 +109,7 @@ void oracle_air_velocity(oracle_state *s) {
     const uint32_t xc = s->xres / 4, yc = s->yres / 4;
     const float evx = s->edge_velocity_x, evy = s->edge_velocity_y;
     const float *opv = s->air_opv;
+#pragma omp parallel for schedule(static)
     for (uint32_t y = 0; y < yc; ++y)
         for (uint32_t x = 0; x < xc; ++x) {
             uint32_t idx = y * xc + x;
@@ -217,6 +219,7 @@ void oracle_air_smooth_advect(oracle_state *s) {
     const uint32_t xc = s->xres / 4, yc = s->yres / 4, n = xc * yc;
     const float *ovx = s->air_ovx, *ovy = s->air_ovy, *opv = s->air_opv;
     const uint32_t *ba = s->block_air;
+#pragma omp parallel for schedule(static)
     for (uint32_t y = 0; y < yc; ++y)
         for (uint32_t x = 0; x < xc; ++x) {
             uint32_t idx = y * xc + x;
@@ -301,6 +304,7 @@ void oracle_air_heat_convect(oracle_state *s) {
     memcpy(vx0, s->air_vx, sizeof(float) * n);
     memcpy(vy0, s->air_vy, sizeof(float) * n);
     const float amb = s->ambient_air_temp;
+#pragma omp parallel for schedule(static)
     for (uint32_t y = 0; y < yc; ++y)
         for (uint32_t x = 0; x < xc; ++x) {
             uint32_t idx = y * xc + x;
@@ -418,6 +422,7 @@ void oracle_air_block_masks(oracle_state *s) {
 /* settle.rs:119-160; PT_GAS=10, PT_NBLE=97, hot_air is word 11 of the row */
 void oracle_hot_air_pressure(oracle_state *s) {
     const uint32_t xc = s->xres / 4, yc = s->yres / 4;
+#pragma omp parallel for schedule(static)
     for (uint32_t cy = 0; cy < yc; ++cy)
         for (uint32_t cx = 0; cx < xc; ++cx) {
             uint32_t cell = cy * xc + cx;

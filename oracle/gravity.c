@@ -57,6 +57,7 @@ void oracle_fft_twiddle_f32(uint32_t max_len, float *re, float *im) {
 void oracle_fft_transform_f32(float *data, uint32_t rows, uint32_t len, uint32_t inner, uint32_t stride,
                               const float *tw_re, const float *tw_im, uint32_t tw_bits, float sign) {
     uint32_t bits = bit_count(len);
+#pragma omp parallel for schedule(static)
     for (uint32_t r = 0; r < rows; ++r)
         for (uint32_t i = 0; i < len; ++i) {
             uint32_t src = i, rev = 0;
@@ -75,6 +76,7 @@ void oracle_fft_transform_f32(float *data, uint32_t rows, uint32_t len, uint32_t
         }
     for (uint32_t half = 1; half < len; half *= 2) {
         uint32_t shift = tw_bits - bit_count(half * 2);
+#pragma omp parallel for schedule(static)
         for (uint32_t r = 0; r < rows; ++r)
             for (uint32_t k = 0; k < len / 2; ++k) {
                 uint32_t l2 = half * 2, block = k / half, pos = k % half;
@@ -319,6 +321,7 @@ void oracle_build_gravity_mass(oracle_state *s) {
 /* apply_gravity_kernel (gravity_kernel.rs:434-507), newton_grav scale 1.0 */
 void oracle_apply_gravity(oracle_state *s) {
     const uint32_t xres = s->xres, yres = s->yres, xc = xres / 4;
+#pragma omp parallel for schedule(static)
     for (uint32_t i = 0; i < s->slot_bound; ++i) {
         uint32_t ty = s->part_ty[i];
         if (ty == 0 || ty >= 256) continue;
@@ -344,8 +347,7 @@ void oracle_apply_gravity(oracle_state *s) {
     }
 }
 
-/* step_gravity_gpu (gravity_kernel.rs:826-1016) without the UI tool wells
- * and the CEXP/GSNS overrides (both come from outside the tick's passes). */
+/* step_gravity_gpu (gravity_kernel.rs:826-1016). */
 void oracle_step_gravity(oracle_state *s) {
     const uint32_t n = (s->xres / 4) * (s->yres / 4);
     if (!s->newton_grav) {
@@ -353,11 +355,35 @@ void oracle_step_gravity(oracle_state *s) {
         memset(s->grav_fy, 0, sizeof(float) * n);
         return;
     }
+    /* seed_gravity_tool_mass_kernel (gravity_kernel.rs:241-262, launched
+     * at :903): a non-zero well replaces the snapshot cell, then is spent. */
+    for (uint32_t c = 0; c < n; ++c) {
+        const float tool = s->grav_tool_mass[c];
+        if (tool != 0.0f) {
+            float scaled = tool * 65536.0f;
+            if (scaled >= 0.0f) scaled += 0.5f;
+            else scaled -= 0.5f;
+            s->grav_mass_prev[c] = ora_f2i(scaled);
+            s->grav_tool_mass[c] = 0.0f;
+        }
+    }
     oracle_gravity_solve(s);
     oracle_apply_gravity(s);
     memset(s->grav_mass, 0, sizeof(int32_t) * n);
     oracle_build_gravity_mass(s);
+    /* apply_grav_mass_override_kernel (gravity_kernel.rs:379-391, launched
+     * at :991): an assigned cell replaces the built mass. */
+    for (uint32_t c = 0; c < n; ++c)
+        if (s->grav_mass_override_set[c] != 0u) s->grav_mass[c] = s->grav_mass_override[c];
     memcpy(s->grav_mass_prev, s->grav_mass, sizeof(int32_t) * n);
+}
+
+/* The modext clear at the start of the element pass
+ * (reaction_kernel/modext/mod.rs:53-99): zero the value and flag grids. */
+void oracle_clear_grav_override(oracle_state *s) {
+    const uint32_t n = (s->xres / 4) * (s->yres / 4);
+    memset(s->grav_mass_override, 0, sizeof(int32_t) * n);
+    memset(s->grav_mass_override_set, 0, sizeof(uint32_t) * n);
 }
 
 /* rebuild_gravity_mask (gravity_kernel.rs:658-710): flood fill from the
@@ -393,3 +419,13 @@ void oracle_gravity_mask(const uint32_t *wall, uint32_t xc, uint32_t yc, uint32_
 #undef SEED
     free(stack);
 }
+
+/* Thread count of the OpenMP loops (the CPU baseline times 1 and all). */
+#ifdef _OPENMP
+#include <omp.h>
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oracle_max_threads(void) { return omp_get_max_threads(); }
+#else
+void oracle_set_threads(int n) { (void)n; }
+int oracle_max_threads(void) { return 1; }
+#endif

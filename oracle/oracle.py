@@ -54,6 +54,8 @@ ARRAYS = [
     ("grav_fy", np.float32, "cell"),
     ("elem_props", np.uint32, "props"), ("move_lut", np.uint32, "lut"),
     ("photon_mask", np.uint32, "table"), ("spawn_defaults", np.uint32, "table"),
+    ("grav_tool_mass", np.float32, "cell"), ("grav_mass_override", np.int32, "cell"),
+    ("grav_mass_override_set", np.uint32, "cell"),
 ]
 _PTR = {np.uint32: _u32p, np.int32: _i32p, np.float32: _f32p}
 
@@ -114,6 +116,10 @@ def lib():
             f = getattr(_lib, name)
             f.argtypes = [ctypes.POINTER(OracleStateC)] + extra
             f.restype = None
+        _lib.oracle_set_threads.argtypes = [ctypes.c_int]
+        _lib.oracle_set_threads.restype = None
+        _lib.oracle_max_threads.argtypes = []
+        _lib.oracle_max_threads.restype = ctypes.c_int
         for name, argt in _OPTIONAL.items():
             f = getattr(_lib, name, None)
             if f is not None:
@@ -130,7 +136,7 @@ STATE_FUNCS = [
     "oracle_step_air", "oracle_air_pressure", "oracle_air_velocity", "oracle_air_smooth_advect",
     "oracle_air_heat_convect", "oracle_air_copy_back", "oracle_air_block_masks",
     "oracle_hot_air_pressure", "oracle_gravity_solve", "oracle_build_gravity_mass",
-    "oracle_apply_gravity", "oracle_step_gravity",
+    "oracle_apply_gravity", "oracle_step_gravity", "oracle_clear_grav_override",
 ]
 
 

@@ -99,19 +99,33 @@ static uint32_t neighbour_slot(const oracle_state *s, int x, int y) {
     return (packed >> 9) - 1;
 }
 
-/* thermal_conduction_kernel (thermal_kernel.rs:459-605) with rules R2/R3. */
+/* thermal_conduction_kernel (thermal_kernel.rs:459-605) with rules R2/R3.
+ * The per-slot work reads only pre-pass state (temperatures, pmap, air_hv)
+ * and writes the slot's own equilibrium, so it runs in parallel; each slot
+ * leaves its ambient-heat write and gate failure in a per-slot record, and
+ * one serial pass in ascending slot order applies them (the highest gated
+ * slot's write per cell, R2; the failure count per cell, R3). */
+enum { COND_NONE = 0, COND_HV = 1, COND_GATED = 2, COND_FAIL = 3 };
+
 void oracle_thermal_conduction(oracle_state *s, uint32_t seed) {
     const uint32_t xres = s->xres, yres = s->yres, xc = xres / 4, n_cells = xc * (yres / 4), rows = 256;
+    const uint32_t nslots = s->slot_bound;
     const float *hv0 = s->air_hv;
     float *hv_new = malloc(sizeof(float) * n_cells);
     uint32_t *hv_key = calloc(n_cells, 4), *fails = calloc(n_cells, 4);
-    for (uint32_t i = 0; i < s->slot_bound; ++i) {
+    uint8_t *kind = malloc(nslots ? nslots : 1);
+    uint32_t *kcell = malloc(sizeof(uint32_t) * (nslots ? nslots : 1));
+    float *kw = malloc(sizeof(float) * (nslots ? nslots : 1));
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (uint32_t i = 0; i < nslots; ++i) {
+        kind[i] = COND_NONE;
         uint32_t ty = s->part_ty[i];
         if (ty == 0 || ty >= rows) continue;
         uint32_t hcond = P(s, ty, COL_HEAT_CONDUCT);
         int x = ora_f2i(s->part_x[i]), y = ora_f2i(s->part_y[i]);
         if (x < 0 || x >= (int)xres || y < 0 || y >= (int)yres) continue;
         uint32_t ci = ((uint32_t)y / 4) * xc + (uint32_t)x / 4;
+        kcell[i] = ci;
         int32_t life = s->part_life[i], tmp = s->part_tmp[i];
         uint32_t ctype = s->part_ctype[i];
         float gel = 1.0f;
@@ -122,6 +136,7 @@ void oracle_thermal_conduction(oracle_state *s, uint32_t seed) {
         uint32_t roll = oracle_gpu_hash(seed ^ (i * PARTICLE_HASH_SALT) ^ ty) % 250u;
         int gate = !is_heat_insulator(ty, life, tmp, hcond) && roll < threshold;
         if (gate) {
+            kind[i] = COND_GATED;
             float self_temp = s->part_temp[i];
             if (s->aheat_enable && (P(s, ty, COL_PROPERTIES) & 0x00200000u) == 0) {
                 float dtemp = hv0[ci] - s->part_temp[i];
@@ -129,11 +144,8 @@ void oracle_thermal_conduction(oracle_state *s, uint32_t seed) {
                 float alpha = 0.4f * hc;
                 if (alpha > 0.04f) alpha = 0.04f;
                 self_temp = ora_clampf(s->part_temp[i] + alpha * dtemp / hc, 0.0f, 9999.0f);
-                float w = ora_clampf(hv0[ci] - alpha * dtemp, 0.0f, 9999.0f);
-                if (i + 1 > hv_key[ci]) {
-                    hv_key[ci] = i + 1;
-                    hv_new[ci] = w;
-                }
+                kw[i] = ora_clampf(hv0[ci] - alpha * dtemp, 0.0f, 9999.0f);
+                kind[i] = COND_HV;
             }
             float hc_self = heat_capacity(ty, ctype, rows);
             float c_heat = self_temp * hc_self, hc_total = hc_self;
@@ -141,7 +153,7 @@ void oracle_thermal_conduction(oracle_state *s, uint32_t seed) {
                 for (int nx = 0; nx < 3; ++nx) {
                     if (nx == 1 && ny == 1) continue;
                     uint32_t nid = neighbour_slot(s, x + nx - 1, y + ny - 1);
-                    if (nid == 0xFFFFFFFFu || nid >= s->slot_bound) continue;
+                    if (nid == 0xFFFFFFFFu || nid >= nslots) continue;
                     uint32_t nty = s->part_ty[nid];
                     if (nty == 0 || nty >= rows) continue;
                     uint32_t nc = P(s, nty, COL_HEAT_CONDUCT);
@@ -153,8 +165,16 @@ void oracle_thermal_conduction(oracle_state *s, uint32_t seed) {
                 }
             s->part_equil[i] = ora_clampf(c_heat / hc_total, 0.0f, 9999.0f);
         } else {
-            fails[ci] += 1;
+            kind[i] = COND_FAIL;
             s->part_equil[i] = -1.0f;
+        }
+    }
+    for (uint32_t i = 0; i < nslots; ++i) {
+        if (kind[i] == COND_HV) {
+            hv_key[kcell[i]] = i + 1;
+            hv_new[kcell[i]] = kw[i];
+        } else if (kind[i] == COND_FAIL) {
+            fails[kcell[i]] += 1;
         }
     }
     for (uint32_t c = 0; c < n_cells; ++c) {
@@ -169,11 +189,17 @@ void oracle_thermal_conduction(oracle_state *s, uint32_t seed) {
     free(hv_new);
     free(hv_key);
     free(fails);
+    free(kind);
+    free(kcell);
+    free(kw);
 }
 
 /* thermal_apply_kernel (thermal_kernel.rs:616-708). */
 void oracle_thermal_apply(oracle_state *s) {
     const uint32_t xres = s->xres, yres = s->yres, rows = 256;
+    /* Each slot reads only its own temperature and the (unchanged)
+     * equilibria, so slots are independent. */
+#pragma omp parallel for schedule(dynamic, 4096)
     for (uint32_t j = 0; j < s->slot_bound; ++j) {
         uint32_t jty = s->part_ty[j];
         if (jty == 0 || jty >= rows) continue;

@@ -59,6 +59,9 @@ typedef struct oracle_state {
     const uint32_t *move_lut;     /* 256 x 12 + 256 x 256 */
     const uint32_t *photon_mask;  /* 256 */
     const uint32_t *spawn_defaults; /* 256 x 8 (to_gpu_spawn_defaults_u32) */
+    /* gravity tool wells and element mass overrides (cell grids,
+     * buffers/mod.rs:316-347) */
+    float *grav_tool_mass; int32_t *grav_mass_override; uint32_t *grav_mass_override_set;
 } oracle_state;
 
 /* air_kernel::step_air_gpu (air_kernel/mod.rs:120-299). */
@@ -86,6 +89,10 @@ void oracle_gravity_solve_f64(const oracle_state *s, double *fx, double *fy);
 void oracle_build_gravity_mass(oracle_state *s);
 void oracle_apply_gravity(oracle_state *s);
 void oracle_step_gravity(oracle_state *s);
+/* The element pass's clear of the mass override channel
+ * (reaction_kernel/modext/mod.rs:53-99), the only part of
+ * step_reactions_gpu the tick here performs. */
+void oracle_clear_grav_override(oracle_state *s);
 void oracle_gravity_mask(const uint32_t *wall, uint32_t xc, uint32_t yc, uint32_t *mask);
 
 /* RecalcFreeParticles + CheckStacking (buffers/spatial.rs:81-472). */

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from tpt_hip import elements as E
 from tpt_hip import lib as L
 from tpt_hip import scene as S
 
@@ -88,3 +89,51 @@ def test_newton_grav_off_clears_forces(gpu):
         ctx.load_state(st)
         ctx.step_gravity(0)
         assert not ctx.download("grav_fx", np.float32).any()
+
+
+def test_tool_wells_and_overrides_against_oracle(gpu):
+    """PGRV/NGRV wells seeded before the solve, CEXP/GSNS-style assignments
+    after the deposit, over three gravity steps with fresh uploads between
+    them; then one whole tick consumes the wells and clears the channel."""
+    st = S.make_scene("612x384", scale_particles=0.1)
+    rng = np.random.default_rng(11)
+    n = st.xcells * st.ycells
+    grvt = rng.choice(st.slot_bound, 40, replace=False)
+    st.arrays["part_ty"][grvt] = E.pt("GRVT")
+    st.arrays["part_tmp"][grvt] = rng.integers(-150, 150, len(grvt))
+    ref = st.copy()
+    o = O.OracleState(ref)
+
+    def uploads(ctx, t):
+        tool = np.zeros(n, np.float32)
+        cells = rng.choice(n, 12, replace=False)
+        tool[cells] = rng.uniform(-40, 40, 12).astype(np.float32)
+        tool[cells[:3]] = np.float32([2.5, -2.5, 1.49]) / np.float32(65536)
+        ov = np.zeros(n, np.int32)
+        se = np.zeros(n, np.uint32)
+        oc = rng.choice(n, 8, replace=False)
+        ov[oc] = rng.integers(-10 ** 6, 10 ** 6, 8)
+        se[oc[: 6 - t]] = 1
+        for name, arr in (("grav_tool_mass", tool), ("grav_mass_override", ov), ("grav_mass_override_set", se)):
+            ref.arrays[name][:] = arr
+            ctx.upload(name, arr)
+
+    print("[gravity wells] 612x384, 3 gravity steps + 1 tick", flush=True)
+    with L.Context(st.cfg) as ctx:
+        ctx.load_state(st)
+        for t in range(3):
+            uploads(ctx, t)
+            ctx.step_gravity(L.frame_seed(t))
+            o.call("oracle_step_gravity")
+            got = {k: ctx.download(k, np.int32) for k in ("grav_mass", "grav_mass_prev")}
+            for k, v in got.items():
+                assert np.array_equal(v, ref.arrays[k]), (t, k)
+            assert not ctx.download("grav_tool_mass", np.float32).any()
+            assert np.array_equal(ctx.download("grav_mass_override_set", np.uint32), ref.arrays["grav_mass_override_set"])
+            fx = ctx.download("grav_fx", np.float32)
+            assert rel_err(fx, ref.arrays["grav_fx"].astype(np.float64)) < TOL
+        uploads(ctx, 0)
+        ctx.step(3)
+        assert not ctx.download("grav_tool_mass", np.float32).any()
+        assert not ctx.download("grav_mass_override", np.int32).any()
+        assert not ctx.download("grav_mass_override_set", np.uint32).any()

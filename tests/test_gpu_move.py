@@ -187,3 +187,71 @@ def test_wire_walls_whole_tick_bit_exact(gpu):
             o.call("oracle_step", frame)
         ctx.fetch(st)
     compare(ref, st, INT_FIELDS + FLOAT_FIELDS + ["wall_emap", "block_air", "block_airh"], "wire walls tick 10")
+
+
+def long_wire_scene():
+    """2048x2048 (512x512 cells): a serpentine of WALLELEC that climbs 480
+    rows twice (one row per sweep upward), so the closure reaches its
+    ycells + 2 sweep bound before it converges; ladder rungs; combs; random
+    runs.  Each structure is fed by a WL_DETECT cell with a DUST particle on
+    it: only cells charged this tick (emap 16) feed the closure, since the
+    air step has decayed older charges by then."""
+    st = S.make_scene("2048x2048", scale_particles=0.004, newton_grav=False, seed=9)
+    xc, yc = st.xcells, st.ycells
+    bmap = st.arrays["wall_bmap"].reshape(yc, xc)
+    feeds = []
+    bmap[20:500, 20] = 1          # column A, fed at its foot: climbs
+    bmap[20, 20:41] = 1           # bridge at the top
+    bmap[20:501, 40] = 1          # column B: falls in one sweep
+    bmap[500, 40:61] = 1          # bridge at the bottom
+    bmap[20:501, 60] = 1          # column C: climbs again
+    feeds.append((20, 500))
+    for y in range(100, 400, 2):  # a ladder of singleton rungs between three-wide caps
+        bmap[y, 100] = 1
+        bmap[y + 1, 99:102] = 1
+    feeds.append((100, 400))
+    bmap[300, 150:400] = 6        # a comb: spine with teeth above and below
+    for x in range(152, 400, 6):
+        bmap[260:300, x] = 6
+        bmap[301:340, x + 3] = 6
+    feeds.append((149, 300))
+    rng = np.random.default_rng(71)
+    for _ in range(300):
+        w = int(rng.choice([3, 3, 1, 2, 6, 12]))
+        x0, y0 = int(rng.integers(420, xc - 40)), int(rng.integers(2, yc - 40))
+        if rng.random() < 0.6:
+            bmap[y0, x0:x0 + int(rng.integers(2, 36))] = w
+        else:
+            bmap[y0:y0 + int(rng.integers(2, 36)), x0] = w
+    a = st.arrays
+    occupied = set(zip((a["part_x"][:st.slot_bound]).astype(int), (a["part_y"][:st.slot_bound]).astype(int)))
+    k = st.slot_bound
+    for (cx, cy) in feeds:
+        bmap[cy, cx] = 3
+        px, py = 4 * cx + 1, 4 * cy + 1
+        assert (px, py) not in occupied
+        a["part_ty"][k], a["part_x"][k], a["part_y"][k], a["part_temp"][k] = E.pt("DUST"), px, py, 300.0
+        k += 1
+    st.slot_bound = k
+    S.derive_block_masks(st)
+    S.rebuild_pmap(st)
+    return st
+
+
+def test_wire_closure_on_the_largest_grid(gpu):
+    """The row-parallel closure against the serial oracle on 512x512 cells,
+    including a wire that needs more sweeps than the bound allows."""
+    st = long_wire_scene()
+    ref = st.copy()
+    o = O.OracleState(ref)
+    print("[tick parity] 2048x2048 long wires, 3 ticks", flush=True)
+    with L.Context(st.cfg) as ctx:
+        ctx.load_state(st)
+        for frame in range(3):
+            ctx.step(frame)
+            o.call("oracle_step", frame)
+            print(f"  tick {frame} done", flush=True)
+        ctx.fetch(st)
+    em = ref.arrays["wall_emap"].reshape(512, 512)
+    assert 0 < np.count_nonzero(em[20:501, 60]) < 481      # column C is partly charged: the bound stopped it
+    compare(ref, st, INT_FIELDS + ["wall_emap", "block_air"], "long wires")

@@ -165,3 +165,92 @@ def test_apply_loss_then_force():
     assert a["part_vx"][0] == pytest.approx(0.8 + 0.5) and a["part_vy"][0] == pytest.approx(0.8 - 0.25)
     assert a["part_vx"][1] == 0.0                # METL: Loss 0, solids get no force
     assert a["part_vx"][2] == 1.0                # inside the 4-px margin: untouched
+
+
+# --- PGRV/NGRV tool wells and element mass overrides (gravity_kernel.rs:241-262, 379-391) ---
+
+def well_state(prev=None, tool=None, particles=()):
+    st = S.SimState(S.default_config(128, 128, capacity=64))
+    st.cfg["newton_grav"] = 1
+    st.arrays["grav_mask"][:] = 0xFFFFFFFF
+    for cell, v in (prev or {}).items():
+        st.arrays["grav_mass_prev"][cell] = v
+    for cell, v in (tool or {}).items():
+        st.arrays["grav_tool_mass"][cell] = v
+    for i, (name, x, y, tmp) in enumerate(particles):
+        a = st.arrays
+        a["part_ty"][i], a["part_x"][i], a["part_y"][i], a["part_tmp"][i] = E.pt(name), x, y, tmp
+        st.slot_bound = i + 1
+    return st
+
+
+def field_of(masses, n=32):
+    """FP64 field of {cell: fixed-point mass} on an all-open 32x32-cell grid."""
+    st = S.SimState(S.default_config(4 * n, 4 * n, capacity=64))
+    st.arrays["grav_mask"][:] = 0xFFFFFFFF
+    for cell, v in masses.items():
+        st.arrays["grav_mass_prev"][cell] = v
+    return O.gravity_solve_f64(st)
+
+
+def close_field(st, want, rel):
+    fx, fy = want
+    scale = max(np.abs(fx).max(), np.abs(fy).max())
+    return (np.abs(st.arrays["grav_fx"] - fx).max() / scale < rel and
+            np.abs(st.arrays["grav_fy"] - fy).max() / scale < rel)
+
+
+@pytest.mark.parametrize("units,q", [(2.5, 3), (-2.5, -3), (1.49, 1), (-1.49, -1), (0.5, 1), (7.0, 7)])
+def test_tool_well_rounds_half_away_into_the_solve(units, q):
+    """A well of `units`/65536 mass enters the solve as round-half-away(units)
+    fixed-point units (tool * 65536 +- 0.5, truncated); the field is linear in
+    that integer, so 3 vs 2 or 1 vs 2 units differ by a third or more."""
+    c = 12 * 32 + 9
+    st = well_state(tool={c: np.float32(units) / np.float32(65536.0)})
+    O.OracleState(st).call("oracle_step_gravity")
+    assert close_field(st, field_of({c: q}), 1e-3)
+    assert not close_field(st, field_of({c: q + (1 if q > 0 else -1)}), 1e-2)
+    assert not st.arrays["grav_tool_mass"].any()           # spent
+
+
+def test_tool_well_replaces_its_cell_and_keeps_the_others():
+    """The seed writes only non-zero tool cells: other snapshot cells keep the
+    previous tick's particle mass, and a well on a massive cell replaces it."""
+    a, b = 10 * 32 + 10, 20 * 32 + 21
+    st = well_state(prev={a: 40 * 65536, b: 25 * 65536}, tool={b: 3.0})
+    O.OracleState(st).call("oracle_step_gravity")
+    assert close_field(st, field_of({a: 40 * 65536, b: 3 * 65536}), 1e-4)
+    assert not close_field(st, field_of({a: 40 * 65536, b: 28 * 65536}), 1e-3)
+
+
+def test_override_replaces_the_built_mass_and_joins_the_snapshot():
+    """apply_grav_mass_override_kernel: a flagged cell takes the assigned
+    value over the particle deposit (GRVT tmp 50 -> 10 units); an unflagged
+    value is ignored; the snapshot copy carries the result."""
+    a, b, c = 2 * 32 + 2, 5 * 32 + 7, 9 * 32 + 9
+    st = well_state(particles=[("GRVT", 10, 10, 50), ("GRVT", 38, 38, 50)])
+    ov, se = st.arrays["grav_mass_override"], st.arrays["grav_mass_override_set"]
+    ov[a], se[a] = 7, 1
+    ov[b], se[b] = -123, 1
+    ov[c], se[c] = 999, 0
+    O.OracleState(st).call("oracle_step_gravity")
+    m = st.arrays["grav_mass"]
+    assert m[a] == 7 and m[b] == -123 and m[c] == 10 * 65536
+    assert np.count_nonzero(m) == 3
+    assert np.array_equal(st.arrays["grav_mass_prev"], m)
+    assert ov[a] == 7 and se[a] == 1                       # the gravity step does not clear the channel
+
+
+def test_tick_clears_the_override_channel_after_one_use():
+    """The whole tick clears the channel where the element pass would, so an
+    assignment lands in exactly one snapshot."""
+    st = S.make_scene("612x384", scale_particles=0.02)
+    cell = 30 * st.xcells + 40
+    st.arrays["grav_mass_override"][cell] = 5 * 65536
+    st.arrays["grav_mass_override_set"][cell] = 1
+    o = O.OracleState(st)
+    o.call("oracle_step", 0)
+    assert st.arrays["grav_mass_prev"][cell] == 5 * 65536
+    assert not st.arrays["grav_mass_override"].any() and not st.arrays["grav_mass_override_set"].any()
+    o.call("oracle_step", 1)
+    assert st.arrays["grav_mass_prev"][cell] == 0

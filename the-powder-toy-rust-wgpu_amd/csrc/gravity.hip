@@ -251,7 +251,7 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__res
         const float gx = on ? sm[i * 2 + 0].x : 0.0f, gy = on ? sm[i * 2 + 1].x : 0.0f;
         fx[c] = gx;
         fy[c] = gy;
-        rec[c] = make_uint4(__float_as_uint(gx), __float_as_uint(gy), wall[c], emap[c]);
+        if (rec) rec[c] = make_uint4(__float_as_uint(gx), __float_as_uint(gy), wall[c], emap[c]);
         // The mass grid this tick's deposit accumulates into (the solve
         // reads the previous tick's snapshot, consumed by now, which the
         // deposit refills alongside); null for a bare solve.
@@ -260,6 +260,14 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__res
             prev_clear[c] = 0;
         }
     }
+}
+
+// exp(-2 pi i k / n) for k < len, evaluated in double and rounded once.
+__global__ void grav_twiddles(float2 *__restrict__ tw, uint32_t n, uint32_t len) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= len) return;
+    const double a = -2.0 * 3.14159265358979323846 * (double)k / (double)n;
+    tw[k] = make_float2((float)cos(a), (float)sin(a));
 }
 
 // Kernel-spectrum construction (once per context).
@@ -422,6 +430,36 @@ __global__ __launch_bounds__(GRAV_BLOCK) void grav_apply_mass(const uint32_t *__
     mass_one(i, ty, x, y, m.life, m.tmp, m.temp, m.mask, m.props, m.mass, m.prev, xres, yres, xc, m.yc, m.mode);
 }
 
+// seed_gravity_tool_mass_kernel (gravity_kernel.rs:241-262): a painted
+// well replaces the snapshot's cell with its rounded fixed-point mass and is
+// consumed; cells without a well keep the particle mass.
+__global__ void grav_tool_seed(int32_t *__restrict__ prev, float *__restrict__ tool, uint32_t cells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const float t = tool[c];
+    if (t != 0.0f) {
+        float scaled = t * 65536.0f;
+        if (scaled >= 0.0f) scaled += 0.5f;
+        else scaled -= 0.5f;
+        prev[c] = tpt_f2i(scaled);
+        tool[c] = 0.0f;
+    }
+}
+
+// apply_grav_mass_override_kernel (gravity_kernel.rs:379-391): a flagged
+// cell's assigned mass replaces what the deposit built.  `prev` is the
+// snapshot when the deposit wrote both grids (it is then the copy the
+// reference makes next).
+__global__ void grav_override_apply(int32_t *__restrict__ mass, int32_t *__restrict__ prev,
+                                    const int32_t *__restrict__ ov, const uint32_t *__restrict__ set,
+                                    uint32_t cells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells || set[c] == 0u) return;
+    const int32_t v = ov[c];
+    mass[c] = v;
+    if (prev) prev[c] = v;
+}
+
 __global__ void fill_u32(uint32_t *__restrict__ p, uint32_t v, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -460,6 +498,44 @@ FftDims dims_of(const tpt_ctx *ctx) {
     return d;
 }
 
+// The twiddle table and both kernel spectra of the padded grid `d`: the
+// Green's function samples (gravity_gpu/mod.rs:170-200) transformed in place.
+// The inverse transform is unnormalised; the kernel carries 1/(W*H)
+// (gravity_gpu/mod.rs:176-180).
+void build_spectra(float2 *tw, float2 *kx, float2 *ky, const FftDims &d, uint32_t ntw, uint32_t tw_len,
+                   hipStream_t stream) {
+    hipLaunchKernelGGL(grav_twiddles, dim3((tw_len + 255) / 256), dim3(256), 0, stream, tw, ntw, tw_len);
+    const size_t grid = (size_t)d.W * d.H;
+    const float scale = -M_GRAV / (float)(d.W * d.H);
+    hipLaunchKernelGGL(grav_kernel_field, dim3((grid + 255) / 256), dim3(256), 0, stream, kx, ky, d, scale);
+    for (float2 *k : {kx, ky}) {
+        hipLaunchKernelGGL(grav_rows_inplace, dim3(d.H), dim3(FFT_THREADS), rows_lds(d, 1), stream, k, tw, d);
+        hipLaunchKernelGGL(grav_cols_inplace, dim3(d.W >> d.logcols), dim3(FFT_THREADS), cols_lds(d), stream, k, tw,
+                           d);
+    }
+}
+
+// The three solve launches: padded forward rows, columns with both kernel
+// products and inverse columns, inverse rows with the force extraction.
+// `prof` (nullable) receives the pass timings.
+void launch_solve(const int32_t *mass, const uint32_t *mask, float *fx, float *fy, const uint32_t *wall,
+                  const uint32_t *emap, uint4 *rec, int32_t *mass_clear, int32_t *prev_clear, float2 *work,
+                  float2 *sx, float2 *sy, const float2 *kx, const float2 *ky, const float2 *tw, const FftDims &d,
+                  hipStream_t stream, tpt_ctx *prof) {
+    if (prof) tpt_pass_begin(prof, PASS_GRAV_FFT_ROWS);
+    hipLaunchKernelGGL(grav_rows_fwd, dim3(d.yc), dim3(FFT_THREADS), rows_lds(d, 1), stream, mass, mask, work, tw,
+                       d);
+    if (prof) tpt_pass_end(prof, PASS_GRAV_FFT_ROWS);
+    if (prof) tpt_pass_begin(prof, PASS_GRAV_FFT_COLS);
+    hipLaunchKernelGGL(grav_cols, dim3(d.W >> d.logcols), dim3(FFT_THREADS), cols_lds(d), stream, work, kx, ky, sx,
+                       sy, tw, d, d.yc, d.yc);
+    if (prof) tpt_pass_end(prof, PASS_GRAV_FFT_COLS);
+    if (prof) tpt_pass_begin(prof, PASS_GRAV_FFT_INV);
+    hipLaunchKernelGGL(grav_rows_inv, dim3(d.yc), dim3(FFT_THREADS), rows_lds(d, 2), stream, sx, sy, mask, fx, fy,
+                       wall, emap, rec, mass_clear, prev_clear, tw, d);
+    if (prof) tpt_pass_end(prof, PASS_GRAV_FFT_INV);
+}
+
 int ensure_fft(tpt_ctx *ctx) {
     tpt_fft_state &f = ctx->fft;
     if (f.kernels_ready) return TPT_OK;
@@ -478,29 +554,7 @@ int ensure_fft(tpt_ctx *ctx) {
         hipMalloc(&f.kernel_y, grid * sizeof(float2)) != hipSuccess ||
         hipMalloc(&f.twiddle, tw_len * sizeof(float2)) != hipSuccess)
         return tpt_set_error(TPT_ENOMEM, "gravity FFT buffers (%ux%u) could not be allocated", f.width, f.height);
-    float2 *tw = (float2 *)malloc(tw_len * sizeof(float2));
-    if (!tw) return tpt_set_error(TPT_ENOMEM, "host twiddle table");
-    for (uint32_t k = 0; k < tw_len; ++k) {
-        const double a = -2.0 * 3.14159265358979323846 * (double)k / (double)ntw;
-        tw[k] = make_float2((float)cos(a), (float)sin(a));
-    }
-    hipError_t e = hipMemcpyAsync(f.twiddle, tw, tw_len * sizeof(float2), hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    free(tw);
-    if (e != hipSuccess) return tpt_set_error(TPT_EDEVICE, "twiddle upload: %s", hipGetErrorString(e));
-
-    const FftDims d = dims_of(ctx);
-    // The inverse transform is unnormalised; the kernel carries 1/(W*H)
-    // (gravity_gpu/mod.rs:176-180).
-    const float scale = -M_GRAV / (float)(f.width * f.height);
-    hipLaunchKernelGGL(grav_kernel_field, dim3((grid + 255) / 256), dim3(256), 0, ctx->stream, f.kernel_x,
-                       f.kernel_y, d, scale);
-    for (float2 *k : {f.kernel_x, f.kernel_y}) {
-        hipLaunchKernelGGL(grav_rows_inplace, dim3(f.height), dim3(FFT_THREADS), rows_lds(d, 1), ctx->stream, k,
-                           f.twiddle, d);
-        hipLaunchKernelGGL(grav_cols_inplace, dim3(f.width >> d.logcols), dim3(FFT_THREADS), cols_lds(d), ctx->stream,
-                           k, f.twiddle, d);
-    }
+    build_spectra(f.twiddle, f.kernel_x, f.kernel_y, dims_of(ctx), ntw, tw_len, ctx->stream);
     int rc = tpt_check_launch("gravity kernel spectra");
     if (rc) return rc;
     f.kernels_ready = true;
@@ -512,22 +566,12 @@ int solve(tpt_ctx *ctx, bool clear_mass) {
     if (rc) return rc;
     const tpt_fft_state &f = ctx->fft;
     const FftDims d = dims_of(ctx);
-    tpt_pass_begin(ctx, PASS_GRAV_FFT_ROWS);
-    hipLaunchKernelGGL(grav_rows_fwd, dim3(d.yc), dim3(FFT_THREADS), rows_lds(d, 1), ctx->stream,
-                       tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_PREV), tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK),
-                       f.work, f.twiddle, d);
-    tpt_pass_end(ctx, PASS_GRAV_FFT_ROWS);
-    tpt_pass_begin(ctx, PASS_GRAV_FFT_COLS);
-    hipLaunchKernelGGL(grav_cols, dim3(d.W >> d.logcols), dim3(FFT_THREADS), cols_lds(d), ctx->stream, f.work, f.kernel_x, f.kernel_y, f.spec_x, f.spec_y, f.twiddle, d, d.yc, d.yc);
-    tpt_pass_end(ctx, PASS_GRAV_FFT_COLS);
-    tpt_pass_begin(ctx, PASS_GRAV_FFT_INV);
-    hipLaunchKernelGGL(grav_rows_inv, dim3(d.yc), dim3(FFT_THREADS), rows_lds(d, 2), ctx->stream,
-                       f.spec_x, f.spec_y, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK),
-                       tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), tpt_buf<float>(ctx, TPT_BUF_GRAV_FY),
-                       tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP),
-                       ctx->cell_rec, clear_mass ? tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS) : nullptr,
-                       tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_PREV), f.twiddle, d);
-    tpt_pass_end(ctx, PASS_GRAV_FFT_INV);
+    launch_solve(tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_PREV), tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASK),
+                 tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), tpt_buf<float>(ctx, TPT_BUF_GRAV_FY),
+                 tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), ctx->cell_rec,
+                 clear_mass ? tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS) : nullptr,
+                 tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_PREV), f.work, f.spec_x, f.spec_y, f.kernel_x, f.kernel_y,
+                 f.twiddle, d, ctx->stream, ctx);
     return tpt_check_launch("gravity solve");
 }
 
@@ -543,6 +587,15 @@ int tpt_gravity_step(tpt_ctx *ctx, uint32_t) {
         hipLaunchKernelGGL(fill_u32, cgrid, blk, 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_FX), 0u, cells);
         hipLaunchKernelGGL(fill_u32, cgrid, blk, 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_FY), 0u, cells);
         return tpt_check_launch("gravity force clear");
+    }
+    if (ctx->tool_armed) {
+        // Step 0 of step_gravity_gpu (gravity_kernel.rs:895-913): the wells
+        // join the snapshot this solve reads.
+        tpt_pass_begin(ctx, PASS_GRAV_MASS);
+        hipLaunchKernelGGL(grav_tool_seed, cgrid, blk, 0, ctx->stream, tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_PREV),
+                           tpt_buf<float>(ctx, TPT_BUF_GRAV_TOOL_MASS), cells);
+        tpt_pass_end(ctx, PASS_GRAV_MASS);
+        ctx->tool_armed = false;
     }
     int rc = solve(ctx, true);
     if (rc) return rc;
@@ -567,6 +620,15 @@ int tpt_gravity_step(tpt_ctx *ctx, uint32_t) {
                        ctx->cell_rec, tpt_buf<uint32_t>(ctx, TPT_BUF_MOVE_LUT), m, slots, (int)ctx->cfg.xres,
                        (int)ctx->cfg.yres, (int)ctx->xcells);
     tpt_pass_end(ctx, PASS_GRAV_APPLY);
+    if (ctx->override_armed) {
+        // gravity_kernel.rs:987-999, between the build and the snapshot copy.
+        tpt_pass_begin(ctx, PASS_GRAV_BUILD);
+        hipLaunchKernelGGL(grav_override_apply, cgrid, blk, 0, ctx->stream, mass,
+                           dual ? tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_PREV) : nullptr,
+                           tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_OVERRIDE),
+                           tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASS_OVERRIDE_SET), cells);
+        tpt_pass_end(ctx, PASS_GRAV_BUILD);
+    }
     if (!dual) {
         // prev = current: the snapshot the next tick's solve reads.
         tpt_pass_begin(ctx, PASS_GRAV_BUILD);
@@ -575,6 +637,18 @@ int tpt_gravity_step(tpt_ctx *ctx, uint32_t) {
         tpt_pass_end(ctx, PASS_GRAV_BUILD);
     }
     return tpt_check_launch("gravity apply/build");
+}
+
+int tpt_gravity_override_clear(tpt_ctx *ctx) {
+    if (!ctx->override_armed) return TPT_OK;
+    const uint32_t cells = ctx->cells;
+    const dim3 cgrid((cells + 255) / 256), blk(256);
+    hipLaunchKernelGGL(fill_u32, cgrid, blk, 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASS_OVERRIDE), 0u,
+                       cells);
+    hipLaunchKernelGGL(fill_u32, cgrid, blk, 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_GRAV_MASS_OVERRIDE_SET),
+                       0u, cells);
+    ctx->override_armed = false;
+    return tpt_check_launch("gravity override clear");
 }
 
 int tpt_gravity_free(tpt_ctx *ctx) {

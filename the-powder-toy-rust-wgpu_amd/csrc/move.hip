@@ -924,61 +924,102 @@ __global__ void mv_emap_gate(const uint32_t *__restrict__ bmap, const uint32_t *
 }
 
 // emap_closure_kernel (mod.rs:594-810): the recursive half of set_emap as
-// repeated top-to-bottom sweeps, one thread, as the reference runs it; it
-// returns at once unless the gate saw both an uncharged and a charged wire.
-__global__ void mv_emap_closure(const uint32_t *__restrict__ bmap, uint32_t *__restrict__ emap,
-                                uint32_t *__restrict__ gate, int xc, int yc) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    const uint32_t g = *gate;
-    *gate = 0u;
-    if (g != 3u) return;
+// repeated top-to-bottom sweeps until one changes nothing, at most ycells + 2
+// of them; it returns at once unless the gate saw both an uncharged and a
+// charged wire.
+//
+// The reference runs the sweep on one thread.  Whether an uncharged run
+// fills depends only on the emap of the rows above and below it (and the
+// static walls), never on its own row, so the runs of one row are
+// independent: one workgroup takes a row at a time with a thread per cell,
+// and rows go in the reference's order, so every row sees the rows above it
+// as this sweep left them and the rows below as the previous sweep did.  The
+// result is the serial sweep's, also when the pass bound stops it early.
+// A cell's run start comes from a block max-scan; the run's activation is an
+// LDS OR keyed by that start (double-buffered by row parity, so the next
+// row can clear its buffer without another barrier).
+constexpr int EMAP_THREADS = 1024;
+static_assert(TPT_MAX_DIM / TPT_CELL <= (unsigned)EMAP_THREADS, "one thread per cell of a row");
+
+__device__ __forceinline__ int block_scan_max(int v, int *wave_tot) {
+    const int lane = (int)(threadIdx.x & 63u), w = (int)(threadIdx.x >> 6);
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(v, (unsigned)d, 64);
+        if (lane >= d) v = v > o ? v : o;
+    }
+    if (lane == 63) wave_tot[w] = v;
+    __syncthreads();
+    int prefix = -1;
+    for (int k = 0; k < w; ++k) prefix = prefix > wave_tot[k] ? prefix : wave_tot[k];
+    __syncthreads();
+    return v > prefix ? v : prefix;
+}
+
+// Would a charged neighbour of the uncharged cell (x, y) fill its run?  The
+// three arms of the serial kernel, for this cell's column only.
+__device__ bool emap_activates(const uint32_t *bmap, const uint32_t *emap, int x, int y, int xc, int yc) {
+    if (y + 1 < yc) {  // charged span below
+        const uint32_t se = emap[(y + 1) * xc + x];
+        const int sy = y + 1;
+        const bool single = (se & EMAP_ENDS) == EMAP_ENDS;
+        if ((se & 0x3Fu) == EMAP_CHARGED && !ladder(bmap, single, x, sy - 2, -1, xc, yc) &&
+            ((se & EMAP_ENDS) || y + 1 == yc - 1 || wire_at(bmap, x - 1, y, xc, yc) ||
+             wire_at(bmap, x + 1, y, xc, yc) || wire_at(bmap, x - 1, sy + 1, xc, yc) ||
+             !wire_at(bmap, x, sy + 1, xc, yc) || wire_at(bmap, x + 1, sy + 1, xc, yc)))
+            return true;
+    }
+    if (y > 0) {  // charged span above
+        const uint32_t se = emap[(y - 1) * xc + x];
+        const int sy = y - 1;
+        const bool single = (se & EMAP_ENDS) == EMAP_ENDS;
+        if ((se & 0x3Fu) == EMAP_CHARGED && !ladder(bmap, single, x, y + 1, 1, xc, yc) &&
+            ((se & EMAP_ENDS) || wire_at(bmap, x - 1, y, xc, yc) || wire_at(bmap, x + 1, y, xc, yc) ||
+             wire_at(bmap, x - 1, sy - 1, xc, yc) || !wire_at(bmap, x, sy - 1, xc, yc) ||
+             wire_at(bmap, x + 1, sy - 1, xc, yc)))
+            return true;
+    }
+    // ladder jumps from a singleton two rows away
+    const bool up = y + 2 < yc && (emap[(y + 2) * xc + x] & EMAP_ENDS) == EMAP_ENDS;
+    const bool down = y >= 2 && (emap[(y - 2) * xc + x] & EMAP_ENDS) == EMAP_ENDS;
+    return ladder(bmap, up, x, y, -1, xc, yc) || ladder(bmap, down, x, y, 1, xc, yc);
+}
+
+__global__ __launch_bounds__(EMAP_THREADS) void mv_emap_closure(const uint32_t *__restrict__ bmap,
+                                                               uint32_t *__restrict__ emap,
+                                                               uint32_t *__restrict__ gate, int xc, int yc) {
+    __shared__ int offs[EMAP_THREADS];
+    __shared__ int act[2][EMAP_THREADS];
+    __shared__ int wave_tot[EMAP_THREADS / 64];
+    __shared__ uint32_t g_sh;
+    const int x = (int)threadIdx.x;
+    if (x == 0) {
+        g_sh = *gate;
+        *gate = 0u;
+    }
+    __syncthreads();
+    if (g_sh != 3u) return;
+    const bool col = x < xc;
     bool changed = true;
     for (int pass = 0; changed && pass < yc + 2; ++pass) {
-        changed = false;
+        bool mine = false;
         for (int y = 0; y < yc; ++y) {
-            const uint32_t row = (uint32_t)(y * xc);
-            for (int x = 0; x < xc;) {
-                if (!wire_off(bmap, emap, row + (uint32_t)x)) {
-                    ++x;
-                    continue;
-                }
-                int x1 = x, x2 = x;
-                while (x2 + 1 < xc && wire_off(bmap, emap, row + (uint32_t)x2 + 1u)) ++x2;
-                bool act = false;
-                for (int xx = x1; xx <= x2 && !act && y + 1 < yc; ++xx) {  // charged span below
-                    const uint32_t se = emap[(y + 1) * xc + xx];
-                    const int sy = y + 1;
-                    const bool single = (se & EMAP_ENDS) == EMAP_ENDS;
-                    if ((se & 0x3Fu) == EMAP_CHARGED && !ladder(bmap, single, xx, sy - 2, -1, xc, yc) &&
-                        ((se & EMAP_ENDS) || y + 1 == yc - 1 || wire_at(bmap, xx - 1, y, xc, yc) ||
-                         wire_at(bmap, xx + 1, y, xc, yc) || wire_at(bmap, xx - 1, sy + 1, xc, yc) ||
-                         !wire_at(bmap, xx, sy + 1, xc, yc) || wire_at(bmap, xx + 1, sy + 1, xc, yc)))
-                        act = true;
-                }
-                for (int xx = x1; xx <= x2 && !act && y > 0; ++xx) {  // charged span above
-                    const uint32_t se = emap[(y - 1) * xc + xx];
-                    const int sy = y - 1;
-                    const bool single = (se & EMAP_ENDS) == EMAP_ENDS;
-                    if ((se & 0x3Fu) == EMAP_CHARGED && !ladder(bmap, single, xx, y + 1, 1, xc, yc) &&
-                        ((se & EMAP_ENDS) || wire_at(bmap, xx - 1, y, xc, yc) || wire_at(bmap, xx + 1, y, xc, yc) ||
-                         wire_at(bmap, xx - 1, sy - 1, xc, yc) || !wire_at(bmap, xx, sy - 1, xc, yc) ||
-                         wire_at(bmap, xx + 1, sy - 1, xc, yc)))
-                        act = true;
-                }
-                for (int xx = x1; xx <= x2 && !act; ++xx) {  // ladder jumps from a singleton
-                    const bool up = y + 2 < yc && (emap[(y + 2) * xc + xx] & EMAP_ENDS) == EMAP_ENDS;
-                    const bool down = y >= 2 && (emap[(y - 2) * xc + xx] & EMAP_ENDS) == EMAP_ENDS;
-                    if (ladder(bmap, up, xx, y, -1, xc, yc) || ladder(bmap, down, xx, y, 1, xc, yc)) act = true;
-                }
-                if (act) {
-                    for (int xx = x1; xx <= x2; ++xx)
-                        emap[row + (uint32_t)xx] =
-                            EMAP_CHARGED | (xx == x1 ? EMAP_LEFT : 0u) | (xx == x2 ? EMAP_RIGHT : 0u);
-                    changed = true;
-                }
-                x = x2 + 1;
+            const uint32_t c = (uint32_t)(y * xc + x);
+            const bool off = col && wire_off(bmap, emap, c);
+            int *a = act[y & 1];
+            a[x] = 0;
+            offs[x] = off ? 1 : 0;
+            if (!__syncthreads_or(off ? 1 : 0)) continue;
+            const bool first = off && (x == 0 || offs[x - 1] == 0);
+            const bool last = off && (x + 1 == xc || offs[x + 1] == 0);
+            const int rs = block_scan_max(first ? x : -1, wave_tot);
+            if (off && emap_activates(bmap, emap, x, y, xc, yc)) atomicOr(&a[rs], 1);
+            __syncthreads();
+            if (off && a[rs]) {
+                emap[c] = EMAP_CHARGED | (first ? EMAP_LEFT : 0u) | (last ? EMAP_RIGHT : 0u);
+                mine = true;
             }
         }
+        changed = __syncthreads_or(mine ? 1 : 0) != 0;
     }
 }
 
@@ -1737,7 +1778,7 @@ int tpt_movement_step(tpt_ctx *ctx) {
             hipLaunchKernelGGL(mv_detect_fill, cg, b, 0, ctx->stream, emap_w, ctx->cell_scratch_u2, xc, ctx->cells);
             hipLaunchKernelGGL(mv_emap_gate, cg, b, 0, ctx->stream, wall, emap_w, ctx->counters + TPT_CTR_EMAP_GATE,
                                ctx->cells);
-            hipLaunchKernelGGL(mv_emap_closure, dim3(1), dim3(64), 0, ctx->stream, wall, emap_w,
+            hipLaunchKernelGGL(mv_emap_closure, dim3(1), dim3(EMAP_THREADS), 0, ctx->stream, wall, emap_w,
                                ctx->counters + TPT_CTR_EMAP_GATE, xc, (int)ctx->ycells);
         }
         hipLaunchKernelGGL(mv_emap_strip, cg, b, 0, ctx->stream, emap_w, ctx->cells);

@@ -49,6 +49,10 @@ int tick(tpt_ctx *ctx, uint64_t frame, bool from_device) {
     if ((rc = tpt_gravity_step(ctx, 0u))) return rc;
     if ((rc = tpt_recalc_step(ctx, !from_device && (frame % 10u) == 0u))) return rc;
     if ((rc = tpt_thermal_step(ctx, ctx->cfg.heat_enable ? 0 : 1))) return rc;
+    // The element pass (step_reactions_gpu, lib.rs:379) runs here in the
+    // reference; of it this library keeps only the clear of the mass
+    // override channel its modext pass performs first.
+    if ((rc = tpt_gravity_override_clear(ctx))) return rc;
     if ((rc = tpt_movement_step(ctx))) return rc;
     return ctx->cfg.water_equal_test ? tpt_water_equal_step(ctx) : TPT_OK;
 }

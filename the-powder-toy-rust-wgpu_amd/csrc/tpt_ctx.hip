@@ -77,7 +77,7 @@ static int alloc_buffers(tpt_ctx *c) {
         size_t words = 0;
         if (id <= TPT_BUF_INTENT_TARGET) words = cap;
         else if (id <= TPT_BUF_CELL_WINNER) words = c->pixels;
-        else if (id <= TPT_BUF_GRAV_FY) words = c->cells;
+        else if (id <= TPT_BUF_GRAV_MASS_OVERRIDE_SET) words = c->cells;
         else if (id == TPT_BUF_ELEM_PROPS) words = (size_t)TPT_PT_NUM * TPT_ELEM_STRIDE;
         else if (id == TPT_BUF_MOVE_LUT) words = TPT_MOVE_LUT_WORDS;
         else if (id == TPT_BUF_PHOTON_MASK) words = TPT_PT_NUM;
@@ -214,7 +214,14 @@ extern "C" int tpt_upload(tpt_ctx *ctx, int buf, const void *src, size_t off, si
     if (rc) return rc;
     if (!src && bytes) return tpt_set_error(TPT_EINVAL, "null source");
     if (!bytes) return TPT_OK;
-    ctx->maps_fresh = ctx->rec_cover = ctx->rebuild_fresh = false;
+    // The gravity tool and override grids touch no particle or map state;
+    // uploading them arms the one-shot kernels that consume them.
+    if (buf == TPT_BUF_GRAV_TOOL_MASS)
+        ctx->tool_armed = true;
+    else if (buf == TPT_BUF_GRAV_MASS_OVERRIDE || buf == TPT_BUF_GRAV_MASS_OVERRIDE_SET)
+        ctx->override_armed = true;
+    else
+        ctx->maps_fresh = ctx->rec_cover = ctx->rebuild_fresh = false;
     ctx->epoch++;
     if (buf == TPT_BUF_WALL_BMAP || buf == TPT_BUF_WALL_EMAP) {
         const uint32_t *w = (const uint32_t *)src;

@@ -132,6 +132,12 @@ struct tpt_ctx {
     // span-end marker bits.
     bool has_wire = false;
     bool emap_markers = false;
+    // The gravity tool grid / override channel may hold non-zero cells:
+    // set by their uploads, cleared once the gravity step consumed the tool
+    // grid / the tick cleared the channel.  The device only ever zeroes
+    // them, so the host knows when their kernels have work.
+    bool tool_armed = false;
+    bool override_armed = false;
 
     // Graph capture of the steady-state tick (tick.hip).  `capturing` makes
     // the passes record every kernel whose launch is otherwise decided on
@@ -225,6 +231,9 @@ int tpt_air_step(tpt_ctx *ctx, const tpt_tick_begin *tb = nullptr);
 int tpt_gravity_step(tpt_ctx *ctx, uint32_t seed);
 int tpt_gravity_solve_only(tpt_ctx *ctx);
 int tpt_gravity_free(tpt_ctx *ctx);
+// Zeroes the element mass-override channel (value and flag grids) at the
+// point of the tick where the reference's element pass clears it.
+int tpt_gravity_override_clear(tpt_ctx *ctx);
 int tpt_recalc_step(tpt_ctx *ctx, int stacking);
 int tpt_thermal_step(tpt_ctx *ctx, int legacy);
 int tpt_movement_step(tpt_ctx *ctx);

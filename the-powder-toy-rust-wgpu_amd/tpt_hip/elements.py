@@ -198,3 +198,28 @@ def spawn_defaults(elements=None):
             _i32_bits(e["default_tmp"]), _i32_bits(e["default_tmp2"]), _i32_bits(e["default_tmp3"]),
             _i32_bits(e["default_tmp4"]), 0]
     return out
+
+
+def _cached_default(fn):
+    """The tables of the built-in registry are constant: build them once and
+    hand out copies (callers may edit theirs, e.g. a LUT with zeroed ranks)."""
+    cache = {}
+
+    def wrapper(elements=None):
+        if elements is not None:
+            return fn(elements)
+        if "v" not in cache:
+            cache["v"] = fn(None)
+        return cache["v"].copy()
+
+    wrapper.__doc__ = fn.__doc__
+    wrapper.__name__ = fn.__name__
+    return wrapper
+
+
+element_props_table = _cached_default(element_props_table)
+build_can_move = _cached_default(build_can_move)
+weight_ranks = _cached_default(weight_ranks)
+move_lut = _cached_default(move_lut)
+photon_mask = _cached_default(photon_mask)
+spawn_defaults = _cached_default(spawn_defaults)

@@ -44,6 +44,7 @@ CELL_ARRAYS = [
     ("block_air", np.uint32), ("block_airh", np.uint32), ("elem_block_air", np.uint32),
     ("elem_block_airh", np.uint32), ("grav_mask", np.uint32), ("grav_mass", np.int32),
     ("grav_mass_prev", np.int32), ("grav_fx", np.float32), ("grav_fy", np.float32),
+    ("grav_tool_mass", np.float32), ("grav_mass_override", np.int32), ("grav_mass_override_set", np.uint32),
 ]
 # Buffer ids of include/tpt_hip.h, in enum order.
 BUFFER_ORDER = [n for n, _ in SLOT_ARRAYS] + [n for n, _ in PIXEL_ARRAYS] + \
@@ -252,7 +253,8 @@ WORKLOADS = {
     # name: (xres, yres, seed, particles, wall rectangles, patches)
     "612x384": (612, 384, 0, 20000, 0, ()),
     "1600x900": (1600, 900, 1, 403200, 0, (("FIRE", 28800), ("OIL", 43200))),
-    "2048x2048": (2048, 2048, 1, 2097152, 64, ()),
+    # the 1600x900 mix: FIRE and OIL patches of 2% and 3% of the pixels
+    "2048x2048": (2048, 2048, 1, 2097152, 64, (("FIRE", 83886), ("OIL", 125829))),
 }
 
 
