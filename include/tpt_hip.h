@@ -180,6 +180,20 @@ TPT_API int tpt_step_gravity(tpt_ctx *ctx, uint32_t seed);
  * grids from TPT_BUF_GRAV_MASS_PREV (fixed point, 1/65536 units). */
 TPT_API int tpt_gravity_solve(tpt_ctx *ctx);
 
+/* No counterpart in the reference.  tpt_gravity_solve for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): it allocates nothing and
+ * takes no context.  Solves fx, fy (xcells*ycells floats each) from the
+ * fixed-point mass grid (i32, 1/65536 units) and the gravity mask (u32), all
+ * device pointers, on `stream` (a hipStream_t; NULL is the default stream).
+ * The FFT's buffers live in a 29.4 MB static device array that each GPU the
+ * library runs on holds until the process ends; padded grids beyond
+ * 1024x1024 (xcells or ycells above 512, canvases beyond 2048x2048 px) are
+ * rejected with TPT_EINVAL.  Calls that overlap on one GPU race on that
+ * array. */
+TPT_API int tpt_gravity_solve_noalloc(const int32_t *mass, const uint32_t *mask, float *fx, float *fy,
+                                      uint32_t xcells, uint32_t ycells, void *stream);
+
 /* SimBuffers::recalc_free_particles_device
  * (crates/tpt-sim/src/buffers/spatial.rs:371-472). */
 TPT_API int tpt_recalc_free_particles(tpt_ctx *ctx, uint32_t seed, int stacking);

@@ -25,6 +25,7 @@
 #include "tpt_internal.h"
 
 #include <math.h>
+#include <mutex>
 #include <stdlib.h>
 
 namespace {
@@ -657,4 +658,88 @@ int tpt_gravity_free(tpt_ctx *ctx) {
         if (p) (void)hipFree(p);
     f = tpt_fft_state();
     return TPT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tpt_gravity_solve_noalloc: the gravity solve for callers that cannot
+// allocate.  Its FFT buffers live in `scratch`, a static device array each
+// GPU holds for the life of the process; the padded grid may be at most
+// NOALLOC_MAX_SIDE on each side (canvases up to 2048x2048 px).  Layout:
+// twiddles | kernel spectrum x | kernel spectrum y | work | spectrum x |
+// spectrum y.  Every launch writes each scratch element it later reads, so
+// nothing has to start at zero; calls that overlap on one GPU race.
+
+namespace {
+
+constexpr uint32_t NOALLOC_MAX_SIDE = 1024;
+constexpr size_t NOALLOC_TW = NOALLOC_MAX_SIDE / 2;
+constexpr size_t NOALLOC_GRID = (size_t)NOALLOC_MAX_SIDE * NOALLOC_MAX_SIDE;
+constexpr size_t NOALLOC_ROWS = (size_t)NOALLOC_MAX_SIDE * (NOALLOC_MAX_SIDE / 2);  // W x ycells
+constexpr size_t NOALLOC_ELEMS = NOALLOC_TW + 2 * NOALLOC_GRID + 3 * NOALLOC_ROWS;  // 29.4 MB of float2
+
+}  // namespace
+
+static __device__ __align__(256) float2 scratch[NOALLOC_ELEMS];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float2 *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of the current GPU's copy of `scratch`, looked up once
+// per device (host code must never pass the symbol itself to a kernel).
+int scratch_for_current_device(float2 **out) {
+    int dev = 0;
+    TPT_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return tpt_set_error(TPT_EDEVICE, "device %d beyond the %d the scratch table covers", dev,
+                             NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return tpt_set_error(TPT_EDEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                                 hipGetErrorString(e));
+        g_scratch_ptr[dev] = (float2 *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TPT_OK;
+}
+
+}  // namespace
+
+extern "C" int tpt_gravity_solve_noalloc(const int32_t *mass, const uint32_t *mask, float *fx, float *fy,
+                                         uint32_t xcells, uint32_t ycells, void *stream) {
+    if (!mass || !mask || !fx || !fy) return tpt_set_error(TPT_EINVAL, "null grid pointer");
+    if (xcells == 0 || ycells == 0) return tpt_set_error(TPT_EINVAL, "empty cell grid");
+    const uint32_t W = npow2(xcells * 2), H = npow2(ycells * 2);
+    if (W > NOALLOC_MAX_SIDE || H > NOALLOC_MAX_SIDE)
+        return tpt_set_error(TPT_EINVAL,
+                             "cell grid %ux%u needs a %ux%u FFT grid; the allocation-free solve holds at most "
+                             "%ux%u (%zu bytes of scratch)",
+                             xcells, ycells, W, H, NOALLOC_MAX_SIDE, NOALLOC_MAX_SIDE, NOALLOC_ELEMS * sizeof(float2));
+    float2 *base = nullptr;
+    int rc = scratch_for_current_device(&base);
+    if (rc) return rc;
+    FftDims d;
+    d.xc = (int)xcells;
+    d.yc = (int)ycells;
+    d.W = (int)W;
+    d.H = (int)H;
+    d.logW = ilog2(W);
+    d.logH = ilog2(H);
+    const uint32_t ntw = W > H ? W : H;
+    d.tw_bits = ilog2(ntw);
+    d.logcols = 0;
+    const uint32_t tw_len = ntw / 2 ? ntw / 2 : 1;
+    const size_t grid = (size_t)W * H, rows = (size_t)W * ycells;
+    float2 *tw = base, *kx = tw + NOALLOC_TW, *ky = kx + grid, *work = ky + grid, *sx = work + rows,
+           *sy = sx + rows;
+    hipStream_t st = (hipStream_t)stream;
+    build_spectra(tw, kx, ky, d, ntw, tw_len, st);
+    launch_solve(mass, mask, fx, fy, nullptr, nullptr, nullptr, nullptr, nullptr, work, sx, sy, kx, ky, tw, d, st,
+                 nullptr);
+    return tpt_check_launch("allocation-free gravity solve");
 }

@@ -52,6 +52,7 @@ SIGNATURES = {
     "tpt_step_air": ([_vp], ctypes.c_int),
     "tpt_step_gravity": ([_vp, ctypes.c_uint32], ctypes.c_int),
     "tpt_gravity_solve": ([_vp], ctypes.c_int),
+    "tpt_gravity_solve_noalloc": ([_vp, _vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp], ctypes.c_int),
     "tpt_recalc_free_particles": ([_vp, ctypes.c_uint32, ctypes.c_int], ctypes.c_int),
     "tpt_step_thermal": ([_vp, ctypes.c_uint32, ctypes.c_int], ctypes.c_int),
     "tpt_step_water_equal": ([_vp, ctypes.c_uint32], ctypes.c_int),
