@@ -245,6 +245,68 @@ TPT_API int tpt_batch_step(tpt_batch *batch, uint64_t first_frame, uint32_t tick
 TPT_API int tpt_batch_step_launches(tpt_batch *batch, uint64_t first_frame, uint32_t ticks);
 TPT_API int tpt_batch_synchronize(tpt_batch *batch);
 
+/* ---------------------------------------------------------------------------
+ * The learned air surrogate (no counterpart in the reference): a small
+ * convolutional U-Net (32/64/128 channels, 3x3 convolutions, GELU; bf16
+ * weights on MFMA, fp32 accumulation) that advances a context's air grids
+ * (pressure, velocity, ambient heat) by 8 ticks per call in place of 8 calls
+ * of air_kernel::step_air_gpu (air_kernel/mod.rs:120-299).  It is trained on
+ * trajectories of that plain air step by its own forward and backward
+ * kernels.  Scenes for training and evaluation carry heat sources: cells
+ * whose ambient heat the plain ticks (tpt_sur_plain_ticks) and the
+ * surrogate both hold at a fixed temperature. */
+#define TPT_SUR_MAX_BLOBS 16
+#define TPT_SUR_MAX_RECTS 32
+#define TPT_SUR_MAX_SOURCES 8
+typedef struct tpt_sur_scene_desc {
+    uint32_t n_blobs, n_rects, n_sources, reserved;
+    float blob[TPT_SUR_MAX_BLOBS][4];        /* centre x, y (cells), amplitude, sigma (cells) */
+    uint32_t rect[TPT_SUR_MAX_RECTS][4];     /* WL_WALL cells [x0, x1) x [y0, y1): x0, y0, x1, y1 */
+    uint32_t source[TPT_SUR_MAX_SOURCES][4]; /* heat-source cells, same form */
+    float source_temp[TPT_SUR_MAX_SOURCES];  /* K */
+} tpt_sur_scene_desc;
+
+typedef struct tpt_sur tpt_sur;
+/* A network for an xcells x ycells air grid (multiples of 4), He-initialised. */
+TPT_API int tpt_sur_create(uint32_t xcells, uint32_t ycells, tpt_sur **out);
+TPT_API int tpt_sur_destroy(tpt_sur *sur);
+/* Parameters: one fp32 vector (every layer's [cout][k] matrix, bias in
+ * column 9*cin); set resets the optimiser state. */
+TPT_API int tpt_sur_param_count(tpt_sur *sur, size_t *n);
+TPT_API int tpt_sur_init_params(tpt_sur *sur, uint64_t seed);
+TPT_API int tpt_sur_set_params(tpt_sur *sur, const float *params, size_t n);
+TPT_API int tpt_sur_get_params(tpt_sur *sur, float *params, size_t n);
+TPT_API int tpt_sur_get_grads(tpt_sur *sur, float *grads, size_t n);
+/* Normalisation: channel c of the input and of the predicted change is
+ * field / scale[c] (heat as (hv - ambient) / scale[3]; vx and vy use
+ * scale[1]). */
+TPT_API int tpt_sur_set_scales(tpt_sur *sur, const float scales[4], float ambient);
+/* Host-buffer forward (x0: H x W x 8 bf16 bits in, out: H x W x 8 fp32) and
+ * backward (g_out: H x W x 8 bf16 bits; the weight gradients replace the
+ * gradient vector) of the network alone. */
+TPT_API int tpt_sur_forward_raw(tpt_sur *sur, const uint16_t *x0, float *out);
+TPT_API int tpt_sur_backward_raw(tpt_sur *sur, const uint16_t *g_out);
+/* A scene on a context with the surrogate's grid: blobs of pressure, zero
+ * velocity, ambient heat, WL_WALL rectangles and border ring, heat sources. */
+TPT_API int tpt_sur_scene(tpt_sur *sur, tpt_ctx *ctx, const tpt_sur_scene_desc *scene);
+/* `ticks` plain air steps (tpt_step_air) with the heat sources held. */
+TPT_API int tpt_sur_plain_ticks(tpt_sur *sur, tpt_ctx *ctx, uint32_t ticks);
+/* Training set: n_traj trajectories of n_snap snapshots 8 plain ticks apart. */
+TPT_API int tpt_sur_dataset_alloc(tpt_sur *sur, uint32_t n_traj, uint32_t n_snap);
+TPT_API int tpt_sur_dataset_capture(tpt_sur *sur, tpt_ctx *ctx, uint32_t traj, uint32_t snap);
+/* Adam training on (trajectory, snapshot) pairs, predicting snapshot + 1;
+ * gradients summed over `accum` pairs per step. */
+TPT_API int tpt_sur_train(tpt_sur *sur, const uint32_t *pairs, uint32_t n_pairs, uint32_t accum, float lr,
+                          float *mean_loss);
+TPT_API int tpt_sur_eval_loss(tpt_sur *sur, const uint32_t *pairs, uint32_t n_pairs, float *mean_loss);
+/* 8 air ticks of the context by the network.  With `check`, a prediction
+ * with a non-finite value or whose energy sum(p^2 + vx^2 + vy^2) exceeds
+ * energy_ratio times the input's is replaced by 8 plain ticks from the
+ * input; *fell_back (nullable) says which ran. */
+TPT_API int tpt_sur_step_air(tpt_sur *sur, tpt_ctx *ctx, int check, float energy_ratio, int *fell_back);
+/* The last checked call's energy before and after and its non-finite count. */
+TPT_API int tpt_sur_last_check(tpt_sur *sur, float out[3]);
+
 /* The tick seed of lib.rs:315-318. */
 TPT_API uint32_t tpt_frame_seed(uint64_t frame_count);
 

@@ -65,6 +65,25 @@ SIGNATURES = {
     "tpt_batch_step": ([_vp, ctypes.c_uint64, ctypes.c_uint32], ctypes.c_int),
     "tpt_batch_step_launches": ([_vp, ctypes.c_uint64, ctypes.c_uint32], ctypes.c_int),
     "tpt_batch_synchronize": ([_vp], ctypes.c_int),
+    "tpt_sur_create": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)], ctypes.c_int),
+    "tpt_sur_destroy": ([_vp], ctypes.c_int),
+    "tpt_sur_param_count": ([_vp, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "tpt_sur_init_params": ([_vp, ctypes.c_uint64], ctypes.c_int),
+    "tpt_sur_set_params": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
+    "tpt_sur_get_params": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
+    "tpt_sur_get_grads": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
+    "tpt_sur_set_scales": ([_vp, _vp, ctypes.c_float], ctypes.c_int),
+    "tpt_sur_forward_raw": ([_vp, _vp, _vp], ctypes.c_int),
+    "tpt_sur_backward_raw": ([_vp, _vp], ctypes.c_int),
+    "tpt_sur_scene": ([_vp, _vp, _vp], ctypes.c_int),
+    "tpt_sur_plain_ticks": ([_vp, _vp, ctypes.c_uint32], ctypes.c_int),
+    "tpt_sur_dataset_alloc": ([_vp, ctypes.c_uint32, ctypes.c_uint32], ctypes.c_int),
+    "tpt_sur_dataset_capture": ([_vp, _vp, ctypes.c_uint32, ctypes.c_uint32], ctypes.c_int),
+    "tpt_sur_train": ([_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.POINTER(ctypes.c_float)],
+                      ctypes.c_int),
+    "tpt_sur_eval_loss": ([_vp, _vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "tpt_sur_step_air": ([_vp, _vp, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "tpt_sur_last_check": ([_vp, _vp], ctypes.c_int),
     "tpt_frame_seed": ([ctypes.c_uint64], ctypes.c_uint32),
     "tpt_synchronize": ([_vp], ctypes.c_int),
     "tpt_set_profiling": ([_vp, ctypes.c_int], ctypes.c_int),
