@@ -8,7 +8,11 @@ synthetic 2048x2048 workload (512x512 air cells, ~2.1M particles, 64 wall
 rectangles), with every buffer resident in HBM before timing starts.
 `roofline` reports the dominant kernel's achieved bandwidth from HIP events
 recorded on the library's own stream; `cpu_baseline` times the CPU oracle
-(single thread) on a bounded number of ticks of the same workload.
+on a bounded number of ticks of the same workload (all threads, and per
+workload on all threads and one).  `batched` is the 256-canvas form (graph
+replay at the autotuned work split against one canvas at a time and against
+per-canvas stream launches); `surrogate` is the learned air step against 8
+plain air ticks on the 512x512 air grid.
 """
 import argparse
 import json
@@ -113,69 +117,114 @@ def oracle_pass(o, name, frame):
         o.call("oracle_step", frame)
 
 
-def batched(args):
-    """The batched form: N independent canvases with sides drawn from
+def batched_leg(n, steps, warmup, log=None):
+    """The batched form: n independent canvases with sides drawn from
     128..2048 in steps of 4 (seed 2), each with its own seed, a fill drawn
     from 10-50%, the 612x384 workload's element mix and init rules.  Times
-    the same ticks as per-canvas tpt_step launches and as tpt_batch_step
-    (one eager tick, then replays of one graph holding every canvas)."""
+    the same ticks three ways: one canvas at a time (the reference steps one
+    engine after another), per-canvas launches on their own streams, and
+    tpt_batch_step (graph replay) at the work split the autotuner picks."""
     import numpy as np
     from tpt_hip import lib as L
     from tpt_hip import scene as S
 
     rng = np.random.default_rng(2)
-    n = args.batched
     shapes = [(4 * int(rng.integers(32, 513)), 4 * int(rng.integers(32, 513))) for _ in range(n)]
     fills = rng.uniform(0.10, 0.50, n)
-    cfgs, particles = [], 0
     t0 = time.perf_counter()
-    for k, (w, h) in enumerate(shapes):
-        cfgs.append(S.default_config(w, h))
-    batch = L.Batch(cfgs)
+    batch = L.Batch([S.default_config(w, h) for w, h in shapes])
+    particles = 0
     for k, (w, h) in enumerate(shapes):
         st = S.make_scene(xres=w, yres=h, seed=1000 + k, particles=int(fills[k] * w * h))
         batch.contexts[k].load_state(st)
         particles += int((st.arrays["part_ty"] != 0).sum())
         del st
-        if k % 32 == 31:
-            print(f"[batched] {k + 1}/{n} canvases loaded ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+        if log and k % 64 == 63:
+            log(f"[batched] {k + 1}/{n} canvases loaded ({time.perf_counter() - t0:.0f} s)")
     frame = 0
-    batch.step_launches(frame, args.warmup)
-    frame += args.warmup
+
+    def timed(fn, ticks):
+        nonlocal frame
+        batch.synchronize()
+        t = time.perf_counter()
+        fn(frame, ticks)
+        batch.synchronize()
+        frame += ticks
+        return time.perf_counter() - t
+
+    timed(batch.step_launches, warmup)
+    t_serial = timed(batch.step_serial, steps)
+    t_launch = timed(batch.step_launches, steps)
+    lanes, table = batch.autotune(frame, 3)
+    frame += 1 + 4 * len(table)
     batch.synchronize()
-    t0 = time.perf_counter()
-    batch.step_launches(frame, args.steps)
-    batch.synchronize()
-    t_launch = time.perf_counter() - t0
-    frame += args.steps
-    batch.step(frame, 2)  # capture
-    batch.synchronize()
-    frame += 2
-    t0 = time.perf_counter()
-    batch.step(frame, args.steps)
-    batch.synchronize()
-    t_graph = time.perf_counter() - t0
+    timed(batch.step, 2)  # eager tick + one replay after the tuning
+    t_graph = timed(batch.step, steps + 1) * steps / (steps + 1)  # its eager first tick included
     batch.close()
     px = sum(w * h for w, h in shapes)
-    out = {
-        "metric": "aggregate canvas-ticks/s, batched form: graph replay vs per-canvas launches",
-        "value": round(n * args.steps / t_graph, 2),
-        "unit": "canvas-ticks/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(t_graph / args.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "saved scene" if args.scene else "synthetic",
-        "config": {"workload": f"{n} canvases, sides 128..2048 step 4 (seed 2), fill 10-50%",
-                   "pixels": px, "particles": particles, "parallelism": "single-gpu"},
-        "per_canvas_launches": {"value": round(n * args.steps / t_launch, 2), "ms_per_step": round(t_launch / args.steps * 1e3, 3)},
-        "speedup_vs_launches": round(t_launch / t_graph, 3),
+    rate = lambda t: round(n * steps / t, 2)  # noqa: E731
+    return {
+        "value": rate(t_graph), "unit": "canvas-ticks/s", "ms_per_batch_tick": round(t_graph / steps * 1e3, 3),
+        "workload": f"{n} canvases, sides 128..2048 step 4 (seed 2), fill 10-50%",
+        "pixels": px, "particles": particles,
+        "serial_one_canvas_at_a_time": {"value": rate(t_serial), "ms_per_batch_tick": round(t_serial / steps * 1e3, 3)},
+        "per_canvas_stream_launches": {"value": rate(t_launch), "ms_per_batch_tick": round(t_launch / steps * 1e3, 3)},
+        "speedup_vs_serial": round(t_serial / t_graph, 3),
+        "speedup_vs_stream_launches": round(t_launch / t_graph, 3),
+        "autotune": {"lanes": lanes, "candidates_ms_per_batch_tick": {str(k): round(v, 3) for k, v in table}},
     }
+
+
+def batched(args):
+    out = batched_leg(args.batched, args.steps, args.warmup,
+                      log=lambda m: print(m, file=sys.stderr, flush=True))
+    out.update({"metric": "aggregate canvas-ticks/s, batched form", "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f32", "data": "synthetic"})
     print(json.dumps(out))
+
+
+# Algorithmic FLOPs of one surrogate call: 2 * pixels * 9 * cin * cout per
+# 3x3 convolution of the U-Net (real channels, 512x512 grid).
+SURROGATE_LAYERS = [(0, 6, 32), (1, 32, 64), (1, 64, 64), (2, 64, 128), (2, 128, 128), (2, 128, 64),
+                    (1, 64, 64), (1, 64, 32), (0, 32, 32), (0, 32, 4)]
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_leg(seeds=16):
+    """The learned air step on the 512x512 air grid: call time against 8
+    plain air ticks, relative L2 against the plain solver after 64 ticks on
+    held-out scenes, fallback rate, MFMA utilisation."""
+    from tpt_hip import lib as L
+    from tpt_hip import scene as S
+    from tpt_hip import surrogate as SU
+
+    if not os.path.exists(SU.DEFAULT_WEIGHTS):
+        return None
+    xc = 512
+    cfg = SU.air_config(xc, xc)
+    ctx = L.Context(cfg)
+    ctx.load_state(S.SimState(cfg))
+    sur = SU.Surrogate(xc, xc)
+    meta = sur.load(SU.DEFAULT_WEIGHTS)
+    t = SU.timing(sur, ctx, reps=50)
+    ev = SU.evaluate(sur, ctx, range(10 ** 7, 10 ** 7 + seeds))
+    ctx.close()
+    sur.close()
+    flops = sum(2 * (xc >> lv) ** 2 * 9 * ci * co for lv, ci, co in SURROGATE_LAYERS)
+    tf = flops / (t["surrogate_call"] * 1e-6) / 1e12
+    return {
+        "workload": "512x512 air grid, 8 ticks per call, held-out scenes (4-16 blobs, 0-32 walls, 0-8 heat sources)",
+        "call_us": round(t["surrogate_call"], 1), "call_checked_us": round(t["surrogate_call_checked"], 1),
+        "plain_8_ticks_us": round(t["plain_8_ticks"], 1),
+        "speedup_vs_plain": round(t["plain_8_ticks"] / t["surrogate_call"], 3),
+        "speedup_vs_plain_checked": round(t["plain_8_ticks"] / t["surrogate_call_checked"], 3),
+        "rel_l2_64_ticks": ev["rel_l2"], "rel_l2_8_ticks": ev["rel_l2_8_ticks"], "eval_seeds": seeds,
+        "fallback_rate": ev["fallback_rate"],
+        "roofline": {"bound": "mfma", "achieved": round(tf, 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4), "flops_per_call": flops},
+        "trained_epochs": meta.get("epochs"), "train_trajectories": meta.get("traj"),
+    }
 
 
 def main():
@@ -189,7 +238,10 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     ap.add_argument("--batched", type=int, default=0, metavar="N",
-                    help="the batched form instead: N mixed-size canvases, graph replay vs per-canvas launches")
+                    help="the batched form alone: N mixed-size canvases")
+    ap.add_argument("--batch-canvases", type=int, default=256,
+                    help="canvases of the batched leg of the default run (0 skips it)")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
     ap.add_argument("--water-equal", action="store_true",
                     help="run the workload with water_equal_test set (the optional pass after movement)")
     ap.add_argument("--scene", default=None, metavar="PATH",
@@ -346,6 +398,12 @@ def main():
                 cpu_configs[wl] = row
         olib.oracle_set_threads(all_threads)
 
+    batched_out = None
+    if args.batch_canvases and not args.scene:
+        batched_out = batched_leg(args.batch_canvases, min(args.steps, 10), 2,
+                                  log=lambda m: print(m, file=sys.stderr, flush=True))
+    surrogate_out = None if args.no_surrogate else surrogate_leg()
+
     out = {
         "metric": "ticks/s @2048x2048 plain vs batched vs air-surrogate (relL2<=1e-2); HBM GB/s",
         "value": round(ticks_per_s, 2),
@@ -368,6 +426,8 @@ def main():
         "graph_replay": {"value": round(args.steps / dt_graph, 2), "ms_per_step": round(dt_graph / args.steps * 1e3, 4)},
         "tick_roofline": tick_roofline,
         "per_kernel": per_kernel,
+        "batched": batched_out,
+        "surrogate": surrogate_out,
     }
     print(json.dumps(out))
 

@@ -241,8 +241,24 @@ TPT_API int tpt_batch_create(uint32_t n, const tpt_config *cfgs, tpt_batch **out
 TPT_API int tpt_batch_destroy(tpt_batch *batch);
 TPT_API int tpt_batch_context(tpt_batch *batch, uint32_t k, tpt_ctx **ctx);
 TPT_API int tpt_batch_step(tpt_batch *batch, uint64_t first_frame, uint32_t ticks);
-/* The same ticks as per-canvas tpt_step launches (the comparison point). */
+/* The same ticks as per-canvas tpt_step launches, each canvas on its own
+ * stream (canvases overlap on the GPU). */
 TPT_API int tpt_batch_step_launches(tpt_batch *batch, uint64_t first_frame, uint32_t ticks);
+/* The same ticks one canvas at a time, as the reference steps one
+ * SimulationEngine after another (tpt-app main.rs:656-672): each canvas's
+ * tick waits on the GPU for the previous canvas's. */
+TPT_API int tpt_batch_step_serial(tpt_batch *batch, uint64_t first_frame, uint32_t ticks);
+/* Work split of the captured tick: the canvases go, largest first, to
+ * `lanes` serial chains that run side by side (0: one chain per canvas). */
+TPT_API int tpt_batch_set_lanes(tpt_batch *batch, uint32_t lanes);
+/* Autotuner: one eager tick, then trial_ticks graph replays at each
+ * candidate lane count (1, 2, 4, ... and one per canvas), timed with HIP
+ * events; keeps the fastest (*lanes_out) and its graph.  The canvases
+ * advance through all of these ticks. */
+TPT_API int tpt_batch_autotune(tpt_batch *batch, uint64_t first_frame, uint32_t trial_ticks, uint32_t *lanes_out);
+/* The autotuner's record: lane counts and milliseconds per batch tick. */
+TPT_API int tpt_batch_tuning(tpt_batch *batch, uint32_t *lanes, float *ms_per_tick, uint32_t max_entries,
+                             uint32_t *n_entries);
 TPT_API int tpt_batch_synchronize(tpt_batch *batch);
 
 /* ---------------------------------------------------------------------------
@@ -277,10 +293,10 @@ TPT_API int tpt_sur_init_params(tpt_sur *sur, uint64_t seed);
 TPT_API int tpt_sur_set_params(tpt_sur *sur, const float *params, size_t n);
 TPT_API int tpt_sur_get_params(tpt_sur *sur, float *params, size_t n);
 TPT_API int tpt_sur_get_grads(tpt_sur *sur, float *grads, size_t n);
-/* Normalisation: channel c of the input and of the predicted change is
- * field / scale[c] (heat as (hv - ambient) / scale[3]; vx and vy use
- * scale[1]). */
-TPT_API int tpt_sur_set_scales(tpt_sur *sur, const float scales[4], float ambient);
+/* Normalisation of pressure, velocity and ambient heat: input channel c is
+ * field / in_scales[c] (heat as (hv - ambient) / in_scales[3]); output
+ * channel c is the 8-tick change / out_scales[c].  vx and vy use index 1. */
+TPT_API int tpt_sur_set_scales(tpt_sur *sur, const float in_scales[4], const float out_scales[4], float ambient);
 /* Host-buffer forward (x0: H x W x 8 bf16 bits in, out: H x W x 8 fp32) and
  * backward (g_out: H x W x 8 bf16 bits; the weight gradients replace the
  * gradient vector) of the network alone. */
@@ -299,6 +315,9 @@ TPT_API int tpt_sur_dataset_capture(tpt_sur *sur, tpt_ctx *ctx, uint32_t traj, u
 TPT_API int tpt_sur_train(tpt_sur *sur, const uint32_t *pairs, uint32_t n_pairs, uint32_t accum, float lr,
                           float *mean_loss);
 TPT_API int tpt_sur_eval_loss(tpt_sur *sur, const uint32_t *pairs, uint32_t n_pairs, float *mean_loss);
+/* Bound on the global L2 norm of each step's mean gradient (default 1; 0
+ * turns clipping off). */
+TPT_API int tpt_sur_set_clip(tpt_sur *sur, float clip);
 /* 8 air ticks of the context by the network.  With `check`, a prediction
  * with a non-finite value or whose energy sum(p^2 + vx^2 + vy^2) exceeds
  * energy_ratio times the input's is replaced by 8 plain ticks from the

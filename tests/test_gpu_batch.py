@@ -78,3 +78,47 @@ def test_batch_launches_equal_graph(gpu):
             c.fetch(s)
     for k in range(len(scenes)):
         same(a[k], b[k], f"canvas {k}")
+
+
+def run_batch(scenes, drive):
+    """Load `scenes` into a fresh batch, apply drive(batch), return the states."""
+    out = [s.copy() for s in scenes]
+    with L.Batch([s.cfg for s in scenes]) as b:
+        for c, s in zip(b.contexts, out):
+            c.load_state(s)
+        drive(b)
+        b.synchronize()
+        for c, s in zip(b.contexts, out):
+            c.fetch(s)
+    return out
+
+
+def test_serial_lanes_and_autotune_equal_launches(gpu):
+    """One canvas at a time, the graph split into 2 and 3 lanes, and the
+    autotuned graph all give the per-canvas launches' results."""
+    shapes = [(400, 300, 1, 8000), (800, 200, 2, 12000), (256, 256, 5, 5000), (128, 512, 6, 4000), (640, 480, 8, 30000)]
+    scenes = [S.make_scene(xres=x, yres=y, seed=sd, particles=n) for x, y, sd, n in shapes]
+    print("[batch] serial / lanes / autotune vs launches", flush=True)
+    ref = run_batch(scenes, lambda b: b.step_launches(0, 12))
+    got = run_batch(scenes, lambda b: b.step_serial(0, 12))
+    for k in range(len(scenes)):
+        same(ref[k], got[k], f"serial canvas {k}")
+    for lanes in (2, 3):
+        def drive(b, lanes=lanes):
+            b.set_lanes(lanes)
+            b.step(0, 12)
+        got = run_batch(scenes, drive)
+        for k in range(len(scenes)):
+            same(ref[k], got[k], f"{lanes} lanes canvas {k}")
+    seen = {}
+
+    def tuned(b):
+        best, table = b.autotune(0, 1)            # 1 eager tick + 2 replays per candidate
+        seen["best"], seen["table"] = best, table
+        done = 1 + 2 * len(table)
+        b.step(done, 12 - done)
+    got = run_batch(scenes, tuned)
+    assert [t[0] for t in seen["table"]] == [1, 2, 4, 0] and seen["best"] in (1, 2, 4, 0)
+    assert all(ms > 0 for _, ms in seen["table"])
+    for k in range(len(scenes)):
+        same(ref[k], got[k], f"autotuned canvas {k}")

@@ -90,7 +90,10 @@ struct tpt_sur {
            *g_u2 = nullptr, *g_d2a = nullptr, *g_a3b = nullptr, *g_a3 = nullptr, *g_p2 = nullptr,
            *g_a2b = nullptr, *g_a2 = nullptr, *g_p1 = nullptr, *g_a1 = nullptr;
     float *loss_acc = nullptr;      // device accumulators: loss, energy in, energy out, non-finite count
-    float scale[4] = {10.0f, 2.0f, 2.0f, 200.0f};
+    float *gnorm = nullptr;         // device: sum of squares of the batch gradient
+    float clip = 1.0f;              // global gradient-norm clip (0: none)
+    float in_scale[4] = {10.0f, 1.0f, 1.0f, 500.0f};   // input normalisation (state)
+    float scale[4] = {1.0f, 1.0f, 1.0f, 10.0f};        // output normalisation (8-tick change)
     float ambient = 295.15f;
     // heat sources of the current scene: temperature per cell, 0 elsewhere
     float *src = nullptr;
@@ -122,8 +125,8 @@ template <typename T> int alloc_n(tpt_sur *s, T **p, size_t n) { return alloc(s,
 
 dim3 grid1(size_t n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
 
-template <int NT> void set_conv_lds(int bytes) {
-    (void)hipFuncSetAttribute((const void *)conv3x3<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+template <int NT, int MS> void set_conv_lds(int bytes) {
+    (void)hipFuncSetAttribute((const void *)conv3x3<NT, MS>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 template <int MT> void set_wgrad_lds(int bytes) {
     (void)hipFuncSetAttribute((const void *)conv3x3_wgrad<MT>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -149,12 +152,20 @@ void conv(const tpt_sur *s, hipStream_t st, const bf16_t *in, int cin, int cout,
     a.kchunks = kchunks;
     a.gelu = gelu;
     a.bias_col = bias_col;
-    const size_t lds = conv_lds_bytes(cin);
-    const int nt = cout >= 64 ? 4 : cout >= 32 ? 2 : 1;
-    const dim3 g((a.W + CONV_SEG - 1) / CONV_SEG, (a.H + CONV_ROWS - 1) / CONV_ROWS, (cout + 16 * nt - 1) / (16 * nt));
-    if (nt == 4) hipLaunchKernelGGL(conv3x3<4>, g, dim3(CONV_THREADS), lds, st, a);
-    else if (nt == 2) hipLaunchKernelGGL(conv3x3<2>, g, dim3(CONV_THREADS), lds, st, a);
-    else hipLaunchKernelGGL(conv3x3<1>, g, dim3(CONV_THREADS), lds, st, a);
+    // Four M sub-tiles per wave (a full 64-pixel row) where the input tile
+    // leaves room for two workgroups per CU, two for the 128-channel inputs.
+    const int nt = cout >= 64 ? 4 : cout >= 32 ? 2 : 1, ms = cin <= 64 ? 4 : 2;
+    const size_t lds = conv_lds_bytes(cin, ms);
+    const dim3 g((a.W + CONV_SEG - 1) / CONV_SEG, (a.H + ms - 1) / ms, (cout + 16 * nt - 1) / (16 * nt));
+    if (ms == 4) {
+        if (nt == 4) hipLaunchKernelGGL((conv3x3<4, 4>), g, dim3(CONV_THREADS), lds, st, a);
+        else if (nt == 2) hipLaunchKernelGGL((conv3x3<2, 4>), g, dim3(CONV_THREADS), lds, st, a);
+        else hipLaunchKernelGGL((conv3x3<1, 4>), g, dim3(CONV_THREADS), lds, st, a);
+    } else {
+        if (nt == 4) hipLaunchKernelGGL((conv3x3<4, 2>), g, dim3(CONV_THREADS), lds, st, a);
+        else if (nt == 2) hipLaunchKernelGGL((conv3x3<2, 2>), g, dim3(CONV_THREADS), lds, st, a);
+        else hipLaunchKernelGGL((conv3x3<1, 2>), g, dim3(CONV_THREADS), lds, st, a);
+    }
 }
 
 // Weight gradient of layer l into s->grad (accumulated when `accumulate`).
@@ -184,6 +195,7 @@ void wgrad(tpt_sur *s, hipStream_t st, int l, const bf16_t *x, const bf16_t *dz,
                        s->grad + L.offset, groups, L.cout, a.cout16, L.kpad, accumulate);
 }
 
+// `z` receives the pre-activation when training needs it (null otherwise).
 void fwd_layer(tpt_sur *s, hipStream_t st, int l, const bf16_t *in, bf16_t *out, bf16_t *z, float *out_f32) {
     const Layer &L = s->layer[l];
     conv(s, st, in, L.cin, L.cout, L.level, L.w, L.kpad, L.kchunks, out, z, nullptr, out_f32, L.gelu, 1);
@@ -196,22 +208,25 @@ void dgrad_layer(tpt_sur *s, hipStream_t st, int l, const bf16_t *dz, bf16_t *dx
     conv(s, st, dz, L.cout, L.cin, L.level, L.wt, L.kpad_t, L.kchunks_t, dx, nullptr, zref, nullptr, 0, 0);
 }
 
-void forward(tpt_sur *s, hipStream_t st) {
+// The network on s->x0 into s->out; `train` keeps the pre-activations the
+// backward pass needs.
+void forward(tpt_sur *s, hipStream_t st, bool train) {
     const int H = s->H, W = s->W;
-    fwd_layer(s, st, 0, s->x0, s->a1, s->z1, nullptr);
+    const auto Z = [train](bf16_t *z) { return train ? z : nullptr; };
+    fwd_layer(s, st, 0, s->x0, s->a1, Z(s->z1), nullptr);
     hipLaunchKernelGGL(pool2, grid1(cells_at(s, 1) * 32), dim3(256), 0, st, s->a1, s->p1, H, W, 32);
-    fwd_layer(s, st, 1, s->p1, s->a2, s->z2, nullptr);
-    fwd_layer(s, st, 2, s->a2, s->a2b, s->z2b, nullptr);
+    fwd_layer(s, st, 1, s->p1, s->a2, Z(s->z2), nullptr);
+    fwd_layer(s, st, 2, s->a2, s->a2b, Z(s->z2b), nullptr);
     hipLaunchKernelGGL(pool2, grid1(cells_at(s, 2) * 64), dim3(256), 0, st, s->a2b, s->p2, H / 2, W / 2, 64);
-    fwd_layer(s, st, 3, s->p2, s->a3, s->z3, nullptr);
-    fwd_layer(s, st, 4, s->a3, s->a3b, s->z3b, nullptr);
-    fwd_layer(s, st, 5, s->a3b, s->d2a, s->zd2a, nullptr);
+    fwd_layer(s, st, 3, s->p2, s->a3, Z(s->z3), nullptr);
+    fwd_layer(s, st, 4, s->a3, s->a3b, Z(s->z3b), nullptr);
+    fwd_layer(s, st, 5, s->a3b, s->d2a, Z(s->zd2a), nullptr);
     hipLaunchKernelGGL(up2_add, grid1(cells_at(s, 1) * 64), dim3(256), 0, st, s->d2a, s->a2b, s->u2, H / 2, W / 2,
                        64);
-    fwd_layer(s, st, 6, s->u2, s->d2b, s->zd2b, nullptr);
-    fwd_layer(s, st, 7, s->d2b, s->d1a, s->zd1a, nullptr);
+    fwd_layer(s, st, 6, s->u2, s->d2b, Z(s->zd2b), nullptr);
+    fwd_layer(s, st, 7, s->d2b, s->d1a, Z(s->zd1a), nullptr);
     hipLaunchKernelGGL(up2_add, grid1(cells_at(s, 0) * 32), dim3(256), 0, st, s->d1a, s->a1, s->u1, H, W, 32);
-    fwd_layer(s, st, 8, s->u1, s->d1b, s->zd1b, nullptr);
+    fwd_layer(s, st, 8, s->u1, s->d1b, Z(s->zd1b), nullptr);
     fwd_layer(s, st, 9, s->d1b, nullptr, nullptr, s->out);
 }
 
@@ -507,6 +522,7 @@ extern "C" int tpt_sur_create(uint32_t xcells, uint32_t ycells, tpt_sur **out) {
         if (!rc) rc = alloc_n(s, a.p, a.n);
     if (!rc) rc = alloc_n(s, &s->out, n0 * OUT_CH);
     if (!rc) rc = alloc_n(s, &s->loss_acc, 4);
+    if (!rc) rc = alloc_n(s, &s->gnorm, 1);
     if (!rc) rc = alloc_n(s, &s->src, n0);
     if (!rc) rc = alloc_n(s, &s->saved, 4 * n0);
     if (!rc && hipStreamCreate(&s->stream) != hipSuccess) rc = tpt_set_error(TPT_EDEVICE, "surrogate stream");
@@ -521,9 +537,12 @@ extern "C" int tpt_sur_create(uint32_t xcells, uint32_t ycells, tpt_sur **out) {
         L.wt = s->wtb + wt_off;
         wt_off += (size_t)L.rows_t * L.kpad_t;
     }
-    set_conv_lds<1>((int)conv_lds_bytes(128));
-    set_conv_lds<2>((int)conv_lds_bytes(128));
-    set_conv_lds<4>((int)conv_lds_bytes(128));
+    set_conv_lds<1, 4>((int)conv_lds_bytes(64, 4));
+    set_conv_lds<2, 4>((int)conv_lds_bytes(64, 4));
+    set_conv_lds<4, 4>((int)conv_lds_bytes(64, 4));
+    set_conv_lds<1, 2>((int)conv_lds_bytes(128, 2));
+    set_conv_lds<2, 2>((int)conv_lds_bytes(128, 2));
+    set_conv_lds<4, 2>((int)conv_lds_bytes(128, 2));
     set_wgrad_lds<1>((int)wgrad_lds_bytes(128, 128));
     set_wgrad_lds<2>((int)wgrad_lds_bytes(128, 128));
     set_wgrad_lds<4>((int)wgrad_lds_bytes(128, 128));
@@ -598,11 +617,14 @@ extern "C" int tpt_sur_get_grads(tpt_sur *s, float *grads, size_t n) {
     return TPT_OK;
 }
 
-extern "C" int tpt_sur_set_scales(tpt_sur *s, const float scales[4], float ambient) {
-    if (!s || !scales) return tpt_set_error(TPT_EINVAL, "null argument");
+extern "C" int tpt_sur_set_scales(tpt_sur *s, const float in_scales[4], const float out_scales[4], float ambient) {
+    if (!s || !in_scales || !out_scales) return tpt_set_error(TPT_EINVAL, "null argument");
+    for (int c = 0; c < 4; ++c)
+        if (!(in_scales[c] > 0.0f) || !(out_scales[c] > 0.0f))
+            return tpt_set_error(TPT_EINVAL, "scale %d must be positive", c);
     for (int c = 0; c < 4; ++c) {
-        if (!(scales[c] > 0.0f)) return tpt_set_error(TPT_EINVAL, "scale %d must be positive", c);
-        s->scale[c] = scales[c];
+        s->in_scale[c] = in_scales[c];
+        s->scale[c] = out_scales[c];
     }
     s->ambient = ambient;
     return TPT_OK;
@@ -615,7 +637,7 @@ extern "C" int tpt_sur_forward_raw(tpt_sur *s, const uint16_t *x0, float *out) {
     if (!s || !x0 || !out) return tpt_set_error(TPT_EINVAL, "null argument");
     const size_t n = cells_at(s, 0);
     TPT_HIP_CHECK(hipMemcpyAsync(s->x0, x0, n * IN_CH * 2, hipMemcpyHostToDevice, s->stream));
-    forward(s, s->stream);
+    forward(s, s->stream, true);
     TPT_HIP_CHECK(hipMemcpyAsync(out, s->out, n * OUT_CH * 4, hipMemcpyDeviceToHost, s->stream));
     TPT_HIP_CHECK(hipStreamSynchronize(s->stream));
     return tpt_check_launch("surrogate forward");
@@ -702,8 +724,8 @@ extern "C" int tpt_sur_train(tpt_sur *s, const uint32_t *pairs, uint32_t n_pairs
         const uint32_t tr = pairs[2 * i], sn = pairs[2 * i + 1];
         const Planes cur = snap_planes(s, tr, sn), nxt = snap_planes(s, tr, sn + 1);
         hipLaunchKernelGGL(sur_prep, grid1(n), dim3(256), 0, st, cur, s->walls + (size_t)tr * n, nullptr,
-                           s->srcs + (size_t)tr * n, s->x0, n, s->scale[0], s->scale[1], s->scale[3], s->ambient);
-        forward(s, st);
+                           s->srcs + (size_t)tr * n, s->x0, n, s->in_scale[0], s->in_scale[1], s->in_scale[3], s->ambient);
+        forward(s, st, true);
         hipLaunchKernelGGL(sur_loss, grid1(n), dim3(256), 0, st, s->out, cur, nxt, s->g_out, s->loss_acc, n,
                            s->scale[0], s->scale[1], s->scale[3]);
         backward(s, st, in_batch > 0 ? 1 : 0);
@@ -711,8 +733,11 @@ extern "C" int tpt_sur_train(tpt_sur *s, const uint32_t *pairs, uint32_t n_pairs
             s->adam_t += 1;
             const float b1 = 0.9f, b2 = 0.999f;
             const float c1 = 1.0f - powf(b1, (float)s->adam_t), c2 = 1.0f - powf(b2, (float)s->adam_t);
+            hipLaunchKernelGGL(sur_zero_f, dim3(1), dim3(64), 0, st, s->gnorm, 1);
+            hipLaunchKernelGGL(grad_sumsq, grid1(s->n_params), dim3(256), 0, st, s->grad, (int)s->n_params, s->gnorm);
             hipLaunchKernelGGL(adam_step, grid1(s->n_params), dim3(256), 0, st, s->wf, s->wb, s->grad, s->m1, s->m2,
-                               (int)s->n_params, lr, b1, b2, 1e-8f, c1, c2, 1.0f / (float)in_batch);
+                               (int)s->n_params, lr, b1, b2, 1e-8f, c1, c2, 1.0f / (float)in_batch, s->gnorm,
+                               s->clip);
             refresh_transposes(s, st);
             in_batch = 0;
         }
@@ -722,6 +747,13 @@ extern "C" int tpt_sur_train(tpt_sur *s, const uint32_t *pairs, uint32_t n_pairs
     TPT_HIP_CHECK(hipStreamSynchronize(st));
     if (mean_loss) *mean_loss = acc[0] / (float)n_pairs;
     return tpt_check_launch("surrogate training");
+}
+
+// Global L2 norm bound of the averaged batch gradient (0 turns clipping off).
+extern "C" int tpt_sur_set_clip(tpt_sur *s, float clip) {
+    if (!s || !(clip >= 0.0f)) return tpt_set_error(TPT_EINVAL, "bad clip");
+    s->clip = clip;
+    return TPT_OK;
 }
 
 // Loss of the current parameters on dataset pairs, without training.
@@ -736,8 +768,8 @@ extern "C" int tpt_sur_eval_loss(tpt_sur *s, const uint32_t *pairs, uint32_t n_p
         if (tr >= s->n_traj || sn + 1 >= s->n_snap) return tpt_set_error(TPT_EINVAL, "pair %u out of range", i);
         const Planes cur = snap_planes(s, tr, sn), nxt = snap_planes(s, tr, sn + 1);
         hipLaunchKernelGGL(sur_prep, grid1(n), dim3(256), 0, st, cur, s->walls + (size_t)tr * n, nullptr,
-                           s->srcs + (size_t)tr * n, s->x0, n, s->scale[0], s->scale[1], s->scale[3], s->ambient);
-        forward(s, st);
+                           s->srcs + (size_t)tr * n, s->x0, n, s->in_scale[0], s->in_scale[1], s->in_scale[3], s->ambient);
+        forward(s, st, false);
         hipLaunchKernelGGL(sur_loss, grid1(n), dim3(256), 0, st, s->out, cur, nxt, s->g_out, s->loss_acc, n,
                            s->scale[0], s->scale[1], s->scale[3]);
     }
@@ -764,8 +796,8 @@ extern "C" int tpt_sur_step_air(tpt_sur *s, tpt_ctx *ctx, int check, float energ
         hipLaunchKernelGGL(sur_zero_f, dim3(1), dim3(64), 0, st, s->loss_acc, 4);
     }
     hipLaunchKernelGGL(sur_prep, grid1(n), dim3(256), 0, st, p, nullptr, tpt_buf<uint32_t>(ctx, TPT_BUF_BLOCK_AIR),
-                       s->src, s->x0, n, s->scale[0], s->scale[1], s->scale[3], s->ambient);
-    forward(s, st);
+                       s->src, s->x0, n, s->in_scale[0], s->in_scale[1], s->in_scale[3], s->ambient);
+    forward(s, st, false);
     hipLaunchKernelGGL(sur_apply, grid1(n), dim3(256), 0, st, s->out, tpt_buf<float>(ctx, TPT_BUF_AIR_PV),
                        tpt_buf<float>(ctx, TPT_BUF_AIR_VX), tpt_buf<float>(ctx, TPT_BUF_AIR_VY),
                        tpt_buf<float>(ctx, TPT_BUF_AIR_HV), s->src, s->loss_acc, n, s->scale[0], s->scale[1],

@@ -47,17 +47,17 @@ __device__ __forceinline__ float gelu_grad(float z) {
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 // ---------------------------------------------------------------------------
-// Convolution: 256 threads, output tile CONV_ROWS x CONV_SEG pixels by
-// 16 * NT output channels.  Wave w takes row w / 2 of the tile and pixels
-// [32 (w % 2), +32) of it: two 16-pixel M sub-tiles by NT 16-channel N
-// sub-tiles.  The input tile (rows y0-1 .. y0+CONV_ROWS, columns x0-1 ..
-// x0+CONV_SEG) sits in LDS with a pixel stride of cin + 8 bf16, which
-// spreads the 16 pixels of a fragment read over the banks.
+// Convolution: 256 threads (4 waves).  Each wave owns 16 * MS consecutive
+// pixels of one row (MS 16-pixel M sub-tiles) and 16 * NT output channels
+// (NT 16-channel N sub-tiles), so a workgroup covers MS rows of a 64-pixel
+// segment.  The input tile (rows y0-1 .. y0+MS, columns x0-1 .. x0+64)
+// sits in LDS with a pixel stride of cin + 8 bf16, which spreads the 16
+// pixels of a fragment read over the banks.  The K loop fetches the next
+// chunk's fragments (LDS patches, weight rows from L2) before it issues the
+// current chunk's MFMAs.
 
 constexpr int CONV_THREADS = 256;
 constexpr int CONV_SEG = 64;
-constexpr int CONV_ROWS = 2;
-constexpr int CONV_LROWS = CONV_ROWS + 2;
 constexpr int CONV_LCOLS = CONV_SEG + 2;
 
 struct ConvArgs {
@@ -74,18 +74,39 @@ struct ConvArgs {
     int bias_col;         // the k = 9 * cin column meets a constant 1
 };
 
-__host__ __device__ inline size_t conv_lds_bytes(int cin) {
-    return (size_t)CONV_LROWS * CONV_LCOLS * (cin + 8) * sizeof(bf16_t);
+__host__ __device__ inline size_t conv_lds_bytes(int cin, int ms) {
+    return (size_t)(ms + 2) * CONV_LCOLS * (cin + 8) * sizeof(bf16_t);
 }
 
-template <int NT>
+template <int NT, int MS>
+__device__ __forceinline__ void conv_fetch(const bf16_t *tile, const bf16_t *const *wrow, const ConvArgs &a, int kc,
+                                           int g, int m, int row, int px0, int ps, uint4 (&av)[MS],
+                                           uint4 (&bv)[NT]) {
+    const int k0 = 32 * kc + 8 * g, tap = k0 >> a.lcin, ci0 = k0 & (a.cin - 1);
+    if (tap < 9) {
+        const int dy = tap / 3, dx = tap - 3 * dy;
+        const bf16_t *base = tile + ((row + dy) * CONV_LCOLS + px0 + m + dx) * ps + ci0;
+#pragma unroll
+        for (int j = 0; j < MS; ++j) av[j] = *reinterpret_cast<const uint4 *>(base + 16 * j * ps);
+    } else {
+        const uint4 z = (tap == 9 && ci0 == 0 && a.bias_col) ? make_uint4(0x3F80u, 0u, 0u, 0u)  // bf16 1.0
+                                                             : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int j = 0; j < MS; ++j) av[j] = z;
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const uint4 *>(wrow[t] + k0);
+}
+
+template <int NT, int MS>
 __global__ __launch_bounds__(CONV_THREADS) void conv3x3(ConvArgs a) {
     extern __shared__ __align__(16) bf16_t tile[];
+    constexpr int WAVES_PER_ROW = 4 / MS;
     const int ps = a.cin + 8;
-    const int x0 = blockIdx.x * CONV_SEG, y0 = blockIdx.y * CONV_ROWS, nbase = blockIdx.z * 16 * NT;
+    const int x0 = blockIdx.x * CONV_SEG, y0 = blockIdx.y * MS, nbase = blockIdx.z * 16 * NT;
     const int tid = threadIdx.x;
     // stage the input tile with a zero halo ('same' padding)
-    const int v8 = a.cin >> 3, total = CONV_LROWS * CONV_LCOLS * v8;
+    const int v8 = a.cin >> 3, total = (MS + 2) * CONV_LCOLS * v8;
     for (int e = tid; e < total; e += CONV_THREADS) {
         const int v = e % v8, rc = e / v8, c = rc % CONV_LCOLS, r = rc / CONV_LCOLS;
         const int gy = y0 - 1 + r, gx = x0 - 1 + c;
@@ -96,48 +117,43 @@ __global__ __launch_bounds__(CONV_THREADS) void conv3x3(ConvArgs a) {
     }
     __syncthreads();
     const int lane = tid & 63, wave = tid >> 6, m = lane & 15, g = lane >> 4;
-    const int row = wave >> 1, half = wave & 1;
-    f32x4 acc[2][NT];
+    const int row = wave / WAVES_PER_ROW, px0 = (wave % WAVES_PER_ROW) * 16 * MS;
+    f32x4 acc[MS][NT];
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < MS; ++j)
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[j][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const bf16_t *wrow[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) wrow[t] = a.w + (size_t)(nbase + 16 * t + m) * a.kpad;
-    const uint4 one = make_uint4(0x3F80u, 0u, 0u, 0u);  // bf16 1.0 in element 0
+    uint4 av[MS], bv[NT];
+    conv_fetch<NT, MS>(tile, wrow, a, 0, g, m, row, px0, ps, av, bv);
     for (int kc = 0; kc < a.kchunks; ++kc) {
-        const int k0 = 32 * kc + 8 * g, tap = k0 >> a.lcin, ci0 = k0 & (a.cin - 1);
-        uint4 av[2];
-        if (tap < 9) {
-            const int dy = tap / 3, dx = tap - 3 * dy;
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int px = 32 * half + 16 * j + m;
-                av[j] = *reinterpret_cast<const uint4 *>(tile + ((row + dy) * CONV_LCOLS + px + dx) * ps + ci0);
-            }
-        } else {
-            const uint4 z = (tap == 9 && ci0 == 0 && a.bias_col) ? one : make_uint4(0u, 0u, 0u, 0u);
-            av[0] = av[1] = z;
-        }
+        uint4 an[MS], bn[NT];
+        if (kc + 1 < a.kchunks) conv_fetch<NT, MS>(tile, wrow, a, kc + 1, g, m, row, px0, ps, an, bn);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const bf16x8 b = as_bf16x8(*reinterpret_cast<const uint4 *>(wrow[t] + k0));
-            acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av[0]), b, acc[0][t], 0, 0, 0);
-            acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av[1]), b, acc[1][t], 0, 0, 0);
+            const bf16x8 b = as_bf16x8(bv[t]);
+#pragma unroll
+            for (int j = 0; j < MS; ++j)
+                acc[j][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av[j]), b, acc[j][t], 0, 0, 0);
         }
+#pragma unroll
+        for (int j = 0; j < MS; ++j) av[j] = an[j];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) bv[t] = bn[t];
     }
     const int y = y0 + row;
     if (y >= a.H) return;
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < MS; ++j)
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int co = nbase + 16 * t + m;
             if (co >= a.cout) continue;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int x = x0 + 32 * half + 16 * j + 4 * g + i;
+                const int x = x0 + px0 + 16 * j + 4 * g + i;
                 if (x >= a.W) continue;
                 const size_t o = ((size_t)y * a.W + x) * a.cout + co;
                 float v = acc[j][t][i];
@@ -313,13 +329,28 @@ __global__ void transpose_flip(const bf16_t *__restrict__ w, bf16_t *__restrict_
     wt[idx] = v;
 }
 
-// Adam on the fp32 master parameters, then the bf16 copy.
+// Sum of squares of the gradient vector into *acc (one atomic per wave).
+__global__ void grad_sumsq(const float *__restrict__ grad, int n, float *__restrict__ acc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float v = i < n ? grad[i] * grad[i] : 0.0f;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(acc, v);
+}
+
+// Adam on the fp32 master parameters, then the bf16 copy.  The gradient is
+// the sum over the batch times `gscale`, scaled down so that its global L2
+// norm (sqrt(*sumsq) * gscale) is at most `clip` when clip > 0.
 __global__ void adam_step(float *__restrict__ wf, bf16_t *__restrict__ wb, const float *__restrict__ grad,
                           float *__restrict__ m1, float *__restrict__ m2, int n, float lr, float b1, float b2,
-                          float eps, float c1, float c2, float gscale) {
+                          float eps, float c1, float c2, float gscale, const float *__restrict__ sumsq, float clip) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float gr = grad[i] * gscale;
+    float f = gscale;
+    if (clip > 0.0f) {
+        const float norm = sqrtf(*sumsq) * gscale;
+        if (norm > clip) f *= clip / norm;
+    }
+    const float gr = grad[i] * f;
     const float a = b1 * m1[i] + (1.0f - b1) * gr;
     const float b = b2 * m2[i] + (1.0f - b2) * gr * gr;
     m1[i] = a;

@@ -17,7 +17,9 @@
 
 #include "tpt_internal.h"
 
+#include <algorithm>
 #include <new>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -157,6 +159,10 @@ struct tpt_batch {
     hipGraphExec_t graph = nullptr;
     std::vector<uint64_t> epochs;
     std::vector<uint32_t> slots;
+    // Work split of the captured tick: the canvases are dealt to `lanes`
+    // serial chains that run side by side (0: every canvas its own chain).
+    uint32_t lanes = 0;
+    std::vector<std::pair<uint32_t, float>> tuning;  // autotuner: (lanes, ms per batch tick)
 };
 
 static void batch_free(tpt_batch *b) {
@@ -219,20 +225,45 @@ static int stream_after(hipStream_t waiter, hipStream_t on, hipEvent_t ev) {
     return TPT_OK;
 }
 
+// The canvases of each lane, largest first, dealt greedily to the lane
+// with the least work so far (work = pixels + particles' slot range).
+static std::vector<std::vector<size_t>> batch_lanes(const tpt_batch *b) {
+    const size_t n = b->ctx.size();
+    const size_t L = (b->lanes == 0 || b->lanes >= n) ? n : b->lanes;
+    std::vector<size_t> order(n);
+    for (size_t k = 0; k < n; ++k) order[k] = k;
+    auto work = [b](size_t k) { return (uint64_t)b->ctx[k]->pixels + b->ctx[k]->slot_bound; };
+    std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return work(x) > work(y); });
+    std::vector<std::vector<size_t>> lanes(L);
+    std::vector<uint64_t> load(L, 0);
+    for (size_t k : order) {
+        const size_t l = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        lanes[l].push_back(k);
+        load[l] += work(k);
+    }
+    return lanes;
+}
+
 static int batch_capture(tpt_batch *b) {
     if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }
+    const std::vector<std::vector<size_t>> lanes = batch_lanes(b);
     TPT_HIP_CHECK(hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
     int rc = TPT_OK;
     hipError_t e = hipEventRecord(b->fork, b->main);
-    for (size_t k = 0; k < b->ctx.size() && rc == TPT_OK && e == hipSuccess; ++k) {
-        tpt_ctx *c = b->ctx[k];
-        e = hipStreamWaitEvent(c->stream, b->fork, 0);
-        if (e == hipSuccess) rc = record_tick(c);
-        if (rc == TPT_OK && e == hipSuccess) e = hipEventRecord(b->join[k], c->stream);
-        if (rc == TPT_OK && e == hipSuccess) e = hipStreamWaitEvent(b->main, b->join[k], 0);
+    for (size_t l = 0; l < lanes.size() && rc == TPT_OK && e == hipSuccess; ++l) {
+        hipEvent_t after = b->fork;  // each lane's first canvas starts at the fork
+        for (size_t k : lanes[l]) {
+            if (rc != TPT_OK || e != hipSuccess) break;
+            tpt_ctx *c = b->ctx[k];
+            e = hipStreamWaitEvent(c->stream, after, 0);
+            if (e == hipSuccess) rc = record_tick(c);
+            if (rc == TPT_OK && e == hipSuccess) e = hipEventRecord(b->join[k], c->stream);
+            after = b->join[k];
+        }
+        if (rc == TPT_OK && e == hipSuccess) e = hipStreamWaitEvent(b->main, after, 0);
     }
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(b->main, &g);
@@ -281,6 +312,99 @@ extern "C" int tpt_batch_step_launches(tpt_batch *b, uint64_t first_frame, uint3
             int rc = require(c);
             if (rc || (rc = tick(c, first_frame + t, false))) return rc;
         }
+    return TPT_OK;
+}
+
+// One canvas at a time, the way the reference steps its engines: canvas k's
+// tick waits on the GPU for canvas k-1's (no host synchronisation).
+extern "C" int tpt_batch_step_serial(tpt_batch *b, uint64_t first_frame, uint32_t ticks) {
+    if (!b) return tpt_set_error(TPT_EINVAL, "null batch");
+    const size_t n = b->ctx.size();
+    for (uint32_t t = 0; t < ticks; ++t)
+        for (size_t k = 0; k < n; ++k) {
+            tpt_ctx *c = b->ctx[k];
+            int rc = require(c);
+            const size_t prev = k ? k - 1 : n - 1;
+            if (!rc && (t || k) && n > 1) rc = stream_after(c->stream, b->ctx[prev]->stream, b->pre[prev]);
+            if (rc || (rc = tick(c, first_frame + t, false))) return rc;
+        }
+    return TPT_OK;
+}
+
+// Times the captured tick at several work splits on live replays and keeps
+// the fastest.  Candidates: 1, 2, 4, ... lanes up to the canvas count, and
+// one lane per canvas.  Advances every canvas by 1 + sum(trial_ticks + 1)
+// ticks; results equal tpt_batch_step's for the same number of ticks.
+extern "C" int tpt_batch_autotune(tpt_batch *b, uint64_t first_frame, uint32_t trial_ticks, uint32_t *lanes_out) {
+    if (!b || trial_ticks == 0) return tpt_set_error(TPT_EINVAL, "batch autotune needs a batch and trial ticks");
+    int rc = TPT_OK;
+    for (tpt_ctx *c : b->ctx)
+        if ((rc = require(c)) || (rc = tick(c, first_frame, false))) return rc;
+    for (size_t k = 0; k < b->ctx.size(); ++k)
+        if ((rc = stream_after(b->main, b->ctx[k]->stream, b->pre[k]))) return rc;
+    const uint32_t n = (uint32_t)b->ctx.size();
+    std::vector<uint32_t> cands;
+    for (uint32_t L = 1; L < n; L *= 2) cands.push_back(L);
+    cands.push_back(0);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    TPT_HIP_CHECK(hipEventCreate(&t0));
+    TPT_HIP_CHECK(hipEventCreate(&t1));
+    b->tuning.clear();
+    uint32_t best = 0;
+    float best_ms = 0.0f;
+    for (uint32_t L : cands) {
+        b->lanes = L;
+        if ((rc = batch_capture(b))) break;
+        hipError_t e = hipGraphLaunch(b->graph, b->main);  // warm
+        if (e == hipSuccess) e = hipEventRecord(t0, b->main);
+        for (uint32_t t = 0; t < trial_ticks && e == hipSuccess; ++t) e = hipGraphLaunch(b->graph, b->main);
+        if (e == hipSuccess) e = hipEventRecord(t1, b->main);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        if (e != hipSuccess) {
+            rc = tpt_set_error(TPT_EDEVICE, "batch autotune: %s", hipGetErrorString(e));
+            break;
+        }
+        ms /= (float)trial_ticks;
+        b->tuning.push_back({L, ms});
+        if (b->tuning.size() == 1 || ms < best_ms) {
+            best = L;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc) return rc;
+    b->lanes = best;
+    if ((rc = batch_capture(b))) return rc;
+    for (tpt_ctx *c : b->ctx)
+        if ((rc = stream_after(c->stream, b->main, b->done))) return rc;
+    if (lanes_out) *lanes_out = best;
+    return TPT_OK;
+}
+
+extern "C" int tpt_batch_tuning(tpt_batch *b, uint32_t *lanes, float *ms_per_tick, uint32_t max_entries,
+                                uint32_t *n_entries) {
+    if (!b || !n_entries) return tpt_set_error(TPT_EINVAL, "null argument");
+    uint32_t n = 0;
+    for (const auto &t : b->tuning) {
+        if (n < max_entries) {
+            if (lanes) lanes[n] = t.first;
+            if (ms_per_tick) ms_per_tick[n] = t.second;
+        }
+        ++n;
+    }
+    *n_entries = n;
+    return TPT_OK;
+}
+
+extern "C" int tpt_batch_set_lanes(tpt_batch *b, uint32_t lanes) {
+    if (!b) return tpt_set_error(TPT_EINVAL, "null batch");
+    if (lanes != b->lanes) {
+        b->lanes = lanes;
+        b->epochs.clear();  // recapture on the next tpt_batch_step
+    }
     return TPT_OK;
 }
 

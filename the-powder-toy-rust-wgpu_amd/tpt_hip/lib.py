@@ -65,6 +65,10 @@ SIGNATURES = {
     "tpt_batch_step": ([_vp, ctypes.c_uint64, ctypes.c_uint32], ctypes.c_int),
     "tpt_batch_step_launches": ([_vp, ctypes.c_uint64, ctypes.c_uint32], ctypes.c_int),
     "tpt_batch_synchronize": ([_vp], ctypes.c_int),
+    "tpt_batch_step_serial": ([_vp, ctypes.c_uint64, ctypes.c_uint32], ctypes.c_int),
+    "tpt_batch_set_lanes": ([_vp, ctypes.c_uint32], ctypes.c_int),
+    "tpt_batch_autotune": ([_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)], ctypes.c_int),
+    "tpt_batch_tuning": ([_vp, _vp, _vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)], ctypes.c_int),
     "tpt_sur_create": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)], ctypes.c_int),
     "tpt_sur_destroy": ([_vp], ctypes.c_int),
     "tpt_sur_param_count": ([_vp, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
@@ -72,7 +76,7 @@ SIGNATURES = {
     "tpt_sur_set_params": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
     "tpt_sur_get_params": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
     "tpt_sur_get_grads": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
-    "tpt_sur_set_scales": ([_vp, _vp, ctypes.c_float], ctypes.c_int),
+    "tpt_sur_set_scales": ([_vp, _vp, _vp, ctypes.c_float], ctypes.c_int),
     "tpt_sur_forward_raw": ([_vp, _vp, _vp], ctypes.c_int),
     "tpt_sur_backward_raw": ([_vp, _vp], ctypes.c_int),
     "tpt_sur_scene": ([_vp, _vp, _vp], ctypes.c_int),
@@ -82,6 +86,7 @@ SIGNATURES = {
     "tpt_sur_train": ([_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.POINTER(ctypes.c_float)],
                       ctypes.c_int),
     "tpt_sur_eval_loss": ([_vp, _vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "tpt_sur_set_clip": ([_vp, ctypes.c_float], ctypes.c_int),
     "tpt_sur_step_air": ([_vp, _vp, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "tpt_sur_last_check": ([_vp, _vp], ctypes.c_int),
     "tpt_frame_seed": ([ctypes.c_uint64], ctypes.c_uint32),
@@ -275,6 +280,22 @@ class Batch:
 
     def step_launches(self, first_frame, ticks):
         _check(load().tpt_batch_step_launches(self._h, first_frame, ticks))
+
+    def step_serial(self, first_frame, ticks):
+        _check(load().tpt_batch_step_serial(self._h, first_frame, ticks))
+
+    def set_lanes(self, lanes):
+        _check(load().tpt_batch_set_lanes(self._h, lanes))
+
+    def autotune(self, first_frame, trial_ticks):
+        """Returns (chosen lanes, [(lanes, ms per batch tick), ...])."""
+        best = ctypes.c_uint32()
+        _check(load().tpt_batch_autotune(self._h, first_frame, trial_ticks, ctypes.byref(best)))
+        lanes = (ctypes.c_uint32 * 64)()
+        ms = (ctypes.c_float * 64)()
+        n = ctypes.c_uint32()
+        _check(load().tpt_batch_tuning(self._h, lanes, ms, 64, ctypes.byref(n)))
+        return best.value, [(int(lanes[i]), float(ms[i])) for i in range(min(n.value, 64))]
 
     def synchronize(self):
         _check(load().tpt_batch_synchronize(self._h))

@@ -27,6 +27,9 @@ from . import lib as L
 from . import scene as S
 
 TICKS_PER_CALL = 8
+# input normalisation: pressure, vx, vy, heat above ambient (the scenes'
+# pressure blobs reach +-30, velocities a few cells per tick, sources 3000 K)
+IN_SCALES = (10.0, 1.0, 1.0, 500.0)
 MAX_BLOBS, MAX_RECTS, MAX_SOURCES = 16, 32, 8
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "air_surrogate_512.npz")
@@ -77,20 +80,21 @@ def air_config(xc, yc):
 class Surrogate:
     """A tpt_sur network for an xc x yc air grid."""
 
-    def __init__(self, xc, yc, scales=(10.0, 2.0, 2.0, 200.0), ambient=295.15):
+    def __init__(self, xc, yc, in_scales=IN_SCALES, out_scales=(1.0, 1.0, 1.0, 10.0), ambient=295.15):
         self.xc, self.yc = xc, yc
         self._h = ctypes.c_void_p()
         L._check(L.load().tpt_sur_create(xc, yc, ctypes.byref(self._h)))
         n = ctypes.c_size_t()
         L._check(L.load().tpt_sur_param_count(self._h, ctypes.byref(n)))
         self.n_params = n.value
-        self.set_scales(scales, ambient)
+        self.set_scales(in_scales, out_scales, ambient)
 
-    def set_scales(self, scales, ambient=295.15):
-        self.scales = tuple(float(s) for s in scales)
-        arr = (ctypes.c_float * 4)(*self.scales)
+    def set_scales(self, in_scales, out_scales, ambient=295.15):
+        self.in_scales = tuple(float(s) for s in in_scales)
+        self.out_scales = tuple(float(s) for s in out_scales)
         self.ambient = float(ambient)
-        L._check(L.load().tpt_sur_set_scales(self._h, arr, ctypes.c_float(self.ambient)))
+        L._check(L.load().tpt_sur_set_scales(self._h, (ctypes.c_float * 4)(*self.in_scales),
+                                             (ctypes.c_float * 4)(*self.out_scales), ctypes.c_float(self.ambient)))
 
     def close(self):
         if self._h:
@@ -149,6 +153,9 @@ class Surrogate:
                                         ctypes.byref(loss)))
         return loss.value
 
+    def set_clip(self, clip):
+        L._check(L.load().tpt_sur_set_clip(self._h, ctypes.c_float(clip)))
+
     def eval_loss(self, pairs):
         pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1)
         loss = ctypes.c_float()
@@ -162,12 +169,13 @@ class Surrogate:
         return bool(fb.value)
 
     def save(self, path, meta=None):
-        np.savez(path, params=self.params(), scales=np.float32(self.scales), ambient=np.float32(self.ambient),
+        np.savez(path, params=self.params(), in_scales=np.float32(self.in_scales),
+                 out_scales=np.float32(self.out_scales), ambient=np.float32(self.ambient),
                  grid=np.int32([self.xc, self.yc]), meta=json.dumps(meta or {}))
 
     def load(self, path):
         with np.load(path, allow_pickle=False) as z:
-            self.set_scales(tuple(z["scales"]), float(z["ambient"]))
+            self.set_scales(tuple(z["in_scales"]), tuple(z["out_scales"]), float(z["ambient"]))
             self.set_params(z["params"])
             return json.loads(str(z["meta"]))
 
@@ -206,23 +214,33 @@ def delta_scales(ctx, sur, seeds, ticks=TICKS_PER_CALL):
                 ds[k].append(b[k] - a[k])
     sp = float(np.std(np.concatenate(ds["pv"])))
     sv = float(np.std(np.concatenate(ds["vx"] + ds["vy"])))
-    sh = float(np.std(np.concatenate(ds["hv"])))
+    # heat changes only near the sources: the spread of the cells that change
+    h = np.concatenate(ds["hv"])
+    h = h[np.abs(h) > 1e-3]
+    sh = float(np.std(h)) if h.size else 1.0
     return (max(sp, 1e-3), max(sv, 1e-3), max(sv, 1e-3), max(sh, 1e-3))
 
 
-def rel_l2(pred, ref, ambient):
-    """Relative L2 error per field, and over all four fields jointly (each
-    field normalised by the reference's norm), heat measured from ambient."""
-    out, tot_e, tot_r = {}, 0.0, 0.0
-    for k in ("pv", "vx", "vy", "hv"):
+FIELDS = ("pv", "vx", "vy", "hv")
+
+
+def sq_errors(pred, ref, ambient):
+    """Per field: (squared L2 error, squared L2 norm of the reference), heat
+    measured from ambient."""
+    out = {}
+    for k in FIELDS:
         a, b = pred[k].astype(np.float64), ref[k].astype(np.float64)
         if k == "hv":
             a, b = a - ambient, b - ambient
-        e, r = np.linalg.norm(a - b), np.linalg.norm(b)
-        out[k] = float(e / r) if r > 0 else float(e)
-        tot_e += (e / r) ** 2 if r > 0 else 0.0
-        tot_r += 1.0
-    out["joint"] = float(np.sqrt(tot_e / tot_r))
+        out[k] = (float(np.sum((a - b) ** 2)), float(np.sum(b * b)))
+    return out
+
+
+def rel_l2(sums):
+    """Relative L2 error per field over a whole evaluation set (errors and
+    reference norms summed over its scenes), and the mean over the fields."""
+    out = {k: float(np.sqrt(sums[k][0] / sums[k][1])) if sums[k][1] > 0 else 0.0 for k in FIELDS}
+    out["mean"] = float(np.mean([out[k] for k in FIELDS]))
     return out
 
 
@@ -230,22 +248,30 @@ def evaluate(sur, ctx, seeds, calls=8, check=True, energy_ratio=1.05, persistenc
     """Relative L2 error after `calls` surrogate calls (8 ticks each) against
     the plain solver from the same scene, per seed; fallback count."""
     n = sur.xc * sur.yc
-    errs, fallbacks, total = [], 0, 0
+    sums = {k: [0.0, 0.0] for k in FIELDS}
+    first = {k: [0.0, 0.0] for k in FIELDS}   # after the first call (8 ticks)
+    fallbacks, total = 0, 0
     for seed in seeds:
         desc = random_scene(seed, sur.xc, sur.yc)
         sur.scene(ctx, desc)
-        sur.plain_ticks(ctx, TICKS_PER_CALL * calls)
+        sur.plain_ticks(ctx, TICKS_PER_CALL)
+        ref1 = air_state(ctx, n)
+        sur.plain_ticks(ctx, TICKS_PER_CALL * (calls - 1))
         ref = air_state(ctx, n)
         sur.scene(ctx, desc)
-        if not persistence:
-            for _ in range(calls):
+        for c in range(calls):
+            if not persistence:
                 fallbacks += sur.step_air(ctx, check=check, energy_ratio=energy_ratio)
                 total += 1
-        pred = air_state(ctx, n)
-        errs.append(rel_l2(pred, ref, sur.ambient))
-    mean = {k: float(np.mean([e[k] for e in errs])) for k in errs[0]}
-    return {"rel_l2": mean, "fallbacks": fallbacks, "calls": total,
-            "fallback_rate": (fallbacks / total) if total else 0.0}
+            if c == 0:
+                for k, (e, r) in sq_errors(air_state(ctx, n), ref1, sur.ambient).items():
+                    first[k][0] += e
+                    first[k][1] += r
+        for k, (e, r) in sq_errors(air_state(ctx, n), ref, sur.ambient).items():
+            sums[k][0] += e
+            sums[k][1] += r
+    return {"rel_l2": rel_l2(sums), "rel_l2_8_ticks": rel_l2(first), "fallbacks": fallbacks, "calls": total,
+            "fallback_rate": (fallbacks / total) if total else 0.0, "seeds": len(seeds)}
 
 
 def timing(sur, ctx, seed=12345, reps=20):
@@ -276,9 +302,10 @@ def train_main(args):
     ctx.load_state(S.SimState(air_config(xc, yc)))
     sur = Surrogate(xc, yc)
     scales = delta_scales(ctx, sur, range(10 ** 6, 10 ** 6 + 8))
-    sur.set_scales(scales)
-    log(f"scales {scales}")
+    sur.set_scales(IN_SCALES, scales)
+    log(f"output scales {scales}")
     sur.init_params(args.seed)
+    sur.set_clip(args.clip)
     train_seeds = list(range(args.traj))
     t0 = time.perf_counter()
     build_dataset(sur, ctx, train_seeds, log=log)
@@ -296,7 +323,8 @@ def train_main(args):
         losses = []
         for c0 in range(0, len(order), chunk):
             frac = steps_done / max(total_steps, 1)
-            lr = args.lr * 0.5 * (1.0 + np.cos(np.pi * min(frac, 1.0)))
+            warm = min(1.0, (steps_done + 1) / args.warmup_steps)
+            lr = args.lr * warm * 0.5 * (1.0 + np.cos(np.pi * min(frac, 1.0)))
             sel = pairs[order[c0:c0 + chunk]]
             losses.append(sur.train(sel, args.accum, lr))
             steps_done += (len(sel) + args.accum - 1) // args.accum
@@ -339,7 +367,9 @@ def main(argv=None):
     t.add_argument("--traj", type=int, default=2000)
     t.add_argument("--epochs", type=int, default=6)
     t.add_argument("--accum", type=int, default=4)
-    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--warmup-steps", type=int, default=500)
+    t.add_argument("--clip", type=float, default=1.0)
     t.add_argument("--seed", type=int, default=7)
     t.add_argument("--eval-seeds", type=int, default=0)
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
