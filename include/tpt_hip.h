@@ -202,6 +202,16 @@ TPT_API int tpt_recalc_free_particles(tpt_ctx *ctx, uint32_t seed, int stacking)
  * (crates/tpt-sim/src/thermal_kernel.rs:1574-1717). */
 TPT_API int tpt_step_thermal(tpt_ctx *ctx, uint32_t seed, int legacy);
 
+/* reaction_kernel::step_reactions_gpu (crates/tpt-sim/src/reaction_kernel/mod.rs:1123-1237)
+ * as far as this library implements it: the modext clear of the mass
+ * override channel and the life pass (life_and_local_reaction_kernel,
+ * reaction_kernel/life.rs:47-205: fire/plasma/TRON burn-in lifetimes, fire
+ * burning out to smoke or steam, plasma to noble gas or steam, sparks
+ * settling into their conductor, the element kill at zero life, the stasis
+ * flag).  The combustion, fluid, energy, electronics and other element
+ * passes of the reference are not implemented. */
+TPT_API int tpt_step_reactions(tpt_ctx *ctx, uint32_t seed);
+
 /* move_kernel::step_movement_gpu (crates/tpt-sim/src/move_kernel/mod.rs:1176-1227),
  * including the e == 2 pair pass between the intent and commit launches
  * (reaction_kernel/energy_entry.rs:90-393) and the drain of the photons it
@@ -217,7 +227,7 @@ TPT_API int tpt_step_movement(tpt_ctx *ctx, uint32_t seed);
 TPT_API int tpt_step_water_equal(tpt_ctx *ctx, uint32_t seed);
 
 /* SimulationEngine::step (crates/tpt-sim/src/lib.rs:303-459) for the passes
- * above, with seed = rotl(frame_count*0x9E3779B9, 13) ^ 0x85EBCA6B
+ * above (the reaction step between thermal and movement), with seed = rotl(frame_count*0x9E3779B9, 13) ^ 0x85EBCA6B
  * (lib.rs:315-318) and CheckStacking on every 10th frame (lib.rs:359). */
 TPT_API int tpt_step(tpt_ctx *ctx, uint64_t frame_count);
 

@@ -1650,7 +1650,7 @@ void oracle_step(oracle_state *s, uint64_t frame_count) {
     oracle_step_gravity(s);
     oracle_recalc_free_particles(s, seed, (frame_count % 10u) == 0);
     oracle_step_thermal(s, seed, s->heat_enable ? 0 : 1);
-    oracle_clear_grav_override(s);  /* the element pass's first step (lib.rs:379) */
+    oracle_step_reactions(s, seed);  /* the element pass (lib.rs:379), as far as it is restated */
     oracle_step_movement(s, seed);
     if (s->water_equal_test) oracle_water_equal(s, seed);
 }

@@ -124,6 +124,12 @@ void oracle_water_equal(oracle_state *s, uint32_t seed);
 uint32_t oracle_hash_u32(uint32_t x);
 uint32_t oracle_interact_wavelengths(int32_t filt_tmp, uint32_t filt_ctype, float filt_temp, uint32_t orig_wl);
 
+/* Of reaction_kernel::step_reactions_gpu (reaction_kernel/mod.rs:1123-1237):
+ * the life pass (life.rs:47-205) and, with it, the modext clear of the mass
+ * override channel.  The other reaction passes are not restated. */
+void oracle_life_pass(oracle_state *s, uint32_t seed);
+void oracle_step_reactions(oracle_state *s, uint32_t seed);
+
 /* SimulationEngine::step (lib.rs:303-459) over the passes above. */
 void oracle_step(oracle_state *s, uint64_t frame_count);
 

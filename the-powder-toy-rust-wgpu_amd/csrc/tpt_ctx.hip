@@ -20,7 +20,7 @@ const char *const tpt_pass_names[PASS_COUNT] = {
     "thermal_conduction", "thermal_conduction_cells", "thermal_apply", "phase_transition",
     "phase_transition_cells", "refresh_pmap_types", "air_deposit", "air_deposit_cells", "move_detect",
     "move_prepare", "move_velocity_intent", "move_saw_pair", "move_commit", "move_photon_hits_drain",
-    "move_rebuild_maps", "water_equal",
+    "move_rebuild_maps", "water_equal", "reaction_life",
 };
 
 static thread_local char g_err[512];

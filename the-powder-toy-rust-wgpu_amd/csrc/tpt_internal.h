@@ -62,6 +62,7 @@ enum tpt_pass {
     PASS_MOVE_DRAIN,      // blocked-photon hits + creations queued by the pair pass
     PASS_MOVE_REBUILD,    // pmap/photons rebuild
     PASS_WATER_EQUAL,     // water equalisation (when enabled)
+    PASS_REACTION_LIFE,   // reaction life pass (burn-out, spark settling, life kill)
     PASS_COUNT
 };
 
@@ -234,6 +235,9 @@ int tpt_gravity_free(tpt_ctx *ctx);
 // Zeroes the element mass-override channel (value and flag grids) at the
 // point of the tick where the reference's element pass clears it.
 int tpt_gravity_override_clear(tpt_ctx *ctx);
+// The reaction step as far as this library runs it: the override clear and
+// the life pass.
+int tpt_reactions_step(tpt_ctx *ctx);
 int tpt_recalc_step(tpt_ctx *ctx, int stacking);
 int tpt_thermal_step(tpt_ctx *ctx, int legacy);
 int tpt_movement_step(tpt_ctx *ctx);

@@ -56,6 +56,7 @@ SIGNATURES = {
     "tpt_recalc_free_particles": ([_vp, ctypes.c_uint32, ctypes.c_int], ctypes.c_int),
     "tpt_step_thermal": ([_vp, ctypes.c_uint32, ctypes.c_int], ctypes.c_int),
     "tpt_step_water_equal": ([_vp, ctypes.c_uint32], ctypes.c_int),
+    "tpt_step_reactions": ([_vp, ctypes.c_uint32], ctypes.c_int),
     "tpt_step_movement": ([_vp, ctypes.c_uint32], ctypes.c_int),
     "tpt_step": ([_vp, ctypes.c_uint64], ctypes.c_int),
     "tpt_run": ([_vp, ctypes.c_uint64, ctypes.c_uint32], ctypes.c_int),
@@ -231,6 +232,9 @@ class Context:
 
     def step_movement(self, seed):
         _check(load().tpt_step_movement(self._h, seed))
+
+    def step_reactions(self, seed):
+        _check(load().tpt_step_reactions(self._h, seed))
 
     def step_water_equal(self, seed):
         _check(load().tpt_step_water_equal(self._h, seed))
