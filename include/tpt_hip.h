@@ -328,6 +328,11 @@ TPT_API int tpt_sur_eval_loss(tpt_sur *sur, const uint32_t *pairs, uint32_t n_pa
 /* Bound on the global L2 norm of each step's mean gradient (default 1; 0
  * turns clipping off). */
 TPT_API int tpt_sur_set_clip(tpt_sur *sur, float clip);
+/* Inference with the pools, upsample-adds and input preparation folded into
+ * the convolutions that read them (fused = 1, the default) or run as
+ * separate kernels (0); the two give identical bits.  No counterpart in
+ * the reference (the surrogate has none). */
+TPT_API int tpt_sur_set_fused(tpt_sur *sur, int fused);
 /* 8 air ticks of the context by the network.  With `check`, a prediction
  * with a non-finite value or whose energy sum(p^2 + vx^2 + vy^2) exceeds
  * energy_ratio times the input's is replaced by 8 plain ticks from the

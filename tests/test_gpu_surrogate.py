@@ -146,6 +146,29 @@ def small_world(xc=64, yc=64):
     return ctx, SU.Surrogate(xc, yc)
 
 
+@pytest.mark.parametrize("xc,yc", [(64, 64), (96, 40)])
+def test_fused_inference_matches_separate_kernels(gpu, xc, yc):
+    """The inference path with the pools, upsample-adds and input
+    preparation folded into the convolutions gives the bits of the
+    separate kernels, over three calls."""
+    ctx, sur = small_world(xc, yc)
+    sur.init_params(5)
+    desc = SU.random_scene(77, xc, yc)
+    n = xc * yc
+    got = []
+    for fused in (False, True):
+        sur.set_fused(fused)
+        sur.scene(ctx, desc)
+        for _ in range(3):
+            sur.step_air(ctx, check=False)
+        got.append(SU.air_state(ctx, n))
+    for k in got[0]:
+        assert np.array_equal(got[0][k].view(np.uint32), got[1][k].view(np.uint32)), k
+    assert np.isfinite(got[1]["pv"]).all()
+    ctx.close()
+    sur.close()
+
+
 def test_training_lowers_the_loss(gpu):
     ctx, sur = small_world()
     sur.set_scales(SU.IN_SCALES, SU.delta_scales(ctx, sur, range(900, 903)))

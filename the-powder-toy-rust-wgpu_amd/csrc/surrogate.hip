@@ -23,6 +23,8 @@
 #include "surrogate_kernels.h"
 
 #include <math.h>
+
+#include <algorithm>
 #include <new>
 #include <random>
 #include <vector>
@@ -80,6 +82,7 @@ struct tpt_sur {
     size_t part_elems = 0;
     int adam_t = 0;
     // activations (NHWC bf16)
+    bf16_t *x0lo = nullptr;         // low bf16 half of the input (x0 + x0lo = the fp32 input)
     bf16_t *x0 = nullptr, *a1 = nullptr, *z1 = nullptr, *p1 = nullptr, *a2 = nullptr, *z2 = nullptr,
            *a2b = nullptr, *z2b = nullptr, *p2 = nullptr, *a3 = nullptr, *z3 = nullptr, *a3b = nullptr,
            *z3b = nullptr, *d2a = nullptr, *zd2a = nullptr, *u2 = nullptr, *d2b = nullptr, *zd2b = nullptr,
@@ -91,7 +94,9 @@ struct tpt_sur {
            *g_a2b = nullptr, *g_a2 = nullptr, *g_p1 = nullptr, *g_a1 = nullptr;
     float *loss_acc = nullptr;      // device accumulators: loss, energy in, energy out, non-finite count
     float *gnorm = nullptr;         // device: sum of squares of the batch gradient
+    float *red = nullptr;           // device: per-block partial sums (up to 4 per block)
     float clip = 1.0f;              // global gradient-norm clip (0: none)
+    bool fused = true;              // inference folds the glue kernels into the convolutions
     float in_scale[4] = {10.0f, 1.0f, 1.0f, 500.0f};   // input normalisation (state)
     float scale[4] = {1.0f, 1.0f, 1.0f, 10.0f};        // output normalisation (8-tick change)
     float ambient = 295.15f;
@@ -125,19 +130,66 @@ template <typename T> int alloc_n(tpt_sur *s, T **p, size_t n) { return alloc(s,
 
 dim3 grid1(size_t n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
 
-template <int NT, int MS> void set_conv_lds(int bytes) {
-    (void)hipFuncSetAttribute((const void *)conv3x3<NT, MS>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+template <int NT, int KS, bool LO> void set_conv_lds(int bytes) {
+    (void)hipFuncSetAttribute((const void *)conv3x3<NT, KS, LO>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 template <int MT> void set_wgrad_lds(int bytes) {
     (void)hipFuncSetAttribute((const void *)conv3x3_wgrad<MT>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
+struct Planes {
+    const float *pv, *vx, *vy, *hv;
+};
+
+// x0, x0lo: the 8 input channels of every cell as two bf16 halves (high
+// and the rounded remainder).
+__global__ void sur_prep(Planes p, const uint8_t *__restrict__ wall8, const uint32_t *__restrict__ wall32,
+                         const float *__restrict__ src, bf16_t *__restrict__ x0, bf16_t *__restrict__ x0lo, int n,
+                         float sp, float sv, float sh, float amb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float wall = wall8 ? (float)(wall8[i] != 0) : (float)(wall32[i] != 0u);
+    float v[8];
+    prep_channels(p.pv[i], p.vx[i], p.vy[i], p.hv[i], wall, src[i], sp, sv, sh, amb, v);
+    float hi[8], lo[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        hi[c] = bf2f(f2bf(v[c]));
+        lo[c] = v[c] - hi[c];
+    }
+    *reinterpret_cast<uint4 *>(x0 + (size_t)i * IN_CH) = pack8(hi);
+    *reinterpret_cast<uint4 *>(x0lo + (size_t)i * IN_CH) = pack8(lo);
+}
+
+// A conv's staging mode and its extra inputs (STAGE_* in the kernels).
+struct StageIn {
+    int stage = STAGE_PLAIN;
+    const bf16_t *in2 = nullptr;                                       // STAGE_UPADD
+    const float *pv = nullptr, *vx = nullptr, *vy = nullptr, *hv = nullptr;  // STAGE_PREP
+    const float *src = nullptr;
+    const uint32_t *wall = nullptr;
+};
+
 // One convolution: forward (w, bias column, optional GELU, pre-activation
 // stored) or data gradient (wt, GELU derivative of zref).
-void conv(const tpt_sur *s, hipStream_t st, const bf16_t *in, int cin, int cout, int level, const bf16_t *w, int kpad,
-          int kchunks, bf16_t *out, bf16_t *zout, const bf16_t *zref, float *out_f32, int gelu, int bias_col) {
-    ConvArgs a;
+void conv(const tpt_sur *s, hipStream_t st, const bf16_t *in, const bf16_t *in_lo, int cin, int cout, int level,
+          const bf16_t *w, int kpad, int kchunks, bf16_t *out, bf16_t *zout, const bf16_t *zref, float *out_f32,
+          int gelu, int bias_col, const StageIn &sin = StageIn()) {
+    ConvArgs a{};
     a.in = in;
+    a.in_lo = in_lo;
+    a.in2 = sin.in2;
+    a.stage = sin.stage;
+    a.pv = sin.pv;
+    a.vx = sin.vx;
+    a.vy = sin.vy;
+    a.hv = sin.hv;
+    a.src = sin.src;
+    a.wall = sin.wall;
+    a.sp = s->in_scale[0];
+    a.sv = s->in_scale[1];
+    a.sh = s->in_scale[3];
+    a.amb = s->ambient;
     a.w = w;
     a.out = out;
     a.zout = zout;
@@ -152,20 +204,36 @@ void conv(const tpt_sur *s, hipStream_t st, const bf16_t *in, int cin, int cout,
     a.kchunks = kchunks;
     a.gelu = gelu;
     a.bias_col = bias_col;
-    // Four M sub-tiles per wave (a full 64-pixel row) where the input tile
-    // leaves room for two workgroups per CU, two for the 128-channel inputs.
-    const int nt = cout >= 64 ? 4 : cout >= 32 ? 2 : 1, ms = cin <= 64 ? 4 : 2;
-    const size_t lds = conv_lds_bytes(cin, ms);
-    const dim3 g((a.W + CONV_SEG - 1) / CONV_SEG, (a.H + ms - 1) / ms, (cout + 16 * nt - 1) / (16 * nt));
-    if (ms == 4) {
-        if (nt == 4) hipLaunchKernelGGL((conv3x3<4, 4>), g, dim3(CONV_THREADS), lds, st, a);
-        else if (nt == 2) hipLaunchKernelGGL((conv3x3<2, 4>), g, dim3(CONV_THREADS), lds, st, a);
-        else hipLaunchKernelGGL((conv3x3<1, 4>), g, dim3(CONV_THREADS), lds, st, a);
+    // 16 * nt output channels per wave; the K chunks split over ks waves
+    // while the grid holds fewer than 2048 waves (8 per CU) and each wave
+    // keeps at least two chunks.
+    const int nt = cout >= 64 ? 4 : cout >= 32 ? 2 : 1;
+    const int segs = (a.W + CONV_SEG - 1) / CONV_SEG, ntiles = (cout + 16 * nt - 1) / (16 * nt);
+    int ks = 1;
+    const bool lo = in_lo != nullptr || sin.stage == STAGE_PREP;
+    while (!lo && ks < 4 && segs * ((a.H + 4 / ks - 1) / (4 / ks)) * ntiles * 4 < 2048 && kchunks >= 4 * ks) ks *= 2;
+    const size_t lds = conv_lds_bytes(cin, ks, nt, lo);
+    const dim3 g(segs, (a.H + 4 / ks - 1) / (4 / ks), ntiles);
+#define TPT_CONV_LAUNCH(NT_, KS_, LO_) \
+    hipLaunchKernelGGL((conv3x3<NT_, KS_, LO_>), g, dim3(CONV_THREADS), lds, st, a)
+    if (lo) {
+        if (nt == 4) TPT_CONV_LAUNCH(4, 1, true);
+        else if (nt == 2) TPT_CONV_LAUNCH(2, 1, true);
+        else TPT_CONV_LAUNCH(1, 1, true);
+    } else if (ks == 1) {
+        if (nt == 4) TPT_CONV_LAUNCH(4, 1, false);
+        else if (nt == 2) TPT_CONV_LAUNCH(2, 1, false);
+        else TPT_CONV_LAUNCH(1, 1, false);
+    } else if (ks == 2) {
+        if (nt == 4) TPT_CONV_LAUNCH(4, 2, false);
+        else if (nt == 2) TPT_CONV_LAUNCH(2, 2, false);
+        else TPT_CONV_LAUNCH(1, 2, false);
     } else {
-        if (nt == 4) hipLaunchKernelGGL((conv3x3<4, 2>), g, dim3(CONV_THREADS), lds, st, a);
-        else if (nt == 2) hipLaunchKernelGGL((conv3x3<2, 2>), g, dim3(CONV_THREADS), lds, st, a);
-        else hipLaunchKernelGGL((conv3x3<1, 2>), g, dim3(CONV_THREADS), lds, st, a);
+        if (nt == 4) TPT_CONV_LAUNCH(4, 4, false);
+        else if (nt == 2) TPT_CONV_LAUNCH(2, 4, false);
+        else TPT_CONV_LAUNCH(1, 4, false);
     }
+#undef TPT_CONV_LAUNCH
 }
 
 // Weight gradient of layer l into s->grad (accumulated when `accumulate`).
@@ -196,36 +264,66 @@ void wgrad(tpt_sur *s, hipStream_t st, int l, const bf16_t *x, const bf16_t *dz,
 }
 
 // `z` receives the pre-activation when training needs it (null otherwise).
-void fwd_layer(tpt_sur *s, hipStream_t st, int l, const bf16_t *in, bf16_t *out, bf16_t *z, float *out_f32) {
+void fwd_layer(tpt_sur *s, hipStream_t st, int l, const bf16_t *in, bf16_t *out, bf16_t *z, float *out_f32,
+               const StageIn &sin = StageIn()) {
     const Layer &L = s->layer[l];
-    conv(s, st, in, L.cin, L.cout, L.level, L.w, L.kpad, L.kchunks, out, z, nullptr, out_f32, L.gelu, 1);
+    conv(s, st, in, l == 0 && sin.stage != STAGE_PREP ? s->x0lo : nullptr, L.cin, L.cout, L.level, L.w, L.kpad,
+         L.kchunks, out, z, nullptr, out_f32, L.gelu, 1, sin);
 }
 
 // Data gradient through layer l: dx = conv(dz, Wt), times GELU'(zref) when
 // given.
 void dgrad_layer(tpt_sur *s, hipStream_t st, int l, const bf16_t *dz, bf16_t *dx, const bf16_t *zref) {
     const Layer &L = s->layer[l];
-    conv(s, st, dz, L.cout, L.cin, L.level, L.wt, L.kpad_t, L.kchunks_t, dx, nullptr, zref, nullptr, 0, 0);
+    conv(s, st, dz, nullptr, L.cout, L.cin, L.level, L.wt, L.kpad_t, L.kchunks_t, dx, nullptr, zref, nullptr, 0, 0);
 }
 
 // The network on s->x0 into s->out; `train` keeps the pre-activations the
-// backward pass needs.
-void forward(tpt_sur *s, hipStream_t st, bool train) {
+// backward pass needs.  Inference with `s->fused` set folds the pools and
+// the upsample-adds into the staging of the convolutions that read them,
+// and, given `prep` (the air planes), the input preparation into the first
+// layer; the results are bit for bit those of the separate kernels.
+void forward(tpt_sur *s, hipStream_t st, bool train, const StageIn *prep = nullptr) {
     const int H = s->H, W = s->W;
+    if (!train && s->fused) {
+        fwd_layer(s, st, 0, s->x0, s->a1, nullptr, nullptr, prep ? *prep : StageIn());
+        StageIn pool;
+        pool.stage = STAGE_POOL;
+        fwd_layer(s, st, 1, s->a1, s->a2, nullptr, nullptr, pool);
+        fwd_layer(s, st, 2, s->a2, s->a2b, nullptr, nullptr);
+        fwd_layer(s, st, 3, s->a2b, s->a3, nullptr, nullptr, pool);
+        fwd_layer(s, st, 4, s->a3, s->a3b, nullptr, nullptr);
+        fwd_layer(s, st, 5, s->a3b, s->d2a, nullptr, nullptr);
+        StageIn up;
+        up.stage = STAGE_UPADD;
+        up.in2 = s->d2a;
+        fwd_layer(s, st, 6, s->a2b, s->d2b, nullptr, nullptr, up);
+        fwd_layer(s, st, 7, s->d2b, s->d1a, nullptr, nullptr);
+        up.in2 = s->d1a;
+        fwd_layer(s, st, 8, s->a1, s->d1b, nullptr, nullptr, up);
+        fwd_layer(s, st, 9, s->d1b, nullptr, nullptr, s->out);
+        return;
+    }
+    if (prep) {
+        const size_t n = cells_at(s, 0);
+        hipLaunchKernelGGL(sur_prep, grid1(n), dim3(256), 0, st, Planes{prep->pv, prep->vx, prep->vy, prep->hv}, nullptr,
+                           prep->wall, prep->src, s->x0, s->x0lo, (int)n, s->in_scale[0], s->in_scale[1],
+                           s->in_scale[3], s->ambient);
+    }
     const auto Z = [train](bf16_t *z) { return train ? z : nullptr; };
     fwd_layer(s, st, 0, s->x0, s->a1, Z(s->z1), nullptr);
-    hipLaunchKernelGGL(pool2, grid1(cells_at(s, 1) * 32), dim3(256), 0, st, s->a1, s->p1, H, W, 32);
+    hipLaunchKernelGGL(pool2, grid1(cells_at(s, 1) * 4), dim3(256), 0, st, s->a1, s->p1, H, W, 32);
     fwd_layer(s, st, 1, s->p1, s->a2, Z(s->z2), nullptr);
     fwd_layer(s, st, 2, s->a2, s->a2b, Z(s->z2b), nullptr);
-    hipLaunchKernelGGL(pool2, grid1(cells_at(s, 2) * 64), dim3(256), 0, st, s->a2b, s->p2, H / 2, W / 2, 64);
+    hipLaunchKernelGGL(pool2, grid1(cells_at(s, 2) * 8), dim3(256), 0, st, s->a2b, s->p2, H / 2, W / 2, 64);
     fwd_layer(s, st, 3, s->p2, s->a3, Z(s->z3), nullptr);
     fwd_layer(s, st, 4, s->a3, s->a3b, Z(s->z3b), nullptr);
     fwd_layer(s, st, 5, s->a3b, s->d2a, Z(s->zd2a), nullptr);
-    hipLaunchKernelGGL(up2_add, grid1(cells_at(s, 1) * 64), dim3(256), 0, st, s->d2a, s->a2b, s->u2, H / 2, W / 2,
+    hipLaunchKernelGGL(up2_add, grid1(cells_at(s, 1) * 8), dim3(256), 0, st, s->d2a, s->a2b, s->u2, H / 2, W / 2,
                        64);
     fwd_layer(s, st, 6, s->u2, s->d2b, Z(s->zd2b), nullptr);
     fwd_layer(s, st, 7, s->d2b, s->d1a, Z(s->zd1a), nullptr);
-    hipLaunchKernelGGL(up2_add, grid1(cells_at(s, 0) * 32), dim3(256), 0, st, s->d1a, s->a1, s->u1, H, W, 32);
+    hipLaunchKernelGGL(up2_add, grid1(cells_at(s, 0) * 4), dim3(256), 0, st, s->d1a, s->a1, s->u1, H, W, 32);
     fwd_layer(s, st, 8, s->u1, s->d1b, Z(s->zd1b), nullptr);
     fwd_layer(s, st, 9, s->d1b, nullptr, nullptr, s->out);
 }
@@ -238,13 +336,13 @@ void backward(tpt_sur *s, hipStream_t st, int accumulate) {
     dgrad_layer(s, st, 9, s->g_out, s->g_d1b, s->zd1b);
     wgrad(s, st, 8, s->u1, s->g_d1b, accumulate);
     dgrad_layer(s, st, 8, s->g_d1b, s->g_u1, nullptr);
-    hipLaunchKernelGGL(up2_bwd_gelu, grid1(cells_at(s, 1) * 32), dim3(256), 0, st, s->g_u1, s->zd1a, s->g_d1a, H, W,
+    hipLaunchKernelGGL(up2_bwd_gelu, grid1(cells_at(s, 1) * 4), dim3(256), 0, st, s->g_u1, s->zd1a, s->g_d1a, H, W,
                        32);
     wgrad(s, st, 7, s->d2b, s->g_d1a, accumulate);
     dgrad_layer(s, st, 7, s->g_d1a, s->g_d2b, s->zd2b);
     wgrad(s, st, 6, s->u2, s->g_d2b, accumulate);
     dgrad_layer(s, st, 6, s->g_d2b, s->g_u2, nullptr);
-    hipLaunchKernelGGL(up2_bwd_gelu, grid1(cells_at(s, 2) * 64), dim3(256), 0, st, s->g_u2, s->zd2a, s->g_d2a,
+    hipLaunchKernelGGL(up2_bwd_gelu, grid1(cells_at(s, 2) * 8), dim3(256), 0, st, s->g_u2, s->zd2a, s->g_d2a,
                        H / 2, W / 2, 64);
     wgrad(s, st, 5, s->a3b, s->g_d2a, accumulate);
     dgrad_layer(s, st, 5, s->g_d2a, s->g_a3b, s->z3b);
@@ -252,13 +350,13 @@ void backward(tpt_sur *s, hipStream_t st, int accumulate) {
     dgrad_layer(s, st, 4, s->g_a3b, s->g_a3, s->z3);
     wgrad(s, st, 3, s->p2, s->g_a3, accumulate);
     dgrad_layer(s, st, 3, s->g_a3, s->g_p2, nullptr);
-    hipLaunchKernelGGL(pool2_bwd_add_gelu, grid1(cells_at(s, 1) * 64), dim3(256), 0, st, s->g_p2, s->g_u2, s->z2b,
+    hipLaunchKernelGGL(pool2_bwd_add_gelu, grid1(cells_at(s, 1) * 8), dim3(256), 0, st, s->g_p2, s->g_u2, s->z2b,
                        s->g_a2b, H / 2, W / 2, 64);
     wgrad(s, st, 2, s->a2, s->g_a2b, accumulate);
     dgrad_layer(s, st, 2, s->g_a2b, s->g_a2, s->z2);
     wgrad(s, st, 1, s->p1, s->g_a2, accumulate);
     dgrad_layer(s, st, 1, s->g_a2, s->g_p1, nullptr);
-    hipLaunchKernelGGL(pool2_bwd_add_gelu, grid1(cells_at(s, 0) * 32), dim3(256), 0, st, s->g_p1, s->g_u1, s->z1,
+    hipLaunchKernelGGL(pool2_bwd_add_gelu, grid1(cells_at(s, 0) * 4), dim3(256), 0, st, s->g_p1, s->g_u1, s->z1,
                        s->g_a1, H, W, 32);
     wgrad(s, st, 0, s->x0, s->g_a1, accumulate);
 }
@@ -274,33 +372,10 @@ void refresh_transposes(tpt_sur *s, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // Air-grid glue.
 
-struct Planes {
-    const float *pv, *vx, *vy, *hv;
-};
-
-// x0: the 8 input channels of every cell.
-__global__ void sur_prep(Planes p, const uint8_t *__restrict__ wall8, const uint32_t *__restrict__ wall32,
-                         const float *__restrict__ src, bf16_t *__restrict__ x0, int n, float sp, float sv, float sh,
-                         float amb) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float wall = wall8 ? (float)(wall8[i] != 0) : (float)(wall32[i] != 0u);
-    const float t = src[i];
-    bf16_t *o = x0 + (size_t)i * IN_CH;
-    o[0] = f2bf(p.pv[i] / sp);
-    o[1] = f2bf(p.vx[i] / sv);
-    o[2] = f2bf(p.vy[i] / sv);
-    o[3] = f2bf((p.hv[i] - amb) / sh);
-    o[4] = f2bf(wall);
-    o[5] = f2bf(t > 0.0f ? (t - amb) / sh : 0.0f);
-    o[6] = 0;
-    o[7] = 0;
-}
-
 // Loss and its gradient against the next snapshot: target channel c =
 // (next_c - cur_c) / scale_c; loss = mean squared error over 4 channels.
 __global__ void sur_loss(const float *__restrict__ out, Planes cur, Planes nxt, bf16_t *__restrict__ g_out,
-                         float *__restrict__ acc, int n, float sp, float sv, float sh) {
+                         float *__restrict__ red, int n, float sp, float sv, float sh) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     float l = 0.0f;
     if (i < n) {
@@ -317,16 +392,15 @@ __global__ void sur_loss(const float *__restrict__ out, Planes cur, Planes nxt, 
 #pragma unroll
         for (int c = 4; c < OUT_CH; ++c) g[c] = 0;
     }
-    // wave sum, one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(acc, l / (4.0f * (float)n));
+    float v[1] = {l / (4.0f * (float)n)};
+    block_sum<1>(v, red);
 }
 
 // state += prediction * scale; heat sources pinned; the plain step's clamps;
 // energy sums before/after and a non-finite count for the check.
 __global__ void sur_apply(const float *__restrict__ out, float *__restrict__ pv, float *__restrict__ vx,
                           float *__restrict__ vy, float *__restrict__ hv, const float *__restrict__ src,
-                          float *__restrict__ acc, int n, float sp, float sv, float sh) {
+                          float *__restrict__ red, int n, float sp, float sv, float sh) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     float e_in = 0.0f, e_out = 0.0f, bad = 0.0f;
     if (i < n) {
@@ -346,21 +420,20 @@ __global__ void sur_apply(const float *__restrict__ out, float *__restrict__ pv,
         vy[i] = v;
         hv[i] = h;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        e_in += __shfl_down(e_in, off, 64);
-        e_out += __shfl_down(e_out, off, 64);
-        bad += __shfl_down(bad, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(acc + 1, e_in);
-        atomicAdd(acc + 2, e_out);
-        if (bad != 0.0f) atomicAdd(acc + 3, bad);
+    if (red) {
+        float v[3] = {e_in, e_out, bad};
+        block_sum<3>(v, red);
     }
 }
 
 __global__ void sur_pin(float *__restrict__ hv, const float *__restrict__ src, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && src[i] > 0.0f) hv[i] = src[i];
+}
+
+__global__ void sur_zero_u16(uint16_t *__restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 __global__ void sur_zero_f(float *__restrict__ p, size_t n) {
@@ -509,7 +582,7 @@ extern "C" int tpt_sur_create(uint32_t xcells, uint32_t ycells, tpt_sur **out) {
         bf16_t **p;
         size_t n;
     } acts[] = {
-        {&s->x0, n0 * IN_CH}, {&s->a1, n0 * 32}, {&s->z1, n0 * 32}, {&s->p1, n1 * 32}, {&s->a2, n1 * 64},
+        {&s->x0, n0 * IN_CH}, {&s->x0lo, n0 * IN_CH}, {&s->a1, n0 * 32}, {&s->z1, n0 * 32}, {&s->p1, n1 * 32}, {&s->a2, n1 * 64},
         {&s->z2, n1 * 64}, {&s->a2b, n1 * 64}, {&s->z2b, n1 * 64}, {&s->p2, n2 * 64}, {&s->a3, n2 * 128},
         {&s->z3, n2 * 128}, {&s->a3b, n2 * 128}, {&s->z3b, n2 * 128}, {&s->d2a, n2 * 64}, {&s->zd2a, n2 * 64},
         {&s->u2, n1 * 64}, {&s->d2b, n1 * 64}, {&s->zd2b, n1 * 64}, {&s->d1a, n1 * 32}, {&s->zd1a, n1 * 32},
@@ -523,6 +596,7 @@ extern "C" int tpt_sur_create(uint32_t xcells, uint32_t ycells, tpt_sur **out) {
     if (!rc) rc = alloc_n(s, &s->out, n0 * OUT_CH);
     if (!rc) rc = alloc_n(s, &s->loss_acc, 4);
     if (!rc) rc = alloc_n(s, &s->gnorm, 1);
+    if (!rc) rc = alloc_n(s, &s->red, 4 * std::max<size_t>(grid1(n0).x, grid1(s->n_params).x));
     if (!rc) rc = alloc_n(s, &s->src, n0);
     if (!rc) rc = alloc_n(s, &s->saved, 4 * n0);
     if (!rc && hipStreamCreate(&s->stream) != hipSuccess) rc = tpt_set_error(TPT_EDEVICE, "surrogate stream");
@@ -537,12 +611,18 @@ extern "C" int tpt_sur_create(uint32_t xcells, uint32_t ycells, tpt_sur **out) {
         L.wt = s->wtb + wt_off;
         wt_off += (size_t)L.rows_t * L.kpad_t;
     }
-    set_conv_lds<1, 4>((int)conv_lds_bytes(64, 4));
-    set_conv_lds<2, 4>((int)conv_lds_bytes(64, 4));
-    set_conv_lds<4, 4>((int)conv_lds_bytes(64, 4));
-    set_conv_lds<1, 2>((int)conv_lds_bytes(128, 2));
-    set_conv_lds<2, 2>((int)conv_lds_bytes(128, 2));
-    set_conv_lds<4, 2>((int)conv_lds_bytes(128, 2));
+    set_conv_lds<1, 1, false>((int)conv_lds_bytes(128, 1, 1));
+    set_conv_lds<2, 1, false>((int)conv_lds_bytes(128, 1, 2));
+    set_conv_lds<4, 1, false>((int)conv_lds_bytes(128, 1, 4));
+    set_conv_lds<1, 2, false>((int)conv_lds_bytes(128, 2, 1));
+    set_conv_lds<2, 2, false>((int)conv_lds_bytes(128, 2, 2));
+    set_conv_lds<4, 2, false>((int)conv_lds_bytes(128, 2, 4));
+    set_conv_lds<1, 4, false>((int)conv_lds_bytes(128, 4, 1));
+    set_conv_lds<2, 4, false>((int)conv_lds_bytes(128, 4, 2));
+    set_conv_lds<4, 4, false>((int)conv_lds_bytes(128, 4, 4));
+    set_conv_lds<1, 1, true>((int)conv_lds_bytes(IN_CH, 1, 1, true));
+    set_conv_lds<2, 1, true>((int)conv_lds_bytes(IN_CH, 1, 2, true));
+    set_conv_lds<4, 1, true>((int)conv_lds_bytes(IN_CH, 1, 4, true));
     set_wgrad_lds<1>((int)wgrad_lds_bytes(128, 128));
     set_wgrad_lds<2>((int)wgrad_lds_bytes(128, 128));
     set_wgrad_lds<4>((int)wgrad_lds_bytes(128, 128));
@@ -630,13 +710,15 @@ extern "C" int tpt_sur_set_scales(tpt_sur *s, const float in_scales[4], const fl
     return TPT_OK;
 }
 
-// Raw forward / backward on host buffers (tests): x0 is H x W x 8 bf16 bits,
+// Raw forward / backward on host buffers (tests): x0 is H x W x 8 bf16 bits
+// (an input that bf16 holds exactly: the low half is zero),
 // out H x W x 8 fp32; g_out (H x W x 8 bf16 bits) seeds the backward and the
 // weight gradients replace s->grad.
 extern "C" int tpt_sur_forward_raw(tpt_sur *s, const uint16_t *x0, float *out) {
     if (!s || !x0 || !out) return tpt_set_error(TPT_EINVAL, "null argument");
     const size_t n = cells_at(s, 0);
     TPT_HIP_CHECK(hipMemcpyAsync(s->x0, x0, n * IN_CH * 2, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(sur_zero_u16, grid1(n * IN_CH), dim3(256), 0, s->stream, s->x0lo, n * IN_CH);
     forward(s, s->stream, true);
     TPT_HIP_CHECK(hipMemcpyAsync(out, s->out, n * OUT_CH * 4, hipMemcpyDeviceToHost, s->stream));
     TPT_HIP_CHECK(hipStreamSynchronize(s->stream));
@@ -724,17 +806,20 @@ extern "C" int tpt_sur_train(tpt_sur *s, const uint32_t *pairs, uint32_t n_pairs
         const uint32_t tr = pairs[2 * i], sn = pairs[2 * i + 1];
         const Planes cur = snap_planes(s, tr, sn), nxt = snap_planes(s, tr, sn + 1);
         hipLaunchKernelGGL(sur_prep, grid1(n), dim3(256), 0, st, cur, s->walls + (size_t)tr * n, nullptr,
-                           s->srcs + (size_t)tr * n, s->x0, n, s->in_scale[0], s->in_scale[1], s->in_scale[3], s->ambient);
+                           s->srcs + (size_t)tr * n, s->x0, s->x0lo, n, s->in_scale[0], s->in_scale[1], s->in_scale[3], s->ambient);
         forward(s, st, true);
-        hipLaunchKernelGGL(sur_loss, grid1(n), dim3(256), 0, st, s->out, cur, nxt, s->g_out, s->loss_acc, n,
+        hipLaunchKernelGGL(sur_loss, grid1(n), dim3(256), 0, st, s->out, cur, nxt, s->g_out, s->red, n,
                            s->scale[0], s->scale[1], s->scale[3]);
+        hipLaunchKernelGGL(sum_blocks, dim3(1), dim3(256), 0, st, s->red, (int)grid1(n).x, 1, s->loss_acc);
         backward(s, st, in_batch > 0 ? 1 : 0);
         if (++in_batch == accum || i + 1 == n_pairs) {
             s->adam_t += 1;
             const float b1 = 0.9f, b2 = 0.999f;
             const float c1 = 1.0f - powf(b1, (float)s->adam_t), c2 = 1.0f - powf(b2, (float)s->adam_t);
             hipLaunchKernelGGL(sur_zero_f, dim3(1), dim3(64), 0, st, s->gnorm, 1);
-            hipLaunchKernelGGL(grad_sumsq, grid1(s->n_params), dim3(256), 0, st, s->grad, (int)s->n_params, s->gnorm);
+            hipLaunchKernelGGL(grad_sumsq, grid1(s->n_params), dim3(256), 0, st, s->grad, (int)s->n_params, s->red);
+            hipLaunchKernelGGL(sum_blocks, dim3(1), dim3(256), 0, st, s->red, (int)grid1(s->n_params).x, 1,
+                               s->gnorm);
             hipLaunchKernelGGL(adam_step, grid1(s->n_params), dim3(256), 0, st, s->wf, s->wb, s->grad, s->m1, s->m2,
                                (int)s->n_params, lr, b1, b2, 1e-8f, c1, c2, 1.0f / (float)in_batch, s->gnorm,
                                s->clip);
@@ -756,6 +841,14 @@ extern "C" int tpt_sur_set_clip(tpt_sur *s, float clip) {
     return TPT_OK;
 }
 
+// Inference with the glue kernels folded into the convolutions (1, the
+// default) or as separate kernels (0); both give the same bits.
+extern "C" int tpt_sur_set_fused(tpt_sur *s, int fused) {
+    if (!s) return tpt_set_error(TPT_EINVAL, "null surrogate");
+    s->fused = fused != 0;
+    return TPT_OK;
+}
+
 // Loss of the current parameters on dataset pairs, without training.
 extern "C" int tpt_sur_eval_loss(tpt_sur *s, const uint32_t *pairs, uint32_t n_pairs, float *mean_loss) {
     if (!s || !pairs || !n_pairs || !mean_loss) return tpt_set_error(TPT_EINVAL, "bad arguments");
@@ -768,10 +861,11 @@ extern "C" int tpt_sur_eval_loss(tpt_sur *s, const uint32_t *pairs, uint32_t n_p
         if (tr >= s->n_traj || sn + 1 >= s->n_snap) return tpt_set_error(TPT_EINVAL, "pair %u out of range", i);
         const Planes cur = snap_planes(s, tr, sn), nxt = snap_planes(s, tr, sn + 1);
         hipLaunchKernelGGL(sur_prep, grid1(n), dim3(256), 0, st, cur, s->walls + (size_t)tr * n, nullptr,
-                           s->srcs + (size_t)tr * n, s->x0, n, s->in_scale[0], s->in_scale[1], s->in_scale[3], s->ambient);
+                           s->srcs + (size_t)tr * n, s->x0, s->x0lo, n, s->in_scale[0], s->in_scale[1], s->in_scale[3], s->ambient);
         forward(s, st, false);
-        hipLaunchKernelGGL(sur_loss, grid1(n), dim3(256), 0, st, s->out, cur, nxt, s->g_out, s->loss_acc, n,
+        hipLaunchKernelGGL(sur_loss, grid1(n), dim3(256), 0, st, s->out, cur, nxt, s->g_out, s->red, n,
                            s->scale[0], s->scale[1], s->scale[3]);
+        hipLaunchKernelGGL(sum_blocks, dim3(1), dim3(256), 0, st, s->red, (int)grid1(n).x, 1, s->loss_acc);
     }
     float acc[4];
     TPT_HIP_CHECK(hipMemcpyAsync(acc, s->loss_acc, sizeof acc, hipMemcpyDeviceToHost, st));
@@ -795,13 +889,20 @@ extern "C" int tpt_sur_step_air(tpt_sur *s, tpt_ctx *ctx, int check, float energ
         hipLaunchKernelGGL(sur_capture, grid1(n), dim3(256), 0, st, p, s->saved, n);
         hipLaunchKernelGGL(sur_zero_f, dim3(1), dim3(64), 0, st, s->loss_acc, 4);
     }
-    hipLaunchKernelGGL(sur_prep, grid1(n), dim3(256), 0, st, p, nullptr, tpt_buf<uint32_t>(ctx, TPT_BUF_BLOCK_AIR),
-                       s->src, s->x0, n, s->in_scale[0], s->in_scale[1], s->in_scale[3], s->ambient);
-    forward(s, st, false);
+    StageIn prep;
+    prep.stage = STAGE_PREP;
+    prep.pv = p.pv;
+    prep.vx = p.vx;
+    prep.vy = p.vy;
+    prep.hv = p.hv;
+    prep.src = s->src;
+    prep.wall = tpt_buf<uint32_t>(ctx, TPT_BUF_BLOCK_AIR);
+    forward(s, st, false, &prep);
     hipLaunchKernelGGL(sur_apply, grid1(n), dim3(256), 0, st, s->out, tpt_buf<float>(ctx, TPT_BUF_AIR_PV),
                        tpt_buf<float>(ctx, TPT_BUF_AIR_VX), tpt_buf<float>(ctx, TPT_BUF_AIR_VY),
-                       tpt_buf<float>(ctx, TPT_BUF_AIR_HV), s->src, s->loss_acc, n, s->scale[0], s->scale[1],
-                       s->scale[3]);
+                       tpt_buf<float>(ctx, TPT_BUF_AIR_HV), s->src, check ? s->red : nullptr, n, s->scale[0],
+                       s->scale[1], s->scale[3]);
+    if (check) hipLaunchKernelGGL(sum_blocks, dim3(1), dim3(256), 0, st, s->red, (int)grid1(n).x, 3, s->loss_acc + 1);
     if ((rc = tpt_check_launch("surrogate step"))) return rc;
     ctx->maps_fresh = ctx->rec_cover = false;
     int fb = 0;

@@ -12,7 +12,8 @@
 //  * the data gradient runs the same convolution with W transposed and
 //    flipped (Wt[ci][tap' * cout + co] = W[co][(8 - tap') * cin + ci]).
 //
-// bf16 values are stored as uint16_t and converted with round-to-nearest-even.
+// bf16 values are stored as uint16_t and converted with round-to-nearest-even
+// (gfx950's v_cvt_pk_bf16_f32).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,44 +25,102 @@ typedef uint16_t bf16_t;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ bf16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7F800000u) == 0x7F800000u) return (bf16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (bf16_t)(u >> 16);
+__device__ __forceinline__ bf16_t f2bf(float f) { return __builtin_bit_cast(bf16_t, (__bf16)f); }
+// Two floats to packed bf16 (low half first), one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t f2bf2(float lo, float hi) {
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    typedef float f32x2_t __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t{lo, hi}), bf16x2_t));
 }
 __device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// GELU, tanh form: 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))).
-__device__ __forceinline__ float gelu_t(float z) {
-    const float u = 0.7978845608f * (z + 0.044715f * z * z * z);
-    return tanhf(u);
+// GELU, tanh form: 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))), evaluated
+// as z * sigmoid(2 sqrt(2/pi) (z + 0.044715 z^3)) (0.5 (1 + tanh u) =
+// sigmoid(2 u)) with the hardware exponential and reciprocal.
+__device__ __forceinline__ float gelu_s(float z) {
+    const float u = 1.5957691216f * (z + 0.044715f * z * z * z);
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-u));
 }
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + gelu_t(z)); }
+__device__ __forceinline__ float gelu(float z) { return z * gelu_s(z); }
+// d/dz: s + z s (1 - s) * 2 sqrt(2/pi) (1 + 3 * 0.044715 z^2).
 __device__ __forceinline__ float gelu_grad(float z) {
-    const float t = gelu_t(z);
-    const float du = 0.7978845608f * (1.0f + 3.0f * 0.044715f * z * z);
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * du;
+    const float s = gelu_s(z);
+    return s + z * s * (1.0f - s) * (1.5957691216f * (1.0f + 3.0f * 0.044715f * z * z));
 }
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
+// 8 bf16 <-> 8 floats, 16-byte vectors.
+__device__ __forceinline__ void unpack8(uint4 v, float (&f)[8]) {
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        f[2 * i] = __uint_as_float(w[i] << 16);
+        f[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float (&f)[8]) {
+    uint32_t w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = f2bf2(f[2 * i], f[2 * i + 1]);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+__device__ __forceinline__ uint4 ld8(const bf16_t *p) { return *reinterpret_cast<const uint4 *>(p); }
+
 // ---------------------------------------------------------------------------
-// Convolution: 256 threads (4 waves).  Each wave owns 16 * MS consecutive
-// pixels of one row (MS 16-pixel M sub-tiles) and 16 * NT output channels
-// (NT 16-channel N sub-tiles), so a workgroup covers MS rows of a 64-pixel
-// segment.  The input tile (rows y0-1 .. y0+MS, columns x0-1 .. x0+64)
-// sits in LDS with a pixel stride of cin + 8 bf16, which spreads the 16
-// pixels of a fragment read over the banks.  The K loop fetches the next
-// chunk's fragments (LDS patches, weight rows from L2) before it issues the
-// current chunk's MFMAs.
+// Convolution: 256 threads (4 waves).  The MFMA's A operand is the weight
+// matrix (rows = output channels), its B operand the input patches (columns
+// = pixels), so that a lane's accumulator holds 4 consecutive output
+// channels of one pixel and leaves as one 8-byte store.  Each wave owns the
+// 64 pixels of one row segment (4 sub-tiles of 16 pixels) and 16 * NT output
+// channels (NT sub-tiles of 16).  KS waves share one pixel tile and split its
+// K chunks (split K for the small grids of the coarse levels); their partial
+// sums meet in LDS and each of them finishes a quarter or half of the tile.
+// A workgroup covers 4 / KS rows of a 64-pixel segment.  The input tile
+// (rows y0-1 .. y0+4/KS, columns x0-1 .. x0+64) sits in LDS with a pixel
+// stride of cin + 8 bf16, which spreads the 16 pixels of a fragment read
+// over the banks.  The K loop is unrolled by two with separate fragment
+// registers, so the loads of one chunk are in flight while the other
+// chunk's MFMAs issue.
 
 constexpr int CONV_THREADS = 256;
 constexpr int CONV_SEG = 64;
 constexpr int CONV_LCOLS = CONV_SEG + 2;
 
+// How a conv stages its input tile:
+//  STAGE_PLAIN  in is H x W x cin;
+//  STAGE_POOL   in is 2H x 2W x cin and the tile holds its 2x2 average
+//               (pool2's arithmetic and rounding);
+//  STAGE_UPADD  in is the skip (H x W x cin), in2 the low level
+//               (H/2 x W/2 x cin); the tile holds up2_add's sum;
+//  STAGE_PREP   (LO kernels, cin 8) the tile holds the network's input
+//               built from the air planes (prep_channels), in two bf16 halves.
+enum { STAGE_PLAIN = 0, STAGE_POOL = 1, STAGE_UPADD = 2, STAGE_PREP = 3 };
+
+// The 8 input channels of one cell: pressure, velocity, heat above ambient
+// (each over its scale), wall mask, heat-source temperature above ambient
+// over the heat scale, two zeros.
+__device__ __forceinline__ void prep_channels(float pv, float vx, float vy, float hv, float wall, float src,
+                                              float sp, float sv, float sh, float amb, float (&v)[8]) {
+    v[0] = pv / sp;
+    v[1] = vx / sv;
+    v[2] = vy / sv;
+    v[3] = (hv - amb) / sh;
+    v[4] = wall;
+    v[5] = src > 0.0f ? (src - amb) / sh : 0.0f;
+    v[6] = 0.0f;
+    v[7] = 0.0f;
+}
+
 struct ConvArgs {
-    const bf16_t *in;     // H x W x cin
+    const bf16_t *in;     // H x W x cin (see the staging modes)
+    const bf16_t *in_lo;  // LO kernels: the low bf16 half of the input (in + in_lo = the fp32 value)
+    const bf16_t *in2;    // STAGE_UPADD: the low-level input
+    int stage;            // STAGE_*
+    // STAGE_PREP: the air planes, wall word per cell, source temperatures, scales
+    const float *pv, *vx, *vy, *hv, *src;
+    const uint32_t *wall;
+    float sp, sv, sh, amb;
     const bf16_t *w;      // cout rows (padded to 64) x kpad
     bf16_t *out;          // H x W x cout, bf16 (may be null when out_f32 is set)
     bf16_t *zout;         // forward: pre-activation, or null
@@ -74,96 +133,253 @@ struct ConvArgs {
     int bias_col;         // the k = 9 * cin column meets a constant 1
 };
 
-__host__ __device__ inline size_t conv_lds_bytes(int cin, int ms) {
-    return (size_t)(ms + 2) * CONV_LCOLS * (cin + 8) * sizeof(bf16_t);
+__host__ __device__ inline size_t conv_lds_bytes(int cin, int ks, int nt, bool lo = false) {
+    const size_t tile = (size_t)(lo ? 2 : 1) * (4 / ks + 2) * CONV_LCOLS * (cin + 8) * sizeof(bf16_t);
+    const size_t part = ks > 1 ? (size_t)4 * 4 * nt * 64 * sizeof(f32x4) : 0;
+    return tile > part ? tile : part;
 }
 
-template <int NT, int MS>
+template <int NT>
 __device__ __forceinline__ void conv_fetch(const bf16_t *tile, const bf16_t *const *wrow, const ConvArgs &a, int kc,
-                                           int g, int m, int row, int px0, int ps, uint4 (&av)[MS],
-                                           uint4 (&bv)[NT]) {
+                                           int g, int m, int row, int ps, int bias, uint4 (&xv)[4],
+                                           uint4 (&wv)[NT]) {
     const int k0 = 32 * kc + 8 * g, tap = k0 >> a.lcin, ci0 = k0 & (a.cin - 1);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) wv[t] = *reinterpret_cast<const uint4 *>(wrow[t] + k0);
     if (tap < 9) {
         const int dy = tap / 3, dx = tap - 3 * dy;
-        const bf16_t *base = tile + ((row + dy) * CONV_LCOLS + px0 + m + dx) * ps + ci0;
+        const bf16_t *base = tile + ((row + dy) * CONV_LCOLS + m + dx) * ps + ci0;
 #pragma unroll
-        for (int j = 0; j < MS; ++j) av[j] = *reinterpret_cast<const uint4 *>(base + 16 * j * ps);
+        for (int j = 0; j < 4; ++j) xv[j] = *reinterpret_cast<const uint4 *>(base + 16 * j * ps);
     } else {
-        const uint4 z = (tap == 9 && ci0 == 0 && a.bias_col) ? make_uint4(0x3F80u, 0u, 0u, 0u)  // bf16 1.0
+        const uint4 z = (tap == 9 && ci0 == 0 && bias) ? make_uint4(0x3F80u, 0u, 0u, 0u)  // bf16 1.0
                                                              : make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-        for (int j = 0; j < MS; ++j) av[j] = z;
+        for (int j = 0; j < 4; ++j) xv[j] = z;
     }
-#pragma unroll
-    for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const uint4 *>(wrow[t] + k0);
 }
 
-template <int NT, int MS>
+template <int NT>
+__device__ __forceinline__ void conv_mfma(f32x4 (&acc)[4][NT], const uint4 (&xv)[4], const uint4 (&wv)[NT]) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const bf16x8 w = as_bf16x8(wv[t]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            acc[j][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, as_bf16x8(xv[j]), acc[j][t], 0, 0, 0);
+    }
+}
+
+// Output channels co .. co + 3 of pixel (x, y): the pre-activation (zout),
+// GELU or the data gradient's GELU derivative, the fp32 and bf16 outputs.
+__device__ __forceinline__ void conv_store(const ConvArgs &a, f32x4 v, int x, int y, int co) {
+    if (co >= a.cout || x >= a.W || y >= a.H) return;
+    const size_t o = ((size_t)y * a.W + x) * a.cout + co;
+    if (a.zout) {
+        *reinterpret_cast<uint2 *>(a.zout + o) = make_uint2(f2bf2(v[0], v[1]), f2bf2(v[2], v[3]));
+    }
+    if (a.gelu) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = gelu(v[i]);
+    }
+    if (a.zref) {
+        const uint2 zr = *reinterpret_cast<const uint2 *>(a.zref + o);
+        v[0] *= gelu_grad(bf2f((bf16_t)(zr.x & 0xFFFFu)));
+        v[1] *= gelu_grad(bf2f((bf16_t)(zr.x >> 16)));
+        v[2] *= gelu_grad(bf2f((bf16_t)(zr.y & 0xFFFFu)));
+        v[3] *= gelu_grad(bf2f((bf16_t)(zr.y >> 16)));
+    }
+    if (a.out_f32) *reinterpret_cast<float4 *>(a.out_f32 + o) = make_float4(v[0], v[1], v[2], v[3]);
+    if (a.out) *reinterpret_cast<uint2 *>(a.out + o) = make_uint2(f2bf2(v[0], v[1]), f2bf2(v[2], v[3]));
+}
+
+// Stages the input tile (rows y0-1 .. y0+rows, columns x0-1 .. x0+64, zero
+// outside the grid; 8 channels of one pixel per element) into LDS.  Each
+// thread issues the loads of STAGE_BATCH elements before it combines and
+// stores any of them, so a tile costs a few memory latencies rather than
+// one per element.
+constexpr int STAGE_BATCH = 4;
+
+template <bool LO>
+__device__ __forceinline__ void conv_stage(const ConvArgs &a, bf16_t *tile, int tile_elems, int ps, int x0, int y0,
+                                           int rows) {
+    const int v8 = a.cin >> 3, total = (rows + 2) * CONV_LCOLS * v8;
+    for (int base = threadIdx.x; base < total; base += CONV_THREADS * STAGE_BATCH) {
+        int dst[STAGE_BATCH];
+        bool ok[STAGE_BATCH];
+        size_t src[STAGE_BATCH];
+        int gy[STAGE_BATCH], gx[STAGE_BATCH], ch[STAGE_BATCH];
+#pragma unroll
+        for (int k = 0; k < STAGE_BATCH; ++k) {
+            const int e = base + k * CONV_THREADS;
+            const int v = e % v8, rc = e / v8, c = rc % CONV_LCOLS, r = rc / CONV_LCOLS;
+            gy[k] = y0 - 1 + r;
+            gx[k] = x0 - 1 + c;
+            ch[k] = 8 * v;
+            dst[k] = e < total ? (r * CONV_LCOLS + c) * ps + 8 * v : -1;
+            ok[k] = e < total && gy[k] >= 0 && gy[k] < a.H && gx[k] >= 0 && gx[k] < a.W;
+            src[k] = ((size_t)gy[k] * a.W + gx[k]) * a.cin + 8 * v;
+        }
+        uint4 val[STAGE_BATCH], vlo[STAGE_BATCH];
+        if (a.stage == STAGE_POOL) {
+            uint4 q[STAGE_BATCH][4];
+            const size_t W2 = 2 * (size_t)a.W;
+#pragma unroll
+            for (int k = 0; k < STAGE_BATCH; ++k) {
+                const size_t b = ((2 * (size_t)gy[k]) * W2 + 2 * gx[k]) * a.cin + ch[k];
+                const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+                q[k][0] = ok[k] ? ld8(a.in + b) : z;
+                q[k][1] = ok[k] ? ld8(a.in + b + a.cin) : z;
+                q[k][2] = ok[k] ? ld8(a.in + b + W2 * a.cin) : z;
+                q[k][3] = ok[k] ? ld8(a.in + b + W2 * a.cin + a.cin) : z;
+            }
+#pragma unroll
+            for (int k = 0; k < STAGE_BATCH; ++k) {
+                float f0[8], f1[8], f2[8], f3[8], o[8];
+                unpack8(q[k][0], f0);
+                unpack8(q[k][1], f1);
+                unpack8(q[k][2], f2);
+                unpack8(q[k][3], f3);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) o[i] = 0.25f * (((f0[i] + f1[i]) + f2[i]) + f3[i]);
+                val[k] = ok[k] ? pack8(o) : make_uint4(0u, 0u, 0u, 0u);
+            }
+        } else if (a.stage == STAGE_UPADD) {
+            uint4 lq[STAGE_BATCH], kq[STAGE_BATCH];
+#pragma unroll
+            for (int k = 0; k < STAGE_BATCH; ++k) {
+                const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+                lq[k] = ok[k] ? ld8(a.in2 + ((size_t)(gy[k] / 2) * (a.W / 2) + gx[k] / 2) * a.cin + ch[k]) : z;
+                kq[k] = ok[k] ? ld8(a.in + src[k]) : z;
+            }
+#pragma unroll
+            for (int k = 0; k < STAGE_BATCH; ++k) {
+                float l[8], kk[8];
+                unpack8(lq[k], l);
+                unpack8(kq[k], kk);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) l[i] += kk[i];
+                val[k] = ok[k] ? pack8(l) : make_uint4(0u, 0u, 0u, 0u);
+            }
+        } else if (LO && a.stage == STAGE_PREP) {
+            float pv[STAGE_BATCH], vx[STAGE_BATCH], vy[STAGE_BATCH], hv[STAGE_BATCH], sr[STAGE_BATCH];
+            uint32_t wl[STAGE_BATCH];
+#pragma unroll
+            for (int k = 0; k < STAGE_BATCH; ++k) {
+                const size_t cell = ok[k] ? (size_t)gy[k] * a.W + gx[k] : 0;
+                pv[k] = a.pv[cell];
+                vx[k] = a.vx[cell];
+                vy[k] = a.vy[cell];
+                hv[k] = a.hv[cell];
+                sr[k] = a.src[cell];
+                wl[k] = a.wall[cell];
+            }
+#pragma unroll
+            for (int k = 0; k < STAGE_BATCH; ++k) {
+                float c8[8], hi[8], lo[8];
+                prep_channels(pv[k], vx[k], vy[k], hv[k], (float)(wl[k] != 0u), sr[k], a.sp, a.sv, a.sh, a.amb, c8);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    hi[i] = bf2f(f2bf(c8[i]));
+                    lo[i] = c8[i] - hi[i];
+                }
+                val[k] = ok[k] ? pack8(hi) : make_uint4(0u, 0u, 0u, 0u);
+                vlo[k] = ok[k] ? pack8(lo) : make_uint4(0u, 0u, 0u, 0u);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < STAGE_BATCH; ++k) {
+                const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+                val[k] = ok[k] ? ld8(a.in + src[k]) : z;
+                if (LO) vlo[k] = ok[k] ? ld8(a.in_lo + src[k]) : z;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < STAGE_BATCH; ++k) {
+            if (dst[k] < 0) continue;
+            *reinterpret_cast<uint4 *>(tile + dst[k]) = val[k];
+            if (LO) *reinterpret_cast<uint4 *>(tile + tile_elems + dst[k]) = vlo[k];
+        }
+    }
+}
+
+// LO: the input comes as two bf16 halves (in, in_lo) that meet the same
+// weights, so the first layer sees its fp32 input to about 16 bits.
+template <int NT, int KS, bool LO>
 __global__ __launch_bounds__(CONV_THREADS) void conv3x3(ConvArgs a) {
     extern __shared__ __align__(16) bf16_t tile[];
-    constexpr int WAVES_PER_ROW = 4 / MS;
-    const int ps = a.cin + 8;
-    const int x0 = blockIdx.x * CONV_SEG, y0 = blockIdx.y * MS, nbase = blockIdx.z * 16 * NT;
+    constexpr int ROWS = 4 / KS;
+    const int ps = a.cin + 8, tile_elems = (ROWS + 2) * CONV_LCOLS * ps;
+    const int x0 = blockIdx.x * CONV_SEG, y0 = blockIdx.y * ROWS, nbase = blockIdx.z * 16 * NT;
     const int tid = threadIdx.x;
-    // stage the input tile with a zero halo ('same' padding)
-    const int v8 = a.cin >> 3, total = (MS + 2) * CONV_LCOLS * v8;
-    for (int e = tid; e < total; e += CONV_THREADS) {
-        const int v = e % v8, rc = e / v8, c = rc % CONV_LCOLS, r = rc / CONV_LCOLS;
-        const int gy = y0 - 1 + r, gx = x0 - 1 + c;
-        uint4 val = make_uint4(0u, 0u, 0u, 0u);
-        if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
-            val = *reinterpret_cast<const uint4 *>(a.in + ((size_t)gy * a.W + gx) * a.cin + 8 * v);
-        *reinterpret_cast<uint4 *>(tile + (r * CONV_LCOLS + c) * ps + 8 * v) = val;
-    }
+    conv_stage<LO>(a, tile, tile_elems, ps, x0, y0, ROWS);
     __syncthreads();
     const int lane = tid & 63, wave = tid >> 6, m = lane & 15, g = lane >> 4;
-    const int row = wave / WAVES_PER_ROW, px0 = (wave % WAVES_PER_ROW) * 16 * MS;
-    f32x4 acc[MS][NT];
+    const int row = wave / KS, kg = wave % KS;
+    const int kbeg = kg * a.kchunks / KS, kend = (kg + 1) * a.kchunks / KS;
+    f32x4 acc[4][NT];
 #pragma unroll
-    for (int j = 0; j < MS; ++j)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[j][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const bf16_t *wrow[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) wrow[t] = a.w + (size_t)(nbase + 16 * t + m) * a.kpad;
-    uint4 av[MS], bv[NT];
-    conv_fetch<NT, MS>(tile, wrow, a, 0, g, m, row, px0, ps, av, bv);
-    for (int kc = 0; kc < a.kchunks; ++kc) {
-        uint4 an[MS], bn[NT];
-        if (kc + 1 < a.kchunks) conv_fetch<NT, MS>(tile, wrow, a, kc + 1, g, m, row, px0, ps, an, bn);
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const bf16x8 b = as_bf16x8(bv[t]);
-#pragma unroll
-            for (int j = 0; j < MS; ++j)
-                acc[j][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av[j]), b, acc[j][t], 0, 0, 0);
+    uint4 x0v[4], w0v[NT], x1v[4], w1v[NT];
+    int kc = kbeg;
+    if (LO) {
+        // short K (the first layer): fetch, then both halves through the MFMAs
+        for (; kc < kend; ++kc) {
+            conv_fetch<NT>(tile, wrow, a, kc, g, m, row, ps, a.bias_col, x0v, w0v);
+            conv_fetch<NT>(tile + tile_elems, wrow, a, kc, g, m, row, ps, 0, x1v, w1v);
+            conv_mfma<NT>(acc, x0v, w0v);
+            conv_mfma<NT>(acc, x1v, w0v);
         }
-#pragma unroll
-        for (int j = 0; j < MS; ++j) av[j] = an[j];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) bv[t] = bn[t];
+    } else {
+        if (kc < kend) conv_fetch<NT>(tile, wrow, a, kc, g, m, row, ps, a.bias_col, x0v, w0v);
+        for (; kc + 1 < kend; kc += 2) {
+            conv_fetch<NT>(tile, wrow, a, kc + 1, g, m, row, ps, a.bias_col, x1v, w1v);
+            conv_mfma<NT>(acc, x0v, w0v);
+            if (kc + 2 < kend) conv_fetch<NT>(tile, wrow, a, kc + 2, g, m, row, ps, a.bias_col, x0v, w0v);
+            conv_mfma<NT>(acc, x1v, w1v);
+        }
+        if (kc < kend) conv_mfma<NT>(acc, x0v, w0v);
     }
     const int y = y0 + row;
-    if (y >= a.H) return;
+    // lane: pixel x0 + 16 j + m, output channels nbase + 16 t + 4 g + (0..3)
+    if (KS == 1) {
 #pragma unroll
-    for (int j = 0; j < MS; ++j)
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) conv_store(a, acc[j][t], x0 + 16 * j + m, y, nbase + 16 * t + 4 * g);
+        return;
+    }
+    // split K: every wave parks its partial sums, then wave kg of a row adds
+    // the KS partials of pixel sub-tiles j = kg, kg + KS, ...
+    f32x4 *part = reinterpret_cast<f32x4 *>(tile);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) part[((wave * 4 + j) * NT + t) * 64 + lane] = acc[j][t];
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < 4 / KS; ++jj) {
+        const int j = kg + KS * jj;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const int co = nbase + 16 * t + m;
-            if (co >= a.cout) continue;
+            f32x4 s = part[(((row * KS) * 4 + j) * NT + t) * 64 + lane];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int x = x0 + px0 + 16 * j + 4 * g + i;
-                if (x >= a.W) continue;
-                const size_t o = ((size_t)y * a.W + x) * a.cout + co;
-                float v = acc[j][t][i];
-                if (a.zout) a.zout[o] = f2bf(v);
-                if (a.gelu) v = gelu(v);
-                if (a.zref) v *= gelu_grad(bf2f(a.zref[o]));
-                if (a.out_f32) a.out_f32[o] = v;
-                if (a.out) a.out[o] = f2bf(v);
+            for (int q = 1; q < KS; ++q) {
+                const f32x4 u = part[(((row * KS + q) * 4 + j) * NT + t) * 64 + lane];
+                s[0] += u[0];
+                s[1] += u[1];
+                s[2] += u[2];
+                s[3] += u[3];
             }
+            conv_store(a, s, x0 + 16 * j + m, y, nbase + 16 * t + 4 * g);
         }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -270,48 +486,73 @@ __global__ void wgrad_reduce(const float *__restrict__ part, float *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-// U-Net glue (NHWC bf16).
+// U-Net glue (NHWC bf16).  One thread per 8 channels of a pixel (C is a
+// multiple of 8), 16-byte loads and stores.
+
 
 // 2x2 average pool: out (H/2 x W/2 x C).
 __global__ void pool2(const bf16_t *__restrict__ in, bf16_t *__restrict__ out, int H, int W, int C) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x, h = H / 2, w = W / 2;
-    if (idx >= h * w * C) return;
-    const int c = idx % C, p = idx / C, x = p % w, y = p / w;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x, h = H / 2, w = W / 2, c8 = C >> 3;
+    if (idx >= h * w * c8) return;
+    const int c = 8 * (idx % c8), p = idx / c8, x = p % w, y = p / w;
     const size_t b = ((size_t)(2 * y) * W + 2 * x) * C + c;
-    const float s = bf2f(in[b]) + bf2f(in[b + C]) + bf2f(in[b + (size_t)W * C]) + bf2f(in[b + (size_t)W * C + C]);
-    out[idx] = f2bf(0.25f * s);
+    float f0[8], f1[8], f2[8], f3[8], o[8];
+    unpack8(ld8(in + b), f0);
+    unpack8(ld8(in + b + C), f1);
+    unpack8(ld8(in + b + (size_t)W * C), f2);
+    unpack8(ld8(in + b + (size_t)W * C + C), f3);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o[i] = 0.25f * (((f0[i] + f1[i]) + f2[i]) + f3[i]);
+    *reinterpret_cast<uint4 *>(out + (size_t)p * C + c) = pack8(o);
 }
 
 // out = nearest-neighbour 2x upsample of lo (H/2 x W/2 x C) + skip (H x W x C).
 __global__ void up2_add(const bf16_t *__restrict__ lo, const bf16_t *__restrict__ skip, bf16_t *__restrict__ out,
                         int H, int W, int C) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= H * W * C) return;
-    const int c = idx % C, p = idx / C, x = p % W, y = p / W;
-    out[idx] = f2bf(bf2f(lo[((size_t)(y / 2) * (W / 2) + x / 2) * C + c]) + bf2f(skip[idx]));
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x, c8 = C >> 3;
+    if (idx >= H * W * c8) return;
+    const int c = 8 * (idx % c8), p = idx / c8, x = p % W, y = p / W;
+    float l[8], k[8];
+    unpack8(ld8(lo + ((size_t)(y / 2) * (W / 2) + x / 2) * C + c), l);
+    unpack8(ld8(skip + (size_t)p * C + c), k);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) l[i] += k[i];
+    *reinterpret_cast<uint4 *>(out + (size_t)p * C + c) = pack8(l);
 }
 
 // Backward of up2_add for the low branch: out_lo = (sum over each 2x2 of
 // d_hi) * gelu'(z_lo).
 __global__ void up2_bwd_gelu(const bf16_t *__restrict__ dhi, const bf16_t *__restrict__ zlo,
                              bf16_t *__restrict__ out, int H, int W, int C) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x, h = H / 2, w = W / 2;
-    if (idx >= h * w * C) return;
-    const int c = idx % C, p = idx / C, x = p % w, y = p / w;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x, h = H / 2, w = W / 2, c8 = C >> 3;
+    if (idx >= h * w * c8) return;
+    const int c = 8 * (idx % c8), p = idx / c8, x = p % w, y = p / w;
     const size_t b = ((size_t)(2 * y) * W + 2 * x) * C + c;
-    const float s = bf2f(dhi[b]) + bf2f(dhi[b + C]) + bf2f(dhi[b + (size_t)W * C]) + bf2f(dhi[b + (size_t)W * C + C]);
-    out[idx] = f2bf(s * gelu_grad(bf2f(zlo[idx])));
+    float f0[8], f1[8], f2[8], f3[8], z[8], o[8];
+    unpack8(ld8(dhi + b), f0);
+    unpack8(ld8(dhi + b + C), f1);
+    unpack8(ld8(dhi + b + (size_t)W * C), f2);
+    unpack8(ld8(dhi + b + (size_t)W * C + C), f3);
+    unpack8(ld8(zlo + (size_t)p * C + c), z);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o[i] = (((f0[i] + f1[i]) + f2[i]) + f3[i]) * gelu_grad(z[i]);
+    *reinterpret_cast<uint4 *>(out + (size_t)p * C + c) = pack8(o);
 }
 
 // Backward of pool2 plus a skip gradient: out = (0.25 d_lo[up] + d_skip) *
 // gelu'(z), at full resolution.
 __global__ void pool2_bwd_add_gelu(const bf16_t *__restrict__ dlo, const bf16_t *__restrict__ dskip,
                                    const bf16_t *__restrict__ z, bf16_t *__restrict__ out, int H, int W, int C) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= H * W * C) return;
-    const int c = idx % C, p = idx / C, x = p % W, y = p / W;
-    const float d = 0.25f * bf2f(dlo[((size_t)(y / 2) * (W / 2) + x / 2) * C + c]) + bf2f(dskip[idx]);
-    out[idx] = f2bf(d * gelu_grad(bf2f(z[idx])));
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x, c8 = C >> 3;
+    if (idx >= H * W * c8) return;
+    const int c = 8 * (idx % c8), p = idx / c8, x = p % W, y = p / W;
+    float l[8], k[8], zz[8];
+    unpack8(ld8(dlo + ((size_t)(y / 2) * (W / 2) + x / 2) * C + c), l);
+    unpack8(ld8(dskip + (size_t)p * C + c), k);
+    unpack8(ld8(z + (size_t)p * C + c), zz);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) l[i] = (0.25f * l[i] + k[i]) * gelu_grad(zz[i]);
+    *reinterpret_cast<uint4 *>(out + (size_t)p * C + c) = pack8(l);
 }
 
 // Wt[ci][tap' * cout + co] = W[co][(8 - tap') * cin + ci] (no bias column);
@@ -329,12 +570,47 @@ __global__ void transpose_flip(const bf16_t *__restrict__ w, bf16_t *__restrict_
     wt[idx] = v;
 }
 
-// Sum of squares of the gradient vector into *acc (one atomic per wave).
-__global__ void grad_sumsq(const float *__restrict__ grad, int n, float *__restrict__ acc) {
+// Block sums of NC values per thread: block b writes red[b * NC + c]
+// (blockDim.x = 256); sum_blocks adds them up in a fixed order.
+template <int NC>
+__device__ __forceinline__ void block_sum(float (&v)[NC], float *__restrict__ red) {
+    __shared__ float wsum[4][NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+        for (int off = 32; off > 0; off >>= 1) v[c] += __shfl_down(v[c], off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) wsum[wave][c] = v[c];
+    __syncthreads();
+    if (threadIdx.x < NC)
+        red[(size_t)blockIdx.x * NC + threadIdx.x] =
+            ((wsum[0][threadIdx.x] + wsum[1][threadIdx.x]) + wsum[2][threadIdx.x]) + wsum[3][threadIdx.x];
+}
+
+// acc[c] += sum over b < nblocks of red[b * nc + c], for c < nc (one block
+// of 256 threads).
+__global__ void sum_blocks(const float *__restrict__ red, int nblocks, int nc, float *__restrict__ acc) {
+    __shared__ float part[256];
+    for (int c = 0; c < nc; ++c) {
+        float v = 0.0f;
+        for (int b = threadIdx.x; b < nblocks; b += 256) v += red[(size_t)b * nc + c];
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) acc[c] += part[0];
+        __syncthreads();
+    }
+}
+
+// Block sums of squares of the gradient vector (256 threads per block).
+__global__ void grad_sumsq(const float *__restrict__ grad, int n, float *__restrict__ red) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    float v = i < n ? grad[i] * grad[i] : 0.0f;
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(acc, v);
+    float v[1] = {i < n ? grad[i] * grad[i] : 0.0f};
+    block_sum<1>(v, red);
 }
 
 // Adam on the fp32 master parameters, then the bf16 copy.  The gradient is

@@ -156,6 +156,9 @@ class Surrogate:
     def set_clip(self, clip):
         L._check(L.load().tpt_sur_set_clip(self._h, ctypes.c_float(clip)))
 
+    def set_fused(self, fused):
+        L._check(L.load().tpt_sur_set_fused(self._h, 1 if fused else 0))
+
     def eval_loss(self, pairs):
         pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1)
         loss = ctypes.c_float()
