@@ -104,7 +104,8 @@ def lib():
                             ("oracle_commit", []), ("oracle_rebuild_maps", []),
                             ("oracle_step_movement", [ctypes.c_uint32]), ("oracle_step", [ctypes.c_uint64]),
                             ("oracle_water_equal", [ctypes.c_uint32]),
-                            ("oracle_life_pass", [ctypes.c_uint32]), ("oracle_step_reactions", [ctypes.c_uint32])):
+                            ("oracle_life_pass", [ctypes.c_uint32]), ("oracle_combustion_pass", [ctypes.c_uint32]),
+                            ("oracle_fluid_pass", [ctypes.c_uint32]), ("oracle_step_reactions", [ctypes.c_uint32])):
             f = getattr(_lib, name)
             f.argtypes = [ctypes.POINTER(OracleStateC)] + extra
             f.restype = None

@@ -124,10 +124,16 @@ void oracle_water_equal(oracle_state *s, uint32_t seed);
 uint32_t oracle_hash_u32(uint32_t x);
 uint32_t oracle_interact_wavelengths(int32_t filt_tmp, uint32_t filt_ctype, float filt_temp, uint32_t orig_wl);
 
-/* Of reaction_kernel::step_reactions_gpu (reaction_kernel/mod.rs:1123-1237):
- * the life pass (life.rs:47-205) and, with it, the modext clear of the mass
- * override channel.  The other reaction passes are not restated. */
+/* Of reaction_kernel::step_reactions_gpu (reaction_kernel/mod.rs:1123-1237),
+ * in the driver's order: the modext clear of the mass-override channel, the
+ * life pass (life.rs:47-205), the combustion pass's FIRE/PLSM heating and
+ * ignition and WTRV branches (combustion/fire.rs:26-408, 983-1013) and the
+ * fluid pass's WATR/DSTW/SLTW/ICEI/SNOW branches (fluid.rs:149-400), the
+ * last two under the canonical rule stated in reactions.c.  The other
+ * reaction passes and element branches are not restated. */
 void oracle_life_pass(oracle_state *s, uint32_t seed);
+void oracle_combustion_pass(oracle_state *s, uint32_t seed);
+void oracle_fluid_pass(oracle_state *s, uint32_t seed);
 void oracle_step_reactions(oracle_state *s, uint32_t seed);
 
 /* SimulationEngine::step (lib.rs:303-459) over the passes above. */

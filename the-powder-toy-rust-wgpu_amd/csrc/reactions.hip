@@ -1,23 +1,68 @@
 // The part of reaction_kernel::step_reactions_gpu
-// (crates/tpt-sim/src/reaction_kernel/mod.rs:1123-1237) this library runs:
-// the modext clear of the gravity mass-override channel
-// (reaction_kernel/modext/mod.rs:53-99) and the life pass
-// (life_and_local_reaction_kernel, reaction_kernel/life.rs:47-205): fire,
-// plasma and TRON burn-in lifetimes, fire burning out to smoke or steam,
-// plasma to noble gas or steam, sparks settling back into their conductor,
-// the element kill at zero life, and the stasis flag.  Each slot reads and
-// writes only its own state, so one thread per slot reproduces the
-// reference exactly.
+// (crates/tpt-sim/src/reaction_kernel/mod.rs:1123-1237) this library runs,
+// in the driver's order:
+//  * the modext clear of the gravity mass-override channel
+//    (reaction_kernel/modext/mod.rs:53-99);
+//  * the life pass (life_and_local_reaction_kernel, life.rs:47-205): fire,
+//    plasma and TRON burn-in lifetimes, fire burning out to smoke or steam,
+//    plasma to noble gas or steam, sparks settling back into their
+//    conductor, the element kill at zero life, and the stasis flag.  Each
+//    slot reads and writes only its own state;
+//  * the combustion pass's FIRE/PLSM heating and 5x5 ignition with the
+//    COAL/BCOL timer, and its WTRV branch (combustion_reaction_kernel,
+//    combustion/fire.rs:26-408, 983-1013);
+//  * the fluid pass's WATR, DSTW, SLTW, ICEI and SNOW branches
+//    (fluid_reaction_kernel, fluid.rs:149-400).
+//
+// The two pixel-parallel passes race in the reference: a burner heats and
+// ignites neighbours that other burners read, and a water kills a fire that
+// another water reads.  They run here under one canonical rule, the one the
+// oracle restates as a serial loop over pixels in ascending order:
+//  * an acting particle decides from the state its pass started with (a
+//    burner's own temperature, which pixels hold burners, the neighbour
+//    types a steam or water rule matches, the cell pressure of the ignition
+//    roll);
+//  * the particle acted upon changes in the acting pixels' ascending order,
+//    each action seeing the earlier ones (heat composes; a target keeps the
+//    first conversion: a later rule applies only while the target still has
+//    the type the rule matched);
+//  * a particle's own update happens at its own pixel's turn.
+// On the GPU each pass is a gather: one thread per pixel walks its window
+// in ascending pixel order and composes what the actors around it do to
+// its particle (replaying an actor's neighbour loop where an earlier
+// neighbour can stop it).  A decide kernel reads only the pass's starting
+// state and parks type and temperature changes in rx_side; an apply kernel
+// writes them.  Per-cell class flags, built by the life kernel, let pixels
+// with no actor and target nearby return after reading nine flag words.
 
 #include "tpt_internal.h"
 
 namespace {
 
 constexpr int BLK = 256;
-constexpr uint32_t T_WATR = 2, T_FIRE = 4, T_METL = 11, T_SPRK = 12, T_SMKE = 16, T_SLTW = 19, T_WTRV = 21,
-                   T_PLSM = 28, T_PSCN = 50, T_NSCN = 51, T_ETRD = 55, T_SWCH = 60, T_INWR = 63, T_NBLE = 97,
-                   T_HFLM = 100, T_TRON = 151, T_RSST = 191;
+constexpr uint32_t T_WATR = 2, T_FIRE = 4, T_STNE = 5, T_METL = 11, T_SPRK = 12, T_SNOW = 13, T_ICEI = 14,
+                   T_SMKE = 16, T_SAND = 17, T_SALT = 18, T_SLTW = 19, T_WTRV = 21, T_BCOL = 27, T_PLSM = 28,
+                   T_DSTW = 35, T_PLNT = 45, T_PSCN = 50, T_NSCN = 51, T_ETRD = 55, T_SWCH = 60, T_COAL = 62,
+                   T_INWR = 63, T_SPNG = 78, T_FRZZ = 83, T_FRZW = 84, T_RBDM = 93, T_LRBD = 94, T_NBLE = 97,
+                   T_HFLM = 100, T_TRON = 151, T_ROCK = 189, T_RSST = 191, T_BASE = 193;
 constexpr uint32_t PROP_LIFE_KILL = 0x8000u, FLAG_STASIS = 0x20u;
+// GpuElementProps words the element passes read.
+constexpr uint32_t W_FLAMMABLE = 12, W_EXPLOSIVE = 13, W_DEFAULT_TEMP = 16;
+constexpr float STEAM_IGNITE_K = 285.15f, STEAM_COOL_K = 1273.0f;
+// fluid.rs:99-110
+constexpr uint32_t SALT_WATR = 0x2545F491u, SALT_DSTW = 0x9E3779B9u, SALT_SLTW = 0x85EBCA6Bu,
+                   SALT_ICE = 0xC2B2AE35u, ROLL_DERIVE = 0x517CC1B7u;
+
+// Per-cell class flags (rx_flags).
+constexpr uint32_t F_BURN = 1u << 0,      // FIRE or PLSM outside stasis
+    F_WTRV_HOT = 1u << 1,                 // WTRV with ctype FIRE
+    F_WTRV = 1u << 2, F_RBDM = 1u << 3,   // RBDM or LRBD
+    F_WATR = 1u << 4, F_DSTW = 1u << 5, F_SLTW = 1u << 6, F_ICE = 1u << 7,  // ICEI or SNOW
+    F_ICE_FRZW = 1u << 8,                 // ICEI or SNOW with ctype FRZW
+    F_SALT = 1u << 9, F_FIRE = 1u << 10, F_ROCK = 1u << 11, F_SMKE = 1u << 12, F_PLNT = 1u << 13,
+    F_FRZZ = 1u << 14;
+// rx_side.x
+constexpr uint32_t SIDE_TYPE = 0x80000000u, SIDE_TEMP = 0x40000000u;
 
 // hash_u32 (reaction_kernel/shared.rs:1015-1023), MurmurHash3's finaliser.
 __device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
@@ -30,6 +75,39 @@ __device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
     return h;
 }
 
+// chance (combustion/mod.rs:280-286).
+__device__ __forceinline__ bool chance(uint32_t seed, uint32_t num, uint32_t den) {
+    return den != 0u && hash_u32(seed) % den < num;
+}
+
+__device__ __forceinline__ uint32_t slot_of(uint32_t v) {
+    const uint32_t raw = v >> 9;
+    return raw ? raw - 1u : 0u;
+}
+
+__device__ __forceinline__ uint32_t class_flags(uint32_t ty, uint32_t ctype, bool frozen) {
+    uint32_t f = 0u;
+    if ((ty == T_FIRE || ty == T_PLSM) && !frozen) f |= F_BURN;
+    if (ty == T_FIRE) f |= F_FIRE;
+    if (ty == T_WTRV) f |= F_WTRV | (ctype == T_FIRE ? F_WTRV_HOT : 0u);
+    if (ty == T_RBDM || ty == T_LRBD) f |= F_RBDM;
+    if (ty == T_WATR) f |= F_WATR;
+    if (ty == T_DSTW) f |= F_DSTW;
+    if (ty == T_SLTW) f |= F_SLTW;
+    if (ty == T_ICEI || ty == T_SNOW) f |= F_ICE | (ctype == T_FRZW ? F_ICE_FRZW : 0u);
+    if (ty == T_SALT) f |= F_SALT;
+    if (ty == T_ROCK) f |= F_ROCK;
+    if (ty == T_SMKE) f |= F_SMKE;
+    if (ty == T_PLNT) f |= F_PLNT;
+    if (ty == T_FRZZ) f |= F_FRZZ;
+    return f;
+}
+
+__global__ void rx_flags_clear(uint32_t *__restrict__ flags, uint32_t cells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < cells) flags[c] = 0u;
+}
+
 struct LifeArgs {
     uint32_t *ty;
     int32_t *life, *tmp;
@@ -37,6 +115,7 @@ struct LifeArgs {
     float *temp;
     const float *x, *y;
     const uint32_t *wall, *emap, *props, *counters;
+    uint32_t *cell_flags;
     uint32_t slots;
     int xres, yres, xc, yc;
 };
@@ -49,13 +128,16 @@ __global__ __launch_bounds__(BLK) void rx_life(LifeArgs a) {
     const uint32_t seed = a.counters[TPT_CTR_SEED];
     const int xi = tpt_f2i(a.x[i] + 0.5f), yi = tpt_f2i(a.y[i] + 0.5f);
     bool frozen = false;
+    int cell = -1;
     if (xi >= 0 && yi >= 0 && xi < a.xres && yi < a.yres) {
-        const int cell = (yi / TPT_CELL) * a.xc + xi / TPT_CELL;
+        cell = (yi / TPT_CELL) * a.xc + xi / TPT_CELL;
         frozen = a.wall[cell] == 18u && (a.emap[cell] & 0xFFu) < 8u;
     }
     const uint32_t flags = a.flags[i];
     if (frozen) {
         if (!(flags & FLAG_STASIS)) a.flags[i] = flags | FLAG_STASIS;
+        const uint32_t f = class_flags(ty, a.ctype[i], true);
+        if (f) atomicOr(a.cell_flags + cell, f);
         return;
     }
     if (flags & FLAG_STASIS) a.flags[i] = flags & ~FLAG_STASIS;
@@ -114,10 +196,409 @@ __global__ __launch_bounds__(BLK) void rx_life(LifeArgs a) {
     if (new_ty == ty && (props & PROP_LIFE_KILL) && life <= 0) dead = true;
     if (dead) {
         a.ty[i] = 0u;
-    } else {
-        if (new_ty != ty) a.ty[i] = new_ty;
-        if (life != life0) a.life[i] = life;
+        return;
     }
+    if (new_ty != ty) a.ty[i] = new_ty;
+    if (life != life0) a.life[i] = life;
+    if (cell >= 0) {
+        const uint32_t f = class_flags(new_ty, a.ctype[i], false);
+        if (f) atomicOr(a.cell_flags + cell, f);
+    }
+}
+
+// The element passes' view of the canvas.
+struct RxArgs {
+    const uint32_t *pmap;
+    uint32_t *ty;
+    int32_t *life, *tmp;
+    uint32_t *ctype;
+    const uint32_t *pflags;
+    float *temp;
+    const float *vx, *vy;
+    float *air_pv;
+    const uint32_t *props, *counters;
+    uint32_t *cell_flags, *pvcount;
+    uint2 *side;
+    int xres, yres, xc, yc;
+    int legacy;
+};
+
+__device__ __forceinline__ uint32_t pmap_at(const RxArgs &a, int x, int y) {
+    return (x < 0 || y < 0 || x >= a.xres || y >= a.yres) ? 0u : a.pmap[y * a.xres + x];
+}
+
+// OR of the class flags of the 3x3 cells around the cell of (x, y): every
+// pixel within 2 of (x, y) lies in one of them.
+__device__ __forceinline__ uint32_t flags_around(const RxArgs &a, int x, int y, uint32_t *own) {
+    const int cx = x / TPT_CELL, cy = y / TPT_CELL;
+    uint32_t f = 0u;
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int yy = cy + dy;
+        if (yy < 0 || yy >= a.yc) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int xx = cx + dx;
+            if (xx < 0 || xx >= a.xc) continue;
+            const uint32_t v = a.cell_flags[yy * a.xc + xx];
+            f |= v;
+            if (!dx && !dy) *own = v;
+        }
+    }
+    return f;
+}
+
+__device__ __forceinline__ bool combustion_relevant(uint32_t around, uint32_t own) {
+    return (around & F_BURN) || (own & F_WTRV_HOT) || ((own & F_WTRV) && (around & F_RBDM));
+}
+
+__device__ __forceinline__ bool fluid_relevant(uint32_t around, uint32_t own) {
+    return ((around & F_WATR) && (around & (F_SALT | F_RBDM | F_FIRE | F_SLTW | F_ROCK))) ||
+           ((around & F_DSTW) && (around & (F_SALT | F_SLTW | F_WATR | F_RBDM | F_FIRE | F_SMKE))) ||
+           ((around & F_SLTW) && (around & (F_SALT | F_PLNT | F_RBDM | F_FIRE))) ||
+           ((around & F_ICE) && (around & (F_SALT | F_SLTW | F_FRZZ))) || (own & F_ICE_FRZW);
+}
+
+// The particle a gather thread composes.
+struct Subject {
+    uint32_t ty;
+    int32_t life, tmp;
+    uint32_t ctype;
+    float temp;
+};
+
+// Combustion decide: the FIRE/PLSM burners of the 5x5 window act on the
+// pixel's particle in ascending pixel order (fire.rs:143-408), the particle
+// runs its own WTRV branch (fire.rs:983-1013) at its own pixel's turn.
+__global__ __launch_bounds__(BLK) void rx_comb_decide(RxArgs a) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.xres * a.yres) return;
+    const int x = q % a.xres, y = q / a.xres;
+    uint32_t own = 0u;
+    const uint32_t around = flags_around(a, x, y, &own);
+    if (!combustion_relevant(around, own)) return;
+    const uint32_t v = a.pmap[q];
+    if (!v) return;
+    const uint32_t id = slot_of(v);
+    Subject s;
+    s.ty = a.ty[id];
+    if (s.ty == 0u) return;
+    const uint32_t ty0 = s.ty;
+    s.life = a.life[id];
+    s.tmp = a.tmp[id];
+    s.ctype = a.ctype[id];
+    s.temp = a.temp[id];
+    const Subject s0 = s;
+    const bool frozen = (a.pflags[id] & FLAG_STASIS) != 0u;
+    const uint32_t seed = a.counters[TPT_CTR_SEED];
+    const float fire_t = __uint_as_float(a.props[T_FIRE * 32u + W_DEFAULT_TEMP]);
+    const bool burners = (around & F_BURN) != 0u;
+    int ign_cell = -1;
+    for (int ddy = -2; ddy <= 2; ++ddy) {
+        for (int ddx = -2; ddx <= 2; ++ddx) {
+            const int bx = x + ddx, by = y + ddy;
+            if (bx < 0 || by < 0 || bx >= a.xres || by >= a.yres) continue;
+            if (!ddx && !ddy) {
+                // the particle's own update: the WTRV branch
+                if (s.ty != T_WTRV || frozen) continue;
+                const uint32_t prand = hash_u32(seed ^ ((uint32_t)q * 2654435761u));
+                const float t = s.temp;
+                bool ignited = false;
+                for (uint32_t n = 0; n < 9u && !ignited; ++n) {
+                    const int dx = (int)(n % 3u) - 1, dy = (int)(n / 3u) - 1;
+                    if (!dx && !dy) continue;
+                    const uint32_t nv = pmap_at(a, x + dx, y + dy);
+                    if (!nv) continue;
+                    const uint32_t nty = a.ty[slot_of(nv)];
+                    if ((nty == T_RBDM || nty == T_LRBD) && t > STEAM_IGNITE_K && chance(prand ^ n, 1u, 100u)) {
+                        s.ty = T_FIRE;
+                        s.life = 4;
+                        s.ctype = T_WATR;
+                        ignited = true;
+                    }
+                }
+                if (!ignited && t > STEAM_COOL_K && s.ctype == T_FIRE) s.temp = t - t / 1000.0f;
+                continue;
+            }
+            if (!burners) continue;
+            const uint32_t bv = a.pmap[by * a.xres + bx];
+            if (!bv) continue;
+            const uint32_t b = slot_of(bv), bty = a.ty[b];
+            if ((bty != T_FIRE && bty != T_PLSM) || (a.pflags[b] & FLAG_STASIS)) continue;
+            // burner at (bx, by); this particle is its neighbour n
+            const float tb = a.temp[b];
+            const uint32_t n = (uint32_t)((2 - ddy) * 5 + (2 - ddx));
+            const uint32_t bq = (uint32_t)(by * a.xres + bx);
+            const uint32_t prand = hash_u32(seed ^ (bq * 2654435761u));
+            const uint32_t nroll = hash_u32(prand ^ (n * 0x9E3779B9u));
+            if ((s.ty == T_COAL || s.ty == T_BCOL) && s.life > 100 && chance(nroll, 1u, 500u)) s.life = 99;
+            if (tb > s.temp) {
+                float h = s.temp + (tb - s.temp) * 0.05f;
+                if (h > 9999.0f) h = 9999.0f;
+                s.temp = h;
+            }
+            const int32_t nflam = s.ty < 256u ? (int32_t)a.props[s.ty * 32u + W_FLAMMABLE] : 0;
+            const bool nexp = s.ty < 256u && (int32_t)a.props[s.ty * 32u + W_EXPLOSIVE] != 0;
+            if (nflam <= 0) continue;
+            uint32_t surround = 0u;
+            for (int k = 0; k < 9; ++k) {
+                const int sx = k % 3 - 1, sy = k / 3 - 1;
+                if ((sx || sy) && pmap_at(a, bx + sx, by + sy) == 0u) ++surround;
+            }
+            bool ignitable = surround > 0u || nexp;
+            if (s.ty == T_SPNG && s.life > 0) ignitable = false;
+            if (!ignitable) continue;
+            const float pv = a.air_pv[(y / TPT_CELL) * a.xc + x / TPT_CELL];
+            const int32_t num = tpt_f2i((float)nflam + pv * 10.0f);
+            bool hit = false;
+            if (num >= 1000) hit = true;
+            else if (num > 0) hit = chance(seed ^ (id * 2654435761u) ^ n, (uint32_t)num, 1000u);
+            if (!hit) continue;
+            s.ty = T_FIRE;
+            s.life = 180 + (int32_t)(hash_u32(id ^ seed) % 80u);
+            float ft = fire_t + (float)(nflam / 2);
+            if (ft < 0.0f) ft = 0.0f;
+            if (ft > 9999.0f) ft = 9999.0f;
+            s.temp = ft;
+            s.tmp = 0;
+            s.ctype = 0u;
+            if (nexp) ign_cell = (by / TPT_CELL) * a.xc + bx / TPT_CELL;
+        }
+    }
+    if (s.life != s0.life) a.life[id] = s.life;
+    if (s.tmp != s0.tmp) a.tmp[id] = s.tmp;
+    if (s.ctype != s0.ctype) a.ctype[id] = s.ctype;
+    uint32_t side = 0u;
+    if (s.ty != ty0) side |= SIDE_TYPE | s.ty;
+    if (__float_as_uint(s.temp) != __float_as_uint(s0.temp)) side |= SIDE_TEMP;
+    if (side) a.side[q] = make_uint2(side, __float_as_uint(s.temp));
+    if (s.ty == T_FIRE && ty0 != T_FIRE) atomicOr(a.cell_flags + (y / TPT_CELL) * a.xc + x / TPT_CELL, F_FIRE);
+    if (ign_cell >= 0) atomicAdd(a.pvcount + ign_cell, 1u);
+}
+
+// Applies parked type/temperature writes of the pixels `relevant` selects,
+// and (for the combustion pass) each burner cell's explosive pressure.
+template <int PASS>
+__global__ __launch_bounds__(BLK) void rx_apply(RxArgs a) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.xres * a.yres) return;
+    const int x = q % a.xres, y = q / a.xres;
+    uint32_t own = 0u;
+    const uint32_t around = flags_around(a, x, y, &own);
+    if (PASS == 0 && (own & F_BURN) && x % TPT_CELL == 0 && y % TPT_CELL == 0) {
+        const int c = (y / TPT_CELL) * a.xc + x / TPT_CELL;
+        const uint32_t k = a.pvcount[c];
+        if (k) {
+            float pv = a.air_pv[c];
+            for (uint32_t i = 0; i < k; ++i) pv += 0.25f;
+            a.air_pv[c] = pv;
+            a.pvcount[c] = 0u;
+        }
+    }
+    if (!(PASS == 0 ? combustion_relevant(around, own) : fluid_relevant(around, own))) return;
+    const uint2 sd = a.side[q];
+    if (!sd.x) return;
+    const uint32_t id = slot_of(a.pmap[q]);
+    if (sd.x & SIDE_TYPE) a.ty[id] = sd.x & 0x1FFu;
+    if (sd.x & SIDE_TEMP) a.temp[id] = __uint_as_float(sd.y);
+    a.side[q] = make_uint2(0u, 0u);
+}
+
+__device__ __forceinline__ bool is_actor(uint32_t ty) {
+    return ty == T_WATR || ty == T_DSTW || ty == T_SLTW || ty == T_ICEI || ty == T_SNOW;
+}
+
+// One fluid actor's neighbour loop (fluid.rs:180-399) at pixel (px, py),
+// from the pass's starting state.  With n_stop < 9 it stops before
+// neighbour n_stop and reports, in *conv / *conv_ctype, what the rule does
+// to that neighbour when the actor reaches it (*reached); with n_stop == 9
+// it runs the whole loop and reports the actor's own results.
+struct ActorOut {
+    uint32_t new_ty;
+    int32_t life;
+    uint32_t ctype;
+    float temp;
+    bool reached;
+    uint32_t conv, conv_ctype;  // conversion of neighbour n_stop (0xFFFFFFFF: none)
+};
+
+__device__ ActorOut fluid_actor(const RxArgs &a, uint32_t seed, int px, int py, uint32_t aid, uint32_t ty, uint32_t n_stop) {
+    ActorOut o;
+    o.new_ty = ty;
+    o.life = a.life[aid];
+    o.ctype = a.ctype[aid];
+    o.temp = a.temp[aid];
+    o.reached = false;
+    o.conv = 0xFFFFFFFFu;
+    o.conv_ctype = 0xFFFFFFFFu;
+    const uint32_t idx = (uint32_t)(py * a.xres + px);
+    const float speed = fabsf(a.vx[aid]) + fabsf(a.vy[aid]);
+    const float own_t = o.temp;
+    if ((ty == T_ICEI || ty == T_SNOW) && o.ctype == T_FRZW) {
+        float t = o.temp - 1.0f;
+        if (t < 0.0f) t = 0.0f;
+        if (t > 9999.0f) t = 9999.0f;
+        o.temp = t;
+    }
+    const uint32_t salt = ty == T_WATR ? SALT_WATR : ty == T_DSTW ? SALT_DSTW : ty == T_SLTW ? SALT_SLTW : SALT_ICE;
+    bool stop = false;
+    for (uint32_t n = 0; n < 9u && !stop; ++n) {
+        const int dx = (int)(n % 3u) - 1, dy = (int)(n / 3u) - 1;
+        if (!dx && !dy) continue;
+        const uint32_t nv = pmap_at(a, px + dx, py + dy);
+        if (!nv) continue;
+        if (n == n_stop) o.reached = true;
+        const uint32_t nid = slot_of(nv), nty = a.ty[nid];
+        const uint32_t r = hash_u32(seed ^ (idx * salt) ^ (n * ROLL_DERIVE));
+        uint32_t conv = 0xFFFFFFFFu, conv_ctype = 0xFFFFFFFFu;
+        if (ty == T_WATR) {
+            if (nty == T_SALT && r % 50u == 0u) {
+                o.new_ty = T_SLTW;
+                if (hash_u32(r ^ 0x51ED270Bu) % 3u == 0u) conv = T_SLTW;
+            } else if ((nty == T_RBDM || nty == T_LRBD) && (a.legacy || own_t > 285.15f) && r % 100u == 0u) {
+                o.new_ty = T_FIRE;
+                o.life = 4;
+                o.ctype = T_WATR;
+            } else if (nty == T_FIRE && a.ctype[nid] != T_WATR) {
+                conv = 0u;
+                if (hash_u32(r ^ 0x2C1B3E4Du) % 30u == 0u) stop = true;
+            } else if (nty == T_SLTW && r % 2000u == 0u) {
+                o.new_ty = T_SLTW;
+            } else if (nty == T_ROCK && speed >= 0.5f && r % 1000u == 0u) {
+                conv = hash_u32(r ^ 0x7F4A7C15u) % 3u == 0u ? T_SAND : T_STNE;
+            }
+        } else if (ty == T_DSTW) {
+            if (nty == T_SALT && r % 50u == 0u) {
+                o.new_ty = T_SLTW;
+                if (hash_u32(r ^ 0x51ED270Bu) % 3u == 0u) conv = T_SLTW;
+            } else if (nty == T_SLTW) {
+                if (r % 2000u == 0u) o.new_ty = T_SLTW;
+                else if (hash_u32(r ^ 0x3AE91F27u) % 100u == 0u) o.new_ty = T_WATR;
+            } else if (nty == T_WATR && r % 100u == 0u) {
+                o.new_ty = T_WATR;
+            } else if ((nty == T_RBDM || nty == T_LRBD) && (a.legacy || own_t > 12.0f) && r % 100u == 0u) {
+                o.new_ty = T_FIRE;
+                o.life = 4;
+            } else if (nty == T_FIRE) {
+                conv = 0u;
+                if (hash_u32(r ^ 0x2C1B3E4Du) % 30u == 0u) stop = true;
+            } else if (nty == T_SMKE) {
+                const float nt = a.temp[nid];
+                if (nt > 313.15f && nt < 333.15f && own_t > 313.15f && own_t < 333.15f && r % 100u == 0u) {
+                    o.new_ty = T_BASE;
+                    o.life = 1;
+                    conv = 0u;
+                }
+            }
+        } else if (ty == T_SLTW) {
+            if (nty == T_SALT && r % 2000u == 0u) {
+                conv = T_SLTW;
+            } else if (nty == T_PLNT && r % 40u == 0u) {
+                conv = 0u;
+            } else if ((nty == T_RBDM || nty == T_LRBD) && (a.legacy || own_t > 285.15f) && r % 100u == 0u) {
+                o.new_ty = T_FIRE;
+                o.life = 4;
+                o.ctype = T_WATR;
+            } else if (nty == T_FIRE && a.ctype[nid] != T_WATR) {
+                conv = 0u;
+                if (hash_u32(r ^ 0x2C1B3E4Du) % 30u == 0u) stop = true;
+            }
+        } else {  // ICEI, SNOW
+            const uint32_t div = ty == T_ICEI ? 200u : 333u;
+            if ((nty == T_SALT || nty == T_SLTW) && (ty == T_SNOW || o.temp > 252.05f) && r % div == 0u) {
+                o.new_ty = T_SLTW;
+                conv = T_SLTW;
+                stop = true;
+            } else if (ty == T_ICEI && nty == T_FRZZ && r % 200u == 0u) {
+                conv = T_ICEI;
+                conv_ctype = T_FRZW;
+            }
+        }
+        if (n == n_stop) {
+            o.conv = conv;
+            o.conv_ctype = conv_ctype;
+            return o;
+        }
+    }
+    // WATR, DSTW and SLTW end dead after a killing extinguish; ICEI and SNOW only stop
+    if (stop && ty != T_ICEI && ty != T_SNOW) o.new_ty = 0u;
+    return o;
+}
+
+// Fluid decide: the actors of the 3x3 window act on the pixel's particle in
+// ascending pixel order; the particle runs its own branch at its own turn.
+__global__ __launch_bounds__(BLK) void rx_fluid_decide(RxArgs a) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.xres * a.yres) return;
+    const int x = q % a.xres, y = q / a.xres;
+    uint32_t own = 0u;
+    const uint32_t around = flags_around(a, x, y, &own);
+    if (!fluid_relevant(around, own)) return;
+    const uint32_t v = a.pmap[q];
+    if (!v) return;
+    const uint32_t id = slot_of(v), ty0 = a.ty[id];
+    if (ty0 == 0u) return;
+    const uint32_t seed = a.counters[TPT_CTR_SEED];
+    uint32_t ty = ty0;
+    int32_t life = a.life[id];
+    uint32_t ctype = a.ctype[id];
+    const int32_t life0 = life;
+    const uint32_t ctype0 = ctype;
+    float temp = a.temp[id];
+    const float temp0 = temp;
+    for (int ddy = -1; ddy <= 1; ++ddy) {
+        for (int ddx = -1; ddx <= 1; ++ddx) {
+            const int px = x + ddx, py = y + ddy;
+            if (px < 0 || py < 0 || px >= a.xres || py >= a.yres) continue;
+            if (!ddx && !ddy) {
+                if (!is_actor(ty0) || (a.pflags[id] & FLAG_STASIS)) continue;
+                const ActorOut o = fluid_actor(a, seed, x, y, id, ty0, 9u);
+                ty = o.new_ty;
+                life = o.life;
+                ctype = o.ctype;
+                temp = o.temp;
+                continue;
+            }
+            const uint32_t pv = a.pmap[py * a.xres + px];
+            if (!pv) continue;
+            const uint32_t aid = slot_of(pv), aty = a.ty[aid];
+            if (!is_actor(aty) || (a.pflags[aid] & FLAG_STASIS)) continue;
+            const uint32_t n = (uint32_t)((1 - ddy) * 3 + (1 - ddx));
+            const ActorOut o = fluid_actor(a, seed, px, py, aid, aty, n);
+            if (!o.reached || o.conv == 0xFFFFFFFFu || ty != ty0) continue;
+            ty = o.conv;
+            if (o.conv_ctype != 0xFFFFFFFFu) ctype = o.conv_ctype;
+        }
+    }
+    if (life != life0) a.life[id] = life;
+    if (ctype != ctype0) a.ctype[id] = ctype;
+    uint32_t side = 0u;
+    if (ty != ty0) side |= SIDE_TYPE | ty;
+    if (__float_as_uint(temp) != __float_as_uint(temp0)) side |= SIDE_TEMP;
+    if (side) a.side[q] = make_uint2(side, __float_as_uint(temp));
+}
+
+RxArgs rx_args(tpt_ctx *ctx) {
+    RxArgs a;
+    a.pmap = tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP);
+    a.ty = tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY);
+    a.life = tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE);
+    a.tmp = tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP);
+    a.ctype = tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE);
+    a.pflags = tpt_buf<uint32_t>(ctx, TPT_BUF_PART_FLAGS);
+    a.temp = tpt_buf<float>(ctx, TPT_BUF_PART_TEMP);
+    a.vx = tpt_buf<float>(ctx, TPT_BUF_PART_VX);
+    a.vy = tpt_buf<float>(ctx, TPT_BUF_PART_VY);
+    a.air_pv = tpt_buf<float>(ctx, TPT_BUF_AIR_PV);
+    a.props = tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS);
+    a.counters = ctx->counters;
+    a.cell_flags = ctx->rx_flags;
+    a.pvcount = ctx->rx_pvcount;
+    a.side = ctx->rx_side;
+    a.xres = (int)ctx->cfg.xres;
+    a.yres = (int)ctx->cfg.yres;
+    a.xc = (int)ctx->xcells;
+    a.yc = (int)ctx->ycells;
+    a.legacy = ctx->cfg.heat_enable ? 0 : 1;
+    return a;
 }
 
 }  // namespace
@@ -126,17 +607,33 @@ int tpt_reactions_step(tpt_ctx *ctx) {
     int rc = tpt_gravity_override_clear(ctx);
     if (rc) return rc;
     const uint32_t slots = ctx->slot_bound;
-    LifeArgs a{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY),    tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
+    hipLaunchKernelGGL(rx_flags_clear, dim3((ctx->cells + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, ctx->rx_flags,
+                       ctx->cells);
+    LifeArgs l{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY),    tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
                tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP),    tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE),
                tpt_buf<uint32_t>(ctx, TPT_BUF_PART_FLAGS), tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),
                tpt_buf<float>(ctx, TPT_BUF_PART_X),        tpt_buf<float>(ctx, TPT_BUF_PART_Y),
                tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP),  tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP),
                tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS), ctx->counters,
-               slots,                                      (int)ctx->cfg.xres,
-               (int)ctx->cfg.yres,                         (int)ctx->xcells,
-               (int)ctx->ycells};
+               ctx->rx_flags,                              slots,
+               (int)ctx->cfg.xres,                         (int)ctx->cfg.yres,
+               (int)ctx->xcells,                           (int)ctx->ycells};
     tpt_pass_begin(ctx, PASS_REACTION_LIFE);
-    hipLaunchKernelGGL(rx_life, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, a);
+    hipLaunchKernelGGL(rx_life, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, l);
     tpt_pass_end(ctx, PASS_REACTION_LIFE);
-    return tpt_check_launch("rx_life");
+    if ((rc = tpt_check_launch("rx_life"))) return rc;
+    const RxArgs a = rx_args(ctx);
+    const dim3 grid((ctx->pixels + BLK - 1) / BLK);
+    tpt_pass_begin(ctx, PASS_REACTION_COMBUSTION);
+    hipLaunchKernelGGL(rx_comb_decide, grid, dim3(BLK), 0, ctx->stream, a);
+    hipLaunchKernelGGL(rx_apply<0>, grid, dim3(BLK), 0, ctx->stream, a);
+    tpt_pass_end(ctx, PASS_REACTION_COMBUSTION);
+    if ((rc = tpt_check_launch("rx_combustion"))) return rc;
+    tpt_pass_begin(ctx, PASS_REACTION_FLUID);
+    hipLaunchKernelGGL(rx_fluid_decide, grid, dim3(BLK), 0, ctx->stream, a);
+    hipLaunchKernelGGL(rx_apply<1>, grid, dim3(BLK), 0, ctx->stream, a);
+    tpt_pass_end(ctx, PASS_REACTION_FLUID);
+    // Positions are unchanged: the pixel records still cover every slot
+    // (types and temperatures are read from the slot arrays downstream).
+    return tpt_check_launch("rx_fluid");
 }

@@ -20,7 +20,7 @@ const char *const tpt_pass_names[PASS_COUNT] = {
     "thermal_conduction", "thermal_conduction_cells", "thermal_apply", "phase_transition",
     "phase_transition_cells", "refresh_pmap_types", "air_deposit", "air_deposit_cells", "move_detect",
     "move_prepare", "move_velocity_intent", "move_saw_pair", "move_commit", "move_photon_hits_drain",
-    "move_rebuild_maps", "water_equal", "reaction_life",
+    "move_rebuild_maps", "water_equal", "reaction_life", "reaction_combustion", "reaction_fluid",
 };
 
 static thread_local char g_err[512];
@@ -103,6 +103,9 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMalloc(&c->slot_owner, cap * 4) != hipSuccess ||
         hipMalloc(&c->create_rec, (cap * TPT_CREATE_REC_WORDS + TPT_SPAWN_BITMAP_WORDS) * 4) != hipSuccess ||
         hipMalloc(&c->cell_rec, (size_t)c->cells * sizeof(uint4)) != hipSuccess ||
+        hipMalloc(&c->rx_flags, (size_t)c->cells * 4) != hipSuccess ||
+        hipMalloc(&c->rx_side, (size_t)c->pixels * sizeof(uint2)) != hipSuccess ||
+        hipMalloc(&c->rx_pvcount, (size_t)c->cells * 4) != hipSuccess ||
         hipMalloc(&c->water.col_min, (size_t)c->cfg.xres * 4) != hipSuccess ||
         hipMalloc(&c->water.col_max, (size_t)c->cfg.xres * 4) != hipSuccess ||
         hipMalloc(&c->water.col_surf, (size_t)c->cfg.xres * 4) != hipSuccess ||
@@ -114,6 +117,9 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMemsetAsync(c->cell_scratch_u, 0, cb, c->stream) != hipSuccess ||
         hipMemsetAsync(c->cell_scratch_u2, 0, cb, c->stream) != hipSuccess ||
         hipMemsetAsync(c->counters, 0, 256, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->rx_flags, 0, (size_t)c->cells * 4, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->rx_side, 0, (size_t)c->pixels * sizeof(uint2), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->rx_pvcount, 0, (size_t)c->cells * 4, c->stream) != hipSuccess ||
         hipMemsetAsync(c->slot_owner, 0xFF, cap * 4, c->stream) != hipSuccess)
         return tpt_set_error(TPT_EDEVICE, "scratch clear failed");
     return TPT_OK;
@@ -125,7 +131,7 @@ static void free_all(tpt_ctx *c) {
     void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
                      c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters,
                      c->pix_rec, c->pix_heat, c->offmap, c->pix_type, c->pix_count_mv, c->slot_owner, c->create_rec, c->cell_rec,
-                     c->water.col_min, c->water.col_max, c->water.col_surf, c->water.col_target,
+                     c->rx_flags, c->rx_side, c->rx_pvcount, c->water.col_min, c->water.col_max, c->water.col_surf, c->water.col_target,
                      c->water.winner};
     for (void *p : extra)
         if (p) (void)hipFree(p);

@@ -63,6 +63,8 @@ enum tpt_pass {
     PASS_MOVE_REBUILD,    // pmap/photons rebuild
     PASS_WATER_EQUAL,     // water equalisation (when enabled)
     PASS_REACTION_LIFE,   // reaction life pass (burn-out, spark settling, life kill)
+    PASS_REACTION_COMBUSTION,  // fire heating and ignition, steam
+    PASS_REACTION_FLUID,  // water family and ice
     PASS_COUNT
 };
 
@@ -160,6 +162,13 @@ struct tpt_ctx {
     // transitions, then {air_vx, air_vy, wall_bmap, wall_emap} for the
     // movement planner (rebuilt by each pass before use).
     uint4 *cell_rec = nullptr;
+    // Element passes: per-cell flags of the element classes present this
+    // tick, per-pixel pending writes (type word, temperature bits) between a
+    // pass's decide and apply kernels (zero at rest), per-cell counts of
+    // explosive ignitions (zero at rest).
+    uint32_t *rx_flags = nullptr;         // cells
+    uint2 *rx_side = nullptr;             // pixels
+    uint32_t *rx_pvcount = nullptr;       // cells
     tpt_water_bufs water;
     tpt_fft_state fft;
 
