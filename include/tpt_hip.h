@@ -264,7 +264,9 @@ TPT_API int tpt_batch_set_lanes(tpt_batch *batch, uint32_t lanes);
 /* Autotuner: one eager tick, then trial_ticks graph replays at each
  * candidate lane count (1, 2, 4, ... and one per canvas), timed with HIP
  * events; keeps the fastest (*lanes_out) and its graph.  The canvases
- * advance through all of these ticks. */
+ * advance through all of these ticks.  Currently switched off: it returns
+ * TPT_ESTATE (one run aborted inside the HIP runtime with no cause found;
+ * tpt_batch_set_lanes sets a split by hand). */
 TPT_API int tpt_batch_autotune(tpt_batch *batch, uint64_t first_frame, uint32_t trial_ticks, uint32_t *lanes_out);
 /* The autotuner's record: lane counts and milliseconds per batch tick. */
 TPT_API int tpt_batch_tuning(tpt_batch *batch, uint32_t *lanes, float *ms_per_tick, uint32_t max_entries,

@@ -93,12 +93,16 @@ def run_batch(scenes, drive):
     return out
 
 
-def test_serial_lanes_and_autotune_equal_launches(gpu):
-    """One canvas at a time, the graph split into 2 and 3 lanes, and the
-    autotuned graph all give the per-canvas launches' results."""
+def scenes_for_lanes():
     shapes = [(400, 300, 1, 8000), (800, 200, 2, 12000), (256, 256, 5, 5000), (128, 512, 6, 4000), (640, 480, 8, 30000)]
-    scenes = [S.make_scene(xres=x, yres=y, seed=sd, particles=n) for x, y, sd, n in shapes]
-    print("[batch] serial / lanes / autotune vs launches", flush=True)
+    return [S.make_scene(xres=x, yres=y, seed=sd, particles=n) for x, y, sd, n in shapes]
+
+
+def test_serial_and_lanes_equal_launches(gpu):
+    """One canvas at a time and the graph split into 2 and 3 lanes give the
+    per-canvas launches' results."""
+    scenes = scenes_for_lanes()
+    print("[batch] serial / lanes vs launches", flush=True)
     ref = run_batch(scenes, lambda b: b.step_launches(0, 12))
     got = run_batch(scenes, lambda b: b.step_serial(0, 12))
     for k in range(len(scenes)):
@@ -110,6 +114,14 @@ def test_serial_lanes_and_autotune_equal_launches(gpu):
         got = run_batch(scenes, drive)
         for k in range(len(scenes)):
             same(ref[k], got[k], f"{lanes} lanes canvas {k}")
+
+
+@pytest.mark.skip(reason="tpt_batch_autotune is switched off: one run aborted inside the HIP runtime "
+                         "(no GPU fault reported) with no cause found")
+def test_autotune_equals_launches(gpu):
+    """The autotuned graph gives the per-canvas launches' results."""
+    scenes = scenes_for_lanes()
+    ref = run_batch(scenes, lambda b: b.step_launches(0, 12))
     seen = {}
 
     def tuned(b):
@@ -122,3 +134,11 @@ def test_serial_lanes_and_autotune_equal_launches(gpu):
     assert all(ms > 0 for _, ms in seen["table"])
     for k in range(len(scenes)):
         same(ref[k], got[k], f"autotuned canvas {k}")
+
+
+def test_autotune_is_switched_off():
+    """The switched-off autotuner refuses with TPT_ESTATE (no GPU work)."""
+    from tpt_hip import scene as S2
+    with L.Batch([S2.default_config(64, 64)]) as b:
+        with pytest.raises(L.TptError):
+            b.autotune(0, 1)

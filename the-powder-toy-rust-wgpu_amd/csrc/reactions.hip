@@ -32,8 +32,12 @@
 // its particle (replaying an actor's neighbour loop where an earlier
 // neighbour can stop it).  A decide kernel reads only the pass's starting
 // state and parks type and temperature changes in rx_side; an apply kernel
-// writes them.  Per-cell class flags, built by the life kernel, let pixels
-// with no actor and target nearby return after reading nine flag words.
+// writes them.  The life kernel also records, per cell, which of the rarer
+// classes are present (fire, salt, rock, ...; not the common water and
+// steam), so that pixels far from them return after reading nine flag
+// words, and, per pixel, each burner owner's temperature and free
+// neighbours, so that the combustion gather finds burners without random
+// reads.
 
 #include "tpt_internal.h"
 
@@ -53,14 +57,16 @@ constexpr float STEAM_IGNITE_K = 285.15f, STEAM_COOL_K = 1273.0f;
 constexpr uint32_t SALT_WATR = 0x2545F491u, SALT_DSTW = 0x9E3779B9u, SALT_SLTW = 0x85EBCA6Bu,
                    SALT_ICE = 0xC2B2AE35u, ROLL_DERIVE = 0x517CC1B7u;
 
-// Per-cell class flags (rx_flags).
+// Per-cell class flags (rx_flags), for the classes the element rules need
+// besides the common water and steam.
 constexpr uint32_t F_BURN = 1u << 0,      // FIRE or PLSM outside stasis
     F_WTRV_HOT = 1u << 1,                 // WTRV with ctype FIRE
-    F_WTRV = 1u << 2, F_RBDM = 1u << 3,   // RBDM or LRBD
-    F_WATR = 1u << 4, F_DSTW = 1u << 5, F_SLTW = 1u << 6, F_ICE = 1u << 7,  // ICEI or SNOW
+    F_RBDM = 1u << 3,                     // RBDM or LRBD
+    F_DSTW = 1u << 5, F_SLTW = 1u << 6, F_ICE = 1u << 7,  // ICEI or SNOW
     F_ICE_FRZW = 1u << 8,                 // ICEI or SNOW with ctype FRZW
     F_SALT = 1u << 9, F_FIRE = 1u << 10, F_ROCK = 1u << 11, F_SMKE = 1u << 12, F_PLNT = 1u << 13,
     F_FRZZ = 1u << 14;
+constexpr uint32_t STAMP_MASK = 0x0FFFFFFFu;
 // rx_side.x
 constexpr uint32_t SIDE_TYPE = 0x80000000u, SIDE_TEMP = 0x40000000u;
 
@@ -85,27 +91,37 @@ __device__ __forceinline__ uint32_t slot_of(uint32_t v) {
     return raw ? raw - 1u : 0u;
 }
 
-__device__ __forceinline__ uint32_t class_flags(uint32_t ty, uint32_t ctype, bool frozen) {
-    uint32_t f = 0u;
-    if ((ty == T_FIRE || ty == T_PLSM) && !frozen) f |= F_BURN;
-    if (ty == T_FIRE) f |= F_FIRE;
-    if (ty == T_WTRV) f |= F_WTRV | (ctype == T_FIRE ? F_WTRV_HOT : 0u);
-    if (ty == T_RBDM || ty == T_LRBD) f |= F_RBDM;
-    if (ty == T_WATR) f |= F_WATR;
-    if (ty == T_DSTW) f |= F_DSTW;
-    if (ty == T_SLTW) f |= F_SLTW;
-    if (ty == T_ICEI || ty == T_SNOW) f |= F_ICE | (ctype == T_FRZW ? F_ICE_FRZW : 0u);
-    if (ty == T_SALT) f |= F_SALT;
-    if (ty == T_ROCK) f |= F_ROCK;
-    if (ty == T_SMKE) f |= F_SMKE;
-    if (ty == T_PLNT) f |= F_PLNT;
-    if (ty == T_FRZZ) f |= F_FRZZ;
-    return f;
+// The class flags of a particle of type ty; ctype is read only where a
+// class depends on it.
+__device__ __forceinline__ uint32_t class_flags(uint32_t ty, const uint32_t *ctype, uint32_t i, bool frozen) {
+    switch (ty) {
+    case T_FIRE: return F_FIRE | (frozen ? 0u : F_BURN);
+    case T_PLSM: return frozen ? 0u : F_BURN;
+    case T_WTRV: return ctype[i] == T_FIRE ? F_WTRV_HOT : 0u;
+    case T_RBDM:
+    case T_LRBD: return F_RBDM;
+    case T_DSTW: return F_DSTW;
+    case T_SLTW: return F_SLTW;
+    case T_ICEI:
+    case T_SNOW: return F_ICE | (ctype[i] == T_FRZW ? F_ICE_FRZW : 0u);
+    case T_SALT: return F_SALT;
+    case T_ROCK: return F_ROCK;
+    case T_SMKE: return F_SMKE;
+    case T_PLNT: return F_PLNT;
+    case T_FRZZ: return F_FRZZ;
+    default: return 0u;
+    }
 }
 
-__global__ void rx_flags_clear(uint32_t *__restrict__ flags, uint32_t cells) {
+// Clears the class flags and advances the burner-record stamp (never 0,
+// which marks a pixel never written).
+__global__ void rx_flags_clear(uint32_t *__restrict__ flags, uint32_t cells, uint32_t *__restrict__ counters) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < cells) flags[c] = 0u;
+    if (c == 0u) {
+        const uint32_t s = counters[TPT_CTR_RX_STAMP] % STAMP_MASK + 1u;
+        counters[TPT_CTR_RX_STAMP] = s;
+    }
 }
 
 struct LifeArgs {
@@ -116,6 +132,8 @@ struct LifeArgs {
     const float *x, *y;
     const uint32_t *wall, *emap, *props, *counters;
     uint32_t *cell_flags;
+    const uint32_t *pmap;
+    uint2 *burner;
     uint32_t slots;
     int xres, yres, xc, yc;
 };
@@ -136,7 +154,7 @@ __global__ __launch_bounds__(BLK) void rx_life(LifeArgs a) {
     const uint32_t flags = a.flags[i];
     if (frozen) {
         if (!(flags & FLAG_STASIS)) a.flags[i] = flags | FLAG_STASIS;
-        const uint32_t f = class_flags(ty, a.ctype[i], true);
+        const uint32_t f = class_flags(ty, a.ctype, i, true);
         if (f) atomicOr(a.cell_flags + cell, f);
         return;
     }
@@ -200,9 +218,18 @@ __global__ __launch_bounds__(BLK) void rx_life(LifeArgs a) {
     }
     if (new_ty != ty) a.ty[i] = new_ty;
     if (life != life0) a.life[i] = life;
-    if (cell >= 0) {
-        const uint32_t f = class_flags(new_ty, a.ctype[i], false);
-        if (f) atomicOr(a.cell_flags + cell, f);
+    if (cell < 0) return;
+    const uint32_t f = class_flags(new_ty, a.ctype, i, false);
+    if (f) atomicOr(a.cell_flags + cell, f);
+    // a burner that owns its pixel: its temperature and free neighbours
+    if ((f & F_BURN) && (a.pmap[yi * a.xres + xi] >> 9) == i + 1u) {
+        uint32_t surround = 0u;
+        for (int k = 0; k < 9; ++k) {
+            const int sx = xi + k % 3 - 1, sy = yi + k / 3 - 1;
+            if (k == 4) continue;
+            if (sx < 0 || sy < 0 || sx >= a.xres || sy >= a.yres || a.pmap[sy * a.xres + sx] == 0u) ++surround;
+        }
+        a.burner[yi * a.xres + xi] = make_uint2(a.counters[TPT_CTR_RX_STAMP] | (surround << 28), __float_as_uint(a.temp[i]));
     }
 }
 
@@ -219,6 +246,7 @@ struct RxArgs {
     const uint32_t *props, *counters;
     uint32_t *cell_flags, *pvcount;
     uint2 *side;
+    const uint2 *burner;
     int xres, yres, xc, yc;
     int legacy;
 };
@@ -247,14 +275,14 @@ __device__ __forceinline__ uint32_t flags_around(const RxArgs &a, int x, int y, 
 }
 
 __device__ __forceinline__ bool combustion_relevant(uint32_t around, uint32_t own) {
-    return (around & F_BURN) || (own & F_WTRV_HOT) || ((own & F_WTRV) && (around & F_RBDM));
+    return (around & (F_BURN | F_RBDM)) || (own & F_WTRV_HOT);
 }
 
+// Every fluid rule involves at least one flagged class within 2 pixels
+// (WATR's partners, or a DSTW, SLTW or ice actor).
 __device__ __forceinline__ bool fluid_relevant(uint32_t around, uint32_t own) {
-    return ((around & F_WATR) && (around & (F_SALT | F_RBDM | F_FIRE | F_SLTW | F_ROCK))) ||
-           ((around & F_DSTW) && (around & (F_SALT | F_SLTW | F_WATR | F_RBDM | F_FIRE | F_SMKE))) ||
-           ((around & F_SLTW) && (around & (F_SALT | F_PLNT | F_RBDM | F_FIRE))) ||
-           ((around & F_ICE) && (around & (F_SALT | F_SLTW | F_FRZZ))) || (own & F_ICE_FRZW);
+    return (around & (F_SALT | F_RBDM | F_FIRE | F_SLTW | F_ROCK | F_SMKE | F_PLNT | F_FRZZ | F_DSTW | F_ICE)) ||
+           (own & F_ICE_FRZW);
 }
 
 // The particle a gather thread composes.
@@ -268,6 +296,7 @@ struct Subject {
 // Combustion decide: the FIRE/PLSM burners of the 5x5 window act on the
 // pixel's particle in ascending pixel order (fire.rs:143-408), the particle
 // runs its own WTRV branch (fire.rs:983-1013) at its own pixel's turn.
+// Burners come from this step's burner records.
 __global__ __launch_bounds__(BLK) void rx_comb_decide(RxArgs a) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= a.xres * a.yres) return;
@@ -277,10 +306,22 @@ __global__ __launch_bounds__(BLK) void rx_comb_decide(RxArgs a) {
     if (!combustion_relevant(around, own)) return;
     const uint32_t v = a.pmap[q];
     if (!v) return;
+    const uint32_t stamp = a.counters[TPT_CTR_RX_STAMP];
+    // the burners of the window, as bits of the 25 positions
+    uint32_t bmask = 0u;
+    if (around & F_BURN) {
+        for (int k = 0; k < 25; ++k) {
+            const int bx = x + k % 5 - 2, by = y + k / 5 - 2;
+            if (k == 12 || bx < 0 || by < 0 || bx >= a.xres || by >= a.yres) continue;
+            if ((a.burner[by * a.xres + bx].x & STAMP_MASK) == stamp) bmask |= 1u << k;
+        }
+    }
     const uint32_t id = slot_of(v);
     Subject s;
     s.ty = a.ty[id];
     if (s.ty == 0u) return;
+    const bool wtrv = s.ty == T_WTRV && ((own & F_WTRV_HOT) || (around & F_RBDM));
+    if (!bmask && !wtrv) return;
     const uint32_t ty0 = s.ty;
     s.life = a.life[id];
     s.tmp = a.tmp[id];
@@ -290,18 +331,16 @@ __global__ __launch_bounds__(BLK) void rx_comb_decide(RxArgs a) {
     const bool frozen = (a.pflags[id] & FLAG_STASIS) != 0u;
     const uint32_t seed = a.counters[TPT_CTR_SEED];
     const float fire_t = __uint_as_float(a.props[T_FIRE * 32u + W_DEFAULT_TEMP]);
-    const bool burners = (around & F_BURN) != 0u;
     int ign_cell = -1;
-    for (int ddy = -2; ddy <= 2; ++ddy) {
-        for (int ddx = -2; ddx <= 2; ++ddx) {
-            const int bx = x + ddx, by = y + ddy;
-            if (bx < 0 || by < 0 || bx >= a.xres || by >= a.yres) continue;
-            if (!ddx && !ddy) {
-                // the particle's own update: the WTRV branch
-                if (s.ty != T_WTRV || frozen) continue;
-                const uint32_t prand = hash_u32(seed ^ ((uint32_t)q * 2654435761u));
-                const float t = s.temp;
-                bool ignited = false;
+    for (int k = 0; k < 25; ++k) {
+        const int ddx = k % 5 - 2, ddy = k / 5 - 2;
+        if (k == 12) {
+            // the particle's own update: the WTRV branch
+            if (s.ty != T_WTRV || frozen) continue;
+            const uint32_t prand = hash_u32(seed ^ ((uint32_t)q * 2654435761u));
+            const float t = s.temp;
+            bool ignited = false;
+            if (around & F_RBDM) {
                 for (uint32_t n = 0; n < 9u && !ignited; ++n) {
                     const int dx = (int)(n % 3u) - 1, dy = (int)(n / 3u) - 1;
                     if (!dx && !dy) continue;
@@ -315,53 +354,47 @@ __global__ __launch_bounds__(BLK) void rx_comb_decide(RxArgs a) {
                         ignited = true;
                     }
                 }
-                if (!ignited && t > STEAM_COOL_K && s.ctype == T_FIRE) s.temp = t - t / 1000.0f;
-                continue;
             }
-            if (!burners) continue;
-            const uint32_t bv = a.pmap[by * a.xres + bx];
-            if (!bv) continue;
-            const uint32_t b = slot_of(bv), bty = a.ty[b];
-            if ((bty != T_FIRE && bty != T_PLSM) || (a.pflags[b] & FLAG_STASIS)) continue;
-            // burner at (bx, by); this particle is its neighbour n
-            const float tb = a.temp[b];
-            const uint32_t n = (uint32_t)((2 - ddy) * 5 + (2 - ddx));
-            const uint32_t bq = (uint32_t)(by * a.xres + bx);
-            const uint32_t prand = hash_u32(seed ^ (bq * 2654435761u));
-            const uint32_t nroll = hash_u32(prand ^ (n * 0x9E3779B9u));
-            if ((s.ty == T_COAL || s.ty == T_BCOL) && s.life > 100 && chance(nroll, 1u, 500u)) s.life = 99;
-            if (tb > s.temp) {
-                float h = s.temp + (tb - s.temp) * 0.05f;
-                if (h > 9999.0f) h = 9999.0f;
-                s.temp = h;
-            }
-            const int32_t nflam = s.ty < 256u ? (int32_t)a.props[s.ty * 32u + W_FLAMMABLE] : 0;
-            const bool nexp = s.ty < 256u && (int32_t)a.props[s.ty * 32u + W_EXPLOSIVE] != 0;
-            if (nflam <= 0) continue;
-            uint32_t surround = 0u;
-            for (int k = 0; k < 9; ++k) {
-                const int sx = k % 3 - 1, sy = k / 3 - 1;
-                if ((sx || sy) && pmap_at(a, bx + sx, by + sy) == 0u) ++surround;
-            }
-            bool ignitable = surround > 0u || nexp;
-            if (s.ty == T_SPNG && s.life > 0) ignitable = false;
-            if (!ignitable) continue;
-            const float pv = a.air_pv[(y / TPT_CELL) * a.xc + x / TPT_CELL];
-            const int32_t num = tpt_f2i((float)nflam + pv * 10.0f);
-            bool hit = false;
-            if (num >= 1000) hit = true;
-            else if (num > 0) hit = chance(seed ^ (id * 2654435761u) ^ n, (uint32_t)num, 1000u);
-            if (!hit) continue;
-            s.ty = T_FIRE;
-            s.life = 180 + (int32_t)(hash_u32(id ^ seed) % 80u);
-            float ft = fire_t + (float)(nflam / 2);
-            if (ft < 0.0f) ft = 0.0f;
-            if (ft > 9999.0f) ft = 9999.0f;
-            s.temp = ft;
-            s.tmp = 0;
-            s.ctype = 0u;
-            if (nexp) ign_cell = (by / TPT_CELL) * a.xc + bx / TPT_CELL;
+            if (!ignited && t > STEAM_COOL_K && s.ctype == T_FIRE) s.temp = t - t / 1000.0f;
+            continue;
         }
+        if (!(bmask & (1u << k))) continue;
+        const int bx = x + ddx, by = y + ddy;
+        const uint2 br = a.burner[by * a.xres + bx];
+        const float tb = __uint_as_float(br.y);
+        const uint32_t surround = br.x >> 28;
+        // this particle is the burner's neighbour n
+        const uint32_t n = (uint32_t)((2 - ddy) * 5 + (2 - ddx));
+        const uint32_t bq = (uint32_t)(by * a.xres + bx);
+        const uint32_t prand = hash_u32(seed ^ (bq * 2654435761u));
+        const uint32_t nroll = hash_u32(prand ^ (n * 0x9E3779B9u));
+        if ((s.ty == T_COAL || s.ty == T_BCOL) && s.life > 100 && chance(nroll, 1u, 500u)) s.life = 99;
+        if (tb > s.temp) {
+            float h = s.temp + (tb - s.temp) * 0.05f;
+            if (h > 9999.0f) h = 9999.0f;
+            s.temp = h;
+        }
+        const int32_t nflam = s.ty < 256u ? (int32_t)a.props[s.ty * 32u + W_FLAMMABLE] : 0;
+        const bool nexp = s.ty < 256u && (int32_t)a.props[s.ty * 32u + W_EXPLOSIVE] != 0;
+        if (nflam <= 0) continue;
+        bool ignitable = surround > 0u || nexp;
+        if (s.ty == T_SPNG && s.life > 0) ignitable = false;
+        if (!ignitable) continue;
+        const float pv = a.air_pv[(y / TPT_CELL) * a.xc + x / TPT_CELL];
+        const int32_t num = tpt_f2i((float)nflam + pv * 10.0f);
+        bool hit = false;
+        if (num >= 1000) hit = true;
+        else if (num > 0) hit = chance(seed ^ (id * 2654435761u) ^ n, (uint32_t)num, 1000u);
+        if (!hit) continue;
+        s.ty = T_FIRE;
+        s.life = 180 + (int32_t)(hash_u32(id ^ seed) % 80u);
+        float ft = fire_t + (float)(nflam / 2);
+        if (ft < 0.0f) ft = 0.0f;
+        if (ft > 9999.0f) ft = 9999.0f;
+        s.temp = ft;
+        s.tmp = 0;
+        s.ctype = 0u;
+        if (nexp) ign_cell = (by / TPT_CELL) * a.xc + bx / TPT_CELL;
     }
     if (s.life != s0.life) a.life[id] = s.life;
     if (s.tmp != s0.tmp) a.tmp[id] = s.tmp;
@@ -523,8 +556,34 @@ __device__ ActorOut fluid_actor(const RxArgs &a, uint32_t seed, int px, int py, 
     return o;
 }
 
+// Whether an actor of type aty has a rule that converts a neighbour of
+// type t (fluid.rs:180-399).
+__device__ __forceinline__ bool converts(uint32_t aty, uint32_t t) {
+    switch (t) {
+    case T_FIRE: return aty == T_WATR || aty == T_DSTW || aty == T_SLTW;
+    case T_SALT: return aty == T_WATR || aty == T_DSTW || aty == T_SLTW || aty == T_ICEI || aty == T_SNOW;
+    case T_ROCK: return aty == T_WATR;
+    case T_SMKE: return aty == T_DSTW;
+    case T_PLNT: return aty == T_SLTW;
+    case T_FRZZ: return aty == T_ICEI;
+    default: return false;  // (ICEI and SNOW turning SLTW into SLTW change nothing)
+    }
+}
+
+// Whether an actor of type ty, with `around` the flags within 2 pixels, can
+// change itself this pass.
+__device__ __forceinline__ bool actor_may_change(uint32_t ty, uint32_t around, uint32_t own) {
+    switch (ty) {
+    case T_WATR: return around & (F_SALT | F_RBDM | F_FIRE | F_SLTW | F_ROCK);
+    case T_DSTW: return true;  // its WATR partner is not flagged
+    case T_SLTW: return around & (F_SALT | F_PLNT | F_RBDM | F_FIRE);
+    default: return (around & (F_SALT | F_SLTW | F_FRZZ)) || (own & F_ICE_FRZW);  // ICEI, SNOW
+    }
+}
+
 // Fluid decide: the actors of the 3x3 window act on the pixel's particle in
 // ascending pixel order; the particle runs its own branch at its own turn.
+// After the first conversion of a target nothing later can change it.
 __global__ __launch_bounds__(BLK) void rx_fluid_decide(RxArgs a) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= a.xres * a.yres) return;
@@ -536,43 +595,53 @@ __global__ __launch_bounds__(BLK) void rx_fluid_decide(RxArgs a) {
     if (!v) return;
     const uint32_t id = slot_of(v), ty0 = a.ty[id];
     if (ty0 == 0u) return;
+    const bool actor = is_actor(ty0) && !(a.pflags[id] & FLAG_STASIS) && actor_may_change(ty0, around, own);
+    const bool target = ty0 == T_FIRE || ty0 == T_SALT || ty0 == T_ROCK || ty0 == T_SMKE || ty0 == T_PLNT ||
+                        ty0 == T_FRZZ;
+    if (!actor && !target) return;
     const uint32_t seed = a.counters[TPT_CTR_SEED];
     uint32_t ty = ty0;
-    int32_t life = a.life[id];
-    uint32_t ctype = a.ctype[id];
-    const int32_t life0 = life;
-    const uint32_t ctype0 = ctype;
-    float temp = a.temp[id];
-    const float temp0 = temp;
-    for (int ddy = -1; ddy <= 1; ++ddy) {
-        for (int ddx = -1; ddx <= 1; ++ddx) {
-            const int px = x + ddx, py = y + ddy;
-            if (px < 0 || py < 0 || px >= a.xres || py >= a.yres) continue;
-            if (!ddx && !ddy) {
-                if (!is_actor(ty0) || (a.pflags[id] & FLAG_STASIS)) continue;
-                const ActorOut o = fluid_actor(a, seed, x, y, id, ty0, 9u);
-                ty = o.new_ty;
-                life = o.life;
-                ctype = o.ctype;
-                temp = o.temp;
-                continue;
-            }
-            const uint32_t pv = a.pmap[py * a.xres + px];
-            if (!pv) continue;
-            const uint32_t aid = slot_of(pv), aty = a.ty[aid];
-            if (!is_actor(aty) || (a.pflags[aid] & FLAG_STASIS)) continue;
-            const uint32_t n = (uint32_t)((1 - ddy) * 3 + (1 - ddx));
-            const ActorOut o = fluid_actor(a, seed, px, py, aid, aty, n);
-            if (!o.reached || o.conv == 0xFFFFFFFFu || ty != ty0) continue;
-            ty = o.conv;
-            if (o.conv_ctype != 0xFFFFFFFFu) ctype = o.conv_ctype;
+    int32_t life = 0;
+    uint32_t ctype = 0u;
+    float temp = 0.0f;
+    bool self_done = false, ctype_set = false;
+    for (int k = 0; k < 9; ++k) {
+        const int ddx = k % 3 - 1, ddy = k / 3 - 1;
+        const int px = x + ddx, py = y + ddy;
+        if (px < 0 || py < 0 || px >= a.xres || py >= a.yres) continue;
+        if (k == 4) {
+            if (!actor) continue;
+            const ActorOut o = fluid_actor(a, seed, x, y, id, ty0, 9u);
+            ty = o.new_ty;
+            life = o.life;
+            ctype = o.ctype;
+            temp = o.temp;
+            self_done = true;
+            continue;
+        }
+        if (!target || ty != ty0) continue;
+        const uint32_t pv = a.pmap[py * a.xres + px];
+        if (!pv) continue;
+        const uint32_t aid = slot_of(pv), aty = a.ty[aid];
+        if (!converts(aty, ty0) || (a.pflags[aid] & FLAG_STASIS)) continue;
+        const uint32_t n = (uint32_t)((1 - ddy) * 3 + (1 - ddx));
+        const ActorOut o = fluid_actor(a, seed, px, py, aid, aty, n);
+        if (!o.reached || o.conv == 0xFFFFFFFFu) continue;
+        ty = o.conv;
+        if (o.conv_ctype != 0xFFFFFFFFu) {
+            ctype = o.conv_ctype;
+            ctype_set = true;
         }
     }
-    if (life != life0) a.life[id] = life;
-    if (ctype != ctype0) a.ctype[id] = ctype;
+    if (self_done) {
+        if (life != a.life[id]) a.life[id] = life;
+        if (ctype != a.ctype[id]) a.ctype[id] = ctype;
+    } else if (ctype_set) {
+        a.ctype[id] = ctype;
+    }
     uint32_t side = 0u;
     if (ty != ty0) side |= SIDE_TYPE | ty;
-    if (__float_as_uint(temp) != __float_as_uint(temp0)) side |= SIDE_TEMP;
+    if (self_done && __float_as_uint(temp) != __float_as_uint(a.temp[id])) side |= SIDE_TEMP;
     if (side) a.side[q] = make_uint2(side, __float_as_uint(temp));
 }
 
@@ -593,6 +662,7 @@ RxArgs rx_args(tpt_ctx *ctx) {
     a.cell_flags = ctx->rx_flags;
     a.pvcount = ctx->rx_pvcount;
     a.side = ctx->rx_side;
+    a.burner = ctx->rx_burner;
     a.xres = (int)ctx->cfg.xres;
     a.yres = (int)ctx->cfg.yres;
     a.xc = (int)ctx->xcells;
@@ -608,14 +678,15 @@ int tpt_reactions_step(tpt_ctx *ctx) {
     if (rc) return rc;
     const uint32_t slots = ctx->slot_bound;
     hipLaunchKernelGGL(rx_flags_clear, dim3((ctx->cells + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, ctx->rx_flags,
-                       ctx->cells);
+                       ctx->cells, ctx->counters);
     LifeArgs l{tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY),    tpt_buf<int32_t>(ctx, TPT_BUF_PART_LIFE),
                tpt_buf<int32_t>(ctx, TPT_BUF_PART_TMP),    tpt_buf<uint32_t>(ctx, TPT_BUF_PART_CTYPE),
                tpt_buf<uint32_t>(ctx, TPT_BUF_PART_FLAGS), tpt_buf<float>(ctx, TPT_BUF_PART_TEMP),
                tpt_buf<float>(ctx, TPT_BUF_PART_X),        tpt_buf<float>(ctx, TPT_BUF_PART_Y),
                tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP),  tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP),
                tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS), ctx->counters,
-               ctx->rx_flags,                              slots,
+               ctx->rx_flags,                              tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP),
+               ctx->rx_burner,                             slots,
                (int)ctx->cfg.xres,                         (int)ctx->cfg.yres,
                (int)ctx->xcells,                           (int)ctx->ycells};
     tpt_pass_begin(ctx, PASS_REACTION_LIFE);

@@ -340,7 +340,17 @@ extern "C" int tpt_batch_step_serial(tpt_batch *b, uint64_t first_frame, uint32_
 // the fastest.  Candidates: 1, 2, 4, ... lanes up to the canvas count, and
 // one lane per canvas.  Advances every canvas by 1 + sum(trial_ticks + 1)
 // ticks; results equal tpt_batch_step's for the same number of ticks.
+// Switched off: one run of this entry point on an MI355X ended in a silent
+// abort inside the HIP runtime (no GPU fault reported) while the same test
+// had passed before; it is the only path that captures and destroys several
+// graphs on one batch, reusing the batch's events, and no cause has been
+// shown.  It returns TPT_ESTATE until that is understood.
+constexpr bool kBatchAutotune = false;
+
 extern "C" int tpt_batch_autotune(tpt_batch *b, uint64_t first_frame, uint32_t trial_ticks, uint32_t *lanes_out) {
+    if (!kBatchAutotune)
+        return tpt_set_error(TPT_ESTATE, "tpt_batch_autotune is switched off (an unexplained abort on the GPU); "
+                                         "use tpt_batch_set_lanes");
     if (!b || trial_ticks == 0) return tpt_set_error(TPT_EINVAL, "batch autotune needs a batch and trial ticks");
     int rc = TPT_OK;
     for (tpt_ctx *c : b->ctx)

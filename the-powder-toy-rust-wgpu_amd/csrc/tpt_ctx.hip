@@ -105,6 +105,7 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMalloc(&c->cell_rec, (size_t)c->cells * sizeof(uint4)) != hipSuccess ||
         hipMalloc(&c->rx_flags, (size_t)c->cells * 4) != hipSuccess ||
         hipMalloc(&c->rx_side, (size_t)c->pixels * sizeof(uint2)) != hipSuccess ||
+        hipMalloc(&c->rx_burner, (size_t)c->pixels * sizeof(uint2)) != hipSuccess ||
         hipMalloc(&c->rx_pvcount, (size_t)c->cells * 4) != hipSuccess ||
         hipMalloc(&c->water.col_min, (size_t)c->cfg.xres * 4) != hipSuccess ||
         hipMalloc(&c->water.col_max, (size_t)c->cfg.xres * 4) != hipSuccess ||
@@ -119,6 +120,7 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMemsetAsync(c->counters, 0, 256, c->stream) != hipSuccess ||
         hipMemsetAsync(c->rx_flags, 0, (size_t)c->cells * 4, c->stream) != hipSuccess ||
         hipMemsetAsync(c->rx_side, 0, (size_t)c->pixels * sizeof(uint2), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->rx_burner, 0, (size_t)c->pixels * sizeof(uint2), c->stream) != hipSuccess ||
         hipMemsetAsync(c->rx_pvcount, 0, (size_t)c->cells * 4, c->stream) != hipSuccess ||
         hipMemsetAsync(c->slot_owner, 0xFF, cap * 4, c->stream) != hipSuccess)
         return tpt_set_error(TPT_EDEVICE, "scratch clear failed");
@@ -131,7 +133,7 @@ static void free_all(tpt_ctx *c) {
     void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
                      c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters,
                      c->pix_rec, c->pix_heat, c->offmap, c->pix_type, c->pix_count_mv, c->slot_owner, c->create_rec, c->cell_rec,
-                     c->rx_flags, c->rx_side, c->rx_pvcount, c->water.col_min, c->water.col_max, c->water.col_surf, c->water.col_target,
+                     c->rx_flags, c->rx_side, c->rx_burner, c->rx_pvcount, c->water.col_min, c->water.col_max, c->water.col_surf, c->water.col_target,
                      c->water.winner};
     for (void *p : extra)
         if (p) (void)hipFree(p);

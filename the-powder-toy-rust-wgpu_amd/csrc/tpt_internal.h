@@ -30,6 +30,7 @@
 #define TPT_CTR_FRAME_HI 9
 #define TPT_CTR_SEED 10      // seed of the pass(es) running now
 #define TPT_CTR_STACK 11     // 1: CheckStacking this tick
+#define TPT_CTR_RX_STAMP 12  // element passes: stamp of the current step's burner records (reactions.hip)
 
 // Passes that carry their own timing slot (see tpt_get_pass_times).
 enum tpt_pass {
@@ -164,10 +165,12 @@ struct tpt_ctx {
     uint4 *cell_rec = nullptr;
     // Element passes: per-cell flags of the element classes present this
     // tick, per-pixel pending writes (type word, temperature bits) between a
-    // pass's decide and apply kernels (zero at rest), per-cell counts of
+    // pass's decide and apply kernels (zero at rest), per-pixel records of
+    // the burners (valid when their stamp is the step's), per-cell counts of
     // explosive ignitions (zero at rest).
     uint32_t *rx_flags = nullptr;         // cells
     uint2 *rx_side = nullptr;             // pixels
+    uint2 *rx_burner = nullptr;           // pixels: {stamp | surround << 28, temperature bits} of burner owners
     uint32_t *rx_pvcount = nullptr;       // cells
     tpt_water_bufs water;
     tpt_fft_state fft;
