@@ -453,7 +453,8 @@ struct ActorOut {
     uint32_t conv, conv_ctype;  // conversion of neighbour n_stop (0xFFFFFFFF: none)
 };
 
-__device__ ActorOut fluid_actor(const RxArgs &a, uint32_t seed, int px, int py, uint32_t aid, uint32_t ty, uint32_t n_stop) {
+__device__ __forceinline__ ActorOut fluid_actor(const RxArgs &a, uint32_t seed, int px, int py, uint32_t aid, uint32_t ty,
+                                                uint32_t n_stop) {
     ActorOut o;
     o.new_ty = ty;
     o.life = a.life[aid];
@@ -472,14 +473,21 @@ __device__ ActorOut fluid_actor(const RxArgs &a, uint32_t seed, int px, int py, 
         o.temp = t;
     }
     const uint32_t salt = ty == T_WATR ? SALT_WATR : ty == T_DSTW ? SALT_DSTW : ty == T_SLTW ? SALT_SLTW : SALT_ICE;
+    // the window (up to n_stop), loaded before any rule runs so that the
+    // loads overlap instead of chaining through the rules
+    uint32_t wv[9], wt[9];
+#pragma unroll
+    for (int n = 0; n < 9; ++n)
+        wv[n] = (n == 4 || (uint32_t)n > n_stop) ? 0u : pmap_at(a, px + n % 3 - 1, py + n / 3 - 1);
+#pragma unroll
+    for (int n = 0; n < 9; ++n) wt[n] = wv[n] ? a.ty[slot_of(wv[n])] : 0u;
     bool stop = false;
-    for (uint32_t n = 0; n < 9u && !stop; ++n) {
-        const int dx = (int)(n % 3u) - 1, dy = (int)(n / 3u) - 1;
-        if (!dx && !dy) continue;
-        const uint32_t nv = pmap_at(a, px + dx, py + dy);
-        if (!nv) continue;
+#pragma unroll
+    for (uint32_t n = 0; n < 9u; ++n) {
+        if (n == 4u || !wv[n]) continue;
+        if (stop) break;
         if (n == n_stop) o.reached = true;
-        const uint32_t nid = slot_of(nv), nty = a.ty[nid];
+        const uint32_t nid = slot_of(wv[n]), nty = wt[n];
         const uint32_t r = hash_u32(seed ^ (idx * salt) ^ (n * ROLL_DERIVE));
         uint32_t conv = 0xFFFFFFFFu, conv_ctype = 0xFFFFFFFFu;
         if (ty == T_WATR) {
@@ -605,32 +613,38 @@ __global__ __launch_bounds__(BLK) void rx_fluid_decide(RxArgs a) {
     uint32_t ctype = 0u;
     float temp = 0.0f;
     bool self_done = false, ctype_set = false;
-    for (int k = 0; k < 9; ++k) {
-        const int ddx = k % 3 - 1, ddy = k / 3 - 1;
-        const int px = x + ddx, py = y + ddy;
-        if (px < 0 || py < 0 || px >= a.xres || py >= a.yres) continue;
-        if (k == 4) {
-            if (!actor) continue;
-            const ActorOut o = fluid_actor(a, seed, x, y, id, ty0, 9u);
-            ty = o.new_ty;
-            life = o.life;
-            ctype = o.ctype;
-            temp = o.temp;
-            self_done = true;
-            continue;
-        }
-        if (!target || ty != ty0) continue;
-        const uint32_t pv = a.pmap[py * a.xres + px];
-        if (!pv) continue;
-        const uint32_t aid = slot_of(pv), aty = a.ty[aid];
-        if (!converts(aty, ty0) || (a.pflags[aid] & FLAG_STASIS)) continue;
-        const uint32_t n = (uint32_t)((1 - ddy) * 3 + (1 - ddx));
-        const ActorOut o = fluid_actor(a, seed, px, py, aid, aty, n);
-        if (!o.reached || o.conv == 0xFFFFFFFFu) continue;
-        ty = o.conv;
-        if (o.conv_ctype != 0xFFFFFFFFu) {
-            ctype = o.conv_ctype;
-            ctype_set = true;
+    if (actor) {  // (actor and target types are disjoint)
+        const ActorOut o = fluid_actor(a, seed, x, y, id, ty0, 9u);
+        ty = o.new_ty;
+        life = o.life;
+        ctype = o.ctype;
+        temp = o.temp;
+        self_done = true;
+    } else {
+        // the neighbours that can convert this target, as bits of the window
+        uint32_t wv[9], wt[9], cand = 0u;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) wv[k] = k == 4 ? 0u : pmap_at(a, x + k % 3 - 1, y + k / 3 - 1);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) wt[k] = wv[k] ? a.ty[slot_of(wv[k])] : 0u;
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            if (wv[k] && converts(wt[k], ty0) && !(a.pflags[slot_of(wv[k])] & FLAG_STASIS)) cand |= 1u << k;
+        // in ascending pixel order, until the first conversion
+        while (cand && ty == ty0) {
+            const int k = __ffs(cand) - 1;
+            cand &= cand - 1u;
+            const int ddx = k % 3 - 1, ddy = k / 3 - 1;
+            const int px = x + ddx, py = y + ddy;
+            const uint32_t aid = slot_of(a.pmap[py * a.xres + px]);
+            const uint32_t n = (uint32_t)((1 - ddy) * 3 + (1 - ddx));
+            const ActorOut o = fluid_actor(a, seed, px, py, aid, a.ty[aid], n);
+            if (!o.reached || o.conv == 0xFFFFFFFFu) continue;
+            ty = o.conv;
+            if (o.conv_ctype != 0xFFFFFFFFu) {
+                ctype = o.conv_ctype;
+                ctype_set = true;
+            }
         }
     }
     if (self_done) {
