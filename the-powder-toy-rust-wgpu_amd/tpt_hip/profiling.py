@@ -66,6 +66,8 @@ def main():
     ap.add_argument("--commands", required=True, help="text file with the rocprofv3 commands that were run")
     ap.add_argument("--title", default="Whole tick, 2048x2048 canvas")
     ap.add_argument("--match", nargs="*", default=None, help="keep kernels whose name contains one of these")
+    ap.add_argument("--mfma", help="rocprofv3 database of a --pmc MfmaUtil MfmaFlopsBF16 run: adds MFMA "
+                                   "utilisation and bf16 MFMA FLOP per dispatch to the kernel-time table")
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
 
@@ -73,11 +75,26 @@ def main():
     if a.match:
         trace = [t for t in trace if any(m in t[0] for m in a.match)]
     if not a.fetch:
-        lines = [f"# {a.title}", "", "Command (run from /tmp with TMPDIR=/tmp):", "", "```",
-                 open(a.commands).read().strip(), "```", "",
-                 "| kernel | calls | avg us | total us | % GPU time |", "|---|---|---|---|---|"]
-        for name, calls, total, avg, pct in trace:
-            lines.append(f"| {name} | {calls} | {avg:.1f} | {total:.0f} | {pct:.1f} |")
+        lines = [f"# {a.title}", "", "Commands (run from /tmp with TMPDIR=/tmp):", "", "```",
+                 open(a.commands).read().strip(), "```", ""]
+        if a.mfma:
+            util = counter_means(a.mfma, "MfmaUtil")
+            flops = counter_means(a.mfma, "MfmaFlopsBF16")
+            lines += ["MfmaUtil: SQ_VALU_MFMA_BUSY_CYCLES over GRBM_GUI_ACTIVE x SIMDs, in percent, per dispatch; "
+                      "MfmaFlopsBF16: bf16 MFMA FLOP per dispatch (SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512); "
+                      "TFLOP/s = that over the kernel's average duration.", "",
+                      "| kernel | calls | avg us | total us | % GPU time | MFMA util % | bf16 MFMA GFLOP/dispatch "
+                      "| TFLOP/s |", "|---|---|---|---|---|---|---|---|"]
+            for name, calls, total, avg, pct in trace:
+                u = util.get(name, (0, 0.0))[1]
+                f = flops.get(name, (0, 0.0))[1]
+                tf = f / (avg * 1e-6) / 1e12 if avg > 0 else 0.0
+                lines.append(f"| {name} | {calls} | {avg:.1f} | {total:.0f} | {pct:.1f} | {u:.1f} | {f / 1e9:.2f} "
+                             f"| {tf:.0f} |")
+        else:
+            lines += ["| kernel | calls | avg us | total us | % GPU time |", "|---|---|---|---|---|"]
+            for name, calls, total, avg, pct in trace:
+                lines.append(f"| {name} | {calls} | {avg:.1f} | {total:.0f} | {pct:.1f} |")
         with open(a.out + ".md", "w") as fh:
             fh.write("\n".join(lines) + "\n")
         return

@@ -8,6 +8,7 @@ network against the plain solver, and saves and loads its parameters.
 
     python -m tpt_hip.surrogate train --traj 2000 --epochs 6 --out weights.npz
     python -m tpt_hip.surrogate eval --weights weights.npz --seeds 256
+    python -m tpt_hip.surrogate calls --calls 30      (a workload for rocprofv3)
 
 Scenes: 4-16 Gaussian pressure blobs (amplitude +-30, sigma 4-10 cells),
 0-32 WL_WALL rectangles (sides 2-32 cells) inside a border ring, 0-8 heat
@@ -362,6 +363,25 @@ def eval_main(args):
     ctx.close()
 
 
+def calls_main(args):
+    """`calls` unchecked surrogate calls and as many 8-tick runs of the plain
+    air step on one scene (a workload for rocprofv3)."""
+    xc = yc = args.grid
+    ctx = L.Context(air_config(xc, yc))
+    ctx.load_state(S.SimState(air_config(xc, yc)))
+    sur = Surrogate(xc, yc)
+    sur.load(args.weights)
+    sur.scene(ctx, random_scene(12345, xc, yc))
+    for _ in range(args.calls):
+        sur.step_air(ctx, check=False)
+    ctx.synchronize()
+    for _ in range(args.calls):
+        sur.plain_ticks(ctx, TICKS_PER_CALL)
+    ctx.synchronize()
+    print(f"{args.calls} surrogate calls and {args.calls} x {TICKS_PER_CALL} plain air ticks", flush=True)
+    ctx.close()
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -381,8 +401,12 @@ def main(argv=None):
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
     e.add_argument("--seeds", type=int, default=256)
     e.add_argument("--energy-ratio", type=float, default=1.05)
+    c = sub.add_parser("calls")
+    c.add_argument("--grid", type=int, default=512)
+    c.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    c.add_argument("--calls", type=int, default=30)
     args = ap.parse_args(argv)
-    return train_main(args) if args.cmd == "train" else eval_main(args)
+    return {"train": train_main, "eval": eval_main, "calls": calls_main}[args.cmd](args)
 
 
 if __name__ == "__main__":
