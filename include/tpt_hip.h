@@ -203,13 +203,20 @@ TPT_API int tpt_recalc_free_particles(tpt_ctx *ctx, uint32_t seed, int stacking)
 TPT_API int tpt_step_thermal(tpt_ctx *ctx, uint32_t seed, int legacy);
 
 /* reaction_kernel::step_reactions_gpu (crates/tpt-sim/src/reaction_kernel/mod.rs:1123-1237)
- * as far as this library implements it: the modext clear of the mass
- * override channel and the life pass (life_and_local_reaction_kernel,
- * reaction_kernel/life.rs:47-205: fire/plasma/TRON burn-in lifetimes, fire
- * burning out to smoke or steam, plasma to noble gas or steam, sparks
- * settling into their conductor, the element kill at zero life, the stasis
- * flag).  The combustion, fluid, energy, electronics and other element
- * passes of the reference are not implemented. */
+ * as far as this library implements it, in the driver's order: the modext
+ * clear of the mass override channel (modext/mod.rs:53-99); the life pass
+ * (life_and_local_reaction_kernel, reaction_kernel/life.rs:47-205:
+ * fire/plasma/TRON burn-in lifetimes, fire burning out to smoke or steam,
+ * plasma to noble gas or steam, sparks settling into their conductor, the
+ * element kill at zero life, the stasis flag); the combustion pass's
+ * FIRE/PLSM heating and 5x5 ignition with the COAL/BCOL timer, SPNG and its
+ * WTRV branch (combustion/fire.rs:26-408, 983-1013); the fluid pass's WATR,
+ * DSTW, SLTW, ICEI and SNOW branches (fluid.rs:149-400).  The two racing
+ * passes follow one canonical order (ascending acting pixel, actors decide
+ * from the pass's starting state, a target keeps its first conversion).
+ * The other element branches of these kernels (wood, H2 fusion, yeast, ...)
+ * and the energy, electronics and other element passes are not
+ * implemented. */
 TPT_API int tpt_step_reactions(tpt_ctx *ctx, uint32_t seed);
 
 /* move_kernel::step_movement_gpu (crates/tpt-sim/src/move_kernel/mod.rs:1176-1227),
