@@ -31,15 +31,19 @@
 // in ascending pixel order and composes what the actors around it do to
 // its particle (replaying an actor's neighbour loop where an earlier
 // neighbour can stop it).  A decide kernel reads only the pass's starting
-// state and parks type and temperature changes in rx_side; an apply kernel
-// writes them.  The life kernel also records, per cell, which of the rarer
-// classes are present (fire, salt, rock, ...; not the common water and
-// steam), so that pixels far from them return after reading nine flag
-// words, and, per pixel, each burner owner's temperature and free
-// neighbours, so that the combustion gather finds burners without random
-// reads.
+// state, parks type and temperature changes in rx_side and lists the
+// pixels it changed; an apply kernel writes the listed changes.  The life
+// kernel also records, per cell, which of the rarer classes are present
+// (fire, salt, rock, ...; not the common water and steam), so that a
+// 64x4-pixel tile far from them returns after reading 54 flag words, and,
+// per pixel, each burner owner's temperature and free neighbours, so that
+// the combustion gather finds burners without random reads.  The other
+// tiles stage the pixels within 2 of them in LDS (burner records for the
+// combustion pass, pmap words and types for the fluid pass), which every
+// window of the tile then reads.
 
 #include "tpt_internal.h"
+
 
 namespace {
 
@@ -58,14 +62,14 @@ constexpr uint32_t SALT_WATR = 0x2545F491u, SALT_DSTW = 0x9E3779B9u, SALT_SLTW =
                    SALT_ICE = 0xC2B2AE35u, ROLL_DERIVE = 0x517CC1B7u;
 
 // Per-cell class flags (rx_flags), for the classes the element rules need
-// besides the common water and steam.
+// besides the common water and steam (SMKE and PLNT need none: they change
+// only beside a DSTW or an SLTW, which are flagged).
 constexpr uint32_t F_BURN = 1u << 0,      // FIRE or PLSM outside stasis
     F_WTRV_HOT = 1u << 1,                 // WTRV with ctype FIRE
     F_RBDM = 1u << 3,                     // RBDM or LRBD
     F_DSTW = 1u << 5, F_SLTW = 1u << 6, F_ICE = 1u << 7,  // ICEI or SNOW
     F_ICE_FRZW = 1u << 8,                 // ICEI or SNOW with ctype FRZW
-    F_SALT = 1u << 9, F_FIRE = 1u << 10, F_ROCK = 1u << 11, F_SMKE = 1u << 12, F_PLNT = 1u << 13,
-    F_FRZZ = 1u << 14;
+    F_SALT = 1u << 9, F_FIRE = 1u << 10, F_ROCK = 1u << 11, F_FRZZ = 1u << 14;
 constexpr uint32_t STAMP_MASK = 0x0FFFFFFFu;
 // rx_side.x
 constexpr uint32_t SIDE_TYPE = 0x80000000u, SIDE_TEMP = 0x40000000u;
@@ -106,8 +110,6 @@ __device__ __forceinline__ uint32_t class_flags(uint32_t ty, const uint32_t *cty
     case T_SNOW: return F_ICE | (ctype[i] == T_FRZW ? F_ICE_FRZW : 0u);
     case T_SALT: return F_SALT;
     case T_ROCK: return F_ROCK;
-    case T_SMKE: return F_SMKE;
-    case T_PLNT: return F_PLNT;
     case T_FRZZ: return F_FRZZ;
     default: return 0u;
     }
@@ -136,7 +138,21 @@ struct LifeArgs {
     uint2 *burner;
     uint32_t slots;
     int xres, yres, xc, yc;
+    int stasis;  // 0: no WL_STASIS cell on the canvas, so no particle is frozen
 };
+
+// A particle's pixel (rounded position) and cell; -1 off the canvas.
+__device__ __forceinline__ int life_cell(const LifeArgs &a, uint32_t i, int *xi, int *yi) {
+    *xi = tpt_f2i(a.x[i] + 0.5f);
+    *yi = tpt_f2i(a.y[i] + 0.5f);
+    if (*xi >= 0 && *yi >= 0 && *xi < a.xres && *yi < a.yres) return (*yi / TPT_CELL) * a.xc + *xi / TPT_CELL;
+    return -1;
+}
+
+// The types whose life this pass reads besides the PROP_LIFE_KILL ones.
+__device__ __forceinline__ bool life_type(uint32_t ty) {
+    return ty == T_FIRE || ty == T_PLSM || ty == T_HFLM || ty == T_TRON || ty == T_SPRK;
+}
 
 __global__ __launch_bounds__(BLK) void rx_life(LifeArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -144,12 +160,14 @@ __global__ __launch_bounds__(BLK) void rx_life(LifeArgs a) {
     const uint32_t ty = a.ty[i];
     if (ty == 0u) return;
     const uint32_t seed = a.counters[TPT_CTR_SEED];
-    const int xi = tpt_f2i(a.x[i] + 0.5f), yi = tpt_f2i(a.y[i] + 0.5f);
-    bool frozen = false;
-    int cell = -1;
-    if (xi >= 0 && yi >= 0 && xi < a.xres && yi < a.yres) {
-        cell = (yi / TPT_CELL) * a.xc + xi / TPT_CELL;
-        frozen = a.wall[cell] == 18u && (a.emap[cell] & 0xFFu) < 8u;
+    // Position and wall are read only where they matter: the stasis test
+    // (canvases with a WL_STASIS cell) and the flagged classes below.
+    int xi = 0, yi = 0, cell = -1;
+    bool frozen = false, located = false;
+    if (a.stasis) {
+        cell = life_cell(a, i, &xi, &yi);
+        located = true;
+        if (cell >= 0) frozen = a.wall[cell] == 18u && (a.emap[cell] & 0xFFu) < 8u;
     }
     const uint32_t flags = a.flags[i];
     if (frozen) {
@@ -160,6 +178,14 @@ __global__ __launch_bounds__(BLK) void rx_life(LifeArgs a) {
     }
     if (flags & FLAG_STASIS) a.flags[i] = flags & ~FLAG_STASIS;
     const uint32_t props = ty < 256u ? a.props[ty * 32u + 1u] : 0u;
+    if (!life_type(ty) && !(props & PROP_LIFE_KILL)) {
+        // no life rule: only the class flags (life and type unchanged)
+        const uint32_t f = class_flags(ty, a.ctype, i, false);
+        if (!f) return;
+        if (!located) cell = life_cell(a, i, &xi, &yi);
+        if (cell >= 0) atomicOr(a.cell_flags + cell, f);
+        return;
+    }
     const int32_t life0 = a.life[i];
     int32_t life = life0;
     uint32_t new_ty = ty;
@@ -218,9 +244,11 @@ __global__ __launch_bounds__(BLK) void rx_life(LifeArgs a) {
     }
     if (new_ty != ty) a.ty[i] = new_ty;
     if (life != life0) a.life[i] = life;
-    if (cell < 0) return;
     const uint32_t f = class_flags(new_ty, a.ctype, i, false);
-    if (f) atomicOr(a.cell_flags + cell, f);
+    if (!f) return;
+    if (!located) cell = life_cell(a, i, &xi, &yi);
+    if (cell < 0) return;
+    atomicOr(a.cell_flags + cell, f);
     // a burner that owns its pixel: its temperature and free neighbours
     if ((f & F_BURN) && (a.pmap[yi * a.xres + xi] >> 9) == i + 1u) {
         uint32_t surround = 0u;
@@ -243,10 +271,12 @@ struct RxArgs {
     float *temp;
     const float *vx, *vy;
     float *air_pv;
-    const uint32_t *props, *counters;
+    const uint32_t *props;
+    uint32_t *counters;
     uint32_t *cell_flags, *pvcount;
     uint2 *side;
     const uint2 *burner;
+    uint32_t *list, *wcount;
     int xres, yres, xc, yc;
     int legacy;
 };
@@ -255,34 +285,89 @@ __device__ __forceinline__ uint32_t pmap_at(const RxArgs &a, int x, int y) {
     return (x < 0 || y < 0 || x >= a.xres || y >= a.yres) ? 0u : a.pmap[y * a.xres + x];
 }
 
-// OR of the class flags of the 3x3 cells around the cell of (x, y): every
-// pixel within 2 of (x, y) lies in one of them.
-__device__ __forceinline__ uint32_t flags_around(const RxArgs &a, int x, int y, uint32_t *own) {
-    const int cx = x / TPT_CELL, cy = y / TPT_CELL;
-    uint32_t f = 0u;
-    for (int dy = -1; dy <= 1; ++dy) {
-        const int yy = cy + dy;
-        if (yy < 0 || yy >= a.yc) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int xx = cx + dx;
-            if (xx < 0 || xx >= a.xc) continue;
-            const uint32_t v = a.cell_flags[yy * a.xc + xx];
-            f |= v;
-            if (!dx && !dy) *own = v;
-        }
-    }
-    return f;
-}
-
 __device__ __forceinline__ bool combustion_relevant(uint32_t around, uint32_t own) {
     return (around & (F_BURN | F_RBDM)) || (own & F_WTRV_HOT);
 }
 
-// Every fluid rule involves at least one flagged class within 2 pixels
-// (WATR's partners, or a DSTW, SLTW or ice actor).
+// Whether a fluid rule can change the pixel's particle: every rule pairs an
+// actor with a neighbour within 1 pixel, and at least one of the two is of
+// a flagged class.  WATR's partners (salt, rubidium, fire, salt water,
+// rock) and the DSTW and SLTW actors are flagged; SMKE changes only beside
+// a DSTW and PLNT only beside an SLTW, both flagged already; ice changes or
+// converts frost only with salt, salt water or FRZZ around (or when it
+// holds FRZW itself).
 __device__ __forceinline__ bool fluid_relevant(uint32_t around, uint32_t own) {
-    return (around & (F_SALT | F_RBDM | F_FIRE | F_SLTW | F_ROCK | F_SMKE | F_PLNT | F_FRZZ | F_DSTW | F_ICE)) ||
-           (own & F_ICE_FRZW);
+    return (around & (F_SALT | F_RBDM | F_FIRE | F_SLTW | F_ROCK | F_DSTW)) ||
+           ((around & F_FRZZ) && (around & F_ICE)) || (own & F_ICE_FRZW);
+}
+
+// The decide kernels work on 64x4-pixel tiles (one cell row, 16 cells) with
+// 256 threads.  A tile first reads the class flags of the 18x3 cells around
+// it into LDS: every pixel within 2 of the tile lies in one of them.  A
+// tile none of whose pixels can be acted on returns there; the others stage
+// what their windows read (pixels within 2 of the tile) in LDS, so that the
+// windows of neighbouring threads, which overlap, are loaded once.
+constexpr int TW = 64, TH = TPT_CELL, HALO = 2, SW = TW + 2 * HALO, SH = TH + 2 * HALO, STAGED = SW * SH;
+constexpr int CFW = TW / TPT_CELL + 2;  // 18 cells x 3 rows
+static_assert(TW * TH == BLK, "one thread per tile pixel");
+
+struct TileGeom {
+    int x0, y0;  // tile origin (px)
+    int x, y;    // this thread's pixel
+    int lx, ly;
+    bool inb;
+};
+
+__device__ __forceinline__ TileGeom tile_geom(const RxArgs &a) {
+    TileGeom g;
+    const int tiles_x = (a.xres + TW - 1) / TW;
+    g.x0 = (int)(blockIdx.x % (unsigned)tiles_x) * TW;
+    g.y0 = (int)(blockIdx.x / (unsigned)tiles_x) * TH;
+    g.lx = (int)(threadIdx.x % TW);
+    g.ly = (int)(threadIdx.x / TW);
+    g.x = g.x0 + g.lx;
+    g.y = g.y0 + g.ly;
+    g.inb = g.x < a.xres && g.y < a.yres;
+    return g;
+}
+
+// Stages the class flags around the tile and returns, for this thread's
+// pixel, the OR of the 3x3 cells around its cell (*own: its cell's flags).
+__device__ __forceinline__ uint32_t tile_flags(const RxArgs &a, const TileGeom &g, uint32_t *s_cf, uint32_t *own) {
+    const int t = (int)threadIdx.x;
+    if (t < CFW * 3) {
+        const int cx = g.x0 / TPT_CELL - 1 + t % CFW, cy = g.y0 / TPT_CELL - 1 + t / CFW;
+        s_cf[t] = (cx >= 0 && cy >= 0 && cx < a.xc && cy < a.yc) ? a.cell_flags[cy * a.xc + cx] : 0u;
+    }
+    __syncthreads();
+    const int c = g.lx / TPT_CELL;
+    uint32_t f = 0u;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) f |= s_cf[r * CFW + c + d];
+    *own = s_cf[CFW + c + 1];
+    return f;
+}
+
+// Pixel (gx, gy) of the staged region, as an LDS index (gx, gy within 2 of
+// the tile).
+__device__ __forceinline__ int st_idx(const TileGeom &g, int gx, int gy) {
+    return (gy - g.y0 + HALO) * SW + (gx - g.x0 + HALO);
+}
+
+// Records `q` for each lane that wants to in this wave's own segment of the
+// changed-pixel list (64 entries at wave slot 4 * tile + wave) and the
+// segment's length.  No atomics: a same-address counter shared by every
+// wave of the grid serialises them.  A wave that records nothing leaves its
+// length at 0, where the apply kernel left it.
+__device__ __forceinline__ void wave_record(uint32_t *list, uint32_t *wcount, bool want, uint32_t q) {
+    const uint64_t m = __ballot(want);
+    if (!m) return;
+    const uint32_t slot = blockIdx.x * (BLK / 64) + threadIdx.x / 64;
+    const int lane = (int)__lane_id();
+    if (want) list[slot * 64u + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = q;
+    if (lane == __ffsll((unsigned long long)m) - 1) wcount[slot] = (uint32_t)__popcll(m);
 }
 
 // The particle a gather thread composes.
@@ -296,24 +381,33 @@ struct Subject {
 // Combustion decide: the FIRE/PLSM burners of the 5x5 window act on the
 // pixel's particle in ascending pixel order (fire.rs:143-408), the particle
 // runs its own WTRV branch (fire.rs:983-1013) at its own pixel's turn.
-// Burners come from this step's burner records.
+// Burners come from this step's burner records, staged per tile.
 __global__ __launch_bounds__(BLK) void rx_comb_decide(RxArgs a) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.xres * a.yres) return;
-    const int x = q % a.xres, y = q / a.xres;
+    __shared__ uint32_t s_cf[CFW * 3];
+    __shared__ uint2 s_br[STAGED];
+    const TileGeom g = tile_geom(a);
     uint32_t own = 0u;
-    const uint32_t around = flags_around(a, x, y, &own);
-    if (!combustion_relevant(around, own)) return;
+    const uint32_t around = tile_flags(a, g, s_cf, &own);
+    const bool rel = g.inb && combustion_relevant(around, own);
+    if (!__syncthreads_or(rel ? 1 : 0)) return;
+    const uint32_t stamp = a.counters[TPT_CTR_RX_STAMP];
+    for (int e = (int)threadIdx.x; e < STAGED; e += BLK) {
+        const int gx = g.x0 - HALO + e % SW, gy = g.y0 - HALO + e / SW;
+        s_br[e] = (gx >= 0 && gy >= 0 && gx < a.xres && gy < a.yres) ? a.burner[gy * a.xres + gx] : make_uint2(0u, 0u);
+    }
+    __syncthreads();
+    if (!rel) return;
+    const int x = g.x, y = g.y, q = y * a.xres + x;
     const uint32_t v = a.pmap[q];
     if (!v) return;
-    const uint32_t stamp = a.counters[TPT_CTR_RX_STAMP];
     // the burners of the window, as bits of the 25 positions
     uint32_t bmask = 0u;
     if (around & F_BURN) {
+#pragma unroll
         for (int k = 0; k < 25; ++k) {
             const int bx = x + k % 5 - 2, by = y + k / 5 - 2;
             if (k == 12 || bx < 0 || by < 0 || bx >= a.xres || by >= a.yres) continue;
-            if ((a.burner[by * a.xres + bx].x & STAMP_MASK) == stamp) bmask |= 1u << k;
+            if ((s_br[st_idx(g, bx, by)].x & STAMP_MASK) == stamp) bmask |= 1u << k;
         }
     }
     const uint32_t id = slot_of(v);
@@ -360,7 +454,7 @@ __global__ __launch_bounds__(BLK) void rx_comb_decide(RxArgs a) {
         }
         if (!(bmask & (1u << k))) continue;
         const int bx = x + ddx, by = y + ddy;
-        const uint2 br = a.burner[by * a.xres + bx];
+        const uint2 br = s_br[st_idx(g, bx, by)];
         const float tb = __uint_as_float(br.y);
         const uint32_t surround = br.x >> 28;
         // this particle is the burner's neighbour n
@@ -405,34 +499,36 @@ __global__ __launch_bounds__(BLK) void rx_comb_decide(RxArgs a) {
     if (side) a.side[q] = make_uint2(side, __float_as_uint(s.temp));
     if (s.ty == T_FIRE && ty0 != T_FIRE) atomicOr(a.cell_flags + (y / TPT_CELL) * a.xc + x / TPT_CELL, F_FIRE);
     if (ign_cell >= 0) atomicAdd(a.pvcount + ign_cell, 1u);
+    wave_record(a.list, a.wcount, side != 0u, (uint32_t)q);
 }
 
-// Applies parked type/temperature writes of the pixels `relevant` selects,
-// and (for the combustion pass) each burner cell's explosive pressure.
-template <int PASS>
-__global__ __launch_bounds__(BLK) void rx_apply(RxArgs a) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.xres * a.yres) return;
-    const int x = q % a.xres, y = q / a.xres;
-    uint32_t own = 0u;
-    const uint32_t around = flags_around(a, x, y, &own);
-    if (PASS == 0 && (own & F_BURN) && x % TPT_CELL == 0 && y % TPT_CELL == 0) {
-        const int c = (y / TPT_CELL) * a.xc + x / TPT_CELL;
-        const uint32_t k = a.pvcount[c];
-        if (k) {
-            float pv = a.air_pv[c];
-            for (uint32_t i = 0; i < k; ++i) pv += 0.25f;
-            a.air_pv[c] = pv;
-            a.pvcount[c] = 0u;
-        }
+// Applies the parked type/temperature writes of the pixels the decide
+// kernel recorded (one block per tile, a lane per entry of its waves'
+// segments) and empties the segments; with `with_pv` also adds each of the
+// tile's cells' explosive ignitions to its pressure, 0.25 per ignition.
+__global__ __launch_bounds__(BLK) void rx_apply(RxArgs a, int with_pv) {
+    const uint32_t slot = blockIdx.x * (BLK / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint32_t n = a.wcount[slot];
+    if (lane < n) {
+        const uint32_t q = a.list[slot * 64u + lane];
+        const uint2 sd = a.side[q];
+        const uint32_t id = slot_of(a.pmap[q]);
+        if (sd.x & SIDE_TYPE) a.ty[id] = sd.x & 0x1FFu;
+        if (sd.x & SIDE_TEMP) a.temp[id] = __uint_as_float(sd.y);
+        a.side[q] = make_uint2(0u, 0u);
     }
-    if (!(PASS == 0 ? combustion_relevant(around, own) : fluid_relevant(around, own))) return;
-    const uint2 sd = a.side[q];
-    if (!sd.x) return;
-    const uint32_t id = slot_of(a.pmap[q]);
-    if (sd.x & SIDE_TYPE) a.ty[id] = sd.x & 0x1FFu;
-    if (sd.x & SIDE_TEMP) a.temp[id] = __uint_as_float(sd.y);
-    a.side[q] = make_uint2(0u, 0u);
+    if (n && lane == 0) a.wcount[slot] = 0u;  // (after every lane of the wave has read it)
+    if (!with_pv || threadIdx.x >= TW / TPT_CELL) return;
+    const TileGeom g = tile_geom(a);
+    const int cx = g.x0 / TPT_CELL + (int)threadIdx.x, cy = g.y0 / TPT_CELL;
+    if (cx >= a.xc) return;
+    const int c = cy * a.xc + cx;
+    const uint32_t cnt = a.pvcount[c];
+    if (!cnt) return;
+    float pv = a.air_pv[c];
+    for (uint32_t i = 0; i < cnt; ++i) pv += 0.25f;
+    a.air_pv[c] = pv;
+    a.pvcount[c] = 0u;
 }
 
 __device__ __forceinline__ bool is_actor(uint32_t ty) {
@@ -443,7 +539,8 @@ __device__ __forceinline__ bool is_actor(uint32_t ty) {
 // from the pass's starting state.  With n_stop < 9 it stops before
 // neighbour n_stop and reports, in *conv / *conv_ctype, what the rule does
 // to that neighbour when the actor reaches it (*reached); with n_stop == 9
-// it runs the whole loop and reports the actor's own results.
+// it runs the whole loop and reports the actor's own results.  The window's
+// pmap words and types come from the tile (s_pm, s_ty).
 struct ActorOut {
     uint32_t new_ty;
     int32_t life;
@@ -453,8 +550,9 @@ struct ActorOut {
     uint32_t conv, conv_ctype;  // conversion of neighbour n_stop (0xFFFFFFFF: none)
 };
 
-__device__ __forceinline__ ActorOut fluid_actor(const RxArgs &a, uint32_t seed, int px, int py, uint32_t aid, uint32_t ty,
-                                                uint32_t n_stop) {
+__device__ __forceinline__ ActorOut fluid_actor(const RxArgs &a, const TileGeom &g, const uint32_t *s_pm,
+                                                const uint32_t *s_ty, uint32_t seed, int px, int py, uint32_t aid,
+                                                uint32_t ty, uint32_t n_stop) {
     ActorOut o;
     o.new_ty = ty;
     o.life = a.life[aid];
@@ -473,14 +571,15 @@ __device__ __forceinline__ ActorOut fluid_actor(const RxArgs &a, uint32_t seed, 
         o.temp = t;
     }
     const uint32_t salt = ty == T_WATR ? SALT_WATR : ty == T_DSTW ? SALT_DSTW : ty == T_SLTW ? SALT_SLTW : SALT_ICE;
-    // the window (up to n_stop), loaded before any rule runs so that the
-    // loads overlap instead of chaining through the rules
+    // the window (up to n_stop); off-canvas entries are staged as empty
+    const int c0 = st_idx(g, px, py);
     uint32_t wv[9], wt[9];
 #pragma unroll
-    for (int n = 0; n < 9; ++n)
-        wv[n] = (n == 4 || (uint32_t)n > n_stop) ? 0u : pmap_at(a, px + n % 3 - 1, py + n / 3 - 1);
-#pragma unroll
-    for (int n = 0; n < 9; ++n) wt[n] = wv[n] ? a.ty[slot_of(wv[n])] : 0u;
+    for (int n = 0; n < 9; ++n) {
+        const int e = c0 + (n / 3 - 1) * SW + (n % 3 - 1);
+        wv[n] = (n == 4 || (uint32_t)n > n_stop) ? 0u : s_pm[e];
+        wt[n] = wv[n] ? s_ty[e] : 0u;
+    }
     bool stop = false;
 #pragma unroll
     for (uint32_t n = 0; n < 9u; ++n) {
@@ -584,26 +683,50 @@ __device__ __forceinline__ bool actor_may_change(uint32_t ty, uint32_t around, u
     switch (ty) {
     case T_WATR: return around & (F_SALT | F_RBDM | F_FIRE | F_SLTW | F_ROCK);
     case T_DSTW: return true;  // its WATR partner is not flagged
-    case T_SLTW: return around & (F_SALT | F_PLNT | F_RBDM | F_FIRE);
+    case T_SLTW: return around & (F_SALT | F_RBDM | F_FIRE);  // (PLNT and SALT change only the neighbour)
     default: return (around & (F_SALT | F_SLTW | F_FRZZ)) || (own & F_ICE_FRZW);  // ICEI, SNOW
     }
 }
 
 // Fluid decide: the actors of the 3x3 window act on the pixel's particle in
 // ascending pixel order; the particle runs its own branch at its own turn.
-// After the first conversion of a target nothing later can change it.
+// After the first conversion of a target nothing later can change it.  The
+// tile stages the pmap words and the types of every pixel within 2 of it:
+// an actor's window, and the windows of the actors next to a target.
 __global__ __launch_bounds__(BLK) void rx_fluid_decide(RxArgs a) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.xres * a.yres) return;
-    const int x = q % a.xres, y = q / a.xres;
+    __shared__ uint32_t s_cf[CFW * 3];
+    __shared__ uint32_t s_pm[STAGED], s_ty[STAGED];
+    const TileGeom g = tile_geom(a);
     uint32_t own = 0u;
-    const uint32_t around = flags_around(a, x, y, &own);
-    if (!fluid_relevant(around, own)) return;
-    const uint32_t v = a.pmap[q];
+    const uint32_t around = tile_flags(a, g, s_cf, &own);
+    const bool rel = g.inb && fluid_relevant(around, own);
+    if (!__syncthreads_or(rel ? 1 : 0)) return;
+    {
+        uint32_t pm[(STAGED + BLK - 1) / BLK];
+#pragma unroll
+        for (int r = 0; r < (STAGED + BLK - 1) / BLK; ++r) {
+            const int e = (int)threadIdx.x + r * BLK;
+            const int gx = g.x0 - HALO + e % SW, gy = g.y0 - HALO + e / SW;
+            pm[r] = (e < STAGED && gx >= 0 && gy >= 0 && gx < a.xres && gy < a.yres) ? a.pmap[gy * a.xres + gx] : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < (STAGED + BLK - 1) / BLK; ++r) {
+            const int e = (int)threadIdx.x + r * BLK;
+            if (e < STAGED) {
+                s_pm[e] = pm[r];
+                s_ty[e] = pm[r] ? a.ty[slot_of(pm[r])] : 0u;
+            }
+        }
+    }
+    __syncthreads();
+    if (!rel) return;
+    const int x = g.x, y = g.y, q = y * a.xres + x;
+    const int me = st_idx(g, x, y);
+    const uint32_t v = s_pm[me];
     if (!v) return;
-    const uint32_t id = slot_of(v), ty0 = a.ty[id];
+    const uint32_t id = slot_of(v), ty0 = s_ty[me];
     if (ty0 == 0u) return;
-    const bool actor = is_actor(ty0) && !(a.pflags[id] & FLAG_STASIS) && actor_may_change(ty0, around, own);
+    const bool actor = is_actor(ty0) && actor_may_change(ty0, around, own) && !(a.pflags[id] & FLAG_STASIS);
     const bool target = ty0 == T_FIRE || ty0 == T_SALT || ty0 == T_ROCK || ty0 == T_SMKE || ty0 == T_PLNT ||
                         ty0 == T_FRZZ;
     if (!actor && !target) return;
@@ -614,7 +737,7 @@ __global__ __launch_bounds__(BLK) void rx_fluid_decide(RxArgs a) {
     float temp = 0.0f;
     bool self_done = false, ctype_set = false;
     if (actor) {  // (actor and target types are disjoint)
-        const ActorOut o = fluid_actor(a, seed, x, y, id, ty0, 9u);
+        const ActorOut o = fluid_actor(a, g, s_pm, s_ty, seed, x, y, id, ty0, 9u);
         ty = o.new_ty;
         life = o.life;
         ctype = o.ctype;
@@ -622,23 +745,29 @@ __global__ __launch_bounds__(BLK) void rx_fluid_decide(RxArgs a) {
         self_done = true;
     } else {
         // the neighbours that can convert this target, as bits of the window
-        uint32_t wv[9], wt[9], cand = 0u;
+        uint32_t cand = 0u;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) wv[k] = k == 4 ? 0u : pmap_at(a, x + k % 3 - 1, y + k / 3 - 1);
-#pragma unroll
-        for (int k = 0; k < 9; ++k) wt[k] = wv[k] ? a.ty[slot_of(wv[k])] : 0u;
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-            if (wv[k] && converts(wt[k], ty0) && !(a.pflags[slot_of(wv[k])] & FLAG_STASIS)) cand |= 1u << k;
+        for (int k = 0; k < 9; ++k) {
+            if (k == 4) continue;
+            const int e = me + (k / 3 - 1) * SW + (k % 3 - 1);
+            if (s_pm[e] && converts(s_ty[e], ty0)) cand |= 1u << k;
+        }
+        uint32_t c2 = cand;
+        while (c2) {
+            const int k = __ffs(c2) - 1;
+            c2 &= c2 - 1u;
+            const int e = me + (k / 3 - 1) * SW + (k % 3 - 1);
+            if (a.pflags[slot_of(s_pm[e])] & FLAG_STASIS) cand &= ~(1u << k);
+        }
         // in ascending pixel order, until the first conversion
         while (cand && ty == ty0) {
             const int k = __ffs(cand) - 1;
             cand &= cand - 1u;
             const int ddx = k % 3 - 1, ddy = k / 3 - 1;
-            const int px = x + ddx, py = y + ddy;
-            const uint32_t aid = slot_of(a.pmap[py * a.xres + px]);
+            const int e = me + ddy * SW + ddx;
+            const uint32_t aid = slot_of(s_pm[e]);
             const uint32_t n = (uint32_t)((1 - ddy) * 3 + (1 - ddx));
-            const ActorOut o = fluid_actor(a, seed, px, py, aid, a.ty[aid], n);
+            const ActorOut o = fluid_actor(a, g, s_pm, s_ty, seed, x + ddx, y + ddy, aid, s_ty[e], n);
             if (!o.reached || o.conv == 0xFFFFFFFFu) continue;
             ty = o.conv;
             if (o.conv_ctype != 0xFFFFFFFFu) {
@@ -657,6 +786,7 @@ __global__ __launch_bounds__(BLK) void rx_fluid_decide(RxArgs a) {
     if (ty != ty0) side |= SIDE_TYPE | ty;
     if (self_done && __float_as_uint(temp) != __float_as_uint(a.temp[id])) side |= SIDE_TEMP;
     if (side) a.side[q] = make_uint2(side, __float_as_uint(temp));
+    wave_record(a.list, a.wcount, side != 0u, (uint32_t)q);
 }
 
 RxArgs rx_args(tpt_ctx *ctx) {
@@ -677,6 +807,8 @@ RxArgs rx_args(tpt_ctx *ctx) {
     a.pvcount = ctx->rx_pvcount;
     a.side = ctx->rx_side;
     a.burner = ctx->rx_burner;
+    a.list = ctx->rx_list;
+    a.wcount = ctx->rx_wcount;
     a.xres = (int)ctx->cfg.xres;
     a.yres = (int)ctx->cfg.yres;
     a.xc = (int)ctx->xcells;
@@ -702,21 +834,23 @@ int tpt_reactions_step(tpt_ctx *ctx) {
                ctx->rx_flags,                              tpt_buf<uint32_t>(ctx, TPT_BUF_PMAP),
                ctx->rx_burner,                             slots,
                (int)ctx->cfg.xres,                         (int)ctx->cfg.yres,
-               (int)ctx->xcells,                           (int)ctx->ycells};
+               (int)ctx->xcells,                           (int)ctx->ycells,
+               ctx->has_stasis ? 1 : 0};
     tpt_pass_begin(ctx, PASS_REACTION_LIFE);
     hipLaunchKernelGGL(rx_life, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream, l);
     tpt_pass_end(ctx, PASS_REACTION_LIFE);
     if ((rc = tpt_check_launch("rx_life"))) return rc;
     const RxArgs a = rx_args(ctx);
-    const dim3 grid((ctx->pixels + BLK - 1) / BLK);
+    const uint32_t tiles = ctx->rx_tiles;  // ((xres + 63) / 64) * (yres / 4)
+    static_assert(TW == 64 && TH == 4, "rx_tiles counts 64x4 tiles");
     tpt_pass_begin(ctx, PASS_REACTION_COMBUSTION);
-    hipLaunchKernelGGL(rx_comb_decide, grid, dim3(BLK), 0, ctx->stream, a);
-    hipLaunchKernelGGL(rx_apply<0>, grid, dim3(BLK), 0, ctx->stream, a);
+    hipLaunchKernelGGL(rx_comb_decide, dim3(tiles), dim3(BLK), 0, ctx->stream, a);
+    hipLaunchKernelGGL(rx_apply, dim3(tiles), dim3(BLK), 0, ctx->stream, a, 1);
     tpt_pass_end(ctx, PASS_REACTION_COMBUSTION);
     if ((rc = tpt_check_launch("rx_combustion"))) return rc;
     tpt_pass_begin(ctx, PASS_REACTION_FLUID);
-    hipLaunchKernelGGL(rx_fluid_decide, grid, dim3(BLK), 0, ctx->stream, a);
-    hipLaunchKernelGGL(rx_apply<1>, grid, dim3(BLK), 0, ctx->stream, a);
+    hipLaunchKernelGGL(rx_fluid_decide, dim3(tiles), dim3(BLK), 0, ctx->stream, a);
+    hipLaunchKernelGGL(rx_apply, dim3(tiles), dim3(BLK), 0, ctx->stream, a, 0);
     tpt_pass_end(ctx, PASS_REACTION_FLUID);
     // Positions are unchanged: the pixel records still cover every slot
     // (types and temperatures are read from the slot arrays downstream).

@@ -746,7 +746,7 @@ extern "C" int tpt_sur_scene(tpt_sur *s, tpt_ctx *ctx, const tpt_sur_scene_desc 
                        tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_BLOCK_AIR),
                        tpt_buf<uint32_t>(ctx, TPT_BUF_BLOCK_AIRH), s->src, s->W, s->H, s->ambient);
     ctx->maps_fresh = ctx->rec_cover = ctx->rebuild_fresh = false;
-    ctx->has_wire = false;
+    ctx->has_wire = ctx->has_stasis = false;
     ctx->epoch++;
     return tpt_check_launch("surrogate scene");
 }

@@ -4,10 +4,10 @@
 // tpt_step follows SimulationEngine::step (crates/tpt-sim/src/lib.rs:303-459)
 // for the passes this library implements: air -> Newtonian gravity (or the
 // force-grid clear) -> RecalcFreeParticles (+ CheckStacking every 10th
-// frame) -> conduction/transitions -> the element reactions' life pass
-// (reaction_kernel::step_reactions_gpu, lib.rs:379; its other passes are
-// not part of this library) -> movement -> water equalisation (when
-// enabled).
+// frame) -> conduction/transitions -> the element passes
+// (reaction_kernel::step_reactions_gpu, lib.rs:379: the mass-override clear,
+// the life, combustion and fluid passes for the elements this library
+// covers) -> movement -> water equalisation (when enabled).
 //
 // The frame counter, the tick seed and the CheckStacking flag live in the
 // context's device counter block, so that a captured tick replays with the
@@ -51,8 +51,7 @@ int tick(tpt_ctx *ctx, uint64_t frame, bool from_device) {
     if ((rc = tpt_gravity_step(ctx, 0u))) return rc;
     if ((rc = tpt_recalc_step(ctx, !from_device && (frame % 10u) == 0u))) return rc;
     if ((rc = tpt_thermal_step(ctx, ctx->cfg.heat_enable ? 0 : 1))) return rc;
-    // The element pass (step_reactions_gpu, lib.rs:379): of it this
-    // library runs the mass-override clear and the life pass.
+    // The element passes (step_reactions_gpu, lib.rs:379).
     if ((rc = tpt_reactions_step(ctx))) return rc;
     if ((rc = tpt_movement_step(ctx))) return rc;
     return ctx->cfg.water_equal_test ? tpt_water_equal_step(ctx) : TPT_OK;

@@ -92,6 +92,7 @@ static int alloc_buffers(tpt_ctx *c) {
     if (hipMemsetAsync(c->buf[TPT_BUF_INTENT_TARGET], 0xFF, cap * 4, c->stream) != hipSuccess)
         return tpt_set_error(TPT_EDEVICE, "intent clear failed");
     size_t cb = (size_t)c->cells * 4;
+    c->rx_tiles = ((c->cfg.xres + 63u) / 64u) * (c->cfg.yres / 4u);  // the element passes' 64x4 tiles
     if (hipMalloc(&c->cell_scratch_u, cb) != hipSuccess || hipMalloc(&c->cell_scratch_f, cb) != hipSuccess ||
         hipMalloc(&c->cell_scratch_u2, cb) != hipSuccess || hipMalloc(&c->cell_scratch_f2, cb) != hipSuccess ||
         hipMalloc(&c->slot_scratch_u, cap * 4) != hipSuccess || hipMalloc(&c->slot_scratch_u2, cap * 4) != hipSuccess ||
@@ -107,6 +108,8 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMalloc(&c->rx_side, (size_t)c->pixels * sizeof(uint2)) != hipSuccess ||
         hipMalloc(&c->rx_burner, (size_t)c->pixels * sizeof(uint2)) != hipSuccess ||
         hipMalloc(&c->rx_pvcount, (size_t)c->cells * 4) != hipSuccess ||
+        hipMalloc(&c->rx_list, (size_t)c->rx_tiles * 256 * 4) != hipSuccess ||
+        hipMalloc(&c->rx_wcount, (size_t)c->rx_tiles * 4 * 4) != hipSuccess ||
         hipMalloc(&c->water.col_min, (size_t)c->cfg.xres * 4) != hipSuccess ||
         hipMalloc(&c->water.col_max, (size_t)c->cfg.xres * 4) != hipSuccess ||
         hipMalloc(&c->water.col_surf, (size_t)c->cfg.xres * 4) != hipSuccess ||
@@ -122,6 +125,7 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMemsetAsync(c->rx_side, 0, (size_t)c->pixels * sizeof(uint2), c->stream) != hipSuccess ||
         hipMemsetAsync(c->rx_burner, 0, (size_t)c->pixels * sizeof(uint2), c->stream) != hipSuccess ||
         hipMemsetAsync(c->rx_pvcount, 0, (size_t)c->cells * 4, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->rx_wcount, 0, (size_t)c->rx_tiles * 4 * 4, c->stream) != hipSuccess ||
         hipMemsetAsync(c->slot_owner, 0xFF, cap * 4, c->stream) != hipSuccess)
         return tpt_set_error(TPT_EDEVICE, "scratch clear failed");
     return TPT_OK;
@@ -133,7 +137,7 @@ static void free_all(tpt_ctx *c) {
     void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
                      c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters,
                      c->pix_rec, c->pix_heat, c->offmap, c->pix_type, c->pix_count_mv, c->slot_owner, c->create_rec, c->cell_rec,
-                     c->rx_flags, c->rx_side, c->rx_burner, c->rx_pvcount, c->water.col_min, c->water.col_max, c->water.col_surf, c->water.col_target,
+                     c->rx_flags, c->rx_side, c->rx_burner, c->rx_pvcount, c->rx_list, c->rx_wcount, c->water.col_min, c->water.col_max, c->water.col_surf, c->water.col_target,
                      c->water.winner};
     for (void *p : extra)
         if (p) (void)hipFree(p);
@@ -233,15 +237,18 @@ extern "C" int tpt_upload(tpt_ctx *ctx, int buf, const void *src, size_t off, si
     ctx->epoch++;
     if (buf == TPT_BUF_WALL_BMAP || buf == TPT_BUF_WALL_EMAP) {
         const uint32_t *w = (const uint32_t *)src;
-        bool found = false;
-        for (size_t k = 0; k < bytes / 4 && !found; ++k) {
+        bool found = false, stasis = false;
+        for (size_t k = 0; k < bytes / 4; ++k) {
             const uint32_t v = w[k];
-            found = buf == TPT_BUF_WALL_BMAP
-                        ? (v == 1u || v == 2u || v == 3u || v == 5u || v == 6u || v == 12u || v == 18u)
-                        : (v & 0xC0u) != 0u;
+            found = found || (buf == TPT_BUF_WALL_BMAP
+                                  ? (v == 1u || v == 2u || v == 3u || v == 5u || v == 6u || v == 12u || v == 18u)
+                                  : (v & 0xC0u) != 0u);
+            stasis = stasis || (buf == TPT_BUF_WALL_BMAP && v == 18u);
         }
+        const bool whole = off == 0 && bytes == ctx->buf_bytes[buf];
         bool &flag = buf == TPT_BUF_WALL_BMAP ? ctx->has_wire : ctx->emap_markers;
-        flag = (off == 0 && bytes == ctx->buf_bytes[buf]) ? found : (flag || found);
+        flag = whole ? found : (flag || found);
+        if (buf == TPT_BUF_WALL_BMAP) ctx->has_stasis = whole ? stasis : (ctx->has_stasis || stasis);
     }
     TPT_HIP_CHECK(hipMemcpyAsync((char *)ctx->buf[buf] + off, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     TPT_HIP_CHECK(hipStreamSynchronize(ctx->stream));

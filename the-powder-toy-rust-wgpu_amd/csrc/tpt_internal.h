@@ -135,6 +135,7 @@ struct tpt_ctx {
     // wall_bmap): the canvas has conductor walls; an uploaded emap carries
     // span-end marker bits.
     bool has_wire = false;
+    bool has_stasis = false;  // some wall cell may be WL_STASIS (18)
     bool emap_markers = false;
     // The gravity tool grid / override channel may hold non-zero cells:
     // set by their uploads, cleared once the gravity step consumed the tool
@@ -172,6 +173,9 @@ struct tpt_ctx {
     uint2 *rx_side = nullptr;             // pixels
     uint2 *rx_burner = nullptr;           // pixels: {stamp | surround << 28, temperature bits} of burner owners
     uint32_t *rx_pvcount = nullptr;       // cells
+    uint32_t *rx_list = nullptr;          // 64 per wave of a 64x4 tile: changed pixels of the current element pass
+    uint32_t *rx_wcount = nullptr;        // per wave of a 64x4 tile: entries in its rx_list segment
+    uint32_t rx_tiles = 0;                // 64x4-pixel tiles of the canvas
     tpt_water_bufs water;
     tpt_fft_state fft;
 
