@@ -249,10 +249,13 @@ TPT_API int tpt_run(tpt_ctx *ctx, uint64_t first_frame, uint32_t ticks);
 /* The batched form (no counterpart in the reference, which steps one
  * canvas per SimulationEngine): n independent canvases, each a tpt_ctx
  * with its own buffers and stream, advanced together.  tpt_batch_step runs
- * one eager tick per canvas, then replays one hipGraph holding every
- * canvas's tick as a parallel branch (fork/join across the canvases'
- * streams).  Canvases are loaded and read through tpt_batch_context and
- * the single-context calls. */
+ * one eager tick per canvas, then replays captured ticks: by default each
+ * canvas's tick is its own hipGraph, replayed on the canvas's stream with
+ * the canvases side by side (tpt_run per canvas); with lanes set
+ * (tpt_batch_set_lanes), one hipGraph holds every canvas's tick, the
+ * canvases dealt to that many serial chains (fork/join across the canvases'
+ * streams).  Canvases are loaded and read through tpt_batch_context and the
+ * single-context calls. */
 typedef struct tpt_batch tpt_batch;
 TPT_API int tpt_batch_create(uint32_t n, const tpt_config *cfgs, tpt_batch **out);
 TPT_API int tpt_batch_destroy(tpt_batch *batch);
@@ -265,12 +268,15 @@ TPT_API int tpt_batch_step_launches(tpt_batch *batch, uint64_t first_frame, uint
  * SimulationEngine after another (tpt-app main.rs:656-672): each canvas's
  * tick waits on the GPU for the previous canvas's. */
 TPT_API int tpt_batch_step_serial(tpt_batch *batch, uint64_t first_frame, uint32_t ticks);
-/* Work split of the captured tick: the canvases go, largest first, to
- * `lanes` serial chains that run side by side (0: one chain per canvas). */
+/* 0 (the default): one graph per canvas, on the canvas's stream.  Otherwise
+ * one graph for the whole batch whose canvases go, largest first, to
+ * `lanes` serial chains that run side by side (lanes >= n: one chain per
+ * canvas). */
 TPT_API int tpt_batch_set_lanes(tpt_batch *batch, uint32_t lanes);
 /* Autotuner: one eager tick, then trial_ticks graph replays at each
- * candidate lane count (1, 2, 4, ... and one per canvas), timed with HIP
- * events; keeps the fastest (*lanes_out) and its graph.  The canvases
+ * candidate lane count of the whole-batch graph (1, 2, 4, ... and one per
+ * canvas), timed with HIP events; keeps the fastest (*lanes_out) and its
+ * graph.  The canvases
  * advance through all of these ticks.  Currently switched off: it returns
  * TPT_ESTATE (one run aborted inside the HIP runtime with no cause found;
  * tpt_batch_set_lanes sets a split by hand). */

@@ -99,7 +99,8 @@ def scenes_for_lanes():
 
 
 def test_serial_and_lanes_equal_launches(gpu):
-    """One canvas at a time and the graph split into 2 and 3 lanes give the
+    """One canvas at a time and the whole-batch graph split into 2, 3 and one
+    lane per canvas give the
     per-canvas launches' results."""
     scenes = scenes_for_lanes()
     print("[batch] serial / lanes vs launches", flush=True)
@@ -107,7 +108,7 @@ def test_serial_and_lanes_equal_launches(gpu):
     got = run_batch(scenes, lambda b: b.step_serial(0, 12))
     for k in range(len(scenes)):
         same(ref[k], got[k], f"serial canvas {k}")
-    for lanes in (2, 3):
+    for lanes in (2, 3, len(scenes)):
         def drive(b, lanes=lanes):
             b.set_lanes(lanes)
             b.step(0, 12)

@@ -203,7 +203,9 @@ constexpr int STAGE_BATCH = 4;
 template <bool LO>
 __device__ __forceinline__ void conv_stage(const ConvArgs &a, bf16_t *tile, int tile_elems, int ps, int x0, int y0,
                                            int rows) {
-    const int v8 = a.cin >> 3, total = (rows + 2) * CONV_LCOLS * v8;
+    // cin is a power of two (8 .. 128): an element's 8-channel group and
+    // column come from shifts and masks
+    const int lv8 = a.lcin - 3, v8 = 1 << lv8, total = (rows + 2) * CONV_LCOLS * v8;
     for (int base = threadIdx.x; base < total; base += CONV_THREADS * STAGE_BATCH) {
         int dst[STAGE_BATCH];
         bool ok[STAGE_BATCH];
@@ -212,7 +214,7 @@ __device__ __forceinline__ void conv_stage(const ConvArgs &a, bf16_t *tile, int 
 #pragma unroll
         for (int k = 0; k < STAGE_BATCH; ++k) {
             const int e = base + k * CONV_THREADS;
-            const int v = e % v8, rc = e / v8, c = rc % CONV_LCOLS, r = rc / CONV_LCOLS;
+            const int v = e & (v8 - 1), rc = e >> lv8, r = rc / CONV_LCOLS, c = rc - r * CONV_LCOLS;
             gy[k] = y0 - 1 + r;
             gx[k] = x0 - 1 + c;
             ch[k] = 8 * v;
