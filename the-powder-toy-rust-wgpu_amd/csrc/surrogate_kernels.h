@@ -159,6 +159,30 @@ __device__ __forceinline__ void conv_fetch(const bf16_t *tile, const bf16_t *con
     }
 }
 
+// conv_fetch for cin >= 32: a K chunk (32 columns) then lies inside one
+// tap, so the tap, its window offset and the chunk's first channel are the
+// same for every lane (scalar arithmetic), and a lane's LDS and weight
+// addresses are its fixed bases plus that offset.  Same operands as
+// conv_fetch.
+template <int NT>
+__device__ __forceinline__ void conv_fetch_wide(const bf16_t *lane_tile, const bf16_t *const *lane_w, const ConvArgs &a,
+                                                int kc, int ps, int g, int bias, uint4 (&xv)[4], uint4 (&wv)[NT]) {
+    const int k0 = 32 * kc, tap = k0 >> a.lcin, cib = k0 & (a.cin - 1);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) wv[t] = *reinterpret_cast<const uint4 *>(lane_w[t] + k0);
+    if (tap < 9) {
+        const int dy = tap / 3, dx = tap - 3 * dy;
+        const bf16_t *base = lane_tile + (dy * CONV_LCOLS + dx) * ps + cib;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xv[j] = *reinterpret_cast<const uint4 *>(base + 16 * j * ps);
+    } else {
+        // the bias chunk: column 9 * cin (lane group 0's first element) meets a 1
+        const uint4 z = (g == 0 && bias) ? make_uint4(0x3F80u, 0u, 0u, 0u) : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xv[j] = z;
+    }
+}
+
 template <int NT>
 __device__ __forceinline__ void conv_mfma(f32x4 (&acc)[4][NT], const uint4 (&xv)[4], const uint4 (&wv)[NT]) {
 #pragma unroll
@@ -337,6 +361,19 @@ __global__ __launch_bounds__(CONV_THREADS) void conv3x3(ConvArgs a) {
             conv_mfma<NT>(acc, x0v, w0v);
             conv_mfma<NT>(acc, x1v, w0v);
         }
+    } else if (a.cin >= 32) {
+        const bf16_t *lane_tile = tile + (row * CONV_LCOLS + m) * ps + 8 * g;
+        const bf16_t *lane_w[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) lane_w[t] = wrow[t] + 8 * g;
+        if (kc < kend) conv_fetch_wide<NT>(lane_tile, lane_w, a, kc, ps, g, a.bias_col, x0v, w0v);
+        for (; kc + 1 < kend; kc += 2) {
+            conv_fetch_wide<NT>(lane_tile, lane_w, a, kc + 1, ps, g, a.bias_col, x1v, w1v);
+            conv_mfma<NT>(acc, x0v, w0v);
+            if (kc + 2 < kend) conv_fetch_wide<NT>(lane_tile, lane_w, a, kc + 2, ps, g, a.bias_col, x0v, w0v);
+            conv_mfma<NT>(acc, x1v, w1v);
+        }
+        if (kc < kend) conv_mfma<NT>(acc, x0v, w0v);
     } else {
         if (kc < kend) conv_fetch<NT>(tile, wrow, a, kc, g, m, row, ps, a.bias_col, x0v, w0v);
         for (; kc + 1 < kend; kc += 2) {
