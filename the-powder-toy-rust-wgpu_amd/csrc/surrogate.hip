@@ -400,14 +400,20 @@ __global__ void sur_loss(const float *__restrict__ out, Planes cur, Planes nxt, 
 // energy sums before/after and a non-finite count for the check.
 __global__ void sur_apply(const float *__restrict__ out, float *__restrict__ pv, float *__restrict__ vx,
                           float *__restrict__ vy, float *__restrict__ hv, const float *__restrict__ src,
-                          float *__restrict__ red, int n, float sp, float sv, float sh) {
+                          float *__restrict__ red, float *__restrict__ saved, int n, float sp, float sv, float sh) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     float e_in = 0.0f, e_out = 0.0f, bad = 0.0f;
     if (i < n) {
-        const float p0 = pv[i], u0 = vx[i], v0 = vy[i];
+        const float p0 = pv[i], u0 = vx[i], v0 = vy[i], h0 = hv[i];
+        if (saved) {  // the state before the call, for the fallback
+            saved[i] = p0;
+            saved[n + i] = u0;
+            saved[2 * n + i] = v0;
+            saved[3 * n + i] = h0;
+        }
         e_in = p0 * p0 + u0 * u0 + v0 * v0;
         const float *o = out + (size_t)i * OUT_CH;
-        float p = p0 + o[0] * sp, u = u0 + o[1] * sv, v = v0 + o[2] * sv, h = hv[i] + o[3] * sh;
+        float p = p0 + o[0] * sp, u = u0 + o[1] * sv, v = v0 + o[2] * sv, h = h0 + o[3] * sh;
         if (!(isfinite(p) && isfinite(u) && isfinite(v) && isfinite(h))) bad = 1.0f;
         p = fminf(fmaxf(p, -256.0f), 256.0f);
         u = fminf(fmaxf(u, -256.0f), 256.0f);
@@ -810,7 +816,7 @@ extern "C" int tpt_sur_train(tpt_sur *s, const uint32_t *pairs, uint32_t n_pairs
         forward(s, st, true);
         hipLaunchKernelGGL(sur_loss, grid1(n), dim3(256), 0, st, s->out, cur, nxt, s->g_out, s->red, n,
                            s->scale[0], s->scale[1], s->scale[3]);
-        hipLaunchKernelGGL(sum_blocks, dim3(1), dim3(256), 0, st, s->red, (int)grid1(n).x, 1, s->loss_acc);
+        hipLaunchKernelGGL(sum_blocks, dim3(1), dim3(256), 0, st, s->red, (int)grid1(n).x, 1, s->loss_acc, 1);
         backward(s, st, in_batch > 0 ? 1 : 0);
         if (++in_batch == accum || i + 1 == n_pairs) {
             s->adam_t += 1;
@@ -819,7 +825,7 @@ extern "C" int tpt_sur_train(tpt_sur *s, const uint32_t *pairs, uint32_t n_pairs
             hipLaunchKernelGGL(sur_zero_f, dim3(1), dim3(64), 0, st, s->gnorm, 1);
             hipLaunchKernelGGL(grad_sumsq, grid1(s->n_params), dim3(256), 0, st, s->grad, (int)s->n_params, s->red);
             hipLaunchKernelGGL(sum_blocks, dim3(1), dim3(256), 0, st, s->red, (int)grid1(s->n_params).x, 1,
-                               s->gnorm);
+                               s->gnorm, 1);
             hipLaunchKernelGGL(adam_step, grid1(s->n_params), dim3(256), 0, st, s->wf, s->wb, s->grad, s->m1, s->m2,
                                (int)s->n_params, lr, b1, b2, 1e-8f, c1, c2, 1.0f / (float)in_batch, s->gnorm,
                                s->clip);
@@ -865,7 +871,7 @@ extern "C" int tpt_sur_eval_loss(tpt_sur *s, const uint32_t *pairs, uint32_t n_p
         forward(s, st, false);
         hipLaunchKernelGGL(sur_loss, grid1(n), dim3(256), 0, st, s->out, cur, nxt, s->g_out, s->red, n,
                            s->scale[0], s->scale[1], s->scale[3]);
-        hipLaunchKernelGGL(sum_blocks, dim3(1), dim3(256), 0, st, s->red, (int)grid1(n).x, 1, s->loss_acc);
+        hipLaunchKernelGGL(sum_blocks, dim3(1), dim3(256), 0, st, s->red, (int)grid1(n).x, 1, s->loss_acc, 1);
     }
     float acc[4];
     TPT_HIP_CHECK(hipMemcpyAsync(acc, s->loss_acc, sizeof acc, hipMemcpyDeviceToHost, st));
@@ -885,10 +891,6 @@ extern "C" int tpt_sur_step_air(tpt_sur *s, tpt_ctx *ctx, int check, float energ
     const int n = (int)cells_at(s, 0);
     hipStream_t st = ctx->stream;
     Planes p = ctx_planes(ctx);
-    if (check) {
-        hipLaunchKernelGGL(sur_capture, grid1(n), dim3(256), 0, st, p, s->saved, n);
-        hipLaunchKernelGGL(sur_zero_f, dim3(1), dim3(64), 0, st, s->loss_acc, 4);
-    }
     StageIn prep;
     prep.stage = STAGE_PREP;
     prep.pv = p.pv;
@@ -900,9 +902,10 @@ extern "C" int tpt_sur_step_air(tpt_sur *s, tpt_ctx *ctx, int check, float energ
     forward(s, st, false, &prep);
     hipLaunchKernelGGL(sur_apply, grid1(n), dim3(256), 0, st, s->out, tpt_buf<float>(ctx, TPT_BUF_AIR_PV),
                        tpt_buf<float>(ctx, TPT_BUF_AIR_VX), tpt_buf<float>(ctx, TPT_BUF_AIR_VY),
-                       tpt_buf<float>(ctx, TPT_BUF_AIR_HV), s->src, check ? s->red : nullptr, n, s->scale[0],
-                       s->scale[1], s->scale[3]);
-    if (check) hipLaunchKernelGGL(sum_blocks, dim3(1), dim3(256), 0, st, s->red, (int)grid1(n).x, 3, s->loss_acc + 1);
+                       tpt_buf<float>(ctx, TPT_BUF_AIR_HV), s->src, check ? s->red : nullptr,
+                       check ? s->saved : nullptr, n, s->scale[0], s->scale[1], s->scale[3]);
+    if (check)
+        hipLaunchKernelGGL(sum_blocks, dim3(1), dim3(256), 0, st, s->red, (int)grid1(n).x, 3, s->loss_acc + 1, 0);
     if ((rc = tpt_check_launch("surrogate step"))) return rc;
     ctx->maps_fresh = ctx->rec_cover = false;
     int fb = 0;

@@ -629,7 +629,7 @@ __device__ __forceinline__ void block_sum(float (&v)[NC], float *__restrict__ re
 
 // acc[c] += sum over b < nblocks of red[b * nc + c], for c < nc (one block
 // of 256 threads).
-__global__ void sum_blocks(const float *__restrict__ red, int nblocks, int nc, float *__restrict__ acc) {
+__global__ void sum_blocks(const float *__restrict__ red, int nblocks, int nc, float *__restrict__ acc, int add) {
     __shared__ float part[256];
     for (int c = 0; c < nc; ++c) {
         float v = 0.0f;
@@ -640,7 +640,7 @@ __global__ void sum_blocks(const float *__restrict__ red, int nblocks, int nc, f
             if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
             __syncthreads();
         }
-        if (threadIdx.x == 0) acc[c] += part[0];
+        if (threadIdx.x == 0) acc[c] = add ? acc[c] + part[0] : part[0];
         __syncthreads();
     }
 }
