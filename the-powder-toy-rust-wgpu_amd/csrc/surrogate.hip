@@ -110,6 +110,7 @@ struct tpt_sur {
     uint32_t n_traj = 0, n_snap = 0;
     // saved air state for the fallback
     float *saved = nullptr;
+    float *check_host = nullptr;    // pinned host copy of the check's sums
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
 };
@@ -605,6 +606,10 @@ extern "C" int tpt_sur_create(uint32_t xcells, uint32_t ycells, tpt_sur **out) {
     if (!rc) rc = alloc_n(s, &s->red, 4 * std::max<size_t>(grid1(n0).x, grid1(s->n_params).x));
     if (!rc) rc = alloc_n(s, &s->src, n0);
     if (!rc) rc = alloc_n(s, &s->saved, 4 * n0);
+    if (!rc && hipHostMalloc((void **)&s->check_host, 4 * sizeof(float), hipHostMallocDefault) != hipSuccess) {
+        s->check_host = nullptr;
+        rc = tpt_set_error(TPT_ENOMEM, "surrogate: pinned check buffer");
+    }
     if (!rc && hipStreamCreate(&s->stream) != hipSuccess) rc = tpt_set_error(TPT_EDEVICE, "surrogate stream");
     if (rc) {
         tpt_sur_destroy(s);
@@ -642,6 +647,7 @@ extern "C" int tpt_sur_destroy(tpt_sur *s) {
     if (!s) return TPT_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (void *p : s->allocs) (void)hipFree(p);
+    if (s->check_host) (void)hipHostFree(s->check_host);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return TPT_OK;
@@ -910,8 +916,8 @@ extern "C" int tpt_sur_step_air(tpt_sur *s, tpt_ctx *ctx, int check, float energ
     ctx->maps_fresh = ctx->rec_cover = false;
     int fb = 0;
     if (check) {
-        float acc[4];
-        TPT_HIP_CHECK(hipMemcpyAsync(acc, s->loss_acc, sizeof acc, hipMemcpyDeviceToHost, st));
+        float *acc = s->check_host;  // pinned: the copy needs no staging
+        TPT_HIP_CHECK(hipMemcpyAsync(acc, s->loss_acc, 4 * sizeof(float), hipMemcpyDeviceToHost, st));
         TPT_HIP_CHECK(hipStreamSynchronize(st));
         if (acc[3] != 0.0f || !(acc[2] <= energy_ratio * acc[1] + 1e-3f * (float)n)) {
             fb = 1;
