@@ -186,6 +186,33 @@ def test_training_lowers_the_loss(gpu):
     sur.close()
 
 
+def test_two_step_rollout_training(gpu):
+    """tpt_sur_set_rollout(2): pairs train from the network's own first
+    step; a pair without a snapshot two calls ahead is refused, a zero
+    learning rate leaves the parameters alone, and training lowers the
+    two-step loss."""
+    ctx, sur = small_world()
+    sur.set_scales(SU.IN_SCALES, SU.delta_scales(ctx, sur, range(900, 903)))
+    SU.build_dataset(sur, ctx, list(range(16)))
+    sur.set_rollout(2)
+    with pytest.raises(RuntimeError):
+        sur.train(np.array([(0, 7)], np.uint32), 1, 0.0)      # snapshot 9 does not exist
+    pairs = np.array([(t, k) for t in range(16) for k in range(7)], np.uint32)
+    p0 = sur.params()
+    before = sur.train(pairs, 4, 0.0)
+    assert np.array_equal(sur.params(), p0)
+    print(f"[surrogate rollout] two-step loss before {before:.4e}", flush=True)
+    rng = np.random.default_rng(1)
+    for ep in range(12):
+        sur.train(pairs[rng.permutation(len(pairs))], 4, 2e-3)
+    after = sur.train(pairs, 4, 0.0)
+    print(f"  after {after:.4e}", flush=True)
+    assert np.isfinite(after) and after < 0.5 * before
+    sur.set_rollout(1)
+    ctx.close()
+    sur.close()
+
+
 def test_energy_check_falls_back_to_plain_ticks_bit_exact(gpu):
     ctx, sur = small_world()
     p = sur.params() * 0.0

@@ -96,6 +96,12 @@ struct tpt_sur {
     float *gnorm = nullptr;         // device: sum of squares of the batch gradient
     float *red = nullptr;           // device: per-block partial sums (up to 4 per block)
     float clip = 1.0f;              // global gradient-norm clip (0: none)
+    // Training horizon: 1 trains on (snapshot k -> k+1); 2 first advances
+    // snapshot k by the network itself (no gradient) and trains on
+    // (that prediction -> snapshot k+2), so that the network learns to
+    // correct the errors of its own rollouts.
+    uint32_t rollout = 1;
+    float *roll = nullptr;          // the predicted planes of a 2-step pair (4 x cells)
     bool fused = true;              // inference folds the glue kernels into the convolutions
     float in_scale[4] = {10.0f, 1.0f, 1.0f, 500.0f};   // input normalisation (state)
     float scale[4] = {1.0f, 1.0f, 1.0f, 10.0f};        // output normalisation (8-tick change)
@@ -433,6 +439,26 @@ __global__ void sur_apply(const float *__restrict__ out, float *__restrict__ pv,
     }
 }
 
+// The state a call would leave: cur + prediction * scale with the apply's
+// clamps and heat sources, into dst (a training rollout's first step).
+__global__ void sur_predict(const float *__restrict__ out, Planes cur, const float *__restrict__ src,
+                            float *__restrict__ dst, int n, float sp, float sv, float sh) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *o = out + (size_t)i * OUT_CH;
+    float p = cur.pv[i] + o[0] * sp, u = cur.vx[i] + o[1] * sv, v = cur.vy[i] + o[2] * sv,
+          h = cur.hv[i] + o[3] * sh;
+    p = fminf(fmaxf(p, -256.0f), 256.0f);
+    u = fminf(fmaxf(u, -256.0f), 256.0f);
+    v = fminf(fmaxf(v, -256.0f), 256.0f);
+    h = fminf(fmaxf(h, 0.0f), 9999.0f);
+    if (src[i] > 0.0f) h = src[i];
+    dst[i] = p;
+    dst[n + i] = u;
+    dst[2 * n + i] = v;
+    dst[3 * n + i] = h;
+}
+
 __global__ void sur_pin(float *__restrict__ hv, const float *__restrict__ src, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && src[i] > 0.0f) hv[i] = src[i];
@@ -606,6 +632,7 @@ extern "C" int tpt_sur_create(uint32_t xcells, uint32_t ycells, tpt_sur **out) {
     if (!rc) rc = alloc_n(s, &s->red, 4 * std::max<size_t>(grid1(n0).x, grid1(s->n_params).x));
     if (!rc) rc = alloc_n(s, &s->src, n0);
     if (!rc) rc = alloc_n(s, &s->saved, 4 * n0);
+    if (!rc) rc = alloc_n(s, &s->roll, 4 * n0);
     if (!rc && hipHostMalloc((void **)&s->check_host, 4 * sizeof(float), hipHostMallocDefault) != hipSuccess) {
         s->check_host = nullptr;
         rc = tpt_set_error(TPT_ENOMEM, "surrogate: pinned check buffer");
@@ -808,17 +835,29 @@ extern "C" int tpt_sur_train(tpt_sur *s, const uint32_t *pairs, uint32_t n_pairs
     if (!s || !pairs || !n_pairs || !accum) return tpt_set_error(TPT_EINVAL, "bad training arguments");
     if (!s->data) return tpt_set_error(TPT_ESTATE, "no dataset");
     for (uint32_t i = 0; i < n_pairs; ++i)
-        if (pairs[2 * i] >= s->n_traj || pairs[2 * i + 1] + 1 >= s->n_snap)
-            return tpt_set_error(TPT_EINVAL, "pair %u out of range", i);
+        if (pairs[2 * i] >= s->n_traj || pairs[2 * i + 1] + s->rollout >= s->n_snap)
+            return tpt_set_error(TPT_EINVAL, "pair %u out of range for a %u-step rollout", i, s->rollout);
     const int n = (int)cells_at(s, 0);
     hipStream_t st = s->stream;
     hipLaunchKernelGGL(sur_zero_f, dim3(1), dim3(64), 0, st, s->loss_acc, 4);
     uint32_t in_batch = 0;
     for (uint32_t i = 0; i < n_pairs; ++i) {
         const uint32_t tr = pairs[2 * i], sn = pairs[2 * i + 1];
-        const Planes cur = snap_planes(s, tr, sn), nxt = snap_planes(s, tr, sn + 1);
-        hipLaunchKernelGGL(sur_prep, grid1(n), dim3(256), 0, st, cur, s->walls + (size_t)tr * n, nullptr,
-                           s->srcs + (size_t)tr * n, s->x0, s->x0lo, n, s->in_scale[0], s->in_scale[1], s->in_scale[3], s->ambient);
+        Planes cur = snap_planes(s, tr, sn);
+        const Planes nxt = snap_planes(s, tr, sn + s->rollout);
+        const uint8_t *walls = s->walls + (size_t)tr * n;
+        const float *srcs = s->srcs + (size_t)tr * n;
+        if (s->rollout == 2) {
+            // the network's own first step, without gradient
+            hipLaunchKernelGGL(sur_prep, grid1(n), dim3(256), 0, st, cur, walls, nullptr, srcs, s->x0, s->x0lo, n,
+                               s->in_scale[0], s->in_scale[1], s->in_scale[3], s->ambient);
+            forward(s, st, false);
+            hipLaunchKernelGGL(sur_predict, grid1(n), dim3(256), 0, st, s->out, cur, srcs, s->roll, n, s->scale[0],
+                               s->scale[1], s->scale[3]);
+            cur = Planes{s->roll, s->roll + n, s->roll + 2 * (size_t)n, s->roll + 3 * (size_t)n};
+        }
+        hipLaunchKernelGGL(sur_prep, grid1(n), dim3(256), 0, st, cur, walls, nullptr, srcs, s->x0, s->x0lo, n,
+                           s->in_scale[0], s->in_scale[1], s->in_scale[3], s->ambient);
         forward(s, st, true);
         hipLaunchKernelGGL(sur_loss, grid1(n), dim3(256), 0, st, s->out, cur, nxt, s->g_out, s->red, n,
                            s->scale[0], s->scale[1], s->scale[3]);
@@ -847,6 +886,12 @@ extern "C" int tpt_sur_train(tpt_sur *s, const uint32_t *pairs, uint32_t n_pairs
 }
 
 // Global L2 norm bound of the averaged batch gradient (0 turns clipping off).
+extern "C" int tpt_sur_set_rollout(tpt_sur *s, uint32_t steps) {
+    if (!s || steps < 1 || steps > 2) return tpt_set_error(TPT_EINVAL, "rollout steps must be 1 or 2");
+    s->rollout = steps;
+    return TPT_OK;
+}
+
 extern "C" int tpt_sur_set_clip(tpt_sur *s, float clip) {
     if (!s || !(clip >= 0.0f)) return tpt_set_error(TPT_EINVAL, "bad clip");
     s->clip = clip;

@@ -88,6 +88,7 @@ SIGNATURES = {
                       ctypes.c_int),
     "tpt_sur_eval_loss": ([_vp, _vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "tpt_sur_set_clip": ([_vp, ctypes.c_float], ctypes.c_int),
+    "tpt_sur_set_rollout": ([_vp, ctypes.c_uint32], ctypes.c_int),
     "tpt_sur_set_fused": ([_vp, ctypes.c_int], ctypes.c_int),
     "tpt_sur_step_air": ([_vp, _vp, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "tpt_sur_last_check": ([_vp, _vp], ctypes.c_int),

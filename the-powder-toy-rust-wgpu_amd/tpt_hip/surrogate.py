@@ -7,6 +7,7 @@ air_kernel/mod.rs:120-299), drives training on the GPU, evaluates the trained
 network against the plain solver, and saves and loads its parameters.
 
     python -m tpt_hip.surrogate train --traj 2000 --epochs 6 --out weights.npz
+    python -m tpt_hip.surrogate train --init weights.npz --rollout --epochs 4 --out tuned.npz
     python -m tpt_hip.surrogate eval --weights weights.npz --seeds 256
     python -m tpt_hip.surrogate calls --calls 30      (a workload for rocprofv3)
 
@@ -157,6 +158,11 @@ class Surrogate:
     def set_clip(self, clip):
         L._check(L.load().tpt_sur_set_clip(self._h, ctypes.c_float(clip)))
 
+    def set_rollout(self, steps):
+        """Training horizon: 1 (snapshot k -> k+1) or 2 (the network's own
+        prediction of snapshot k+1 -> snapshot k+2)."""
+        L._check(L.load().tpt_sur_set_rollout(self._h, steps))
+
     def set_fused(self, fused):
         L._check(L.load().tpt_sur_set_fused(self._h, 1 if fused else 0))
 
@@ -305,10 +311,17 @@ def train_main(args):
     ctx = L.Context(air_config(xc, yc))
     ctx.load_state(S.SimState(air_config(xc, yc)))
     sur = Surrogate(xc, yc)
-    scales = delta_scales(ctx, sur, range(10 ** 6, 10 ** 6 + 8))
-    sur.set_scales(IN_SCALES, scales)
-    log(f"output scales {scales}")
-    sur.init_params(args.seed)
+    history, prior = [], {}
+    if args.init:
+        # fine-tuning: parameters and normalisation of a saved network
+        prior = sur.load(args.init)
+        history = list(prior.get("history", []))
+        log(f"starting from {args.init} ({prior.get('epochs')} epochs)")
+    else:
+        scales = delta_scales(ctx, sur, range(10 ** 6, 10 ** 6 + 8))
+        sur.set_scales(IN_SCALES, scales)
+        log(f"output scales {scales}")
+        sur.init_params(args.seed)
     sur.set_clip(args.clip)
     train_seeds = list(range(args.traj))
     t0 = time.perf_counter()
@@ -317,9 +330,10 @@ def train_main(args):
     pairs = np.array([(t, k) for t in range(args.traj) for k in range(TICKS_PER_CALL)], np.uint32)
     rng = np.random.default_rng(args.seed)
     hold = pairs[rng.choice(len(pairs), size=min(256, len(pairs)), replace=False)]
-    history = []
     steps_done = 0
     total_steps = args.epochs * len(pairs) // args.accum
+    if args.rollout:
+        total_steps *= 2
     for ep in range(args.epochs):
         order = rng.permutation(len(pairs))
         t1 = time.perf_counter()
@@ -330,19 +344,29 @@ def train_main(args):
             warm = min(1.0, (steps_done + 1) / args.warmup_steps)
             lr = args.lr * warm * 0.5 * (1.0 + np.cos(np.pi * min(frac, 1.0)))
             sel = pairs[order[c0:c0 + chunk]]
+            sur.set_rollout(1)
             losses.append(sur.train(sel, args.accum, lr))
             steps_done += (len(sel) + args.accum - 1) // args.accum
+            if args.rollout:
+                # the same pairs again, from the network's own first step
+                two = sel[sel[:, 1] + 2 <= TICKS_PER_CALL]
+                if len(two):
+                    sur.set_rollout(2)
+                    losses.append(sur.train(two, args.accum, lr))
+                    sur.set_rollout(1)
+                    steps_done += (len(two) + args.accum - 1) // args.accum
         ev = sur.eval_loss(hold)
-        history.append({"epoch": ep + 1, "train_loss": float(np.mean(losses)), "eval_loss": ev,
-                        "seconds": time.perf_counter() - t1})
-        log(f"epoch {ep + 1}: train {np.mean(losses):.4e} eval {ev:.4e} ({time.perf_counter() - t1:.1f} s)")
-        sur.save(args.out, {"history": history, "traj": args.traj, "epochs": ep + 1, "seed": args.seed})
+        done_epochs = int(prior.get("epochs") or 0) + ep + 1
+        history.append({"epoch": done_epochs, "train_loss": float(np.mean(losses)), "eval_loss": ev,
+                        "seconds": time.perf_counter() - t1, "rollout": 2 if args.rollout else 1})
+        log(f"epoch {done_epochs}: train {np.mean(losses):.4e} eval {ev:.4e} ({time.perf_counter() - t1:.1f} s)")
+        sur.save(args.out, {"history": history, "traj": args.traj, "epochs": done_epochs, "seed": args.seed})
     log(f"saved {args.out}")
     if args.eval_seeds:
         res = evaluate(sur, ctx, range(10 ** 7, 10 ** 7 + args.eval_seeds))
         log(f"held-out {args.eval_seeds} seeds: {res}")
-        sur.save(args.out, {"history": history, "traj": args.traj, "epochs": args.epochs, "seed": args.seed,
-                            "eval": res})
+        sur.save(args.out, {"history": history, "traj": args.traj, "epochs": history[-1]["epoch"] if history else 0,
+                            "seed": args.seed, "eval": res})
     ctx.close()
 
 
@@ -396,6 +420,9 @@ def main(argv=None):
     t.add_argument("--seed", type=int, default=7)
     t.add_argument("--eval-seeds", type=int, default=0)
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
+    t.add_argument("--init", default=None, help="continue from a saved network (its scales are kept)")
+    t.add_argument("--rollout", action="store_true",
+                   help="train every pair also from the network's own first step (2-step rollouts)")
     e = sub.add_parser("eval")
     e.add_argument("--grid", type=int, default=512)
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
