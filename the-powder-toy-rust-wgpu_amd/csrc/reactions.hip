@@ -693,7 +693,7 @@ __device__ __forceinline__ bool actor_may_change(uint32_t ty, uint32_t around, u
 // After the first conversion of a target nothing later can change it.  The
 // tile stages the pmap words and the types of every pixel within 2 of it:
 // an actor's window, and the windows of the actors next to a target.
-__global__ __launch_bounds__(BLK) void rx_fluid_decide(RxArgs a) {
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8))) void rx_fluid_decide(RxArgs a) {
     __shared__ uint32_t s_cf[CFW * 3];
     __shared__ uint32_t s_pm[STAGED], s_ty[STAGED];
     const TileGeom g = tile_geom(a);

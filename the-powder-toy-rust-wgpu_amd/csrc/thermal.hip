@@ -338,7 +338,7 @@ __device__ __forceinline__ void lds_append(uint32_t *list, uint32_t *count, uint
 // two or so per thread, reducing R2 (highest gated slot carries its hv,
 // a 64-bit LDS max) and R3 (failure count) per cell in LDS; phase 4 merges
 // the off-map list's cell results and writes the cells.
-__global__ __launch_bounds__(256) void th_conduct_px(Parts p, const unsigned long long *__restrict__ rec,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void th_conduct_px(Parts p, const unsigned long long *__restrict__ rec,
                                                      float *__restrict__ air_hv, uint32_t *__restrict__ block_airh,
                                                      const uint32_t *__restrict__ props,
                                                      const float *__restrict__ hv_new, uint32_t *__restrict__ hv_key,
