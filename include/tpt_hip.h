@@ -186,7 +186,7 @@ TPT_API int tpt_gravity_solve(tpt_ctx *ctx);
  * takes no context.  Solves fx, fy (xcells*ycells floats each) from the
  * fixed-point mass grid (i32, 1/65536 units) and the gravity mask (u32), all
  * device pointers, on `stream` (a hipStream_t; NULL is the default stream).
- * The FFT's buffers live in a 29.4 MB static device array that each GPU the
+ * The FFT's buffers live in a 16.8 MB static device array that each GPU the
  * library runs on holds until the process ends; padded grids beyond
  * 1024x1024 (xcells or ycells above 512, canvases beyond 2048x2048 px) are
  * rejected with TPT_EINVAL.  Calls that overlap on one GPU race on that

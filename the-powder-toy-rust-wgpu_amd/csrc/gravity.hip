@@ -12,11 +12,18 @@
 //                   grid are zero, so is their transform): mass -> float,
 //                   mask, zero-pad and forward row FFT, fused.
 //   grav_cols       a tile of adjacent columns per workgroup: forward column
-//                   FFT over the ycells non-zero rows, product with both
-//                   cached kernel spectra, both inverse column FFTs; only the
+//                   FFT over the ycells non-zero rows, product with the
+//                   cached kernel spectrum, inverse column FFT; only the
 //                   ycells rows the extraction reads are stored.
-//   grav_rows_inv   both inverse row FFTs of one row + real-part extraction
-//                   into grav_fx / grav_fy under the gravity mask.
+//   grav_rows_inv   the inverse row FFT of one row + extraction into
+//                   grav_fx / grav_fy under the gravity mask.
+//
+// The two force components share one transform.  The x and y kernels are
+// real, so the field kx + i ky transforms to Kx + i Ky, and with the real
+// mass spectrum F the products F Kx and F Ky are Hermitian: their inverse
+// transforms are real (up to rounding), so the inverse of F (Kx + i Ky) is
+// fx + i fy.  One kernel spectrum, one product and one inverse transform
+// replace two of each.
 //
 // The kernel spectra (build_kernel_field_kernel, fft.rs:209-236, forward
 // transformed) depend on the canvas only and are built once per context.
@@ -175,18 +182,17 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_rows_fwd(const int32_t *__re
     for (int i = threadIdx.x; i < d.W; i += FFT_THREADS) work[(size_t)row * d.W + i] = sm[i];
 }
 
-// Forward column transform, products with both kernel spectra
-// (complex_mul_kernel, fft.rs:129-140) and both inverse column transforms,
-// for 2^logcols adjacent columns per workgroup.  Only rows < rows_in of the
-// input are non-zero and only rows < rows_out of the outputs are stored.
+// Forward column transform, product with the kernel spectrum
+// (complex_mul_kernel, fft.rs:129-140) and inverse column transform, for
+// 2^logcols adjacent columns per workgroup.  Only rows < rows_in of the
+// input are non-zero and only rows < rows_out of the output are stored.
 // Four waves per SIMD (128 VGPRs): the 1024 column workgroups then fit in
 // one round on 256 CUs instead of one and a third (measured 44 -> 33 us at
-// 2048x2048).
+// 2048x2048, with two products then).
 __global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void grav_cols(const float2 *__restrict__ work,
-                                                         const float2 *__restrict__ kx,
-                                                         const float2 *__restrict__ ky, float2 *__restrict__ sx,
-                                                         float2 *__restrict__ sy, const float2 *__restrict__ tw,
-                                                         FftDims d, int rows_in, int rows_out) {
+                                                         const float2 *__restrict__ kc, float2 *__restrict__ spec,
+                                                         const float2 *__restrict__ tw, FftDims d, int rows_in,
+                                                         int rows_out) {
     extern __shared__ float2 dyn[];
     const int logc = d.logcols, cols = 1 << logc, c0 = col_tile(d) * cols, total = d.H << logc;
     float2 *sm = dyn, *tab = dyn + total;
@@ -197,38 +203,23 @@ __global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     }
     __syncthreads();
     lds_fft(sm, tab, d.logH, logc, 1.0f);
-    float2 spec[EPT];
-#pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-        const int e = threadIdx.x + j * FFT_THREADS;
-        if (e < total) spec[j] = sm[e];
+    for (int e = threadIdx.x; e < total; e += FFT_THREADS) {
+        const float2 k = kc[(size_t)(e >> logc) * d.W + c0 + (e & (cols - 1))];
+        const float2 a = sm[e];
+        sm[e] = make_float2(a.x * k.x - a.y * k.y, a.x * k.y + a.y * k.x);
     }
-    for (int pass = 0; pass < 2; ++pass) {
-        const float2 *__restrict__ K = pass ? ky : kx;
-        float2 *__restrict__ out = pass ? sy : sx;
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < EPT; ++j) {
-            const int e = threadIdx.x + j * FFT_THREADS;
-            if (e < total) {
-                const float2 k = K[(size_t)(e >> logc) * d.W + c0 + (e & (cols - 1))];
-                const float2 a = spec[j];
-                sm[e] = make_float2(a.x * k.x - a.y * k.y, a.x * k.y + a.y * k.x);
-            }
-        }
-        __syncthreads();
-        lds_fft(sm, tab, d.logH, logc, -1.0f);
-        for (int e = threadIdx.x; e < (rows_out << logc); e += FFT_THREADS)
-            out[(size_t)(e >> logc) * d.W + c0 + (e & (cols - 1))] = sm[e];
-    }
+    __syncthreads();
+    lds_fft(sm, tab, d.logH, logc, -1.0f);
+    for (int e = threadIdx.x; e < (rows_out << logc); e += FFT_THREADS)
+        spec[(size_t)(e >> logc) * d.W + c0 + (e & (cols - 1))] = sm[e];
 }
 
-// Inverse row transforms of both spectra (two interleaved sequences) +
-// extract_force_kernel (fft.rs:182-200), one row per workgroup.
-// Also packs {fx, fy, wall_bmap, wall_emap} per cell for grav_apply, and
-// zeroes the mass grid for the tick's deposit when asked to.
-__global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__restrict__ sx,
-                                                             const float2 *__restrict__ sy,
+// Inverse row transform of the product + extract_force_kernel
+// (fft.rs:182-200), one row per workgroup: the real part is the x force,
+// the imaginary part the y force.  Also packs {fx, fy, wall_bmap, wall_emap}
+// per cell for grav_apply, and zeroes the mass grid for the tick's deposit
+// when asked to.
+__global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__restrict__ spec,
                                                              const uint32_t *__restrict__ mask,
                                                              float *__restrict__ fx, float *__restrict__ fy,
                                                              const uint32_t *__restrict__ wall,
@@ -237,19 +228,16 @@ __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inv(const float2 *__res
                                                              int32_t *__restrict__ prev_clear,
                                                              const float2 *__restrict__ tw, FftDims d) {
     extern __shared__ float2 dyn[];
-    float2 *sm = dyn, *tab = dyn + 2 * d.W;
+    float2 *sm = dyn, *tab = dyn + d.W;
     const int row = blockIdx.x;
     load_twiddles(tab, tw, d.logW, d.tw_bits);
-    for (int i = threadIdx.x; i < d.W; i += FFT_THREADS) {
-        sm[i * 2 + 0] = sx[(size_t)row * d.W + i];
-        sm[i * 2 + 1] = sy[(size_t)row * d.W + i];
-    }
+    for (int i = threadIdx.x; i < d.W; i += FFT_THREADS) sm[i] = spec[(size_t)row * d.W + i];
     __syncthreads();
-    lds_fft(sm, tab, d.logW, 1, -1.0f);
+    lds_fft(sm, tab, d.logW, 0, -1.0f);
     for (int i = threadIdx.x; i < d.xc; i += FFT_THREADS) {
         const int c = row * d.xc + i;
         const bool on = mask[c] != 0u;
-        const float gx = on ? sm[i * 2 + 0].x : 0.0f, gy = on ? sm[i * 2 + 1].x : 0.0f;
+        const float gx = on ? sm[i].x : 0.0f, gy = on ? sm[i].y : 0.0f;
         fx[c] = gx;
         fy[c] = gy;
         if (rec) rec[c] = make_uint4(__float_as_uint(gx), __float_as_uint(gy), wall[c], emap[c]);
@@ -271,8 +259,9 @@ __global__ void grav_twiddles(float2 *__restrict__ tw, uint32_t n, uint32_t len)
     tw[k] = make_float2((float)cos(a), (float)sin(a));
 }
 
-// Kernel-spectrum construction (once per context).
-__global__ void grav_kernel_field(float2 *__restrict__ kx, float2 *__restrict__ ky, FftDims d, float scale) {
+// Kernel-field construction (once per context): kx + i ky, the x kernel in
+// the real part and the y kernel in the imaginary part.
+__global__ void grav_kernel_field(float2 *__restrict__ k, FftDims d, float scale) {
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= d.W * d.H) return;
     const int px = id % d.W, py = id / d.W;
@@ -286,8 +275,7 @@ __global__ void grav_kernel_field(float2 *__restrict__ kx, float2 *__restrict__ 
         vx = dx * inv;
         vy = dy * inv;
     }
-    kx[id] = make_float2(vx, 0.0f);
-    ky[id] = make_float2(vy, 0.0f);
+    k[id] = make_float2(vx, vy);
 }
 
 __global__ __launch_bounds__(FFT_THREADS) void grav_rows_inplace(float2 *__restrict__ data,
@@ -499,40 +487,37 @@ FftDims dims_of(const tpt_ctx *ctx) {
     return d;
 }
 
-// The twiddle table and both kernel spectra of the padded grid `d`: the
-// Green's function samples (gravity_gpu/mod.rs:170-200) transformed in place.
-// The inverse transform is unnormalised; the kernel carries 1/(W*H)
-// (gravity_gpu/mod.rs:176-180).
-void build_spectra(float2 *tw, float2 *kx, float2 *ky, const FftDims &d, uint32_t ntw, uint32_t tw_len,
-                   hipStream_t stream) {
+// The twiddle table and the kernel spectrum of the padded grid `d`: the
+// Green's function samples (gravity_gpu/mod.rs:170-200), x kernel + i y
+// kernel, transformed in place.  The inverse transform is unnormalised; the
+// kernel carries 1/(W*H) (gravity_gpu/mod.rs:176-180).
+void build_spectra(float2 *tw, float2 *kc, const FftDims &d, uint32_t ntw, uint32_t tw_len, hipStream_t stream) {
     hipLaunchKernelGGL(grav_twiddles, dim3((tw_len + 255) / 256), dim3(256), 0, stream, tw, ntw, tw_len);
     const size_t grid = (size_t)d.W * d.H;
     const float scale = -M_GRAV / (float)(d.W * d.H);
-    hipLaunchKernelGGL(grav_kernel_field, dim3((grid + 255) / 256), dim3(256), 0, stream, kx, ky, d, scale);
-    for (float2 *k : {kx, ky}) {
-        hipLaunchKernelGGL(grav_rows_inplace, dim3(d.H), dim3(FFT_THREADS), rows_lds(d, 1), stream, k, tw, d);
-        hipLaunchKernelGGL(grav_cols_inplace, dim3(d.W >> d.logcols), dim3(FFT_THREADS), cols_lds(d), stream, k, tw,
-                           d);
-    }
+    hipLaunchKernelGGL(grav_kernel_field, dim3((grid + 255) / 256), dim3(256), 0, stream, kc, d, scale);
+    hipLaunchKernelGGL(grav_rows_inplace, dim3(d.H), dim3(FFT_THREADS), rows_lds(d, 1), stream, kc, tw, d);
+    hipLaunchKernelGGL(grav_cols_inplace, dim3(d.W >> d.logcols), dim3(FFT_THREADS), cols_lds(d), stream, kc, tw,
+                       d);
 }
 
-// The three solve launches: padded forward rows, columns with both kernel
-// products and inverse columns, inverse rows with the force extraction.
+// The three solve launches: padded forward rows, columns with the kernel
+// product and inverse columns, inverse rows with the force extraction.
 // `prof` (nullable) receives the pass timings.
 void launch_solve(const int32_t *mass, const uint32_t *mask, float *fx, float *fy, const uint32_t *wall,
                   const uint32_t *emap, uint4 *rec, int32_t *mass_clear, int32_t *prev_clear, float2 *work,
-                  float2 *sx, float2 *sy, const float2 *kx, const float2 *ky, const float2 *tw, const FftDims &d,
-                  hipStream_t stream, tpt_ctx *prof) {
+                  float2 *spec, const float2 *kc, const float2 *tw, const FftDims &d, hipStream_t stream,
+                  tpt_ctx *prof) {
     if (prof) tpt_pass_begin(prof, PASS_GRAV_FFT_ROWS);
     hipLaunchKernelGGL(grav_rows_fwd, dim3(d.yc), dim3(FFT_THREADS), rows_lds(d, 1), stream, mass, mask, work, tw,
                        d);
     if (prof) tpt_pass_end(prof, PASS_GRAV_FFT_ROWS);
     if (prof) tpt_pass_begin(prof, PASS_GRAV_FFT_COLS);
-    hipLaunchKernelGGL(grav_cols, dim3(d.W >> d.logcols), dim3(FFT_THREADS), cols_lds(d), stream, work, kx, ky, sx,
-                       sy, tw, d, d.yc, d.yc);
+    hipLaunchKernelGGL(grav_cols, dim3(d.W >> d.logcols), dim3(FFT_THREADS), cols_lds(d), stream, work, kc, spec,
+                       tw, d, d.yc, d.yc);
     if (prof) tpt_pass_end(prof, PASS_GRAV_FFT_COLS);
     if (prof) tpt_pass_begin(prof, PASS_GRAV_FFT_INV);
-    hipLaunchKernelGGL(grav_rows_inv, dim3(d.yc), dim3(FFT_THREADS), rows_lds(d, 2), stream, sx, sy, mask, fx, fy,
+    hipLaunchKernelGGL(grav_rows_inv, dim3(d.yc), dim3(FFT_THREADS), rows_lds(d, 1), stream, spec, mask, fx, fy,
                        wall, emap, rec, mass_clear, prev_clear, tw, d);
     if (prof) tpt_pass_end(prof, PASS_GRAV_FFT_INV);
 }
@@ -549,13 +534,11 @@ int ensure_fft(tpt_ctx *ctx) {
     const uint32_t ntw = f.width > f.height ? f.width : f.height;
     const uint32_t tw_len = ntw / 2 ? ntw / 2 : 1;
     if (hipMalloc(&f.work, rows_grid * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&f.spec_x, rows_grid * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&f.spec_y, rows_grid * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&f.kernel_x, grid * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&f.kernel_y, grid * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&f.spec, rows_grid * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&f.kernel, grid * sizeof(float2)) != hipSuccess ||
         hipMalloc(&f.twiddle, tw_len * sizeof(float2)) != hipSuccess)
         return tpt_set_error(TPT_ENOMEM, "gravity FFT buffers (%ux%u) could not be allocated", f.width, f.height);
-    build_spectra(f.twiddle, f.kernel_x, f.kernel_y, dims_of(ctx), ntw, tw_len, ctx->stream);
+    build_spectra(f.twiddle, f.kernel, dims_of(ctx), ntw, tw_len, ctx->stream);
     int rc = tpt_check_launch("gravity kernel spectra");
     if (rc) return rc;
     f.kernels_ready = true;
@@ -571,7 +554,7 @@ int solve(tpt_ctx *ctx, bool clear_mass) {
                  tpt_buf<float>(ctx, TPT_BUF_GRAV_FX), tpt_buf<float>(ctx, TPT_BUF_GRAV_FY),
                  tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), ctx->cell_rec,
                  clear_mass ? tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS) : nullptr,
-                 tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_PREV), f.work, f.spec_x, f.spec_y, f.kernel_x, f.kernel_y,
+                 tpt_buf<int32_t>(ctx, TPT_BUF_GRAV_MASS_PREV), f.work, f.spec, f.kernel,
                  f.twiddle, d, ctx->stream, ctx);
     return tpt_check_launch("gravity solve");
 }
@@ -654,7 +637,7 @@ int tpt_gravity_override_clear(tpt_ctx *ctx) {
 
 int tpt_gravity_free(tpt_ctx *ctx) {
     tpt_fft_state &f = ctx->fft;
-    for (float2 *p : {f.work, f.spec_x, f.spec_y, f.kernel_x, f.kernel_y, f.twiddle})
+    for (float2 *p : {f.work, f.spec, f.kernel, f.twiddle})
         if (p) (void)hipFree(p);
     f = tpt_fft_state();
     return TPT_OK;
@@ -665,8 +648,7 @@ int tpt_gravity_free(tpt_ctx *ctx) {
 // allocate.  Its FFT buffers live in `scratch`, a static device array each
 // GPU holds for the life of the process; the padded grid may be at most
 // NOALLOC_MAX_SIDE on each side (canvases up to 2048x2048 px).  Layout:
-// twiddles | kernel spectrum x | kernel spectrum y | work | spectrum x |
-// spectrum y.  Every launch writes each scratch element it later reads, so
+// twiddles | kernel spectrum | work | spectrum.  Every launch writes each scratch element it later reads, so
 // nothing has to start at zero; calls that overlap on one GPU race.
 
 namespace {
@@ -675,7 +657,7 @@ constexpr uint32_t NOALLOC_MAX_SIDE = 1024;
 constexpr size_t NOALLOC_TW = NOALLOC_MAX_SIDE / 2;
 constexpr size_t NOALLOC_GRID = (size_t)NOALLOC_MAX_SIDE * NOALLOC_MAX_SIDE;
 constexpr size_t NOALLOC_ROWS = (size_t)NOALLOC_MAX_SIDE * (NOALLOC_MAX_SIDE / 2);  // W x ycells
-constexpr size_t NOALLOC_ELEMS = NOALLOC_TW + 2 * NOALLOC_GRID + 3 * NOALLOC_ROWS;  // 29.4 MB of float2
+constexpr size_t NOALLOC_ELEMS = NOALLOC_TW + NOALLOC_GRID + 2 * NOALLOC_ROWS;  // 16.8 MB of float2
 
 }  // namespace
 
@@ -735,11 +717,10 @@ extern "C" int tpt_gravity_solve_noalloc(const int32_t *mass, const uint32_t *ma
     d.logcols = 0;
     const uint32_t tw_len = ntw / 2 ? ntw / 2 : 1;
     const size_t grid = (size_t)W * H, rows = (size_t)W * ycells;
-    float2 *tw = base, *kx = tw + NOALLOC_TW, *ky = kx + grid, *work = ky + grid, *sx = work + rows,
-           *sy = sx + rows;
+    float2 *tw = base, *kc = tw + NOALLOC_TW, *work = kc + grid, *spec = work + rows;
     hipStream_t st = (hipStream_t)stream;
-    build_spectra(tw, kx, ky, d, ntw, tw_len, st);
-    launch_solve(mass, mask, fx, fy, nullptr, nullptr, nullptr, nullptr, nullptr, work, sx, sy, kx, ky, tw, d, st,
+    build_spectra(tw, kc, d, ntw, tw_len, st);
+    launch_solve(mass, mask, fx, fy, nullptr, nullptr, nullptr, nullptr, nullptr, work, spec, kc, tw, d, st,
                  nullptr);
     return tpt_check_launch("allocation-free gravity solve");
 }

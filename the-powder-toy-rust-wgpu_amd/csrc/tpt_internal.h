@@ -75,10 +75,8 @@ struct tpt_fft_state {
     uint32_t width = 0, height = 0;   // padded transform size (powers of two)
     uint32_t xcells = 0, ycells = 0;
     float2 *work = nullptr;           // padded mass spectrum, width*height
-    float2 *spec_x = nullptr;         // column-transformed products
-    float2 *spec_y = nullptr;
-    float2 *kernel_x = nullptr;       // cached kernel spectra
-    float2 *kernel_y = nullptr;
+    float2 *spec = nullptr;           // column-transformed product (x force + i y force)
+    float2 *kernel = nullptr;         // cached kernel spectrum of kx + i ky
     float2 *twiddle = nullptr;        // exp(-2 pi i k / N), N = max(width,height)
     bool kernels_ready = false;
 };
