@@ -43,7 +43,6 @@ constexpr int FFT_THREADS = 256;
 constexpr int FFT_ELEMS = 4096;
 constexpr int R4_PER_THREAD = FFT_ELEMS / 4 / FFT_THREADS;  // radix-4 butterflies per thread
 constexpr int R2_PER_THREAD = FFT_ELEMS / 2 / FFT_THREADS;  // radix-2 butterflies per thread
-constexpr int EPT = FFT_ELEMS / FFT_THREADS;                // elements per thread
 constexpr float M_GRAV = 6.673e-1f;
 constexpr int GRAV_BLOCK = 256;
 

@@ -4,12 +4,21 @@
 //   recalc_clear     zero the pixel records / photons / pmap_count (16-byte
 //                    stores); pmap itself is rewritten whole by recalc_extract
 //   recalc_scatter   life decay + kills, map scatter, stack count.  The
-//                    reference's last-writer store on a shared pixel becomes
-//                    a 64-bit atomicMax on the pixel record
-//                    ((slot+1)<<9 | flag<<8 | type) << 32 | temp bits, whose
-//                    high word orders by slot: the highest slot, the serial
-//                    order's last writer, keeps the pixel.  INVIS/FILT (which
-//                    only take an empty pixel) are queued instead.
+//                    reference's last-writer store on a shared pixel must
+//                    leave the highest slot (the serial order's last writer)
+//                    on the pixel record ((slot+1)<<9 | flag<<8 | type) << 32
+//                    | temp bits, whose high word orders by slot.  Every slot
+//                    first stores its record with a plain store (any one of
+//                    a shared pixel's writers survives); recalc_check lists
+//                    the slots that find another slot's record on their
+//                    pixel, and recalc_resolve runs only those through a
+//                    64-bit atomicMax, whose old value tells who lost and
+//                    is counted and listed.  The surviving plain store acts
+//                    as the first atomic.  (A device-scope atomic per
+//                    particle goes past the L2 of its XCD, one 32-byte
+//                    write each; the plain stores combine in L2.)
+//                    INVIS/FILT (which only take an empty pixel) are queued
+//                    instead.
 //   recalc_invis     the queued INVIS/FILT particles take their pixel only if
 //                    no other matter holds it; among several the lowest slot
 //                    wins (the first serial writer), by a CAS-min loop.
@@ -70,6 +79,7 @@ __global__ void recalc_clear(uint4 *__restrict__ rec, uint4 *__restrict__ photon
     if (i == 0) {
         counters[TPT_CTR_INVIS] = 0u;
         counters[TPT_CTR_OFFMAP] = 0u;
+        counters[TPT_CTR_CONFLICT] = 0u;
     }
     if (i >= n4) return;
     const uint4 z = make_uint4(0u, 0u, 0u, 0u);
@@ -87,8 +97,7 @@ __global__ __launch_bounds__(BLK) void recalc_scatter(
     uint32_t *__restrict__ offmap, uint32_t *__restrict__ counters, uint32_t slots, int xres, int yres, int xc,
     uint32_t decay) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool app_invis = false, app_self = false, app_other = false;
-    uint32_t other = 0u;
+    bool app_invis = false, app_self = false;
     uint32_t ty = i < slots ? ty_a[i] : 0u;
     if (ty != 0u && ty < (uint32_t)TPT_PT_NUM) {
         const float fx = px_a[i], fy = py_a[i];
@@ -126,7 +135,7 @@ __global__ __launch_bounds__(BLK) void recalc_scatter(
             const uint32_t idx = (uint32_t)y * (uint32_t)xres + (uint32_t)x;
             const uint32_t packed = ((i + 1u) << 9) | (ty & 0x1FFu);
             if (props & 0x10u) {
-                atomicMax(&photons[idx], packed);
+                photons[idx] = packed;
                 app_self = true;
             } else if (ty == 124u || ty == 134u) {  // INVIS / FILT
                 app_invis = true;
@@ -134,29 +143,95 @@ __global__ __launch_bounds__(BLK) void recalc_scatter(
                 atomicAdd(&pmap_count[idx], 1u);
             } else {
                 const uint32_t key = packed | (same_px ? 0u : 0x100u);
-                const unsigned long long v =
-                    ((unsigned long long)key << 32) | (unsigned long long)__float_as_uint(temp_a[i]);
-                const unsigned long long old = atomicMax(&rec[idx], v);
-                const uint32_t okey = (uint32_t)(old >> 32);
+                rec[idx] = ((unsigned long long)key << 32) | (unsigned long long)__float_as_uint(temp_a[i]);
                 if (!same_px) app_self = true;
-                if (okey > key) {
-                    // Lost to a higher slot: count self, list self.
-                    if (counted_type(ty)) atomicAdd(&pmap_count[idx], 1u);
-                    if (same_px) app_self = true;
-                } else if (okey != 0u) {
-                    // Displaced the previous holder: count it, list it.
-                    if (counted_type(okey & 0xFFu)) atomicAdd(&pmap_count[idx], 1u);
-                    if ((okey & 0x100u) == 0u) {
-                        app_other = true;
-                        other = (okey >> 9) - 1u;
-                    }
-                }
             }
         }
     }
     wave_append(invis_list, &counters[TPT_CTR_INVIS], i, app_invis);
     wave_append(offmap, &counters[TPT_CTR_OFFMAP], i, app_self);
-    wave_append(offmap, &counters[TPT_CTR_OFFMAP], other, app_other);
+}
+
+// Where slot i's scatter store went: kind 0 nowhere (empty, off the canvas,
+// INVIS/FILT), 1 the photon map, 2 the pixel record; `key` is the stored
+// word (the record's high word).
+struct MapStore {
+    uint32_t kind, idx, key;
+};
+
+__device__ __forceinline__ MapStore map_store(uint32_t i, uint32_t ty, float fx, float fy,
+                                              const uint32_t *__restrict__ elem_props, int xres, int yres) {
+    MapStore m{0u, 0u, 0u};
+    if (ty == 0u || ty >= (uint32_t)TPT_PT_NUM || ty == 124u || ty == 134u) return m;
+    const int x = tpt_f2i(fx + 0.5f), y = tpt_f2i(fy + 0.5f);
+    if (x < 0 || y < 0 || x >= xres || y >= yres) return m;
+    m.idx = (uint32_t)y * (uint32_t)xres + (uint32_t)x;
+    m.key = ((i + 1u) << 9) | ty;
+    if (elem_props[ty * 32u + 1u] & 0x10u) {
+        m.kind = 1u;
+    } else {
+        m.kind = 2u;
+        if (tpt_f2i(fx) != x || tpt_f2i(fy) != y) m.key |= 0x100u;
+    }
+    return m;
+}
+
+// Lists the slots whose scatter store was overwritten by another slot's.
+__global__ __launch_bounds__(BLK) void recalc_check(const uint32_t *__restrict__ ty_a, const float *__restrict__ px_a,
+                                                    const float *__restrict__ py_a,
+                                                    const uint32_t *__restrict__ elem_props,
+                                                    const unsigned long long *__restrict__ rec,
+                                                    const uint32_t *__restrict__ photons,
+                                                    uint32_t *__restrict__ conflicts, uint32_t *__restrict__ counters,
+                                                    uint32_t slots, int xres, int yres) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool clash = false;
+    const uint32_t ty = i < slots ? ty_a[i] : 0u;
+    if (ty != 0u) {
+        const MapStore m = map_store(i, ty, px_a[i], py_a[i], elem_props, xres, yres);
+        if (m.kind == 1u) clash = photons[m.idx] != m.key;
+        else if (m.kind == 2u) clash = (uint32_t)(rec[m.idx] >> 32) != m.key;
+    }
+    wave_append(conflicts, &counters[TPT_CTR_CONFLICT], i, clash);
+}
+
+// The listed slots take their pixel by atomicMax; whoever the old value
+// shows lost (this slot, or the holder it displaced) is counted on the
+// pixel and listed off the map.
+__global__ __launch_bounds__(BLK) void recalc_resolve(const uint32_t *__restrict__ ty_a,
+                                                      const float *__restrict__ px_a,
+                                                      const float *__restrict__ py_a,
+                                                      const float *__restrict__ temp_a,
+                                                      const uint32_t *__restrict__ elem_props,
+                                                      unsigned long long *__restrict__ rec,
+                                                      uint32_t *__restrict__ photons,
+                                                      uint32_t *__restrict__ pmap_count,
+                                                      const uint32_t *__restrict__ conflicts,
+                                                      uint32_t *__restrict__ offmap, uint32_t *__restrict__ counters,
+                                                      int xres, int yres) {
+    const uint32_t n = counters[TPT_CTR_CONFLICT];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t i = conflicts[k];
+        const uint32_t ty = ty_a[i];
+        const MapStore m = map_store(i, ty, px_a[i], py_a[i], elem_props, xres, yres);
+        if (m.kind == 1u) {
+            atomicMax(&photons[m.idx], m.key);
+            continue;
+        }
+        const unsigned long long v =
+            ((unsigned long long)m.key << 32) | (unsigned long long)__float_as_uint(temp_a[i]);
+        const uint32_t okey = (uint32_t)(atomicMax(&rec[m.idx], v) >> 32);
+        if (okey > m.key) {
+            // Lost to a higher slot: count self, list self (a flagged slot
+            // is listed already).
+            if (counted_type(ty)) atomicAdd(&pmap_count[m.idx], 1u);
+            if ((m.key & 0x100u) == 0u) offmap[atomicAdd(&counters[TPT_CTR_OFFMAP], 1u)] = i;
+        } else if (okey != 0u) {
+            // Displaced the previous holder: count it, list it.
+            if (counted_type(okey & 0xFFu)) atomicAdd(&pmap_count[m.idx], 1u);
+            if ((okey & 0x100u) == 0u) offmap[atomicAdd(&counters[TPT_CTR_OFFMAP], 1u)] = (okey >> 9) - 1u;
+        }
+    }
 }
 
 __global__ __launch_bounds__(BLK) void recalc_invis(const uint32_t *__restrict__ ty_a, const float *__restrict__ px_a,
@@ -402,6 +477,14 @@ static int map_build(tpt_ctx *ctx, int decay, uint32_t *count, int pass_clear, i
                        tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_BMAP), tpt_buf<uint32_t>(ctx, TPT_BUF_WALL_EMAP), rec,
                        photons, count, ctx->slot_scratch_u, ctx->offmap, ctx->counters, slots, xres, yres,
                        (int)ctx->xcells, decay ? 1u : 0u);
+    hipLaunchKernelGGL(recalc_check, dim3((slots + BLK - 1) / BLK), dim3(BLK), 0, ctx->stream,
+                       tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY), tpt_buf<float>(ctx, TPT_BUF_PART_X),
+                       tpt_buf<float>(ctx, TPT_BUF_PART_Y), tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS),
+                       (const unsigned long long *)rec, photons, ctx->map_conflicts, ctx->counters, slots, xres, yres);
+    hipLaunchKernelGGL(recalc_resolve, dim3(64), dim3(BLK), 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY),
+                       tpt_buf<float>(ctx, TPT_BUF_PART_X), tpt_buf<float>(ctx, TPT_BUF_PART_Y),
+                       tpt_buf<float>(ctx, TPT_BUF_PART_TEMP), tpt_buf<uint32_t>(ctx, TPT_BUF_ELEM_PROPS), rec, photons,
+                       count, ctx->map_conflicts, ctx->offmap, ctx->counters, xres, yres);
     hipLaunchKernelGGL(recalc_invis, dim3(64), dim3(BLK), 0, ctx->stream, tpt_buf<uint32_t>(ctx, TPT_BUF_PART_TY),
                        tpt_buf<float>(ctx, TPT_BUF_PART_X), tpt_buf<float>(ctx, TPT_BUF_PART_Y),
                        tpt_buf<float>(ctx, TPT_BUF_PART_TEMP), rec, ctx->slot_scratch_u, ctx->counters, xres);

@@ -99,6 +99,7 @@ static int alloc_buffers(tpt_ctx *c) {
         hipMalloc(&c->slot_scratch_f, cap * 4) != hipSuccess || hipMalloc(&c->counters, 256) != hipSuccess ||
         hipMalloc(&c->pix_rec, (size_t)c->pixels * 8) != hipSuccess ||
         hipMalloc(&c->pix_heat, (size_t)c->pixels * 8) != hipSuccess || hipMalloc(&c->offmap, cap * 4) != hipSuccess ||
+        hipMalloc(&c->map_conflicts, cap * 4) != hipSuccess ||
         hipMalloc(&c->pix_type, (size_t)c->pixels) != hipSuccess ||
         hipMalloc(&c->pix_count_mv, (size_t)c->pixels * 4) != hipSuccess ||
         hipMalloc(&c->slot_owner, cap * 4) != hipSuccess ||
@@ -136,7 +137,7 @@ static void free_all(tpt_ctx *c) {
         if (c->buf[id]) (void)hipFree(c->buf[id]);
     void *extra[] = {c->cell_scratch_u, c->cell_scratch_f, c->cell_scratch_u2, c->cell_scratch_f2,
                      c->slot_scratch_u, c->slot_scratch_u2, c->slot_scratch_f, c->counters,
-                     c->pix_rec, c->pix_heat, c->offmap, c->pix_type, c->pix_count_mv, c->slot_owner, c->create_rec, c->cell_rec,
+                     c->pix_rec, c->pix_heat, c->offmap, c->map_conflicts, c->pix_type, c->pix_count_mv, c->slot_owner, c->create_rec, c->cell_rec,
                      c->rx_flags, c->rx_side, c->rx_burner, c->rx_pvcount, c->rx_list, c->rx_wcount, c->water.col_min, c->water.col_max, c->water.col_surf, c->water.col_target,
                      c->water.winner};
     for (void *p : extra)

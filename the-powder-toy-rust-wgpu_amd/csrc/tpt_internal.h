@@ -31,6 +31,7 @@
 #define TPT_CTR_SEED 10      // seed of the pass(es) running now
 #define TPT_CTR_STACK 11     // 1: CheckStacking this tick
 #define TPT_CTR_RX_STAMP 12  // element passes: stamp of the current step's burner records (reactions.hip)
+#define TPT_CTR_CONFLICT 13  // map scatter: slots whose pixel another slot also wrote (spatial.hip)
 
 // Passes that carry their own timing slot (see tpt_get_pass_times).
 enum tpt_pass {
@@ -116,6 +117,7 @@ struct tpt_ctx {
     uint2 *pix_heat = nullptr;            // pixels
     uint8_t *pix_type = nullptr;          // pixels: pmap type byte for the movement planner
     uint32_t *offmap = nullptr;           // capacity words
+    uint32_t *map_conflicts = nullptr;    // capacity words: the map scatter's shared-pixel slots
     // True while pix_rec / offmap describe the current particle state: set
     // by RecalcFreeParticles, cleared by anything that may change slots,
     // positions, temperatures or the maps afterwards.
