@@ -343,10 +343,11 @@ TPT_API int tpt_sur_eval_loss(tpt_sur *sur, const uint32_t *pairs, uint32_t n_pa
 /* Bound on the global L2 norm of each step's mean gradient (default 1; 0
  * turns clipping off). */
 TPT_API int tpt_sur_set_clip(tpt_sur *sur, float clip);
-/* Training horizon of tpt_sur_train: 1 (the default) trains each pair
- * (traj, k) on snapshot k -> k+1; 2 first advances snapshot k by the
- * network itself, without gradient, and trains on that prediction ->
- * snapshot k+2 (pairs need k+2 < n_snap).  No counterpart in the reference. */
+/* Training horizon of tpt_sur_train, 1 .. 8: 1 (the default) trains each
+ * pair (traj, k) on snapshot k -> k+1; r > 1 first advances snapshot k
+ * r - 1 times by the network itself, without gradient, and trains on that
+ * prediction -> snapshot k+r (pairs need k+r < n_snap).  No counterpart in
+ * the reference. */
 TPT_API int tpt_sur_set_rollout(tpt_sur *sur, uint32_t steps);
 /* Inference with the pools, upsample-adds and input preparation folded into
  * the convolutions that read them (fused = 1, the default) or run as

@@ -190,7 +190,7 @@ def test_two_step_rollout_training(gpu):
     """tpt_sur_set_rollout(2): pairs train from the network's own first
     step; a pair without a snapshot two calls ahead is refused, a zero
     learning rate leaves the parameters alone, and training lowers the
-    two-step loss."""
+    two-step loss.  The same for three steps; nine are refused."""
     ctx, sur = small_world()
     sur.set_scales(SU.IN_SCALES, SU.delta_scales(ctx, sur, range(900, 903)))
     SU.build_dataset(sur, ctx, list(range(16)))
@@ -208,6 +208,21 @@ def test_two_step_rollout_training(gpu):
     after = sur.train(pairs, 4, 0.0)
     print(f"  after {after:.4e}", flush=True)
     assert np.isfinite(after) and after < 0.5 * before
+    # deeper rollouts: 3 steps need snapshot k+3, the horizon stops at 8
+    sur.set_rollout(3)
+    with pytest.raises(RuntimeError):
+        sur.train(np.array([(0, 6)], np.uint32), 1, 0.0)
+    with pytest.raises(RuntimeError):
+        sur.set_rollout(9)
+    three = np.array([(t, k) for t in range(16) for k in range(6)], np.uint32)
+    p1 = sur.params()
+    before3 = sur.train(three, 4, 0.0)
+    assert np.array_equal(sur.params(), p1)
+    for ep in range(6):
+        sur.train(three[rng.permutation(len(three))], 4, 1e-3)
+    after3 = sur.train(three, 4, 0.0)
+    print(f"  three-step loss {before3:.4e} -> {after3:.4e}", flush=True)
+    assert np.isfinite(after3) and after3 < before3
     sur.set_rollout(1)
     ctx.close()
     sur.close()

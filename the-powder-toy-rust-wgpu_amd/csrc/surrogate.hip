@@ -37,6 +37,7 @@ constexpr int N_LAYERS = 10;
 constexpr int IN_CH = 8;    // 6 inputs + 2 zero channels
 constexpr int OUT_CH = 8;   // 4 outputs + 4 zero channels
 constexpr int TICKS_PER_CALL = 8;
+constexpr int MAX_ROLLOUT = 8;  // calls a training pair may span (a 64-tick trajectory has 9 snapshots)
 
 struct LayerSpec {
     int cin, cout, level, gelu;
@@ -96,12 +97,12 @@ struct tpt_sur {
     float *gnorm = nullptr;         // device: sum of squares of the batch gradient
     float *red = nullptr;           // device: per-block partial sums (up to 4 per block)
     float clip = 1.0f;              // global gradient-norm clip (0: none)
-    // Training horizon: 1 trains on (snapshot k -> k+1); 2 first advances
-    // snapshot k by the network itself (no gradient) and trains on
-    // (that prediction -> snapshot k+2), so that the network learns to
-    // correct the errors of its own rollouts.
+    // Training horizon: 1 trains on (snapshot k -> k+1); r > 1 first
+    // advances snapshot k r - 1 times by the network itself (no gradient)
+    // and trains on (that prediction -> snapshot k+r), so that the network
+    // learns to correct the errors of its own rollouts.
     uint32_t rollout = 1;
-    float *roll = nullptr;          // the predicted planes of a 2-step pair (4 x cells)
+    float *roll = nullptr;          // the predicted planes of a rollout pair (4 x cells)
     bool fused = true;              // inference folds the glue kernels into the convolutions
     float in_scale[4] = {10.0f, 1.0f, 1.0f, 500.0f};   // input normalisation (state)
     float scale[4] = {1.0f, 1.0f, 1.0f, 10.0f};        // output normalisation (8-tick change)
@@ -847,8 +848,9 @@ extern "C" int tpt_sur_train(tpt_sur *s, const uint32_t *pairs, uint32_t n_pairs
         const Planes nxt = snap_planes(s, tr, sn + s->rollout);
         const uint8_t *walls = s->walls + (size_t)tr * n;
         const float *srcs = s->srcs + (size_t)tr * n;
-        if (s->rollout == 2) {
-            // the network's own first step, without gradient
+        for (uint32_t r = 1; r < s->rollout; ++r) {
+            // the network's own steps, without gradient (each reads the
+            // previous prediction and overwrites it cell by cell)
             hipLaunchKernelGGL(sur_prep, grid1(n), dim3(256), 0, st, cur, walls, nullptr, srcs, s->x0, s->x0lo, n,
                                s->in_scale[0], s->in_scale[1], s->in_scale[3], s->ambient);
             forward(s, st, false);
@@ -887,7 +889,8 @@ extern "C" int tpt_sur_train(tpt_sur *s, const uint32_t *pairs, uint32_t n_pairs
 
 // Global L2 norm bound of the averaged batch gradient (0 turns clipping off).
 extern "C" int tpt_sur_set_rollout(tpt_sur *s, uint32_t steps) {
-    if (!s || steps < 1 || steps > 2) return tpt_set_error(TPT_EINVAL, "rollout steps must be 1 or 2");
+    if (!s || steps < 1 || steps > MAX_ROLLOUT)
+        return tpt_set_error(TPT_EINVAL, "rollout steps must be 1 .. %d", MAX_ROLLOUT);
     s->rollout = steps;
     return TPT_OK;
 }

@@ -7,7 +7,7 @@ air_kernel/mod.rs:120-299), drives training on the GPU, evaluates the trained
 network against the plain solver, and saves and loads its parameters.
 
     python -m tpt_hip.surrogate train --traj 2000 --epochs 6 --out weights.npz
-    python -m tpt_hip.surrogate train --init weights.npz --rollout --epochs 4 --out tuned.npz
+    python -m tpt_hip.surrogate train --init weights.npz --rollout 3 --epochs 4 --out tuned.npz
     python -m tpt_hip.surrogate eval --weights weights.npz --seeds 256
     python -m tpt_hip.surrogate calls --calls 30      (a workload for rocprofv3)
 
@@ -331,9 +331,8 @@ def train_main(args):
     rng = np.random.default_rng(args.seed)
     hold = pairs[rng.choice(len(pairs), size=min(256, len(pairs)), replace=False)]
     steps_done = 0
-    total_steps = args.epochs * len(pairs) // args.accum
-    if args.rollout:
-        total_steps *= 2
+    total_steps = args.epochs * sum(int(np.count_nonzero(pairs[:, 1] + r <= TICKS_PER_CALL))
+                                    for r in range(1, args.rollout + 1)) // args.accum
     for ep in range(args.epochs):
         order = rng.permutation(len(pairs))
         t1 = time.perf_counter()
@@ -347,18 +346,18 @@ def train_main(args):
             sur.set_rollout(1)
             losses.append(sur.train(sel, args.accum, lr))
             steps_done += (len(sel) + args.accum - 1) // args.accum
-            if args.rollout:
-                # the same pairs again, from the network's own first step
-                two = sel[sel[:, 1] + 2 <= TICKS_PER_CALL]
-                if len(two):
-                    sur.set_rollout(2)
-                    losses.append(sur.train(two, args.accum, lr))
+            for r in range(2, args.rollout + 1):
+                # the same pairs again, from the network's own r - 1 steps
+                deep = sel[sel[:, 1] + r <= TICKS_PER_CALL]
+                if len(deep):
+                    sur.set_rollout(r)
+                    losses.append(sur.train(deep, args.accum, lr))
                     sur.set_rollout(1)
-                    steps_done += (len(two) + args.accum - 1) // args.accum
+                    steps_done += (len(deep) + args.accum - 1) // args.accum
         ev = sur.eval_loss(hold)
         done_epochs = int(prior.get("epochs") or 0) + ep + 1
         history.append({"epoch": done_epochs, "train_loss": float(np.mean(losses)), "eval_loss": ev,
-                        "seconds": time.perf_counter() - t1, "rollout": 2 if args.rollout else 1})
+                        "seconds": time.perf_counter() - t1, "rollout": args.rollout})
         log(f"epoch {done_epochs}: train {np.mean(losses):.4e} eval {ev:.4e} ({time.perf_counter() - t1:.1f} s)")
         sur.save(args.out, {"history": history, "traj": args.traj, "epochs": done_epochs, "seed": args.seed})
     log(f"saved {args.out}")
@@ -421,8 +420,8 @@ def main(argv=None):
     t.add_argument("--eval-seeds", type=int, default=0)
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
     t.add_argument("--init", default=None, help="continue from a saved network (its scales are kept)")
-    t.add_argument("--rollout", action="store_true",
-                   help="train every pair also from the network's own first step (2-step rollouts)")
+    t.add_argument("--rollout", type=int, default=1,
+                   help="train every pair also from the network's own first 1 .. R-1 steps (R-step rollouts)")
     e = sub.add_parser("eval")
     e.add_argument("--grid", type=int, default=512)
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
