@@ -14,9 +14,9 @@
 //                    pixel, and recalc_resolve runs only those through a
 //                    64-bit atomicMax, whose old value tells who lost and
 //                    is counted and listed.  The surviving plain store acts
-//                    as the first atomic.  (A device-scope atomic per
-//                    particle goes past the L2 of its XCD, one 32-byte
-//                    write each; the plain stores combine in L2.)
+//                    as the first atomic.  (A device-scope atomic executes
+//                    past the L2 of its XCD; one per particle cost more
+//                    than the plain stores and the check's loads together.)
 //                    INVIS/FILT (which only take an empty pixel) are queued
 //                    instead.
 //   recalc_invis     the queued INVIS/FILT particles take their pixel only if
